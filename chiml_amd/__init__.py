@@ -1,0 +1,223 @@
+"""chiml_amd -- MI355X-native time-stepping engine for chiML decks.
+
+Python mirror of the reference's manager interface
+(src/FDTD_MANAGER/parallelFDTDField.hpp: step(), E_/H_, dtcArr(), fluxArr(),
+qeArr(), dt(); src/main.cpp:47-118), backed by libchiml_amd.so through the C
+ABI of include/chiml/chiml_sim.h.  Import works without a GPU; creating a
+simulation needs the built library and a HIP device and fails loudly otherwise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["FDTDField", "load_library", "LIB_PATH", "ChimlError"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libchiml_amd.so")
+_LIB = None
+
+FIELD_NAMES = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
+
+
+class ChimlError(RuntimeError):
+    pass
+
+
+def load_library():
+    """Loads libchiml_amd.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ChimlError(f"{LIB_PATH} is not built; run __graft_entry__.build() (or python chiml_amd/build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    c_int_p = ctypes.POINTER(ctypes.c_int)
+    c_dbl_p = ctypes.POINTER(ctypes.c_double)
+    c_size_p = ctypes.POINTER(ctypes.c_size_t)
+    L.chiml_sim_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+    L.chiml_sim_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.chiml_sim_sync.argtypes = [ctypes.c_void_p]
+    L.chiml_sim_info.argtypes = [ctypes.c_void_p, c_int_p, c_dbl_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]
+    L.chiml_sim_field.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p]
+    L.chiml_sim_detector.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, c_dbl_p, ctypes.c_size_t, c_size_p, c_int_p]
+    L.chiml_sim_flux.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t, c_int_p]
+    L.chiml_sim_pops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_size_t, c_size_p]
+    L.chiml_sim_density.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_dbl_p, c_int_p]
+    L.chiml_sim_cost.argtypes = [ctypes.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]
+    L.chiml_sim_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.chiml_sim_profile_read.argtypes = [ctypes.c_void_p, ctypes.c_char_p, c_dbl_p, c_int_p]
+    L.chiml_sim_footprint.argtypes = [ctypes.c_void_p, c_size_p]
+    L.chiml_sim_estimate_bytes.argtypes = [ctypes.c_char_p, c_size_p]
+    L.chiml_sim_last_error.restype = ctypes.c_char_p
+    L.chiml_sim_destroy.argtypes = [ctypes.c_void_p]
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise ChimlError(load_library().chiml_sim_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Detector:
+    """A point/box time-domain detector (parallelDetectorBase)."""
+
+    def __init__(self, sim, index):
+        self._sim = sim
+        self.index = index
+
+    def series(self):
+        """(times, values[nsamples, npts]) -- what the TXT detector writes."""
+        L = load_library()
+        ns = ctypes.c_size_t()
+        npts = ctypes.c_int()
+        _check(L.chiml_sim_detector(self._sim._h, self.index, None, None, 0, ctypes.byref(ns), ctypes.byref(npts)))
+        t = np.zeros(ns.value)
+        v = np.zeros(ns.value * npts.value)
+        _check(L.chiml_sim_detector(self._sim._h, self.index, _dp(t), _dp(v), ns.value, ctypes.byref(ns), ctypes.byref(npts)))
+        return t, v.reshape(ns.value, npts.value)
+
+
+class Flux:
+    """A DFT flux box (parallelFluxDTC)."""
+
+    def __init__(self, sim, index):
+        self._sim = sim
+        self.index = index
+
+    def getFlux(self):
+        """Total complex flux per frequency (sum over faces, Simpson's rule)."""
+        L = load_library()
+        nf = ctypes.c_int()
+        _check(L.chiml_sim_flux(self._sim._h, self.index, None, 0, ctypes.byref(nf)))
+        buf = np.zeros(2 * nf.value)
+        _check(L.chiml_sim_flux(self._sim._h, self.index, _dp(buf), buf.size, ctypes.byref(nf)))
+        return buf[0::2] + 1j * buf[1::2]
+
+
+class QE:
+    """A Maxwell-Liouville emitter group (parallelQEBase)."""
+
+    def __init__(self, sim, index):
+        self._sim = sim
+        self.index = index
+
+    def population(self, j):
+        """Population detector j series (QEPopDtc): complex per output step."""
+        L = load_library()
+        ns = ctypes.c_size_t()
+        _check(L.chiml_sim_pops(self._sim._h, self.index, j, None, 0, ctypes.byref(ns)))
+        buf = np.zeros(2 * ns.value)
+        _check(L.chiml_sim_pops(self._sim._h, self.index, j, _dp(buf), buf.size, ctypes.byref(ns)))
+        return buf[0::2] + 1j * buf[1::2]
+
+    def density(self, member, emitter, nlevel):
+        L = load_library()
+        buf = np.zeros(2 * nlevel * nlevel)
+        nl = ctypes.c_int()
+        _check(L.chiml_sim_density(self._sim._h, self.index, member, emitter, _dp(buf), ctypes.byref(nl)))
+        return (buf[0::2] + 1j * buf[1::2]).reshape(nlevel, nlevel, order="F")
+
+
+class FDTDField:
+    """Device-resident simulation built from a chiML JSON deck.
+
+    Mirrors parallelFDTDFieldReal: ``step()`` advances one step
+    (parallelFDTDField.hpp:1228), ``dtcArr()/fluxArr()/qeArr()`` give the
+    detectors, flux boxes and emitters (:1766-1781), ``dt`` is the time step.
+    """
+
+    def __init__(self, deck_json):
+        L = load_library()
+        h = ctypes.c_void_p()
+        _check(L.chiml_sim_create(deck_json.encode(), ctypes.byref(h)))
+        self._h = h
+        n3 = (ctypes.c_int * 3)()
+        dt = ctypes.c_double()
+        ns = ctypes.c_int()
+        prec = ctypes.c_int()
+        nd = ctypes.c_int()
+        nf = ctypes.c_int()
+        nq = ctypes.c_int()
+        _check(L.chiml_sim_info(h, n3, ctypes.byref(dt), ctypes.byref(ns), ctypes.byref(prec), ctypes.byref(nd),
+                                ctypes.byref(nf), ctypes.byref(nq)))
+        self.shape = (n3[2], n3[1], n3[0])
+        self.dt = dt.value
+        self.nsteps = ns.value
+        self.precision = prec.value
+        self._ndet, self._nflux, self._nqe = nd.value, nf.value, nq.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().chiml_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        _check(load_library().chiml_sim_step(self._h, int(n)))
+
+    def sync(self):
+        _check(load_library().chiml_sim_sync(self._h))
+
+    def run(self):
+        """The reference's main loop: ceil(tLim/dt) steps (src/main.cpp:54-57)."""
+        self.step(self.nsteps)
+        self.sync()
+
+    def field(self, comp):
+        if isinstance(comp, str):
+            comp = FIELD_NAMES[comp]
+        out = np.zeros(int(np.prod(self.shape)))
+        _check(load_library().chiml_sim_field(self._h, int(comp), _dp(out)))
+        return out.reshape(self.shape)
+
+    def E(self, c):
+        return self.field(c)
+
+    def H(self, c):
+        return self.field(3 + c)
+
+    def dtcArr(self):
+        return [Detector(self, i) for i in range(self._ndet)]
+
+    def fluxArr(self):
+        return [Flux(self, i) for i in range(self._nflux)]
+
+    def qeArr(self):
+        return [QE(self, i) for i in range(self._nqe)]
+
+    def cost(self):
+        """(cells, algorithmic bytes/step of update_h, update_e, emitter state)."""
+        vals = [ctypes.c_double() for _ in range(4)]
+        _check(load_library().chiml_sim_cost(self._h, *[ctypes.byref(v) for v in vals]))
+        return tuple(v.value for v in vals)
+
+    def profile(self, enable=True):
+        _check(load_library().chiml_sim_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        _check(load_library().chiml_sim_profile_read(self._h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def footprint(self):
+        b = ctypes.c_size_t()
+        _check(load_library().chiml_sim_footprint(self._h, ctypes.byref(b)))
+        return b.value
+
+
+def estimate_bytes(deck_json):
+    b = ctypes.c_size_t()
+    _check(load_library().chiml_sim_estimate_bytes(deck_json.encode(), ctypes.byref(b)))
+    return b.value

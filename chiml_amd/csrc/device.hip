@@ -1,0 +1,1188 @@
+// MI355X (gfx950) kernels and the C ABI of include/chiml/chiml_dev.h.
+//
+// One context = one device-resident domain.  Per step (the order of
+// parallelFDTDFieldBase::step(), src/FDTD_MANAGER/parallelFDTDField.hpp:1228-1303):
+//   1. k_update_h  : H += curl E with CPML psi (updateH + updateHPML) and H sources
+//   2. k_update_e  : soft E sources, Lorentz/Drude ADE, E += curl H with CPML psi
+//                    (addPul, updatePolE, updateE/updateD, updateEPML, D2E)
+//   3. emitters    : k_ml_addp then k_ml_density (parallelQEBase::addQE)
+//   4. k_advance   : t_step += 1
+//   5. k_sample    : point detectors and DFT staging (dtc->output, flux->fieldIn)
+//   6. k_dft_flush : every kDftBlock samples, F[s,f] += sum_k W[k,f] U[k,s]
+//
+// Formulation.  The reference keeps D for dispersive, emitter and
+// PML-with-material cells and recomputes E = (D - sum P)/eps after every step
+// (DtoU, src/UTIL/FDTD_up_eq.cpp:838-848).  Here E itself is the state:
+// E' = E + (dt/eps) curl H - (1/eps) sum_p (P_p' - P_p), which is the same
+// recursion in exact arithmetic, and each pole is advanced in current form
+// J' = (alpha + xi - 1) P - xi J + gamma E, P' = P + J' with J = P - P_prev,
+// which is UpdateLorPol's P' = alpha P + xi P_prev + gamma E rewritten
+// (src/UTIL/FDTD_up_eq.cpp:425-448).  In FP32 this avoids the cancellation of
+// large D and P in metals (a Drude pole's P grows without bound while E stays
+// small).  The CPML uses the same b, c, kappa profiles and psi recursions as
+// src/PML/parallelPML.hpp:693-765 and src/PML/parallelPML.cpp:12-40.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/chiml/chiml_dev.h"
+
+namespace {
+
+constexpr int kDftBlock = 32;  // samples staged before one DFT contraction
+constexpr int kMaxLevels = 8;
+
+std::string g_err;
+
+#define HIP_TRY(expr)                                                                                \
+    do {                                                                                             \
+        hipError_t _e = (expr);                                                                      \
+        if (_e != hipSuccess) {                                                                      \
+            throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(_e));            \
+        }                                                                                            \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t b) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = b;
+        if (b == 0) return;
+        HIP_TRY(hipMalloc(&p, b));
+        HIP_TRY(hipMemset(p, 0, b));
+        // the context's stream is non-blocking: finish the clear before any use on it
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------- params ----
+template <typename T>
+struct PmlDev {
+    const T* b[3];
+    const T* cd[3];
+    const T* kinv[3];
+    T* psi[3];
+    int lo[3], hi0[3], hi1[3], S[3];
+};
+
+template <typename T>
+struct StepArgs {
+    T* F[6];  // Ex, Ey, Ez, Hx, Hy, Hz
+    int nx, ny, nz;
+    T inv_d[3];
+    int Tp[3];
+    const uint8_t* mat_e[3];
+    const T* ce[3];
+    const T* inveps[3];
+    const int* flags[3];
+    const int* poleset[3];
+    const uint8_t* mat_h[3];
+    const T* ch[3];
+    const int* pole_count;
+    const T* pole_coef;  // [set][kMaxPoles][3]
+    T* P[3];
+    T* J[3];
+    int box[3][6];
+    long long boxn[3];
+    PmlDev<T> pml[6];
+    int nsrc;
+    const int* src_comp;
+    const int* src_box;
+    const T* src_amp;
+    const int* step;
+};
+
+__device__ __forceinline__ long long lidx(int x, int y, int z, int nx, int ny) {
+    return x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
+}
+
+// CPML term for one derivative direction a of component c:
+// psi' = b psi + (c/d) dV (psi-carrying positions), term = dV/(kappa d) + psi'.
+// Elsewhere in the PML region the term is dV/d (kappa = 1, no psi).
+template <typename T>
+__device__ __forceinline__ T pml_term(const PmlDev<T>& p, int a, int pos, int x, int y, int z, int nx, int ny, T dV, T dVd) {
+    int s;
+    if (pos < p.lo[a]) s = pos;
+    else if (pos >= p.hi0[a] && pos < p.hi1[a]) s = p.lo[a] + pos - p.hi0[a];
+    else return dVd;
+    long long pi;
+    if (a == 0) pi = s + static_cast<long long>(p.S[0]) * (y + static_cast<long long>(ny) * z);
+    else if (a == 1) pi = x + static_cast<long long>(nx) * (s + static_cast<long long>(p.S[1]) * z);
+    else pi = x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * s);
+    T psi = p.b[a][pos] * p.psi[a][pi] + p.cd[a][pos] * dV;
+    p.psi[a][pi] = psi;
+    return p.kinv[a][pos] * dVd + psi;
+}
+
+template <typename T>
+__device__ __forceinline__ T src_value(const StepArgs<T>& a, int comp, int x, int y, int z, int step) {
+    T v = 0;
+    for (int s = 0; s < a.nsrc; ++s) {
+        if (a.src_comp[s] != comp) continue;
+        const int* b = a.src_box + 6 * s;
+        if (x >= b[0] && x < b[1] && y >= b[2] && y < b[3] && z >= b[4] && z < b[5]) v += a.src_amp[static_cast<long long>(step) * a.nsrc + s];
+    }
+    return v;
+}
+
+// ----------------------------------------------------------- H update -------
+// 3D: Hx += ch[(Ey(z+1)-Ey)/dz - (Ez(y+1)-Ez)/dy], Hy += ch[(Ez(x+1)-Ez)/dx -
+// (Ex(z+1)-Ex)/dz], Hz += ch[(Ex(y+1)-Ex)/dy - (Ey(x+1)-Ey)/dx]; 2D TMz keeps
+// the Ez terms of Hx and Hy (OneCompCurlK/J, src/UTIL/FDTD_up_eq.cpp:10-35).
+template <typename T, int DIM, bool HMAP, bool SRC>
+__global__ void __launch_bounds__(256) k_update_h(StepArgs<T> a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    if (x >= nx || y >= ny) return;
+    const long long i = lidx(x, y, z, nx, ny);
+    const long long sy = nx, sz = static_cast<long long>(nx) * ny;
+    const bool inpml = (x < a.Tp[0]) || (x >= nx - a.Tp[0]) || (y < a.Tp[1]) || (y >= ny - a.Tp[1]) ||
+                       (DIM == 3 && ((z < a.Tp[2]) || (z >= nz - a.Tp[2])));
+    int step = 0;
+    if (SRC) step = *a.step;
+    T* Ex = a.F[0];
+    T* Ey = a.F[1];
+    T* Ez = a.F[2];
+    // Hx: valid y < ny-1 (and z < nz-1 in 3D)
+    if (y < ny - 1 && (DIM == 2 || z < nz - 1)) {
+        T ez = Ez[i];
+        T dEz = Ez[i + sy] - ez;
+        T acc;
+        if (DIM == 3) {
+            T dEy = Ey[i + sz] - Ey[i];
+            if (!inpml) acc = dEy * a.inv_d[2] - dEz * a.inv_d[1];
+            else acc = pml_term(a.pml[3], 2, z, x, y, z, nx, ny, dEy, dEy * a.inv_d[2]) - pml_term(a.pml[3], 1, y, x, y, z, nx, ny, dEz, dEz * a.inv_d[1]);
+        } else {
+            if (!inpml) acc = -(dEz * a.inv_d[1]);
+            else acc = -pml_term(a.pml[3], 1, y, x, y, z, nx, ny, dEz, dEz * a.inv_d[1]);
+        }
+        T ch = HMAP ? a.ch[0][a.mat_h[0][i]] : a.ch[0][0];
+        T h = a.F[3][i] + ch * acc;
+        if (SRC) h += src_value(a, 3, x, y, z, step);
+        a.F[3][i] = h;
+    }
+    // Hy: valid x < nx-1 (and z < nz-1 in 3D)
+    if (x < nx - 1 && (DIM == 2 || z < nz - 1)) {
+        T dEz = Ez[i + 1] - Ez[i];
+        T acc;
+        if (DIM == 3) {
+            T dEx = Ex[i + sz] - Ex[i];
+            if (!inpml) acc = dEz * a.inv_d[0] - dEx * a.inv_d[2];
+            else acc = pml_term(a.pml[4], 0, x, x, y, z, nx, ny, dEz, dEz * a.inv_d[0]) - pml_term(a.pml[4], 2, z, x, y, z, nx, ny, dEx, dEx * a.inv_d[2]);
+        } else {
+            if (!inpml) acc = dEz * a.inv_d[0];
+            else acc = pml_term(a.pml[4], 0, x, x, y, z, nx, ny, dEz, dEz * a.inv_d[0]);
+        }
+        T ch = HMAP ? a.ch[1][a.mat_h[1][i]] : a.ch[1][0];
+        T h = a.F[4][i] + ch * acc;
+        if (SRC) h += src_value(a, 4, x, y, z, step);
+        a.F[4][i] = h;
+    }
+    // Hz (3D only): valid x < nx-1, y < ny-1
+    if (DIM == 3 && x < nx - 1 && y < ny - 1) {
+        T dEx = Ex[i + sy] - Ex[i];
+        T dEy = Ey[i + 1] - Ey[i];
+        T acc;
+        if (!inpml) acc = dEx * a.inv_d[1] - dEy * a.inv_d[0];
+        else acc = pml_term(a.pml[5], 1, y, x, y, z, nx, ny, dEx, dEx * a.inv_d[1]) - pml_term(a.pml[5], 0, x, x, y, z, nx, ny, dEy, dEy * a.inv_d[0]);
+        T ch = HMAP ? a.ch[2][a.mat_h[2][i]] : a.ch[2][0];
+        T h = a.F[5][i] + ch * acc;
+        if (SRC) h += src_value(a, 5, x, y, z, step);
+        a.F[5][i] = h;
+    }
+}
+
+// ----------------------------------------------------------- E update -------
+template <typename T, bool SRC>
+__device__ __forceinline__ void e_finish(const StepArgs<T>& a, int c, long long i, int x, int y, int z, T acc, int step) {
+    const uint8_t id = a.mat_e[c] ? a.mat_e[c][i] : 0;
+    const T ce = a.ce[c][id];
+    const int fl = a.flags[c][id];
+    const int ps = a.poleset[c][id];
+    T e_old = a.F[c][i];
+    T src = SRC ? src_value(a, c, x, y, z, step) : T(0);
+    // A soft source on a D cell reaches the ADE but not E: D2E overwrites E
+    // from D, which the source never touched (parallelFDTDField.hpp:1261-1281).
+    T e_base = (fl & CHIML_MAT_DCELL) ? e_old : e_old + src;
+    T e_ade = e_old + src;
+    T jsum = 0;
+    if (ps >= 0) {
+        const int* bx = a.box[c];
+        long long bi = (x - bx[0]) + static_cast<long long>(bx[1] - bx[0]) * ((y - bx[2]) + static_cast<long long>(bx[3] - bx[2]) * (z - bx[4]));
+        const int np = a.pole_count[ps];
+        const T* co = a.pole_coef + static_cast<long long>(ps) * CHIML_MAX_POLES * 3;
+        for (int p = 0; p < np; ++p) {
+            T* Pp = a.P[c] + p * a.boxn[c];
+            T* Jp = a.J[c] + p * a.boxn[c];
+            T P = Pp[bi], J = Jp[bi];
+            T Jn = co[3 * p] * P + co[3 * p + 1] * J + co[3 * p + 2] * e_ade;
+            Pp[bi] = P + Jn;
+            Jp[bi] = Jn;
+            jsum += Jn;
+        }
+    }
+    a.F[c][i] = e_base + ce * acc - a.inveps[c][id] * jsum;
+}
+
+// 3D: Ex += ce[(Hz-Hz(y-1))/dy - (Hy-Hy(z-1))/dz], Ey += ce[(Hx-Hx(z-1))/dz -
+// (Hz-Hz(x-1))/dx], Ez += ce[(Hy-Hy(x-1))/dx - (Hx-Hx(y-1))/dy] (TwoCompCurl,
+// src/UTIL/FDTD_up_eq.cpp:26-35); the ghost border is zero (PEC).
+template <typename T, int DIM, bool SRC>
+__global__ void __launch_bounds__(256) k_update_e(StepArgs<T> a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    if (x >= nx || y >= ny) return;
+    const long long i = lidx(x, y, z, nx, ny);
+    const long long sy = nx, sz = static_cast<long long>(nx) * ny;
+    const bool inpml = (x < a.Tp[0]) || (x >= nx - a.Tp[0]) || (y < a.Tp[1]) || (y >= ny - a.Tp[1]) ||
+                       (DIM == 3 && ((z < a.Tp[2]) || (z >= nz - a.Tp[2])));
+    int step = 0;
+    if (SRC) step = *a.step;
+    const T* Hx = a.F[3];
+    const T* Hy = a.F[4];
+    const T* Hz = a.F[5];
+    if (DIM == 3) {
+        if (x < nx - 1) {  // Ex
+            T dHz = Hz[i] - (y > 0 ? Hz[i - sy] : T(0));
+            T dHy = Hy[i] - (z > 0 ? Hy[i - sz] : T(0));
+            T acc;
+            if (!inpml) acc = dHz * a.inv_d[1] - dHy * a.inv_d[2];
+            else acc = pml_term(a.pml[0], 1, y, x, y, z, nx, ny, dHz, dHz * a.inv_d[1]) - pml_term(a.pml[0], 2, z, x, y, z, nx, ny, dHy, dHy * a.inv_d[2]);
+            e_finish<T, SRC>(a, 0, i, x, y, z, acc, step);
+        }
+        if (y < ny - 1) {  // Ey
+            T dHx = Hx[i] - (z > 0 ? Hx[i - sz] : T(0));
+            T dHz = Hz[i] - (x > 0 ? Hz[i - 1] : T(0));
+            T acc;
+            if (!inpml) acc = dHx * a.inv_d[2] - dHz * a.inv_d[0];
+            else acc = pml_term(a.pml[1], 2, z, x, y, z, nx, ny, dHx, dHx * a.inv_d[2]) - pml_term(a.pml[1], 0, x, x, y, z, nx, ny, dHz, dHz * a.inv_d[0]);
+            e_finish<T, SRC>(a, 1, i, x, y, z, acc, step);
+        }
+        if (z < nz - 1) {  // Ez
+            T dHy = Hy[i] - (x > 0 ? Hy[i - 1] : T(0));
+            T dHx = Hx[i] - (y > 0 ? Hx[i - sy] : T(0));
+            T acc;
+            if (!inpml) acc = dHy * a.inv_d[0] - dHx * a.inv_d[1];
+            else acc = pml_term(a.pml[2], 0, x, x, y, z, nx, ny, dHy, dHy * a.inv_d[0]) - pml_term(a.pml[2], 1, y, x, y, z, nx, ny, dHx, dHx * a.inv_d[1]);
+            e_finish<T, SRC>(a, 2, i, x, y, z, acc, step);
+        }
+    } else {
+        T dHy = Hy[i] - (x > 0 ? Hy[i - 1] : T(0));
+        T dHx = Hx[i] - (y > 0 ? Hx[i - sy] : T(0));
+        T acc;
+        if (!inpml) acc = dHy * a.inv_d[0] - dHx * a.inv_d[1];
+        else acc = pml_term(a.pml[2], 0, x, x, y, z, nx, ny, dHy, dHy * a.inv_d[0]) - pml_term(a.pml[2], 1, y, x, y, z, nx, ny, dHx, dHx * a.inv_d[1]);
+        e_finish<T, SRC>(a, 2, i, x, y, z, acc, step);
+    }
+}
+
+// ----------------------------------------------------- step counter ---------
+__global__ void k_advance(int* step) { *step += 1; }
+
+// ----------------------------------------------- detectors and DFT ----------
+template <typename T>
+struct SampleArgs {
+    const T* F[6];
+    const int* step;
+    int ndet;
+    const int* det_interval;
+    const int* det_npts;
+    const int* det_off;  // prefix sums of npts
+    const int* det_comp;
+    const long long* det_cell;
+    const double* det_conv;
+    const int* det_pow;
+    double* det_out;
+    const long long* det_out_off;  // per detector offset into det_out
+    const int* det_cap;            // per detector max samples
+    int totpts;
+    int ndft;
+    const int* dft_comp;
+    const long long* dft_cell;
+    int dft_interval;
+    T* stage;  // [kDftBlock][ndft]
+};
+
+template <typename T>
+__global__ void k_sample(SampleArgs<T> a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int step = *a.step;
+    if (t < a.totpts) {
+        // locate the detector of point t
+        int d = 0;
+        while (d + 1 < a.ndet && a.det_off[d + 1] <= t) ++d;
+        if (step % a.det_interval[d] == 0) {
+            int slot = step / a.det_interval[d];
+            if (slot < a.det_cap[d]) {
+                int p = t - a.det_off[d];
+                double u = static_cast<double>(a.F[a.det_comp[2 * t]][a.det_cell[2 * t]]);
+                double v = static_cast<double>(a.F[a.det_comp[2 * t + 1]][a.det_cell[2 * t + 1]]);
+                double c = a.det_conv[d];
+                double point = 0.0;
+                if (a.det_pow[d]) {
+                    point = point + c / 2.0 * (u * u);
+                    point = point + c / 2.0 * (v * v);
+                } else {
+                    point = point + c / 2.0 * u;
+                    point = point + c / 2.0 * v;
+                }
+                a.det_out[a.det_out_off[d] + static_cast<long long>(slot) * a.det_npts[d] + p] = point;
+            }
+        }
+    } else if (t < a.totpts + a.ndft) {
+        int s = t - a.totpts;
+        if (step % a.dft_interval == 0) {
+            int k = (step / a.dft_interval) % kDftBlock;
+            a.stage[static_cast<long long>(k) * a.ndft + s] = a.F[a.dft_comp[s]][a.dft_cell[s]];
+        }
+    }
+}
+
+// F[s][f] += sum_k exp(-i w_f t_k) U[k][s] over the staged samples k of the
+// current block (parallelStorageFreqDTCReal::fieldIn rank-1 updates,
+// src/DTC/parallelStorageFreqDTC.cpp:21-30, batched over kDftBlock steps).
+template <typename T>
+__global__ void k_dft_flush(const T* stage, const double* phasor, double* F, const int* stepp, int ndft, int nfreq, int interval,
+                            int force, int step_override) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int step = step_override >= 0 ? step_override : *stepp;
+    if (s >= ndft) return;
+    // samples taken so far: k = 0 .. kmax (kmax = step/interval)
+    if (step % interval != 0 && !force) return;
+    const int kmax = step / interval;
+    if (!force && (kmax % kDftBlock) != kDftBlock - 1) return;
+    const int kfirst = kmax - (kmax % kDftBlock);
+    const int nk = kmax - kfirst + 1;
+    for (int f = 0; f < nfreq; ++f) {
+        double re = 0.0, im = 0.0;
+        for (int k = 0; k < nk; ++k) {
+            double u = static_cast<double>(stage[static_cast<long long>(k) * ndft + s]);
+            const double* w = phasor + 2 * (static_cast<long long>(kfirst + k) * nfreq + f);
+            re += w[0] * u;
+            im += w[1] * u;
+        }
+        F[2 * (static_cast<long long>(s) * nfreq + f)] += re;
+        F[2 * (static_cast<long long>(s) * nfreq + f) + 1] += im;
+    }
+}
+
+// ------------------------------------------------------------ emitters ------
+struct QeDev {
+    int N, nmem, nemit;
+    const int* xyz;         // [nemit][3]
+    const double2* h0;      // [nmem][N*N]
+    const double2* mu;      // [3][N*N]
+    int ngam;
+    const int* gam_row;
+    const int* gam_col;
+    const double* gam_val;
+    double na, hbar, dt;
+    int bmin[3], bn[3];     // emitter box
+    double* Pbox[3];        // (bn+2)^3 node polarization, z-extent bn+1+zoff
+    double* Aprev[3];       // per touched E cell (bn+1)^3 previous back-coupling
+    double2* rho;           // [N*N][nmem*nemit]
+    double2* hist;          // [4][N*N][nmem*nemit]
+    int npop;
+    const int* pop_flat;
+    int pop_interval;
+    double2* pop_partial;   // [npop][nblocks]
+    double2* pop_out;       // [npop][cap]
+    int pop_cap;
+    int dim;
+    const double* eps_node;
+};
+
+// addP (src/UTIL/FDTD_up_eq.cpp:1367-1380) in increment form: a cell's
+// back-coupling A = P(p)/(2 eps(p)) + P(p+off)/(2 eps(p+off)) is subtracted
+// once per step from E derived from D, so E changes by A_prev - A_cur.
+template <typename T>
+__global__ void k_ml_addp(QeDev q, T* E0, T* E1, T* E2, int nx, int ny, int nz) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    T* E = c == 0 ? E0 : (c == 1 ? E1 : E2);
+    if (!E) return;
+    const int ex = q.bn[0] + 1, ey = q.bn[1] + 1, ez = (q.dim == 3) ? q.bn[2] + 1 : 1;
+    if (t >= ex * ey * ez) return;
+    const int x = t % ex, y = (t / ex) % ey, z = t / (ex * ey);
+    // E cell at point (bmin - 1 + x, ...) <-> P box index (x, y, z)
+    const int px = q.bmin[0] - 1 + x, py = q.bmin[1] - 1 + y, pz = (q.dim == 3) ? q.bmin[2] - 1 + z : 0;
+    if (px < 0 || py < 0 || pz < 0 || px >= nx || py >= ny || pz >= nz) return;
+    int off[3] = {0, 0, 0};
+    if (q.dim == 3) off[c] = 1;
+    const int PX = q.bn[0] + 2, PY = q.bn[1] + 2;
+    const int zi = (q.dim == 3) ? z : 0;
+    const double* P = q.Pbox[c];
+    double p0 = P[x + PX * (y + PY * zi)];
+    double p1 = P[(x + off[0]) + PX * ((y + off[1]) + PY * (zi + off[2]))];
+    const long long i0 = px + static_cast<long long>(nx) * (py + static_cast<long long>(ny) * pz);
+    int qx = px + off[0], qy = py + off[1], qz = pz + off[2];
+    double e1 = (qx < nx && qy < ny && qz < nz) ? q.eps_node[qx + static_cast<long long>(nx) * (qy + static_cast<long long>(ny) * qz)] : 1.0;
+    double A = 0.5 * p0 / q.eps_node[i0] + 0.5 * p1 / e1;
+    double* Ap = q.Aprev[c] + t;
+    double prev = *Ap;
+    *Ap = A;
+    E[i0] = static_cast<T>(static_cast<double>(E[i0]) + (prev - A));
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+// d rho/dt = (i/hbar)(H rho - rho H) + sum_j Gamma[i][j] rho_j, column-major
+// (denDeriv, src/ML/parallelQE.hpp:727-744).
+template <int N>
+__device__ __forceinline__ void den_deriv(const QeDev& q, const double2* H, const double2* r, double2* out) {
+    const double ih = 1.0 / q.hbar;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                s1 = cadd(s1, cmul(H[i + k * N], r[k + j * N]));
+                s2 = cadd(s2, cmul(r[i + k * N], H[k + j * N]));
+            }
+            // (i/hbar) s1 - (i/hbar) s2
+            double dr = s1.x - s2.x, di = s1.y - s2.y;
+            out[i + j * N] = make_double2(-di * ih, dr * ih);
+        }
+    for (int g = 0; g < q.ngam; ++g) {
+        double2 v = r[q.gam_col[g]];
+        out[q.gam_row[g]].x += v.x * q.gam_val[g];
+        out[q.gam_row[g]].y += v.y * q.gam_val[g];
+    }
+}
+
+// One thread per emitter: every broadening member advanced with the
+// Adams-Bashforth-Moulton predictor-corrector (PCABAM4, parallelQE.hpp:751-770),
+// cold-started from zero history; the derivative history is a 4-slot ring
+// instead of the reference's three copies per step (density.hpp:65-70).
+template <typename T, int N>
+__global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const T* E1, const T* E2, int nx, int ny, int nz,
+                                                    const int* stepp) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int step = *stepp;  // number of earlier density updates
+    constexpr int S = N * N;
+    __shared__ double2 red[128];
+    double2 pop[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    if (e < q.nemit) {
+        const int px = q.xyz[3 * e], py = q.xyz[3 * e + 1], pz = q.xyz[3 * e + 2];
+        double ef[3] = {0.0, 0.0, 0.0};
+        const T* Es[3] = {E0, E1, E2};
+        for (int c = 0; c < 3; ++c) {
+            if (!Es[c]) continue;
+            long long i0 = px + static_cast<long long>(nx) * (py + static_cast<long long>(ny) * pz);
+            if (q.dim == 2) ef[c] = static_cast<double>(Es[c][i0]);
+            else {
+                int o[3] = {0, 0, 0};
+                o[c] = -1;
+                int qx = px + o[0], qy = py + o[1], qz = pz + o[2];
+                double v1 = (qx >= 0 && qy >= 0 && qz >= 0) ? static_cast<double>(Es[c][qx + static_cast<long long>(nx) * (qy + static_cast<long long>(ny) * qz)]) : 0.0;
+                ef[c] = 0.5 * static_cast<double>(Es[c][i0]) + 0.5 * v1;
+            }
+        }
+        // Ring slots: the derivative from update j lives in slot j & 3, so at
+        // update u: f_n -> (u-1)&3, f_{n-1} -> (u-2)&3, f_{n-2} -> (u-3)&3,
+        // f_{n-3} -> u&3, and the new f_n overwrites slot u&3.
+        const int u = step & 3;
+        const int s0 = (u + 3) & 3, s1 = (u + 2) & 3, s2 = (u + 1) & 3, s3 = u;
+        const long long ld = static_cast<long long>(q.nmem) * q.nemit;
+        double P[3] = {0.0, 0.0, 0.0};
+        const double dt = q.dt;
+        for (int m = 0; m < q.nmem; ++m) {
+            const long long col = static_cast<long long>(m) * q.nemit + e;
+            double2 H[S], pred[S], r[S], fp[S];
+            // H = h0 - sum_c E_c mu_c (Hamiltonian::getHam)
+            for (int k = 0; k < S; ++k) {
+                double2 h = q.h0[m * S + k];
+                for (int c = 0; c < 3; ++c) {
+                    double2 mu = q.mu[c * S + k];
+                    h.x -= ef[c] * mu.x;
+                    h.y -= ef[c] * mu.y;
+                }
+                H[k] = h;
+            }
+            // Predictor rho + dt/24 (55 f_n - 59 f_{n-1} + 37 f_{n-2} - 9 f_{n-3})
+            // and the f_pred-free part of the corrector
+            // rho + dt/24 (19 f_n - 5 f_{n-1} + f_{n-2}).
+            for (int k = 0; k < S; ++k) {
+                double2 rk = q.rho[k * ld + col];
+                double2 f0 = q.hist[s0 * S * ld + k * ld + col];
+                double2 f1 = q.hist[s1 * S * ld + k * ld + col];
+                double2 f2 = q.hist[s2 * S * ld + k * ld + col];
+                double2 f3 = q.hist[s3 * S * ld + k * ld + col];
+                double2 p = rk;
+                p = cadd(p, cscale(f0, 55.0 * dt / 24.0));
+                p = cadd(p, cscale(f1, -59.0 * dt / 24.0));
+                p = cadd(p, cscale(f2, 37.0 * dt / 24.0));
+                p = cadd(p, cscale(f3, -9.0 * dt / 24.0));
+                pred[k] = p;
+                double2 cr = rk;
+                cr = cadd(cr, cscale(f0, 19.0 * dt / 24.0));
+                cr = cadd(cr, cscale(f1, -5.0 * dt / 24.0));
+                cr = cadd(cr, cscale(f2, dt / 24.0));
+                r[k] = cr;
+            }
+            den_deriv<N>(q, H, pred, fp);
+            for (int k = 0; k < S; ++k) {
+                r[k] = cadd(r[k], cscale(fp[k], 9.0 * dt / 24.0));
+                q.rho[k * ld + col] = r[k];
+            }
+            den_deriv<N>(q, H, r, fp);
+            for (int k = 0; k < S; ++k) q.hist[s3 * S * ld + k * ld + col] = fp[k];
+            for (int j = 0; j < q.npop && j < 4; ++j) pop[j] = cadd(pop[j], r[q.pop_flat[j]]);
+            // updateQEPol: P_c += na Re(sum conj(rho_k) mu_k)
+            for (int c = 0; c < 3; ++c) {
+                if (!Es[c]) continue;
+                double s = 0.0;
+                for (int k = 0; k < S; ++k) {
+                    double2 mu = q.mu[c * S + k];
+                    s += r[k].x * mu.x + r[k].y * mu.y;
+                }
+                P[c] += q.na * s;
+            }
+        }
+        const int bx = px - q.bmin[0] + 1, by = py - q.bmin[1] + 1, bz = (q.dim == 3) ? pz - q.bmin[2] + 1 : 0;
+        const int PX = q.bn[0] + 2, PY = q.bn[1] + 2;
+        for (int c = 0; c < 3; ++c)
+            if (Es[c]) q.Pbox[c][bx + PX * (by + PY * bz)] = P[c];
+    }
+    // deterministic per-block population partial sums
+    for (int j = 0; j < q.npop && j < 4; ++j) {
+        red[threadIdx.x] = pop[j];
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) q.pop_partial[j * gridDim.x + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_ml_pop_finish(QeDev q, int nblocks, const int* stepp) {
+    const int j = threadIdx.x;
+    const int step = *stepp;
+    if (j >= q.npop) return;
+    if (step % q.pop_interval != 0) return;
+    int slot = step / q.pop_interval;
+    if (slot >= q.pop_cap) return;
+    double2 s = make_double2(0.0, 0.0);
+    for (int b = 0; b < nblocks; ++b) s = cadd(s, q.pop_partial[j * nblocks + b]);
+    q.pop_out[static_cast<long long>(j) * q.pop_cap + slot] = make_double2(s.x / q.nemit, s.y / q.nemit);
+}
+
+__global__ void k_zero(double* p, long long n) {
+    long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+}  // namespace
+
+// -------------------------------------------------------------- context -----
+struct QeHost {
+    QeDev dev;
+    std::vector<std::unique_ptr<DevBuf>> bufs;
+    int nblocks = 0;
+    int N = 0;
+};
+
+struct chiml_dev {
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int prec = 64;
+    int nx = 0, ny = 0, nz = 0, dim = 3;
+    int nsteps = 0, steps_done = 0;
+    std::vector<std::unique_ptr<DevBuf>> bufs;
+    DevBuf step_ctr;
+    // parameter blocks for both precisions (only one used)
+    StepArgs<float> a32{};
+    StepArgs<double> a64{};
+    SampleArgs<float> s32{};
+    SampleArgs<double> s64{};
+    bool hmap = false, has_src = false;
+    int ndet = 0, totpts = 0, ndft = 0, nfreq = 0, dft_interval = 1;
+    std::vector<int> det_npts, det_cap, det_interval;
+    std::vector<long long> det_out_off;
+    double* det_out = nullptr;
+    double* dft_F = nullptr;
+    double* dft_tmp = nullptr;
+    void* dft_stage = nullptr;
+    const double* dft_phasor = nullptr;
+    std::vector<QeHost> qes;
+    size_t footprint = 0;
+    bool profile = false;
+    std::map<std::string, std::pair<double, int>> prof;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+
+    template <typename T>
+    T* upload(const T* src, size_t n) {
+        bufs.emplace_back(new DevBuf);
+        bufs.back()->alloc(n * sizeof(T));
+        footprint += n * sizeof(T);
+        if (src && n) HIP_TRY(hipMemcpy(bufs.back()->p, src, n * sizeof(T), hipMemcpyHostToDevice));
+        return bufs.back()->as<T>();
+    }
+    template <typename T>
+    T* zeros(size_t n) { return upload<T>(nullptr, n); }
+};
+
+namespace {
+
+template <typename T>
+T* upload_cast(chiml_dev* c, const double* src, size_t n) {
+    std::vector<T> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = static_cast<T>(src[i]);
+    return c->upload<T>(tmp.data(), n);
+}
+
+template <typename T>
+void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>& s) {
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const size_t N = static_cast<size_t>(nx) * ny * nz;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    for (int k = 0; k < 3; ++k) {
+        a.inv_d[k] = static_cast<T>(1.0 / p->d[k]);
+        a.Tp[k] = p->pml_T[k];
+    }
+    const bool is3 = nz > 1;
+    for (int k = 0; k < 6; ++k) {
+        bool present = is3 || k == 2 || k == 3 || k == 4;
+        a.F[k] = present ? c->zeros<T>(N) : nullptr;
+    }
+    for (int k = 0; k < 3; ++k) {
+        bool present = is3 || k == 2;
+        a.mat_e[k] = (present && p->mat_e[k]) ? c->upload<uint8_t>(p->mat_e[k], N) : nullptr;
+        int nm = present ? p->nmat_e[k] : 1;
+        if (present && (nm <= 0 || nm > CHIML_MAX_MAT)) throw std::runtime_error("material table size out of range");
+        if (present) {
+            a.ce[k] = upload_cast<T>(c, p->mat_e_ce[k], nm);
+            a.inveps[k] = upload_cast<T>(c, p->mat_e_inveps[k], nm);
+            a.flags[k] = c->upload<int>(p->mat_e_flags[k], nm);
+            a.poleset[k] = c->upload<int>(p->mat_e_poleset[k], nm);
+        }
+        bool hpresent = is3 || k < 2;
+        a.mat_h[k] = (hpresent && p->mat_h[k]) ? c->upload<uint8_t>(p->mat_h[k], N) : nullptr;
+        if (hpresent) {
+            int nh = p->nmat_h[k] > 0 ? p->nmat_h[k] : 1;
+            a.ch[k] = upload_cast<T>(c, p->mat_h_ch[k], nh);
+            if (a.mat_h[k]) c->hmap = true;
+        }
+    }
+    int nps = p->npoleset > 0 ? p->npoleset : 1;
+    std::vector<int> pc(nps, 0);
+    std::vector<double> pco(static_cast<size_t>(nps) * CHIML_MAX_POLES * 3, 0.0);
+    for (int k = 0; k < p->npoleset; ++k) {
+        pc[k] = p->pole_count[k];
+        if (pc[k] > CHIML_MAX_POLES) throw std::runtime_error("too many poles");
+        for (int j = 0; j < CHIML_MAX_POLES * 3; ++j) pco[k * CHIML_MAX_POLES * 3 + j] = p->pole_coef[k * CHIML_MAX_POLES * 3 + j];
+    }
+    a.pole_count = c->upload<int>(pc.data(), nps);
+    a.pole_coef = upload_cast<T>(c, pco.data(), pco.size());
+    int maxp = 0;
+    for (int k = 0; k < p->npoleset; ++k) maxp = std::max(maxp, pc[k]);
+    for (int k = 0; k < 3; ++k) {
+        for (int j = 0; j < 6; ++j) a.box[k][j] = p->disp_box[k][j];
+        long long bn = 0;
+        if (p->disp_box[k][1] > p->disp_box[k][0] && p->disp_box[k][3] > p->disp_box[k][2] && p->disp_box[k][5] > p->disp_box[k][4])
+            bn = static_cast<long long>(p->disp_box[k][1] - p->disp_box[k][0]) * (p->disp_box[k][3] - p->disp_box[k][2]) *
+                 (p->disp_box[k][5] - p->disp_box[k][4]);
+        a.boxn[k] = bn;
+        a.P[k] = (bn > 0 && maxp > 0) ? c->zeros<T>(bn * maxp) : nullptr;
+        a.J[k] = (bn > 0 && maxp > 0) ? c->zeros<T>(bn * maxp) : nullptr;
+    }
+    const int nn[3] = {nx, ny, nz};
+    for (int comp = 0; comp < 6; ++comp) {
+        PmlDev<T>& pm = a.pml[comp];
+        for (int ax = 0; ax < 3; ++ax) {
+            pm.b[ax] = pm.cd[ax] = pm.kinv[ax] = nullptr;
+            pm.psi[ax] = nullptr;
+            pm.lo[ax] = p->psi_lo[comp][ax];
+            pm.hi0[ax] = p->psi_hi0[comp][ax];
+            pm.hi1[ax] = p->psi_hi1[comp][ax];
+            pm.S[ax] = pm.lo[ax] + (pm.hi1[ax] - pm.hi0[ax]);
+            if (!p->pml_b[comp][ax]) {
+                pm.lo[ax] = 0;
+                pm.hi0[ax] = pm.hi1[ax] = nn[ax];
+                pm.S[ax] = 0;
+                continue;
+            }
+            pm.b[ax] = upload_cast<T>(c, p->pml_b[comp][ax], nn[ax]);
+            pm.cd[ax] = upload_cast<T>(c, p->pml_cd[comp][ax], nn[ax]);
+            pm.kinv[ax] = upload_cast<T>(c, p->pml_kinv[comp][ax], nn[ax]);
+            size_t ps = static_cast<size_t>(pm.S[ax]);
+            for (int o = 0; o < 3; ++o)
+                if (o != ax) ps *= nn[o];
+            pm.psi[ax] = ps ? c->zeros<T>(ps) : nullptr;
+        }
+    }
+    a.nsrc = p->nsrc;
+    c->has_src = p->nsrc > 0;
+    if (p->nsrc > 0) {
+        a.src_comp = c->upload<int>(p->src_comp, p->nsrc);
+        a.src_box = c->upload<int>(p->src_box, static_cast<size_t>(p->nsrc) * 6);
+        a.src_amp = upload_cast<T>(c, p->src_amp, static_cast<size_t>(p->nsteps) * p->nsrc);
+    }
+    a.step = c->step_ctr.as<int>();
+    // detectors and DFT
+    for (int k = 0; k < 6; ++k) s.F[k] = a.F[k];
+    s.step = a.step;
+    c->ndet = p->ndet;
+    std::vector<int> off(p->ndet + 1, 0);
+    std::vector<long long> ooff(p->ndet + 1, 0);
+    c->det_npts.assign(p->det_npts, p->det_npts + p->ndet);
+    c->det_interval.assign(p->det_interval, p->det_interval + p->ndet);
+    c->det_cap.resize(p->ndet);
+    for (int d = 0; d < p->ndet; ++d) {
+        if (p->det_interval[d] <= 0) throw std::runtime_error("detector interval must be positive");
+        off[d + 1] = off[d] + p->det_npts[d];
+        c->det_cap[d] = p->nsteps / p->det_interval[d] + 1;
+        ooff[d + 1] = ooff[d] + static_cast<long long>(c->det_cap[d]) * p->det_npts[d];
+    }
+    c->det_out_off.assign(ooff.begin(), ooff.end());
+    c->totpts = off[p->ndet];
+    s.ndet = p->ndet;
+    s.totpts = c->totpts;
+    if (p->ndet > 0) {
+        s.det_interval = c->upload<int>(p->det_interval, p->ndet);
+        s.det_npts = c->upload<int>(p->det_npts, p->ndet);
+        s.det_off = c->upload<int>(off.data(), p->ndet + 1);
+        s.det_comp = c->upload<int>(p->det_comp, 2 * static_cast<size_t>(c->totpts));
+        s.det_cell = c->upload<long long>(p->det_cell, 2 * static_cast<size_t>(c->totpts));
+        s.det_conv = c->upload<double>(p->det_conv, p->ndet);
+        s.det_pow = c->upload<int>(p->det_pow, p->ndet);
+        c->det_out = c->zeros<double>(static_cast<size_t>(ooff[p->ndet]));
+        s.det_out = c->det_out;
+        s.det_out_off = c->upload<long long>(ooff.data(), p->ndet + 1);
+        s.det_cap = c->upload<int>(c->det_cap.data(), p->ndet);
+    }
+    c->ndft = p->ndft;
+    c->nfreq = p->nfreq;
+    c->dft_interval = p->dft_interval > 0 ? p->dft_interval : 1;
+    s.ndft = p->ndft;
+    s.dft_interval = c->dft_interval;
+    if (p->ndft > 0) {
+        s.dft_comp = c->upload<int>(p->dft_comp, p->ndft);
+        s.dft_cell = c->upload<long long>(p->dft_cell, p->ndft);
+        T* st = c->zeros<T>(static_cast<size_t>(kDftBlock) * p->ndft);
+        s.stage = st;
+        c->dft_stage = st;
+        c->dft_F = c->zeros<double>(2 * static_cast<size_t>(p->ndft) * p->nfreq);
+        c->dft_tmp = c->zeros<double>(2 * static_cast<size_t>(p->ndft) * p->nfreq);
+        c->dft_phasor = c->upload<double>(p->dft_phasor, 2 * static_cast<size_t>(p->nsteps / c->dft_interval + 1) * p->nfreq);
+    }
+    // emitters
+    const double* epsn = nullptr;
+    if (p->nqe > 0) epsn = c->upload<double>(p->eps_node, N);
+    for (int g = 0; g < p->nqe; ++g) {
+        const chiml_qe_t& qi = p->qe[g];
+        QeHost qh;
+        QeDev& q = qh.dev;
+        if (qi.nlevel < 2 || qi.nlevel > kMaxLevels) throw std::runtime_error("emitter level count out of range (2..8)");
+        q.N = qi.nlevel;
+        qh.N = qi.nlevel;
+        q.nmem = qi.nmembers;
+        q.nemit = qi.nemit;
+        const int S = q.N * q.N;
+        q.xyz = c->upload<int>(qi.emit_xyz, 3 * static_cast<size_t>(qi.nemit));
+        q.h0 = reinterpret_cast<const double2*>(c->upload<double>(qi.h0, 2 * static_cast<size_t>(qi.nmembers) * S));
+        q.mu = reinterpret_cast<const double2*>(c->upload<double>(qi.mu, 2 * 3 * static_cast<size_t>(S)));
+        q.ngam = qi.ngam;
+        q.gam_row = c->upload<int>(qi.gam_row, std::max(qi.ngam, 1));
+        q.gam_col = c->upload<int>(qi.gam_col, std::max(qi.ngam, 1));
+        q.gam_val = c->upload<double>(qi.gam_val, std::max(qi.ngam, 1));
+        q.na = qi.na;
+        q.hbar = qi.hbar;
+        q.dt = p->dt;
+        q.dim = is3 ? 3 : 2;
+        for (int k = 0; k < 3; ++k) {
+            q.bmin[k] = qi.box_min[k];
+            q.bn[k] = qi.box_n[k];
+        }
+        if (!is3) q.bn[2] = 1;
+        const size_t pb = static_cast<size_t>(q.bn[0] + 2) * (q.bn[1] + 2) * (q.bn[2] + 1 + (is3 ? 1 : 0));
+        const size_t ab = static_cast<size_t>(q.bn[0] + 1) * (q.bn[1] + 1) * (is3 ? q.bn[2] + 1 : 1);
+        for (int k = 0; k < 3; ++k) {
+            q.Pbox[k] = c->zeros<double>(pb);
+            q.Aprev[k] = c->zeros<double>(ab);
+        }
+        const size_t ld = static_cast<size_t>(q.nmem) * q.nemit;
+        std::vector<double> rho0(2 * S * ld, 0.0);
+        for (int m = 0; m < q.nmem; ++m)
+            for (int e = 0; e < q.nemit; ++e) rho0[2 * (0 * ld + static_cast<size_t>(m) * q.nemit + e)] = qi.weight[m];
+        q.rho = reinterpret_cast<double2*>(c->upload<double>(rho0.data(), rho0.size()));
+        q.hist = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * S * ld));
+        q.npop = std::min(qi.npop, 4);
+        if (qi.npop > 4) throw std::runtime_error("at most 4 population detectors per emitter group");
+        q.pop_flat = c->upload<int>(qi.pop_flat, std::max(qi.npop, 1));
+        q.pop_interval = qi.pop_interval > 0 ? qi.pop_interval : 1;
+        qh.nblocks = (q.nemit + 127) / 128;
+        q.pop_partial = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(qh.nblocks)));
+        q.pop_cap = p->nsteps / q.pop_interval + 1;
+        q.pop_out = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(q.pop_cap)));
+        q.eps_node = epsn;
+        c->qes.push_back(std::move(qh));
+    }
+}
+
+template <typename T>
+void launch_sample(chiml_dev* c, SampleArgs<T>& s) {
+    int tot = c->totpts + c->ndft;
+    if (tot == 0) return;
+    hipLaunchKernelGGL(k_sample<T>, dim3((tot + 255) / 256), dim3(256), 0, c->stream, s);
+}
+
+template <typename T>
+void launch_flush(chiml_dev* c, int force) {
+    if (c->ndft == 0) return;
+    hipLaunchKernelGGL(k_dft_flush<T>, dim3((c->ndft + 127) / 128), dim3(128), 0, c->stream, static_cast<const T*>(c->dft_stage),
+                       c->dft_phasor, c->dft_F, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, force, -1);
+}
+
+struct Timed {
+    chiml_dev* c;
+    const char* name;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Timed(chiml_dev* c_, const char* n) : c(c_), name(n) {
+        if (c->profile) {
+            (void)hipEventCreate(&e0);
+            (void)hipEventCreate(&e1);
+            (void)hipEventRecord(e0, c->stream);
+        }
+    }
+    ~Timed() {
+        if (c->profile) {
+            (void)hipEventRecord(e1, c->stream);
+            c->pending.push_back({name, {e0, e1}});
+        }
+    }
+};
+
+template <typename T, int DIM>
+void launch_fields(chiml_dev* c, StepArgs<T>& a) {
+    dim3 blk(64, 4, 1);
+    dim3 grd((c->nx + 63) / 64, (c->ny + 3) / 4, c->nz);
+    {
+        Timed t(c, "update_h");
+        if (c->hmap) {
+            if (c->has_src) hipLaunchKernelGGL((k_update_h<T, DIM, true, true>), grd, blk, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_update_h<T, DIM, true, false>), grd, blk, 0, c->stream, a);
+        } else {
+            if (c->has_src) hipLaunchKernelGGL((k_update_h<T, DIM, false, true>), grd, blk, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_update_h<T, DIM, false, false>), grd, blk, 0, c->stream, a);
+        }
+    }
+    {
+        Timed t(c, "update_e");
+        if (c->has_src) hipLaunchKernelGGL((k_update_e<T, DIM, true>), grd, blk, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_update_e<T, DIM, false>), grd, blk, 0, c->stream, a);
+    }
+}
+
+template <typename T>
+void launch_ml(chiml_dev* c, StepArgs<T>& a) {
+    for (auto& qh : c->qes) {
+        QeDev& q = qh.dev;
+        int ex = q.bn[0] + 1, ey = q.bn[1] + 1, ez = (q.dim == 3) ? q.bn[2] + 1 : 1;
+        int n = ex * ey * ez;
+        hipLaunchKernelGGL(k_ml_addp<T>, dim3((n + 255) / 256, 3), dim3(256), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny,
+                           c->nz);
+        dim3 g(qh.nblocks);
+        Timed t(c, "ml_density");
+        switch (qh.N) {
+#define CHIML_ML_CASE(NN) \
+    case NN: hipLaunchKernelGGL((k_ml_density<T, NN>), g, dim3(128), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
+            CHIML_ML_CASE(2)
+            CHIML_ML_CASE(3)
+            CHIML_ML_CASE(4)
+            CHIML_ML_CASE(5)
+            CHIML_ML_CASE(6)
+            CHIML_ML_CASE(7)
+            CHIML_ML_CASE(8)
+#undef CHIML_ML_CASE
+        }
+        if (q.npop > 0) hipLaunchKernelGGL(k_ml_pop_finish, dim3(1), dim3(64), 0, c->stream, q, qh.nblocks, a.step);
+    }
+}
+
+template <typename T>
+void do_steps(chiml_dev* c, StepArgs<T>& a, SampleArgs<T>& s, int n) {
+    for (int k = 0; k < n; ++k) {
+        if (c->dim == 3) launch_fields<T, 3>(c, a);
+        else launch_fields<T, 2>(c, a);
+        if (!c->qes.empty()) launch_ml<T>(c, a);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->step_ctr.as<int>());
+        launch_sample<T>(c, s);
+        launch_flush<T>(c, 0);
+        HIP_TRY(hipGetLastError());
+    }
+    c->steps_done += n;
+}
+
+void collect_profile(chiml_dev* c) {
+    for (auto& pe : c->pending) {
+        float ms = 0.0f;
+        (void)hipEventSynchronize(pe.second.second);
+        (void)hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
+        auto& slot = c->prof[pe.first];
+        slot.first += ms;
+        slot.second += 1;
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    c->pending.clear();
+}
+
+int fail(chiml_dev* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    g_err = msg;
+    return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
+    if (!p || !out) return fail(nullptr, CHIML_E_INVALID, "null argument");
+    *out = nullptr;
+    if (p->nx < 2 || p->ny < 2 || p->nz < 1) return fail(nullptr, CHIML_E_INVALID, "grid too small");
+    if (p->precision != 32 && p->precision != 64) return fail(nullptr, CHIML_E_INVALID, "precision must be 32 or 64");
+    if (p->nsteps < 0) return fail(nullptr, CHIML_E_INVALID, "nsteps must be >= 0");
+    if (p->nsrc > CHIML_MAX_SRC) return fail(nullptr, CHIML_E_INVALID, "too many sources");
+    auto c = std::make_unique<chiml_dev>();
+    try {
+        int ndev = 0;
+        HIP_TRY(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw std::runtime_error("no HIP device");
+        HIP_TRY(hipGetDevice(&c->device));
+        HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->prec = p->precision;
+        c->nx = p->nx;
+        c->ny = p->ny;
+        c->nz = p->nz;
+        c->dim = p->nz > 1 ? 3 : 2;
+        c->nsteps = p->nsteps;
+        c->step_ctr.alloc(sizeof(int));
+        if (c->prec == 32) build<float>(c.get(), p, c->a32, c->s32);
+        else build<double>(c.get(), p, c->a64, c->s64);
+        // t = 0 samples (taken by the reference's constructor, parallelFDTDField.cpp:832-837)
+        if (c->prec == 32) {
+            launch_sample<float>(c.get(), c->s32);
+        } else {
+            launch_sample<double>(c.get(), c->s64);
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    } catch (std::exception& e) {
+        return fail(nullptr, CHIML_E_HIP, e.what());
+    }
+    *out = c.release();
+    return CHIML_OK;
+}
+
+int chiml_dev_step(chiml_dev_t* c, int n) {
+    if (!c || n < 0) return fail(c, CHIML_E_INVALID, "invalid argument");
+    if (c->steps_done + n > c->nsteps) return fail(c, CHIML_E_STATE, "step count exceeds the per-step tables built at create");
+    try {
+        if (c->prec == 32) do_steps<float>(c, c->a32, c->s32, n);
+        else do_steps<double>(c, c->a64, c->s64, n);
+        if (c->profile) collect_profile(c);
+    } catch (std::exception& e) {
+        return fail(c, CHIML_E_HIP, e.what());
+    }
+    return CHIML_OK;
+}
+
+int chiml_dev_step_batch(chiml_dev_t** ctxs, int n, int nsteps) {
+    if (!ctxs || n < 0) return fail(nullptr, CHIML_E_INVALID, "invalid argument");
+    for (int s = 0; s < nsteps; ++s)
+        for (int i = 0; i < n; ++i) {
+            int rc = chiml_dev_step(ctxs[i], 1);
+            if (rc) return rc;
+        }
+    return CHIML_OK;
+}
+
+int chiml_dev_sync(chiml_dev_t* c) {
+    if (!c) return fail(nullptr, CHIML_E_INVALID, "null context");
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, CHIML_E_HIP, hipGetErrorString(e));
+    return CHIML_OK;
+}
+
+int chiml_dev_read_field(chiml_dev_t* c, int comp, double* host) {
+    if (!c || !host || comp < 0 || comp > 5) return fail(c, CHIML_E_INVALID, "invalid argument");
+    size_t N = static_cast<size_t>(c->nx) * c->ny * c->nz;
+    try {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->prec == 32) {
+            if (!c->a32.F[comp]) { std::memset(host, 0, N * sizeof(double)); return CHIML_OK; }
+            std::vector<float> tmp(N);
+            HIP_TRY(hipMemcpy(tmp.data(), c->a32.F[comp], N * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; ++i) host[i] = tmp[i];
+        } else {
+            if (!c->a64.F[comp]) { std::memset(host, 0, N * sizeof(double)); return CHIML_OK; }
+            HIP_TRY(hipMemcpy(host, c->a64.F[comp], N * sizeof(double), hipMemcpyDeviceToHost));
+        }
+    } catch (std::exception& e) {
+        return fail(c, CHIML_E_HIP, e.what());
+    }
+    return CHIML_OK;
+}
+
+int chiml_dev_read_points(chiml_dev_t* c, int d, double* values, size_t cap, size_t* nsamples) {
+    if (!c || d < 0 || d >= c->ndet || !nsamples) return fail(c, CHIML_E_INVALID, "invalid argument");
+    int ns = c->steps_done / c->det_interval[d] + 1;
+    if (ns > c->det_cap[d]) ns = c->det_cap[d];
+    *nsamples = static_cast<size_t>(ns);
+    size_t n = static_cast<size_t>(ns) * c->det_npts[d];
+    if (!values) return CHIML_OK;
+    if (cap < n) return fail(c, CHIML_E_INVALID, "buffer too small");
+    try {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipMemcpy(values, c->det_out + c->det_out_off[d], n * sizeof(double), hipMemcpyDeviceToHost));
+    } catch (std::exception& e) {
+        return fail(c, CHIML_E_HIP, e.what());
+    }
+    return CHIML_OK;
+}
+
+int chiml_dev_read_dft(chiml_dev_t* c, double* reim, size_t cap) {
+    if (!c) return fail(nullptr, CHIML_E_INVALID, "null context");
+    size_t n = 2 * static_cast<size_t>(c->ndft) * c->nfreq;
+    if (n == 0) return CHIML_OK;
+    if (!reim || cap < n) return fail(c, CHIML_E_INVALID, "buffer too small");
+    try {
+        // Contract the partially filled staging block into a copy of F.
+        int kmax = c->steps_done / c->dft_interval;
+        bool partial = (kmax % kDftBlock) != kDftBlock - 1;
+        std::vector<double> host(n);
+        if (partial) {
+            // Contract samples kfirst..kmax of the open block into a copy of F.
+            double* tmp = c->dft_tmp;
+            HIP_TRY(hipMemcpyAsync(tmp, c->dft_F, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            int last = kmax * c->dft_interval;
+            if (c->prec == 32)
+                hipLaunchKernelGGL(k_dft_flush<float>, dim3((c->ndft + 127) / 128), dim3(128), 0, c->stream, static_cast<const float*>(c->dft_stage),
+                                   c->dft_phasor, tmp, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, 1, last);
+            else
+                hipLaunchKernelGGL(k_dft_flush<double>, dim3((c->ndft + 127) / 128), dim3(128), 0, c->stream, static_cast<const double*>(c->dft_stage),
+                                   c->dft_phasor, tmp, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, 1, last);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            HIP_TRY(hipMemcpy(reim, tmp, n * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            HIP_TRY(hipMemcpy(reim, c->dft_F, n * sizeof(double), hipMemcpyDeviceToHost));
+        }
+    } catch (std::exception& e) {
+        return fail(c, CHIML_E_HIP, e.what());
+    }
+    return CHIML_OK;
+}
+
+int chiml_dev_read_pops(chiml_dev_t* c, int q, int j, double* reim, size_t cap, size_t* nsamples) {
+    if (!c || q < 0 || q >= static_cast<int>(c->qes.size()) || !nsamples) return fail(c, CHIML_E_INVALID, "invalid argument");
+    QeDev& d = c->qes[q].dev;
+    if (j < 0 || j >= d.npop) return fail(c, CHIML_E_INVALID, "population detector index out of range");
+    int ns = (c->steps_done - 1) / d.pop_interval + 1;
+    if (c->steps_done == 0) ns = 0;
+    if (ns > d.pop_cap) ns = d.pop_cap;
+    *nsamples = static_cast<size_t>(ns);
+    if (!reim) return CHIML_OK;
+    if (cap < 2 * static_cast<size_t>(ns)) return fail(c, CHIML_E_INVALID, "buffer too small");
+    try {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipMemcpy(reim, d.pop_out + static_cast<size_t>(j) * d.pop_cap, 2 * ns * sizeof(double), hipMemcpyDeviceToHost));
+    } catch (std::exception& e) {
+        return fail(c, CHIML_E_HIP, e.what());
+    }
+    return CHIML_OK;
+}
+
+int chiml_dev_read_density(chiml_dev_t* c, int q, int m, int e, double* reim) {
+    if (!c || q < 0 || q >= static_cast<int>(c->qes.size()) || !reim) return fail(c, CHIML_E_INVALID, "invalid argument");
+    QeDev& d = c->qes[q].dev;
+    if (m < 0 || m >= d.nmem || e < 0 || e >= d.nemit) return fail(c, CHIML_E_INVALID, "index out of range");
+    const int S = d.N * d.N;
+    const size_t ld = static_cast<size_t>(d.nmem) * d.nemit;
+    try {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < S; ++k)
+            HIP_TRY(hipMemcpy(reim + 2 * k, d.rho + k * ld + static_cast<size_t>(m) * d.nemit + e, sizeof(double2), hipMemcpyDeviceToHost));
+    } catch (std::exception& ex) {
+        return fail(c, CHIML_E_HIP, ex.what());
+    }
+    return d.N;
+}
+
+int chiml_dev_steps_done(chiml_dev_t* c, int* steps) {
+    if (!c || !steps) return fail(c, CHIML_E_INVALID, "invalid argument");
+    *steps = c->steps_done;
+    return CHIML_OK;
+}
+
+int chiml_dev_profile(chiml_dev_t* c, int enable) {
+    if (!c) return fail(nullptr, CHIML_E_INVALID, "null context");
+    c->profile = enable != 0;
+    c->prof.clear();
+    return CHIML_OK;
+}
+
+int chiml_dev_profile_read(chiml_dev_t* c, const char* kernel, double* total_ms, int* launches) {
+    if (!c || !kernel || !total_ms || !launches) return fail(c, CHIML_E_INVALID, "invalid argument");
+    collect_profile(c);
+    auto it = c->prof.find(kernel);
+    *total_ms = it == c->prof.end() ? 0.0 : it->second.first;
+    *launches = it == c->prof.end() ? 0 : it->second.second;
+    return CHIML_OK;
+}
+
+int chiml_dev_footprint(chiml_dev_t* c, size_t* bytes) {
+    if (!c || !bytes) return fail(c, CHIML_E_INVALID, "invalid argument");
+    *bytes = c->footprint;
+    return CHIML_OK;
+}
+
+const char* chiml_dev_last_error(const chiml_dev_t* c) { return c ? c->err.c_str() : g_err.c_str(); }
+const char* chiml_dev_global_error(void) { return g_err.c_str(); }
+
+void chiml_dev_destroy(chiml_dev_t* c) {
+    if (!c) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& pe : c->pending) {
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    c->bufs.clear();
+    c->qes.clear();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // extern "C"

@@ -1,0 +1,1127 @@
+// JSON-driven manager: builds the device tables of one simulation from a
+// chiML deck and exposes the C ABI of include/chiml/chiml_sim.h.
+//
+// It plays the role of parallelFDTDFieldReal's constructor
+// (src/FDTD_MANAGER/parallelFDTDField.cpp:13-852): material maps
+// (setupPhysFields, parallelFDTDField.hpp:867-947), the material-in-PML scan
+// (:274-370), CPML profiles (src/PML/parallelPML.hpp:102-806), ADE
+// coefficients (src/OBJECTS/Obj.cpp:299-371), sources
+// (src/SOURCE/parallelSourceNormal.hpp:69-150), point detectors
+// (src/DTC/parallelDTC.hpp:44-95), flux boxes (src/DTC/parallelFlux.cpp:13-71,
+// 338-543) and emitters (src/ML/parallelQE.hpp:190-470).  Instead of x-run
+// lists it produces per-cell material ids and 1D CPML profiles that the GPU
+// kernels read directly.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/chiml/chiml_dev.h"
+#include "../../include/chiml/chiml_sim.h"
+#include "setup.hpp"
+
+namespace chiml {
+namespace {
+
+std::string g_sim_err;
+
+// Staggering of each Yee component along x, y, z (1 = half-integer position):
+// Ex (1,0,0) Ey (0,1,0) Ez (0,0,1) Hx (0,1,1) Hy (1,0,1) Hz (1,1,0)
+// (offsets of setupPhysFields, parallelFDTDField.hpp:264-272).
+const int kStag[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+
+double fact(int k) { return std::tgamma(k + 1.0); }
+
+// Wigner 3j symbol, Racah formula (replaces GSL's gsl_sf_coupling_3j used by
+// BasisSet::sphHarmonicIntegrator, src/ML/BasisSet.cpp:46-52).
+double threej(int j1, int j2, int j3, int m1, int m2, int m3) {
+    if (m1 + m2 + m3 != 0) return 0.0;
+    if (j3 < std::abs(j1 - j2) || j3 > j1 + j2) return 0.0;
+    if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3) return 0.0;
+    double tri = fact(j1 + j2 - j3) * fact(j1 - j2 + j3) * fact(-j1 + j2 + j3) / fact(j1 + j2 + j3 + 1);
+    double pre = std::sqrt(tri * fact(j1 + m1) * fact(j1 - m1) * fact(j2 + m2) * fact(j2 - m2) * fact(j3 + m3) * fact(j3 - m3));
+    double s = 0.0;
+    int kmin = std::max(0, std::max(j2 - j3 - m1, j1 - j3 + m2));
+    int kmax = std::min(j1 + j2 - j3, std::min(j1 - m1, j2 + m2));
+    for (int k = kmin; k <= kmax; ++k)
+        s += ((k % 2) ? -1.0 : 1.0) /
+             (fact(k) * fact(j1 + j2 - j3 - k) * fact(j1 - m1 - k) * fact(j2 + m2 - k) * fact(j3 - j2 + m1 + k) * fact(j3 - j1 - m2 + k));
+    return (((j1 - j2 - m3) % 2) != 0 ? -1.0 : 1.0) * pre * s;
+}
+
+// Angular integral <Y_a|Y_b|Y_c> with the reference's phase convention.
+double ylmIntegral(std::array<int, 2> a, std::array<int, 2> b, std::array<int, 2> c) {
+    double v = std::sqrt((a[0] * 2 + 1) * (b[0] * 2 + 1) * (c[0] * 2 + 1) / (4 * kPi));
+    return v * std::pow(-1.0, a[1]) * threej(a[0], b[0], c[0], 0, 0, 0) * threej(a[0], b[0], c[0], -a[1], b[1], c[1]);
+}
+
+// Dipole operator matrices in a spherical-harmonic basis, flat index ii*N+jj
+// (BasisSet::expectationVals, src/ML/BasisSet.cpp:17-43).
+std::vector<cplx> basisDipole(const std::vector<std::array<int, 2>>& B, int dir) {
+    const int N = static_cast<int>(B.size());
+    std::vector<cplx> out(N * N);
+    const std::array<int, 2> y10 = {{1, 0}}, y11 = {{1, 1}}, y1m = {{1, -1}};
+    for (int ii = 0; ii < N; ++ii)
+        for (int jj = 0; jj < N; ++jj) {
+            cplx v;
+            if (dir == 0) v = ylmIntegral(B[ii], y1m, B[jj]) - ylmIntegral(B[ii], y11, B[jj]);
+            else if (dir == 1) v = cplx(0.0, ylmIntegral(B[ii], y11, B[jj]) + ylmIntegral(B[ii], y1m, B[jj]));
+            else v = std::sqrt(2.0) * ylmIntegral(B[ii], y10, B[jj]);
+            out[ii * N + jj] = v * std::sqrt(2 * kPi / 3);
+        }
+    return out;
+}
+
+// Column of the Lindblad superoperator for operator A (column-major N x N):
+// A^H r A - (A^H A r + r A^H A)/2, as built by parallelQEBase::linblad
+// (src/ML/parallelQE.hpp:371-392).
+std::vector<double> lindbladColumn(const std::vector<double>& A, int unit, int N) {
+    auto at = [N](const std::vector<double>& M, int r, int c) { return M[r + c * N]; };
+    std::vector<double> rho(N * N, 0.0), out(N * N, 0.0);
+    rho[unit] = 1.0;
+    std::vector<double> AhA(N * N, 0.0);
+    for (int r = 0; r < N; ++r)
+        for (int c = 0; c < N; ++c)
+            for (int k = 0; k < N; ++k) AhA[r + c * N] += at(A, k, r) * at(A, k, c);
+    for (int r = 0; r < N; ++r)
+        for (int c = 0; c < N; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < N; ++k)
+                for (int l = 0; l < N; ++l) s += at(A, k, r) * at(rho, k, l) * at(A, l, c);
+            double t1 = 0.0, t2 = 0.0;
+            for (int k = 0; k < N; ++k) {
+                t1 += at(AhA, r, k) * at(rho, k, c);
+                t2 += at(rho, r, k) * at(AhA, k, c);
+            }
+            out[r + c * N] = s - 0.5 * t1 - 0.5 * t2;
+        }
+    return out;
+}
+
+struct Store {  // one frequency-domain storage box of a flux face
+    int comp;
+    std::array<int, 3> lo, n;  // clipped box in interior points
+    std::vector<int> ids;      // DFT sample id per box point (x fastest)
+};
+struct Face {
+    int transCor1, cor;
+    std::array<int, 3> sz;
+    double weight;
+    std::vector<Store> Ej, Ek, Hj, Hk;
+    int padEj = -1, padEk = -1, padHj = -1, padHk = -1;
+};
+struct FluxBox {
+    std::vector<Face> faces;
+    std::vector<double> freqs;
+    int timeInt = 1;
+};
+
+}  // namespace
+
+class FDTDField {
+  public:
+    explicit FDTDField(const std::string& deck);
+    ~FDTDField() {
+        if (dev_) chiml_dev_destroy(dev_);
+    }
+    ProgramInputs IP;
+    std::array<int, 3> n{};
+    bool is3D = false;
+    bool dielectricMatInPML = false;
+    chiml_dev_t* dev_ = nullptr;
+    std::vector<FluxBox> fluxes;
+    int ndft = 0, nfreq = 0, dftInterval = 1;
+    std::vector<int> detNpts;
+    std::vector<double> detTConv;
+    std::vector<int> detInterval;
+    std::vector<int> qeN;
+    double cells = 0, bytesH = 0, bytesE = 0, bytesML = 0;
+    size_t estimate = 0;
+
+    std::vector<cplx> flux(int f);
+
+  private:
+    int objAt(std::array<double, 3> pt, bool ml) const;
+    void scanMatInPML();
+};
+
+int FDTDField::objAt(std::array<double, 3> pt, bool ml) const {
+    int id = 0;
+    for (size_t oo = 1; oo < IP.objs.size(); ++oo) {
+        const Object& o = *IP.objs[oo];
+        bool relevant = o.ML || !o.pols.empty() || o.anyEpsNotOne() || o.mu != 1.0;
+        if (relevant && o.isObj(pt, IP.d[0], ml)) id = static_cast<int>(oo);
+    }
+    return id;
+}
+
+// Material-in-PML scan, src/FDTD_MANAGER/parallelFDTDField.hpp:274-370.  The
+// reference re-uses the upper-side loop bounds after the first (jj, kk) pair,
+// so the lower-side PML is probed on one line only; that is kept.
+void FDTDField::scanMatInPML() {
+    const auto& d = IP.d;
+    for (int pp = 0; pp < 3; ++pp) {
+        const int ci = pp, cj = (pp + 1) % 3, ck = (pp + 2) % 3;
+        std::array<int, 3> mn, mx;
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = static_cast<int>(-1 * static_cast<double>(n[a]) / 2);
+            mx[a] = static_cast<int>(static_cast<double>(n[a]) / 2);
+        }
+        mx[ci] = static_cast<int>(IP.pmlThickness[ci] - n[ci] / 2.0);
+        for (auto& op : IP.objs) {
+            const Object& o = *op;
+            if (!o.anyEpsNotOne() && o.mu == 1.0 && o.pols.empty() && !o.ML) continue;
+            const bool diel = o.anyEpsNotOne() || !o.pols.empty();
+            for (int jj = mn[cj]; jj < mx[cj]; ++jj)
+                for (int kk = mn[ck]; kk < mx[ck]; ++kk)
+                    for (int pass = 0; pass < 2; ++pass) {
+                        if (pass == 1) {
+                            mn[ci] = n[ci] / 2 - IP.pmlThickness[ci];
+                            mx[ci] = n[ci] / 2;
+                        }
+                        if (dielectricMatInPML) continue;
+                        for (int ii = mn[ci]; ii < mx[ci] && !dielectricMatInPML; ++ii) {
+                            std::array<double, 3> pt = {{0, 0, 0}};
+                            pt[ci] = ii * d[ci];
+                            pt[cj] = jj * d[cj];
+                            pt[ck] = kk * d[ck];
+                            pt[0] += d[0] / 2.0;
+                            if (o.isObj(pt, d[0], false)) dielectricMatInPML = diel;
+                            pt[1] += d[1] / 2.0;
+                            pt[0] -= d[0] / 2.0;
+                            if (!dielectricMatInPML && o.isObj(pt, d[0], false)) dielectricMatInPML = diel;
+                            pt[2] += d[2] / 2.0;
+                            pt[1] -= d[0] / 2.0;
+                            if (!dielectricMatInPML && o.isObj(pt, d[0], false)) dielectricMatInPML = diel;
+                        }
+                    }
+            if (o.mu != 1.0) throw std::logic_error("magnetic materials (mu != 1) are outside the supported time-stepping path");
+        }
+    }
+}
+
+FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
+    is3D = IP.is3D();
+    n = IP.nVec();
+    if (!is3D) n[2] = 1;
+    const int nx = n[0], ny = n[1], nz = n[2];
+    const size_t N = static_cast<size_t>(nx) * ny * nz;
+    const double dt = IP.dt;
+    const int nsteps = IP.nSteps();
+    scanMatInPML();
+    for (auto& o : IP.objs)
+        if (o->mu != 1.0) throw std::logic_error("magnetic materials (mu != 1) are outside the supported time-stepping path");
+
+    chiml_problem_t P;
+    std::memset(&P, 0, sizeof(P));
+    P.nx = nx;
+    P.ny = ny;
+    P.nz = nz;
+    P.precision = IP.fp32 ? 32 : 64;
+    P.dt = dt;
+    for (int a = 0; a < 3; ++a) P.d[a] = IP.d[a];
+    P.nsteps = nsteps;
+    const std::array<int, 3> T = {{IP.pmlThickness[0], IP.pmlThickness[1], is3D ? IP.pmlThickness[2] : 0}};
+    for (int a = 0; a < 3; ++a) P.pml_T[a] = T[a];
+    auto inPML = [&](int x, int y, int z) {
+        return x < T[0] || x >= nx - T[0] || y < T[1] || y >= ny - T[1] || (is3D && (z < T[2] || z >= nz - T[2]));
+    };
+    // Centred coordinate of interior point p on axis a for offset o
+    // (setupPhysFields, parallelFDTDField.hpp:888-890).
+    auto coord = [&](int a, int p, double o) { return (p + o - (n[a] - n[a] % 2) / 2.0) * IP.d[a]; };
+    const int nAll[3] = {nx, ny, is3D ? IP.nVec()[2] : 1};
+
+    // ---- material maps ------------------------------------------------------
+    std::vector<int> ecomps = is3D ? std::vector<int>{0, 1, 2} : std::vector<int>{2};
+    std::vector<int> hcomps = is3D ? std::vector<int>{3, 4, 5} : std::vector<int>{3, 4};
+    std::vector<std::vector<int>> physOf(6);  // object id per point (geoML), per component
+    std::vector<std::vector<uint8_t>> matE(3);
+    std::vector<double> ceT[3], ievT[3];
+    std::vector<int> flT[3], psT[3];
+    // pole sets, one per object with poles
+    std::vector<int> poleSetOf(IP.objs.size(), -1);
+    std::vector<int> pcount;
+    std::vector<double> pcoef;
+    for (size_t oo = 0; oo < IP.objs.size(); ++oo) {
+        const Object& o = *IP.objs[oo];
+        if (o.alpha.empty()) continue;
+        if (o.alpha.size() > CHIML_MAX_POLES) throw std::logic_error("too many poles on one object");
+        poleSetOf[oo] = static_cast<int>(pcount.size());
+        pcount.push_back(static_cast<int>(o.alpha.size()));
+        std::vector<double> co(CHIML_MAX_POLES * 3, 0.0);
+        for (size_t p = 0; p < o.alpha.size(); ++p) {
+            // current form of P' = alpha P + xi P_prev + gamma E:
+            // J' = (alpha + xi - 1) P - xi J + gamma E, alpha + xi - 1 = -(w dt)^2/(1 + g dt)
+            const Pole& pl = o.pols[p];
+            double od = pl.omg * dt;
+            co[3 * p] = -(od * od) / (1.0 + pl.gam * dt);
+            co[3 * p + 1] = -o.xi[p];
+            co[3 * p + 2] = o.gamma[p];
+        }
+        pcoef.insert(pcoef.end(), co.begin(), co.end());
+    }
+    int dispBox[3][6];
+    for (int c : ecomps) {
+        const int* st = kStag[c];
+        std::map<std::tuple<int, double, int>, int> ids;
+        matE[c].assign(N, 0);
+        physOf[c].assign(N, 0);
+        int bx[6] = {nx, -1, ny, -1, nz, -1};
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    std::array<double, 3> pt = {{coord(0, x, 0.5 * st[0]), coord(1, y, 0.5 * st[1]), is3D ? coord(2, z, 0.5 * st[2]) : 0.0}};
+                    if (!is3D) pt[2] = (-1 + 0.5 * st[2] - (nAll[2] - nAll[2] % 2) / 2.0) * IP.d[2];  // kk - 1 with kk = 0
+                    int phys = objAt(pt, true);
+                    int epsObj = objAt(pt, false);
+                    double eps = IP.objs[epsObj]->eps[c];
+                    const size_t i = x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z);
+                    physOf[c][i] = phys;
+                    bool pml = inPML(x, y, z);
+                    const Object& po = *IP.objs[phys];
+                    bool disp = !po.alpha.empty() || po.ML;
+                    if (pml && disp && !dielectricMatInPML)
+                        throw std::logic_error("dispersive or emitter material inside the PML without dielectric material in the PML is not supported");
+                    auto key = std::make_tuple(phys, eps, pml ? 1 : 0);
+                    auto it = ids.find(key);
+                    int id;
+                    if (it == ids.end()) {
+                        id = static_cast<int>(ids.size());
+                        if (id >= CHIML_MAX_MAT) throw std::logic_error("too many distinct materials");
+                        ids[key] = id;
+                        double ce = (pml && !dielectricMatInPML) ? dt : dt / eps;
+                        ceT[c].push_back(ce);
+                        ievT[c].push_back(1.0 / eps);
+                        int fl = (disp || (pml && dielectricMatInPML)) ? CHIML_MAT_DCELL : 0;
+                        flT[c].push_back(fl);
+                        psT[c].push_back(poleSetOf[phys]);
+                    } else {
+                        id = it->second;
+                    }
+                    matE[c][i] = static_cast<uint8_t>(id);
+                    if (poleSetOf[phys] >= 0) {
+                        bx[0] = std::min(bx[0], x);
+                        bx[1] = std::max(bx[1], x + 1);
+                        bx[2] = std::min(bx[2], y);
+                        bx[3] = std::max(bx[3], y + 1);
+                        bx[4] = std::min(bx[4], z);
+                        bx[5] = std::max(bx[5], z + 1);
+                    }
+                }
+        if (bx[1] < 0) bx[0] = bx[1] = bx[2] = bx[3] = bx[4] = bx[5] = 0;
+        for (int j = 0; j < 6; ++j) dispBox[c][j] = bx[j];
+        P.mat_e[c] = matE[c].data();
+        P.nmat_e[c] = static_cast<int>(ceT[c].size());
+        P.mat_e_ce[c] = ceT[c].data();
+        P.mat_e_inveps[c] = ievT[c].data();
+        P.mat_e_flags[c] = flT[c].data();
+        P.mat_e_poleset[c] = psT[c].data();
+        for (int j = 0; j < 6; ++j) P.disp_box[c][j] = dispBox[c][j];
+    }
+    static const double kOne = 1.0;
+    double chv = dt;
+    for (int hc : hcomps) {
+        int c = hc - 3;
+        P.mat_h[c] = nullptr;
+        P.nmat_h[c] = 1;
+        P.mat_h_ch[c] = &chv;
+        const int* st = kStag[hc];
+        physOf[hc].assign(N, 0);
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    std::array<double, 3> pt = {{coord(0, x, 0.5 * st[0]), coord(1, y, 0.5 * st[1]), is3D ? coord(2, z, 0.5 * st[2]) : 0.0}};
+                    if (!is3D) pt[2] = (-1 + 0.5 * st[2] - (nAll[2] - nAll[2] % 2) / 2.0) * IP.d[2];  // kk - 1 with kk = 0
+                    physOf[hc][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)] = objAt(pt, true);
+                }
+    }
+    (void)kOne;
+    P.npoleset = static_cast<int>(pcount.size());
+    P.pole_count = pcount.data();
+    P.pole_coef = pcoef.data();
+
+    // ---- CPML profiles -------------------------------------------------------
+    // eta_eff per plane: mean eps_c over the plane times mean mu
+    // (genEtaEff/genEtaEffVal, parallelPML.hpp:192-275), lower side planes
+    // 0..T-1, upper side planes n-T..n-1 stored in that order and read with the
+    // distance from the outer edge, as the reference does.
+    std::vector<double> prof[6][3][3];  // [comp][axis][b, cd, kinv]
+    const double sMaxBase = IP.pmlSigOptRat * 0.8 * (IP.pmlM + 1);
+    auto kappaF = [&](double ii, double iiMax) {
+        return (0.0 <= ii && ii <= iiMax) ? 1.0 + (IP.pmlKappaMax - 1.0) * std::pow((iiMax - ii) / iiMax, IP.pmlM) : 1.0;
+    };
+    auto sigmaF = [&](double ii, double iiMax, double eta, double sMax) {
+        return (0.0 <= ii && ii <= iiMax) ? sMax / eta * std::pow((iiMax - ii) / iiMax, IP.pmlM) : 0.0;
+    };
+    auto aF = [&](double ii, double iiMax) { return (0.0 <= ii && ii <= iiMax) ? IP.pmlAMax * std::pow(ii / iiMax, IP.pmlMa) : 0.0; };
+    for (int comp = 0; comp < 6; ++comp) {
+        bool present = is3D || comp == 2 || comp == 3 || comp == 4;
+        for (int a = 0; a < 3; ++a) {
+            P.pml_b[comp][a] = nullptr;
+            P.psi_lo[comp][a] = 0;
+            P.psi_hi0[comp][a] = P.psi_hi1[comp][a] = (a == 0 ? nx : (a == 1 ? ny : nz));
+        }
+        if (!present) continue;
+        const bool E = comp < 3;
+        const int ci = comp % 3;
+        const int cc = comp % 3;  // eps component used for this grid's eta
+        for (int a = 0; a < 3; ++a) {
+            if (a == ci) continue;          // derivative axes only
+            if (!is3D && a == 2) continue;  // no z derivatives in 2D
+            const int Ta = T[a];
+            if (Ta == 0) continue;
+            const int na = (a == 0 ? nx : (a == 1 ? ny : nz));
+            // eta_eff planes along a for this component (plane size excludes the
+            // staggered last point, parallelPML.hpp:223-232)
+            const int* st = kStag[comp];
+            std::array<int, 3> ps = {{nx - st[0], ny - st[1], is3D ? nz - st[2] : 1}};
+            auto etaPlane = [&](int plane) {
+                double es = 0.0, ms = 0.0;
+                size_t cnt = static_cast<size_t>(ps[(a + 1) % 3]) * ps[(a + 2) % 3];
+                std::array<int, 3> q;
+                q[a] = plane;
+                for (int u = 0; u < ps[(a + 1) % 3]; ++u)
+                    for (int v = 0; v < ps[(a + 2) % 3]; ++v) {
+                        q[(a + 1) % 3] = u;
+                        q[(a + 2) % 3] = v;
+                        if (q[a] >= na - st[a]) continue;  // marked -1 in the reference map
+                        int id = physOf[comp][q[0] + static_cast<size_t>(nx) * (q[1] + static_cast<size_t>(ny) * q[2])];
+                        es += IP.objs[id]->eps[cc] / static_cast<double>(cnt);
+                        ms += IP.objs[id]->mu / static_cast<double>(cnt);
+                    }
+                return es * ms;
+            };
+            std::vector<double> etaLo, etaHi;
+            for (int pl = 0; pl < Ta; ++pl) {
+                double v = etaPlane(pl);
+                if (v != 0.0) etaLo.push_back(v);
+            }
+            for (int pl = na - Ta; pl < na; ++pl) {
+                double v = etaPlane(pl);
+                if (v != 0.0 && static_cast<int>(etaHi.size()) < Ta) etaHi.push_back(v);
+            }
+            std::vector<double>& b = prof[comp][a][0];
+            std::vector<double>& cd = prof[comp][a][1];
+            std::vector<double>& kinv = prof[comp][a][2];
+            b.assign(na, 1.0);
+            cd.assign(na, 0.0);
+            kinv.assign(na, 1.0);
+            const double sMax = sMaxBase / IP.d[a];
+            const double distOff = E ? 0.0 : 0.5;
+            const int lo = E ? Ta : Ta - 1;
+            const int hi0 = na - Ta, hi1 = E ? na : na - 1;
+            auto fill = [&](int pos, double dist, const std::vector<double>& eta) {
+                size_t ei = static_cast<size_t>(dist);
+                if (ei >= eta.size()) throw std::logic_error("CPML eta_eff index out of range");
+                double sig = sigmaF(dist, Ta - 1.0, eta[ei], sMax);
+                double kap = kappaF(dist, Ta - 1.0);
+                double av = aF(dist, Ta - 1.0);
+                double bv = std::exp(-1.0 * (sig / kap + av) * dt);
+                double cv = (sig == 0 && av == 0) ? 0 : sig * (bv - 1.0) / (kap * (sig + kap * av));
+                b[pos] = bv;
+                cd[pos] = cv / IP.d[a];
+                kinv[pos] = 1.0 / kap;
+            };
+            for (int pos = 0; pos < lo; ++pos) fill(pos, pos + distOff, etaLo);
+            for (int pos = hi0; pos < hi1; ++pos) fill(pos, (na - 1) - (pos + distOff), etaHi);
+            P.pml_b[comp][a] = b.data();
+            P.pml_cd[comp][a] = cd.data();
+            P.pml_kinv[comp][a] = kinv.data();
+            P.psi_lo[comp][a] = std::max(lo, 0);
+            P.psi_hi0[comp][a] = hi0;
+            P.psi_hi1[comp][a] = hi1;
+        }
+    }
+
+    // ---- sources -------------------------------------------------------------
+    std::vector<int> srcComp, srcBox;
+    std::vector<double> srcAmp;
+    std::vector<const SourceSpec*> activeSrc;
+    for (auto& s : IP.srcs) {
+        int comp = static_cast<int>(s.pol);
+        bool present = is3D || comp == 2 || comp == 3 || comp == 4;
+        if (!present) throw std::logic_error("source on a field component that does not exist in 2D TMz");
+        std::array<int, 6> bx;
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            int na = (a == 0 ? nx : (a == 1 ? ny : nz));
+            if (a == 2 && !is3D) {
+                bx[4] = 0;
+                bx[5] = 1;
+                continue;
+            }
+            int l;
+            if (s.loc[a] >= 0 && s.loc[a] < na) l = s.loc[a];
+            else if (s.loc[a] < 0 && s.loc[a] + s.sz[a] > 0) l = 0;
+            else { ok = false; break; }
+            int sz = (s.sz[a] + s.loc[a] > na) ? na - l : s.loc[a] + s.sz[a] - l;
+            bx[2 * a] = l;
+            bx[2 * a + 1] = l + sz;
+        }
+        if (!ok) continue;
+        srcComp.push_back(comp);
+        srcBox.insert(srcBox.end(), bx.begin(), bx.end());
+        activeSrc.push_back(&s);
+    }
+    // Per-step increments dt * Re(sum of pulses(tcur)), tcur accumulated as the
+    // reference does (tcur_ += dt, parallelFDTDField.hpp:1290).
+    {
+        srcAmp.assign(static_cast<size_t>(nsteps) * activeSrc.size(), 0.0);
+        double tcur = 0.0;
+        for (int st = 0; st < nsteps; ++st) {
+            for (size_t k = 0; k < activeSrc.size(); ++k) {
+                cplx pv(0.0, 0.0);
+                for (auto& pul : activeSrc[k]->pulses) pv += pulseValue(pul, tcur);
+                srcAmp[st * activeSrc.size() + k] = dt * std::real(pv);
+            }
+            tcur += dt;
+        }
+    }
+    P.nsrc = static_cast<int>(srcComp.size());
+    P.src_comp = srcComp.data();
+    P.src_box = srcBox.data();
+    P.src_amp = srcAmp.data();
+
+    // ---- point detectors -----------------------------------------------------
+    std::vector<int> dInt, dNp, dComp, dPow;
+    std::vector<long long> dCell;
+    std::vector<double> dConv;
+    auto cellIdx = [&](int x, int y, int z) -> long long {
+        x = std::max(0, std::min(x, nx - 1));
+        y = std::max(0, std::min(y, ny - 1));
+        z = std::max(0, std::min(z, nz - 1));
+        return x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
+    };
+    for (auto& D : IP.dtcs) {
+        int iv = static_cast<int>(std::floor(D.timeInterval / dt));
+        if (iv == 0) throw std::logic_error("The time step of a detector is less than the main grid or set to 0.");
+        double conv = 1.0, tconv = 1.0;
+        if (D.SI) {
+            tconv = IP.a / SPEED_OF_LIGHT;
+            conv = IP.I0 / IP.a;
+            if (D.type == DtcType::EX || D.type == DtcType::EY || D.type == DtcType::EZ) conv /= EPS0 * SPEED_OF_LIGHT;
+        }
+        std::vector<std::pair<int, std::array<int, 3>>> fl;
+        bool pw = (D.type == DtcType::EPOW || D.type == DtcType::HPOW);
+        if (pw) {
+            conv *= conv;
+            if (is3D && D.type == DtcType::EPOW) fl = {{0, {{-1, 0, 0}}}, {1, {{0, -1, 0}}}, {2, {{0, 0, -1}}}};
+            else if (is3D) fl = {{3, {{1, 0, 0}}}, {4, {{0, 1, 0}}}, {5, {{0, 0, 1}}}};
+            else if (D.type == DtcType::EPOW) fl = {{2, {{0, 0, 0}}}};
+            else fl = {{3, {{1, 0, 0}}}, {4, {{0, 1, 0}}}};
+            if (fl.size() != 1) throw std::logic_error("power detectors over several components are not supported");
+        } else {
+            int comp = static_cast<int>(D.type);
+            bool present = is3D || comp == 2 || comp == 3 || comp == 4;
+            if (!present) throw std::logic_error("detector on a field component that does not exist");
+            fl = {{comp, {{0, 0, 0}}}};
+        }
+        std::array<int, 3> sz = D.sz;
+        if (!is3D) sz[2] = 1;
+        int np = 0;
+        for (int jj = 0; jj < sz[1]; ++jj)
+            for (int kk = 0; kk < sz[2]; ++kk)
+                for (int ii = 0; ii < sz[0]; ++ii) {
+                    int x = D.loc[0] + ii, y = D.loc[1] + jj, z = is3D ? D.loc[2] + kk : 0;
+                    auto& f = fl[0];
+                    dComp.push_back(f.first);
+                    dComp.push_back(f.first);
+                    dCell.push_back(cellIdx(x, y, z));
+                    dCell.push_back(cellIdx(x + f.second[0], y + f.second[1], z + f.second[2]));
+                    ++np;
+                }
+        dInt.push_back(iv);
+        dNp.push_back(np);
+        dConv.push_back(conv);
+        dPow.push_back(pw ? 1 : 0);
+        detNpts.push_back(np);
+        detTConv.push_back(tconv);
+        detInterval.push_back(iv);
+    }
+    P.ndet = static_cast<int>(dInt.size());
+    P.det_interval = dInt.data();
+    P.det_npts = dNp.data();
+    P.det_comp = dComp.data();
+    P.det_cell = dCell.data();
+    P.det_conv = dConv.data();
+    P.det_pow = dPow.data();
+
+    // ---- flux boxes -> DFT cell set ------------------------------------------
+    std::unordered_map<long long, int> cellId;
+    std::vector<int> dftComp;
+    std::vector<long long> dftCell;
+    auto sampleId = [&](int comp, int x, int y, int z) {
+        long long cidx = x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
+        long long key = cidx * 8 + comp;
+        auto it = cellId.find(key);
+        if (it != cellId.end()) return it->second;
+        int id = static_cast<int>(dftComp.size());
+        cellId[key] = id;
+        dftComp.push_back(comp);
+        dftCell.push_back(cidx);
+        return id;
+    };
+    // A storage box of the reference (global loc may start at -1) clipped to
+    // the domain, with the ghost point at -1 excluded (it is always zero).
+    auto makeStore = [&](int comp, std::array<int, 3> loc, std::array<int, 3> sz) {
+        Store s;
+        s.comp = comp;
+        for (int a = 0; a < 3; ++a) {
+            if (a == 2 && !is3D) {
+                s.lo[2] = 0;
+                s.n[2] = 1;
+                continue;
+            }
+            int na = (a == 0 ? nx : (a == 1 ? ny : nz));
+            int ll;  // reference local index (ghost = 0)
+            if (loc[a] >= 0 && loc[a] < na) ll = loc[a] + 1;
+            else if (loc[a] == -1) ll = 0;
+            else if (loc[a] < 0 && loc[a] + sz[a] > 0) ll = 1;
+            else throw std::logic_error("flux storage box outside the cell");
+            int lsz = (sz[a] + loc[a] > na) ? (na + 2) - ll - 1 : loc[a] + sz[a] - (ll - 1);
+            s.lo[a] = ll - 1;  // may be -1 (ghost)
+            s.n[a] = lsz;
+        }
+        for (int z = 0; z < s.n[2]; ++z)
+            for (int y = 0; y < s.n[1]; ++y)
+                for (int x = 0; x < s.n[0]; ++x) {
+                    int px = s.lo[0] + x, py = s.lo[1] + y, pz = s.lo[2] + z;
+                    if (px < 0 || py < 0 || pz < 0) s.ids.push_back(-1);
+                    else s.ids.push_back(sampleId(comp, px, py, pz));
+                }
+        return s;
+    };
+    for (auto& F : IP.fluxes) {
+        FluxBox fb;
+        fb.freqs = F.freqs;
+        fb.timeInt = F.timeInt;
+        if (!fluxes.empty() && (fluxes[0].freqs != F.freqs || fluxes[0].timeInt != F.timeInt))
+            throw std::logic_error("flux boxes of one simulation must share their frequency list and time interval");
+        std::array<int, 3> fsz = F.sz;
+        if (!is3D) fsz[2] = 1;
+        auto makeFace = [&](int dir, bool pl) {
+            Face f;
+            int Ej, Ek, Hj, Hk, corI, corJ, corK, transCor2;
+            if (dir == 0) {
+                Ej = 1; Hj = 4; Ek = 2; Hk = 5;
+                if (!is3D) { f.cor = 1; f.transCor1 = 2; transCor2 = 0; }
+                else { f.cor = 2; f.transCor1 = 1; transCor2 = 0; }
+                corI = 0; corJ = 1; corK = 2;
+                f.sz = {{1, fsz[1], fsz[2]}};
+            } else if (dir == 1) {
+                Ej = 2; Hj = 5; Ek = 0; Hk = 3;
+                f.cor = 0; f.transCor1 = 2; transCor2 = 1;
+                corI = 1; corJ = 2; corK = 0;
+                f.sz = {{fsz[0], 1, fsz[2]}};
+            } else {
+                Ej = 0; Hj = 3; Ek = 1; Hk = 4;
+                f.cor = 0; f.transCor1 = 1; transCor2 = 2;
+                corI = 2; corJ = 0; corK = 1;
+                f.sz = {{fsz[0], fsz[1], 1}};
+            }
+            auto present = [&](int comp) { return is3D || comp == 2 || comp == 3 || comp == 4; };
+            std::array<int, 3> loc = F.loc;
+            if (!is3D) loc[2] = 0;
+            if (pl) loc[transCor2] = F.loc[transCor2] + fsz[transCor2] - 1;
+            std::array<int, 3> lo = loc;
+            if (is3D) {
+                std::array<int, 3> so = f.sz;
+                lo[corK] -= 1;
+                so[corK] += 1;
+                f.Ek.push_back(makeStore(Ek, lo, so));
+                f.Hj.push_back(makeStore(Hj, lo, so));
+                lo[corI] -= 1;
+                f.Hj.push_back(makeStore(Hj, lo, so));
+                lo[corK] += 1;
+                so[corK] -= 1;
+                lo[corJ] -= 1;
+                so[corJ] += 1;
+                f.Hk.push_back(makeStore(Hk, lo, so));
+                lo[corI] += 1;
+                f.Hk.push_back(makeStore(Hk, lo, so));
+                f.Ej.push_back(makeStore(Ej, lo, so));
+                f.padEk = corK; f.padHj = corK; f.padHk = corJ; f.padEj = corJ;
+            } else if (present(Ek)) {
+                lo[corI] += -1;
+                f.Ek.push_back(makeStore(Ek, loc, f.sz));
+                f.Hj.push_back(makeStore(Hj, loc, f.sz));
+                f.Hj.push_back(makeStore(Hj, lo, f.sz));
+            } else {
+                lo[corI] += -1;
+                f.Ej.push_back(makeStore(Ej, loc, f.sz));
+                f.Hk.push_back(makeStore(Hk, loc, f.sz));
+                f.Hk.push_back(makeStore(Hk, lo, f.sz));
+            }
+            f.weight = pl ? 1.0 : -1.0;
+            fb.faces.push_back(std::move(f));
+        };
+        if (!is3D) {
+            if (fsz[0] > 1 && fsz[1] > 1) {
+                makeFace(1, false); makeFace(1, true); makeFace(0, false); makeFace(0, true);
+            } else if (fsz[0] == 1) makeFace(0, true);
+            else makeFace(1, true);
+        } else {
+            if (fsz[0] > 1 && fsz[1] > 1 && fsz[2] > 1) {
+                makeFace(0, true); makeFace(0, false); makeFace(1, true); makeFace(1, false); makeFace(2, true); makeFace(2, false);
+            } else if (fsz[0] == 1) makeFace(0, true);
+            else if (fsz[1] == 1) makeFace(1, true);
+            else if (fsz[2] == 1) makeFace(2, true);
+        }
+        fluxes.push_back(std::move(fb));
+    }
+    std::vector<double> phasor;
+    if (!fluxes.empty()) {
+        nfreq = static_cast<int>(fluxes[0].freqs.size());
+        dftInterval = fluxes[0].timeInt;
+        int nsamp = nsteps / dftInterval + 1;
+        phasor.assign(2 * static_cast<size_t>(nsamp) * nfreq, 0.0);
+        // exp(-i w tcur) at each sampling instant (parallelFlux.hpp:298), with
+        // tcur accumulated step by step as in the reference.
+        double tcur = 0.0;
+        for (int st = 0; st <= nsteps; ++st) {
+            if (st % dftInterval == 0) {
+                int k = st / dftInterval;
+                for (int f = 0; f < nfreq; ++f) {
+                    cplx w = std::exp(cplx(0.0, -1.0 * tcur * fluxes[0].freqs[f]));
+                    phasor[2 * (static_cast<size_t>(k) * nfreq + f)] = w.real();
+                    phasor[2 * (static_cast<size_t>(k) * nfreq + f) + 1] = w.imag();
+                }
+            }
+            tcur += dt;
+        }
+    }
+    ndft = static_cast<int>(dftComp.size());
+    P.ndft = ndft;
+    P.dft_comp = dftComp.data();
+    P.dft_cell = dftCell.data();
+    P.nfreq = nfreq;
+    P.dft_interval = dftInterval;
+    P.dft_phasor = phasor.data();
+
+    // ---- emitters --------------------------------------------------------------
+    std::vector<chiml_qe_t> qdesc;
+    std::vector<std::vector<int>> qxyz, qrow, qcol, qpop;
+    std::vector<std::vector<double>> qh0, qmu, qw, qval;
+    std::vector<double> epsNode;
+    if (!IP.qes.empty()) {
+        epsNode.assign(N, 1.0);
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    std::array<double, 3> pt = {{coord(0, x, 0.0), coord(1, y, 0.0), is3D ? coord(2, z, 0.0) : 0.0}};
+                    if (!is3D) pt[2] = (-1 - (nAll[2] - nAll[2] % 2) / 2.0) * IP.d[2];  // kk - 1 with kk = 0
+                    epsNode[x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)] = IP.objs[objAt(pt, false)]->eps[0];
+                }
+    }
+    qdesc.resize(IP.qes.size());
+    qxyz.resize(IP.qes.size());
+    qrow.resize(IP.qes.size());
+    qcol.resize(IP.qes.size());
+    qpop.resize(IP.qes.size());
+    qh0.resize(IP.qes.size());
+    qmu.resize(IP.qes.size());
+    qw.resize(IP.qes.size());
+    qval.resize(IP.qes.size());
+    for (size_t g = 0; g < IP.qes.size(); ++g) {
+        const QESpec& qs = IP.qes[g];
+        const int L = qs.nlevel;
+        if (qs.locs.empty()) throw std::logic_error("quantum emitter object contains no grid points");
+        chiml_qe_t& q = qdesc[g];
+        std::memset(&q, 0, sizeof(q));
+        q.nlevel = L;
+        q.nmembers = static_cast<int>(qs.members.size());
+        q.nemit = static_cast<int>(qs.locs.size());
+        std::array<int, 3> mn = {{1 << 30, 1 << 30, 1 << 30}}, mx = {{0, 0, 0}};
+        for (auto& l : qs.locs) {
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = std::min(mn[a], l[a]);
+                mx[a] = std::max(mx[a], l[a]);
+            }
+            qxyz[g].push_back(l[0]);
+            qxyz[g].push_back(l[1]);
+            qxyz[g].push_back(is3D ? l[2] : 0);
+        }
+        for (int a = 0; a < 3; ++a) {
+            q.box_min[a] = mn[a];
+            q.box_n[a] = mx[a] - mn[a] + 1;
+        }
+        if (!is3D) {
+            q.box_min[2] = 0;
+            q.box_n[2] = 1;
+        }
+        q.emit_xyz = qxyz[g].data();
+        // h0 per member (diagonal energies, Hamiltonian.cpp:11-29, or the
+        // explicit matrix extension), dipoles = basis matrices x couplings.
+        for (auto& m : qs.members) {
+            std::vector<cplx> h(L * L, 0.0);
+            if (!qs.h0Full.empty()) h = qs.h0Full;
+            else
+                for (int k = 0; k < L; ++k) h[k * L + k] = m.first[k];
+            for (auto& v : h) {
+                qh0[g].push_back(v.real());
+                qh0[g].push_back(v.imag());
+            }
+            qw[g].push_back(m.second);
+        }
+        if (static_cast<int>(qs.couplings.size()) > L * L) throw std::logic_error("more couplings than nlevel^2");
+        for (int c = 0; c < 3; ++c) {
+            std::vector<cplx> mu;
+            if (!qs.muFull[c].empty()) mu = qs.muFull[c];
+            else {
+                mu = basisDipole(qs.basis, c);
+                for (size_t k = 0; k < qs.couplings.size(); ++k) mu[k] *= qs.couplings[k];
+            }
+            bool present = is3D || c == 2;
+            for (auto& v : mu) {
+                qmu[g].push_back(present ? v.real() : 0.0);
+                qmu[g].push_back(present ? v.imag() : 0.0);
+            }
+        }
+        // Lindblad superoperator Gamma[i][j] (parallelQE.hpp:285-329) or the
+        // explicit gam matrix; entries are applied as d rho_i += Gamma[i][j] rho_j.
+        std::vector<std::vector<double>> G;
+        if (!qs.gam.empty()) G = qs.gam;
+        else {
+            G.assign(L * L, std::vector<double>(L * L, 0.0));
+            for (auto& r : qs.relax) {
+                std::vector<double> A(L * L, 0.0);
+                A[r.n0 * L + r.nf] = 1.0;
+                for (int ii = 0; ii < L * L; ++ii) {
+                    auto col = lindbladColumn(A, ii, L);
+                    for (int j = 0; j < L * L; ++j) G[ii][j] += r.rate * col[j];
+                }
+                if (r.dephasing > 0.0) {
+                    std::vector<double> B(L * L, 0.0);
+                    B[r.n0 * L + r.n0] = std::sqrt(2.0);
+                    for (int ii = 0; ii < L * L; ++ii) {
+                        auto col = lindbladColumn(B, ii, L);
+                        for (int j = 0; j < L * L; ++j) G[ii][j] += r.dephasing * col[j];
+                    }
+                }
+            }
+        }
+        for (size_t i = 0; i < G.size(); ++i)
+            for (size_t j = 0; j < G[i].size(); ++j)
+                if (G[i][j] != 0.0) {
+                    qrow[g].push_back(static_cast<int>(i));
+                    qcol[g].push_back(static_cast<int>(j));
+                    qval[g].push_back(G[i][j]);
+                }
+        q.h0 = qh0[g].data();
+        q.mu = qmu[g].data();
+        q.weight = qw[g].data();
+        q.ngam = static_cast<int>(qval[g].size());
+        q.gam_row = qrow[g].data();
+        q.gam_col = qcol[g].data();
+        q.gam_val = qval[g].data();
+        q.na = qs.na;
+        q.hbar = HBAR * EPS0 * std::pow(SPEED_OF_LIGHT, 3) / std::pow(IP.a * IP.I0, 2.0);
+        qpop[g] = qs.dtcLevs;
+        q.npop = static_cast<int>(qpop[g].size());
+        q.pop_flat = qpop[g].data();
+        q.pop_interval = qs.popTimeInt;
+        qeN.push_back(L);
+        bytesML += static_cast<double>(q.nemit) * q.nmembers * (7.0 * 16.0 * L * L);
+    }
+    P.nqe = static_cast<int>(qdesc.size());
+    P.qe = qdesc.data();
+    P.eps_node = epsNode.empty() ? nullptr : epsNode.data();
+
+    // ---- algorithmic bytes per step ------------------------------------------
+    const double s = IP.fp32 ? 4.0 : 8.0;
+    const int F = is3D ? 3 : 2;  // H components updated
+    cells = static_cast<double>(N);
+    bytesH = cells * (2.0 * F * s + (is3D ? 3.0 : 1.0) * s);  // read+write H, read E
+    bytesE = cells * (2.0 * (is3D ? 3.0 : 1.0) * s + F * s + (is3D ? 3.0 : 1.0));  // read+write E, read H, 1 B id per E component
+    double psiWords = 0.0;
+    for (int comp = 0; comp < 6; ++comp)
+        for (int a = 0; a < 3; ++a)
+            if (P.pml_b[comp][a]) {
+                double S = P.psi_lo[comp][a] + (P.psi_hi1[comp][a] - P.psi_hi0[comp][a]);
+                double other = 1.0;
+                for (int o = 0; o < 3; ++o)
+                    if (o != a) other *= (o == 0 ? nx : (o == 1 ? ny : nz));
+                double w = S * other;
+                if (comp < 3) bytesE += 2.0 * s * w;
+                else bytesH += 2.0 * s * w;
+                psiWords += w;
+            }
+    for (int c : ecomps) {
+        const int* b = dispBox[c];
+        double cnt = 0;
+        for (int z = b[4]; z < b[5]; ++z)
+            for (int y = b[2]; y < b[3]; ++y)
+                for (int x = b[0]; x < b[1]; ++x)
+                    if (psT[c][matE[c][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)]] >= 0)
+                        cnt += pcount[psT[c][matE[c][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)]]];
+        bytesE += cnt * 4.0 * s;  // P and J read + write per pole
+    }
+    estimate = static_cast<size_t>(cells * (6.0 * s + 3.0) + psiWords * s + bytesML / 7.0 * 5.0 +
+                                   static_cast<double>(ndft) * (nfreq * 16.0 + 32 * s));
+
+    int rc = chiml_dev_create(&P, &dev_);
+    if (rc != CHIML_OK) throw std::runtime_error(std::string("chiml_dev_create: ") + chiml_dev_global_error());
+}
+
+// Total flux per frequency: face-centre averages of the DFT fields, the
+// Ej Hk* - Ek Hj* integrand over nt^2, Simpson's rule over the face and the
+// +/-1 face weights (parallelFlux.hpp:325-534, combineField + getFlux).
+static cplx simpson(const cplx* f, int n, double d) {
+    if (n == 1) return f[0] * d;
+    cplx r(0.0, 0.0);
+    if (n % 2 == 0) {
+        for (int i = 0; i < (n - 2) / 2; i++) r += d / 6.0 * (f[i * 2] + 4.0 * f[i * 2 + 1] + f[(i + 1) * 2]);
+        for (int i = 1; i < n / 2; i++) r += d / 6.0 * (f[i * 2 - 1] + 4.0 * f[i * 2] + f[i * 2 + 1]);
+        r += d / 4.0 * (f[0] + f[1] + f[n - 1] + f[n - 2]);
+    } else {
+        for (int i = 0; i < (n - 1) / 2; i++) r += d / 3.0 * (f[i * 2] + 4.0 * f[i * 2 + 1] + f[(i + 1) * 2]);
+    }
+    return r;
+}
+
+std::vector<cplx> FDTDField::flux(int fi) {
+    FluxBox& fb = fluxes.at(fi);
+    std::vector<double> F(2 * static_cast<size_t>(ndft) * nfreq);
+    if (ndft > 0 && chiml_dev_read_dft(dev_, F.data(), F.size()) != CHIML_OK)
+        throw std::runtime_error(chiml_dev_last_error(dev_));
+    int steps = 0;
+    chiml_dev_steps_done(dev_, &steps);
+    const double nt = steps / fb.timeInt + 1;
+    std::vector<cplx> total(nfreq, 0.0);
+    for (auto& f : fb.faces) {
+        const int n1 = f.sz[f.transCor1], n2 = f.sz[f.cor];
+        auto combine = [&](std::vector<Store>& st, int pad) {
+            std::vector<cplx> out;
+            if (st.empty()) return out;
+            out.assign(static_cast<size_t>(nfreq) * n1 * n2, 0.0);
+            const int nOff = pad >= 0 ? 2 : 1;
+            const double w = 1.0 / static_cast<double>(st.size() * nOff);
+            for (auto& s : st)
+                for (int o = 0; o < nOff; ++o)
+                    for (int i1 = 0; i1 < n1; ++i1)
+                        for (int i2 = 0; i2 < n2; ++i2) {
+                            std::array<int, 3> q = {{0, 0, 0}};
+                            q[f.transCor1] = i1;
+                            q[f.cor] = i2;
+                            if (o == 1) q[pad] += 1;
+                            if (q[0] >= s.n[0] || q[1] >= s.n[1] || q[2] >= s.n[2]) continue;
+                            int id = s.ids[q[0] + static_cast<size_t>(s.n[0]) * (q[1] + static_cast<size_t>(s.n[1]) * q[2])];
+                            if (id < 0) continue;
+                            for (int k = 0; k < nfreq; ++k)
+                                out[(static_cast<size_t>(k) * n1 + i1) * n2 + i2] +=
+                                    w * cplx(F[2 * (static_cast<size_t>(id) * nfreq + k)], F[2 * (static_cast<size_t>(id) * nfreq + k) + 1]);
+                        }
+            return out;
+        };
+        auto Ej = combine(f.Ej, f.padEj), Ek = combine(f.Ek, f.padEk), Hj = combine(f.Hj, f.padHj), Hk = combine(f.Hk, f.padHk);
+        std::vector<cplx> g(static_cast<size_t>(n1) * n2);
+        for (int k = 0; k < nfreq; ++k) {
+            for (int i1 = 0; i1 < n1; ++i1)
+                for (int i2 = 0; i2 < n2; ++i2) {
+                    size_t idx = (static_cast<size_t>(k) * n1 + i1) * n2 + i2;
+                    cplx v(0.0, 0.0);
+                    if (!Ej.empty()) v = Ej[idx] * std::conj(Hk[idx]) / (nt * nt);
+                    if (!Ek.empty()) v += -1.0 * (Ek[idx] * std::conj(Hj[idx]) / (nt * nt));
+                    g[i2 + static_cast<size_t>(n2) * i1] = v;
+                }
+            cplx sres;
+            const double d1 = IP.d[f.transCor1], d2 = IP.d[f.cor];
+            if (n1 > 1 && n2 > 1) {
+                std::vector<cplx> r(n1);
+                for (int j = 0; j < n1; ++j) r[j] = simpson(&g[static_cast<size_t>(j) * n2], n2, d2);
+                sres = simpson(r.data(), n1, d1);
+            } else if (n2 > 1) {
+                sres = simpson(g.data(), n2, d2);
+            } else {
+                sres = simpson(g.data(), n1, d1);
+            }
+            total[k] += f.weight * sres;
+        }
+    }
+    return total;
+}
+
+}  // namespace chiml
+
+// ------------------------------------------------------------------ C ABI ---
+struct chiml_sim {
+    std::unique_ptr<chiml::FDTDField> f;
+};
+
+extern "C" {
+
+static int sim_fail(const std::string& m, int code = CHIML_E_INVALID) {
+    chiml::g_sim_err = m;
+    return code;
+}
+
+int chiml_sim_create(const char* deck, chiml_sim_t** out) {
+    if (!deck || !out) return sim_fail("null argument");
+    *out = nullptr;
+    try {
+        auto s = new chiml_sim;
+        try {
+            s->f = std::make_unique<chiml::FDTDField>(deck);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+        return CHIML_OK;
+    } catch (std::exception& e) {
+        return sim_fail(e.what());
+    }
+}
+
+int chiml_sim_step(chiml_sim_t* s, int n) {
+    if (!s) return sim_fail("null simulation");
+    int rc = chiml_dev_step(s->f->dev_, n);
+    if (rc) chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
+    return rc;
+}
+
+int chiml_sim_sync(chiml_sim_t* s) {
+    if (!s) return sim_fail("null simulation");
+    int rc = chiml_dev_sync(s->f->dev_);
+    if (rc) chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
+    return rc;
+}
+
+int chiml_sim_info(chiml_sim_t* s, int* n3, double* dt, int* nsteps, int* prec, int* ndet, int* nflux, int* nqe) {
+    if (!s) return sim_fail("null simulation");
+    auto& F = *s->f;
+    if (n3) {
+        n3[0] = F.n[0];
+        n3[1] = F.n[1];
+        n3[2] = F.n[2];
+    }
+    if (dt) *dt = F.IP.dt;
+    if (nsteps) *nsteps = F.IP.nSteps();
+    if (prec) *prec = F.IP.fp32 ? 32 : 64;
+    if (ndet) *ndet = static_cast<int>(F.detNpts.size());
+    if (nflux) *nflux = static_cast<int>(F.fluxes.size());
+    if (nqe) *nqe = static_cast<int>(F.qeN.size());
+    return CHIML_OK;
+}
+
+int chiml_sim_field(chiml_sim_t* s, int comp, double* out) {
+    if (!s) return sim_fail("null simulation");
+    int rc = chiml_dev_read_field(s->f->dev_, comp, out);
+    if (rc) chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
+    return rc;
+}
+
+int chiml_sim_detector(chiml_sim_t* s, int d, double* times, double* values, size_t cap, size_t* ns, int* npts) {
+    if (!s || !ns || !npts) return sim_fail("null argument");
+    auto& F = *s->f;
+    if (d < 0 || d >= static_cast<int>(F.detNpts.size())) return sim_fail("detector index out of range");
+    *npts = F.detNpts[d];
+    int rc = chiml_dev_read_points(F.dev_, d, nullptr, 0, ns);
+    if (rc) return sim_fail(chiml_dev_last_error(F.dev_), rc);
+    if (!values || cap < *ns) return CHIML_OK;
+    rc = chiml_dev_read_points(F.dev_, d, values, cap * F.detNpts[d], ns);
+    if (rc) return sim_fail(chiml_dev_last_error(F.dev_), rc);
+    if (times) {
+        // output time of sample k: tcur after k*interval steps, accumulated
+        double t = 0.0;
+        size_t k = 0;
+        for (int st = 0; k < *ns; ++st) {
+            if (st % F.detInterval[d] == 0) times[k++] = t * F.detTConv[d];
+            t += F.IP.dt;
+        }
+    }
+    return CHIML_OK;
+}
+
+int chiml_sim_flux(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq) {
+    if (!s || !nfreq) return sim_fail("null argument");
+    try {
+        auto v = s->f->flux(f);
+        *nfreq = static_cast<int>(v.size());
+        if (reim && cap >= 2 * v.size())
+            for (size_t k = 0; k < v.size(); ++k) {
+                reim[2 * k] = v[k].real();
+                reim[2 * k + 1] = v[k].imag();
+            }
+        return CHIML_OK;
+    } catch (std::exception& e) {
+        return sim_fail(e.what());
+    }
+}
+
+int chiml_sim_pops(chiml_sim_t* s, int q, int j, double* reim, size_t cap, size_t* ns) {
+    if (!s) return sim_fail("null simulation");
+    int rc = chiml_dev_read_pops(s->f->dev_, q, j, reim, cap, ns);
+    if (rc) chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
+    return rc;
+}
+
+int chiml_sim_density(chiml_sim_t* s, int q, int m, int e, double* reim, int* nlevel) {
+    if (!s) return sim_fail("null simulation");
+    int rc = chiml_dev_read_density(s->f->dev_, q, m, e, reim);
+    if (rc < 0) {
+        chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
+        return rc;
+    }
+    if (nlevel) *nlevel = rc;
+    return CHIML_OK;
+}
+
+int chiml_sim_cost(chiml_sim_t* s, double* cells, double* bh, double* be, double* bml) {
+    if (!s) return sim_fail("null simulation");
+    if (cells) *cells = s->f->cells;
+    if (bh) *bh = s->f->bytesH;
+    if (be) *be = s->f->bytesE;
+    if (bml) *bml = s->f->bytesML;
+    return CHIML_OK;
+}
+
+int chiml_sim_profile(chiml_sim_t* s, int enable) {
+    if (!s) return sim_fail("null simulation");
+    return chiml_dev_profile(s->f->dev_, enable);
+}
+
+int chiml_sim_profile_read(chiml_sim_t* s, const char* kernel, double* total_ms, int* launches) {
+    if (!s) return sim_fail("null simulation");
+    return chiml_dev_profile_read(s->f->dev_, kernel, total_ms, launches);
+}
+
+int chiml_sim_footprint(chiml_sim_t* s, size_t* bytes) {
+    if (!s) return sim_fail("null simulation");
+    return chiml_dev_footprint(s->f->dev_, bytes);
+}
+
+int chiml_sim_estimate_bytes(const char* deck, size_t* bytes) {
+    if (!deck || !bytes) return sim_fail("null argument");
+    try {
+        auto IP = chiml::parseInputs(deck);
+        auto n = IP.nVec();
+        if (!IP.is3D()) n[2] = 1;
+        double cells = static_cast<double>(n[0]) * n[1] * n[2];
+        double s = IP.fp32 ? 4.0 : 8.0;
+        double psi = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            if (!IP.is3D() && a == 2) continue;
+            psi += 4.0 * 2.0 * IP.pmlThickness[a] * cells / n[a];
+        }
+        double ml = 0.0;
+        for (auto& q : IP.qes) ml += static_cast<double>(q.locs.size()) * q.members.size() * 5.0 * 16.0 * q.nlevel * q.nlevel;
+        *bytes = static_cast<size_t>(cells * (6.0 * s + 3.0) + psi * s + ml + 64.0 * 1024 * 1024);
+        return CHIML_OK;
+    } catch (std::exception& e) {
+        return sim_fail(e.what());
+    }
+}
+
+const char* chiml_sim_last_error(void) { return chiml::g_sim_err.c_str(); }
+
+void chiml_sim_destroy(chiml_sim_t* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,618 @@
+// Input deck interpretation and object geometry; see setup.hpp.
+#include "setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <numeric>
+#include <stdexcept>
+
+namespace chiml {
+
+using json::Value;
+
+// String -> enum conversions (src/INPUTS/parallelInputs.cpp:843-862, 952-1041).
+Pol string2pol(const std::string& p) {
+    if (p == "Ex") return Pol::EX;
+    if (p == "Ey") return Pol::EY;
+    if (p == "Ez") return Pol::EZ;
+    if (p == "Hx") return Pol::HX;
+    if (p == "Hy") return Pol::HY;
+    if (p == "Hz") return Pol::HZ;
+    throw std::logic_error("POLARIZATION undefined or unsupported: " + p);
+}
+PulseShape string2prof(const std::string& p) {
+    if (p == "gaussian") return PulseShape::GAUSSIAN;
+    if (p == "BH") return PulseShape::BH;
+    if (p == "rectangle") return PulseShape::RECT;
+    if (p == "continuous") return PulseShape::CONTINUOUS;
+    if (p == "ricker") return PulseShape::RICKER;
+    if (p == "ramped_cont") return PulseShape::RAMP_CONT;
+    throw std::logic_error("Pulse profile undefined: " + p);
+}
+DtcType string2out(const std::string& t) {
+    if (t == "Ex") return DtcType::EX;
+    if (t == "Ey") return DtcType::EY;
+    if (t == "Ez") return DtcType::EZ;
+    if (t == "Hx") return DtcType::HX;
+    if (t == "Hy") return DtcType::HY;
+    if (t == "Hz") return DtcType::HZ;
+    if (t == "Epow") return DtcType::EPOW;
+    if (t == "Hpow") return DtcType::HPOW;
+    throw std::logic_error("Detector type undefined or unsupported: " + t);
+}
+DtcClass string2dtcclass(const std::string& c) {
+    if (c == "bin") return DtcClass::BIN;
+    if (c == "txt") return DtcClass::TXT;
+    if (c == "cout") return DtcClass::COUT;
+    if (c == "freq") return DtcClass::FREQ;
+    throw std::logic_error("Detector class undefined or unsupported: " + c);
+}
+
+double ev2FDTD(double eV, double a) { return eV / 4.135666e-15 * a / SPEED_OF_LIGHT; }
+
+int ProgramInputs::nSteps() const { return static_cast<int>(std::ceil(tMax / dt)); }
+
+std::array<int, 3> ProgramInputs::nVec() const {
+    std::array<int, 3> n;
+    for (int i = 0; i < 3; ++i) n[i] = static_cast<int>(std::floor(size[i] / d[i] + 0.5)) + 1;
+    return n;
+}
+
+// ---------------------------------------------------------------- pulses ----
+// Pulse shapes of src/UTIL/PulseFxn.hpp:19-89 evaluated at time t; the Emax
+// factor is the last parameter as appended in
+// src/FDTD_MANAGER/parallelFDTDField.cpp:474.
+cplx pulseValue(const PulseSpec& p, double tt) {
+    const std::vector<cplx>& q = p.param;
+    switch (p.shape) {
+        case PulseShape::GAUSSIAN:  // {-2 pi i f, 1/fwidth, t_off, t0, E0}
+            return (tt <= std::real(q[2])) ? q[4] * std::exp(q[0] * tt - std::pow((tt - q[3]) / q[1], 2.0) / 2.0) : cplx(0.0);
+        case PulseShape::CONTINUOUS:  // {-2 pi i f, E0}
+            return q[1] * std::exp(q[0] * tt);
+        case PulseShape::RAMP_CONT:  // {-2 pi i f, ramp, E0}
+            return (std::abs(q[2] * q[1] * tt) <= std::abs(q[2])) ? std::abs(q[1] * tt) * q[2] * std::exp(q[0] * tt)
+                                                                   : q[2] * std::exp(q[0] * tt);
+        case PulseShape::BH: {  // {-2 pi i f, tau, t0, BH1..BH4, E0}
+            if (std::real(q[2] - q[1] / 2.0) <= tt && tt <= std::real(q[2] + q[1] / 2.0))
+                return q[7] * std::exp(q[0] * (tt - q[2])) *
+                       (q[3] + q[4] * std::cos(2.0 * kPi * (tt - q[2]) / q[1]) + q[5] * std::cos(4.0 * kPi * (tt - q[2]) / q[1]) +
+                        q[6] * std::cos(6.0 * kPi * (tt - q[2]) / q[1]));
+            return 0.0;
+        }
+        case PulseShape::RECT:  // {tau, t0, n, -2 pi i f, E0}
+            if (tt < std::real(q[1] + q[0]) && tt > std::real(q[1] - q[0]))
+                return q[4] / (std::pow(2.0 * (tt - q[1]) / q[0], q[2]) + 1.0) * std::exp(q[3] * (tt - q[1]));
+            return 0.0;
+        case PulseShape::RICKER: {  // {-2 pi i f, fwidth, cutoff, E0}
+            // The turn-off test real(cutoff*fwidth/(-2 pi i f)) is 0, so the
+            // pulse is zero for every t >= 0 (rickerPulse, PulseFxn.hpp:86-89).
+            if (tt < std::real(q[2] * q[1] / q[0])) {
+                cplx arg = kPi * (tt * q[0] - q[1]);
+                return q[3] * (1.0 - 2.0 * std::pow(arg, 2.0)) * std::exp(-std::pow(arg, 2.0));
+            }
+            return 0.0;
+        }
+    }
+    return 0.0;
+}
+
+// --------------------------------------------------------------- objects ----
+void Object::setUpConsts(double dt) {
+    // Lorentz ADE coefficients, src/OBJECTS/Obj.cpp:311-324.
+    alpha.clear();
+    xi.clear();
+    gamma.clear();
+    for (const Pole& p : pols) {
+        if (std::abs(p.sigP) == 0.0) continue;
+        double od = p.omg * dt;
+        alpha.push_back((2.0 - std::pow(od, 2.0)) / (1.0 + p.gam * dt));
+        xi.push_back((p.gam * dt - 1.0) / (1.0 + p.gam * dt));
+        gamma.push_back((p.sigP * std::pow(od, 2.0)) / (1.0 + p.gam * dt));
+    }
+}
+
+bool Object::dielectric() const { return anyEpsNotOne() || !pols.empty() || ML; }
+
+static std::array<double, 3> toObjectSpace(const Object& o, const std::array<double, 3>& v) {
+    // RealSpace2ObjectSpace, src/OBJECTS/Obj.cpp:13-21: v_trans[r] = T_r . (v - loc)
+    std::array<double, 3> c{{v[0] - o.loc[0], v[1] - o.loc[1], v[2] - o.loc[2]}};
+    std::array<double, 3> r;
+    for (int i = 0; i < 3; ++i) r[i] = o.T[3 * i] * c[0] + o.T[3 * i + 1] * c[1] + o.T[3 * i + 2] * c[2];
+    return r;
+}
+
+bool Object::isObj(const std::array<double, 3>& v, double dx, bool useML) const {
+    const std::vector<double>& g = useML ? geoML : geo;
+    switch (shape) {
+        case Shape::SPHERE: {  // src/OBJECTS/Obj.cpp:373-380
+            double d2 = 0.0;
+            for (int i = 0; i < 3; ++i) d2 += (v[i] - loc[i]) * (v[i] - loc[i]);
+            return !(std::sqrt(d2) > g[0] + dx / 1.0e6);
+        }
+        case Shape::BLOCK: {  // src/OBJECTS/Obj.cpp:400-410
+            auto t = toObjectSpace(*this, v);
+            for (int i = 0; i < 3; ++i)
+                if ((t[i] > g[i] / 2.0 + dx / 1.0e6) || (t[i] < -1.0 * g[i] / 2.0 - dx / 1.0e6)) return false;
+            return true;
+        }
+        case Shape::CYLINDER: {  // src/OBJECTS/Obj.cpp:523-533; axis = object y
+            auto t = toObjectSpace(*this, v);
+            if ((t[1] < -1.0 * g[1] / 2.0 - dx * 1e-6) || (t[1] > g[1] / 2.0 + dx * 1e-6)) return false;
+            if (std::sqrt(t[0] * t[0] + t[2] * t[2]) > g[0] + 1.0e-6 * dx) return false;
+            return true;
+        }
+        case Shape::ELLIPSOID: {  // src/OBJECTS/Obj.cpp:437-457 with no cuts
+            auto t = toObjectSpace(*this, v);
+            double s = 0.0;
+            int nax = (g[2] != 0.0) ? 3 : 2;
+            for (int i = 0; i < nax; ++i) s += std::pow((t[i] - dx / 1.0e6) / (g[i] / 2.0), 2.0);
+            if (1.0 < s) return false;
+            for (int i = 0; i < 3; ++i) {
+                if (g[i] == 0.0) continue;
+                if ((2.0 * t[i] / g[i] < -1.0) || (2.0 * t[i] / g[i] > 1.0)) return false;
+                if ((t[i] + g[i] / 2.0) / g[i] >= 1.0) return false;
+            }
+            return true;
+        }
+    }
+    return false;
+}
+
+// Named dispersive materials (getMater, src/INPUTS/parallelInputs.cpp:1066-1180).
+// Lorentz-Drude parameters are the published Rakic et al., Appl. Opt. 37, 5271
+// (1998) table: {wp, then (f, Gamma, Omega) per oscillator} in eV.
+static const std::vector<double> kRakicAu = {9.03, 0.760, 0.053, 0.0, 0.024, 0.241, 0.415, 0.010, 0.345, 0.830,
+                                             0.071, 0.870, 2.969, 0.601, 2.494, 4.304, 4.384, 2.214, 13.32};
+static const std::vector<double> kRakicAg = {9.01, 0.845, 0.048, 0.0, 0.065, 3.886, 0.816, 0.124, 0.452, 4.481,
+                                             0.011, 0.065, 8.185, 0.840, 0.916, 9.083, 5.646, 2.419, 20.29};
+
+static std::vector<Pole> metalPoles(const std::vector<double>& params, double a) {
+    // getMetal, src/INPUTS/parallelInputs.cpp:1187-1221.
+    std::vector<Pole> out;
+    double wp = ev2FDTD(params[0], a);
+    for (size_t ii = 0; ii < (params.size() - 1) / 3; ++ii) {
+        double f = params[3 * ii + 1];
+        double GAM = ev2FDTD(params[3 * ii + 2], a);
+        double OMG = ev2FDTD(params[3 * ii + 3], a);
+        if (OMG == 0.0) OMG = ev2FDTD(1.0e-20, a);
+        Pole p;
+        p.sigP = f * std::pow(wp / OMG, 2);
+        p.gam = GAM * kPi;
+        p.omg = OMG * 2.0 * kPi;
+        out.push_back(p);
+    }
+    return out;
+}
+
+static void setUnitVectors(Object& o, const Value& item, bool is2D) {
+    // unit_vectors / orTheta / orPhi, src/INPUTS/parallelInputs.cpp:1438-1466;
+    // coordTransform_ rows normalised, src/OBJECTS/Obj.cpp:76-83.  A zero unit
+    // vector (the 2D default third axis) maps to a zero object coordinate
+    // here; in the reference it divides 0 by 0.
+    std::array<std::array<double, 3>, 3> u{};
+    const Value& uv = json::get_child_or_empty(item, "unit_vectors");
+    if (uv.size() > 0) {
+        for (size_t c = 0; c < uv.size() && c < 3; ++c) {
+            double tmp[3];
+            json::get_arr3(uv.child(c), "uvec", tmp);
+            u[c] = {{tmp[0], tmp[1], tmp[2]}};
+        }
+    } else {
+        double th = kPi / 2.0 - json::get_num(item, "orTheta", 90.0) * kPi / 180.0;
+        double ph = -1.0 * json::get_num(item, "orPhi", 0.0) * kPi / 180.0;
+        if (is2D) {
+            u[0] = {{std::cos(ph), -1.0 * std::sin(ph), 0}};
+            u[1] = {{std::sin(ph), std::cos(ph), 0}};
+            u[2] = {{0, 0, 0}};
+        } else {
+            u[0] = {{std::cos(th) * std::cos(ph), -1.0 * std::cos(th) * std::sin(ph), std::sin(th)}};
+            u[1] = {{std::sin(ph), std::cos(ph), 0}};
+            u[2] = {{-1.0 * std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)}};
+        }
+    }
+    for (int r = 0; r < 3; ++r) {
+        double n = std::sqrt(u[r][0] * u[r][0] + u[r][1] * u[r][1] + u[r][2] * u[r][2]);
+        for (int c = 0; c < 3; ++c) o.T[3 * r + c] = (n > 0.0) ? u[r][c] / n : 0.0;
+    }
+}
+
+static std::shared_ptr<Object> ptreeToObject(const Value& item, const ProgramInputs& IP) {
+    // src/INPUTS/parallelInputs.cpp:1263-1515 (custom and named materials;
+    // block, sphere, cylinder and ellipsoid shapes).
+    auto o = std::make_shared<Object>();
+    double loc[3];
+    json::get_arr3(item, "loc", loc);
+    o->loc = {{loc[0], loc[1], loc[2]}};
+    std::string material = json::get_str(item, "material");
+    if (material == "custom") {
+        double e = json::get_num(item, "eps", 1.0);
+        o->epsScalar = e;
+        o->eps = {{e, e, e}};
+        // Extension: diagonal anisotropic eps_infty per Yee component.
+        const Value* ed = item.path("eps_diag");
+        if (ed && ed->size() == 3)
+            for (int c = 0; c < 3; ++c) o->eps[c] = ed->child(c).as_num();
+        o->mu = json::get_num(item, "mu", 1.0);
+        if (json::get_num(item, "tellegen", 0.0) != 0.0)
+            throw std::logic_error("chiral (tellegen) materials are outside the supported time-stepping path");
+        const Value& pols = json::get_child_or_empty(item, "pols");
+        for (size_t i = 0; i < pols.size(); ++i) {
+            const Value& pv = pols.child(i);
+            if (json::get_str(pv, "dipOrE", "isotropic") != "isotropic" || json::get_num(pv, "sigma_m", 0.0) != 0.0 ||
+                json::get_num(pv, "tau", 0.0) != 0.0 || json::get_bool(pv, "molecular_trans", false))
+                throw std::logic_error("only isotropic electric Lorentz/Drude poles are supported");
+            Pole p;
+            p.gam = json::get_num(pv, "gamma") * kPi;
+            p.omg = json::get_num(pv, "omega") * 2 * kPi;
+            p.sigP = json::get_num(pv, "sigma_p", 0.0);
+            o->pols.push_back(p);
+        }
+    } else if (material == "Au" || material == "au" || material == "AU") {
+        o->epsScalar = 1.0 + 1.0e-14;
+        o->eps = {{o->epsScalar, o->epsScalar, o->epsScalar}};
+        o->pols = metalPoles(kRakicAu, IP.a);
+    } else if (material == "Ag" || material == "ag" || material == "AG") {
+        o->epsScalar = 1.0 + 1.1e-14;
+        o->eps = {{o->epsScalar, o->epsScalar, o->epsScalar}};
+        o->pols = metalPoles(kRakicAg, IP.a);
+    } else if (material == "vac" || material == "Vac" || material == "VAC") {
+    } else if (material == "SiO2") {
+        o->epsScalar = 2.1025;
+        o->eps = {{2.1025, 2.1025, 2.1025}};
+    } else if (material == "TiO2") {
+        o->epsScalar = 6.20001;
+        o->eps = {{6.20001, 6.20001, 6.20001}};
+    } else if (material == "Si") {
+        o->epsScalar = 11.8336;
+        o->eps = {{11.8336, 11.8336, 11.8336}};
+    } else {
+        throw std::logic_error("The material name " + material + " is not supported");
+    }
+    setUnitVectors(*o, item, !IP.is3D());
+    std::string s = json::get_str(item, "shape");
+    if (s == "block") {
+        o->shape = Shape::BLOCK;
+        o->geo = json::get_vec(item, "size");
+        o->geo.resize(3, 0.0);
+        if (json::get_num(item, "rad_curve", 0.0) != 0.0) throw std::logic_error("rounded blocks are not supported");
+    } else if (s == "sphere") {
+        o->shape = Shape::SPHERE;
+        o->geo = {json::get_num(item, "radius", 0.0)};
+    } else if (s == "cylinder") {
+        o->shape = Shape::CYLINDER;
+        o->geo = {json::get_num(item, "radius", 0.0), json::get_num(item, "length", 0.0)};
+    } else if (s == "ellipsoid") {
+        o->shape = Shape::ELLIPSOID;
+        o->geo = json::get_vec(item, "size");
+        o->geo.resize(3, 0.0);
+    } else {
+        throw std::logic_error("shape " + s + " is not supported");
+    }
+    o->geoML = o->geo;
+    return o;
+}
+
+static void addMLBuff(Object& o, double d) {
+    // addMLBuff per shape, src/OBJECTS/Obj.hpp:537, 653, 776, 949.
+    switch (o.shape) {
+        case Shape::SPHERE: o.geoML[0] += 3.0 * d; break;
+        case Shape::CYLINDER: o.geoML[0] += 3.0 * d; o.geoML[1] += 3.0 * d; break;
+        case Shape::BLOCK:
+        case Shape::ELLIPSOID:
+            for (int i = 0; i < 3; ++i) o.geoML[i] += 3.0 * d;
+            break;
+    }
+}
+
+static std::vector<cplx> readCplxMatrix(const Value& v, int n, double scale) {
+    std::vector<double> re = json::get_vec(v, "re"), im = json::get_vec(v, "im");
+    if (re.size() != static_cast<size_t>(n * n)) throw std::logic_error("matrix extension needs nlevel^2 real parts");
+    if (im.empty()) im.assign(n * n, 0.0);
+    if (im.size() != static_cast<size_t>(n * n)) throw std::logic_error("matrix extension needs nlevel^2 imaginary parts");
+    std::vector<cplx> m(n * n);
+    for (int i = 0; i < n * n; ++i) m[i] = cplx(re[i], im[i]) * scale;
+    return m;
+}
+
+ProgramInputs parseInputs(const std::string& text) {
+    Value IP = json::parse(text);
+    ProgramInputs P;
+    // Computational cell, src/INPUTS/parallelInputs.cpp:12-33.
+    P.periodic = json::get_bool(IP, "CompCell.PBC", false);
+    if (P.periodic) throw std::logic_error("periodic (Bloch) boundaries are outside the supported time-stepping path");
+    if (json::get_bool(IP, "CompCell.cplxFields", false))
+        throw std::logic_error("complex fields are outside the supported time-stepping path");
+    P.res = json::get_int(IP, "CompCell.res", -1);
+    P.courant = json::get_num(IP, "CompCell.courant", 0.5);
+    P.a = json::get_num(IP, "CompCell.a", 1e-7);
+    P.tMax = json::get_num(IP, "CompCell.tLim");
+    P.I0 = json::get_num(IP, "CompCell.I0", P.a * EPS0 * SPEED_OF_LIGHT);
+    double tmp[3];
+    json::get_arr3(IP, "CompCell.size", tmp);
+    P.size = {{tmp[0], tmp[1], tmp[2]}};
+    json::get_arr3(IP, "CompCell.stepSize", tmp, 1.0 / P.res);
+    P.d = {{tmp[0], tmp[1], tmp[2]}};
+    // dt includes the z term even in 2D (parallelInputs.cpp:27).
+    P.dt = json::get_num(IP, "CompCell.dt",
+                         P.courant / std::sqrt(1.0 / (P.d[0] * P.d[0]) + 1.0 / (P.d[1] * P.d[1]) + 1.0 / (P.d[2] * P.d[2])));
+    P.pmlSigOptRat = json::get_num(IP, "PML.sigOptRat", 1.0);
+    P.pmlKappaMax = json::get_num(IP, "PML.kappaMax", 1.0);
+    P.pmlAMax = json::get_num(IP, "PML.aMax", 0.0);
+    P.pmlMa = json::get_num(IP, "PML.ma", 1.0);
+    P.pmlM = json::get_num(IP, "PML.m", 3.0);
+    std::string prec = json::get_str(IP, "GPU.precision", "fp64");
+    if (prec != "fp32" && prec != "fp64") throw std::logic_error("GPU.precision must be fp32 or fp64");
+    P.fp32 = (prec == "fp32");
+    if (!P.is3D()) {
+        std::string pol = json::get_str(IP, "CompCell.pol", "Ez");
+        if (pol != "Ez" && pol != "Hx" && pol != "Hy")
+            throw std::logic_error("2D runs are supported in the TMz polarisation (Ez, Hx, Hy)");
+        P.pol = string2pol(pol);
+    }
+    for (int i = 0; i < 3; ++i)
+        if (P.d[i] < 0) throw std::logic_error("Please define a positive step size.");
+    if (P.dt > 1.0 / std::sqrt(1.0 / (P.d[0] * P.d[0]) + 1.0 / (P.d[1] * P.d[1]) + 1.0 / (P.d[2] * P.d[2])))
+        throw std::logic_error("Time step is larger than the stable time step for the calculation.");
+    json::get_arr3(IP, "PML.thickness", tmp);
+    for (int i = 0; i < 3; ++i) {
+        if (tmp[i] * 2 > P.size[i]) throw std::logic_error("PML size is larger than the cell size, this will lead to infinte fields.");
+        P.pmlThickness[i] = find_pt(tmp[i], P.d[i]);
+    }
+    if (json::get_child_or_empty(IP, "TFSF").size() > 0)
+        throw std::logic_error("TFSF surfaces are not supported on this path; use a soft source sheet");
+
+    // Soft sources, parallelInputs.cpp:115-216.
+    const Value& SL = json::get_child_or_empty(IP, "SourceList");
+    for (size_t ii = 0; ii < SL.size(); ++ii) {
+        const Value& it = SL.child(ii);
+        SourceSpec s;
+        s.pol = string2pol(json::get_str(it, "pol"));
+        const Value& PL = json::get_child_or_empty(it, "PulseList");
+        for (size_t pp = 0; pp < PL.size(); ++pp) {
+            const Value& pul = PL.child(pp);
+            PulseSpec ps;
+            ps.shape = string2prof(json::get_str(pul, "profile"));
+            double Emax = json::get_num(pul, "Field_Intensity", 1.0) * P.a * EPS0 * SPEED_OF_LIGHT / P.I0;
+            std::vector<cplx> f;
+            switch (ps.shape) {
+                case PulseShape::GAUSSIAN:
+                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                    f.push_back(1.0 / json::get_num(pul, "fwidth"));
+                    f.push_back(json::get_num(pul, "cutoff") * f[1]);
+                    f.push_back(json::get_num(pul, "t_0", std::real(f[1] * f[2])));
+                    f[2] += f[3];
+                    break;
+                case PulseShape::BH:
+                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                    f.push_back(json::get_num(pul, "tau", 1.0 / json::get_num(pul, "fwidth")));
+                    // The reference's t_0 default reads past the end of its
+                    // parameter list (parallelInputs.cpp:145), so t_0 is required.
+                    f.push_back(json::get_num(pul, "t_0"));
+                    f.push_back(json::get_num(pul, "BH1", 0.35875));
+                    f.push_back(json::get_num(pul, "BH2", 0.48829));
+                    f.push_back(json::get_num(pul, "BH3", 0.14128));
+                    f.push_back(json::get_num(pul, "BH4", 0.01168));
+                    break;
+                case PulseShape::RECT:
+                    f.push_back(json::get_num(pul, "tau"));
+                    f.push_back(json::get_num(pul, "t_0"));
+                    f.push_back(json::get_num(pul, "n", 30) * 2.0);
+                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                    break;
+                case PulseShape::CONTINUOUS:
+                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                    break;
+                case PulseShape::RAMP_CONT:
+                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                    f.push_back(json::get_num(pul, "ramp_val"));
+                    break;
+                case PulseShape::RICKER:
+                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                    f.push_back(json::get_num(pul, "fwidth"));
+                    f.push_back(json::get_num(pul, "cutoff"));
+                    break;
+            }
+            f.push_back(Emax);  // parallelFDTDField.cpp:474
+            ps.param = f;
+            s.pulses.push_back(ps);
+        }
+        double tsz[3], tloc[3];
+        json::get_arr3(it, "size", tsz);
+        for (int c = 0; c < 3; ++c) s.sz[c] = find_pt(tsz[c], P.d[c]) + 1;
+        s.phi = json::get_num(it, "phi", 90);
+        if (!(s.phi == 90 || s.phi == 180 || s.phi == 270 || s.phi == 0))
+            throw std::logic_error("oblique sources are not supported");
+        json::get_arr3(it, "loc", tloc);
+        for (int c = 0; c < 3; ++c) {
+            if (tloc[c] + tsz[c] / 2.0 > P.size[c] / 2.0 || tloc[c] - tsz[c] / 2.0 < -1.0 * P.size[c] / 2.0)
+                throw std::logic_error("The source is at least partially outside the FDTD Cell");
+            s.loc[c] = find_pt(tloc[c] + P.size[c] / 2.0 - tsz[c] / 2.0, P.d[c]);
+        }
+        P.srcs.push_back(s);
+    }
+
+    // Objects: vacuum background first (parallelInputs.cpp:410-421).
+    auto bg = std::make_shared<Object>();
+    bg->shape = Shape::BLOCK;
+    bg->geo = {P.size[0], P.size[1], P.size[2]};
+    bg->geoML = bg->geo;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) bg->T[3 * r + c] = (r == c) ? 1.0 : 0.0;
+    P.objs.push_back(bg);
+    std::array<int, 3> nv = P.nVec();
+    const double e_conv = ELEMENTARY_CHARGE * EPS0 * std::pow(SPEED_OF_LIGHT / P.I0, 2) / P.a;
+    const Value& OL = json::get_child_or_empty(IP, "ObjectList");
+    for (size_t oi = 0; oi < OL.size(); ++oi) {
+        const Value& it = OL.child(oi);
+        auto obj = ptreeToObject(it, P);
+        const Value& BS = json::get_child_or_empty(it, "Basis_Set");
+        // Later objects overwrite earlier emitter points (parallelInputs.cpp:438-451).
+        double dmin = std::min(P.d[0], std::min(P.d[1], P.d[2]));
+        for (auto& q : P.qes) {
+            for (size_t ll = 0; ll < q.locs.size(); ++ll) {
+                std::array<double, 3> l = {{(q.locs[ll][0] - (nv[0] - 1) / 2.0 - 1) * P.d[0],
+                                            (q.locs[ll][1] - (nv[1] - 1) / 2.0 - 1) * P.d[1],
+                                            (q.locs[ll][2] - (nv[2] - 1) / 2.0 - 1) * P.d[2]}};
+                if (obj->isObj(l, dmin, false)) {
+                    q.locs.erase(q.locs.begin() + ll);
+                    --ll;
+                }
+            }
+        }
+        if (BS.size() > 0) {
+            QESpec q;
+            for (size_t b = 0; b < BS.size(); ++b)
+                q.basis.push_back({{json::get_int(BS.child(b), "l", 0), json::get_int(BS.child(b), "m", 0)}});
+            q.nlevel = static_cast<int>(q.basis.size());
+            q.na = json::get_num(it, "mol_den", 1.0) * std::pow(P.a, 3);
+            // Energy levels: delta-function levels (parallelInputs.cpp:462-493).
+            struct Lev { std::vector<double> E, w; int described; };
+            std::vector<Lev> levs;
+            const Value& EL = json::get_child_or_empty(it, "Energy_Levels");
+            for (size_t e = 0; e < EL.size(); ++e) {
+                const Value& lv = EL.child(e);
+                Lev L;
+                L.E = json::get_vec(lv, "E_cen");
+                for (double& x : L.E) x *= e_conv;
+                L.w = json::get_vec(lv, "weights");
+                if (L.w.empty()) L.w.assign(L.E.size(), 1.0);
+                if (json::get_str(lv, "distribution", "Delta_Fxn") != "Delta_Fxn" || json::get_int(lv, "nstates", 1) != 1)
+                    throw std::logic_error("only delta-function energy levels are supported");
+                L.described = json::get_int(lv, "levs_described", 1);
+                levs.push_back(L);
+            }
+            // GenerateAllELevCombos (parallelFDTDField.cpp:1698-1715).
+            std::vector<std::pair<std::vector<double>, double>> members;
+            std::vector<double> cur;
+            std::function<void(size_t, double)> rec = [&](size_t depth, double w) {
+                if (depth == levs.size()) {
+                    members.push_back({cur, w});
+                    return;
+                }
+                for (size_t s = 0; s < levs[depth].E.size(); ++s) {
+                    for (int k = 0; k < levs[depth].described; ++k) cur.push_back(levs[depth].E[s]);
+                    rec(depth + 1, w * levs[depth].w[s]);
+                    for (int k = 0; k < levs[depth].described; ++k) cur.pop_back();
+                }
+            };
+            rec(0, 1.0);
+            q.members = members;
+            for (auto& m : q.members)
+                if (static_cast<int>(m.first.size()) != q.nlevel)
+                    throw std::logic_error("number of energy levels must equal the basis size");
+            const Value& CP = json::get_child_or_empty(it, "couplings");
+            for (size_t c = 0; c < CP.size(); ++c)
+                q.couplings.push_back(CP.child(c).as_num() * 1.0e-21 / std::pow(P.a, 2.0) / P.I0);
+            const Value& GV = json::get_child_or_empty(it, "gam");
+            for (size_t g = 0; g < GV.size(); ++g) {
+                std::vector<double> row;
+                const Value& gg = json::get_child_or_empty(GV.child(g), "g");
+                for (size_t k = 0; k < gg.size(); ++k) row.push_back(gg.child(k).as_num() * P.a / SPEED_OF_LIGHT);
+                q.gam.push_back(row);
+            }
+            // Emitter points (parallelInputs.cpp:596-614).
+            for (int zz = 0; zz < nv[2]; ++zz)
+                for (int yy = 0; yy < nv[1]; ++yy)
+                    for (int xx = 0; xx < nv[0]; ++xx) {
+                        std::array<double, 3> l = {{(xx - (nv[0] - 1) / 2.0) * P.d[0], (yy - (nv[1] - 1) / 2.0) * P.d[1],
+                                                    (zz - (nv[2] - 1) / 2.0) * P.d[2]}};
+                        if (obj->isObj(l, dmin, false)) q.locs.push_back({{xx, yy, zz}});
+                    }
+            const Value& RO = json::get_child_or_empty(it, "RelaxationOperators");
+            for (size_t r = 0; r < RO.size(); ++r) {
+                RelaxSpec rs;
+                rs.n0 = json::get_int(RO.child(r), "state_i", 0);
+                rs.nf = json::get_int(RO.child(r), "state_f", 0);
+                rs.rate = json::get_num(RO.child(r), "rate") * P.a / SPEED_OF_LIGHT;
+                rs.dephasing = json::get_num(RO.child(r), "dephasing_rate", 0.0) * P.a / SPEED_OF_LIGHT;
+                q.relax.push_back(rs);
+            }
+            const Value& DL = json::get_child_or_empty(it, "dtc_levs");
+            for (size_t k = 0; k < DL.size(); ++k) q.dtcLevs.push_back(static_cast<int>(DL.child(k).as_num()));
+            q.popTimeInt = json::get_int(it, "levDTC_timeInt", 1);
+            // Extension: explicit Hermitian H0 (eV) and dipole matrices (Debye).
+            const Value* H0 = it.path("H0_eV");
+            if (H0) q.h0Full = readCplxMatrix(*H0, q.nlevel, e_conv);
+            const Value* MU = it.path("dipole_Debye");
+            if (MU) {
+                const char* ax[3] = {"x", "y", "z"};
+                for (int c = 0; c < 3; ++c) {
+                    const Value* m = MU->find(ax[c]);
+                    if (m) q.muFull[c] = readCplxMatrix(*m, q.nlevel, 1.0e-21 / std::pow(P.a, 2.0) / P.I0);
+                    else q.muFull[c].assign(q.nlevel * q.nlevel, 0.0);
+                }
+            }
+            q.objIndex = static_cast<int>(P.objs.size());
+            obj->ML = true;
+            addMLBuff(*obj, std::max(P.d[0], std::max(P.d[1], P.d[2])));
+            P.qes.push_back(q);
+        }
+        P.objs.push_back(obj);
+    }
+
+    // Detectors, parallelInputs.cpp:671-755.
+    const Value& DLs = json::get_child_or_empty(IP, "DetectorList");
+    for (size_t dd = 0; dd < DLs.size(); ++dd) {
+        const Value& it = DLs.child(dd);
+        DetectorSpec D;
+        D.type = string2out(json::get_str(it, "type"));
+        D.name = json::get_str(it, "fname", "dtc") + "_field_" + std::to_string(dd) + ".dat";
+        D.cls = string2dtcclass(json::get_str(it, "dtc_class", "cout"));
+        D.SI = json::get_bool(it, "SI", true);
+        D.timeInterval = json::get_num(it, "Time_Interval", P.dt);
+        double tsz[3], tloc[3];
+        json::get_arr3(it, "size", tsz);
+        for (int c = 0; c < 3; ++c) D.sz[c] = find_pt(tsz[c], P.d[c]) + 1;
+        json::get_arr3(it, "loc", tloc);
+        for (int c = 0; c < 3; ++c) {
+            if (tloc[c] - tsz[c] / 2.0 < -1.0 * P.size[c] / 2.0 || tloc[c] + tsz[c] / 2.0 > P.size[c] / 2.0)
+                throw std::logic_error("A detector is outside the FDTD cell.");
+            D.loc[c] = find_pt(tloc[c] + P.size[c] / 2.0 - tsz[c] / 2.0, P.d[c]);
+        }
+        if (D.cls == DtcClass::FREQ) throw std::logic_error("frequency detectors: use a FluxList entry on this path");
+        P.dtcs.push_back(D);
+    }
+
+    // Flux boxes, parallelInputs.cpp:757-838.
+    const Value& FL = json::get_child_or_empty(IP, "FluxList");
+    for (size_t dd = 0; dd < FL.size(); ++dd) {
+        const Value& it = FL.child(dd);
+        FluxSpec F;
+        F.name = json::get_str(it, "name", "flux");
+        double tsz[3], tloc[3];
+        json::get_arr3(it, "size", tsz);
+        for (int c = 0; c < 3; ++c) F.sz[c] = find_pt(tsz[c], P.d[c]) + 1;
+        json::get_arr3(it, "loc", tloc);
+        for (int c = 0; c < 3; ++c) {
+            if (tloc[c] - tsz[c] / 2.0 < -1.0 * P.size[c] || tloc[c] + tsz[c] / 2.0 > P.size[c])
+                throw std::logic_error("A flux region is outside the FDTD cell.");
+            F.loc[c] = find_pt(tloc[c] + P.size[c] / 2.0 - tsz[c] / 2, P.d[c]);
+        }
+        F.weight = json::get_num(it, "weight", 1.0);
+        F.timeInt = static_cast<int>(std::floor(json::get_num(it, "Time_Interval", P.dt) / P.dt + 0.50));
+        if (F.timeInt <= 0) F.timeInt = 1;
+        double fCen = json::get_num(it, "fcen", -1.0), fWidth = json::get_num(it, "fwidth", -1.0);
+        double lamL = json::get_num(it, "lamL", -1.0), lamR = json::get_num(it, "lamR", -1.0);
+        int nFreq = json::get_int(it, "nfreq", -1);
+        if (nFreq <= 0) throw std::logic_error("The flux regions need to have a number of frequencies specified");
+        F.freqs.assign(nFreq, 0.0);
+        if (fCen != -1.0 && fWidth != -1.0) {
+            if (lamL != -1.0 && lamR != -1.0) throw std::logic_error("Both a freq and wavelength range is defined");
+            double dOmg = fWidth / static_cast<double>(nFreq - 1);
+            for (int ii = 0; ii < nFreq; ++ii) F.freqs[ii] = (fCen - fWidth / 2.0 + ii * dOmg) * 2.0 * kPi;
+        } else if (lamL != -1.0 && lamR != -1.0) {
+            double dLam = (lamR - lamL) / static_cast<double>(nFreq - 1);
+            for (int ii = 0; ii < nFreq; ++ii) F.freqs[ii] = 2.0 * kPi / (lamL + ii * dLam);
+        } else {
+            throw std::logic_error("All fluxes must either have fcen and fwidth defined or lamL and lamR defined");
+        }
+        F.SI = json::get_bool(it, "SI", false);
+        F.crossSec = json::get_bool(it, "cross_sec", false);
+        P.fluxes.push_back(F);
+    }
+    for (auto& o : P.objs) o->setUpConsts(P.dt);
+    return P;
+}
+
+}  // namespace chiml

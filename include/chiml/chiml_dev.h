@@ -1,0 +1,180 @@
+/*
+ * chiml_dev.h -- C ABI of the MI355X time-stepping library (device layer).
+ *
+ * Plain C types only: pointers, sizes and numbers.  A context holds one
+ * device-resident simulation domain (one GPU, one domain, no halo exchange).
+ *
+ * The reference drives its step loop through C++ objects bound once in the
+ * FDTD manager constructor and called from step()
+ * (src/FDTD_MANAGER/parallelFDTDField.hpp:1228-1303).  This ABI is the
+ * boundary those objects call into on the GPU:
+ *
+ *   chiml_dev_create        replaces the grid/list/PML/ADE/source/detector/
+ *                           QE construction of parallelFDTDFieldReal
+ *                           (src/FDTD_MANAGER/parallelFDTDField.cpp:13-852);
+ *                           the caller passes flat tables it built on the host.
+ *   chiml_dev_step          replaces n calls of parallelFDTDFieldBase::step()
+ *                           (parallelFDTDField.hpp:1228-1303; the loop in
+ *                           src/main.cpp:56-57).
+ *   chiml_dev_read_field    replaces reading E_/H_ grids (parallelFDTDField.hpp:173-174).
+ *   chiml_dev_read_points   replaces parallelDetectorBase::output series
+ *                           (src/DTC/parallelDTC.hpp:119; parallelDTC_TXT.cpp:25-54).
+ *   chiml_dev_read_dft      replaces the frequency-domain storage read by
+ *                           parallelFluxDTC::getFlux (src/DTC/parallelFlux.hpp:406).
+ *   chiml_dev_read_pops     replaces QEPopDtc::toFile data (src/ML/QEPopDtc.cpp:36-61).
+ *   chiml_dev_read_density  replaces reading Density::density() (src/ML/density.hpp).
+ *
+ * Conventions: every call returns 0 (CHIML_OK) or a negative CHIML_E_* code;
+ * chiml_dev_last_error() gives the message.  A context is not thread-safe:
+ * one host thread per context.  Several contexts may share one GPU.
+ * Field arrays use the interior-point layout index x + nx*(y + ny*z),
+ * components 0..5 = Ex, Ey, Ez, Hx, Hy, Hz; a 2D domain (nz == 1) is TMz and
+ * holds Ez, Hx, Hy.
+ */
+#ifndef CHIML_DEV_H
+#define CHIML_DEV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHIML_OK 0
+#define CHIML_E_INVALID (-1)
+#define CHIML_E_HIP (-2)
+#define CHIML_E_NOMEM (-3)
+#define CHIML_E_STATE (-4)
+
+#define CHIML_MAX_MAT 256
+#define CHIML_MAX_POLES 8
+#define CHIML_MAX_SRC 64
+
+/* Material flags (per E-component material id). */
+#define CHIML_MAT_DCELL 1 /* E is derived from D (dispersive, emitter or PML-with-material cell) */
+
+typedef struct chiml_dev chiml_dev_t;
+
+/* One quantum-emitter group (one ObjectList entry with a basis set). */
+typedef struct {
+    int nlevel;            /* N */
+    int nmembers;          /* broadening members (Hamiltonians) */
+    int nemit;             /* emitter points */
+    const int* emit_xyz;   /* [nemit][3] interior point coordinates */
+    const double* h0;      /* [nmembers][N*N][2] column-major complex, FDTD units */
+    const double* mu;      /* [3][N*N][2] complex dipole matrices (x, y, z) */
+    const double* weight;  /* [nmembers] initial rho_00 of each member */
+    int ngam;              /* Lindblad superoperator entries (COO) */
+    const int* gam_row;    /* out index i in d rho_i += g * rho_j */
+    const int* gam_col;    /* j */
+    const double* gam_val; /* g */
+    double na;             /* molecular density times a^3 */
+    double hbar;           /* hbar in FDTD units */
+    int box_min[3];        /* box of emitter points (points) */
+    int box_n[3];
+    int npop;              /* population detectors */
+    const int* pop_flat;   /* flat density index per detector */
+    int pop_interval;      /* levDTC_timeInt */
+} chiml_qe_t;
+
+typedef struct {
+    int nx, ny, nz;       /* interior points per axis; nz == 1 selects 2D TMz */
+    int precision;        /* 32 or 64: arithmetic type of fields, ADE and CPML */
+    double dt;
+    double d[3];
+    int nsteps;           /* length of the per-step tables below */
+
+    /* Materials: per-component uint8 id maps (NULL means every cell has id 0)
+     * and per-id coefficient tables. */
+    const uint8_t* mat_e[3];
+    int nmat_e[3];
+    const double* mat_e_ce[3];     /* dt/eps_c, or dt in a PML without material */
+    const double* mat_e_inveps[3]; /* 1/eps_c (D-cells) */
+    const int* mat_e_flags[3];     /* CHIML_MAT_* */
+    const int* mat_e_poleset[3];   /* pole set per id, -1 for none */
+    const uint8_t* mat_h[3];       /* NULL with mu == 1 everywhere */
+    int nmat_h[3];
+    const double* mat_h_ch[3];     /* dt/mu_c */
+
+    /* Lorentz/Drude poles in current form: J' = c1 P + c2 J + g E, P' = P + J'. */
+    int npoleset;
+    const int* pole_count;  /* [npoleset] */
+    const double* pole_coef; /* [npoleset][CHIML_MAX_POLES][3] = c1, c2, g */
+    int disp_box[3][6];      /* per E component: x0,x1,y0,y1,z0,z1 (empty if x0 >= x1) */
+
+    /* CPML.  pml_T: slab thickness per axis.  For component c (0..5) and
+     * derivative axis a, profiles along a (length n_a) of b, c/d and 1/kappa,
+     * and the psi-carrying positions [0, psi_lo) U [psi_hi0, psi_hi1). */
+    int pml_T[3];
+    const double* pml_b[6][3];
+    const double* pml_cd[6][3];
+    const double* pml_kinv[6][3];
+    int psi_lo[6][3];
+    int psi_hi0[6][3];
+    int psi_hi1[6][3];
+
+    /* Soft sources: boxes [x0,x1) x [y0,y1) x [z0,z1) on one component, with
+     * per-step increments amp[step*nsrc + s] = dt * Re(sum of pulses). */
+    int nsrc;
+    const int* src_comp;  /* [nsrc] 0..5 */
+    const int* src_box;   /* [nsrc][6] */
+    const double* src_amp; /* [nsteps][nsrc] */
+
+    /* Point detectors: npt sampled values per output, each the mean of two
+     * cells times conv; sampled when t_step % interval == 0, t_step >= 0. */
+    int ndet;
+    const int* det_interval; /* [ndet] */
+    const int* det_npts;     /* [ndet] */
+    const int* det_comp;     /* [sum npts][2] component of the two cells */
+    const long long* det_cell; /* [sum npts][2] interior index of the two cells */
+    const double* det_conv;  /* [ndet] */
+    const int* det_pow;      /* [ndet] 1: conv/2 (|a|^2 + |b|^2) */
+
+    /* DFT accumulators: nsamp cells, nfreq frequencies, phasor table for
+     * every sampling instant k (sample k taken at t_step = k*interval). */
+    int ndft;
+    const int* dft_comp;     /* [ndft] */
+    const long long* dft_cell; /* [ndft] */
+    int nfreq;
+    int dft_interval;
+    const double* dft_phasor; /* [nsteps/dft_interval + 1][nfreq][2] exp(-i w t) */
+
+    /* Maxwell-Liouville emitter groups. */
+    int nqe;
+    const chiml_qe_t* qe;
+    const double* eps_node;  /* (nx*ny*nz) eps at the nodes, used by the emitter back-coupling */
+} chiml_problem_t;
+
+int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out);
+int chiml_dev_step(chiml_dev_t* ctx, int nsteps);
+int chiml_dev_sync(chiml_dev_t* ctx);
+/* Copies one field component to host memory as float64 (n = nx*ny*nz). */
+int chiml_dev_read_field(chiml_dev_t* ctx, int comp, double* host);
+/* Number of samples stored so far for detector d; values [ns][npts]. */
+int chiml_dev_read_points(chiml_dev_t* ctx, int d, double* values, size_t cap, size_t* nsamples);
+/* DFT accumulators as [ndft][nfreq] complex (re, im interleaved). */
+int chiml_dev_read_dft(chiml_dev_t* ctx, double* reim, size_t cap);
+/* Population series of detector j of emitter group q: [ns] complex. */
+int chiml_dev_read_pops(chiml_dev_t* ctx, int q, int j, double* reim, size_t cap, size_t* nsamples);
+/* Density matrix of (group q, member m, emitter e): N*N complex column-major. */
+int chiml_dev_read_density(chiml_dev_t* ctx, int q, int m, int e, double* reim);
+/* Steps taken so far and the device stream (hipStream_t) used. */
+int chiml_dev_steps_done(chiml_dev_t* ctx, int* steps);
+/* Per-step kernel timing: enables HIP events around the dominant kernel. */
+int chiml_dev_profile(chiml_dev_t* ctx, int enable);
+int chiml_dev_profile_read(chiml_dev_t* ctx, const char* kernel, double* total_ms, int* launches);
+/* Device memory the context holds (bytes). */
+int chiml_dev_footprint(chiml_dev_t* ctx, size_t* bytes);
+const char* chiml_dev_last_error(const chiml_dev_t* ctx);
+const char* chiml_dev_global_error(void);
+void chiml_dev_destroy(chiml_dev_t* ctx);
+
+/* Batched stepping: advances several contexts of the same class by nsteps,
+ * interleaving their launches on their own streams. */
+int chiml_dev_step_batch(chiml_dev_t** ctxs, int n, int nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHIML_DEV_H */

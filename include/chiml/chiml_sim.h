@@ -1,0 +1,58 @@
+/*
+ * chiml_sim.h -- C ABI of the JSON-driven manager (host layer).
+ *
+ * chiml_sim_create takes a chiML input deck (the reference's JSON schema,
+ * src/INPUTS/parallelInputs.cpp:12-838) and builds the device context the way
+ * the reference's parallelFDTDFieldReal constructor builds its grids, lists,
+ * PMLs, sources, detectors, flux boxes and emitters
+ * (src/FDTD_MANAGER/parallelFDTDField.cpp:13-852).  chiml_sim_step is the
+ * reference's `for (tt < nSteps) FF.step()` (src/main.cpp:54-57); the read
+ * functions return what the reference writes at the end of main()
+ * (src/main.cpp:67-118): detector series, flux spectra and populations.
+ *
+ * All calls return 0 on success or a negative CHIML_E_* code
+ * (include/chiml/chiml_dev.h); chiml_sim_last_error() gives the message.
+ */
+#ifndef CHIML_SIM_H
+#define CHIML_SIM_H
+
+#include <stddef.h>
+
+#include "chiml_dev.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chiml_sim chiml_sim_t;
+
+int chiml_sim_create(const char* deck_json, chiml_sim_t** out);
+/* Advance n steps (asynchronous on the device). */
+int chiml_sim_step(chiml_sim_t* s, int n);
+int chiml_sim_sync(chiml_sim_t* s);
+/* n3 = interior points (nx, ny, nz; nz = 1 in 2D), dt, total steps ceil(tLim/dt),
+ * precision (32/64), number of point detectors, flux boxes, emitter groups. */
+int chiml_sim_info(chiml_sim_t* s, int* n3, double* dt, int* nsteps, int* precision, int* ndet, int* nflux, int* nqe);
+int chiml_sim_field(chiml_sim_t* s, int comp, double* out);
+/* Detector d: writes times[ns] and values[ns*npts] when the buffers are large enough. */
+int chiml_sim_detector(chiml_sim_t* s, int d, double* times, double* values, size_t cap_samples, size_t* nsamples, int* npts);
+/* Total flux of flux box f per frequency (complex, re/im interleaved), as
+ * parallelFluxDTC::getFlux sums faces (src/DTC/parallelFlux.hpp:406-534). */
+int chiml_sim_flux(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq);
+int chiml_sim_pops(chiml_sim_t* s, int q, int j, double* reim, size_t cap, size_t* nsamples);
+int chiml_sim_density(chiml_sim_t* s, int q, int member, int emitter, double* reim, int* nlevel);
+/* Cell-updates per step and the algorithmic HBM bytes per step of the field
+ * kernels (update_h + update_e), used for the roofline figure. */
+int chiml_sim_cost(chiml_sim_t* s, double* cells, double* bytes_h, double* bytes_e, double* bytes_ml);
+int chiml_sim_profile(chiml_sim_t* s, int enable);
+int chiml_sim_profile_read(chiml_sim_t* s, const char* kernel, double* total_ms, int* launches);
+int chiml_sim_footprint(chiml_sim_t* s, size_t* bytes);
+/* Estimated device bytes for a deck without creating it (admission control). */
+int chiml_sim_estimate_bytes(const char* deck_json, size_t* bytes);
+const char* chiml_sim_last_error(void);
+void chiml_sim_destroy(chiml_sim_t* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHIML_SIM_H */

@@ -1,0 +1,1615 @@
+// ORACLE -- test infrastructure only.
+//
+// A CPU FP64 restatement of the reference's time-stepping path, used by the
+// tests, by the smoke check and as bench.py's CPU baseline.  It is never part
+// of the product path: the HIP library does not link, load or call it.
+//
+// It follows the reference operation for operation and in the same order:
+//   * run lists built exactly as initializeList/getBlasLists/fillBlasLists/
+//     populateUpLists (src/FDTD_MANAGER/parallelFDTDField.hpp:628-856),
+//   * Yee curls as four BLAS-1 axpys per run (src/UTIL/FDTD_up_eq.cpp:10-35),
+//   * CPML psi/grid lists as getPsiUpList/getGridUpList and the addPsi kernels
+//     (src/PML/parallelPML.hpp:102-806, src/PML/parallelPML.cpp:12-40),
+//   * Lorentz/Drude ADE in D form (UpdateLorPol + DtoU,
+//     src/UTIL/FDTD_up_eq.cpp:425-448, 838-848),
+//   * soft sources (src/SOURCE/parallelSourceNormal.{hpp:69-150,cpp:15-37}),
+//   * point detectors (src/DTC/parallelDTC.hpp:44-95, parallelDTC_TXT.cpp:25-54),
+//   * DFT flux boxes (src/DTC/parallelFlux.{cpp:13-71,338-543},
+//     parallelFlux.hpp:296-615, parallelStorageFreqDTC.cpp:21-30),
+//   * Maxwell-Liouville emitters (src/ML/parallelQE.{hpp,cpp},
+//     src/ML/Hamiltonian.cpp, src/ML/density.hpp, src/ML/QEPopDtc.*),
+// for a single MPI rank without periodic boundaries (the reference binds the
+// boundary-condition slots to no-ops in that case,
+// src/FDTD_MANAGER/parallelFDTDField.cpp:179-184, 344-349).
+//
+// Parity pins: see tests/test_oracle_pins.py (closed forms and invariants)
+// and the golden fixtures under tests/golden/.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../chiml_amd/csrc/setup.hpp"
+
+namespace orc {
+using namespace chiml;
+
+// ----------------------------------------------------------------- BLAS-1 ---
+static void daxpy(int n, double a, const double* x, int incx, double* y, int incy) {
+    for (int i = 0; i < n; ++i) y[i * incy] += a * x[i * incx];
+}
+static void dscal(int n, double a, double* x, int inc) {
+    for (int i = 0; i < n; ++i) x[i * inc] *= a;
+}
+static void dcopy(int n, const double* x, int incx, double* y, int incy) {
+    for (int i = 0; i < n; ++i) y[i * incy] = x[i * incx];
+}
+
+// A reference parallelGrid for one rank: ghost border of one point,
+// index x + lx*(z + y*lz) (src/GRID/parallelGrid.hpp:563).
+template <typename T>
+struct Grid {
+    int lx = 0, ly = 0, lz = 0;
+    std::vector<T> v;
+    bool ok() const { return !v.empty(); }
+    void init(int x, int y, int z, T val = T(0)) {
+        lx = x;
+        ly = y;
+        lz = z;
+        v.assign(static_cast<size_t>(x) * y * z, val);
+    }
+    int ind(int x, int y, int z) const { return x + lx * (z + y * lz); }
+    T& at(int x, int y, int z) { return v[ind(x, y, z)]; }
+    T* p(int i) { return v.data() + i; }
+    int ln(int a) const { return a == 0 ? lx : (a == 1 ? ly : lz); }
+};
+using DGrid = Grid<double>;
+using IGrid = Grid<int>;
+
+struct Run {
+    std::array<int, 6> up;   // len, ind, ind(i-off), ind(j+), ind(k+), obj
+    std::array<double, 4> pref;  // 1, -dt/(eps dj), -dt/(eps dk), eps
+};
+
+static bool isE(Pol p) { return p == Pol::EX || p == Pol::EY || p == Pol::EZ; }
+
+// ------------------------------------------------------------------ CPML ----
+struct PsiParam {
+    int transSz, stride, ind, indOff;
+    double b, c;
+};
+struct GridParam {
+    int nAx, stride, ind, indOff;
+    double Db, DbField;
+};
+
+struct CPML {
+    Pol pol;
+    int i_, j_, k_;
+    double m, ma, sigmaMax[3], kappaMax, aMax, dt;
+    std::array<int, 3> nvec;  // PML thickness
+    std::array<int, 3> lnmn{{0, 0, 0}}, lnpl{{0, 0, 0}};
+    std::array<double, 3> d;
+    std::vector<double> eta[3][2];
+    DGrid *gi = nullptr, *gj = nullptr, *gk = nullptr;
+    DGrid psi_j, psi_k;
+    std::vector<PsiParam> psiJ, psiK;
+    std::vector<GridParam> gridK, gridJ;
+
+    double bVal(double sig, double a, double kap) const { return std::exp(-1.0 * (sig / kap + a) * dt); }
+    double cVal(double sig, double a, double kap) const {
+        return (sig == 0 && a == 0) ? 0 : sig * (bVal(sig, a, kap) - 1.0) / (kap * (sig + kap * a));
+    }
+    double aValF(double ii, double iiMax) const { return (0.0 <= ii && ii <= iiMax) ? aMax * std::pow(ii / iiMax, ma) : 0.0; }
+    double kappa(double ii, double iiMax) const {
+        return (0.0 <= ii && ii <= iiMax) ? 1.0 + (kappaMax - 1.0) * std::pow((iiMax - ii) / iiMax, m) : 1.0;
+    }
+    double sigma(double ii, double iiMax, double eta_eff, double sMax) const {
+        return (0.0 <= ii && ii <= iiMax) ? sMax / eta_eff * std::pow((iiMax - ii) / iiMax, m) : 0.0;
+    }
+
+    // genEtaEffVal/genEtaEff, src/PML/parallelPML.hpp:192-275.  The plane
+    // average uses the eps_infty of this component for anisotropic objects.
+    void genEtaEff(IGrid& phys, int normDir, bool pl, const std::vector<std::shared_ptr<Object>>& objs, int comp, bool Hfield,
+                   std::vector<double>& out) {
+        std::array<int, 3> ps = {{gi->lx - 2, gi->ly - 2, gi->lz - 2}};
+        if (pol == Pol::EX || pol == Pol::HY || pol == Pol::HZ) ps[0] -= 1;
+        if (pol == Pol::EY || pol == Pol::HX || pol == Pol::HZ) ps[1] -= 1;
+        if (pol == Pol::EZ || pol == Pol::HX || pol == Pol::HY) ps[2] -= 1;
+        if (gi->lz == 1) ps[2] = 1;
+        int n = phys.ln(normDir) - 2;
+        int mn = pl ? n - nvec[normDir] : 0;
+        int mx = pl ? n : nvec[normDir];
+        int z3 = (phys.lz != 1) ? 1 : 0;
+        for (int ii = mn; ii < mx; ++ii) {
+            std::vector<int> vals;
+            if (normDir == 0) {  // getPlaneYZ: sz = {z, y}
+                int ySz = std::min(ps[1], phys.ly - 2);
+                for (int yy = 0; yy < ySz; ++yy)
+                    for (int zz = 0; zz < ps[2]; ++zz) vals.push_back(phys.at(ii + 1, yy + 1, zz + z3));
+                vals.resize(static_cast<size_t>(ps[2]) * ps[1], 0);
+            } else if (normDir == 1) {  // getPlaneXZ: sz = {x, z}
+                for (int zz = 0; zz < ps[2]; ++zz)
+                    for (int xx = 0; xx < ps[0]; ++xx) vals.push_back(phys.at(xx + 1, ii + 1, zz + z3));
+            } else {  // getPlaneXY: sz = {x, y}
+                int ySz = std::min(ps[1], phys.ly - 2);
+                for (int yy = 0; yy < ySz; ++yy)
+                    for (int xx = 0; xx < ps[0]; ++xx) vals.push_back(phys.at(xx + 1, yy + 1, ii + z3));
+                vals.resize(static_cast<size_t>(ps[0]) * ps[1], 0);
+            }
+            double es = 0.0, ms = 0.0;
+            for (int v : vals) {
+                if (v != -1) {
+                    es += objs[v]->eps[comp] / static_cast<double>(vals.size());
+                    ms += objs[v]->mu / static_cast<double>(vals.size());
+                }
+            }
+            double etaVal = es * ms;
+            etaVal = (etaVal != 0.0) ? etaVal : 0;
+            (void)Hfield;
+            if (static_cast<int>(out.size()) < nvec[normDir] && etaVal != 0.0) out.push_back(etaVal);
+        }
+    }
+
+    // getPsiUpList, src/PML/parallelPML.hpp:321-451.
+    void getPsiUpList(int dir, bool pl, int startPt, int nDir, int dirMax, std::vector<PsiParam>& list) {
+        PsiParam param{};
+        bool E = isE(pol);
+        int transSz2 = 1, cor_norm, cor_t1, cor_t2, t1Off = 0, t2Off = 0, ccStart = 0;
+        std::vector<double>* eta_eff;
+        double sMax;
+        if (dir == 0) {
+            sMax = sigmaMax[0];
+            cor_norm = 0;
+            cor_t1 = (gi->lz == 1) ? 1 : 2;
+            cor_t2 = (gi->lz == 1) ? 2 : 1;
+            param.stride = gi->lx;
+            eta_eff = &eta[0][pl ? 1 : 0];
+            if (pol == Pol::EY || pol == Pol::HX || pol == Pol::HZ) (gi->lz == 1) ? t1Off = 1 : t2Off = 1;
+            if (pol == Pol::HY || pol == Pol::HX || pol == Pol::EZ) (gi->lz == 1) ? t2Off = 1 : t1Off = 1;
+            if (pol == Pol::HZ || pol == Pol::HY || pol == Pol::EX) pl ? ccStart = 1 : dirMax -= 1;
+        } else if (dir == 1) {
+            sMax = sigmaMax[1];
+            cor_norm = 1;
+            cor_t1 = 0;
+            cor_t2 = 2;
+            param.stride = 1;
+            eta_eff = &eta[1][pl ? 1 : 0];
+            if (pol == Pol::EX || pol == Pol::HY || pol == Pol::HZ) t1Off = 1;
+            if (pol == Pol::HX || pol == Pol::HY || pol == Pol::EZ) t2Off = 1;
+            if (pl && (pol == Pol::HZ || pol == Pol::HX || pol == Pol::EY)) ccStart = 1;
+            if (!pl && startPt + dirMax == nDir && (pol == Pol::HZ || pol == Pol::HX || pol == Pol::EY)) dirMax -= 1;
+        } else {
+            sMax = sigmaMax[2];
+            cor_norm = 2;
+            cor_t1 = 0;
+            cor_t2 = 1;
+            param.stride = 1;
+            eta_eff = &eta[2][pl ? 1 : 0];
+            if (pol == Pol::EX || pol == Pol::HY || pol == Pol::HZ) t1Off = 1;
+            if (pol == Pol::HX || pol == Pol::HZ || pol == Pol::EY) t2Off = 1;
+            if (pol == Pol::HX || pol == Pol::HY || pol == Pol::EZ) pl ? ccStart = 1 : dirMax -= 1;
+        }
+        param.transSz = gi->ln(cor_t1) - 2 - t1Off;
+        transSz2 = gi->ln(cor_t2) - 2 - t2Off;
+        if (gi->lz == 1 && cor_t2 == 2) transSz2 = 1;
+        double distOff = E ? 0.0 : 0.5;
+        std::array<int, 3> loc = {{-1, -1, -1}};
+        loc[cor_t1] = 1;
+        for (int cc = ccStart; cc < dirMax; ++cc) {
+            loc[cor_norm] = pl ? gi->ln(cor_norm) - 2 - cc : cc + 1;
+            double dist = (loc[cor_norm] - 1) + distOff;
+            if (pl) dist = (gi->ln(cor_norm) - 2) - 1 - dist;
+            double sig = sigma(dist, static_cast<double>(nDir - 1), eta_eff->at(static_cast<size_t>(dist)), sMax);
+            double kap = kappa(dist, static_cast<double>(nDir - 1));
+            double a = aValF(dist, static_cast<double>(nDir - 1));
+            param.b = bVal(sig, a, kap);
+            param.c = cVal(sig, a, kap) / d[cor_norm];
+            if (!E) param.c *= -1.0;
+            for (int jj = 0; jj < transSz2; ++jj) {
+                loc[cor_t2] = 1 + jj;
+                if (gi->lz == 1) loc[2] = 0;
+                std::array<int, 3> lo = loc;
+                lo[cor_norm] += E ? -1 : 1;
+                param.ind = gi->ind(loc[0], loc[1], loc[2]);
+                param.indOff = gi->ind(lo[0], lo[1], lo[2]);
+                list.push_back(param);
+            }
+        }
+    }
+
+    // getGridUpList + getAxLists, src/PML/parallelPML.hpp:462-656.
+    void getGridUpList(int dir, int derivDir, bool pl, DGrid* grid, std::vector<GridParam>& list) {
+        GridParam param{};
+        std::array<int, 3> mn = {{0, 0, 0}}, mx = {{0, 0, 0}};
+        int cor_ii = 0, cor_jj = 0, cor_kk = 0, off_ii = 0, cd_ii = 0;
+        param.Db = dt;
+        double DbFieldBase = dt;
+        std::array<int, 3> offset = {{0, 0, 0}};
+        bool E = isE(pol);
+        if (dir == 0) {
+            cor_ii = 0; cor_jj = 1; cor_kk = 2;
+            param.stride = gi->lx;
+            if (pl && (pol == Pol::HZ || pol == Pol::EX || pol == Pol::HY)) off_ii = -1;
+            else if (!pl) off_ii = 1;
+        } else if (dir == 1) {
+            cor_ii = 1; cor_jj = 2; cor_kk = 0;
+            param.stride = 1;
+            if (pl && (pol == Pol::HZ || pol == Pol::HX || pol == Pol::EY)) off_ii = -1;
+            if (!pl) off_ii = 1;
+        } else {
+            cor_ii = 2; cor_jj = 0; cor_kk = 1;
+            param.stride = 1;
+            if (pl && (pol == Pol::EZ || pol == Pol::HX || pol == Pol::HY)) off_ii = -1;
+            else if (!pl) off_ii = 1;
+        }
+        if (derivDir == 0) {
+            cd_ii = 0;
+            if (pol == Pol::HZ || pol == Pol::EY) param.Db *= -1.0;
+            if (pol == Pol::HY || pol == Pol::EY) DbFieldBase *= -1.0;
+        } else if (derivDir == 1) {
+            cd_ii = 1;
+            if (pol == Pol::HX || pol == Pol::EZ) param.Db *= -1.0;
+            if (pol == Pol::HZ || pol == Pol::EZ) DbFieldBase *= -1.0;
+        } else {
+            cd_ii = 2;
+            if (pol == Pol::HY || pol == Pol::EX) param.Db *= -1.0;
+            if (pol == Pol::HX || pol == Pol::EX) DbFieldBase *= -1.0;
+        }
+        DbFieldBase /= d[cd_ii];
+        offset[cd_ii] = E ? -1 : 1;
+        mn[cor_ii] += pl ? gi->ln(cor_ii) - lnpl[cor_ii] - 1 : 1;
+        mx[cor_ii] += off_ii + (pl ? gi->ln(cor_ii) - 1 : lnmn[cor_ii]);
+        mn[cor_jj] = 1;
+        mx[cor_jj] = gi->ln(cor_jj) - 1;
+        mn[cor_kk] = 1;
+        mx[cor_kk] = gi->ln(cor_kk) - 1;
+        // one rank: "cor == 1 && rank != size-1" is false
+        if (dir == i_) {
+            mn[cor_jj] += lnmn[cor_jj];
+            mx[cor_jj] -= lnpl[cor_jj] - (E ? 0 : 1);
+            mn[cor_kk] += lnmn[cor_kk];
+            mx[cor_kk] -= lnpl[cor_kk] - (E ? 0 : 1);
+        } else if (cd_ii != cor_ii) {
+            mn[cd_ii] += lnmn[cd_ii];
+            mx[cd_ii] -= lnpl[cd_ii] - (E ? 0 : 1);
+        }
+        if (cor_ii != 0 && (pol == Pol::HZ || pol == Pol::EX || pol == Pol::HY)) mx[0] -= 1;
+        if (cor_ii != 1 && (pol == Pol::HZ || pol == Pol::HX || pol == Pol::EY)) mx[1] -= 1;
+        if (cor_ii != 2 && gi->lz != 1 && (pol == Pol::HX || pol == Pol::HY || pol == Pol::EZ)) mx[2] -= 1;
+        if (gi->lz == 1) {
+            mn[2] = 0;
+            mx[2] = 1;
+        }
+        // getAxLists
+        std::vector<std::array<int, 4>> ax;
+        if (param.stride == 1) {
+            for (int kk = mn[2]; kk < mx[2]; ++kk)
+                for (int jj = mn[1]; jj < mx[1]; ++jj) ax.push_back({{mn[0], jj, kk, mx[0] - mn[0]}});
+        } else if (gi->lz == 1) {
+            for (int kk = mn[2]; kk < mx[2]; ++kk)
+                for (int ii = mn[0]; ii < mx[0]; ++ii) ax.push_back({{ii, mn[1], kk, mx[1] - mn[1]}});
+        } else {
+            for (int jj = mn[1]; jj < mx[1]; ++jj)
+                for (int ii = mn[0]; ii < mx[0]; ++ii) ax.push_back({{ii, jj, mn[2], mx[2] - mn[2]}});
+        }
+        double distOff = E ? 0.0 : 0.5;
+        for (auto& a : ax) {
+            double dist = a[cd_ii] - 1 + distOff;
+            if (pl) dist = (gi->ln(cd_ii) - 2) - 1 - dist;
+            double kap = kappa(dist, static_cast<double>(nvec[cd_ii] - 1));
+            param.DbField = DbFieldBase / kap;
+            param.ind = grid->ind(a[0], a[1], a[2]);
+            param.indOff = grid->ind(a[0] + offset[0], a[1] + offset[1], a[2] + offset[2]);
+            param.nAx = a[3];
+            list.push_back(param);
+        }
+    }
+
+    void initalizeLists(int ln_pml, int dir, bool pl, int startPt, int nDir, int dirMax) {
+        if (ln_pml == 0) return;
+        if (gk) {
+            if (dir == j_) getPsiUpList(dir, pl, startPt, nDir, dirMax, psiJ);
+            getGridUpList(dir, j_, pl, gk, gridK);
+        }
+        if (gj) {
+            if (dir == k_) getPsiUpList(dir, pl, startPt, nDir, dirMax, psiK);
+            getGridUpList(dir, k_, pl, gj, gridJ);
+        }
+    }
+
+    CPML(DGrid* grid_i, DGrid* grid_j, DGrid* grid_k, Pol p, const ProgramInputs& IP, double dt_, bool matInPML,
+         IGrid& phys, const std::vector<std::shared_ptr<Object>>& objs, int comp)
+        : pol(p), m(IP.pmlM), ma(IP.pmlMa), kappaMax(IP.pmlKappaMax), aMax(IP.pmlAMax), dt(dt_), nvec(IP.pmlThickness), d(IP.d),
+          gi(grid_i), gj(grid_j), gk(grid_k) {
+        (void)matInPML;
+        for (int a = 0; a < 3; ++a) sigmaMax[a] = IP.pmlSigOptRat * 0.8 * (m + 1) / d[a];
+        if (p == Pol::HX || p == Pol::EX) { i_ = 0; j_ = 1; k_ = 2; }
+        else if (p == Pol::HY || p == Pol::EY) { i_ = 1; j_ = 2; k_ = 0; }
+        else { i_ = 2; j_ = 0; k_ = 1; }
+        if (gk && (gi->lz != 1 || j_ != 2)) psi_j.init(gk->lx, gk->ly, gk->lz);
+        if (gj && (gi->lz != 1 || k_ != 2)) psi_k.init(gj->lx, gj->ly, gj->lz);
+        // findLnVecs, parallelPML.hpp:280-308 (procLoc = 0)
+        for (int a = 0; a < 3; ++a) {
+            if (a == 2 && gi->lz == 1) break;
+            int nl = gi->ln(a) - 2;
+            if (0 < nvec[a]) lnmn[a] = (nl < nvec[a]) ? nl : nvec[a];
+            if (nl > nl - nvec[a]) lnpl[a] = (0 > nl - nvec[a]) ? nl : nl - (nl - nvec[a]);
+        }
+        bool Hf = !isE(p);
+        genEtaEff(phys, 0, false, objs, comp, Hf, eta[0][0]);
+        genEtaEff(phys, 0, true, objs, comp, Hf, eta[0][1]);
+        genEtaEff(phys, 1, false, objs, comp, Hf, eta[1][0]);
+        genEtaEff(phys, 1, true, objs, comp, Hf, eta[1][1]);
+        if (gi->lz != 1) {
+            genEtaEff(phys, 2, false, objs, comp, Hf, eta[2][0]);
+            genEtaEff(phys, 2, true, objs, comp, Hf, eta[2][1]);
+        }
+        initalizeLists(lnmn[0], 0, false, 0, nvec[0], lnmn[0]);
+        initalizeLists(lnpl[0], 0, true, gi->lx - 2, nvec[0], lnpl[0]);
+        initalizeLists(lnmn[1], 1, false, 0, nvec[1], lnmn[1]);
+        initalizeLists(lnpl[1], 1, true, gi->ly - 2, nvec[1], lnpl[1]);
+        initalizeLists(lnmn[2], 2, false, 0, nvec[2], lnmn[2]);
+        initalizeLists(lnpl[2], 2, true, gi->lz - 2, nvec[2], lnpl[2]);
+    }
+
+    // addPsi / addGridOnly / updatePsiField, src/PML/parallelPML.cpp:12-40.
+    static void updatePsiField(const std::vector<PsiParam>& l, DGrid& psi, DGrid& grid) {
+        for (auto& q : l) {
+            dscal(q.transSz, q.b, psi.p(q.ind), q.stride);
+            daxpy(q.transSz, q.c, grid.p(q.ind), q.stride, psi.p(q.ind), q.stride);
+            daxpy(q.transSz, -1.0 * q.c, grid.p(q.indOff), q.stride, psi.p(q.ind), q.stride);
+        }
+    }
+    static void addPsi(const std::vector<GridParam>& gl, const std::vector<PsiParam>& pl, DGrid& gi, DGrid& psi, DGrid& grid) {
+        updatePsiField(pl, psi, grid);
+        for (auto& q : gl) {
+            daxpy(q.nAx, q.DbField, grid.p(q.ind), q.stride, gi.p(q.ind), q.stride);
+            daxpy(q.nAx, -1.0 * q.DbField, grid.p(q.indOff), q.stride, gi.p(q.ind), q.stride);
+            daxpy(q.nAx, q.Db, psi.p(q.ind), q.stride, gi.p(q.ind), q.stride);
+        }
+    }
+    static void addGridOnly(const std::vector<GridParam>& gl, DGrid& gi, DGrid& grid) {
+        for (auto& q : gl) {
+            daxpy(q.nAx, q.DbField, grid.p(q.ind), q.stride, gi.p(q.ind), q.stride);
+            daxpy(q.nAx, -1.0 * q.DbField, grid.p(q.indOff), q.stride, gi.p(q.ind), q.stride);
+        }
+    }
+    void updateGrid() {
+        if (psi_j.ok()) addPsi(gridK, psiJ, *gi, psi_j, *gk);
+        else if (gk) addGridOnly(gridK, *gi, *gk);
+        if (psi_k.ok()) addPsi(gridJ, psiK, *gi, psi_k, *gj);
+        else if (gj) addGridOnly(gridJ, *gi, *gj);
+    }
+};
+
+// --------------------------------------------------------------- sources ----
+struct Source {
+    std::vector<PulseSpec> pulses;
+    DGrid* grid = nullptr;
+    bool active = false;
+    std::array<int, 3> loc, sz, add1, add2;
+    int stride = 1;
+    std::vector<double> pulVec;
+};
+
+// -------------------------------------------------------------- detectors ---
+struct PointDetector {
+    DtcType type;
+    int timeInt;
+    double tConv = 1.0, conv = 1.0;
+    std::vector<std::pair<DGrid*, std::array<int, 3>>> fields;
+    std::array<int, 3> loc, sz;
+    std::vector<double> times;
+    std::vector<double> values;  // per sample: sz product values
+};
+
+// Frequency-domain storage box (parallelStorageFreqDTC): F[f, p] accumulates
+// Re/Im(exp(-i w t)) * U(p) for the points of a box.
+struct FreqStore {
+    DGrid* grid;
+    std::array<int, 3> loc, sz;  // global point coords (may start at -1)
+    std::array<int, 3> lloc, lsz;
+    std::vector<double> fre, fim;  // [point][f]
+};
+
+struct FluxFace {
+    int dir;
+    bool pl;
+    double weight;
+    int cor, transCor1, transCor2, corI, corJ, corK;
+    std::array<int, 3> sz;
+    std::vector<FreqStore> Ej, Ek, Hj, Hk;
+    int padEj = -1, padEk = -1, padHj = -1, padHk = -1;  // averaged direction (-1: none)
+};
+
+struct Flux {
+    double weight;
+    int timeInt;
+    std::vector<double> freqs;
+    std::array<int, 3> loc, sz;
+    std::vector<FluxFace> faces;
+    int t_step = 0;
+    double fluxConv = 1.0;
+};
+
+// -------------------------------------------------------------- emitters ----
+struct QEDensity {
+    std::array<int, 3> loc;  // box-relative location
+    std::vector<cplx> rho, f0, f1, f2, f3;  // density and derivative history n, n-1, n-2, n-3
+};
+
+struct QE {
+    int N;
+    double na, hbar;
+    std::vector<std::vector<std::pair<int, double>>> gam;  // sparse rows
+    struct Member {
+        std::vector<cplx> h0, mu[3], negmu[3];
+        bool use[3];
+        std::vector<QEDensity> den;
+    };
+    std::vector<Member> members;
+    std::array<int, 3> mn, nbox;  // box min (points) and size
+    int zOff;
+    DGrid P[3];  // (n+2, n+2, n+1+zOff)
+    std::vector<cplx> e[3];  // per box point
+    std::vector<int> dtcLevs;
+    int popTimeInt;
+    std::vector<std::vector<cplx>> pops;  // per level
+    std::vector<cplx> curPop;
+    int pop_t_step = 0;
+    int npoints;
+};
+
+// ------------------------------------------------------------------- sim ----
+struct Sim {
+    ProgramInputs IP;
+    std::array<int, 3> n;  // points per axis (n_vec_)
+    int lx, ly, lz;
+    double dt, tcur = 0.0;
+    int t_step = 0;
+    bool is3D;
+    DGrid E[3], H[3], D[3];
+    IGrid physE[3], physH[3], physNode;
+    DGrid epsRel[3], muRel[3], epsNode;
+    bool dielectricMatInPML = false, magMatInPML = false;
+    std::vector<Run> upE[3], upH[3], upD[3], upLorD[3];
+    std::vector<DGrid> lorP[3], prevLorP[3];
+    std::unique_ptr<CPML> EPML[3], HPML[3];
+    std::vector<Source> srcs;
+    std::vector<PointDetector> dtcs;
+    std::vector<Flux> fluxes;
+    std::vector<QE> qes;
+    std::vector<double> scratch;
+
+    explicit Sim(const std::string& json);
+    void setupPhysFields(IGrid& phys, DGrid& epMu, bool Efield, std::array<double, 3> offPt, std::array<int, 3> endOff, int comp);
+    void detectMatInPML();
+    void fillBlasLists(std::array<int, 3> mn, std::array<int, 3> mx, IGrid& phys, DGrid& epMu, bool includeU, bool Efield,
+                       std::vector<std::array<int, 5>>& upU, std::vector<std::array<int, 5>>& upDl);
+    void getBlasLists(bool Efield, std::array<int, 3> mn, std::array<int, 3> mx, IGrid& phys, DGrid& epMu, CPML& pml,
+                      std::array<int, 3> fieldEnd, std::vector<std::array<int, 5>>& upU, std::vector<std::array<int, 5>>& upDl,
+                      std::vector<std::array<int, 5>>& upLorDl);
+    void populate(const std::vector<std::array<int, 5>>& bl, std::array<int, 3> derivOff, DGrid& epMu, double dj, double dk,
+                  bool upUList, std::vector<Run>& out);
+    void initializeList(IGrid& phys, DGrid& epMu, CPML& pml, bool Efield, std::array<int, 3> derivOff, std::array<int, 3> fieldEnd,
+                        double dj, double dk, std::vector<Run>& upU, std::vector<Run>& upDl, std::vector<Run>& upLorDl);
+    void buildSources();
+    void buildDetectors();
+    void buildFlux();
+    void buildQE();
+    void step();
+    void outputDetectors();
+    void fluxFieldIn();
+    void addQE();
+    void updateDensity(QE& q);
+    void PCABAM4(QE& q, QE::Member& mem, QEDensity& den, std::vector<cplx>& H);
+    void denDeriv(QE& q, const std::vector<cplx>& H, const cplx* den, cplx* out);
+    void getHam(QE::Member& mem, const cplx Ex, const cplx Ey, const cplx Ez, std::vector<cplx>& H);
+    std::vector<cplx> getFlux(int fi);
+};
+
+// setupPhysFields, src/FDTD_MANAGER/parallelFDTDField.hpp:867-947.
+void Sim::setupPhysFields(IGrid& phys, DGrid& epMu, bool Efield, std::array<double, 3> offPt, std::array<int, 3> endOff, int comp) {
+    int nz = is3D ? n[2] + 2 : 1;
+    phys.init(n[0] + 2, n[1] + 2, nz, 0);
+    epMu.init(n[0] + 2, n[1] + 2, nz, 1.0);
+    int zmin = (nz == 1) ? 0 : 1;
+    int zmax = (nz == 1) ? 1 : phys.lz - 1;
+    auto& objs = IP.objs;
+    for (size_t oo = 1; oo < objs.size(); ++oo) {
+        const Object& o = *objs[oo];
+        bool relevant = o.ML || !o.pols.empty() || o.anyEpsNotOne() || o.mu != 1.0;
+        if (!relevant) continue;
+        for (int ii = 1; ii < phys.lx - 1; ++ii)
+            for (int jj = 1; jj < phys.ly - 1; ++jj)
+                for (int kk = zmin; kk < zmax; ++kk) {
+                    std::array<double, 3> pt = {{((ii - 1) + offPt[0] - (n[0] - n[0] % 2) / 2.0) * IP.d[0],
+                                                 ((jj - 1) + offPt[1] - (n[1] - n[1] % 2) / 2.0) * IP.d[1],
+                                                 ((kk - 1) + offPt[2] - (n[2] - n[2] % 2) / 2.0) * IP.d[2]}};
+                    if (o.isObj(pt, IP.d[0], true)) phys.at(ii, jj, kk) = static_cast<int>(oo);
+                    if (o.isObj(pt, IP.d[0], false)) epMu.at(ii, jj, kk) = Efield ? o.eps[comp] : o.mu;
+                }
+    }
+    zmin = 0;
+    zmax = (nz == 1) ? 1 : phys.lz;
+    if (nz != 1) {
+        for (int ii = 0; ii < phys.lx; ++ii)
+            for (int jj = 0; jj < phys.ly; ++jj) {
+                phys.at(ii, jj, 0) = -1;
+                phys.at(ii, jj, zmax - 1) = -1;
+                if (endOff[2] == 1) {
+                    phys.at(ii, jj, zmax - 2) = -1;
+                    epMu.at(ii, jj, zmax - 2) = 0.0;
+                }
+            }
+    }
+    for (int ii = 0; ii < phys.lx; ++ii)
+        for (int kk = zmin; kk < zmax; ++kk) {
+            phys.at(ii, 0, kk) = -1;
+            phys.at(ii, phys.ly - 1, kk) = -1;
+            if (endOff[1] == 1 && phys.ly > 3) {
+                phys.at(ii, phys.ly - 2, kk) = -1;
+                epMu.at(ii, phys.ly - 2, kk) = 0.0;
+            }
+        }
+    for (int kk = zmin; kk < zmax; ++kk)
+        for (int jj = 0; jj < phys.ly; ++jj) {
+            phys.at(0, jj, kk) = -1;
+            phys.at(phys.lx - 1, jj, kk) = -1;
+            if (endOff[0] == 1 && phys.lx > 3) {
+                phys.at(phys.lx - 2, jj, kk) = -1;
+                epMu.at(phys.lx - 2, jj, kk) = 0.0;
+            }
+        }
+}
+
+// Material-in-PML scan, src/FDTD_MANAGER/parallelFDTDField.hpp:274-370,
+// restated with its loop bounds (the lower-side range is replaced by the
+// upper-side range after the first (jj, kk) pair, as in the reference).
+void Sim::detectMatInPML() {
+    const auto& d = IP.d;
+    for (int pp = 0; pp < 3; ++pp) {
+        int ci = pp, cj = (pp + 1) % 3, ck = (pp + 2) % 3;
+        std::array<int, 3> mn, mx;
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = static_cast<int>(-1 * static_cast<double>(n[a]) / 2);
+            mx[a] = static_cast<int>(static_cast<double>(n[a]) / 2);
+        }
+        mx[ci] = static_cast<int>(IP.pmlThickness[ci] - n[ci] / 2.0);
+        for (auto& objp : IP.objs) {
+            const Object& o = *objp;
+            if (!o.anyEpsNotOne() && o.mu == 1.0 && o.pols.empty() && !o.ML) continue;
+            bool dielc = !(!o.anyEpsNotOne() && o.pols.empty());
+            bool mag = !(o.mu == 1.0);
+            for (int jj = mn[cj]; jj < mx[cj]; ++jj)
+                for (int kk = mn[ck]; kk < mx[ck]; ++kk) {
+                    for (int pass = 0; pass < 2; ++pass) {
+                        if (pass == 1) {
+                            mn[ci] = n[ci] / 2 - IP.pmlThickness[ci];
+                            mx[ci] = n[ci] / 2;
+                        }
+                        for (int ii = mn[ci]; ii < mx[ci]; ++ii) {
+                            std::array<double, 3> pt = {{0, 0, 0}};
+                            pt[ci] = static_cast<double>(ii) * d[ci];
+                            pt[cj] = static_cast<double>(jj) * d[cj];
+                            pt[ck] = static_cast<double>(kk) * d[ck];
+                            pt[0] += d[0] / 2.0;
+                            if (!dielectricMatInPML && o.isObj(pt, d[0], false)) dielectricMatInPML = dielc;
+                            pt[1] += d[1] / 2.0;
+                            if (!magMatInPML && o.isObj(pt, d[0], false)) magMatInPML = mag;
+                            pt[0] -= d[0] / 2.0;
+                            if (!dielectricMatInPML && o.isObj(pt, d[0], false)) dielectricMatInPML = dielc;
+                            pt[2] += d[2] / 2.0;
+                            if (!magMatInPML && o.isObj(pt, d[0], false)) magMatInPML = mag;
+                            pt[1] -= d[0] / 2.0;
+                            if (!dielectricMatInPML && o.isObj(pt, d[0], false)) dielectricMatInPML = dielc;
+                            pt[0] += d[2] / 2.0;
+                            if (!magMatInPML && o.isObj(pt, d[0], false)) magMatInPML = mag;
+                        }
+                    }
+                }
+        }
+    }
+}
+
+// fillBlasLists, parallelFDTDField.hpp:751-785 (isotropic poles only).
+void Sim::fillBlasLists(std::array<int, 3> mn, std::array<int, 3> mx, IGrid& phys, DGrid& epMu, bool includeU, bool Efield,
+                        std::vector<std::array<int, 5>>& upU, std::vector<std::array<int, 5>>& upDl) {
+    for (int jj = mn[1]; jj < mx[1]; ++jj)
+        for (int kk = mn[2]; kk < mx[2]; ++kk) {
+            int ii = mn[0];
+            while (ii < mx[0]) {
+                int st = ii;
+                while ((ii < mx[0] - 1) && (phys.at(ii, jj, kk) == phys.at(ii + 1, jj, kk)) && (epMu.at(ii, jj, kk) == epMu.at(ii + 1, jj, kk)))
+                    ++ii;
+                const Object& o = *IP.objs[phys.at(st, jj, kk)];
+                if (includeU && ((!Efield) || (Efield && o.gamma.size() < 1 && !o.ML)))
+                    upU.push_back({{st, jj, kk, ii - st + 1, phys.at(st, jj, kk)}});
+                else
+                    upDl.push_back({{st, jj, kk, ii - st + 1, phys.at(st, jj, kk)}});
+                ++ii;
+            }
+        }
+}
+
+// getBlasLists, parallelFDTDField.hpp:802-856.
+void Sim::getBlasLists(bool Efield, std::array<int, 3> mn, std::array<int, 3> mx, IGrid& phys, DGrid& epMu, CPML& pml,
+                       std::array<int, 3> fe, std::vector<std::array<int, 5>>& upU, std::vector<std::array<int, 5>>& upDl,
+                       std::vector<std::array<int, 5>>& upLorDl) {
+    std::vector<std::array<int, 5>> notInc;
+    bool mat = (Efield && dielectricMatInPML) || (!Efield && magMatInPML);
+    int xl = mat ? pml.lnmn[0] : 0, xr = mat ? pml.lnpl[0] : 0;
+    if (xr != 0) xr -= fe[0];
+    int yb = mat ? pml.lnmn[1] : 0, yt = mat ? pml.lnpl[1] : 0;
+    if (yt != 0) yt -= fe[1];
+    bool z3 = is3D;
+    int zb = (z3 && mat) ? pml.lnpl[2] : 0, zf = (z3 && mat) ? pml.lnmn[2] : 0;
+    if (zf != 0) zf -= fe[2];
+    bool inc = !(dielectricMatInPML || magMatInPML);
+    fillBlasLists({{mn[0], mn[1], mn[2]}}, {{mx[0], mn[1] + yb, mx[2]}}, phys, epMu, inc, Efield, notInc, upLorDl);
+    fillBlasLists({{mn[0], mx[1] - yt, mn[2]}}, {{mx[0], mx[1], mx[2]}}, phys, epMu, inc, Efield, notInc, upLorDl);
+    fillBlasLists({{mn[0], mn[1] + yb, mn[2]}}, {{mx[0], mx[1] - yt, mn[2] + zb}}, phys, epMu, inc, Efield, notInc, upLorDl);
+    fillBlasLists({{mn[0], mn[1] + yb, mx[2] - zf}}, {{mx[0], mx[1] - yt, mx[2]}}, phys, epMu, inc, Efield, notInc, upLorDl);
+    fillBlasLists({{mn[0], mn[1] + yb, mn[2] + zb}}, {{mn[0] + xl, mx[1] - yt, mx[2] - zf}}, phys, epMu, inc, Efield, notInc, upLorDl);
+    fillBlasLists({{mx[0] - xr, mn[1] + yb, mn[2] + zb}}, {{mx[0], mx[1] - yt, mx[2] - zf}}, phys, epMu, inc, Efield, notInc, upLorDl);
+    fillBlasLists({{mn[0] + xl, mn[1] + yb, mn[2] + zb}}, {{mx[0] - xr, mx[1] - yt, mx[2] - zf}}, phys, epMu, true, Efield, notInc, upLorDl);
+    mn[0] += pml.lnmn[0];
+    mx[0] -= pml.lnpl[0];
+    if (pml.lnpl[0] != 0) mx[0] += fe[0];
+    mn[1] += pml.lnmn[1];
+    mx[1] -= pml.lnpl[1];
+    if (pml.lnpl[1] != 0) mx[1] += fe[1];
+    mn[2] += pml.lnpl[2];  // lnz_back() returns the upper-side thickness
+    mx[2] -= pml.lnmn[2];  // lnz_front() returns the lower-side thickness
+    if (pml.lnmn[2] != 0) mx[2] += fe[2];
+    fillBlasLists(mn, mx, phys, epMu, true, Efield, upU, upDl);
+}
+
+// populateUpLists, parallelFDTDField.hpp:628-649.
+void Sim::populate(const std::vector<std::array<int, 5>>& bl, std::array<int, 3> dO, DGrid& epMu, double dj, double dk, bool upUList,
+                   std::vector<Run>& out) {
+    for (auto& up : bl) {
+        double ep = upUList ? epMu.at(up[0], up[1], up[2]) : 1.0;
+        Run r;
+        if (epMu.lz == 1)
+            r.up = {{up[3], epMu.ind(up[0], up[1], up[2]), epMu.ind(up[0] - dO[1], up[1] - dO[2], up[2]),
+                     epMu.ind(up[0] + dO[0], up[1] + dO[1], up[2]), epMu.ind(up[0] + dO[2], up[1] + dO[0], up[2]), up[4]}};
+        else
+            r.up = {{up[3], epMu.ind(up[0], up[1], up[2]), epMu.ind(up[0] - dO[1], up[1] - dO[2], up[2] - dO[0]),
+                     epMu.ind(up[0] + dO[0], up[1] + dO[1], up[2] + dO[2]), epMu.ind(up[0] + dO[2], up[1] + dO[0], up[2] + dO[1]), up[4]}};
+        r.pref = {{1.0, -1.0 * dt / (ep * dj), -1.0 * dt / (ep * dk), epMu.at(up[0], up[1], up[2])}};
+        out.push_back(r);
+    }
+}
+
+// initializeList, parallelFDTDField.hpp:666-682.
+void Sim::initializeList(IGrid& phys, DGrid& epMu, CPML& pml, bool Efield, std::array<int, 3> dO, std::array<int, 3> fe, double dj,
+                         double dk, std::vector<Run>& upU, std::vector<Run>& upDl, std::vector<Run>& upLorDl) {
+    std::array<int, 3> mn = {{1, 1, (phys.lz == 1 ? 0 : 1)}};
+    std::array<int, 3> mx = {{phys.lx - 1 - fe[0], phys.ly - 1, (phys.lz == 1 ? 1 : phys.lz - 1 - fe[2])}};
+    mx[1] -= fe[1];
+    std::vector<std::array<int, 5>> tU, tD, tL;
+    getBlasLists(Efield, mn, mx, phys, epMu, pml, fe, tU, tD, tL);
+    populate(tU, dO, epMu, dj, dk, true, upU);
+    populate(tD, dO, epMu, dj, dk, false, upDl);
+    populate(tL, dO, epMu, dj, dk, false, upLorDl);
+}
+
+Sim::Sim(const std::string& json) : IP(parseInputs(json)) {
+    dt = IP.dt;
+    is3D = IP.is3D();
+    n = IP.nVec();
+    lx = n[0] + 2;
+    ly = n[1] + 2;
+    lz = is3D ? n[2] + 2 : 1;
+    // Material maps (parallelFDTDField.hpp:264-272).
+    setupPhysFields(physNode, epsNode, true, {{0.0, 0.0, 0.0}}, {{0, 0, 0}}, 0);
+    setupPhysFields(physE[0], epsRel[0], true, {{0.5, 0.0, 0.0}}, {{1, 0, 0}}, 0);
+    setupPhysFields(physE[1], epsRel[1], true, {{0.0, 0.5, 0.0}}, {{0, 1, 0}}, 1);
+    setupPhysFields(physE[2], epsRel[2], true, {{0.0, 0.0, 0.5}}, {{0, 0, 1}}, 2);
+    setupPhysFields(physH[0], muRel[0], false, {{0.0, 0.5, 0.5}}, {{0, 1, 1}}, 0);
+    setupPhysFields(physH[1], muRel[1], false, {{0.5, 0.0, 0.5}}, {{1, 0, 1}}, 1);
+    setupPhysFields(physH[2], muRel[2], false, {{0.5, 0.5, 0.0}}, {{1, 1, 0}}, 2);
+    detectMatInPML();
+    if (magMatInPML) throw std::logic_error("magnetic material in the PML is outside the supported time-stepping path");
+    bool disp = false;
+    for (auto& o : IP.objs)
+        if (!o->pols.empty() || o->ML || o->anyEpsNotOne()) disp = true;
+    if (dielectricMatInPML) disp = true;
+    // Field grids: 3D all six, 2D the TMz set (parallelFDTDField.hpp:391-443).
+    std::vector<int> ecomp = is3D ? std::vector<int>{0, 1, 2} : std::vector<int>{2};
+    std::vector<int> hcomp = is3D ? std::vector<int>{0, 1, 2} : std::vector<int>{0, 1};
+    for (int c : ecomp) {
+        E[c].init(lx, ly, lz);
+        if (disp) D[c].init(lx, ly, lz);
+    }
+    for (int c : hcomp) H[c].init(lx, ly, lz);
+    size_t npol = 0;
+    for (auto& o : IP.objs) npol = std::max(npol, o->gamma.size());
+    for (int c : ecomp)
+        for (size_t p = 0; p < npol; ++p) {
+            lorP[c].emplace_back();
+            lorP[c].back().init(lx, ly, lz);
+            prevLorP[c].emplace_back();
+            prevLorP[c].back().init(lx, ly, lz);
+        }
+    scratch.assign(2 * (n[0] + n[1] + n[2]), 0.0);
+    auto ptrE = [&](int c) -> DGrid* { return E[c].ok() ? &E[c] : nullptr; };
+    auto ptrH = [&](int c) -> DGrid* { return H[c].ok() ? &H[c] : nullptr; };
+    auto& d = IP.d;
+    // PMLs and update lists (parallelFDTDField.cpp:57-82, 226-250).
+    if (is3D) {
+        HPML[2] = std::make_unique<CPML>(&H[2], ptrE(0), ptrE(1), Pol::HZ, IP, dt, dielectricMatInPML, physH[2], IP.objs, 2);
+        DGrid* e0 = dielectricMatInPML ? &D[0] : &E[0];
+        DGrid* e1 = dielectricMatInPML ? &D[1] : &E[1];
+        EPML[0] = std::make_unique<CPML>(e0, ptrH(1), ptrH(2), Pol::EX, IP, dt, dielectricMatInPML, physE[0], IP.objs, 0);
+        EPML[1] = std::make_unique<CPML>(e1, ptrH(2), ptrH(0), Pol::EY, IP, dt, dielectricMatInPML, physE[1], IP.objs, 1);
+        std::vector<Run> upB2, upLorB2;  // empty without magnetic materials
+        initializeList(physH[2], muRel[2], *HPML[2], false, {{1, 0, 0}}, {{1, 1, 0}}, d[0], d[1], upH[2], upB2, upLorB2);
+        initializeList(physE[0], epsRel[0], *EPML[0], true, {{0, -1, 0}}, {{1, 0, 0}}, d[1], d[2], upE[0], upD[0], upLorD[0]);
+        initializeList(physE[1], epsRel[1], *EPML[1], true, {{0, 0, -1}}, {{0, 1, 0}}, d[2], d[0], upE[1], upD[1], upLorD[1]);
+    }
+    {
+        HPML[0] = std::make_unique<CPML>(&H[0], ptrE(1), ptrE(2), Pol::HX, IP, dt, dielectricMatInPML, physH[0], IP.objs, 0);
+        HPML[1] = std::make_unique<CPML>(&H[1], ptrE(2), ptrE(0), Pol::HY, IP, dt, dielectricMatInPML, physH[1], IP.objs, 1);
+        DGrid* e2 = dielectricMatInPML ? &D[2] : &E[2];
+        EPML[2] = std::make_unique<CPML>(e2, ptrH(0), ptrH(1), Pol::EZ, IP, dt, dielectricMatInPML, physE[2], IP.objs, 2);
+        initializeList(physE[2], epsRel[2], *EPML[2], true, {{-1, 0, 0}}, {{0, 0, 1}}, d[0], d[1], upE[2], upD[2], upLorD[2]);
+        std::vector<Run> dummy1, dummy2, dummy3, dummy4;
+        initializeList(physH[0], muRel[0], *HPML[0], false, {{0, 1, 0}}, {{0, 1, 1}}, d[1], d[2], upH[0], dummy1, dummy2);
+        initializeList(physH[1], muRel[1], *HPML[1], false, {{0, 0, 1}}, {{1, 0, 1}}, d[2], d[0], upH[1], dummy3, dummy4);
+    }
+    buildQE();
+    buildSources();
+    buildFlux();
+    buildDetectors();
+    // Detectors and flux take their t = 0 sample in the constructor
+    // (parallelFDTDField.cpp:832-837).
+    outputDetectors();
+    fluxFieldIn();
+}
+
+// Source placement, parallelSourceNormal.hpp:69-150 (one rank).
+void Sim::buildSources() {
+    for (auto& s : IP.srcs) {
+        Source src;
+        src.pulses = s.pulses;
+        int c = static_cast<int>(s.pol);
+        src.grid = (c < 3) ? &E[c] : &H[c - 3];
+        if (!src.grid->ok()) throw std::logic_error("source on a field component that does not exist");
+        std::array<int, 3> ll, lsz = {{0, 0, 0}};
+        for (int i = 0; i < 3; ++i) {
+            int lnv = src.grid->ln(i);
+            if (s.loc[i] >= 0 && s.loc[i] < lnv - 2) ll[i] = s.loc[i] + 1;
+            else if (s.loc[i] < 0 && s.loc[i] + s.sz[i] > 0) ll[i] = 1;
+            else ll[i] = -1;
+        }
+        if (src.grid->lz == 1) ll[2] = 0;
+        if (ll[0] != -1 && ll[1] != -1 && ll[2] != -1) {
+            for (int i = 0; i < 3; ++i) {
+                int lnv = src.grid->ln(i);
+                if (s.sz[i] + s.loc[i] > lnv - 2) lsz[i] = lnv - ll[i] - 1;
+                else lsz[i] = s.loc[i] + s.sz[i] - (ll[i] - 1);
+            }
+            if (src.grid->lz == 1) lsz[2] = 1;
+            src.active = true;
+            src.loc = ll;
+            if (lsz[0] >= lsz[1] && lsz[0] >= lsz[2]) {
+                src.stride = 1;
+                src.sz = {{lsz[0], lsz[1], lsz[2]}};
+                src.add1 = {{0, 1, 0}};
+                src.add2 = {{0, 0, 1}};
+            } else if (lsz[1] > lsz[2]) {
+                src.stride = src.grid->lx * src.grid->lz;
+                src.sz = {{lsz[1], lsz[0], lsz[2]}};
+                src.add1 = {{1, 0, 0}};
+                src.add2 = {{0, 0, 1}};
+            } else {
+                src.stride = src.grid->lx;
+                src.sz = {{lsz[2], lsz[0], lsz[1]}};
+                src.add1 = {{1, 0, 0}};
+                src.add2 = {{0, 1, 0}};
+            }
+            src.pulVec.assign(src.sz[0], 0.0);
+        }
+        srcs.push_back(src);
+    }
+}
+
+static DGrid* dtcField(Sim& S, DtcType t) {
+    switch (t) {
+        case DtcType::EX: return &S.E[0];
+        case DtcType::EY: return &S.E[1];
+        case DtcType::EZ: return &S.E[2];
+        case DtcType::HX: return &S.H[0];
+        case DtcType::HY: return &S.H[1];
+        case DtcType::HZ: return &S.H[2];
+        default: return nullptr;
+    }
+}
+
+// Point detectors, parallelDTC.hpp:44-95 and parallelFDTDField.cpp:690-830.
+void Sim::buildDetectors() {
+    for (auto& D : IP.dtcs) {
+        PointDetector P;
+        P.type = D.type;
+        P.timeInt = static_cast<int>(std::floor(D.timeInterval / dt));
+        if (P.timeInt == 0) throw std::logic_error("The time step of a detector is less than the main grid or set to 0.");
+        P.loc = D.loc;
+        P.sz = D.sz;
+        if (!is3D) P.sz[2] = 1;
+        if (D.SI) {
+            P.tConv = IP.a / SPEED_OF_LIGHT;
+            P.conv = IP.I0 / IP.a;
+            if (D.type == DtcType::EX || D.type == DtcType::EY || D.type == DtcType::EZ) P.conv /= EPS0 * SPEED_OF_LIGHT;
+        }
+        if (D.type == DtcType::EPOW || D.type == DtcType::HPOW) {
+            P.conv *= P.conv;
+            if (is3D && D.type == DtcType::EPOW)
+                P.fields = {{&E[0], {{-1, 0, 0}}}, {&E[1], {{0, -1, 0}}}, {&E[2], {{0, 0, -1}}}};
+            else if (is3D)
+                P.fields = {{&H[0], {{1, 0, 0}}}, {&H[1], {{0, 1, 0}}}, {&H[2], {{0, 0, 1}}}};
+            else if (D.type == DtcType::EPOW)
+                P.fields = {{&E[2], {{0, 0, 0}}}};
+            else
+                P.fields = {{&H[0], {{1, 0, 0}}}, {&H[1], {{0, 1, 0}}}};
+        } else {
+            DGrid* g = dtcField(*this, D.type);
+            if (!g || !g->ok()) throw std::logic_error("detector on a field component that does not exist");
+            P.fields = {{g, {{0, 0, 0}}}};
+        }
+        dtcs.push_back(P);
+    }
+}
+
+void Sim::outputDetectors() {
+    for (auto& P : dtcs) {
+        if (t_step % P.timeInt != 0) continue;
+        P.times.push_back(tcur * P.tConv);
+        bool pow_ = (P.type == DtcType::EPOW || P.type == DtcType::HPOW);
+        for (int jj = 0; jj < P.sz[1]; ++jj)
+            for (int kk = 0; kk < P.sz[2]; ++kk)
+                for (int ii = 0; ii < P.sz[0]; ++ii) {
+                    double point = 0.0;
+                    for (auto& f : P.fields) {
+                        DGrid& g = *f.first;
+                        int zb = is3D ? 1 : 0;
+                        double a = g.at(P.loc[0] + ii + 1, P.loc[1] + jj + 1, P.loc[2] + kk + zb);
+                        double b = g.at(P.loc[0] + ii + 1 + f.second[0], P.loc[1] + jj + 1 + f.second[1], P.loc[2] + kk + zb + f.second[2]);
+                        if (pow_) {
+                            point = point + P.conv / 2.0 * std::pow(std::abs(a), 2.0);
+                            point = point + P.conv / 2.0 * std::pow(std::abs(b), 2.0);
+                        } else {
+                            point = point + P.conv / 2.0 * a;
+                            point = point + P.conv / 2.0 * b;
+                        }
+                    }
+                    P.values.push_back(point);
+                }
+    }
+}
+
+// ------------------------------------------------------------------ flux ----
+static FreqStore makeStore(DGrid* g, std::array<int, 3> loc, std::array<int, 3> sz, int nf, bool is3D) {
+    FreqStore s;
+    s.grid = g;
+    s.loc = loc;
+    s.sz = sz;
+    for (int i = 0; i < 3; ++i) {
+        int lnv = g->ln(i);
+        int ll;
+        if (i == 2 && !is3D) {
+            s.lloc[2] = 0;
+            s.lsz[2] = 1;
+            continue;
+        }
+        if (loc[i] >= 0 && loc[i] < lnv - 2) ll = loc[i] + 1;
+        else if (loc[i] == -1) ll = 0;
+        else if (loc[i] < 0 && loc[i] + sz[i] > 0) ll = 1;
+        else throw std::logic_error("flux storage box outside the cell");
+        s.lloc[i] = ll;
+        s.lsz[i] = (sz[i] + loc[i] > lnv - 2) ? lnv - ll - 1 : loc[i] + sz[i] - (ll - 1);
+    }
+    size_t np = static_cast<size_t>(s.lsz[0]) * s.lsz[1] * s.lsz[2];
+    s.fre.assign(np * nf, 0.0);
+    s.fim.assign(np * nf, 0.0);
+    return s;
+}
+
+// makeParamIn (Real), src/DTC/parallelFlux.cpp:338-543.
+void Sim::buildFlux() {
+    for (auto& F : IP.fluxes) {
+        Flux fx;
+        fx.weight = F.weight;
+        fx.timeInt = F.timeInt;
+        fx.freqs = F.freqs;
+        fx.loc = F.loc;
+        fx.sz = F.sz;
+        if (!is3D) fx.sz[2] = 1;
+        int nf = static_cast<int>(F.freqs.size());
+        auto make = [&](int dir, bool pl) {
+            FluxFace f;
+            f.dir = dir;
+            f.pl = pl;
+            DGrid *Ej, *Ek, *Hj, *Hk;
+            auto gE = [&](int c) -> DGrid* { return E[c].ok() ? &E[c] : nullptr; };
+            auto gH = [&](int c) -> DGrid* { return H[c].ok() ? &H[c] : nullptr; };
+            if (dir == 0) {
+                Ej = gE(1); Hj = gH(1); Ek = gE(2); Hk = gH(2);
+                if ((Ej && Ej->lz == 1) || (Ek && Ek->lz == 1)) { f.cor = 1; f.transCor1 = 2; f.transCor2 = 0; }
+                else { f.cor = 2; f.transCor1 = 1; f.transCor2 = 0; }
+                f.corI = 0; f.corJ = 1; f.corK = 2;
+                f.sz = {{1, fx.sz[1], fx.sz[2]}};
+            } else if (dir == 1) {
+                Ej = gE(2); Hj = gH(2); Ek = gE(0); Hk = gH(0);
+                f.cor = 0; f.transCor1 = 2; f.transCor2 = 1;
+                f.corI = 1; f.corJ = 2; f.corK = 0;
+                f.sz = {{fx.sz[0], 1, fx.sz[2]}};
+            } else {
+                Ej = gE(0); Hj = gH(0); Ek = gE(1); Hk = gH(1);
+                f.cor = 0; f.transCor1 = 1; f.transCor2 = 2;
+                f.corI = 2; f.corJ = 0; f.corK = 1;
+                f.sz = {{fx.sz[0], fx.sz[1], 1}};
+            }
+            std::array<int, 3> loc = fx.loc;
+            if (pl) loc[f.transCor2] = fx.loc[f.transCor2] + fx.sz[f.transCor2] - 1;
+            std::array<int, 3> lo = loc;
+            if (E[2].ok() && H[2].ok()) {
+                std::array<int, 3> so = f.sz;
+                lo[f.corK] -= 1;
+                so[f.corK] += 1;
+                f.Ek.push_back(makeStore(Ek, lo, so, nf, is3D));
+                f.Hj.push_back(makeStore(Hj, lo, so, nf, is3D));
+                lo[f.corI] -= 1;
+                f.Hj.push_back(makeStore(Hj, lo, so, nf, is3D));
+                lo[f.corK] += 1;
+                so[f.corK] -= 1;
+                lo[f.corJ] -= 1;
+                so[f.corJ] += 1;
+                f.Hk.push_back(makeStore(Hk, lo, so, nf, is3D));
+                lo[f.corI] += 1;
+                f.Hk.push_back(makeStore(Hk, lo, so, nf, is3D));
+                f.Ej.push_back(makeStore(Ej, lo, so, nf, is3D));
+                f.padEk = f.corK; f.padHj = f.corK; f.padHk = f.corJ; f.padEj = f.corJ;
+            } else if (E[2].ok()) {
+                if (Ek) {
+                    lo[f.corI] += -1;
+                    f.Ek.push_back(makeStore(Ek, loc, f.sz, nf, is3D));
+                    f.Hj.push_back(makeStore(Hj, loc, f.sz, nf, is3D));
+                    f.Hj.push_back(makeStore(Hj, lo, f.sz, nf, is3D));
+                } else {
+                    lo[f.corI] += -1;
+                    f.Ej.push_back(makeStore(Ej, loc, f.sz, nf, is3D));
+                    f.Hk.push_back(makeStore(Hk, loc, f.sz, nf, is3D));
+                    f.Hk.push_back(makeStore(Hk, lo, f.sz, nf, is3D));
+                }
+            }
+            f.weight = pl ? 1.0 : -1.0;
+            fx.faces.push_back(f);
+        };
+        if (!(E[2].ok() && H[2].ok())) {
+            if (fx.sz[0] > 1 && fx.sz[1] > 1) {
+                make(1, false); make(1, true); make(0, false); make(0, true);
+            } else if (fx.sz[0] == 1) {
+                make(0, true);
+            } else {
+                make(1, true);
+            }
+        } else {
+            if (fx.sz[0] > 1 && fx.sz[1] > 1 && fx.sz[2] > 1) {
+                make(0, true); make(0, false); make(1, true); make(1, false); make(2, true); make(2, false);
+            } else if (fx.sz[0] == 1) make(0, true);
+            else if (fx.sz[1] == 1) make(1, true);
+            else if (fx.sz[2] == 1) make(2, true);
+        }
+        fluxes.push_back(fx);
+    }
+}
+
+// fieldIn: fftFact = exp(-i w t) then a rank-1 update per box row
+// (parallelFlux.hpp:296-311, parallelStorageFreqDTC.cpp:21-30).
+void Sim::fluxFieldIn() {
+    for (auto& fx : fluxes) {
+        if (t_step % fx.timeInt != 0) continue;
+        int nf = static_cast<int>(fx.freqs.size());
+        std::vector<cplx> fft(nf);
+        for (int f = 0; f < nf; ++f) fft[f] = std::exp(cplx(0.0, -1.0 * tcur * fx.freqs[f]));
+        auto acc = [&](FreqStore& s) {
+            int idx = 0;
+            for (int kk = 0; kk < s.lsz[2]; ++kk)
+                for (int jj = 0; jj < s.lsz[1]; ++jj)
+                    for (int ii = 0; ii < s.lsz[0]; ++ii, ++idx) {
+                        double u = s.grid->at(s.lloc[0] + ii, s.lloc[1] + jj, s.lloc[2] + kk);
+                        for (int f = 0; f < nf; ++f) {
+                            s.fre[idx * nf + f] += fft[f].real() * u;
+                            s.fim[idx * nf + f] += fft[f].imag() * u;
+                        }
+                    }
+        };
+        for (auto& face : fx.faces) {
+            for (auto& s : face.Ej) acc(s);
+            for (auto& s : face.Ek) acc(s);
+            for (auto& s : face.Hj) acc(s);
+            for (auto& s : face.Hk) acc(s);
+        }
+        ++fx.t_step;
+    }
+}
+
+// Simpson's rule of the reference (parallelFlux.hpp:548-591).
+static cplx simps(const cplx* f, int n, double d) {
+    if (n == 1) return f[0] * d;
+    cplx r(0.0, 0.0);
+    if (n % 2 == 0) {
+        for (int ii = 0; ii < (n - 2) / 2; ii++) r += d / 6.0 * (f[ii * 2] + 4.0 * f[ii * 2 + 1] + f[(ii + 1) * 2]);
+        for (int ii = 1; ii < n / 2; ii++) r += d / 6.0 * (f[ii * 2 - 1] + 4.0 * f[ii * 2] + f[ii * 2 + 1]);
+        r += d / 4.0 * (f[0] + f[1] + f[n - 1] + f[n - 2]);
+    } else {
+        for (int ii = 0; ii < (n - 1) / 2; ii++) r += d / 3.0 * (f[ii * 2] + 4.0 * f[ii * 2 + 1] + f[(ii + 1) * 2]);
+    }
+    return r;
+}
+
+// combineField + getFlux (no incident normalisation),
+// parallelFlux.hpp:325-395, 406-534.  Returns the total flux per frequency.
+std::vector<cplx> Sim::getFlux(int fi) {
+    Flux& fx = fluxes[fi];
+    int nf = static_cast<int>(fx.freqs.size());
+    int nt = fx.t_step;
+    std::vector<cplx> total(nf, 0.0);
+    for (auto& face : fx.faces) {
+        int n1 = face.sz[face.transCor1], n2 = face.sz[face.cor];
+        double d1 = IP.d[face.transCor1], d2 = IP.d[face.cor];
+        // Face-centre frequency fields, [f][i1][i2].
+        auto combine = [&](std::vector<FreqStore>& st, int pad) {
+            std::vector<cplx> out;
+            if (st.empty()) return out;
+            out.assign(static_cast<size_t>(nf) * n1 * n2, 0.0);
+            int nOff = (pad >= 0) ? 2 : 1;
+            double w = 1.0 / static_cast<double>(st.size() * nOff);
+            for (auto& s : st) {
+                // store index of the box point at offset (a along transCor1, b along cor)
+                for (int o = 0; o < nOff; ++o)
+                    for (int i1 = 0; i1 < n1; ++i1)
+                        for (int i2 = 0; i2 < n2; ++i2) {
+                            std::array<int, 3> q = {{0, 0, 0}};
+                            q[face.transCor1] = i1;
+                            q[face.cor] = i2;
+                            if (o == 1) q[pad] += 1;
+                            if (q[0] >= s.lsz[0] || q[1] >= s.lsz[1] || q[2] >= s.lsz[2]) continue;
+                            int idx = q[0] + s.lsz[0] * (q[1] + s.lsz[1] * q[2]);
+                            for (int f = 0; f < nf; ++f)
+                                out[(static_cast<size_t>(f) * n1 + i1) * n2 + i2] += w * cplx(s.fre[idx * nf + f], s.fim[idx * nf + f]);
+                        }
+            }
+            return out;
+        };
+        std::vector<cplx> Ej = combine(face.Ej, face.padEj), Ek = combine(face.Ek, face.padEk);
+        std::vector<cplx> Hj = combine(face.Hj, face.padHj), Hk = combine(face.Hk, face.padHk);
+        for (int f = 0; f < nf; ++f) {
+            // flux_grid(x = i2 along cor, y = i1 along transCor1)
+            std::vector<cplx> g(static_cast<size_t>(n1) * n2, 0.0);
+            for (int i1 = 0; i1 < n1; ++i1)
+                for (int i2 = 0; i2 < n2; ++i2) {
+                    size_t k = (static_cast<size_t>(f) * n1 + i1) * n2 + i2;
+                    cplx v(0.0, 0.0);
+                    if (!Ej.empty()) v = Ej[k] * std::conj(Hk[k]) / std::pow(static_cast<double>(nt), 2.0);
+                    if (!Ek.empty()) v += -1.0 * (Ek[k] * std::conj(Hj[k]) / std::pow(static_cast<double>(nt), 2.0));
+                    g[i2 + n2 * i1] = v;
+                }
+            cplx s;
+            if (n1 > 1 && n2 > 1) {
+                std::vector<cplx> r(n1);
+                for (int jj = 0; jj < n1; ++jj) r[jj] = simps(&g[jj * n2], n2, d2);
+                s = simps(r.data(), n1, d1);
+            } else if (n2 > 1) {
+                s = simps(&g[0], n2, d2);
+            } else {
+                s = simps(&g[0], n1, d1);
+            }
+            total[f] += fx.fluxConv * face.weight * s;
+        }
+    }
+    return total;
+}
+
+// ------------------------------------------------------------- emitters ----
+// Lindblad superoperator column as built by linblad (parallelQE.hpp:371-392):
+// L(rho) = A^H rho A - 1/2 A^H A rho - 1/2 rho A^H A, column-major N x N.
+static std::vector<double> lindblad(const std::vector<double>& A, const std::vector<double>& den, int N) {
+    auto mm = [&](const std::vector<double>& X, bool tx, const std::vector<double>& Y, bool ty) {
+        std::vector<double> R(N * N, 0.0);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < N; ++k) {
+                    double x = tx ? X[k + i * N] : X[i + k * N];
+                    double y = ty ? Y[j + k * N] : Y[k + j * N];
+                    s += x * y;
+                }
+                R[i + j * N] = s;
+            }
+        return R;
+    };
+    std::vector<double> out = mm(mm(A, true, den, false), false, A, false);
+    std::vector<double> t1 = mm(mm(A, true, A, false), false, den, false);
+    std::vector<double> t2 = mm(mm(den, false, A, true), false, A, false);
+    for (int i = 0; i < N * N; ++i) out[i] += -0.5 * t1[i] - 0.5 * t2[i];
+    return out;
+}
+
+// 3j symbol via the Racah formula (replaces gsl_sf_coupling_3j, used by
+// BasisSet::sphHarmonicIntegrator, src/ML/BasisSet.cpp:46-52).
+static double fact(int k) { return std::tgamma(k + 1.0); }
+double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+    if (m1 + m2 + m3 != 0) return 0.0;
+    if (j3 < std::abs(j1 - j2) || j3 > j1 + j2) return 0.0;
+    if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3) return 0.0;
+    double tri = fact(j1 + j2 - j3) * fact(j1 - j2 + j3) * fact(-j1 + j2 + j3) / fact(j1 + j2 + j3 + 1);
+    double pre = std::sqrt(tri * fact(j1 + m1) * fact(j1 - m1) * fact(j2 + m2) * fact(j2 - m2) * fact(j3 + m3) * fact(j3 - m3));
+    double s = 0.0;
+    int kmin = std::max(0, std::max(j2 - j3 - m1, j1 - j3 + m2));
+    int kmax = std::min(j1 + j2 - j3, std::min(j1 - m1, j2 + m2));
+    for (int k = kmin; k <= kmax; ++k)
+        s += ((k % 2) ? -1.0 : 1.0) /
+             (fact(k) * fact(j1 + j2 - j3 - k) * fact(j1 - m1 - k) * fact(j2 + m2 - k) * fact(j3 - j2 + m1 + k) * fact(j3 - j1 - m2 + k));
+    int ph = j1 - j2 - m3;
+    return ((ph % 2 != 0) ? -1.0 : 1.0) * pre * s;
+}
+static double sphInt(std::array<int, 2> a, std::array<int, 2> b, std::array<int, 2> c) {
+    double sph = std::sqrt((a[0] * 2 + 1) * (b[0] * 2 + 1) * (c[0] * 2 + 1) / (4 * kPi));
+    sph *= std::pow(-1.0, a[1]) * wigner3j(a[0], b[0], c[0], 0, 0, 0) * wigner3j(a[0], b[0], c[0], -a[1], b[1], c[1]);
+    return sph;
+}
+// BasisSet::expectationVals, src/ML/BasisSet.cpp:17-43.
+static std::vector<cplx> expectation(const std::vector<std::array<int, 2>>& B, int dir) {
+    int N = static_cast<int>(B.size());
+    std::vector<cplx> r;
+    std::array<int, 2> op0 = {{1, 0}}, op1 = {{1, 1}}, op2 = {{1, -1}};
+    for (int ii = 0; ii < N; ++ii)
+        for (int jj = 0; jj < N; ++jj) {
+            if (dir == 0) r.push_back(sphInt(B[ii], op2, B[jj]) - sphInt(B[ii], op1, B[jj]));
+            else if (dir == 1) r.push_back(cplx(0, sphInt(B[ii], op1, B[jj]) + sphInt(B[ii], op2, B[jj])));
+            else r.push_back(std::sqrt(2) * sphInt(B[ii], op0, B[jj]));
+        }
+    for (auto& x : r) x *= std::sqrt(2 * kPi / 3);
+    return r;
+}
+
+void Sim::buildQE() {
+    for (auto& qs : IP.qes) {
+        QE q;
+        int N = qs.nlevel;
+        q.N = N;
+        q.na = qs.na;
+        q.hbar = HBAR * EPS0 * std::pow(SPEED_OF_LIGHT, 3) / std::pow(IP.a * IP.I0, 2.0);
+        q.zOff = is3D ? 1 : 0;
+        if (qs.locs.empty()) throw std::logic_error("quantum emitter object contains no grid points");
+        // Relaxation superoperator (parallelQE.hpp:269-329 or the explicit gam).
+        std::vector<std::vector<double>> gam;
+        if (!qs.gam.empty()) {
+            gam = qs.gam;
+        } else {
+            gam.assign(N * N, std::vector<double>(N * N, 0.0));
+            for (auto& rt : qs.relax) {
+                std::vector<double> A(N * N, 0.0), den(N * N, 0.0);
+                A[rt.n0 * N + rt.nf] = 1.0;
+                for (int ii = 0; ii < N * N; ++ii) {
+                    std::fill(den.begin(), den.end(), 0.0);
+                    den[ii] = 1.0;
+                    auto L = lindblad(A, den, N);
+                    daxpy(N * N, rt.rate, L.data(), 1, gam[ii].data(), 1);
+                }
+                if (rt.dephasing > 0.0) {
+                    A[rt.n0 * N + rt.nf] = 0.0;
+                    A[rt.n0 * N + rt.n0] = std::sqrt(2.0);
+                    for (int ii = 0; ii < N * N; ++ii) {
+                        std::fill(den.begin(), den.end(), 0.0);
+                        den[ii] = 1.0;
+                        auto L = lindblad(A, den, N);
+                        daxpy(N * N, rt.dephasing, L.data(), 1, gam[ii].data(), 1);
+                    }
+                }
+            }
+        }
+        q.gam.resize(gam.size());
+        for (size_t i = 0; i < gam.size(); ++i)
+            for (size_t j = 0; j < gam[i].size(); ++j)
+                if (gam[i][j] != 0.0) q.gam[i].push_back({static_cast<int>(j), gam[i][j]});
+        // Box of emitter points (generateDenLists, parallelQE.hpp:394-470).
+        std::array<int, 3> mn = {{lx, ly, lz}}, mx = {{0, 0, 0}};
+        for (auto& l : qs.locs)
+            for (int i = 0; i < 3; ++i) {
+                mn[i] = std::min(mn[i], l[i]);
+                mx[i] = std::max(mx[i], l[i]);
+            }
+        q.mn = mn;
+        for (int i = 0; i < 3; ++i) q.nbox[i] = mx[i] - mn[i] + 1;
+        if (!is3D) q.nbox[2] = 1;
+        for (int c = 0; c < 3; ++c) {
+            q.P[c].init(q.nbox[0] + 2, q.nbox[1] + 2, q.nbox[2] + 1 + q.zOff);
+            q.e[c].assign(static_cast<size_t>(q.nbox[0]) * q.nbox[1] * q.nbox[2], 0.0);
+        }
+        std::vector<cplx> ex[3];
+        for (int c = 0; c < 3; ++c) ex[c] = expectation(qs.basis, c);
+        for (auto& mem : qs.members) {
+            QE::Member M;
+            M.h0.assign(N * N, 0.0);
+            if (!qs.h0Full.empty()) M.h0 = qs.h0Full;
+            else
+                for (int ii = 0; ii < N; ++ii) M.h0[ii * N + ii] = mem.first[ii];
+            for (int c = 0; c < 3; ++c) {
+                if (!qs.muFull[c].empty()) M.mu[c] = qs.muFull[c];
+                else {
+                    M.mu[c] = ex[c];
+                    if (static_cast<int>(qs.couplings.size()) > N * N) throw std::logic_error("more couplings than nlevel^2");
+                    for (size_t ii = 0; ii < qs.couplings.size(); ++ii) M.mu[c][ii] *= qs.couplings[ii];
+                }
+                M.negmu[c] = M.mu[c];
+                for (auto& x : M.negmu[c]) x *= -1.0;
+                M.use[c] = std::any_of(M.mu[c].begin(), M.mu[c].end(), [](cplx v) { return v != 0.0; });
+            }
+            for (auto& l : qs.locs) {
+                QEDensity d;
+                d.loc = {{l[0] - mn[0], l[1] - mn[1], l[2] - mn[2]}};
+                d.rho.assign(N * N, 0.0);
+                d.f0 = d.f1 = d.f2 = d.f3 = d.rho;
+                d.rho[0] = cplx(mem.second, 0.0);
+                M.den.push_back(d);
+            }
+            q.members.push_back(M);
+        }
+        q.dtcLevs = qs.dtcLevs;
+        q.popTimeInt = qs.popTimeInt;
+        q.pops.resize(qs.dtcLevs.size());
+        q.curPop.assign(qs.dtcLevs.size(), 0.0);
+        q.npoints = static_cast<int>(qs.locs.size());
+        qes.push_back(q);
+    }
+}
+
+// Hamiltonian::getHam, src/ML/Hamiltonian.cpp:59-69.
+void Sim::getHam(QE::Member& mem, const cplx Ex, const cplx Ey, const cplx Ez, std::vector<cplx>& H) {
+    H = mem.h0;
+    if (Ex == 0.0 && Ey == 0.0 && Ez == 0.0) return;
+    const cplx e[3] = {Ex, Ey, Ez};
+    for (int c = 0; c < 3; ++c)
+        if (mem.use[c])
+            for (size_t i = 0; i < H.size(); ++i) H[i] += e[c] * mem.negmu[c][i];
+}
+
+// denDeriv, parallelQE.hpp:727-744 (two zgemm, column-major) + sparse Lindblad.
+void Sim::denDeriv(QE& q, const std::vector<cplx>& H, const cplx* den, cplx* out) {
+    int N = q.N;
+    cplx ioh(0.0, 1.0 / q.hbar), nioh(0.0, -1.0 / q.hbar);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            cplx s(0.0, 0.0);
+            for (int k = 0; k < N; ++k) s += H[i + k * N] * den[k + j * N];
+            out[i + j * N] = ioh * s;
+        }
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            cplx s(0.0, 0.0);
+            for (int k = 0; k < N; ++k) s += den[i + k * N] * H[k + j * N];
+            out[i + j * N] += nioh * s;
+        }
+    for (size_t ii = 0; ii < q.gam.size(); ++ii)
+        for (auto& kv : q.gam[ii]) out[ii] += den[kv.first] * kv.second;
+}
+
+// PCABAM4, parallelQE.hpp:751-770.
+void Sim::PCABAM4(QE& q, QE::Member& mem, QEDensity& d, std::vector<cplx>& H) {
+    int S = q.N * q.N;
+    std::vector<cplx> pred(d.rho), fpred(S);
+    for (int i = 0; i < S; ++i) pred[i] += 55.0 * dt / 24.0 * d.f0[i];
+    for (int i = 0; i < S; ++i) pred[i] += -59.0 * dt / 24.0 * d.f1[i];
+    for (int i = 0; i < S; ++i) pred[i] += 37.0 * dt / 24.0 * d.f2[i];
+    for (int i = 0; i < S; ++i) pred[i] += -9.0 * dt / 24.0 * d.f3[i];
+    size_t bi = d.loc[0] + static_cast<size_t>(q.nbox[0]) * (d.loc[1] + static_cast<size_t>(q.nbox[1]) * d.loc[2]);
+    getHam(mem, q.e[0][bi], q.e[1][bi], q.e[2][bi], H);
+    denDeriv(q, H, pred.data(), fpred.data());
+    for (int i = 0; i < S; ++i) d.rho[i] += 9.0 * dt / 24.0 * fpred[i];
+    for (int i = 0; i < S; ++i) d.rho[i] += 19.0 * dt / 24.0 * d.f0[i];
+    for (int i = 0; i < S; ++i) d.rho[i] += -5.0 * dt / 24.0 * d.f1[i];
+    for (int i = 0; i < S; ++i) d.rho[i] += dt / 24.0 * d.f2[i];
+    d.f3 = d.f2;
+    d.f2 = d.f1;
+    d.f1 = d.f0;
+    getHam(mem, q.e[0][bi], q.e[1][bi], q.e[2][bi], H);
+    denDeriv(q, H, d.rho.data(), d.f0.data());
+}
+
+// updateDensity, parallelQE.hpp:614-678, for one rank.  The field at an
+// emitter point p is Ez(p) in 2D (getE_TM) and the mean of the two Yee edges
+// E_c(p) and E_c(p - e_c) in 3D (getE_TE).
+void Sim::updateDensity(QE& q) {
+    for (int c = 0; c < 3; ++c) {
+        if (!E[c].ok()) continue;
+        for (int z = 0; z < q.nbox[2]; ++z)
+            for (int y = 0; y < q.nbox[1]; ++y)
+                for (int x = 0; x < q.nbox[0]; ++x) {
+                    int px = q.mn[0] + x, py = q.mn[1] + y, pz = q.mn[2] + z;
+                    int zb = is3D ? 1 : 0;
+                    double v;
+                    if (!is3D) v = E[c].at(px + 1, py + 1, 0);
+                    else {
+                        std::array<int, 3> o = {{0, 0, 0}};
+                        o[c] = -1;
+                        v = 0.5 * E[c].at(px + 1, py + 1, pz + zb) + 0.5 * E[c].at(px + 1 + o[0], py + 1 + o[1], pz + zb + o[2]);
+                    }
+                    q.e[c][x + static_cast<size_t>(q.nbox[0]) * (y + static_cast<size_t>(q.nbox[1]) * z)] = cplx(v, 0.0);
+                }
+    }
+    for (int c = 0; c < 3; ++c) std::fill(q.P[c].v.begin(), q.P[c].v.end(), 0.0);
+    std::vector<cplx> H;
+    for (auto& mem : q.members)
+        for (auto& d : mem.den) {
+            PCABAM4(q, mem, d, H);
+            for (size_t l = 0; l < q.dtcLevs.size(); ++l)
+                if (q.pop_t_step % q.popTimeInt == 0) q.curPop[l] += d.rho[q.dtcLevs[l]];
+            for (int c = 0; c < 3; ++c) {
+                if (!E[c].ok()) continue;
+                // updateQEPol: P += na * Re(zdotc(rho, mu_c))
+                cplx s(0.0, 0.0);
+                for (int i = 0; i < q.N * q.N; ++i) s += std::conj(d.rho[i]) * mem.mu[c][i];
+                int zi = is3D ? d.loc[2] + 1 : 0;
+                q.P[c].at(d.loc[0] + 1, d.loc[1] + 1, zi) += q.na * std::real(s);
+            }
+        }
+    for (size_t l = 0; l < q.dtcLevs.size(); ++l) {
+        if (q.pop_t_step % q.popTimeInt == 0) q.pops[l].push_back(q.curPop[l] / static_cast<double>(q.npoints));
+        q.curPop[l] = 0.0;
+    }
+    ++q.pop_t_step;
+}
+
+// addQE, parallelQE.hpp:682-718 (addP, FDTD_up_eq.cpp:1367-1380).
+void Sim::addQE() {
+    for (auto& q : qes) {
+        for (int c = 0; c < 3; ++c) {
+            if (!E[c].ok()) continue;
+            std::array<int, 3> off = {{0, 0, 0}};
+            if (is3D) off[c] = 1;
+            int nzb = is3D ? q.nbox[2] + 1 : 1;
+            for (int z = 0; z < nzb; ++z)
+                for (int y = 0; y < q.nbox[1] + 1; ++y)
+                    for (int x = 0; x < q.nbox[0] + 1; ++x) {
+                        // E local (mn + x + ...) <- P(x, y, z) and P(x+off)
+                        int gx = q.mn[0] + x, gy = q.mn[1] + y, gz = is3D ? q.mn[2] + z : 0;
+                        if (gx >= lx || gy >= ly || gz >= lz) continue;
+                        int pz = is3D ? z : 0;
+                        double a = -0.5 * q.P[c].at(x, y, pz) / epsNode.at(gx, gy, gz);
+                        E[c].at(gx, gy, gz) += a;
+                        double b = -0.5 * q.P[c].at(x + off[0], y + off[1], pz + off[2]) / epsNode.at(gx + off[0], gy + off[1], gz + off[2]);
+                        E[c].at(gx, gy, gz) += b;
+                    }
+        }
+        updateDensity(q);
+    }
+}
+
+// step(), parallelFDTDField.hpp:1228-1303 (no magnetic, chiral or TFSF terms).
+void Sim::step() {
+    // updateH: H += curl E on the run lists.
+    for (int ii = 0; ii < 3; ++ii) {
+        if (!H[ii].ok()) continue;
+        DGrid* gj = E[(ii + 1) % 3].ok() ? &E[(ii + 1) % 3] : nullptr;
+        DGrid* gk = E[(ii + 2) % 3].ok() ? &E[(ii + 2) % 3] : nullptr;
+        for (auto& r : upH[ii]) {
+            auto& u = r.up;
+            auto& pf = r.pref;
+            if (gj) {
+                daxpy(u[0], pf[2], gj->p(u[1]), 1, H[ii].p(u[1]), 1);
+                daxpy(u[0], -1.0 * pf[2], gj->p(u[4]), 1, H[ii].p(u[1]), 1);
+            }
+            if (gk) {
+                daxpy(u[0], -1.0 * pf[1], gk->p(u[1]), 1, H[ii].p(u[1]), 1);
+                daxpy(u[0], pf[1], gk->p(u[3]), 1, H[ii].p(u[1]), 1);
+            }
+        }
+    }
+    for (int ii = 0; ii < 3; ++ii)
+        if (HPML[ii] && H[ii].ok()) HPML[ii]->updateGrid();
+    // Soft sources: U += dt * Re(sum of pulses at tcur).
+    for (auto& s : srcs) {
+        if (!s.active) continue;
+        cplx pv(0.0, 0.0);
+        for (auto& p : s.pulses) pv += pulseValue(p, tcur);
+        std::fill(s.pulVec.begin(), s.pulVec.end(), std::real(pv));
+        for (int kk = 0; kk < s.sz[2]; ++kk)
+            for (int jj = 0; jj < s.sz[1]; ++jj)
+                daxpy(s.sz[0], dt, s.pulVec.data(), 1,
+                      &s.grid->at(s.loc[0] + jj * s.add1[0] + kk * s.add2[0], s.loc[1] + jj * s.add1[1] + kk * s.add2[1],
+                                  s.loc[2] + jj * s.add1[2] + kk * s.add2[2]),
+                      s.stride);
+    }
+    // updatePolE: Lorentz ADE on the upLorD runs (E before the D update).
+    for (int ii = 0; ii < 3; ++ii) {
+        if (!E[ii].ok()) continue;
+        for (auto& r : upLorD[ii]) {
+            const Object& o = *IP.objs[r.up[5]];
+            for (size_t pp = 0; pp < o.alpha.size(); ++pp) {
+                double* P = lorP[ii][pp].p(r.up[1]);
+                double* Pp = prevLorP[ii][pp].p(r.up[1]);
+                dcopy(r.up[0], P, 1, scratch.data(), 1);
+                dscal(r.up[0], o.alpha[pp], P, 1);
+                daxpy(r.up[0], o.xi[pp], Pp, 1, P, 1);
+                daxpy(r.up[0], o.gamma[pp], E[ii].p(r.up[1]), 1, P, 1);
+                dcopy(r.up[0], scratch.data(), 1, Pp, 1);
+            }
+        }
+    }
+    // updateD and updateE (TwoCompCurl / OneCompCurlJ / OneCompCurlK).
+    auto curlE = [&](std::vector<Run>& lst, DGrid& U, int ii) {
+        DGrid* gj = H[(ii + 1) % 3].ok() ? &H[(ii + 1) % 3] : nullptr;
+        DGrid* gk = H[(ii + 2) % 3].ok() ? &H[(ii + 2) % 3] : nullptr;
+        for (auto& r : lst) {
+            auto& u = r.up;
+            auto& pf = r.pref;
+            if (gj) {
+                daxpy(u[0], pf[2], gj->p(u[1]), 1, U.p(u[1]), 1);
+                daxpy(u[0], -1.0 * pf[2], gj->p(u[4]), 1, U.p(u[1]), 1);
+            }
+            if (gk) {
+                daxpy(u[0], -1.0 * pf[1], gk->p(u[1]), 1, U.p(u[1]), 1);
+                daxpy(u[0], pf[1], gk->p(u[3]), 1, U.p(u[1]), 1);
+            }
+        }
+    };
+    for (int ii = 0; ii < 3; ++ii)
+        if (D[ii].ok()) curlE(upD[ii], D[ii], ii);
+    for (int ii = 0; ii < 3; ++ii)
+        if (E[ii].ok()) curlE(upE[ii], E[ii], ii);
+    for (int ii = 0; ii < 3; ++ii)
+        if (EPML[ii] && E[ii].ok()) EPML[ii]->updateGrid();
+    // D2E: E = D/eps - sum_p P_p/eps on the upLorD runs.
+    for (int ii = 0; ii < 3; ++ii) {
+        if (!E[ii].ok()) continue;
+        for (auto& r : upLorD[ii]) {
+            double ep = r.pref[3];
+            dcopy(r.up[0], D[ii].p(r.up[1]), 1, E[ii].p(r.up[1]), 1);
+            dscal(r.up[0], 1.0 / ep, E[ii].p(r.up[1]), 1);
+            for (size_t pp = 0; pp < lorP[ii].size(); ++pp) daxpy(r.up[0], -1.0 / ep, lorP[ii][pp].p(r.up[1]), 1, E[ii].p(r.up[1]), 1);
+        }
+    }
+    addQE();
+    tcur += dt;
+    ++t_step;
+    outputDetectors();
+    fluxFieldIn();
+}
+
+}  // namespace orc
+
+// ------------------------------------------------------------------ C API ---
+// Field layout returned to callers: interior points only, x fastest, then y,
+// then z (index x + nx*(y + ny*z)), components 0..5 = Ex,Ey,Ez,Hx,Hy,Hz.
+extern "C" {
+
+struct OracleHandle {
+    std::unique_ptr<orc::Sim> sim;
+    std::string err;
+};
+
+static thread_local std::string g_oracle_err;
+
+void* oracle_create(const char* json) {
+    try {
+        auto h = new OracleHandle;
+        h->sim = std::make_unique<orc::Sim>(json);
+        return h;
+    } catch (std::exception& e) {
+        g_oracle_err = e.what();
+        return nullptr;
+    }
+}
+const char* oracle_last_error() { return g_oracle_err.c_str(); }
+void oracle_destroy(void* h) { delete static_cast<OracleHandle*>(h); }
+
+int oracle_step(void* hp, int nsteps) {
+    auto* h = static_cast<OracleHandle*>(hp);
+    try {
+        for (int i = 0; i < nsteps; ++i) h->sim->step();
+        return 0;
+    } catch (std::exception& e) {
+        g_oracle_err = e.what();
+        return -1;
+    }
+}
+
+int oracle_info(void* hp, int* n3, double* dt, int* nsteps, int* is3d) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    n3[0] = S.n[0];
+    n3[1] = S.n[1];
+    n3[2] = S.is3D ? S.n[2] : 1;
+    *dt = S.dt;
+    *nsteps = S.IP.nSteps();
+    *is3d = S.is3D ? 1 : 0;
+    return 0;
+}
+
+int oracle_flags(void* hp, int* flags) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    flags[0] = S.dielectricMatInPML ? 1 : 0;
+    flags[1] = S.magMatInPML ? 1 : 0;
+    return 0;
+}
+
+int oracle_field(void* hp, int comp, double* out) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    orc::DGrid& g = comp < 3 ? S.E[comp] : S.H[comp - 3];
+    int nz = S.is3D ? S.n[2] : 1;
+    size_t k = 0;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < S.n[1]; ++y)
+            for (int x = 0; x < S.n[0]; ++x) out[k++] = g.ok() ? g.at(x + 1, y + 1, S.is3D ? z + 1 : 0) : 0.0;
+    return 0;
+}
+
+int oracle_num_detectors(void* hp) { return static_cast<int>(static_cast<OracleHandle*>(hp)->sim->dtcs.size()); }
+
+// Returns the number of samples; writes times[ns] and values[ns*npts].
+int oracle_detector(void* hp, int d, double* times, double* values, int cap_samples, int* npts) {
+    auto& P = static_cast<OracleHandle*>(hp)->sim->dtcs.at(d);
+    int np = P.sz[0] * P.sz[1] * P.sz[2];
+    *npts = np;
+    int ns = static_cast<int>(P.times.size());
+    int nc = std::min(ns, cap_samples);
+    for (int i = 0; i < nc; ++i) times[i] = P.times[i];
+    for (int i = 0; i < nc * np; ++i) values[i] = P.values[i];
+    return ns;
+}
+
+int oracle_flux(void* hp, int f, double* re, double* im, int cap) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    auto v = S.getFlux(f);
+    int n = static_cast<int>(v.size());
+    for (int i = 0; i < std::min(n, cap); ++i) {
+        re[i] = v[i].real();
+        im[i] = v[i].imag();
+    }
+    return n;
+}
+
+int oracle_pops(void* hp, int q, int lev, double* re, double* im, int cap) {
+    auto& Q = static_cast<OracleHandle*>(hp)->sim->qes.at(q);
+    auto& v = Q.pops.at(lev);
+    int n = static_cast<int>(v.size());
+    for (int i = 0; i < std::min(n, cap); ++i) {
+        re[i] = v[i].real();
+        im[i] = v[i].imag();
+    }
+    return n;
+}
+
+// Density matrix of member m, emitter e (column-major, N*N complex).
+int oracle_density(void* hp, int q, int m, int e, double* re, double* im) {
+    auto& Q = static_cast<OracleHandle*>(hp)->sim->qes.at(q);
+    auto& d = Q.members.at(m).den.at(e);
+    for (int i = 0; i < Q.N * Q.N; ++i) {
+        re[i] = d.rho[i].real();
+        im[i] = d.rho[i].imag();
+    }
+    return Q.N;
+}
+
+double oracle_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) { return orc::wigner3j(j1, j2, j3, m1, m2, m3); }
+
+double oracle_pulse(const char* json, int src, double t) {
+    try {
+        auto IP = chiml::parseInputs(json);
+        std::complex<double> v(0.0, 0.0);
+        for (auto& p : IP.srcs.at(src).pulses) v += chiml::pulseValue(p, t);
+        return v.real();
+    } catch (...) {
+        return std::nan("");
+    }
+}
+
+}  // extern "C"

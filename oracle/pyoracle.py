@@ -1,0 +1,124 @@
+"""ctypes wrapper around the FP64 oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+module, as the checker; the product path never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build() or make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.oracle_create.restype = ctypes.c_void_p
+        L.oracle_create.argtypes = [ctypes.c_char_p]
+        L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.oracle_flags.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_num_detectors.argtypes = [ctypes.c_void_p]
+        L.oracle_detector.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_flux.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.oracle_pops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.oracle_density.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.oracle_wigner3j.restype = ctypes.c_double
+        L.oracle_wigner3j.argtypes = [ctypes.c_int] * 6
+        L.oracle_pulse.restype = ctypes.c_double
+        L.oracle_pulse.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """CPU FP64 restatement of the reference step loop for one JSON deck."""
+
+    def __init__(self, deck_json):
+        L = lib()
+        self._h = L.oracle_create(deck_json.encode())
+        if not self._h:
+            raise ValueError(L.oracle_last_error().decode())
+        n3 = (ctypes.c_int * 3)()
+        dt = ctypes.c_double()
+        ns = ctypes.c_int()
+        is3d = ctypes.c_int()
+        L.oracle_info(self._h, n3, ctypes.byref(dt), ctypes.byref(ns), ctypes.byref(is3d))
+        self.shape = (n3[2], n3[1], n3[0])  # (z, y, x) for numpy
+        self.dt = dt.value
+        self.nsteps = ns.value
+        self.is3d = bool(is3d.value)
+        fl = (ctypes.c_int * 2)()
+        L.oracle_flags(self._h, fl)
+        self.dielectric_mat_in_pml = bool(fl[0])
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_destroy(self._h)
+            self._h = None
+
+    def step(self, n=1):
+        if lib().oracle_step(self._h, int(n)) != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+
+    def field(self, comp):
+        """comp 0..5 = Ex,Ey,Ez,Hx,Hy,Hz; array of shape (nz, ny, nx)."""
+        out = np.zeros(int(np.prod(self.shape)), dtype=np.float64)
+        lib().oracle_field(self._h, int(comp), _dp(out))
+        return out.reshape(self.shape)
+
+    def num_detectors(self):
+        return lib().oracle_num_detectors(self._h)
+
+    def detector(self, d, cap=1 << 20):
+        npts = ctypes.c_int()
+        times = np.zeros(1, dtype=np.float64)
+        ns = lib().oracle_detector(self._h, d, _dp(times), _dp(times), 0, ctypes.byref(npts))
+        times = np.zeros(ns, dtype=np.float64)
+        vals = np.zeros(ns * max(npts.value, 1), dtype=np.float64)
+        lib().oracle_detector(self._h, d, _dp(times), _dp(vals), ns, ctypes.byref(npts))
+        return times, vals.reshape(ns, npts.value)
+
+    def flux(self, f, nmax=4096):
+        re = np.zeros(nmax)
+        im = np.zeros(nmax)
+        n = lib().oracle_flux(self._h, f, _dp(re), _dp(im), nmax)
+        return re[:n] + 1j * im[:n]
+
+    def pops(self, q, lev, nmax=1 << 20):
+        re = np.zeros(nmax)
+        im = np.zeros(nmax)
+        n = lib().oracle_pops(self._h, q, lev, _dp(re), _dp(im), nmax)
+        return re[:n] + 1j * im[:n]
+
+    def density(self, q, member, emitter, nlev):
+        re = np.zeros(nlev * nlev)
+        im = np.zeros(nlev * nlev)
+        lib().oracle_density(self._h, q, member, emitter, _dp(re), _dp(im))
+        return (re + 1j * im).reshape(nlev, nlev, order="F")
+
+
+def wigner3j(j1, j2, j3, m1, m2, m3):
+    return lib().oracle_wigner3j(j1, j2, j3, m1, m2, m3)
+
+
+def pulse(deck_json, src, t):
+    return lib().oracle_pulse(deck_json.encode(), src, t)
