@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the chiML time-stepping path on one MI355X.
+
+Workload: 3D Yee 256^3, dx = 2 nm, 16-cell CPML, diagonal anisotropic
+background eps = (2.0, 2.25, 2.5), Drude + 2-pole Lorentz sphere of radius 40
+cells, soft plane-wave sheet, 40-frequency DFT flux box of 120^3 points.
+A step is one pass of the reference's step() over the whole domain; the metric
+is Mcell-updates/s = cells * steps / seconds / 1e6 with the fields resident in
+HBM.  The FP32 path is the headline (`value`); the FP64 path is measured on
+the same deck and reported beside it.
+
+roofline: the dominant kernel's algorithmic bytes per launch (every state
+word it must read and write once: fields, material ids, CPML psi, pole
+currents) divided by its mean launch time, measured with HIP events on the
+simulation's stream around each of its launches in the timed region, against
+8 TB/s.  The dominant kernel is the one with the largest time per step in the
+warmup, where every launch is timed (the "kernels" breakdown).  traffic
+comes from the committed rocprofv3 --pmc summary
+(profiles/traffic_config2_<precision>.json) when it is present.
+cpu_baseline: the FP64 CPU restatement of the reference step loop (the
+oracle) timed on this host for a few steps of the same 256^3 deck.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=50)
+    p.add_argument("--n", type=int, default=256, help="points per axis")
+    p.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="first run a plain 256 MiB device copy three times (the known-size kernel that scales "
+                        "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
+    return p.parse_args()
+
+
+KERNELS = ("update_h", "update_h_pml", "update_e", "update_e_pml", "dft_flush", "ml_density")
+
+
+def run_gpu(n, steps, warmup, precision):
+    import torch
+    import chiml_amd
+    from chiml_amd import decks
+
+    deck = decks.config2_3d(n=n, steps=warmup + steps, precision=precision)
+    sim = chiml_amd.FDTDField(deck)
+    cells = sim.cost()[0]
+    # warmup: every launch timed with HIP events -> per-kernel breakdown
+    sim.profile(True)
+    sim.step(warmup)
+    sim.sync()
+    kern = {}
+    for k in KERNELS:
+        ms, nl = sim.profile_read(k)
+        if nl > 0:
+            kern[k] = {"ms_per_launch": ms / nl, "launches": nl, "ms_per_step": ms / max(warmup, 1),
+                       "bytes_per_launch": sim.kernel_bytes(k)}
+    cand = [k for k in kern if kern[k]["bytes_per_launch"] > 0]
+    dominant = max(cand, key=lambda k: kern[k]["ms_per_step"]) if cand else "update_e"
+    # timed region: only the dominant kernel carries events (two records per step)
+    sim.profile_only(dominant)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    dom_ms, dom_n = sim.profile_read(dominant)
+    sim.profile(False)
+    flux = sim.fluxArr()[0].getFlux() if sim.fluxArr() else None
+    wall = t1 - t0
+    res = {
+        "cells": cells,
+        "wall_s": wall,
+        "mcells": cells * steps / wall / 1e6,
+        "ms_per_step": wall / steps * 1e3,
+        "kernels": kern,
+        "dominant": dominant,
+        "dominant_ms": dom_ms / max(dom_n, 1),
+        "dominant_launches": dom_n,
+        "dominant_bytes": sim.kernel_bytes(dominant),
+        "footprint": sim.footprint(),
+        "flux0": None if flux is None else [float(flux[0].real), float(flux[0].imag)],
+    }
+    sim.close()
+    return res
+
+
+def load_traffic(precision):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_config2_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def roofline(r, precision):
+    name = r["dominant"]
+    ms = r["dominant_ms"]
+    b = r["dominant_bytes"]
+    achieved = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    traffic = None
+    t = load_traffic(precision)
+    if t and name in t.get("kernels", {}):
+        traffic = t["kernels"][name]["hbm_bytes_per_launch"]
+    return name, {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "kernel": name,
+        "ms_per_launch": round(ms, 5),
+        "launches": r["dominant_launches"],
+        "algorithmic_bytes_per_launch": b,
+    }
+
+
+def kernel_table(r):
+    out = {}
+    for k, v in r["kernels"].items():
+        e = {"ms_per_launch": round(v["ms_per_launch"], 5), "launches": v["launches"], "ms_per_step": round(v["ms_per_step"], 5)}
+        if v["bytes_per_launch"] > 0:
+            e["GBps"] = round(v["bytes_per_launch"] / (v["ms_per_launch"] * 1e-3) / 1e9, 1)
+        out[k] = e
+    return out
+
+
+def cpu_baseline(n, steps):
+    from chiml_amd import decks
+    from oracle.pyoracle import Oracle
+
+    deck = decks.config2_3d(n=n, steps=steps + 1, precision="fp64")
+    t0 = time.perf_counter()
+    o = Oracle(deck)
+    setup = time.perf_counter() - t0
+    o.step(1)  # first touch
+    t1 = time.perf_counter()
+    o.step(steps)
+    t2 = time.perf_counter()
+    cells = n ** 3
+    return {
+        "value": round(cells * steps / (t2 - t1) / 1e6, 3),
+        "unit": "Mcell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"FP64 oracle (reference step loop restated, BLAS-1 per x-run), same {n}^3 deck, "
+                  f"{steps} timed steps after 1 untimed step; setup {setup:.1f} s excluded",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU slabs are a design note only)"}))
+        sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    r32 = run_gpu(args.n, args.steps, args.warmup, args.precision)
+    kname, roof = roofline(r32, args.precision)
+    out = {
+        "metric": "Mcell-updates/s + % HBM roofline (FP32 vs FP64); sweep p50/p99 latency",
+        "value": round(r32["mcells"], 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D Yee {args.n}^3 dx=2nm, 16-cell CPML, aniso eps (2.0,2.25,2.5), Drude+2 Lorentz sphere r=40, "
+                        f"soft plane-wave sheet, 40-freq DFT flux box 120^3",
+            "cells": int(r32["cells"]),
+            "precision": args.precision,
+            "parallelism": "single GPU, one device-resident domain",
+        },
+        "roofline": roof,
+        "kernels": kernel_table(r32),
+    }
+    if not args.no_fp64 and args.precision == "fp32":
+        r64 = run_gpu(args.n, args.steps, args.warmup, "fp64")
+        k64, roof64 = roofline(r64, "fp64")
+        out["fp64"] = {
+            "value": round(r64["mcells"], 1),
+            "ms_per_step": round(r64["ms_per_step"], 4),
+            "roofline": roof64,
+            "kernels": kernel_table(r64),
+        }
+        out["fp32_over_fp64"] = round(r32["mcells"] / r64["mcells"], 3)
+        if r32["flux0"] and r64["flux0"]:
+            a = complex(*r32["flux0"])
+            b = complex(*r64["flux0"])
+            out["flux_f0_rel_err_fp32_vs_fp64"] = abs(a - b) / abs(b) if abs(b) > 0 else None
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -45,7 +45,9 @@ def load_library():
     L.chiml_sim_pops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_size_t, c_size_p]
     L.chiml_sim_density.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_dbl_p, c_int_p]
     L.chiml_sim_cost.argtypes = [ctypes.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]
+    L.chiml_sim_kernel_bytes.argtypes = [ctypes.c_void_p, ctypes.c_char_p, c_dbl_p]
     L.chiml_sim_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.chiml_sim_profile_only.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     L.chiml_sim_profile_read.argtypes = [ctypes.c_void_p, ctypes.c_char_p, c_dbl_p, c_int_p]
     L.chiml_sim_footprint.argtypes = [ctypes.c_void_p, c_size_p]
     L.chiml_sim_estimate_bytes.argtypes = [ctypes.c_char_p, c_size_p]
@@ -202,8 +204,18 @@ class FDTDField:
         _check(load_library().chiml_sim_cost(self._h, *[ctypes.byref(v) for v in vals]))
         return tuple(v.value for v in vals)
 
+    def kernel_bytes(self, kernel):
+        """Algorithmic HBM bytes per launch of a kernel (0 if unknown)."""
+        b = ctypes.c_double()
+        _check(load_library().chiml_sim_kernel_bytes(self._h, kernel.encode(), ctypes.byref(b)))
+        return b.value
+
     def profile(self, enable=True):
         _check(load_library().chiml_sim_profile(self._h, 1 if enable else 0))
+
+    def profile_only(self, kernel):
+        """Times only the launches of one kernel (two HIP events per launch)."""
+        _check(load_library().chiml_sim_profile_only(self._h, kernel.encode()))
 
     def profile_read(self, kernel):
         ms = ctypes.c_double()

@@ -296,6 +296,332 @@ __global__ void __launch_bounds__(256) k_update_e(StepArgs<T> a) {
     }
 }
 
+// ------------------------------------------------ 3D split kernels ---------
+// The 3D field update is split by region: an interior kernel over the box of
+// cells outside every CPML slab (no CPML code, two cells per thread with
+// 8/16-byte vector loads) and a slab kernel over the CPML cells only.  A
+// probe of this stencil shape on MI355X reached 4.9 TB/s with one cell per
+// thread and 5.5 TB/s with two; CPML branches inside interior waves were the
+// main loss of a single whole-domain kernel.
+template <typename T> struct V2;
+template <> struct V2<float> { using type = float2; };
+template <> struct V2<double> { using type = double2; };
+
+template <typename T>
+struct MatLds {
+    T ce, inveps;
+    int flags, poleset;
+};
+
+// Per-block flag: does this block's cell range touch any source box?
+template <typename T>
+__device__ __forceinline__ bool block_has_src(const StepArgs<T>& a, int cx0, int cx1, int cy0, int cy1, int cz0, int cz1) {
+    for (int s = 0; s < a.nsrc; ++s) {
+        const int* b = a.src_box + 6 * s;
+        if (cx0 < b[1] && b[0] < cx1 && cy0 < b[3] && b[2] < cy1 && cz0 < b[5] && b[4] < cz1) return true;
+    }
+    return false;
+}
+
+template <typename T>
+__device__ __forceinline__ T ade_apply(const StepArgs<T>& a, int c, int ps, int x, int y, int z, T e_ade) {
+    const int* bx = a.box[c];
+    long long bi = (x - bx[0]) + static_cast<long long>(bx[1] - bx[0]) * ((y - bx[2]) + static_cast<long long>(bx[3] - bx[2]) * (z - bx[4]));
+    const int np = a.pole_count[ps];
+    const T* co = a.pole_coef + static_cast<long long>(ps) * CHIML_MAX_POLES * 3;
+    T jsum = 0;
+    for (int p = 0; p < np; ++p) {
+        T* Pp = a.P[c] + p * a.boxn[c];
+        T* Jp = a.J[c] + p * a.boxn[c];
+        T P = Pp[bi], J = Jp[bi];
+        T Jn = co[3 * p] * P + co[3 * p + 1] * J + co[3 * p + 2] * e_ade;
+        Pp[bi] = P + Jn;
+        Jp[bi] = Jn;
+        jsum += Jn;
+    }
+    return jsum;
+}
+
+// E of one cell from its curl term, material and sources (the E-form of
+// updatePolE + updateE/updateD + D2E).
+template <typename T>
+__device__ __forceinline__ T e_new(const StepArgs<T>& a, const MatLds<T>& m, int c, int x, int y, int z, T e_old, T acc, T src) {
+    T e_base = (m.flags & CHIML_MAT_DCELL) ? e_old : e_old + src;
+    T jsum = 0;
+    if (m.poleset >= 0) jsum = ade_apply(a, c, m.poleset, x, y, z, e_old + src);
+    return e_base + m.ce * acc - m.inveps * jsum;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_mat(const StepArgs<T>& a, MatLds<T> (*mt)[CHIML_MAX_MAT], const int* nm) {
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nt = blockDim.x * blockDim.y;
+    for (int c = 0; c < 3; ++c)
+        for (int k = tid; k < nm[c]; k += nt) mt[c][k] = MatLds<T>{a.ce[c][k], a.inveps[c][k], a.flags[c][k], a.poleset[c][k]};
+    __syncthreads();
+}
+
+struct Box3 {
+    int x0, x1, y0, y1, z0, z1;
+};
+
+// Interior H update, two cells (x, x+1) per thread; the caller guarantees
+// x0, x1 - x0 and nx even so that the pair is aligned.
+template <typename T, bool SRC>
+__global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b) {
+    using V = typename V2<T>::type;
+    const int x = b.x0 + 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = b.y0 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = b.z0 + blockIdx.z;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    bool bsrc = false;
+    if (SRC) bsrc = block_has_src(a, b.x0 + 2 * blockIdx.x * blockDim.x, b.x0 + 2 * (blockIdx.x + 1) * blockDim.x,
+                                  b.y0 + blockIdx.y * blockDim.y, b.y0 + (blockIdx.y + 1) * blockDim.y, z, z + 1);
+    if (x >= b.x1 || y >= b.y1) return;
+    const int sy = nx;
+    const int sz = nx * ny;
+    const int i = x + nx * (y + ny * z);
+    const T *Ex = a.F[0], *Ey = a.F[1], *Ez = a.F[2];
+    const V ex = *reinterpret_cast<const V*>(Ex + i), ey = *reinterpret_cast<const V*>(Ey + i), ez = *reinterpret_cast<const V*>(Ez + i);
+    const bool zok = z < nz - 1, yok = y < ny - 1;
+    const T idx = a.inv_d[0], idy = a.inv_d[1], idz = a.inv_d[2];
+    int step = 0;
+    if (SRC && bsrc) step = *a.step;
+    if (yok && zok) {  // Hx = Hx + ch[(Ey(z+1)-Ey)/dz - (Ez(y+1)-Ez)/dy]
+        const V eyz = *reinterpret_cast<const V*>(Ey + i + sz), ezy = *reinterpret_cast<const V*>(Ez + i + sy);
+        V h = *reinterpret_cast<const V*>(a.F[3] + i);
+        const T ch = a.ch[0][0];
+        h.x += ch * ((eyz.x - ey.x) * idz - (ezy.x - ez.x) * idy);
+        h.y += ch * ((eyz.y - ey.y) * idz - (ezy.y - ez.y) * idy);
+        if (SRC && bsrc) {
+            h.x += src_value(a, 3, x, y, z, step);
+            h.y += src_value(a, 3, x + 1, y, z, step);
+        }
+        *reinterpret_cast<V*>(a.F[3] + i) = h;
+    }
+    if (zok) {  // Hy = Hy + ch[(Ez(x+1)-Ez)/dx - (Ex(z+1)-Ex)/dz]
+        const V exz = *reinterpret_cast<const V*>(Ex + i + sz);
+        const T ez2 = Ez[i + 2 < nx * ny * nz ? i + 2 : i];
+        V h = *reinterpret_cast<const V*>(a.F[4] + i);
+        const T ch = a.ch[1][0];
+        const bool x1ok = x + 1 < nx - 1;
+        h.x += ch * ((ez.y - ez.x) * idx - (exz.x - ex.x) * idz);
+        if (x1ok) h.y += ch * ((ez2 - ez.y) * idx - (exz.y - ex.y) * idz);
+        if (SRC && bsrc) {
+            h.x += src_value(a, 4, x, y, z, step);
+            if (x1ok) h.y += src_value(a, 4, x + 1, y, z, step);
+        }
+        *reinterpret_cast<V*>(a.F[4] + i) = h;
+    }
+    if (yok) {  // Hz = Hz + ch[(Ex(y+1)-Ex)/dy - (Ey(x+1)-Ey)/dx]
+        const V exy = *reinterpret_cast<const V*>(Ex + i + sy);
+        const T ey2 = Ey[i + 2 < nx * ny * nz ? i + 2 : i];
+        V h = *reinterpret_cast<const V*>(a.F[5] + i);
+        const T ch = a.ch[2][0];
+        const bool x1ok = x + 1 < nx - 1;
+        h.x += ch * ((exy.x - ex.x) * idy - (ey.y - ey.x) * idx);
+        if (x1ok) h.y += ch * ((exy.y - ex.y) * idy - (ey2 - ey.y) * idx);
+        if (SRC && bsrc) {
+            h.x += src_value(a, 5, x, y, z, step);
+            if (x1ok) h.y += src_value(a, 5, x + 1, y, z, step);
+        }
+        *reinterpret_cast<V*>(a.F[5] + i) = h;
+    }
+}
+
+// Interior E update, two cells per thread.
+template <typename T, bool SRC>
+__global__ void __launch_bounds__(256) k_e3_int(StepArgs<T> a, Box3 b, int nm0, int nm1, int nm2) {
+    using V = typename V2<T>::type;
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int nm[3] = {nm0, nm1, nm2};
+    load_mat(a, mt, nm);
+    const int x = b.x0 + 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = b.y0 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = b.z0 + blockIdx.z;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    bool bsrc = false;
+    if (SRC) bsrc = block_has_src(a, b.x0 + 2 * blockIdx.x * blockDim.x, b.x0 + 2 * (blockIdx.x + 1) * blockDim.x,
+                                  b.y0 + blockIdx.y * blockDim.y, b.y0 + (blockIdx.y + 1) * blockDim.y, z, z + 1);
+    if (x >= b.x1 || y >= b.y1) return;
+    const int sy = nx;
+    const int sz = nx * ny;
+    const int i = x + nx * (y + ny * z);
+    const T *Hx = a.F[3], *Hy = a.F[4], *Hz = a.F[5];
+    const V hx = *reinterpret_cast<const V*>(Hx + i), hy = *reinterpret_cast<const V*>(Hy + i), hz = *reinterpret_cast<const V*>(Hz + i);
+    const T idx = a.inv_d[0], idy = a.inv_d[1], idz = a.inv_d[2];
+    int step = 0;
+    if (SRC && bsrc) step = *a.step;
+    const T hzxm = x > 0 ? Hz[i - 1] : T(0);
+    const T hyxm = x > 0 ? Hy[i - 1] : T(0);
+    {  // Ex = Ex + ce[(Hz - Hz(y-1))/dy - (Hy - Hy(z-1))/dz]
+        const V hzy = y > 0 ? *reinterpret_cast<const V*>(Hz + i - sy) : V{0, 0};
+        const V hyz = z > 0 ? *reinterpret_cast<const V*>(Hy + i - sz) : V{0, 0};
+        const uchar2 id = a.mat_e[0] ? *reinterpret_cast<const uchar2*>(a.mat_e[0] + i) : uchar2{0, 0};
+        V e = *reinterpret_cast<const V*>(a.F[0] + i);
+        T s0 = 0, s1 = 0;
+        if (SRC && bsrc) {
+            s0 = src_value(a, 0, x, y, z, step);
+            s1 = src_value(a, 0, x + 1, y, z, step);
+        }
+        T acc0 = (hz.x - hzy.x) * idy - (hy.x - hyz.x) * idz;
+        T acc1 = (hz.y - hzy.y) * idy - (hy.y - hyz.y) * idz;
+        V o;
+        o.x = (x < nx - 1) ? e_new(a, mt[0][id.x], 0, x, y, z, e.x, acc0, s0) : e.x;
+        o.y = (x + 1 < nx - 1) ? e_new(a, mt[0][id.y], 0, x + 1, y, z, e.y, acc1, s1) : e.y;
+        *reinterpret_cast<V*>(a.F[0] + i) = o;
+    }
+    if (y < ny - 1) {  // Ey = Ey + ce[(Hx - Hx(z-1))/dz - (Hz - Hz(x-1))/dx]
+        const V hxz = z > 0 ? *reinterpret_cast<const V*>(Hx + i - sz) : V{0, 0};
+        const uchar2 id = a.mat_e[1] ? *reinterpret_cast<const uchar2*>(a.mat_e[1] + i) : uchar2{0, 0};
+        V e = *reinterpret_cast<const V*>(a.F[1] + i);
+        T s0 = 0, s1 = 0;
+        if (SRC && bsrc) {
+            s0 = src_value(a, 1, x, y, z, step);
+            s1 = src_value(a, 1, x + 1, y, z, step);
+        }
+        T acc0 = (hx.x - hxz.x) * idz - (hz.x - hzxm) * idx;
+        T acc1 = (hx.y - hxz.y) * idz - (hz.y - hz.x) * idx;
+        V o;
+        o.x = e_new(a, mt[1][id.x], 1, x, y, z, e.x, acc0, s0);
+        o.y = e_new(a, mt[1][id.y], 1, x + 1, y, z, e.y, acc1, s1);
+        *reinterpret_cast<V*>(a.F[1] + i) = o;
+    }
+    if (z < nz - 1) {  // Ez = Ez + ce[(Hy - Hy(x-1))/dx - (Hx - Hx(y-1))/dy]
+        const V hxy = y > 0 ? *reinterpret_cast<const V*>(Hx + i - sy) : V{0, 0};
+        const uchar2 id = a.mat_e[2] ? *reinterpret_cast<const uchar2*>(a.mat_e[2] + i) : uchar2{0, 0};
+        V e = *reinterpret_cast<const V*>(a.F[2] + i);
+        T s0 = 0, s1 = 0;
+        if (SRC && bsrc) {
+            s0 = src_value(a, 2, x, y, z, step);
+            s1 = src_value(a, 2, x + 1, y, z, step);
+        }
+        T acc0 = (hy.x - hyxm) * idx - (hx.x - hxy.x) * idy;
+        T acc1 = (hy.y - hy.x) * idx - (hx.y - hxy.y) * idy;
+        V o;
+        o.x = e_new(a, mt[2][id.x], 2, x, y, z, e.x, acc0, s0);
+        o.y = e_new(a, mt[2][id.y], 2, x + 1, y, z, e.y, acc1, s1);
+        *reinterpret_cast<V*>(a.F[2] + i) = o;
+    }
+}
+
+// CPML slab cells: the union of the z slabs, the y slabs inside the z range
+// and the x slabs inside the y and z ranges, enumerated x fastest.
+struct SlabMap {
+    int nx, ny, nz, Tx, Ty, Tz;
+    long long nA, nB, nC;  // cell counts of the three regions
+};
+
+__device__ __forceinline__ void slab_cell(const SlabMap& m, long long t, int& x, int& y, int& z) {
+    if (t < m.nA) {  // z slabs: z in [0,Tz) U [nz-Tz,nz), all x, y
+        const long long plane = static_cast<long long>(m.nx) * m.ny;
+        int k = static_cast<int>(t / plane);
+        long long r = t - k * plane;
+        z = k < m.Tz ? k : m.nz - 2 * m.Tz + k;
+        y = static_cast<int>(r / m.nx);
+        x = static_cast<int>(r - static_cast<long long>(y) * m.nx);
+        return;
+    }
+    t -= m.nA;
+    if (t < m.nB) {  // y slabs inside the z interior
+        const int nyb = 2 * m.Ty;
+        const long long plane = static_cast<long long>(m.nx) * nyb;
+        int k = static_cast<int>(t / plane);
+        long long r = t - k * plane;
+        z = m.Tz + k;
+        int yy = static_cast<int>(r / m.nx);
+        x = static_cast<int>(r - static_cast<long long>(yy) * m.nx);
+        y = yy < m.Ty ? yy : m.ny - 2 * m.Ty + yy;
+        return;
+    }
+    t -= m.nB;
+    {  // x slabs inside the y and z interiors
+        const int nxb = 2 * m.Tx;
+        const int nyi = m.ny - 2 * m.Ty;
+        const long long plane = static_cast<long long>(nxb) * nyi;
+        int k = static_cast<int>(t / plane);
+        long long r = t - k * plane;
+        z = m.Tz + k;
+        int yy = static_cast<int>(r / nxb);
+        int xx = static_cast<int>(r - static_cast<long long>(yy) * nxb);
+        y = m.Ty + yy;
+        x = xx < m.Tx ? xx : m.nx - 2 * m.Tx + xx;
+    }
+}
+
+template <typename T, bool HMAP, bool SRC>
+__global__ void __launch_bounds__(256) k_h3_slab(StepArgs<T> a, SlabMap m) {
+    const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (t >= m.nA + m.nB + m.nC) return;
+    int x, y, z;
+    slab_cell(m, t, x, y, z);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const long long i = lidx(x, y, z, nx, ny);
+    const long long sy = nx, sz = static_cast<long long>(nx) * ny;
+    int step = 0;
+    if (SRC) step = *a.step;
+    const T *Ex = a.F[0], *Ey = a.F[1], *Ez = a.F[2];
+    if (y < ny - 1 && z < nz - 1) {
+        T dEy = Ey[i + sz] - Ey[i], dEz = Ez[i + sy] - Ez[i];
+        T acc = pml_term(a.pml[3], 2, z, x, y, z, nx, ny, dEy, dEy * a.inv_d[2]) - pml_term(a.pml[3], 1, y, x, y, z, nx, ny, dEz, dEz * a.inv_d[1]);
+        T ch = HMAP ? a.ch[0][a.mat_h[0][i]] : a.ch[0][0];
+        T h = a.F[3][i] + ch * acc;
+        if (SRC) h += src_value(a, 3, x, y, z, step);
+        a.F[3][i] = h;
+    }
+    if (x < nx - 1 && z < nz - 1) {
+        T dEz = Ez[i + 1] - Ez[i], dEx = Ex[i + sz] - Ex[i];
+        T acc = pml_term(a.pml[4], 0, x, x, y, z, nx, ny, dEz, dEz * a.inv_d[0]) - pml_term(a.pml[4], 2, z, x, y, z, nx, ny, dEx, dEx * a.inv_d[2]);
+        T ch = HMAP ? a.ch[1][a.mat_h[1][i]] : a.ch[1][0];
+        T h = a.F[4][i] + ch * acc;
+        if (SRC) h += src_value(a, 4, x, y, z, step);
+        a.F[4][i] = h;
+    }
+    if (x < nx - 1 && y < ny - 1) {
+        T dEx = Ex[i + sy] - Ex[i], dEy = Ey[i + 1] - Ey[i];
+        T acc = pml_term(a.pml[5], 1, y, x, y, z, nx, ny, dEx, dEx * a.inv_d[1]) - pml_term(a.pml[5], 0, x, x, y, z, nx, ny, dEy, dEy * a.inv_d[0]);
+        T ch = HMAP ? a.ch[2][a.mat_h[2][i]] : a.ch[2][0];
+        T h = a.F[5][i] + ch * acc;
+        if (SRC) h += src_value(a, 5, x, y, z, step);
+        a.F[5][i] = h;
+    }
+}
+
+template <typename T, bool SRC>
+__global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int nm0, int nm1, int nm2) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int nm[3] = {nm0, nm1, nm2};
+    load_mat(a, mt, nm);
+    const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (t >= m.nA + m.nB + m.nC) return;
+    int x, y, z;
+    slab_cell(m, t, x, y, z);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const long long i = lidx(x, y, z, nx, ny);
+    const long long sy = nx, sz = static_cast<long long>(nx) * ny;
+    int step = 0;
+    if (SRC) step = *a.step;
+    const T *Hx = a.F[3], *Hy = a.F[4], *Hz = a.F[5];
+    if (x < nx - 1) {
+        T dHz = Hz[i] - (y > 0 ? Hz[i - sy] : T(0)), dHy = Hy[i] - (z > 0 ? Hy[i - sz] : T(0));
+        T acc = pml_term(a.pml[0], 1, y, x, y, z, nx, ny, dHz, dHz * a.inv_d[1]) - pml_term(a.pml[0], 2, z, x, y, z, nx, ny, dHy, dHy * a.inv_d[2]);
+        const uint8_t id = a.mat_e[0] ? a.mat_e[0][i] : 0;
+        a.F[0][i] = e_new(a, mt[0][id], 0, x, y, z, a.F[0][i], acc, SRC ? src_value(a, 0, x, y, z, step) : T(0));
+    }
+    if (y < ny - 1) {
+        T dHx = Hx[i] - (z > 0 ? Hx[i - sz] : T(0)), dHz = Hz[i] - (x > 0 ? Hz[i - 1] : T(0));
+        T acc = pml_term(a.pml[1], 2, z, x, y, z, nx, ny, dHx, dHx * a.inv_d[2]) - pml_term(a.pml[1], 0, x, x, y, z, nx, ny, dHz, dHz * a.inv_d[0]);
+        const uint8_t id = a.mat_e[1] ? a.mat_e[1][i] : 0;
+        a.F[1][i] = e_new(a, mt[1][id], 1, x, y, z, a.F[1][i], acc, SRC ? src_value(a, 1, x, y, z, step) : T(0));
+    }
+    if (z < nz - 1) {
+        T dHy = Hy[i] - (x > 0 ? Hy[i - 1] : T(0)), dHx = Hx[i] - (y > 0 ? Hx[i - sy] : T(0));
+        T acc = pml_term(a.pml[2], 0, x, x, y, z, nx, ny, dHy, dHy * a.inv_d[0]) - pml_term(a.pml[2], 1, y, x, y, z, nx, ny, dHx, dHx * a.inv_d[1]);
+        const uint8_t id = a.mat_e[2] ? a.mat_e[2][i] : 0;
+        a.F[2][i] = e_new(a, mt[2][id], 2, x, y, z, a.F[2][i], acc, SRC ? src_value(a, 2, x, y, z, step) : T(0));
+    }
+}
+
 // ----------------------------------------------------- step counter ---------
 __global__ void k_advance(int* step) { *step += 1; }
 
@@ -358,31 +684,59 @@ __global__ void k_sample(SampleArgs<T> a) {
     }
 }
 
-// F[s][f] += sum_k exp(-i w_f t_k) U[k][s] over the staged samples k of the
+// F[f][s] += sum_k exp(-i w_f t_k) U[k][s] over the staged samples k of the
 // current block (parallelStorageFreqDTCReal::fieldIn rank-1 updates,
 // src/DTC/parallelStorageFreqDTC.cpp:21-30, batched over kDftBlock steps).
+// A thread owns one sampled cell: its staged samples sit in registers, the
+// block's phasors in LDS, and the accumulators are stored frequency-major so
+// that the read-modify-write of F is coalesced across threads.
+constexpr int kDftFreqChunk = 64;
+
 template <typename T>
-__global__ void k_dft_flush(const T* stage, const double* phasor, double* F, const int* stepp, int ndft, int nfreq, int interval,
-                            int force, int step_override) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_dft_flush(const T* stage, const double* phasor, double* F, const int* stepp, int ndft, int nfreq,
+                                                  int interval, int force, int step_override) {
+    __shared__ double2 w[kDftBlock * kDftFreqChunk];
     const int step = step_override >= 0 ? step_override : *stepp;
-    if (s >= ndft) return;
-    // samples taken so far: k = 0 .. kmax (kmax = step/interval)
     if (step % interval != 0 && !force) return;
     const int kmax = step / interval;
     if (!force && (kmax % kDftBlock) != kDftBlock - 1) return;
     const int kfirst = kmax - (kmax % kDftBlock);
     const int nk = kmax - kfirst + 1;
-    for (int f = 0; f < nfreq; ++f) {
-        double re = 0.0, im = 0.0;
-        for (int k = 0; k < nk; ++k) {
-            double u = static_cast<double>(stage[static_cast<long long>(k) * ndft + s]);
-            const double* w = phasor + 2 * (static_cast<long long>(kfirst + k) * nfreq + f);
-            re += w[0] * u;
-            im += w[1] * u;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double u[kDftBlock];
+#pragma unroll
+    for (int k = 0; k < kDftBlock; ++k) u[k] = (s < ndft && k < nk) ? static_cast<double>(stage[static_cast<long long>(k) * ndft + s]) : 0.0;
+    for (int f0 = 0; f0 < nfreq; f0 += kDftFreqChunk) {
+        const int nfc = min(kDftFreqChunk, nfreq - f0);
+        __syncthreads();
+        // slots past the open block's last sample get zero weight (the phasor
+        // table ends at the last sampled step, and 0 * garbage may be NaN)
+        for (int t = threadIdx.x; t < kDftBlock * nfc; t += blockDim.x) {
+            const int k = t / nfc, f = t % nfc;
+            double2 v = make_double2(0.0, 0.0);
+            if (k < nk) {
+                const double* src = phasor + 2 * (static_cast<long long>(kfirst + k) * nfreq + f0 + f);
+                v = make_double2(src[0], src[1]);
+            }
+            w[k * kDftFreqChunk + f] = v;
         }
-        F[2 * (static_cast<long long>(s) * nfreq + f)] += re;
-        F[2 * (static_cast<long long>(s) * nfreq + f) + 1] += im;
+        __syncthreads();
+        if (s < ndft) {
+            for (int f = 0; f < nfc; ++f) {
+                double re = 0.0, im = 0.0;
+#pragma unroll
+                for (int k = 0; k < kDftBlock; ++k) {
+                    const double2 ww = w[k * kDftFreqChunk + f];
+                    re += ww.x * u[k];
+                    im += ww.y * u[k];
+                }
+                double2* dst = reinterpret_cast<double2*>(F) + static_cast<long long>(f0 + f) * ndft + s;
+                double2 acc = *dst;
+                acc.x += re;
+                acc.y += im;
+                *dst = acc;
+            }
+        }
     }
 }
 
@@ -623,6 +977,11 @@ struct chiml_dev {
     SampleArgs<float> s32{};
     SampleArgs<double> s64{};
     bool hmap = false, has_src = false;
+    int nmat[3] = {1, 1, 1};
+    int enq_step = 0;    // step counter value after the step being enqueued
+    bool split = false;  // 3D interior + CPML-slab kernels
+    Box3 ibox{};
+    SlabMap slab{};
     int ndet = 0, totpts = 0, ndft = 0, nfreq = 0, dft_interval = 1;
     std::vector<int> det_npts, det_cap, det_interval;
     std::vector<long long> det_out_off;
@@ -634,6 +993,7 @@ struct chiml_dev {
     std::vector<QeHost> qes;
     size_t footprint = 0;
     bool profile = false;
+    std::string profile_only;  // non-empty: time only launches of this kernel
     std::map<std::string, std::pair<double, int>> prof;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
 
@@ -684,6 +1044,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
             a.inveps[k] = upload_cast<T>(c, p->mat_e_inveps[k], nm);
             a.flags[k] = c->upload<int>(p->mat_e_flags[k], nm);
             a.poleset[k] = c->upload<int>(p->mat_e_poleset[k], nm);
+            c->nmat[k] = nm;
         }
         bool hpresent = is3 || k < 2;
         a.mat_h[k] = (hpresent && p->mat_h[k]) ? c->upload<uint8_t>(p->mat_h[k], N) : nullptr;
@@ -856,34 +1217,68 @@ void launch_sample(chiml_dev* c, SampleArgs<T>& s) {
     hipLaunchKernelGGL(k_sample<T>, dim3((tot + 255) / 256), dim3(256), 0, c->stream, s);
 }
 
-template <typename T>
-void launch_flush(chiml_dev* c, int force) {
-    if (c->ndft == 0) return;
-    hipLaunchKernelGGL(k_dft_flush<T>, dim3((c->ndft + 127) / 128), dim3(128), 0, c->stream, static_cast<const T*>(c->dft_stage),
-                       c->dft_phasor, c->dft_F, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, force, -1);
-}
-
 struct Timed {
     chiml_dev* c;
     const char* name;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    Timed(chiml_dev* c_, const char* n) : c(c_), name(n) {
-        if (c->profile) {
+    bool on;
+    Timed(chiml_dev* c_, const char* n) : c(c_), name(n), on(c_->profile && (c_->profile_only.empty() || c_->profile_only == n)) {
+        if (on) {
             (void)hipEventCreate(&e0);
             (void)hipEventCreate(&e1);
             (void)hipEventRecord(e0, c->stream);
         }
     }
     ~Timed() {
-        if (c->profile) {
+        if (on) {
             (void)hipEventRecord(e1, c->stream);
             c->pending.push_back({name, {e0, e1}});
         }
     }
 };
 
+template <typename T>
+void launch_flush(chiml_dev* c, int force) {
+    if (c->ndft == 0) return;
+    // enq_step = the device step counter once the step being enqueued has advanced it
+    const int st = c->enq_step;
+    if (!force && (st % c->dft_interval != 0 || ((st / c->dft_interval) % kDftBlock) != kDftBlock - 1)) return;
+    Timed t(c, "dft_flush");
+    hipLaunchKernelGGL(k_dft_flush<T>, dim3((c->ndft + 255) / 256), dim3(256), 0, c->stream, static_cast<const T*>(c->dft_stage),
+                       c->dft_phasor, c->dft_F, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, force, -1);
+}
+
+
 template <typename T, int DIM>
 void launch_fields(chiml_dev* c, StepArgs<T>& a) {
+    if (DIM == 3 && c->split) {
+        const Box3 b = c->ibox;
+        dim3 blk(64, 4, 1);
+        dim3 gi(((b.x1 - b.x0) / 2 + 63) / 64, (b.y1 - b.y0 + 3) / 4, b.z1 - b.z0);
+        const long long ns = c->slab.nA + c->slab.nB + c->slab.nC;
+        dim3 gs(static_cast<unsigned>((ns + 255) / 256));
+        {
+            Timed t(c, "update_h");
+            if (c->has_src) hipLaunchKernelGGL((k_h3_int<T, true>), gi, blk, 0, c->stream, a, b);
+            else hipLaunchKernelGGL((k_h3_int<T, false>), gi, blk, 0, c->stream, a, b);
+        }
+        if (ns > 0) {
+            Timed t(c, "update_h_pml");
+            if (c->has_src) hipLaunchKernelGGL((k_h3_slab<T, false, true>), gs, dim3(256), 0, c->stream, a, c->slab);
+            else hipLaunchKernelGGL((k_h3_slab<T, false, false>), gs, dim3(256), 0, c->stream, a, c->slab);
+        }
+        {
+            Timed t(c, "update_e");
+            if (c->has_src) hipLaunchKernelGGL((k_e3_int<T, true>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
+            else hipLaunchKernelGGL((k_e3_int<T, false>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
+        }
+        if (ns > 0) {
+            Timed t(c, "update_e_pml");
+            if (c->has_src) hipLaunchKernelGGL((k_e3_slab<T, true>), gs, dim3(256), 0, c->stream, a, c->slab, c->nmat[0], c->nmat[1], c->nmat[2]);
+            else hipLaunchKernelGGL((k_e3_slab<T, false>), gs, dim3(256), 0, c->stream, a, c->slab, c->nmat[0], c->nmat[1], c->nmat[2]);
+        }
+        return;
+    }
     dim3 blk(64, 4, 1);
     dim3 grd((c->nx + 63) / 64, (c->ny + 3) / 4, c->nz);
     {
@@ -936,6 +1331,7 @@ void do_steps(chiml_dev* c, StepArgs<T>& a, SampleArgs<T>& s, int n) {
         else launch_fields<T, 2>(c, a);
         if (!c->qes.empty()) launch_ml<T>(c, a);
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->step_ctr.as<int>());
+        c->enq_step = c->steps_done + k + 1;
         launch_sample<T>(c, s);
         launch_flush<T>(c, 0);
         HIP_TRY(hipGetLastError());
@@ -990,6 +1386,22 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
         c->step_ctr.alloc(sizeof(int));
         if (c->prec == 32) build<float>(c.get(), p, c->a32, c->s32);
         else build<double>(c.get(), p, c->a64, c->s64);
+        if (c->dim == 3) {
+            const int Tx = p->pml_T[0], Ty = p->pml_T[1], Tz = p->pml_T[2];
+            Box3 b{Tx, p->nx - Tx, Ty, p->ny - Ty, Tz, p->nz - Tz};
+            // the interior kernel needs an even-aligned, even-width x range and no magnetic map
+            if (b.x1 > b.x0 && b.y1 > b.y0 && b.z1 > b.z0 && (b.x0 % 2) == 0 && ((b.x1 - b.x0) % 2) == 0 && (p->nx % 2) == 0 &&
+                !c->hmap) {
+                c->split = true;
+                c->ibox = b;
+                SlabMap& m = c->slab;
+                m.nx = p->nx; m.ny = p->ny; m.nz = p->nz;
+                m.Tx = Tx; m.Ty = Ty; m.Tz = Tz;
+                m.nA = 2LL * Tz * p->nx * p->ny;
+                m.nB = 2LL * Ty * p->nx * (p->nz - 2 * Tz);
+                m.nC = 2LL * Tx * (p->ny - 2 * Ty) * (p->nz - 2 * Tz);
+            }
+        }
         // t = 0 samples (taken by the reference's constructor, parallelFDTDField.cpp:832-837)
         if (c->prec == 32) {
             launch_sample<float>(c.get(), c->s32);
@@ -1088,18 +1500,24 @@ int chiml_dev_read_dft(chiml_dev_t* c, double* reim, size_t cap) {
             HIP_TRY(hipMemcpyAsync(tmp, c->dft_F, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             int last = kmax * c->dft_interval;
             if (c->prec == 32)
-                hipLaunchKernelGGL(k_dft_flush<float>, dim3((c->ndft + 127) / 128), dim3(128), 0, c->stream, static_cast<const float*>(c->dft_stage),
+                hipLaunchKernelGGL(k_dft_flush<float>, dim3((c->ndft + 255) / 256), dim3(256), 0, c->stream, static_cast<const float*>(c->dft_stage),
                                    c->dft_phasor, tmp, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, 1, last);
             else
-                hipLaunchKernelGGL(k_dft_flush<double>, dim3((c->ndft + 127) / 128), dim3(128), 0, c->stream, static_cast<const double*>(c->dft_stage),
+                hipLaunchKernelGGL(k_dft_flush<double>, dim3((c->ndft + 255) / 256), dim3(256), 0, c->stream, static_cast<const double*>(c->dft_stage),
                                    c->dft_phasor, tmp, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, 1, last);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(c->stream));
-            HIP_TRY(hipMemcpy(reim, tmp, n * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(host.data(), tmp, n * sizeof(double), hipMemcpyDeviceToHost));
         } else {
             HIP_TRY(hipStreamSynchronize(c->stream));
-            HIP_TRY(hipMemcpy(reim, c->dft_F, n * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(host.data(), c->dft_F, n * sizeof(double), hipMemcpyDeviceToHost));
         }
+        // device layout [f][s] -> ABI layout [s][f]
+        for (int f = 0; f < c->nfreq; ++f)
+            for (int sm = 0; sm < c->ndft; ++sm) {
+                reim[2 * (static_cast<size_t>(sm) * c->nfreq + f)] = host[2 * (static_cast<size_t>(f) * c->ndft + sm)];
+                reim[2 * (static_cast<size_t>(sm) * c->nfreq + f) + 1] = host[2 * (static_cast<size_t>(f) * c->ndft + sm) + 1];
+            }
     } catch (std::exception& e) {
         return fail(c, CHIML_E_HIP, e.what());
     }
@@ -1149,7 +1567,18 @@ int chiml_dev_steps_done(chiml_dev_t* c, int* steps) {
 
 int chiml_dev_profile(chiml_dev_t* c, int enable) {
     if (!c) return fail(nullptr, CHIML_E_INVALID, "null context");
+    collect_profile(c);
     c->profile = enable != 0;
+    c->profile_only.clear();
+    c->prof.clear();
+    return CHIML_OK;
+}
+
+int chiml_dev_profile_only(chiml_dev_t* c, const char* kernel) {
+    if (!c || !kernel) return fail(c, CHIML_E_INVALID, "invalid argument");
+    collect_profile(c);
+    c->profile = true;
+    c->profile_only = kernel;
     c->prof.clear();
     return CHIML_OK;
 }

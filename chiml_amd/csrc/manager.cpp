@@ -143,6 +143,7 @@ class FDTDField {
     std::vector<int> detInterval;
     std::vector<int> qeN;
     double cells = 0, bytesH = 0, bytesE = 0, bytesML = 0;
+    std::map<std::string, double> kbytes;  // algorithmic HBM bytes per launch
     size_t estimate = 0;
 
     std::vector<cplx> flux(int f);
@@ -834,37 +835,61 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     P.qe = qdesc.data();
     P.eps_node = epsNode.empty() ? nullptr : epsNode.data();
 
-    // ---- algorithmic bytes per step ------------------------------------------
-    const double s = IP.fp32 ? 4.0 : 8.0;
-    const int F = is3D ? 3 : 2;  // H components updated
-    cells = static_cast<double>(N);
-    bytesH = cells * (2.0 * F * s + (is3D ? 3.0 : 1.0) * s);  // read+write H, read E
-    bytesE = cells * (2.0 * (is3D ? 3.0 : 1.0) * s + F * s + (is3D ? 3.0 : 1.0));  // read+write E, read H, 1 B id per E component
-    double psiWords = 0.0;
-    for (int comp = 0; comp < 6; ++comp)
-        for (int a = 0; a < 3; ++a)
-            if (P.pml_b[comp][a]) {
-                double S = P.psi_lo[comp][a] + (P.psi_hi1[comp][a] - P.psi_hi0[comp][a]);
-                double other = 1.0;
-                for (int o = 0; o < 3; ++o)
-                    if (o != a) other *= (o == 0 ? nx : (o == 1 ? ny : nz));
-                double w = S * other;
-                if (comp < 3) bytesE += 2.0 * s * w;
-                else bytesH += 2.0 * s * w;
-                psiWords += w;
-            }
-    for (int c : ecomps) {
-        const int* b = dispBox[c];
-        double cnt = 0;
-        for (int z = b[4]; z < b[5]; ++z)
-            for (int y = b[2]; y < b[3]; ++y)
-                for (int x = b[0]; x < b[1]; ++x)
-                    if (psT[c][matE[c][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)]] >= 0)
-                        cnt += pcount[psT[c][matE[c][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)]]];
-        bytesE += cnt * 4.0 * s;  // P and J read + write per pole
+    // ---- algorithmic bytes per step, per kernel ------------------------------
+    // Each kernel must read every state word it uses once and write what it
+    // updates once: H kernels read H and E and write H; E kernels read E and H,
+    // write E and read one material byte per E component; CPML cells add their
+    // psi words (read + write) and dispersive cells their P and J per pole.
+    {
+        const double s = IP.fp32 ? 4.0 : 8.0;
+        const double nE = is3D ? 3.0 : 1.0, nH = is3D ? 3.0 : 2.0;
+        cells = static_cast<double>(N);
+        // interior box of the 3D split kernels (see device.hip); 2D uses one kernel each
+        const bool split = is3D && T[0] < nx - T[0] && T[1] < ny - T[1] && T[2] < nz - T[2] && (T[0] % 2) == 0 && (nx % 2) == 0;
+        const double Ni = split ? static_cast<double>(nx - 2 * T[0]) * (ny - 2 * T[1]) * (nz - 2 * T[2]) : cells;
+        const double Ns = cells - Ni;
+        auto inInt = [&](int x, int y, int z) {
+            return !split || (x >= T[0] && x < nx - T[0] && y >= T[1] && y < ny - T[1] && z >= T[2] && z < nz - T[2]);
+        };
+        kbytes["update_h"] = Ni * (2.0 * nH + nE) * s;
+        kbytes["update_e"] = Ni * ((2.0 * nE + nH) * s + nE);
+        if (split) {
+            kbytes["update_h_pml"] = Ns * (2.0 * nH + nE) * s;
+            kbytes["update_e_pml"] = Ns * ((2.0 * nE + nH) * s + nE);
+        }
+        double psiWords = 0.0;
+        for (int comp = 0; comp < 6; ++comp)
+            for (int a = 0; a < 3; ++a)
+                if (P.pml_b[comp][a]) {
+                    double S = P.psi_lo[comp][a] + (P.psi_hi1[comp][a] - P.psi_hi0[comp][a]);
+                    double other = 1.0;
+                    for (int o = 0; o < 3; ++o)
+                        if (o != a) other *= (o == 0 ? nx : (o == 1 ? ny : nz));
+                    double w = S * other;
+                    const char* k = comp < 3 ? (split ? "update_e_pml" : "update_e") : (split ? "update_h_pml" : "update_h");
+                    kbytes[k] += 2.0 * s * w;
+                    psiWords += w;
+                }
+        for (int c : ecomps) {
+            const int* b = dispBox[c];
+            for (int z = b[4]; z < b[5]; ++z)
+                for (int y = b[2]; y < b[3]; ++y)
+                    for (int x = b[0]; x < b[1]; ++x) {
+                        int ps = psT[c][matE[c][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)]];
+                        if (ps < 0) continue;
+                        kbytes[inInt(x, y, z) ? "update_e" : "update_e_pml"] += pcount[ps] * 4.0 * s;
+                    }
+        }
+        kbytes["ml_density"] = bytesML;
+        // DFT flush of one block of 32 staged samples: read the staged
+        // samples and the block's phasors, read and write every accumulator
+        if (ndft > 0)
+            kbytes["dft_flush"] = static_cast<double>(ndft) * (32.0 * s + 2.0 * 16.0 * nfreq) + 32.0 * 16.0 * nfreq;
+        bytesH = kbytes["update_h"] + kbytes["update_h_pml"];
+        bytesE = kbytes["update_e"] + kbytes["update_e_pml"];
+        estimate = static_cast<size_t>(cells * (6.0 * s + 3.0) + psiWords * s + bytesML / 7.0 * 5.0 +
+                                       static_cast<double>(ndft) * (nfreq * 16.0 + 32 * s));
     }
-    estimate = static_cast<size_t>(cells * (6.0 * s + 3.0) + psiWords * s + bytesML / 7.0 * 5.0 +
-                                   static_cast<double>(ndft) * (nfreq * 16.0 + 32 * s));
 
     int rc = chiml_dev_create(&P, &dev_);
     if (rc != CHIML_OK) throw std::runtime_error(std::string("chiml_dev_create: ") + chiml_dev_global_error());
@@ -1083,9 +1108,21 @@ int chiml_sim_cost(chiml_sim_t* s, double* cells, double* bh, double* be, double
     return CHIML_OK;
 }
 
+int chiml_sim_kernel_bytes(chiml_sim_t* s, const char* kernel, double* bytes) {
+    if (!s || !kernel || !bytes) return sim_fail("null argument");
+    auto it = s->f->kbytes.find(kernel);
+    *bytes = it == s->f->kbytes.end() ? 0.0 : it->second;
+    return CHIML_OK;
+}
+
 int chiml_sim_profile(chiml_sim_t* s, int enable) {
     if (!s) return sim_fail("null simulation");
     return chiml_dev_profile(s->f->dev_, enable);
+}
+
+int chiml_sim_profile_only(chiml_sim_t* s, const char* kernel) {
+    if (!s) return sim_fail("null simulation");
+    return chiml_dev_profile_only(s->f->dev_, kernel);
 }
 
 int chiml_sim_profile_read(chiml_sim_t* s, const char* kernel, double* total_ms, int* launches) {

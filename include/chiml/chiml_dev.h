@@ -159,10 +159,14 @@ int chiml_dev_read_dft(chiml_dev_t* ctx, double* reim, size_t cap);
 int chiml_dev_read_pops(chiml_dev_t* ctx, int q, int j, double* reim, size_t cap, size_t* nsamples);
 /* Density matrix of (group q, member m, emitter e): N*N complex column-major. */
 int chiml_dev_read_density(chiml_dev_t* ctx, int q, int m, int e, double* reim);
-/* Steps taken so far and the device stream (hipStream_t) used. */
+/* Steps taken so far. */
 int chiml_dev_steps_done(chiml_dev_t* ctx, int* steps);
-/* Per-step kernel timing: enables HIP events around the dominant kernel. */
+/* Kernel timing: HIP events on the context's stream around every launch
+ * (enable = 1), accumulated per kernel name until read; resets the totals. */
 int chiml_dev_profile(chiml_dev_t* ctx, int enable);
+/* As chiml_dev_profile(ctx, 1) but timing only the launches of one kernel,
+ * so that a timed run carries two event records per step at most. */
+int chiml_dev_profile_only(chiml_dev_t* ctx, const char* kernel);
 int chiml_dev_profile_read(chiml_dev_t* ctx, const char* kernel, double* total_ms, int* launches);
 /* Device memory the context holds (bytes). */
 int chiml_dev_footprint(chiml_dev_t* ctx, size_t* bytes);

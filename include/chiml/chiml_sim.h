@@ -44,7 +44,13 @@ int chiml_sim_density(chiml_sim_t* s, int q, int member, int emitter, double* re
 /* Cell-updates per step and the algorithmic HBM bytes per step of the field
  * kernels (update_h + update_e), used for the roofline figure. */
 int chiml_sim_cost(chiml_sim_t* s, double* cells, double* bytes_h, double* bytes_e, double* bytes_ml);
+/* Algorithmic HBM bytes per launch of one kernel ("update_h", "update_e",
+ * "update_h_pml", "update_e_pml", "ml_density", "dft_flush"); 0 for
+ * unknown names. */
+int chiml_sim_kernel_bytes(chiml_sim_t* s, const char* kernel, double* bytes);
+/* Per-kernel HIP-event timing (chiml_dev_profile / chiml_dev_profile_only). */
 int chiml_sim_profile(chiml_sim_t* s, int enable);
+int chiml_sim_profile_only(chiml_sim_t* s, const char* kernel);
 int chiml_sim_profile_read(chiml_sim_t* s, const char* kernel, double* total_ms, int* launches);
 int chiml_sim_footprint(chiml_sim_t* s, size_t* bytes);
 /* Estimated device bytes for a deck without creating it (admission control). */
