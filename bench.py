@@ -13,8 +13,9 @@ roofline: the dominant kernel's algorithmic bytes per launch (every state
 word it must read and write once: fields, material ids, CPML psi, pole
 currents) divided by its mean launch time, measured with HIP events on the
 simulation's stream around each of its launches in the timed region, against
-8 TB/s.  The dominant kernel is the one with the largest time per step in the
-warmup, where every launch is timed (the "kernels" breakdown).  traffic
+8 TB/s.  The dominant kernel is the one with the largest time per step in an
+untimed pass of up to 200 steps after the warmup, where every launch is timed
+(the "kernels" breakdown).  traffic
 comes from the committed rocprofv3 --pmc summary
 (profiles/traffic_config2_<precision>.json) when it is present.
 cpu_baseline: the FP64 CPU restatement of the reference step loop (the
@@ -56,18 +57,21 @@ def run_gpu(n, steps, warmup, precision):
     import chiml_amd
     from chiml_amd import decks
 
-    deck = decks.config2_3d(n=n, steps=warmup + steps, precision=precision)
+    prof = min(steps, 200)
+    deck = decks.config2_3d(n=n, steps=warmup + prof + steps, precision=precision)
     sim = chiml_amd.FDTDField(deck)
     cells = sim.cost()[0]
-    # warmup: every launch timed with HIP events -> per-kernel breakdown
-    sim.profile(True)
     sim.step(warmup)
+    sim.sync()
+    # untimed pass: every launch timed with HIP events -> per-kernel breakdown
+    sim.profile(True)
+    sim.step(prof)
     sim.sync()
     kern = {}
     for k in KERNELS:
         ms, nl = sim.profile_read(k)
         if nl > 0:
-            kern[k] = {"ms_per_launch": ms / nl, "launches": nl, "ms_per_step": ms / max(warmup, 1),
+            kern[k] = {"ms_per_launch": ms / nl, "launches": nl, "ms_per_step": ms / max(prof, 1),
                        "bytes_per_launch": sim.kernel_bytes(k)}
     cand = [k for k in kern if kern[k]["bytes_per_launch"] > 0]
     dominant = max(cand, key=lambda k: kern[k]["ms_per_step"]) if cand else "update_e"

@@ -298,27 +298,55 @@ __global__ void __launch_bounds__(256) k_update_e(StepArgs<T> a) {
 
 // ------------------------------------------------ 3D split kernels ---------
 // The 3D field update is split by region: an interior kernel over the box of
-// cells outside every CPML slab (no CPML code, two cells per thread with
-// 8/16-byte vector loads) and a slab kernel over the CPML cells only.  A
-// probe of this stencil shape on MI355X reached 4.9 TB/s with one cell per
-// thread and 5.5 TB/s with two; CPML branches inside interior waves were the
-// main loss of a single whole-domain kernel.
-template <typename T> struct V2;
-template <> struct V2<float> { using type = float2; };
-template <> struct V2<double> { using type = double2; };
-
+// cells outside every CPML slab (no CPML code) and a slab kernel over the
+// CPML cells only.  Both give each thread W consecutive x cells so that every
+// field, psi and material access of a lane is one 16-byte load or store
+// (W = 4 in FP32, 2 in FP64): on MI355X a plain copy reaches ~3.8 TB/s with
+// 4-byte lanes and ~6.3 TB/s with 16-byte lanes.  The caller guarantees
+// nx % W == 0 and Tx % W == 0, and the psi regions are widened to whole
+// slabs [0, T) and [n - T, n) (identity coefficients on the added positions),
+// so that the W cells of a group share their CPML membership on every axis.
 template <typename T>
 struct MatLds {
     T ce, inveps;
     int flags, poleset;
 };
 
-// Per-block flag: does this block's cell range touch any source box?
-template <typename T>
-__device__ __forceinline__ bool block_has_src(const StepArgs<T>& a, int cx0, int cx1, int cy0, int cy1, int cz0, int cz1) {
+template <typename T, int W>
+struct alignas(sizeof(T) * W) Pk {
+    T v[W];
+};
+
+template <typename T, int W>
+__device__ __forceinline__ Pk<T, W> ldv(const T* p) {
+    return *reinterpret_cast<const Pk<T, W>*>(p);
+}
+template <typename T, int W>
+__device__ __forceinline__ void stv(T* p, const Pk<T, W>& v) {
+    *reinterpret_cast<Pk<T, W>*>(p) = v;
+}
+template <typename T, int W>
+__device__ __forceinline__ Pk<T, W> zpk() {
+    Pk<T, W> r;
+#pragma unroll
+    for (int w = 0; w < W; ++w) r.v[w] = T(0);
+    return r;
+}
+
+// Blocks are dealt to the 8 XCDs round robin in dispatch order; renumber
+// them so that each XCD walks one contiguous eighth of the domain and the
+// y/z neighbour rows a block reads were loaded into the same L2.
+__device__ __forceinline__ long long xcd_block(long long b, long long nb) {
+    const long long q = nb / 8, r = nb % 8, x = b % 8, k = b / 8;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
+// Does the group [x0, x0 + W) x {y} x {z} touch any source box?
+template <typename T, int W>
+__device__ __forceinline__ bool group_has_src(const StepArgs<T>& a, int x0, int y, int z) {
     for (int s = 0; s < a.nsrc; ++s) {
         const int* b = a.src_box + 6 * s;
-        if (cx0 < b[1] && b[0] < cx1 && cy0 < b[3] && b[2] < cy1 && cz0 < b[5] && b[4] < cz1) return true;
+        if (x0 < b[1] && b[0] < x0 + W && y >= b[2] && y < b[3] && z >= b[4] && z < b[5]) return true;
     }
     return false;
 }
@@ -365,261 +393,336 @@ struct Box3 {
     int x0, x1, y0, y1, z0, z1;
 };
 
-// Interior H update, two cells (x, x+1) per thread; the caller guarantees
-// x0, x1 - x0 and nx even so that the pair is aligned.
-template <typename T, bool SRC>
-__global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b) {
-    using V = typename V2<T>::type;
-    const int x = b.x0 + 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = b.y0 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = b.z0 + blockIdx.z;
+// CPML term of derivative axis ax for the W cells of a group (pml_term for a
+// group): term = dV/(kappa d) + psi' with psi' = b psi + (c/d) dV on psi
+// positions, term = dV/d elsewhere.  psi is written back only for the cells
+// in `valid` (cells the reference does not update keep their psi).
+template <typename T, int W>
+__device__ __forceinline__ void pml_group(const PmlDev<T>& p, int ax, int pos0, int x0, int y, int z, int nx, int ny, const T (&dV)[W],
+                                          T invd, unsigned valid, T (&term)[W]) {
+    int s;
+    if (pos0 < p.lo[ax]) s = pos0;
+    else if (pos0 >= p.hi0[ax] && pos0 < p.hi1[ax]) s = p.lo[ax] + pos0 - p.hi0[ax];
+    else {
+#pragma unroll
+        for (int w = 0; w < W; ++w) term[w] = dV[w] * invd;
+        return;
+    }
+    long long pi;
+    if (ax == 0) pi = s + static_cast<long long>(p.S[0]) * (y + static_cast<long long>(ny) * z);
+    else if (ax == 1) pi = x0 + static_cast<long long>(nx) * (s + static_cast<long long>(p.S[1]) * z);
+    else pi = x0 + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * s);
+    Pk<T, W> ps = ldv<T, W>(p.psi[ax] + pi);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const int pos = ax == 0 ? pos0 + w : pos0;
+        const T psi = p.b[ax][pos] * ps.v[w] + p.cd[ax][pos] * dV[w];
+        if (valid & (1u << w)) ps.v[w] = psi;
+        term[w] = p.kinv[ax][pos] * (dV[w] * invd) + psi;
+    }
+    stv<T, W>(p.psi[ax] + pi, ps);
+}
+
+// H update of the group (x0..x0+W-1, y, z): H += ch (curl E) (+ sources).
+template <typename T, int W, bool PML, bool SRC>
+__device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int z) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    bool bsrc = false;
-    if (SRC) bsrc = block_has_src(a, b.x0 + 2 * blockIdx.x * blockDim.x, b.x0 + 2 * (blockIdx.x + 1) * blockDim.x,
-                                  b.y0 + blockIdx.y * blockDim.y, b.y0 + (blockIdx.y + 1) * blockDim.y, z, z + 1);
-    if (x >= b.x1 || y >= b.y1) return;
-    const int sy = nx;
-    const int sz = nx * ny;
-    const int i = x + nx * (y + ny * z);
-    const T *Ex = a.F[0], *Ey = a.F[1], *Ez = a.F[2];
-    const V ex = *reinterpret_cast<const V*>(Ex + i), ey = *reinterpret_cast<const V*>(Ey + i), ez = *reinterpret_cast<const V*>(Ez + i);
-    const bool zok = z < nz - 1, yok = y < ny - 1;
+    const long long sy = nx, sz = static_cast<long long>(nx) * ny;
+    const long long i = lidx(x0, y, z, nx, ny);
     const T idx = a.inv_d[0], idy = a.inv_d[1], idz = a.inv_d[2];
-    int step = 0;
-    if (SRC && bsrc) step = *a.step;
-    if (yok && zok) {  // Hx = Hx + ch[(Ey(z+1)-Ey)/dz - (Ez(y+1)-Ez)/dy]
-        const V eyz = *reinterpret_cast<const V*>(Ey + i + sz), ezy = *reinterpret_cast<const V*>(Ez + i + sy);
-        V h = *reinterpret_cast<const V*>(a.F[3] + i);
+    const T *Ex = a.F[0], *Ey = a.F[1], *Ez = a.F[2];
+    const Pk<T, W> ex = ldv<T, W>(Ex + i), ey = ldv<T, W>(Ey + i), ez = ldv<T, W>(Ez + i);
+    const bool last = x0 + W >= nx;
+    const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;  // cells with x < nx - 1
+    const bool src = SRC && group_has_src<T, W>(a, x0, y, z);
+    const int step = src ? *a.step : 0;
+    T d1[W], d2[W], t1[W], t2[W];
+    if (y < ny - 1 && z < nz - 1) {  // Hx += ch [dEy/dz - dEz/dy]
+        const Pk<T, W> eyz = ldv<T, W>(Ey + i + sz), ezy = ldv<T, W>(Ez + i + sy);
+        Pk<T, W> h = ldv<T, W>(a.F[3] + i);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = eyz.v[w] - ey.v[w];
+            d2[w] = ezy.v[w] - ez.v[w];
+        }
+        if (PML) {
+            pml_group<T, W>(a.pml[3], 2, z, x0, y, z, nx, ny, d1, idz, all, t1);
+            pml_group<T, W>(a.pml[3], 1, y, x0, y, z, nx, ny, d2, idy, all, t2);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                t1[w] = d1[w] * idz;
+                t2[w] = d2[w] * idy;
+            }
+        }
         const T ch = a.ch[0][0];
-        h.x += ch * ((eyz.x - ey.x) * idz - (ezy.x - ez.x) * idy);
-        h.y += ch * ((eyz.y - ey.y) * idz - (ezy.y - ez.y) * idy);
-        if (SRC && bsrc) {
-            h.x += src_value(a, 3, x, y, z, step);
-            h.y += src_value(a, 3, x + 1, y, z, step);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            T hv = h.v[w] + ch * (t1[w] - t2[w]);
+            if (src) hv += src_value(a, 3, x0 + w, y, z, step);
+            h.v[w] = hv;
         }
-        *reinterpret_cast<V*>(a.F[3] + i) = h;
+        stv<T, W>(a.F[3] + i, h);
     }
-    if (zok) {  // Hy = Hy + ch[(Ez(x+1)-Ez)/dx - (Ex(z+1)-Ex)/dz]
-        const V exz = *reinterpret_cast<const V*>(Ex + i + sz);
-        const T ez2 = Ez[i + 2 < nx * ny * nz ? i + 2 : i];
-        V h = *reinterpret_cast<const V*>(a.F[4] + i);
+    if (z < nz - 1) {  // Hy += ch [dEz/dx - dEx/dz], x < nx - 1
+        const Pk<T, W> exz = ldv<T, W>(Ex + i + sz);
+        const T ezn = last ? T(0) : Ez[i + W];
+        Pk<T, W> h = ldv<T, W>(a.F[4] + i);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = (w + 1 < W ? ez.v[w + 1] : ezn) - ez.v[w];
+            d2[w] = exz.v[w] - ex.v[w];
+        }
+        if (PML) {
+            pml_group<T, W>(a.pml[4], 0, x0, x0, y, z, nx, ny, d1, idx, vx, t1);
+            pml_group<T, W>(a.pml[4], 2, z, x0, y, z, nx, ny, d2, idz, vx, t2);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                t1[w] = d1[w] * idx;
+                t2[w] = d2[w] * idz;
+            }
+        }
         const T ch = a.ch[1][0];
-        const bool x1ok = x + 1 < nx - 1;
-        h.x += ch * ((ez.y - ez.x) * idx - (exz.x - ex.x) * idz);
-        if (x1ok) h.y += ch * ((ez2 - ez.y) * idx - (exz.y - ex.y) * idz);
-        if (SRC && bsrc) {
-            h.x += src_value(a, 4, x, y, z, step);
-            if (x1ok) h.y += src_value(a, 4, x + 1, y, z, step);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            if (!(vx & (1u << w))) continue;
+            T hv = h.v[w] + ch * (t1[w] - t2[w]);
+            if (src) hv += src_value(a, 4, x0 + w, y, z, step);
+            h.v[w] = hv;
         }
-        *reinterpret_cast<V*>(a.F[4] + i) = h;
+        stv<T, W>(a.F[4] + i, h);
     }
-    if (yok) {  // Hz = Hz + ch[(Ex(y+1)-Ex)/dy - (Ey(x+1)-Ey)/dx]
-        const V exy = *reinterpret_cast<const V*>(Ex + i + sy);
-        const T ey2 = Ey[i + 2 < nx * ny * nz ? i + 2 : i];
-        V h = *reinterpret_cast<const V*>(a.F[5] + i);
-        const T ch = a.ch[2][0];
-        const bool x1ok = x + 1 < nx - 1;
-        h.x += ch * ((exy.x - ex.x) * idy - (ey.y - ey.x) * idx);
-        if (x1ok) h.y += ch * ((exy.y - ex.y) * idy - (ey2 - ey.y) * idx);
-        if (SRC && bsrc) {
-            h.x += src_value(a, 5, x, y, z, step);
-            if (x1ok) h.y += src_value(a, 5, x + 1, y, z, step);
+    if (y < ny - 1) {  // Hz += ch [dEx/dy - dEy/dx], x < nx - 1
+        const Pk<T, W> exy = ldv<T, W>(Ex + i + sy);
+        const T eyn = last ? T(0) : Ey[i + W];
+        Pk<T, W> h = ldv<T, W>(a.F[5] + i);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = exy.v[w] - ex.v[w];
+            d2[w] = (w + 1 < W ? ey.v[w + 1] : eyn) - ey.v[w];
         }
-        *reinterpret_cast<V*>(a.F[5] + i) = h;
+        if (PML) {
+            pml_group<T, W>(a.pml[5], 1, y, x0, y, z, nx, ny, d1, idy, vx, t1);
+            pml_group<T, W>(a.pml[5], 0, x0, x0, y, z, nx, ny, d2, idx, vx, t2);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                t1[w] = d1[w] * idy;
+                t2[w] = d2[w] * idx;
+            }
+        }
+        const T ch = a.ch[2][0];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            if (!(vx & (1u << w))) continue;
+            T hv = h.v[w] + ch * (t1[w] - t2[w]);
+            if (src) hv += src_value(a, 5, x0 + w, y, z, step);
+            h.v[w] = hv;
+        }
+        stv<T, W>(a.F[5] + i, h);
     }
 }
 
-// Interior E update, two cells per thread.
-template <typename T, bool SRC>
+template <int W>
+__device__ __forceinline__ Pk<uint8_t, W> load_ids(const uint8_t* m, long long i) {
+    if (!m) {
+        Pk<uint8_t, W> z;
+#pragma unroll
+        for (int w = 0; w < W; ++w) z.v[w] = 0;
+        return z;
+    }
+    return *reinterpret_cast<const Pk<uint8_t, W>*>(m + i);
+}
+
+// E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.
+template <typename T, int W, bool PML, bool SRC>
+__device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], int x0, int y, int z) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const long long sy = nx, sz = static_cast<long long>(nx) * ny;
+    const long long i = lidx(x0, y, z, nx, ny);
+    const T idx = a.inv_d[0], idy = a.inv_d[1], idz = a.inv_d[2];
+    const T *Hx = a.F[3], *Hy = a.F[4], *Hz = a.F[5];
+    const Pk<T, W> hx = ldv<T, W>(Hx + i), hy = ldv<T, W>(Hy + i), hz = ldv<T, W>(Hz + i);
+    const bool last = x0 + W >= nx;
+    const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;
+    const bool src = SRC && group_has_src<T, W>(a, x0, y, z);
+    const int step = src ? *a.step : 0;
+    T d1[W], d2[W], t1[W], t2[W];
+    {  // Ex += ce [dHz/dy - dHy/dz], x < nx - 1
+        const Pk<T, W> hzy = y > 0 ? ldv<T, W>(Hz + i - sy) : zpk<T, W>();
+        const Pk<T, W> hyz = z > 0 ? ldv<T, W>(Hy + i - sz) : zpk<T, W>();
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = hz.v[w] - hzy.v[w];
+            d2[w] = hy.v[w] - hyz.v[w];
+        }
+        if (PML) {
+            pml_group<T, W>(a.pml[0], 1, y, x0, y, z, nx, ny, d1, idy, vx, t1);
+            pml_group<T, W>(a.pml[0], 2, z, x0, y, z, nx, ny, d2, idz, vx, t2);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                t1[w] = d1[w] * idy;
+                t2[w] = d2[w] * idz;
+            }
+        }
+        const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[0], i);
+        Pk<T, W> e = ldv<T, W>(a.F[0] + i);
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            if (vx & (1u << w))
+                e.v[w] = e_new(a, mt[0][id.v[w]], 0, x0 + w, y, z, e.v[w], t1[w] - t2[w], src ? src_value(a, 0, x0 + w, y, z, step) : T(0));
+        stv<T, W>(a.F[0] + i, e);
+    }
+    if (y < ny - 1) {  // Ey += ce [dHx/dz - dHz/dx]
+        const Pk<T, W> hxz = z > 0 ? ldv<T, W>(Hx + i - sz) : zpk<T, W>();
+        const T hzm = x0 > 0 ? Hz[i - 1] : T(0);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = hx.v[w] - hxz.v[w];
+            d2[w] = hz.v[w] - (w > 0 ? hz.v[w - 1] : hzm);
+        }
+        if (PML) {
+            pml_group<T, W>(a.pml[1], 2, z, x0, y, z, nx, ny, d1, idz, all, t1);
+            pml_group<T, W>(a.pml[1], 0, x0, x0, y, z, nx, ny, d2, idx, all, t2);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                t1[w] = d1[w] * idz;
+                t2[w] = d2[w] * idx;
+            }
+        }
+        const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[1], i);
+        Pk<T, W> e = ldv<T, W>(a.F[1] + i);
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            e.v[w] = e_new(a, mt[1][id.v[w]], 1, x0 + w, y, z, e.v[w], t1[w] - t2[w], src ? src_value(a, 1, x0 + w, y, z, step) : T(0));
+        stv<T, W>(a.F[1] + i, e);
+    }
+    if (z < nz - 1) {  // Ez += ce [dHy/dx - dHx/dy]
+        const Pk<T, W> hxy = y > 0 ? ldv<T, W>(Hx + i - sy) : zpk<T, W>();
+        const T hym = x0 > 0 ? Hy[i - 1] : T(0);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = hy.v[w] - (w > 0 ? hy.v[w - 1] : hym);
+            d2[w] = hx.v[w] - hxy.v[w];
+        }
+        if (PML) {
+            pml_group<T, W>(a.pml[2], 0, x0, x0, y, z, nx, ny, d1, idx, all, t1);
+            pml_group<T, W>(a.pml[2], 1, y, x0, y, z, nx, ny, d2, idy, all, t2);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                t1[w] = d1[w] * idx;
+                t2[w] = d2[w] * idy;
+            }
+        }
+        const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[2], i);
+        Pk<T, W> e = ldv<T, W>(a.F[2] + i);
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            e.v[w] = e_new(a, mt[2][id.v[w]], 2, x0 + w, y, z, e.v[w], t1[w] - t2[w], src ? src_value(a, 2, x0 + w, y, z, step) : T(0));
+        stv<T, W>(a.F[2] + i, e);
+    }
+}
+
+// Interior box, flattened over groups (x fastest), one group per thread.
+template <typename T, int W>
+__device__ __forceinline__ bool int_group(const Box3& b, long long t, int& x0, int& y, int& z) {
+    const int gw = (b.x1 - b.x0) / W, ny = b.y1 - b.y0;
+    const long long ng = static_cast<long long>(gw) * ny * (b.z1 - b.z0);
+    if (t >= ng) return false;
+    const long long r = t / gw;
+    x0 = b.x0 + static_cast<int>(t - r * gw) * W;
+    y = b.y0 + static_cast<int>(r % ny);
+    z = b.z0 + static_cast<int>(r / ny);
+    return true;
+}
+
+template <typename T, int W, bool SRC>
+__global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b) {
+    const long long blk = xcd_block(blockIdx.x, gridDim.x);
+    int x0, y, z;
+    if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z)) return;
+    h_group<T, W, false, SRC>(a, x0, y, z);
+}
+
+template <typename T, int W, bool SRC>
 __global__ void __launch_bounds__(256) k_e3_int(StepArgs<T> a, Box3 b, int nm0, int nm1, int nm2) {
-    using V = typename V2<T>::type;
     __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
     const int nm[3] = {nm0, nm1, nm2};
     load_mat(a, mt, nm);
-    const int x = b.x0 + 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = b.y0 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = b.z0 + blockIdx.z;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    bool bsrc = false;
-    if (SRC) bsrc = block_has_src(a, b.x0 + 2 * blockIdx.x * blockDim.x, b.x0 + 2 * (blockIdx.x + 1) * blockDim.x,
-                                  b.y0 + blockIdx.y * blockDim.y, b.y0 + (blockIdx.y + 1) * blockDim.y, z, z + 1);
-    if (x >= b.x1 || y >= b.y1) return;
-    const int sy = nx;
-    const int sz = nx * ny;
-    const int i = x + nx * (y + ny * z);
-    const T *Hx = a.F[3], *Hy = a.F[4], *Hz = a.F[5];
-    const V hx = *reinterpret_cast<const V*>(Hx + i), hy = *reinterpret_cast<const V*>(Hy + i), hz = *reinterpret_cast<const V*>(Hz + i);
-    const T idx = a.inv_d[0], idy = a.inv_d[1], idz = a.inv_d[2];
-    int step = 0;
-    if (SRC && bsrc) step = *a.step;
-    const T hzxm = x > 0 ? Hz[i - 1] : T(0);
-    const T hyxm = x > 0 ? Hy[i - 1] : T(0);
-    {  // Ex = Ex + ce[(Hz - Hz(y-1))/dy - (Hy - Hy(z-1))/dz]
-        const V hzy = y > 0 ? *reinterpret_cast<const V*>(Hz + i - sy) : V{0, 0};
-        const V hyz = z > 0 ? *reinterpret_cast<const V*>(Hy + i - sz) : V{0, 0};
-        const uchar2 id = a.mat_e[0] ? *reinterpret_cast<const uchar2*>(a.mat_e[0] + i) : uchar2{0, 0};
-        V e = *reinterpret_cast<const V*>(a.F[0] + i);
-        T s0 = 0, s1 = 0;
-        if (SRC && bsrc) {
-            s0 = src_value(a, 0, x, y, z, step);
-            s1 = src_value(a, 0, x + 1, y, z, step);
-        }
-        T acc0 = (hz.x - hzy.x) * idy - (hy.x - hyz.x) * idz;
-        T acc1 = (hz.y - hzy.y) * idy - (hy.y - hyz.y) * idz;
-        V o;
-        o.x = (x < nx - 1) ? e_new(a, mt[0][id.x], 0, x, y, z, e.x, acc0, s0) : e.x;
-        o.y = (x + 1 < nx - 1) ? e_new(a, mt[0][id.y], 0, x + 1, y, z, e.y, acc1, s1) : e.y;
-        *reinterpret_cast<V*>(a.F[0] + i) = o;
-    }
-    if (y < ny - 1) {  // Ey = Ey + ce[(Hx - Hx(z-1))/dz - (Hz - Hz(x-1))/dx]
-        const V hxz = z > 0 ? *reinterpret_cast<const V*>(Hx + i - sz) : V{0, 0};
-        const uchar2 id = a.mat_e[1] ? *reinterpret_cast<const uchar2*>(a.mat_e[1] + i) : uchar2{0, 0};
-        V e = *reinterpret_cast<const V*>(a.F[1] + i);
-        T s0 = 0, s1 = 0;
-        if (SRC && bsrc) {
-            s0 = src_value(a, 1, x, y, z, step);
-            s1 = src_value(a, 1, x + 1, y, z, step);
-        }
-        T acc0 = (hx.x - hxz.x) * idz - (hz.x - hzxm) * idx;
-        T acc1 = (hx.y - hxz.y) * idz - (hz.y - hz.x) * idx;
-        V o;
-        o.x = e_new(a, mt[1][id.x], 1, x, y, z, e.x, acc0, s0);
-        o.y = e_new(a, mt[1][id.y], 1, x + 1, y, z, e.y, acc1, s1);
-        *reinterpret_cast<V*>(a.F[1] + i) = o;
-    }
-    if (z < nz - 1) {  // Ez = Ez + ce[(Hy - Hy(x-1))/dx - (Hx - Hx(y-1))/dy]
-        const V hxy = y > 0 ? *reinterpret_cast<const V*>(Hx + i - sy) : V{0, 0};
-        const uchar2 id = a.mat_e[2] ? *reinterpret_cast<const uchar2*>(a.mat_e[2] + i) : uchar2{0, 0};
-        V e = *reinterpret_cast<const V*>(a.F[2] + i);
-        T s0 = 0, s1 = 0;
-        if (SRC && bsrc) {
-            s0 = src_value(a, 2, x, y, z, step);
-            s1 = src_value(a, 2, x + 1, y, z, step);
-        }
-        T acc0 = (hy.x - hyxm) * idx - (hx.x - hxy.x) * idy;
-        T acc1 = (hy.y - hy.x) * idx - (hx.y - hxy.y) * idy;
-        V o;
-        o.x = e_new(a, mt[2][id.x], 2, x, y, z, e.x, acc0, s0);
-        o.y = e_new(a, mt[2][id.y], 2, x + 1, y, z, e.y, acc1, s1);
-        *reinterpret_cast<V*>(a.F[2] + i) = o;
-    }
+    const long long blk = xcd_block(blockIdx.x, gridDim.x);
+    int x0, y, z;
+    if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z)) return;
+    e_group<T, W, false, SRC>(a, mt, x0, y, z);
 }
 
-// CPML slab cells: the union of the z slabs, the y slabs inside the z range
-// and the x slabs inside the y and z ranges, enumerated x fastest.
+// CPML slab groups: the union of the z slabs, the y slabs inside the z range
+// and the x slabs inside the y and z ranges, enumerated x fastest, in units
+// of W-cell groups (gx = nx / W groups per row, gT = Tx / W per x slab).
 struct SlabMap {
-    int nx, ny, nz, Tx, Ty, Tz;
-    long long nA, nB, nC;  // cell counts of the three regions
+    int nx, ny, nz, Tx, Ty, Tz, W;
+    long long nA, nB, nC;  // group counts of the three regions
 };
 
-__device__ __forceinline__ void slab_cell(const SlabMap& m, long long t, int& x, int& y, int& z) {
+__device__ __forceinline__ void slab_group(const SlabMap& m, long long t, int& x0, int& y, int& z) {
+    const int gx = m.nx / m.W;
     if (t < m.nA) {  // z slabs: z in [0,Tz) U [nz-Tz,nz), all x, y
-        const long long plane = static_cast<long long>(m.nx) * m.ny;
-        int k = static_cast<int>(t / plane);
-        long long r = t - k * plane;
+        const long long plane = static_cast<long long>(gx) * m.ny;
+        const int k = static_cast<int>(t / plane);
+        const long long r = t - k * plane;
         z = k < m.Tz ? k : m.nz - 2 * m.Tz + k;
-        y = static_cast<int>(r / m.nx);
-        x = static_cast<int>(r - static_cast<long long>(y) * m.nx);
+        y = static_cast<int>(r / gx);
+        x0 = static_cast<int>(r - static_cast<long long>(y) * gx) * m.W;
         return;
     }
     t -= m.nA;
     if (t < m.nB) {  // y slabs inside the z interior
-        const int nyb = 2 * m.Ty;
-        const long long plane = static_cast<long long>(m.nx) * nyb;
-        int k = static_cast<int>(t / plane);
-        long long r = t - k * plane;
+        const long long plane = static_cast<long long>(gx) * 2 * m.Ty;
+        const int k = static_cast<int>(t / plane);
+        const long long r = t - k * plane;
         z = m.Tz + k;
-        int yy = static_cast<int>(r / m.nx);
-        x = static_cast<int>(r - static_cast<long long>(yy) * m.nx);
+        const int yy = static_cast<int>(r / gx);
+        x0 = static_cast<int>(r - static_cast<long long>(yy) * gx) * m.W;
         y = yy < m.Ty ? yy : m.ny - 2 * m.Ty + yy;
         return;
     }
     t -= m.nB;
     {  // x slabs inside the y and z interiors
-        const int nxb = 2 * m.Tx;
+        const int gT = 2 * (m.Tx / m.W);
         const int nyi = m.ny - 2 * m.Ty;
-        const long long plane = static_cast<long long>(nxb) * nyi;
-        int k = static_cast<int>(t / plane);
-        long long r = t - k * plane;
+        const long long plane = static_cast<long long>(gT) * nyi;
+        const int k = static_cast<int>(t / plane);
+        const long long r = t - k * plane;
         z = m.Tz + k;
-        int yy = static_cast<int>(r / nxb);
-        int xx = static_cast<int>(r - static_cast<long long>(yy) * nxb);
+        const int yy = static_cast<int>(r / gT);
+        const int xx = static_cast<int>(r - static_cast<long long>(yy) * gT) * m.W;
         y = m.Ty + yy;
-        x = xx < m.Tx ? xx : m.nx - 2 * m.Tx + xx;
+        x0 = xx < m.Tx ? xx : m.nx - 2 * m.Tx + xx;
     }
 }
 
-template <typename T, bool HMAP, bool SRC>
+template <typename T, int W, bool SRC>
 __global__ void __launch_bounds__(256) k_h3_slab(StepArgs<T> a, SlabMap m) {
-    const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= m.nA + m.nB + m.nC) return;
-    int x, y, z;
-    slab_cell(m, t, x, y, z);
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const long long i = lidx(x, y, z, nx, ny);
-    const long long sy = nx, sz = static_cast<long long>(nx) * ny;
-    int step = 0;
-    if (SRC) step = *a.step;
-    const T *Ex = a.F[0], *Ey = a.F[1], *Ez = a.F[2];
-    if (y < ny - 1 && z < nz - 1) {
-        T dEy = Ey[i + sz] - Ey[i], dEz = Ez[i + sy] - Ez[i];
-        T acc = pml_term(a.pml[3], 2, z, x, y, z, nx, ny, dEy, dEy * a.inv_d[2]) - pml_term(a.pml[3], 1, y, x, y, z, nx, ny, dEz, dEz * a.inv_d[1]);
-        T ch = HMAP ? a.ch[0][a.mat_h[0][i]] : a.ch[0][0];
-        T h = a.F[3][i] + ch * acc;
-        if (SRC) h += src_value(a, 3, x, y, z, step);
-        a.F[3][i] = h;
-    }
-    if (x < nx - 1 && z < nz - 1) {
-        T dEz = Ez[i + 1] - Ez[i], dEx = Ex[i + sz] - Ex[i];
-        T acc = pml_term(a.pml[4], 0, x, x, y, z, nx, ny, dEz, dEz * a.inv_d[0]) - pml_term(a.pml[4], 2, z, x, y, z, nx, ny, dEx, dEx * a.inv_d[2]);
-        T ch = HMAP ? a.ch[1][a.mat_h[1][i]] : a.ch[1][0];
-        T h = a.F[4][i] + ch * acc;
-        if (SRC) h += src_value(a, 4, x, y, z, step);
-        a.F[4][i] = h;
-    }
-    if (x < nx - 1 && y < ny - 1) {
-        T dEx = Ex[i + sy] - Ex[i], dEy = Ey[i + 1] - Ey[i];
-        T acc = pml_term(a.pml[5], 1, y, x, y, z, nx, ny, dEx, dEx * a.inv_d[1]) - pml_term(a.pml[5], 0, x, x, y, z, nx, ny, dEy, dEy * a.inv_d[0]);
-        T ch = HMAP ? a.ch[2][a.mat_h[2][i]] : a.ch[2][0];
-        T h = a.F[5][i] + ch * acc;
-        if (SRC) h += src_value(a, 5, x, y, z, step);
-        a.F[5][i] = h;
-    }
+    int x0, y, z;
+    slab_group(m, t, x0, y, z);
+    h_group<T, W, true, SRC>(a, x0, y, z);
 }
 
-template <typename T, bool SRC>
+template <typename T, int W, bool SRC>
 __global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int nm0, int nm1, int nm2) {
     __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
     const int nm[3] = {nm0, nm1, nm2};
     load_mat(a, mt, nm);
-    const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= m.nA + m.nB + m.nC) return;
-    int x, y, z;
-    slab_cell(m, t, x, y, z);
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const long long i = lidx(x, y, z, nx, ny);
-    const long long sy = nx, sz = static_cast<long long>(nx) * ny;
-    int step = 0;
-    if (SRC) step = *a.step;
-    const T *Hx = a.F[3], *Hy = a.F[4], *Hz = a.F[5];
-    if (x < nx - 1) {
-        T dHz = Hz[i] - (y > 0 ? Hz[i - sy] : T(0)), dHy = Hy[i] - (z > 0 ? Hy[i - sz] : T(0));
-        T acc = pml_term(a.pml[0], 1, y, x, y, z, nx, ny, dHz, dHz * a.inv_d[1]) - pml_term(a.pml[0], 2, z, x, y, z, nx, ny, dHy, dHy * a.inv_d[2]);
-        const uint8_t id = a.mat_e[0] ? a.mat_e[0][i] : 0;
-        a.F[0][i] = e_new(a, mt[0][id], 0, x, y, z, a.F[0][i], acc, SRC ? src_value(a, 0, x, y, z, step) : T(0));
-    }
-    if (y < ny - 1) {
-        T dHx = Hx[i] - (z > 0 ? Hx[i - sz] : T(0)), dHz = Hz[i] - (x > 0 ? Hz[i - 1] : T(0));
-        T acc = pml_term(a.pml[1], 2, z, x, y, z, nx, ny, dHx, dHx * a.inv_d[2]) - pml_term(a.pml[1], 0, x, x, y, z, nx, ny, dHz, dHz * a.inv_d[0]);
-        const uint8_t id = a.mat_e[1] ? a.mat_e[1][i] : 0;
-        a.F[1][i] = e_new(a, mt[1][id], 1, x, y, z, a.F[1][i], acc, SRC ? src_value(a, 1, x, y, z, step) : T(0));
-    }
-    if (z < nz - 1) {
-        T dHy = Hy[i] - (x > 0 ? Hy[i - 1] : T(0)), dHx = Hx[i] - (y > 0 ? Hx[i - sy] : T(0));
-        T acc = pml_term(a.pml[2], 0, x, x, y, z, nx, ny, dHy, dHy * a.inv_d[0]) - pml_term(a.pml[2], 1, y, x, y, z, nx, ny, dHx, dHx * a.inv_d[1]);
-        const uint8_t id = a.mat_e[2] ? a.mat_e[2][i] : 0;
-        a.F[2][i] = e_new(a, mt[2][id], 2, x, y, z, a.F[2][i], acc, SRC ? src_value(a, 2, x, y, z, step) : T(0));
-    }
+    int x0, y, z;
+    slab_group(m, t, x0, y, z);
+    e_group<T, W, true, SRC>(a, mt, x0, y, z);
 }
 
 // ----------------------------------------------------- step counter ---------
@@ -1082,19 +1185,36 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         for (int ax = 0; ax < 3; ++ax) {
             pm.b[ax] = pm.cd[ax] = pm.kinv[ax] = nullptr;
             pm.psi[ax] = nullptr;
-            pm.lo[ax] = p->psi_lo[comp][ax];
-            pm.hi0[ax] = p->psi_hi0[comp][ax];
-            pm.hi1[ax] = p->psi_hi1[comp][ax];
-            pm.S[ax] = pm.lo[ax] + (pm.hi1[ax] - pm.hi0[ax]);
             if (!p->pml_b[comp][ax]) {
                 pm.lo[ax] = 0;
                 pm.hi0[ax] = pm.hi1[ax] = nn[ax];
                 pm.S[ax] = 0;
                 continue;
             }
-            pm.b[ax] = upload_cast<T>(c, p->pml_b[comp][ax], nn[ax]);
-            pm.cd[ax] = upload_cast<T>(c, p->pml_cd[comp][ax], nn[ax]);
-            pm.kinv[ax] = upload_cast<T>(c, p->pml_kinv[comp][ax], nn[ax]);
+            // Widen the psi region to whole slabs [0, T) and [n - T, n): the
+            // added positions carry b = 0, c/d = 0, 1/kappa = 1, for which the
+            // CPML term equals the plain dV/d bit for bit and psi stays 0.
+            const int n = nn[ax], T0 = p->pml_T[ax];
+            const int lo = p->psi_lo[comp][ax], hi0 = p->psi_hi0[comp][ax], hi1 = p->psi_hi1[comp][ax];
+            pm.lo[ax] = std::max(lo, T0);
+            pm.hi0[ax] = std::min(hi0, n - T0);
+            pm.hi1[ax] = std::max(hi1, n);
+            pm.S[ax] = pm.lo[ax] + (pm.hi1[ax] - pm.hi0[ax]);
+            std::vector<double> bb(p->pml_b[comp][ax], p->pml_b[comp][ax] + n), cc(p->pml_cd[comp][ax], p->pml_cd[comp][ax] + n),
+                kk(p->pml_kinv[comp][ax], p->pml_kinv[comp][ax] + n);
+            auto ident = [&](int q0, int q1) {
+                for (int q = std::max(q0, 0); q < std::min(q1, n); ++q) {
+                    bb[q] = 0.0;
+                    cc[q] = 0.0;
+                    kk[q] = 1.0;
+                }
+            };
+            ident(lo, pm.lo[ax]);
+            ident(pm.hi0[ax], hi0);
+            ident(hi1, pm.hi1[ax]);
+            pm.b[ax] = upload_cast<T>(c, bb.data(), n);
+            pm.cd[ax] = upload_cast<T>(c, cc.data(), n);
+            pm.kinv[ax] = upload_cast<T>(c, kk.data(), n);
             size_t ps = static_cast<size_t>(pm.S[ax]);
             for (int o = 0; o < 3; ++o)
                 if (o != ax) ps *= nn[o];
@@ -1252,30 +1372,30 @@ void launch_flush(chiml_dev* c, int force) {
 template <typename T, int DIM>
 void launch_fields(chiml_dev* c, StepArgs<T>& a) {
     if (DIM == 3 && c->split) {
+        constexpr int W = 16 / sizeof(T);
         const Box3 b = c->ibox;
-        dim3 blk(64, 4, 1);
-        dim3 gi(((b.x1 - b.x0) / 2 + 63) / 64, (b.y1 - b.y0 + 3) / 4, b.z1 - b.z0);
+        const long long ni = static_cast<long long>((b.x1 - b.x0) / W) * (b.y1 - b.y0) * (b.z1 - b.z0);
         const long long ns = c->slab.nA + c->slab.nB + c->slab.nC;
-        dim3 gs(static_cast<unsigned>((ns + 255) / 256));
+        const dim3 gi(static_cast<unsigned>((ni + 255) / 256)), gs(static_cast<unsigned>((ns + 255) / 256)), blk(256);
         {
             Timed t(c, "update_h");
-            if (c->has_src) hipLaunchKernelGGL((k_h3_int<T, true>), gi, blk, 0, c->stream, a, b);
-            else hipLaunchKernelGGL((k_h3_int<T, false>), gi, blk, 0, c->stream, a, b);
+            if (c->has_src) hipLaunchKernelGGL((k_h3_int<T, W, true>), gi, blk, 0, c->stream, a, b);
+            else hipLaunchKernelGGL((k_h3_int<T, W, false>), gi, blk, 0, c->stream, a, b);
         }
         if (ns > 0) {
             Timed t(c, "update_h_pml");
-            if (c->has_src) hipLaunchKernelGGL((k_h3_slab<T, false, true>), gs, dim3(256), 0, c->stream, a, c->slab);
-            else hipLaunchKernelGGL((k_h3_slab<T, false, false>), gs, dim3(256), 0, c->stream, a, c->slab);
+            if (c->has_src) hipLaunchKernelGGL((k_h3_slab<T, W, true>), gs, blk, 0, c->stream, a, c->slab);
+            else hipLaunchKernelGGL((k_h3_slab<T, W, false>), gs, blk, 0, c->stream, a, c->slab);
         }
         {
             Timed t(c, "update_e");
-            if (c->has_src) hipLaunchKernelGGL((k_e3_int<T, true>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
-            else hipLaunchKernelGGL((k_e3_int<T, false>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
+            if (c->has_src) hipLaunchKernelGGL((k_e3_int<T, W, true>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
+            else hipLaunchKernelGGL((k_e3_int<T, W, false>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
         }
         if (ns > 0) {
             Timed t(c, "update_e_pml");
-            if (c->has_src) hipLaunchKernelGGL((k_e3_slab<T, true>), gs, dim3(256), 0, c->stream, a, c->slab, c->nmat[0], c->nmat[1], c->nmat[2]);
-            else hipLaunchKernelGGL((k_e3_slab<T, false>), gs, dim3(256), 0, c->stream, a, c->slab, c->nmat[0], c->nmat[1], c->nmat[2]);
+            if (c->has_src) hipLaunchKernelGGL((k_e3_slab<T, W, true>), gs, blk, 0, c->stream, a, c->slab, c->nmat[0], c->nmat[1], c->nmat[2]);
+            else hipLaunchKernelGGL((k_e3_slab<T, W, false>), gs, blk, 0, c->stream, a, c->slab, c->nmat[0], c->nmat[1], c->nmat[2]);
         }
         return;
     }
@@ -1389,17 +1509,20 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
         if (c->dim == 3) {
             const int Tx = p->pml_T[0], Ty = p->pml_T[1], Tz = p->pml_T[2];
             Box3 b{Tx, p->nx - Tx, Ty, p->ny - Ty, Tz, p->nz - Tz};
-            // the interior kernel needs an even-aligned, even-width x range and no magnetic map
-            if (b.x1 > b.x0 && b.y1 > b.y0 && b.z1 > b.z0 && (b.x0 % 2) == 0 && ((b.x1 - b.x0) % 2) == 0 && (p->nx % 2) == 0 &&
-                !c->hmap) {
+            // the split kernels take W = 16 / sizeof(T) cells per thread along x:
+            // x rows and x slabs must hold whole groups, and there is no magnetic map
+            const int W = c->prec == 32 ? 4 : 2;
+            if (b.x1 > b.x0 && b.y1 > b.y0 && b.z1 > b.z0 && (Tx % W) == 0 && (p->nx % W) == 0 && !c->hmap) {
                 c->split = true;
                 c->ibox = b;
                 SlabMap& m = c->slab;
                 m.nx = p->nx; m.ny = p->ny; m.nz = p->nz;
                 m.Tx = Tx; m.Ty = Ty; m.Tz = Tz;
-                m.nA = 2LL * Tz * p->nx * p->ny;
-                m.nB = 2LL * Ty * p->nx * (p->nz - 2 * Tz);
-                m.nC = 2LL * Tx * (p->ny - 2 * Ty) * (p->nz - 2 * Tz);
+                m.W = W;
+                const long long gx = p->nx / W;
+                m.nA = 2LL * Tz * gx * p->ny;
+                m.nB = 2LL * Ty * gx * (p->nz - 2 * Tz);
+                m.nC = 2LL * (Tx / W) * (p->ny - 2 * Ty) * (p->nz - 2 * Tz);
             }
         }
         // t = 0 samples (taken by the reference's constructor, parallelFDTDField.cpp:832-837)

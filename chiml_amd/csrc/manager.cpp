@@ -845,7 +845,8 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         const double nE = is3D ? 3.0 : 1.0, nH = is3D ? 3.0 : 2.0;
         cells = static_cast<double>(N);
         // interior box of the 3D split kernels (see device.hip); 2D uses one kernel each
-        const bool split = is3D && T[0] < nx - T[0] && T[1] < ny - T[1] && T[2] < nz - T[2] && (T[0] % 2) == 0 && (nx % 2) == 0;
+        const int W = IP.fp32 ? 4 : 2;  // x cells per thread of the split kernels
+        const bool split = is3D && T[0] < nx - T[0] && T[1] < ny - T[1] && T[2] < nz - T[2] && (T[0] % W) == 0 && (nx % W) == 0;
         const double Ni = split ? static_cast<double>(nx - 2 * T[0]) * (ny - 2 * T[1]) * (nz - 2 * T[2]) : cells;
         const double Ns = cells - Ni;
         auto inInt = [&](int x, int y, int z) {

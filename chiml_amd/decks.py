@@ -66,6 +66,13 @@ def _cell(n3, d, courant, steps, pml, precision, extra=None):
     return deck, size, dt
 
 
+def empty_deck(n3, d, steps, pml, precision="fp64", courant=0.5):
+    """Deck dict with the computational cell only (no sources or objects):
+    n3 points per axis (n3[2] = 1 for 2D TMz), uniform step d, pml cells."""
+    deck, _, _ = _cell(n3, d, courant, steps, pml, precision)
+    return deck
+
+
 def gaussian_pulse(fcen, fwidth, cutoff, field):
     """Gaussian pulse centred at t_0 = cutoff/fwidth.  t_0 is given explicitly:
     the reference's default is cutoff/fwidth**2 (parallelInputs.cpp:139)."""

@@ -506,7 +506,7 @@ struct Sim {
     void outputDetectors();
     void fluxFieldIn();
     void addQE();
-    void updateDensity(QE& q);
+    void updateDensity(QE& q, bool gather = true);
     void PCABAM4(QE& q, QE::Member& mem, QEDensity& den, std::vector<cplx>& H);
     void denDeriv(QE& q, const std::vector<cplx>& H, const cplx* den, cplx* out);
     void getHam(QE::Member& mem, const cplx Ex, const cplx Ey, const cplx Ez, std::vector<cplx>& H);
@@ -1328,8 +1328,8 @@ void Sim::PCABAM4(QE& q, QE::Member& mem, QEDensity& d, std::vector<cplx>& H) {
 // updateDensity, parallelQE.hpp:614-678, for one rank.  The field at an
 // emitter point p is Ez(p) in 2D (getE_TM) and the mean of the two Yee edges
 // E_c(p) and E_c(p - e_c) in 3D (getE_TE).
-void Sim::updateDensity(QE& q) {
-    for (int c = 0; c < 3; ++c) {
+void Sim::updateDensity(QE& q, bool gather) {
+    for (int c = 0; c < 3 && gather; ++c) {
         if (!E[c].ok()) continue;
         for (int z = 0; z < q.nbox[2]; ++z)
             for (int y = 0; y < q.nbox[1]; ++y)
@@ -1598,6 +1598,50 @@ int oracle_density(void* hp, int q, int m, int e, double* re, double* im) {
     }
     return Q.N;
 }
+
+// Test hooks (not reference behaviour): overwrite a field's interior points,
+// an emitter's density, or integrate one emitter group under a constant
+// field without stepping the grid -- the initial-value problems the closed-
+// form checks need.
+int oracle_set_field(void* hp, int comp, const double* in) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    orc::DGrid& g = comp < 3 ? S.E[comp] : S.H[comp - 3];
+    if (!g.ok()) return -1;
+    int nz = S.is3D ? S.n[2] : 1;
+    size_t k = 0;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < S.n[1]; ++y)
+            for (int x = 0; x < S.n[0]; ++x) g.at(x + 1, y + 1, S.is3D ? z + 1 : 0) = in[k++];
+    return 0;
+}
+
+int oracle_set_density(void* hp, int q, int m, int e, const double* re, const double* im) {
+    auto& Q = static_cast<OracleHandle*>(hp)->sim->qes.at(q);
+    auto& d = Q.members.at(m).den.at(e);
+    for (int i = 0; i < Q.N * Q.N; ++i) d.rho[i] = orc::cplx(re[i], im[i]);
+    return 0;
+}
+
+int oracle_qe_drive(void* hp, int q, const double* E3, int nsteps) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    auto& Q = S.qes.at(q);
+    for (int c = 0; c < 3; ++c) std::fill(Q.e[c].begin(), Q.e[c].end(), orc::cplx(E3[c], 0.0));
+    for (int i = 0; i < nsteps; ++i) S.updateDensity(Q, false);
+    return 0;
+}
+
+// Relaxation superoperator of group q as applied per step: out[i] += sum_j G[i][j] rho[j]
+// (G is n^2 x n^2 dense, row-major, written to gam).
+int oracle_qe_gamma(void* hp, int q, double* gam) {
+    auto& Q = static_cast<OracleHandle*>(hp)->sim->qes.at(q);
+    int S2 = Q.N * Q.N;
+    std::fill(gam, gam + static_cast<size_t>(S2) * S2, 0.0);
+    for (int i = 0; i < static_cast<int>(Q.gam.size()); ++i)
+        for (auto& kv : Q.gam[i]) gam[static_cast<size_t>(i) * S2 + kv.first] = kv.second;
+    return Q.N;
+}
+
+double oracle_hbar(void* hp, int q) { return static_cast<OracleHandle*>(hp)->sim->qes.at(q).hbar; }
 
 double oracle_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) { return orc::wigner3j(j1, j2, j3, m1, m2, m3); }
 

@@ -37,6 +37,13 @@ def lib():
                                   ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.oracle_density.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.oracle_set_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_set_density.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.oracle_qe_drive.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.oracle_qe_gamma.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_hbar.restype = ctypes.c_double
+        L.oracle_hbar.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_wigner3j.restype = ctypes.c_double
         L.oracle_wigner3j.argtypes = [ctypes.c_int] * 6
         L.oracle_pulse.restype = ctypes.c_double
@@ -108,6 +115,30 @@ class Oracle:
         im = np.zeros(nmax)
         n = lib().oracle_pops(self._h, q, lev, _dp(re), _dp(im), nmax)
         return re[:n] + 1j * im[:n]
+
+    def set_field(self, comp, arr):
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).reshape(-1))
+        if a.size != int(np.prod(self.shape)) or lib().oracle_set_field(self._h, int(comp), _dp(a)) != 0:
+            raise ValueError("set_field: bad component or size")
+
+    def set_density(self, q, member, emitter, rho):
+        rho = np.asarray(rho, dtype=np.complex128)
+        re = np.ascontiguousarray(rho.real.reshape(-1, order="F"))
+        im = np.ascontiguousarray(rho.imag.reshape(-1, order="F"))
+        lib().oracle_set_density(self._h, q, member, emitter, _dp(re), _dp(im))
+
+    def qe_drive(self, q, e3, nsteps):
+        """Integrates group q's densities nsteps times under a constant field e3 (no grid step)."""
+        e = np.ascontiguousarray(np.asarray(e3, dtype=np.float64))
+        lib().oracle_qe_drive(self._h, q, _dp(e), int(nsteps))
+
+    def qe_gamma(self, q, nlev):
+        g = np.zeros(nlev ** 4)
+        lib().oracle_qe_gamma(self._h, q, _dp(g))
+        return g.reshape(nlev * nlev, nlev * nlev)
+
+    def hbar(self, q):
+        return lib().oracle_hbar(self._h, q)
 
     def density(self, q, member, emitter, nlev):
         re = np.zeros(nlev * nlev)

@@ -86,3 +86,34 @@ def test_2d_ml_two_level_fp64():
         po = orc.pops(0, j)
         assert pg.shape == po.shape, (pg.shape, po.shape)
         assert rel_linf(pg, po) <= 1e-10
+
+
+def _split_deck(variant, precision):
+    """Decks whose 3D updates take the interior + CPML-slab kernels (x PML
+    thickness and nx multiples of 4), with variants that move cells, sources
+    and slabs around."""
+    d = json.loads(decks.config2_3d(n=32, steps=100, pml=8, d=0.02, sphere_cells=5, flux_cells=14, nfreq=6,
+                                    precision=precision))
+    dx = 0.02
+    if variant == "wide":  # nx = 40, ny = nz = 32
+        d["CompCell"]["size"][0] = 39 * dx
+    elif variant == "zfree":  # no CPML along z: no z slabs
+        d["PML"]["thickness"][2] = 0.0
+    elif variant == "pmlsrc":  # extra H point sources in the low-x slab and in a corner
+        # (E sources on CPML cells filled with material land on D and are dropped, as in the reference)
+        for pol, loc in (("Hy", [-14 * dx, 0.0, 0.0]), ("Hz", [-13 * dx, 13 * dx, -14 * dx])):
+            d["SourceList"].append({"pol": pol, "size": [0.0, 0.0, 0.0], "loc": loc,
+                                    "PulseList": [decks.gaussian_pulse(5.0, 5.0, 3.0, 1.0)]})
+    return json.dumps(d)
+
+
+@pytest.mark.parametrize("variant", ["cube", "wide", "zfree", "pmlsrc"])
+@pytest.mark.parametrize("precision,tol_f,tol_flux", [("fp64", FP64_TOL, 1e-8), ("fp32", 1e-4, 1e-5)])
+def test_3d_split_kernels(variant, precision, tol_f, tol_flux):
+    deck = _split_deck(variant, precision)
+    print(f"split variant={variant} precision={precision}", flush=True)
+    sim, orc = _pair(deck, 100)
+    _fields(sim, orc, range(6), tol_f)
+    err = rel_linf(sim.fluxArr()[0].getFlux(), orc.flux(0))
+    print("flux rel", err, flush=True)
+    assert err <= tol_flux

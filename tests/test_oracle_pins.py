@@ -1,0 +1,454 @@
+"""Pins of the FP64 oracle against what mathematics and physics fix.
+
+The oracle restates the reference's time-stepping loop; these tests check it
+against closed forms and invariants chosen so that a plausible mistake (a
+wrong sign, index offset, transposed operator, missing term) fails one of
+them.  No GPU is used.
+"""
+import json
+import math
+
+import numpy as np
+import pytest
+
+from chiml_amd import decks
+from oracle.pyoracle import Oracle, pulse, wigner3j
+
+
+# ------------------------------------------------------------------ 3j ------
+@pytest.mark.parametrize("args,value", [
+    ((0, 1, 1, 0, 0, 0), -1.0 / math.sqrt(3.0)),
+    ((1, 1, 0, 0, 0, 0), -1.0 / math.sqrt(3.0)),
+    ((1, 1, 2, 0, 0, 0), math.sqrt(2.0 / 15.0)),
+    ((2, 1, 1, 0, 0, 0), math.sqrt(2.0 / 15.0)),
+    ((1, 1, 1, 1, -1, 0), 1.0 / math.sqrt(6.0)),
+    ((1, 1, 0, 1, -1, 0), 1.0 / math.sqrt(3.0)),
+    ((1, 1, 1, 0, 0, 0), 0.0),  # odd j1+j2+j3 with all m = 0
+    ((1, 1, 1, 1, 1, 0), 0.0),  # m1+m2+m3 != 0
+    ((1, 1, 3, 0, 0, 0), 0.0),  # triangle rule
+    ((2, 2, 2, 0, 0, 0), -math.sqrt(2.0 / 35.0)),
+])
+def test_wigner3j_textbook_values(args, value):
+    assert wigner3j(*args) == pytest.approx(value, abs=1e-14)
+
+
+def test_wigner3j_orthogonality():
+    # sum_{m1,m2} (2 j3 + 1) (j1 j2 j3; m1 m2 m3)(j1 j2 j3'; m1 m2 m3') = delta
+    j1, j2 = 2, 1
+    for j3 in range(1, 4):
+        for j3p in range(1, 4):
+            for m3 in range(-min(j3, j3p), min(j3, j3p) + 1):
+                s = 0.0
+                for m1 in range(-j1, j1 + 1):
+                    for m2 in range(-j2, j2 + 1):
+                        s += wigner3j(j1, j2, j3, m1, m2, m3) * wigner3j(j1, j2, j3p, m1, m2, m3)
+                assert (2 * j3 + 1) * s == pytest.approx(1.0 if j3 == j3p else 0.0, abs=1e-13)
+
+
+# --------------------------------------------------------------- pulses -----
+def _source_deck(pulse_spec):
+    deck = decks.empty_deck((16, 16, 1), 0.05, 10, 2)
+    deck["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "loc": [0, 0, 0], "PulseList": [pulse_spec]})
+    return json.dumps(deck)
+
+
+def test_ricker_pulse_is_zero_for_nonnegative_time():
+    """The reference's Ricker turn-off test reduces to t < 0 (its cutoff ratio is
+    purely imaginary), so the pulse is identically zero on the time axis."""
+    deck = _source_deck({"profile": "ricker", "fcen": 0.2, "fwidth": 1.0, "cutoff": 4.0, "Field_Intensity": 1.0})
+    assert all(pulse(deck, 0, t) == 0.0 for t in np.linspace(0.0, 200.0, 4001))
+    run = json.loads(decks.config1_2d(n=48, steps=60, profile="ricker", pml=8, cyl_radius_cells=8,
+                                      src_cell=(16, 24), ndet=2))
+    o = Oracle(json.dumps(run))
+    o.step(60)
+    for c in (2, 3, 4):
+        assert np.all(o.field(c) == 0.0)
+
+
+def test_gaussian_pulse_envelope_and_turn_off():
+    fcen, fw, cutoff, t0, e0 = 0.2, 0.05, 3.0, 30.0, 2.5
+    deck = _source_deck({"profile": "gaussian", "fcen": fcen, "fwidth": fw, "cutoff": cutoff, "t_0": t0,
+                         "Field_Intensity": e0})
+    ref = pulse(deck, 0, t0) / math.cos(2 * math.pi * fcen * t0)
+    for t in np.linspace(0.0, t0 + cutoff / fw - 1e-9, 301):
+        carrier = math.cos(2 * math.pi * fcen * t)
+        if abs(carrier) < 0.2:
+            continue
+        env = pulse(deck, 0, t) / carrier / ref
+        assert env == pytest.approx(math.exp(-((t - t0) * fw) ** 2 / 2.0), rel=1e-12, abs=1e-300)
+    for t in (t0 + cutoff / fw + 1e-6, t0 + cutoff / fw + 10.0):
+        assert pulse(deck, 0, t) == 0.0
+
+
+def test_gaussian_default_t0_is_cutoff_over_fwidth_squared():
+    """Without t_0 the reference centres the pulse at cutoff/fwidth^2."""
+    fcen, fw, cutoff = 0.2, 0.5, 3.0
+    deck = _source_deck({"profile": "gaussian", "fcen": fcen, "fwidth": fw, "cutoff": cutoff, "Field_Intensity": 1.0})
+    t0 = cutoff / fw ** 2
+    env = [abs(pulse(deck, 0, t) / math.cos(2 * math.pi * fcen * t)) for t in (t0 - 0.7, t0 + 0.7)]
+    assert env[0] == pytest.approx(env[1], rel=1e-10)
+
+
+# ---------------------------------------------------- grid and time step ----
+def test_time_step_2d_includes_z_term_and_point_count():
+    d = 0.05
+    deck = decks.empty_deck((33, 20, 1), d, 5, 4)
+    o = Oracle(json.dumps(deck))
+    assert o.dt == pytest.approx(0.5 * d / math.sqrt(3.0), rel=1e-15)
+    assert o.shape == (1, 20, 33)
+    deck["CompCell"]["size"] = [32 * d + 0.49 * d, 19 * d - 0.49 * d, 0.0]
+    o = Oracle(json.dumps(deck))
+    assert o.shape == (1, 20, 33)
+
+
+def test_step_count_is_ceil_tlim_over_dt():
+    d = 0.05
+    deck = decks.empty_deck((16, 16, 1), d, 5, 2)
+    dt = 0.5 * d / math.sqrt(3.0)
+    deck["CompCell"]["tLim"] = 7.2 * dt
+    assert Oracle(json.dumps(deck)).nsteps == 8
+    deck["CompCell"]["tLim"] = 7.0 * dt * (1 - 1e-12)
+    assert Oracle(json.dumps(deck)).nsteps == 7
+
+
+# ----------------------------------------------------- Yee plane waves ------
+def _leapfrog_amplitude(theta, n):
+    """E-mode amplitude after n steps from E = 1, H = 0 at t = -dt/2."""
+    return math.cos(n * theta + theta / 2.0) / math.cos(theta / 2.0)
+
+
+def _center_ratio(f0, fn, sl):
+    a, b = f0[sl].ravel(), fn[sl].ravel()
+    r = float(a @ b / (a @ a))
+    return r, float(np.max(np.abs(b - r * a)))
+
+
+def test_yee_2d_plane_wave_discrete_dispersion():
+    n, pml, d, steps = 72, 6, 0.05, 14
+    deck = decks.empty_deck((n, n, 1), d, steps + 1, pml)
+    o = Oracle(json.dumps(deck))
+    kx, ky = 2 * math.pi / (9.3 * d), 2 * math.pi / (13.7 * d)
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ez0 = np.cos(kx * i * d + ky * j * d + 0.3)
+    o.set_field(2, ez0.reshape(1, n, n))
+    s = o.dt ** 2 * (4 * math.sin(kx * d / 2) ** 2 / d ** 2 + 4 * math.sin(ky * d / 2) ** 2 / d ** 2)
+    theta = math.acos(1 - s / 2)
+    m = pml + steps + 3
+    sl = (0, slice(m, n - m), slice(m, n - m))
+    for step in range(1, steps + 1):
+        o.step(1)
+        r, res = _center_ratio(ez0.reshape(1, n, n), o.field(2), sl)
+        assert r == pytest.approx(_leapfrog_amplitude(theta, step), abs=1e-12)
+        assert res < 1e-12
+
+
+@pytest.mark.parametrize("comp,eps", [(0, 2.0), (1, 2.25)])
+def test_yee_3d_plane_wave_anisotropic(comp, eps):
+    """A transverse wave polarised along axis `comp` in a diagonal-eps medium
+    advances with s = dt^2 K^2 / eps_comp."""
+    n, pml, d, steps = 40, 4, 0.02, 8
+    deck = decks.empty_deck((n, n, n), d, steps + 1, pml)
+    size = [(n - 1) * d] * 3
+    deck["ObjectList"].append({"shape": "block", "material": "custom", "eps": 2.25, "eps_diag": [2.0, 2.25, 2.5],
+                               "pols": [], "size": size, "loc": [0, 0, 0], "Basis_Set": []})
+    o = Oracle(json.dumps(deck))
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ka, kb = 2 * math.pi / (7.1 * d), 2 * math.pi / (11.3 * d)
+    if comp == 0:  # Ex(y, z)
+        f0 = np.cos(ka * y * d + kb * z * d + 0.2)
+    else:  # Ey(x, z)
+        f0 = np.cos(ka * x * d + kb * z * d + 0.2)
+    o.set_field(comp, f0)
+    s = o.dt ** 2 / eps * (4 * math.sin(ka * d / 2) ** 2 + 4 * math.sin(kb * d / 2) ** 2) / d ** 2
+    theta = math.acos(1 - s / 2)
+    m = pml + steps + 3
+    sl = (slice(m, n - m),) * 3
+    for step in range(1, steps + 1):
+        o.step(1)
+        r, res = _center_ratio(f0, o.field(comp), sl)
+        assert r == pytest.approx(_leapfrog_amplitude(theta, step), abs=1e-12)
+        assert res < 1e-12
+
+
+# ------------------------------------------------------ ADE recurrence ------
+def _char_poly(eps_inf, poles, s):
+    """Characteristic polynomial (coefficients, highest power first) of the
+    E-mode sequence of a Lorentz/Drude medium on the Yee grid.
+
+    Pole p: P'' + 2 g P' + w^2 P = sig w^2 E, central differences at step n
+    -> P^{n+1} = a P^n + x P^{n-1} + c E^n.  With D = eps_inf E + sum P and
+    D^{n+1} - 2 D^n + D^{n-1} = -s E^n the z-transform gives
+    (z - 1)^2 (eps_inf prod Q + sum_p c_p z prod_{r!=p} Q_r) + s z prod Q = 0,
+    Q_p = z^2 - a_p z - x_p.
+    """
+    Q = []
+    C = []
+    for (w, g, sig, dt) in poles:
+        a = (2 - (w * dt) ** 2) / (1 + g * dt)
+        x = (g * dt - 1) / (1 + g * dt)
+        C.append(sig * (w * dt) ** 2 / (1 + g * dt))
+        Q.append(np.array([1.0, -a, -x]))
+    prodQ = np.array([1.0])
+    for q in Q:
+        prodQ = np.polymul(prodQ, q)
+    eps_term = eps_inf * prodQ
+    for p, c in enumerate(C):
+        other = np.array([1.0])
+        for r, q in enumerate(Q):
+            if r != p:
+                other = np.polymul(other, q)
+        eps_term = np.polyadd(eps_term, c * np.polymul([1.0, 0.0], other))
+    lhs = np.polymul(np.polymul([1.0, -1.0], [1.0, -1.0]), eps_term)
+    return np.polyadd(lhs, s * np.polymul([1.0, 0.0], prodQ))
+
+
+def test_ade_drude_lorentz_mode_recurrence():
+    n, pml, d, steps = 120, 6, 0.05, 40
+    deck = decks.empty_deck((n, n, 1), d, steps + 1, pml)
+    w_l, g_l, s_l = 0.6, 0.05, 1.5  # Lorentz: omega, gamma inputs (omega*2pi, gamma*pi applied by the parser)
+    w_d, g_d, s_d = 1e-6, 0.02, (0.9 / 1e-6) ** 2  # Drude: tiny omega, huge sigma
+    pols = [{"omega": w_l, "gamma": g_l, "sigma_p": s_l}, {"omega": w_d, "gamma": g_d, "sigma_p": s_d}]
+    eps_inf = 1.7
+    deck["ObjectList"].append({"shape": "block", "material": "custom", "eps": eps_inf, "pols": pols,
+                               "size": [(n - 1) * d] * 2 + [0.0], "loc": [0, 0, 0], "Basis_Set": []})
+    o = Oracle(json.dumps(deck))
+    kx, ky = 2 * math.pi / (10.1 * d), 2 * math.pi / (17.9 * d)
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ez0 = np.cos(kx * i * d + ky * j * d + 0.4).reshape(1, n, n)
+    o.set_field(2, ez0)
+    dt = o.dt
+    s = dt ** 2 * (4 * math.sin(kx * d / 2) ** 2 + 4 * math.sin(ky * d / 2) ** 2) / d ** 2
+    poles = [(2 * math.pi * w_l, math.pi * g_l, s_l, dt), (2 * math.pi * w_d, math.pi * g_d, s_d, dt)]
+    coef = _char_poly(eps_inf, poles, s)
+    m = pml + steps + 3
+    sl = (0, slice(m, n - m), slice(m, n - m))
+    amps = [1.0]
+    for _ in range(steps):
+        o.step(1)
+        r, res = _center_ratio(ez0, o.field(2), sl)
+        assert res < 1e-11
+        amps.append(r)
+    amps = np.array(amps)
+    k = len(coef)
+    # only E is set at t = 0 (D and P start at zero), so D = eps_inf E + sum P
+    # holds from step 1 on: windows start there
+    for t in range(1, len(amps) - k + 1):
+        window = amps[t:t + k][::-1]  # a_{t+k-1} ... a_t  against z^{k-1} ... z^0
+        resid = float(coef @ window)
+        scale = float(np.abs(coef) @ np.abs(window))
+        assert abs(resid) < 1e-12 * scale
+    # a medium without the poles does not satisfy it (the check has teeth)
+    wrong = _char_poly(eps_inf, poles[:1], s)
+    k = len(wrong)
+    window = amps[10:10 + k][::-1]
+    assert abs(float(wrong @ window)) > 1e-6 * float(np.abs(wrong) @ np.abs(window))
+
+
+# --------------------------------------------------------- CPML ---------
+def _cpml_run(n, pml, steps, det_offset):
+    d = 0.05
+    deck = decks.empty_deck((n, n, 1), d, steps, pml)
+    fcen = 1.0 / (20 * d)
+    deck["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "loc": [0.0, 0.0, 0.0],
+                               "PulseList": [decks.gaussian_pulse(fcen, 2.0 * fcen, 4.0, 1.0)]})
+    deck["DetectorList"].append({"type": "Ez", "dtc_class": "cout", "fname": "d", "size": [0, 0, 0],
+                                 "loc": [det_offset * d, det_offset * d * 0.5, 0.0]})
+    o = Oracle(json.dumps(deck))
+    o.step(steps)
+    return o.detector(0)[1][:, 0]
+
+
+def test_cpml_reflection_is_small():
+    """Point source, detector 13 cells from a 20-cell CPML: the signal matches a
+    domain large enough that no reflection returns within the window."""
+    steps = 500
+    small = _cpml_run(101, 20, steps, 18)
+    big = _cpml_run(281, 20, steps, 18)
+    ref = np.max(np.abs(big))
+    assert ref > 0
+    err = np.max(np.abs(small - big)) / ref
+    assert err < 2e-3
+    # without absorption (PML thickness 0) the PEC wall reflects strongly
+    pec = _cpml_run(41, 0, steps, 12)
+    big2 = _cpml_run(281, 20, steps, 12)
+    assert np.max(np.abs(pec - big2)) / np.max(np.abs(big2)) > 0.1
+
+
+# ------------------------------------------------- Maxwell-Liouville -------
+A_UNIT = decks.A_UNIT
+C_SI = 2.99792458e8
+Q_E = 1.602176634e-19
+HBAR_SI = 1.054571817e-34
+DEBYE = 3.33564e-30
+
+
+def _emitter_oracle(d=0.01, t1=1e-15, t2=None, e_ev=2.0, dip=10.0, steps=10):
+    deck = decks.empty_deck((9, 9, 1), d, steps, 2)
+    em = decks.two_level_emitter(0.0, 0.0, (0.0, 0.0, 0.0), e_ev=e_ev, dipole_debye=dip, t1=t1,
+                                 t2=t2 if t2 is not None else 2 * t1)
+    deck["ObjectList"].append(em)
+    return Oracle(json.dumps(deck))
+
+
+def _track(o, steps, e3=(0.0, 0.0, 0.0)):
+    rhos = []
+    for _ in range(steps):
+        o.qe_drive(0, e3, 1)
+        rhos.append(o.density(0, 0, 0, 2))
+    return np.array(rhos)
+
+
+def test_lindblad_superoperator_matches_textbook():
+    """Relaxation 1 -> 0 at rate G with pure dephasing g: the applied operator is
+    D(rho) = L rho L^+ - {L^+ L, rho}/2 with L = |0><1| (rate G) plus
+    2 P rho P - {P, rho} with P = |1><1| (rate g), acting on column-major vec(rho)."""
+    t1, t2 = 3e-13, 1e-13
+    o = _emitter_oracle(t1=t1, t2=t2)
+    G = 1.0 / t1 * A_UNIT / C_SI
+    g = (1.0 / t2 - 1.0 / (2 * t1)) * A_UNIT / C_SI
+    L = np.array([[0.0, 1.0], [0.0, 0.0]])
+    P = np.array([[0.0, 0.0], [0.0, 1.0]])
+    ref = np.zeros((4, 4))
+    for j in range(4):
+        E = np.zeros(4)
+        E[j] = 1.0
+        rho = E.reshape(2, 2, order="F")
+        out = G * (L @ rho @ L.T - 0.5 * (L.T @ L @ rho + rho @ L.T @ L))
+        out += g * (2 * P @ rho @ P - (P @ rho + rho @ P))
+        ref[:, j] = out.reshape(-1, order="F")
+    got = o.qe_gamma(0, 2)
+    # 1e-7: the reference's speed of light is 2.9979245e8 (CODATA 2.99792458e8)
+    assert np.allclose(got, ref, rtol=1e-7, atol=1e-12 * np.max(np.abs(ref)))
+    # the transposed operator (the plausible mistake) differs
+    assert not np.allclose(got, ref.T, rtol=1e-6, atol=1e-6 * np.max(np.abs(ref)))
+
+
+def test_t1_decay_rate_and_trace():
+    t1 = 1e-15
+    o = _emitter_oracle(t1=t1)
+    o.set_density(0, 0, 0, np.array([[0.0, 0.0], [0.0, 1.0]]))
+    steps = 1500
+    rhos = _track(o, steps)
+    G = 1.0 / t1 * A_UNIT / C_SI
+    t = o.dt * np.arange(1, steps + 1)
+    p1 = rhos[:, 1, 1].real
+    # late-time log slope: the integrator's rate, free of the cold start
+    slope = (math.log(p1[1400]) - math.log(p1[400])) / (t[1400] - t[400])
+    assert slope == pytest.approx(-G, rel=1e-6)
+    assert p1[-1] / math.exp(-G * t[-1]) == pytest.approx(1.0, abs=2e-2)
+    tr = rhos[:, 0, 0] + rhos[:, 1, 1]
+    assert np.max(np.abs(tr - 1.0)) < 1e-12
+    assert np.max(np.abs(rhos - np.conj(np.transpose(rhos, (0, 2, 1))))) < 1e-14
+
+
+def test_free_precession_and_dephasing():
+    t1, t2 = 4e-15, 2e-15
+    e_ev = 2.0
+    o = _emitter_oracle(t1=t1, t2=t2, e_ev=e_ev)
+    o.set_density(0, 0, 0, 0.5 * np.ones((2, 2)))
+    steps = 2000
+    rhos = _track(o, steps)
+    c = rhos[:, 0, 1]
+    t = o.dt * np.arange(1, steps + 1)
+    w0 = e_ev * Q_E / HBAR_SI * A_UNIT / C_SI
+    decay = (0.5 / t1 + (1.0 / t2 - 0.5 / t1)) * A_UNIT / C_SI  # = 1/T2
+    i0, i1 = 500, 1900
+    amp_slope = (math.log(abs(c[i1])) - math.log(abs(c[i0]))) / (t[i1] - t[i0])
+    assert amp_slope == pytest.approx(-decay, rel=1e-6)
+    ph = np.unwrap(np.angle(c))
+    ph_slope = (ph[i1] - ph[i0]) / (t[i1] - t[i0])
+    # the reference integrates d rho/dt = +(i/hbar)[H, rho]: rho_01 turns as exp(-i w0 t)
+    assert ph_slope == pytest.approx(-w0, rel=1e-6)
+
+
+def test_rabi_flopping_constant_field():
+    """Constant field E along z on the 2-level system of basis {(0,0),(1,0)}:
+    rho_11 = (W_R/W)^2 sin^2(W t / 2), W_R = 2 mu E / hbar, W^2 = w0^2 + W_R^2."""
+    e_ev, dip = 2.0, 10.0
+    o = _emitter_oracle(t1=1.0, t2=1.0, e_ev=e_ev, dip=dip)  # relaxation negligible
+    w0 = e_ev * Q_E / HBAR_SI * A_UNIT / C_SI
+    mu_si = dip * DEBYE
+    e_field = 0.8 * w0 * HBAR_SI / (2 * mu_si) * C_SI / A_UNIT  # V/m such that W_R = 0.8 w0
+    wr = 2 * mu_si * e_field / HBAR_SI * A_UNIT / C_SI
+    W = math.sqrt(w0 ** 2 + wr ** 2)
+    steps = int(12 * 2 * math.pi / W / o.dt)
+    rhos = _track(o, steps, (0.0, 0.0, e_field))
+    p1 = rhos[:, 1, 1].real
+    t = o.dt * np.arange(1, steps + 1)
+    # fit a + b cos(W t) + c sin(W t) over a scan of W
+    best = None
+    for Wt in np.linspace(0.97 * W, 1.03 * W, 3001):
+        M = np.stack([np.ones_like(t), np.cos(Wt * t), np.sin(Wt * t)], axis=1)
+        coef, res, _, _ = np.linalg.lstsq(M, p1, rcond=None)
+        r = float(np.sum((M @ coef - p1) ** 2))
+        if best is None or r < best[0]:
+            best = (r, Wt, coef)
+    _, w_fit, coef = best
+    assert w_fit == pytest.approx(W, rel=2e-4)
+    assert 2 * coef[0] == pytest.approx(wr ** 2 / W ** 2, rel=2e-2)
+    tr = rhos[:, 0, 0] + rhos[:, 1, 1]
+    assert np.max(np.abs(tr - 1.0)) < 1e-12
+
+
+def test_population_detector_reads_flat_index():
+    """dtc_levs [0, 3] on a 2-level system read rho[0] and rho[3] of the
+    column-major density, i.e. rho_00 and rho_11."""
+    o = _emitter_oracle(t1=1e-15)
+    o.set_density(0, 0, 0, np.array([[0.25, 0.1 + 0.2j], [0.1 - 0.2j, 0.75]]))
+    o.qe_drive(0, (0.0, 0.0, 0.0), 5)
+    rho = o.density(0, 0, 0, 2)
+    assert o.pops(0, 0)[-1] == pytest.approx(rho[0, 0], abs=1e-15)
+    assert o.pops(0, 1)[-1] == pytest.approx(rho[1, 1], abs=1e-15)
+
+
+# ------------------------------------------------------------- flux --------
+def _flux_run(boxes, src_off, n=36, pml=6, d=0.02, steps=400, amp=1.0):
+    deck = decks.empty_deck((n, n, n), d, steps, pml)
+    fcen = 1.0 / (10 * d)
+    deck["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "loc": [src_off[0] * d, src_off[1] * d, src_off[2] * d],
+                               "PulseList": [decks.gaussian_pulse(fcen, fcen, 3.0, amp)]})
+    for k, (cells, off) in enumerate(boxes):
+        s = (cells - 1) * d
+        deck["FluxList"].append({"name": f"f{k}", "size": [s, s, s], "loc": [off[0] * d, off[1] * d, off[2] * d],
+                                 "lamL": 7 * d, "lamR": 16 * d, "nfreq": 6})
+    o = Oracle(json.dumps(deck))
+    o.step(steps)
+    return [o.flux(k) for k in range(len(boxes))]
+
+
+def test_flux_box_invariants():
+    """Net outward flux of a closed box: independent of the box for an enclosed
+    source, near zero for a source outside, positive, and quadratic in the
+    source amplitude.  The bounds are loose: the reference's face-centre
+    averaging and Simpson weights leave a few per cent of discretisation error."""
+    fa, fb, fc = _flux_run([(11, (0, 0, 0)), (17, (0, 0, 0)), (9, (6, 0, 0))], (-1, 0, 0))
+    assert np.all(fb.real > 0)
+    low = slice(0, 3)  # the longest wavelengths, best resolved
+    assert np.max(np.abs(fa[low] - fb[low]) / np.abs(fb[low])) < 0.03
+    assert np.max(np.abs(fc / fb)) < 0.15
+    (f2,) = _flux_run([(11, (0, 0, 0))], (-1, 0, 0), amp=2.0)
+    assert np.allclose(f2, 4.0 * fa, rtol=1e-12, atol=0.0)
+
+
+# -------------------------------------------------------- detectors --------
+def test_point_detector_samples_field_after_time_increment():
+    """Detectors sample at t = k dt after the step's tcur += dt (times in
+    seconds, t a / c), with the E-field SI conversion equal to 1 under the
+    default I0 = a eps0 c."""
+    deck = json.loads(decks.config1_2d(n=40, steps=30, pml=6, cyl_radius_cells=5, src_cell=(14, 20), ndet=3))
+    o = Oracle(json.dumps(deck))
+    vals = []
+    for _ in range(30):
+        o.step(1)
+        vals.append(o.field(2)[0])
+    t, v = o.detector(0)
+    assert len(t) == 31  # the t = 0 sample is taken when the detector is built
+    assert np.allclose(t, o.dt * np.arange(31) * A_UNIT / 2.9979245e8, rtol=1e-12, atol=0)
+    loc = deck["DetectorList"][0]["loc"]
+    d = deck["CompCell"]["stepSize"][0]
+    ix = int(round(loc[0] / d + 19.5))
+    iy = int(round(loc[1] / d + 19.5))
+    assert v[0, 0] == 0.0
+    # conversion a eps0 c / I0 is 1 up to rounding
+    assert np.allclose(v[1:, 0], np.array([f[iy, ix] for f in vals]), rtol=1e-14, atol=0)
+    assert np.max(np.abs(v[:, 0])) > 0
