@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-__all__ = ["FDTDField", "load_library", "LIB_PATH", "ChimlError"]
+__all__ = ["FDTDField", "Server", "load_library", "LIB_PATH", "ChimlError"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libchiml_amd.so")
@@ -51,6 +51,18 @@ def load_library():
     L.chiml_sim_profile_read.argtypes = [ctypes.c_void_p, ctypes.c_char_p, c_dbl_p, c_int_p]
     L.chiml_sim_footprint.argtypes = [ctypes.c_void_p, c_size_p]
     L.chiml_sim_estimate_bytes.argtypes = [ctypes.c_char_p, c_size_p]
+    L.chiml_sim_query.argtypes = [ctypes.c_void_p, c_int_p]
+    L.chiml_sim_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    c_ll_p = ctypes.POINTER(ctypes.c_longlong)
+    L.chiml_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_void_p)]
+    L.chiml_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_char_p, c_ll_p]
+    L.chiml_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_longlong, c_int_p, c_dbl_p, c_dbl_p]
+    L.chiml_server_take.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)]
+    L.chiml_server_error.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_char_p, ctypes.c_size_t]
+    L.chiml_server_drain.argtypes = [ctypes.c_void_p]
+    L.chiml_server_stats.argtypes = [ctypes.c_void_p, c_dbl_p, c_size_p, c_int_p, c_int_p, c_int_p]
+    L.chiml_server_destroy.argtypes = [ctypes.c_void_p]
     L.chiml_sim_last_error.restype = ctypes.c_char_p
     L.chiml_sim_destroy.argtypes = [ctypes.c_void_p]
     _LIB = L
@@ -134,10 +146,13 @@ class FDTDField:
     detectors, flux boxes and emitters (:1766-1781), ``dt`` is the time step.
     """
 
-    def __init__(self, deck_json):
+    def __init__(self, deck_json=None, _handle=None):
         L = load_library()
-        h = ctypes.c_void_p()
-        _check(L.chiml_sim_create(deck_json.encode(), ctypes.byref(h)))
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _check(L.chiml_sim_create(deck_json.encode(), ctypes.byref(h)))
+        else:
+            h = _handle
         self._h = h
         n3 = (ctypes.c_int * 3)()
         dt = ctypes.c_double()
@@ -167,6 +182,15 @@ class FDTDField:
 
     def step(self, n=1):
         _check(load_library().chiml_sim_step(self._h, int(n)))
+
+    def set_graph(self, steps_per_graph):
+        """Replay captured HIP graphs of this many steps (0: plain launches)."""
+        _check(load_library().chiml_sim_set_graph(self._h, int(steps_per_graph)))
+
+    def idle(self):
+        v = ctypes.c_int()
+        _check(load_library().chiml_sim_query(self._h, ctypes.byref(v)))
+        return bool(v.value)
 
     def sync(self):
         _check(load_library().chiml_sim_sync(self._h))
@@ -233,3 +257,66 @@ def estimate_bytes(deck_json):
     b = ctypes.c_size_t()
     _check(load_library().chiml_sim_estimate_bytes(deck_json.encode(), ctypes.byref(b)))
     return b.value
+
+
+class Server:
+    """Sweep server (include/chiml/chiml_server.h): continuous batching of
+    device-resident simulations admitted by HBM footprint."""
+
+    QUEUED, BUILDING, RUNNING, DONE, FAILED = range(5)
+
+    def __init__(self, hbm_budget, max_active=16, chunk_steps=64, graph_steps=0):
+        L = load_library()
+        h = ctypes.c_void_p()
+        _check(L.chiml_server_create(int(hbm_budget), int(max_active), int(chunk_steps), int(graph_steps),
+                                     ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, deck_json):
+        i = ctypes.c_longlong()
+        _check(load_library().chiml_server_submit(self._h, deck_json.encode(), ctypes.byref(i)))
+        return i.value
+
+    def poll(self, rid):
+        """(state, admitted_s, finished_s) -- seconds since submission, -1 if not yet."""
+        st = ctypes.c_int()
+        a = ctypes.c_double()
+        f = ctypes.c_double()
+        _check(load_library().chiml_server_poll(self._h, rid, ctypes.byref(st), ctypes.byref(a), ctypes.byref(f)))
+        return st.value, a.value, f.value
+
+    def take(self, rid):
+        """The finished simulation as an FDTDField (the caller closes it)."""
+        h = ctypes.c_void_p()
+        _check(load_library().chiml_server_take(self._h, rid, ctypes.byref(h)))
+        return FDTDField(_handle=h)
+
+    def error(self, rid):
+        buf = ctypes.create_string_buffer(1024)
+        _check(load_library().chiml_server_error(self._h, rid, buf, 1024))
+        return buf.value.decode()
+
+    def drain(self):
+        _check(load_library().chiml_server_drain(self._h))
+
+    def stats(self):
+        cu = ctypes.c_double()
+        held = ctypes.c_size_t()
+        act = ctypes.c_int()
+        q = ctypes.c_int()
+        peak = ctypes.c_int()
+        _check(load_library().chiml_server_stats(self._h, ctypes.byref(cu), ctypes.byref(held), ctypes.byref(act),
+                                                 ctypes.byref(q), ctypes.byref(peak)))
+        return {"cell_updates": cu.value, "bytes_held": held.value, "active": act.value, "queued": q.value,
+                "peak_active": peak.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().chiml_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -38,7 +38,7 @@ namespace {
 constexpr int kDftBlock = 32;  // samples staged before one DFT contraction
 constexpr int kMaxLevels = 8;
 
-std::string g_err;
+thread_local std::string g_err;  // last error of this host thread
 
 #define HIP_TRY(expr)                                                                                \
     do {                                                                                             \
@@ -57,15 +57,16 @@ struct DevBuf {
     ~DevBuf() {
         if (p) (void)hipFree(p);
     }
-    void alloc(size_t b) {
+    // Zero-filled buffers are cleared on the owning context's stream, so the
+    // clear is ordered before every kernel of that context and never waits
+    // for other contexts sharing the device.
+    void alloc(size_t b, hipStream_t zero_on, bool zero) {
         if (p) (void)hipFree(p);
         p = nullptr;
         bytes = b;
         if (b == 0) return;
         HIP_TRY(hipMalloc(&p, b));
-        HIP_TRY(hipMemset(p, 0, b));
-        // the context's stream is non-blocking: finish the clear before any use on it
-        HIP_TRY(hipDeviceSynchronize());
+        if (zero) HIP_TRY(hipMemsetAsync(p, 0, b, zero_on));
     }
     template <typename T>
     T* as() const { return static_cast<T*>(p); }
@@ -1082,6 +1083,9 @@ struct chiml_dev {
     bool hmap = false, has_src = false;
     int nmat[3] = {1, 1, 1};
     int enq_step = 0;    // step counter value after the step being enqueued
+    int graph_steps = 0;  // > 0: replay captured graphs of this many steps
+    bool device_flush = false;  // launch the DFT flush every step; it decides on the device
+    hipGraphExec_t gexec = nullptr;
     bool split = false;  // 3D interior + CPML-slab kernels
     Box3 ibox{};
     SlabMap slab{};
@@ -1103,8 +1107,9 @@ struct chiml_dev {
     template <typename T>
     T* upload(const T* src, size_t n) {
         bufs.emplace_back(new DevBuf);
-        bufs.back()->alloc(n * sizeof(T));
+        bufs.back()->alloc(n * sizeof(T), stream, src == nullptr);
         footprint += n * sizeof(T);
+        // a blocking copy: complete before any later launch on the stream
         if (src && n) HIP_TRY(hipMemcpy(bufs.back()->p, src, n * sizeof(T), hipMemcpyHostToDevice));
         return bufs.back()->as<T>();
     }
@@ -1342,7 +1347,8 @@ struct Timed {
     const char* name;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool on;
-    Timed(chiml_dev* c_, const char* n) : c(c_), name(n), on(c_->profile && (c_->profile_only.empty() || c_->profile_only == n)) {
+    Timed(chiml_dev* c_, const char* n)
+        : c(c_), name(n), on(c_->profile && !c_->device_flush && (c_->profile_only.empty() || c_->profile_only == n)) {
         if (on) {
             (void)hipEventCreate(&e0);
             (void)hipEventCreate(&e1);
@@ -1362,7 +1368,7 @@ void launch_flush(chiml_dev* c, int force) {
     if (c->ndft == 0) return;
     // enq_step = the device step counter once the step being enqueued has advanced it
     const int st = c->enq_step;
-    if (!force && (st % c->dft_interval != 0 || ((st / c->dft_interval) % kDftBlock) != kDftBlock - 1)) return;
+    if (!force && !c->device_flush && (st % c->dft_interval != 0 || ((st / c->dft_interval) % kDftBlock) != kDftBlock - 1)) return;
     Timed t(c, "dft_flush");
     hipLaunchKernelGGL(k_dft_flush<T>, dim3((c->ndft + 255) / 256), dim3(256), 0, c->stream, static_cast<const T*>(c->dft_stage),
                        c->dft_phasor, c->dft_F, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, force, -1);
@@ -1503,7 +1509,7 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
         c->nz = p->nz;
         c->dim = p->nz > 1 ? 3 : 2;
         c->nsteps = p->nsteps;
-        c->step_ctr.alloc(sizeof(int));
+        c->step_ctr.alloc(sizeof(int), c->stream, true);
         if (c->prec == 32) build<float>(c.get(), p, c->a32, c->s32);
         else build<double>(c.get(), p, c->a64, c->s64);
         if (c->dim == 3) {
@@ -1540,16 +1546,73 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
     return CHIML_OK;
 }
 
+namespace {
+void enqueue_steps(chiml_dev* c, int n) {
+    if (c->prec == 32) do_steps<float>(c, c->a32, c->s32, n);
+    else do_steps<double>(c, c->a64, c->s64, n);
+}
+
+// Captures graph_steps steps of this context's launch sequence once; every
+// launch reads the step number from the device counter, so one graph serves
+// every chunk of the run.
+void ensure_graph(chiml_dev* c) {
+    if (c->gexec) return;
+    hipGraph_t g = nullptr;
+    const int saved = c->steps_done;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_steps(c, c->graph_steps);
+    } catch (...) {
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        c->steps_done = saved;
+        throw;
+    }
+    HIP_TRY(hipStreamEndCapture(c->stream, &g));
+    c->steps_done = saved;  // capture enqueued nothing
+    hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(e);
+}
+}  // namespace
+
 int chiml_dev_step(chiml_dev_t* c, int n) {
     if (!c || n < 0) return fail(c, CHIML_E_INVALID, "invalid argument");
     if (c->steps_done + n > c->nsteps) return fail(c, CHIML_E_STATE, "step count exceeds the per-step tables built at create");
     try {
-        if (c->prec == 32) do_steps<float>(c, c->a32, c->s32, n);
-        else do_steps<double>(c, c->a64, c->s64, n);
+        if (c->graph_steps > 0) {
+            while (n >= c->graph_steps) {
+                ensure_graph(c);
+                HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+                c->steps_done += c->graph_steps;
+                n -= c->graph_steps;
+            }
+        }
+        if (n > 0) enqueue_steps(c, n);
         if (c->profile) collect_profile(c);
     } catch (std::exception& e) {
         return fail(c, CHIML_E_HIP, e.what());
     }
+    return CHIML_OK;
+}
+
+int chiml_dev_set_graph(chiml_dev_t* c, int steps_per_graph) {
+    if (!c || steps_per_graph < 0) return fail(c, CHIML_E_INVALID, "invalid argument");
+    if (c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    c->graph_steps = steps_per_graph;
+    c->device_flush = steps_per_graph > 0;
+    return CHIML_OK;
+}
+
+int chiml_dev_query(chiml_dev_t* c, int* idle) {
+    if (!c || !idle) return fail(c, CHIML_E_INVALID, "invalid argument");
+    hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) *idle = 1;
+    else if (e == hipErrorNotReady) *idle = 0;
+    else return fail(c, CHIML_E_HIP, hipGetErrorString(e));
     return CHIML_OK;
 }
 
@@ -1727,6 +1790,7 @@ const char* chiml_dev_global_error(void) { return g_err.c_str(); }
 void chiml_dev_destroy(chiml_dev_t* c) {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (auto& pe : c->pending) {
         (void)hipEventDestroy(pe.second.first);
         (void)hipEventDestroy(pe.second.second);

@@ -30,7 +30,7 @@
 namespace chiml {
 namespace {
 
-std::string g_sim_err;
+thread_local std::string g_sim_err;  // last error of this host thread
 
 // Staggering of each Yee component along x, y, z (1 = half-integer position):
 // Ex (1,0,0) Ey (0,1,0) Ez (0,0,1) Hx (0,1,1) Hy (1,0,1) Hz (1,1,0)
@@ -974,6 +974,9 @@ std::vector<cplx> FDTDField::flux(int fi) {
     return total;
 }
 
+// Sets this thread's message for chiml_sim_last_error (used by the server).
+void set_sim_error(const std::string& m) { g_sim_err = m; }
+
 }  // namespace chiml
 
 // ------------------------------------------------------------------ C ABI ---
@@ -1016,6 +1019,20 @@ int chiml_sim_step(chiml_sim_t* s, int n) {
 int chiml_sim_sync(chiml_sim_t* s) {
     if (!s) return sim_fail("null simulation");
     int rc = chiml_dev_sync(s->f->dev_);
+    if (rc) chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
+    return rc;
+}
+
+int chiml_sim_query(chiml_sim_t* s, int* idle) {
+    if (!s) return sim_fail("null simulation");
+    int rc = chiml_dev_query(s->f->dev_, idle);
+    if (rc) chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
+    return rc;
+}
+
+int chiml_sim_set_graph(chiml_sim_t* s, int steps_per_graph) {
+    if (!s) return sim_fail("null simulation");
+    int rc = chiml_dev_set_graph(s->f->dev_, steps_per_graph);
     if (rc) chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
     return rc;
 }

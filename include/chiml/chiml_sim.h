@@ -30,6 +30,9 @@ int chiml_sim_create(const char* deck_json, chiml_sim_t** out);
 /* Advance n steps (asynchronous on the device). */
 int chiml_sim_step(chiml_sim_t* s, int n);
 int chiml_sim_sync(chiml_sim_t* s);
+/* chiml_dev_query / chiml_dev_set_graph for the simulation's context. */
+int chiml_sim_query(chiml_sim_t* s, int* idle);
+int chiml_sim_set_graph(chiml_sim_t* s, int steps_per_graph);
 /* n3 = interior points (nx, ny, nz; nz = 1 in 2D), dt, total steps ceil(tLim/dt),
  * precision (32/64), number of point detectors, flux boxes, emitter groups. */
 int chiml_sim_info(chiml_sim_t* s, int* n3, double* dt, int* nsteps, int* precision, int* ndet, int* nflux, int* nqe);

@@ -22,8 +22,36 @@
 // boundary-condition slots to no-ops in that case,
 // src/FDTD_MANAGER/parallelFDTDField.cpp:179-184, 344-349).
 //
-// Parity pins: see tests/test_oracle_pins.py (closed forms and invariants)
-// and the golden fixtures under tests/golden/.
+// Pins (tests/test_oracle_pins.py, closed forms and invariants; the golden
+// fixtures under tests/golden/ only freeze the outputs):
+//   run lists + Yee curls (interior)  discrete plane-wave dispersion, 2D and
+//                                     3D, anisotropic eps per component (1e-12)
+//   ADE (Lorentz + Drude, D form)     characteristic polynomial of the mode
+//                                     recurrence from P'' + 2g P' + w^2 P = s w^2 E
+//   CPML as a whole                   reflection < 2e-3 for 20 cells; PEC wall
+//                                     without it reflects > 0.1
+//   pulses                            Gaussian envelope / turn-off, default t_0,
+//                                     Ricker identically zero (trap T1)
+//   grid / time step                  n = floor(size/d + 0.5) + 1, 2D dt with the
+//                                     z term, ceil(tLim/dt) steps
+//   sources                           cell placement, added after the H update
+//   point detectors                   t = k dt after the increment, SI factors
+//   lindblad, relaxation build        textbook L rho L^+ - {L^+ L, rho}/2
+//   denDeriv + PCABAM4                T1 decay rate, T2 decay + precession
+//                                     frequency, Rabi frequency and amplitude
+//                                     (SI dipole units), trace and hermiticity
+//   wigner3j / expectation (mu_z)     textbook 3j values, orthogonality, Rabi
+//   QEPopDtc flat index               pops equal rho[0], rho[3] (trap T3)
+//   flux                              box-size independence, ~0 for an outside
+//                                     source, quadratic in amplitude (loose)
+// Parity unpinned (no closed form; the restatement follows the reference
+// line by line and only the golden fixtures freeze it):
+//   getPsiUpList / getGridUpList corner ownership and end trims and genEtaEff
+//   plane averages in material-filled PMLs, beyond the reflection bound;
+//   detectMatInPML's scan; Obj::isObj for sphere / cylinder / ellipsoid
+//   rasterisation; flux makeParamIn / combineField offset averaging and the
+//   Simpson weights beyond the invariants; expectation for mu_x / mu_y; the
+//   addQE polarisation back-coupling E -= P/(2 eps) at the two Yee edges.
 #include <algorithm>
 #include <cmath>
 #include <complex>

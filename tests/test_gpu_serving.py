@@ -1,0 +1,96 @@
+"""Sweep server and HIP-graph stepping on the GPU: every result must be the
+same alone and in the mix (bit for bit: the same kernels run on the same
+data; only their interleaving with other simulations differs)."""
+import json
+import time
+
+import numpy as np
+import pytest
+
+import chiml_amd
+from chiml_amd import decks
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix():
+    out = []
+    out.append(decks.config1_2d(n=48, steps=90, pml=8, cyl_radius_cells=6, src_cell=(16, 24), ndet=2,
+                                precision="fp32"))
+    out.append(decks.config2_3d(n=24, steps=70, pml=4, d=0.02, sphere_cells=4, flux_cells=10, nfreq=5,
+                                precision="fp32"))
+    d = json.loads(decks.config3_ml2d(n=48, steps=120, pml=6, slab=(8, 4), precision="fp32", det_cells=2))
+    fc = decks.ev2fdtd(2.0)
+    d["SourceList"][0]["PulseList"] = [{"profile": "gaussian", "fcen": fc, "fwidth": fc, "cutoff": 3.0, "t_0": 0.5,
+                                        "Field_Intensity": 1e10}]
+    out.append(json.dumps(d))
+    out.append(decks.config2_3d(n=32, steps=50, pml=8, d=0.02, sphere_cells=5, flux_cells=12, nfreq=4,
+                                precision="fp64"))
+    out.append(decks.config1_2d(n=64, steps=75, pml=10, cyl_radius_cells=9, src_cell=(20, 32), ndet=3,
+                                precision="fp64"))
+    return out
+
+
+def _results(sim):
+    r = {}
+    for c in range(6):
+        r[f"f{c}"] = sim.field(c)
+    for k, dtc in enumerate(sim.dtcArr()):
+        r[f"d{k}"] = dtc.series()[1]
+    for k, fx in enumerate(sim.fluxArr()):
+        r[f"x{k}"] = fx.getFlux()
+    for k, q in enumerate(sim.qeArr()):
+        r[f"p{k}"] = np.concatenate([q.population(0), q.population(1)])
+    return r
+
+
+def _alone(deck, graph=0):
+    sim = chiml_amd.FDTDField(deck)
+    if graph:
+        sim.set_graph(graph)
+    sim.step(sim.nsteps)
+    sim.sync()
+    r = _results(sim)
+    sim.close()
+    return r
+
+
+def _same(a, b, label):
+    assert a.keys() == b.keys(), label
+    for k in a:
+        assert a[k].shape == b[k].shape, (label, k)
+        assert np.array_equal(a[k], b[k]), (label, k, float(np.max(np.abs(a[k] - b[k]))))
+
+
+@pytest.mark.parametrize("graph", [8, 32])
+def test_graph_steps_equal_plain_launches(graph):
+    for i, deck in enumerate(_mix()):
+        print(f"graph={graph} deck {i}", flush=True)
+        _same(_alone(deck), _alone(deck, graph), f"deck {i}")
+
+
+@pytest.mark.parametrize("graph", [0, 16])
+def test_server_results_equal_alone(graph):
+    mix = _mix()
+    alone = [_alone(d) for d in mix]
+    # budget that holds about two of the larger requests at once: the rest queue
+    budget = 2 * max(chiml_amd.estimate_bytes(d) for d in mix)
+    srv = chiml_amd.Server(budget, max_active=3, chunk_steps=10, graph_steps=graph)
+    try:
+        ids = [srv.submit(d) for d in mix[:3]]
+        time.sleep(0.05)  # the rest arrive while the first ones run
+        ids += [srv.submit(d) for d in mix[3:]]
+        srv.drain()
+        st = srv.stats()
+        print("server stats", st, flush=True)
+        for k, rid in enumerate(ids):
+            state, adm, fin = srv.poll(rid)
+            assert state == chiml_amd.Server.DONE, srv.error(rid)
+            assert 0 <= adm <= fin
+            sim = srv.take(rid)
+            _same(alone[k], _results(sim), f"request {k}")
+            sim.close()
+        assert st["peak_active"] >= 2
+        assert srv.stats()["bytes_held"] == 0
+    finally:
+        srv.close()

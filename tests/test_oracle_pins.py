@@ -452,3 +452,23 @@ def test_point_detector_samples_field_after_time_increment():
     # conversion a eps0 c / I0 is 1 up to rounding
     assert np.allclose(v[1:, 0], np.array([f[iy, ix] for f in vals]), rtol=1e-14, atol=0)
     assert np.max(np.abs(v[:, 0])) > 0
+
+
+def test_source_lands_on_its_cell_after_the_h_update():
+    """One step from rest: H sees E = 0, then the soft source adds dt * pulse(0)
+    to its E cell (tcur is incremented after), and the E curl of H = 0 adds
+    nothing -- only the source cell is non-zero."""
+    n, d = 24, 0.05
+    deck = decks.empty_deck((n, n, 1), d, 3, 4)
+    pul = decks.gaussian_pulse(2.0, 2.0, 3.0, 1.0)
+    pul["t_0"] = 0.3
+    cx, cy = 9, 14
+    deck["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "PulseList": [pul],
+                               "loc": [(cx - (n - 1) / 2) * d, (cy - (n - 1) / 2) * d, 0.0]})
+    js = json.dumps(deck)
+    o = Oracle(js)
+    o.step(1)
+    ez = o.field(2)[0]
+    assert np.count_nonzero(ez) == 1
+    assert ez[cy, cx] == pytest.approx(o.dt * pulse(js, 0, 0.0), rel=1e-14)
+    assert np.all(o.field(3) == 0) and np.all(o.field(4) == 0)
