@@ -43,6 +43,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-steps", type=int, default=3)
     p.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    p.add_argument("--sweep-requests", type=int, default=32,
+                   help="requests in the serving-sweep sample (0 skips the sweep leg)")
+    p.add_argument("--sweep-steps", type=int, nargs=2, default=[100, 1000],
+                   help="step range of the sweep sample (the full sweep uses 1000 10000)")
+    p.add_argument("--no-sweep-baseline", action="store_true", help="skip the one-at-a-time replay")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a plain 256 MiB device copy three times (the known-size kernel that scales "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -147,6 +152,26 @@ def kernel_table(r):
     return out
 
 
+def run_sweep(args):
+    """Serving leg: seeded requests (Poisson 20/s; 2D 128^2-1024^2 and 3D
+    64^3-192^3; dielectric or Drude object; a quarter with 2- or 4-level
+    emitters; FP32 fields) through the sweep server, then the same requests
+    one at a time on the plain GPU path."""
+    import torch
+    from chiml_amd import decks, sweep
+
+    reqs = decks.sweep_requests(args.sweep_requests, steps_range=tuple(args.sweep_steps))
+    total = torch.cuda.get_device_properties(0).total_memory
+    res = {"sample": f"{args.sweep_requests} requests, steps {args.sweep_steps[0]}-{args.sweep_steps[1]}, "
+                     f"seed 5, Poisson 20/s",
+           "server": sweep.run_server(reqs, hbm_budget=int(0.6 * total), max_active=16, chunk_steps=64,
+                                      graph_steps=32)}
+    if not args.no_sweep_baseline:
+        res["one_at_a_time"] = sweep.run_one_at_a_time(reqs)
+        res["throughput_ratio"] = round(res["server"]["mcells_per_s"] / res["one_at_a_time"]["mcells_per_s"], 3)
+    return res
+
+
 def cpu_baseline(n, steps):
     from chiml_amd import decks
     from oracle.pyoracle import Oracle
@@ -222,6 +247,8 @@ def main():
             a = complex(*r32["flux0"])
             b = complex(*r64["flux0"])
             out["flux_f0_rel_err_fp32_vs_fp64"] = abs(a - b) / abs(b) if abs(b) > 0 else None
+    if args.sweep_requests > 0:
+        out["sweep"] = run_sweep(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -275,3 +275,62 @@ def config4_ml3d(n=192, steps=5000, pml=16, d=0.02, layer_cells=10, film_cells=2
         "PulseList": [gaussian_pulse(fcen, 2.0 * fcen, 3.0, 1e8)],
     })
     return json.dumps(deck)
+
+
+def sweep_requests(n_req=512, seed=5, rate_hz=20.0, steps_range=(1000, 10000), emitter_frac=0.25,
+                   sizes_2d=(128, 256, 512, 1024), sizes_3d=(64, 96, 128, 160, 192)):
+    """Seeded parameter-sweep requests: (arrival time in s, deck json) pairs.
+
+    Poisson arrivals at rate_hz; half 2D TMz and half 3D grids; a random
+    cylinder (2D) or sphere (3D) that is a dielectric (eps in [1.5, 12]) or a
+    Drude metal; a fraction with a 2- or 4-level emitter block; steps uniform
+    in steps_range; FP32 fields and FP64 density matrices."""
+    rng = splitmix64(seed)
+    t = 0.0
+    out = []
+    for _ in range(n_req):
+        t += -math.log(1.0 - next(rng)) / rate_hz
+        is3 = next(rng) < 0.5
+        sizes = sizes_3d if is3 else sizes_2d
+        n = sizes[min(int(next(rng) * len(sizes)), len(sizes) - 1)]
+        steps = steps_range[0] + int(next(rng) * (steps_range[1] - steps_range[0] + 1))
+        d = 0.02 if is3 else 0.05
+        pml = 10 if is3 else 12
+        n3 = (n, n, n) if is3 else (n, n, 1)
+        deck, size, dt = _cell(n3, d, 0.5, steps, pml, "fp32")
+        r_cells = max(3, int(n * (0.08 + 0.12 * next(rng))))
+        obj = {"shape": "sphere" if is3 else "cylinder", "material": "custom", "radius": r_cells * d,
+               "loc": [0.0, 0.0, 0.0], "Basis_Set": []}
+        if not is3:
+            obj.update({"length": 1.0, "unit_vectors": [{"uvec": [1, 0, 0]}, {"uvec": [0, 0, 1]}, {"uvec": [0, 1, 0]}]})
+        if next(rng) < 0.5:
+            obj.update({"eps": 1.5 + 10.5 * next(rng), "pols": []})
+        else:
+            obj.update({"eps": 1.0, "pols": gold_like_pols()[:1]})
+        deck["ObjectList"].append(obj)
+        fcen = A_UNIT / 600e-9
+        src_pos = _pos(pml + 3, n, d)
+        if is3:
+            inner = (n - 2 * pml - 2) * d
+            deck["SourceList"].append({"pol": "Ex", "size": [inner, inner, 0.0], "loc": [0.0, 0.0, src_pos],
+                                       "PulseList": [gaussian_pulse(fcen, 2.0 * fcen, 3.0, 1e8)]})
+        else:
+            deck["SourceList"].append({"pol": "Ez", "size": [0.0, 0.0, 0.0], "loc": [src_pos, 0.0, 0.0],
+                                       "PulseList": [gaussian_pulse(fcen, 2.0 * fcen, 3.0, 1e8)]})
+        det = {"type": "Ez", "dtc_class": "cout", "fname": "t", "size": [0.0, 0.0, 0.0],
+               "loc": [-src_pos, 0.0, 0.0]}
+        deck["DetectorList"].append(det)
+        if next(rng) < emitter_frac:
+            nlev = 2 if next(rng) < 0.5 else 4
+            em_cells = max(2, n // 16)
+            ex = (em_cells - 1) * d
+            loc = (d / 2 if n % 2 == 0 else 0.0,) * 2 + ((d / 2 if n % 2 == 0 else 0.0) if is3 else 0.0,)
+            size3 = [ex, ex, ex if is3 else 0.0]
+            loc = (loc[0] + (r_cells + 2) * d, loc[1], loc[2])
+            if nlev == 2:
+                em = two_level_emitter(size3[0], size3[1], loc, z_size=size3[2])
+            else:
+                em = n_level_emitter(4, size3, loc, seed=seed * 1000 + len(out))
+            deck["ObjectList"].append(em)
+        out.append((t, json.dumps(deck)))
+    return out
