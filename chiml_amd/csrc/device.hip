@@ -926,9 +926,18 @@ __device__ __forceinline__ void den_deriv(const QeDev& q, const double2* H, cons
             out[i + j * N] = make_double2(-di * ih, dr * ih);
         }
     for (int g = 0; g < q.ngam; ++g) {
-        double2 v = r[q.gam_col[g]];
-        out[q.gam_row[g]].x += v.x * q.gam_val[g];
-        out[q.gam_row[g]].y += v.y * q.gam_val[g];
+        const int row = q.gam_row[g], cl = q.gam_col[g];
+        const double w = q.gam_val[g];
+        double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < N * N; ++k)
+            if (cl == k) v = r[k];
+#pragma unroll
+        for (int k = 0; k < N * N; ++k)
+            if (row == k) {
+                out[k].x += v.x * w;
+                out[k].y += v.y * w;
+            }
     }
 }
 
@@ -948,14 +957,14 @@ __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const 
         const int px = q.xyz[3 * e], py = q.xyz[3 * e + 1], pz = q.xyz[3 * e + 2];
         double ef[3] = {0.0, 0.0, 0.0};
         const T* Es[3] = {E0, E1, E2};
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
             if (!Es[c]) continue;
             long long i0 = px + static_cast<long long>(nx) * (py + static_cast<long long>(ny) * pz);
             if (q.dim == 2) ef[c] = static_cast<double>(Es[c][i0]);
             else {
-                int o[3] = {0, 0, 0};
-                o[c] = -1;
-                int qx = px + o[0], qy = py + o[1], qz = pz + o[2];
+                // mean of the two Yee edges E_c(p) and E_c(p - e_c) (getE_TE)
+                const int qx = px - (c == 0), qy = py - (c == 1), qz = pz - (c == 2);
                 double v1 = (qx >= 0 && qy >= 0 && qz >= 0) ? static_cast<double>(Es[c][qx + static_cast<long long>(nx) * (qy + static_cast<long long>(ny) * qz)]) : 0.0;
                 ef[c] = 0.5 * static_cast<double>(Es[c][i0]) + 0.5 * v1;
             }
@@ -972,8 +981,10 @@ __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const 
             const long long col = static_cast<long long>(m) * q.nemit + e;
             double2 H[S], pred[S], r[S], fp[S];
             // H = h0 - sum_c E_c mu_c (Hamiltonian::getHam)
+#pragma unroll
             for (int k = 0; k < S; ++k) {
                 double2 h = q.h0[m * S + k];
+#pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     double2 mu = q.mu[c * S + k];
                     h.x -= ef[c] * mu.x;
@@ -984,6 +995,7 @@ __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const 
             // Predictor rho + dt/24 (55 f_n - 59 f_{n-1} + 37 f_{n-2} - 9 f_{n-3})
             // and the f_pred-free part of the corrector
             // rho + dt/24 (19 f_n - 5 f_{n-1} + f_{n-2}).
+#pragma unroll
             for (int k = 0; k < S; ++k) {
                 double2 rk = q.rho[k * ld + col];
                 double2 f0 = q.hist[s0 * S * ld + k * ld + col];
@@ -1003,18 +1015,28 @@ __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const 
                 r[k] = cr;
             }
             den_deriv<N>(q, H, pred, fp);
+#pragma unroll
             for (int k = 0; k < S; ++k) {
                 r[k] = cadd(r[k], cscale(fp[k], 9.0 * dt / 24.0));
                 q.rho[k * ld + col] = r[k];
             }
             den_deriv<N>(q, H, r, fp);
+#pragma unroll
             for (int k = 0; k < S; ++k) q.hist[s3 * S * ld + k * ld + col] = fp[k];
-            for (int j = 0; j < q.npop && j < 4; ++j) pop[j] = cadd(pop[j], r[q.pop_flat[j]]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j >= q.npop) break;
+                const int kf = q.pop_flat[j];
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    if (kf == k) pop[j] = cadd(pop[j], r[k]);
+            }
             // updateQEPol: P_c += na Re(sum conj(rho_k) mu_k)
             for (int c = 0; c < 3; ++c) {
                 if (!Es[c]) continue;
                 double s = 0.0;
-                for (int k = 0; k < S; ++k) {
+    #pragma unroll
+            for (int k = 0; k < S; ++k) {
                     double2 mu = q.mu[c * S + k];
                     s += r[k].x * mu.x + r[k].y * mu.y;
                 }
@@ -1027,7 +1049,9 @@ __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const 
             if (Es[c]) q.Pbox[c][bx + PX * (by + PY * bz)] = P[c];
     }
     // deterministic per-block population partial sums
-    for (int j = 0; j < q.npop && j < 4; ++j) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= q.npop) break;
         red[threadIdx.x] = pop[j];
         __syncthreads();
         for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -1035,6 +1059,179 @@ __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const 
             __syncthreads();
         }
         if (threadIdx.x == 0) q.pop_partial[j * gridDim.x + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+
+// Column j of d rho/dt for the matrix X held in LDS (column-major):
+// (i/hbar)(H X - X H)[:, j] + the Lindblad rows that fall in column j.
+template <int N>
+__device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* H, const double2* X, int j, double2 (&out)[N]) {
+    const double ih = 1.0 / q.hbar;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            s1 = cadd(s1, cmul(H[i + k * N], X[k + j * N]));
+            s2 = cadd(s2, cmul(X[i + k * N], H[k + j * N]));
+        }
+        out[i] = make_double2(-(s1.y - s2.y) * ih, (s1.x - s2.x) * ih);
+    }
+    for (int g = 0; g < q.ngam; ++g) {
+        const int row = q.gam_row[g];
+        if (row / N != j) continue;
+        const double2 v = X[q.gam_col[g]];
+        const double w = q.gam_val[g];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (row - j * N == i) {
+                out[i].x += v.x * w;
+                out[i].y += v.y * w;
+            }
+    }
+}
+
+// N >= 3: a team of N lanes per emitter, lane j owning column j of rho, its
+// history and its derivative; H and the matrix being differentiated are
+// shared through LDS.  Same arithmetic as k_ml_density, without the N^2
+// register arrays per lane that spill from N = 3 on.
+template <typename T, int N>
+__global__ void __launch_bounds__(128) k_ml_team(QeDev q, const T* E0, const T* E1, const T* E2, int nx, int ny, int nz,
+                                                 const int* stepp) {
+    constexpr int S = N * N, EB = 128 / N;
+    __shared__ double2 Hs[EB][S];
+    __shared__ double2 Xs[EB][S];
+    __shared__ double2 red[128];
+    __shared__ double Pl[128][3];
+    const int team = threadIdx.x / N, j = threadIdx.x - team * N;
+    const int e = blockIdx.x * EB + team;
+    const bool act = team < EB && e < q.nemit;
+    const int step = *stepp;
+    double2 pop[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    double P[3] = {0.0, 0.0, 0.0};
+    double ef[3] = {0.0, 0.0, 0.0};
+    const T* Es[3] = {E0, E1, E2};
+    int px = 0, py = 0, pz = 0;
+    if (act) {
+        px = q.xyz[3 * e];
+        py = q.xyz[3 * e + 1];
+        pz = q.xyz[3 * e + 2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (!Es[c]) continue;
+            long long i0 = px + static_cast<long long>(nx) * (py + static_cast<long long>(ny) * pz);
+            if (q.dim == 2) ef[c] = static_cast<double>(Es[c][i0]);
+            else {
+                // mean of the two Yee edges E_c(p) and E_c(p - e_c) (getE_TE)
+                const int qx = px - (c == 0), qy = py - (c == 1), qz = pz - (c == 2);
+                double v1 = (qx >= 0 && qy >= 0 && qz >= 0) ? static_cast<double>(Es[c][qx + static_cast<long long>(nx) * (qy + static_cast<long long>(ny) * qz)]) : 0.0;
+                ef[c] = 0.5 * static_cast<double>(Es[c][i0]) + 0.5 * v1;
+            }
+        }
+    }
+    const int u = step & 3;
+    const int s0 = (u + 3) & 3, s1 = (u + 2) & 3, s2 = (u + 1) & 3, s3 = u;
+    const long long ld = static_cast<long long>(q.nmem) * q.nemit;
+    const double dt = q.dt;
+    double2* Hm = Hs[act ? team : 0];
+    double2* Xm = Xs[act ? team : 0];
+    for (int m = 0; m < q.nmem; ++m) {
+        const long long col = static_cast<long long>(m) * q.nemit + e;
+        double2 rc[N], fp[N];
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int k = i + j * N;
+                double2 h = q.h0[m * S + k];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 mu = q.mu[c * S + k];
+                    h.x -= ef[c] * mu.x;
+                    h.y -= ef[c] * mu.y;
+                }
+                Hm[k] = h;
+                const double2 rk = q.rho[k * ld + col];
+                const double2 f0 = q.hist[s0 * S * ld + k * ld + col];
+                const double2 f1 = q.hist[s1 * S * ld + k * ld + col];
+                const double2 f2 = q.hist[s2 * S * ld + k * ld + col];
+                const double2 f3 = q.hist[s3 * S * ld + k * ld + col];
+                double2 pr = rk;
+                pr = cadd(pr, cscale(f0, 55.0 * dt / 24.0));
+                pr = cadd(pr, cscale(f1, -59.0 * dt / 24.0));
+                pr = cadd(pr, cscale(f2, 37.0 * dt / 24.0));
+                pr = cadd(pr, cscale(f3, -9.0 * dt / 24.0));
+                Xm[k] = pr;
+                double2 cr = rk;
+                cr = cadd(cr, cscale(f0, 19.0 * dt / 24.0));
+                cr = cadd(cr, cscale(f1, -5.0 * dt / 24.0));
+                cr = cadd(cr, cscale(f2, dt / 24.0));
+                rc[i] = cr;
+            }
+        }
+        __syncthreads();
+        if (act) den_deriv_col<N>(q, Hm, Xm, j, fp);
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int k = i + j * N;
+                rc[i] = cadd(rc[i], cscale(fp[i], 9.0 * dt / 24.0));
+                q.rho[k * ld + col] = rc[i];
+                Xm[k] = rc[i];
+            }
+        }
+        __syncthreads();
+        if (act) {
+            den_deriv_col<N>(q, Hm, Xm, j, fp);
+#pragma unroll
+            for (int i = 0; i < N; ++i) q.hist[s3 * S * ld + (i + j * N) * ld + col] = fp[i];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                if (jj >= q.npop) break;
+                const int k = q.pop_flat[jj];
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    if (k == i + j * N) pop[jj] = cadd(pop[jj], rc[i]);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (!Es[c]) continue;
+                double sacc = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const double2 mu = q.mu[c * S + i + j * N];
+                    sacc += rc[i].x * mu.x + rc[i].y * mu.y;
+                }
+                P[c] += q.na * sacc;
+            }
+        }
+        __syncthreads();  // Hs / Xs are rewritten by the next member
+    }
+    // polarisation of the emitter: sum of its lanes' column parts
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Pl[threadIdx.x][c] = P[c];
+    __syncthreads();
+    if (act && j == 0) {
+        const int bx = px - q.bmin[0] + 1, by = py - q.bmin[1] + 1, bz = (q.dim == 3) ? pz - q.bmin[2] + 1 : 0;
+        const int PX = q.bn[0] + 2, PY = q.bn[1] + 2;
+        for (int c = 0; c < 3; ++c) {
+            if (!Es[c]) continue;
+            double sum = 0.0;
+            for (int l = 0; l < N; ++l) sum += Pl[threadIdx.x + l][c];
+            q.Pbox[c][bx + PX * (by + PY * bz)] = sum;
+        }
+    }
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        if (jj >= q.npop) break;
+        red[threadIdx.x] = pop[jj];
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) q.pop_partial[jj * gridDim.x + blockIdx.x] = red[0];
         __syncthreads();
     }
 }
@@ -1326,7 +1523,9 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         if (qi.npop > 4) throw std::runtime_error("at most 4 population detectors per emitter group");
         q.pop_flat = c->upload<int>(qi.pop_flat, std::max(qi.npop, 1));
         q.pop_interval = qi.pop_interval > 0 ? qi.pop_interval : 1;
-        qh.nblocks = (q.nemit + 127) / 128;
+        // emitters per 128-thread block: one per thread for N = 2, one team of N lanes otherwise
+        const int per_block = q.N < 3 ? 128 : 128 / q.N;
+        qh.nblocks = (q.nemit + per_block - 1) / per_block;
         q.pop_partial = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(qh.nblocks)));
         q.pop_cap = p->nsteps / q.pop_interval + 1;
         q.pop_out = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(q.pop_cap)));
@@ -1437,14 +1636,17 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
         switch (qh.N) {
 #define CHIML_ML_CASE(NN) \
     case NN: hipLaunchKernelGGL((k_ml_density<T, NN>), g, dim3(128), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
+#define CHIML_ML_TEAM(NN) \
+    case NN: hipLaunchKernelGGL((k_ml_team<T, NN>), g, dim3(128), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
             CHIML_ML_CASE(2)
-            CHIML_ML_CASE(3)
-            CHIML_ML_CASE(4)
-            CHIML_ML_CASE(5)
-            CHIML_ML_CASE(6)
-            CHIML_ML_CASE(7)
-            CHIML_ML_CASE(8)
+            CHIML_ML_TEAM(3)
+            CHIML_ML_TEAM(4)
+            CHIML_ML_TEAM(5)
+            CHIML_ML_TEAM(6)
+            CHIML_ML_TEAM(7)
+            CHIML_ML_TEAM(8)
 #undef CHIML_ML_CASE
+#undef CHIML_ML_TEAM
         }
         if (q.npop > 0) hipLaunchKernelGGL(k_ml_pop_finish, dim3(1), dim3(64), 0, c->stream, q, qh.nblocks, a.step);
     }

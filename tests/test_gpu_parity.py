@@ -117,3 +117,28 @@ def test_3d_split_kernels(variant, precision, tol_f, tol_flux):
     err = rel_linf(sim.fluxArr()[0].getFlux(), orc.flux(0))
     print("flux rel", err, flush=True)
     assert err <= tol_flux
+
+
+@pytest.mark.parametrize("nlev", [3, 4])
+def test_3d_n_level_emitter_block_fp64(nlev):
+    """A 3x3x3 block of N-level emitters (seeded Hermitian H0 and dipoles)
+    driven by a point source: fields, populations and one density matrix."""
+    d = decks.empty_deck((20, 20, 20), 0.02, 90, 4, precision="fp64")
+    em = decks.n_level_emitter(nlev, [0.04, 0.04, 0.04], (0.01, 0.01, 0.01), seed=40 + nlev)
+    d["ObjectList"].append(em)
+    fc = decks.ev2fdtd(2.0)
+    d["SourceList"].append({"pol": "Ez", "size": [0.0, 0.0, 0.0], "loc": [-0.09, 0.01, 0.01],
+                            "PulseList": [{"profile": "gaussian", "fcen": fc, "fwidth": fc, "cutoff": 3.0,
+                                           "t_0": 0.3, "Field_Intensity": 1e10}]})
+    deck = json.dumps(d)
+    print(f"n-level block N={nlev}", flush=True)
+    sim, orc = _pair(deck, 90)
+    _fields(sim, orc, range(6), FP64_TOL)
+    q = sim.qeArr()[0]
+    for j in range(2):
+        pg, po = q.population(j), orc.pops(0, j)
+        assert pg.shape == po.shape
+        assert rel_linf(pg, po) <= 1e-10, (j, rel_linf(pg, po))
+    rg = q.density(0, 13, nlev)
+    ro = orc.density(0, 0, 13, nlev)
+    assert rel_linf(rg, ro) <= 1e-10
