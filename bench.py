@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--sweep-steps", type=int, nargs=2, default=[100, 1000],
                    help="step range of the sweep sample (the full sweep uses 1000 10000)")
     p.add_argument("--no-sweep-baseline", action="store_true", help="skip the one-at-a-time replay")
+    p.add_argument("--no-configs", action="store_true",
+                   help="skip the other workloads (2D TMz, 2D and 3D Maxwell-Liouville)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a plain 256 MiB device copy three times (the known-size kernel that scales "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -152,6 +154,51 @@ def kernel_table(r):
     return out
 
 
+def run_workload(deck, graph_steps, profile_kernel=None):
+    """Mcell-updates/s of one deck run to its end on the GPU (after a short warmup)."""
+    import torch
+    import chiml_amd
+
+    sim = chiml_amd.FDTDField(deck)
+    cells = sim.cost()[0]
+    warm = min(64, sim.nsteps // 10)
+    if graph_steps:
+        sim.set_graph(graph_steps)
+    sim.step(warm)
+    sim.sync()
+    if profile_kernel and not graph_steps:
+        sim.profile_only(profile_kernel)
+    steps = sim.nsteps - warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    out = {"mcells_per_s": round(cells * steps / (t1 - t0) / 1e6, 1), "ms_per_step": round((t1 - t0) / steps * 1e3, 4),
+           "steps": steps, "cells": int(cells), "graph_steps": graph_steps}
+    if profile_kernel and not graph_steps:
+        ms, n = sim.profile_read(profile_kernel)
+        if n:
+            b = sim.kernel_bytes(profile_kernel)
+            out[profile_kernel] = {"ms_per_launch": round(ms / n, 5), "GBps": round(b / (ms / n * 1e-3) / 1e9, 1)}
+    sim.close()
+    return out
+
+
+def run_configs():
+    """The other workloads of the path, each at its full size: 2D TMz 256^2
+    (2000 steps, FP64), 2D Maxwell-Liouville 1024^2 with 32,768 2-level
+    emitters (20,000 steps, FP32 fields / FP64 rho) and 3D Maxwell-Liouville
+    192^3 with 368,640 6-level emitters over a Drude film (5000 steps)."""
+    from chiml_amd import decks
+
+    res = {}
+    res["tmz_256_cylinder_fp64"] = run_workload(decks.config1_2d(n=256, steps=2000, profile="gaussian"), 64)
+    res["ml2d_1024_2level_fp32"] = run_workload(decks.config3_ml2d(n=1024, steps=20000), 64)
+    res["ml3d_192_6level_fp32"] = run_workload(decks.config4_ml3d(n=192, steps=5000), 0, "ml_density")
+    return res
+
+
 def run_sweep(args):
     """Serving leg: seeded requests (Poisson 20/s; 2D 128^2-1024^2 and 3D
     64^3-192^3; dielectric or Drude object; a quarter with 2- or 4-level
@@ -247,6 +294,8 @@ def main():
             a = complex(*r32["flux0"])
             b = complex(*r64["flux0"])
             out["flux_f0_rel_err_fp32_vs_fp64"] = abs(a - b) / abs(b) if abs(b) > 0 else None
+    if not args.no_configs:
+        out["other_workloads"] = run_configs()
     if args.sweep_requests > 0:
         out["sweep"] = run_sweep(args)
     if not args.no_cpu:
