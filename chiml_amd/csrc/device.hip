@@ -726,6 +726,88 @@ __global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int n
     e_group<T, W, true, SRC>(a, mt, x0, y, z);
 }
 
+// 2D TMz on W-cell groups over the whole domain (CPML handled per group; the
+// x-slab membership is uniform within a group as in 3D).
+template <typename T, int W, bool SRC>
+__global__ void __launch_bounds__(256) k_h2v(StepArgs<T> a) {
+    const int nx = a.nx, ny = a.ny, gx = nx / W;
+    const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(gx) * ny) return;
+    const int y = static_cast<int>(t / gx), x0 = static_cast<int>(t - static_cast<long long>(y) * gx) * W;
+    const long long i = x0 + static_cast<long long>(nx) * y;
+    const T* Ez = a.F[2];
+    const Pk<T, W> ez = ldv<T, W>(Ez + i);
+    const bool last = x0 + W >= nx;
+    const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;
+    const bool src = SRC && group_has_src<T, W>(a, x0, y, 0);
+    const int step = src ? *a.step : 0;
+    T d[W], term[W];
+    if (y < ny - 1) {  // Hx += ch [-dEz/dy]
+        const Pk<T, W> ezy = ldv<T, W>(Ez + i + nx);
+        Pk<T, W> h = ldv<T, W>(a.F[3] + i);
+#pragma unroll
+        for (int w = 0; w < W; ++w) d[w] = ezy.v[w] - ez.v[w];
+        pml_group<T, W>(a.pml[3], 1, y, x0, y, 0, nx, ny, d, a.inv_d[1], all, term);
+        const T ch = a.ch[0][0];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            T hv = h.v[w] + ch * (-term[w]);
+            if (src) hv += src_value(a, 3, x0 + w, y, 0, step);
+            h.v[w] = hv;
+        }
+        stv<T, W>(a.F[3] + i, h);
+    }
+    {  // Hy += ch [dEz/dx], x < nx - 1
+        const T ezn = last ? T(0) : Ez[i + W];
+        Pk<T, W> h = ldv<T, W>(a.F[4] + i);
+#pragma unroll
+        for (int w = 0; w < W; ++w) d[w] = (w + 1 < W ? ez.v[w + 1] : ezn) - ez.v[w];
+        pml_group<T, W>(a.pml[4], 0, x0, x0, y, 0, nx, ny, d, a.inv_d[0], vx, term);
+        const T ch = a.ch[1][0];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            if (!(vx & (1u << w))) continue;
+            T hv = h.v[w] + ch * term[w];
+            if (src) hv += src_value(a, 4, x0 + w, y, 0, step);
+            h.v[w] = hv;
+        }
+        stv<T, W>(a.F[4] + i, h);
+    }
+}
+
+template <typename T, int W, bool SRC>
+__global__ void __launch_bounds__(256) k_e2v(StepArgs<T> a, int nm2) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int nm[3] = {0, 0, nm2};
+    load_mat(a, mt, nm);
+    const int nx = a.nx, ny = a.ny, gx = nx / W;
+    const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(gx) * ny) return;
+    const int y = static_cast<int>(t / gx), x0 = static_cast<int>(t - static_cast<long long>(y) * gx) * W;
+    const long long i = x0 + static_cast<long long>(nx) * y;
+    const T *Hx = a.F[3], *Hy = a.F[4];
+    const Pk<T, W> hx = ldv<T, W>(Hx + i), hy = ldv<T, W>(Hy + i);
+    const Pk<T, W> hxy = y > 0 ? ldv<T, W>(Hx + i - nx) : zpk<T, W>();
+    const T hym = x0 > 0 ? Hy[i - 1] : T(0);
+    const unsigned all = (1u << W) - 1;
+    const bool src = SRC && group_has_src<T, W>(a, x0, y, 0);
+    const int step = src ? *a.step : 0;
+    T d1[W], d2[W], t1[W], t2[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        d1[w] = hy.v[w] - (w > 0 ? hy.v[w - 1] : hym);
+        d2[w] = hx.v[w] - hxy.v[w];
+    }
+    pml_group<T, W>(a.pml[2], 0, x0, x0, y, 0, nx, ny, d1, a.inv_d[0], all, t1);
+    pml_group<T, W>(a.pml[2], 1, y, x0, y, 0, nx, ny, d2, a.inv_d[1], all, t2);
+    const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[2], i);
+    Pk<T, W> e = ldv<T, W>(a.F[2] + i);
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        e.v[w] = e_new(a, mt[2][id.v[w]], 2, x0 + w, y, 0, e.v[w], t1[w] - t2[w], src ? src_value(a, 2, x0 + w, y, 0, step) : T(0));
+    stv<T, W>(a.F[2] + i, e);
+}
+
 // ----------------------------------------------------- step counter ---------
 __global__ void k_advance(int* step) { *step += 1; }
 
@@ -1284,6 +1366,7 @@ struct chiml_dev {
     bool device_flush = false;  // launch the DFT flush every step; it decides on the device
     hipGraphExec_t gexec = nullptr;
     bool split = false;  // 3D interior + CPML-slab kernels
+    bool vec2d = false;  // 2D TMz W-cell group kernels
     Box3 ibox{};
     SlabMap slab{};
     int ndet = 0, totpts = 0, ndft = 0, nfreq = 0, dft_interval = 1;
@@ -1604,6 +1687,22 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
         }
         return;
     }
+    if (DIM == 2 && c->vec2d) {
+        constexpr int W = 16 / sizeof(T);
+        const long long ng = static_cast<long long>(c->nx / W) * c->ny;
+        const dim3 g(static_cast<unsigned>((ng + 255) / 256)), blk(256);
+        {
+            Timed t(c, "update_h");
+            if (c->has_src) hipLaunchKernelGGL((k_h2v<T, W, true>), g, blk, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_h2v<T, W, false>), g, blk, 0, c->stream, a);
+        }
+        {
+            Timed t(c, "update_e");
+            if (c->has_src) hipLaunchKernelGGL((k_e2v<T, W, true>), g, blk, 0, c->stream, a, c->nmat[2]);
+            else hipLaunchKernelGGL((k_e2v<T, W, false>), g, blk, 0, c->stream, a, c->nmat[2]);
+        }
+        return;
+    }
     dim3 blk(64, 4, 1);
     dim3 grd((c->nx + 63) / 64, (c->ny + 3) / 4, c->nz);
     {
@@ -1714,6 +1813,10 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
         c->step_ctr.alloc(sizeof(int), c->stream, true);
         if (c->prec == 32) build<float>(c.get(), p, c->a32, c->s32);
         else build<double>(c.get(), p, c->a64, c->s64);
+        if (c->dim == 2) {
+            const int W = c->prec == 32 ? 4 : 2;
+            c->vec2d = (p->nx % W) == 0 && (p->pml_T[0] % W) == 0 && !c->hmap;
+        }
         if (c->dim == 3) {
             const int Tx = p->pml_T[0], Ty = p->pml_T[1], Tz = p->pml_T[2];
             Box3 b{Tx, p->nx - Tx, Ty, p->ny - Ty, Tz, p->nz - Tz};

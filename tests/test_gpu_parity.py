@@ -142,3 +142,18 @@ def test_3d_n_level_emitter_block_fp64(nlev):
     rg = q.density(0, 13, nlev)
     ro = orc.density(0, 0, 13, nlev)
     assert rel_linf(rg, ro) <= 1e-10
+
+
+def test_2d_tmz_fp32_close_to_fp64_oracle():
+    """FP32 fields against the FP64 oracle: detector series within 1e-5 of the
+    series' peak, fields within 1e-4."""
+    deck = decks.config1_2d(n=64, steps=300, profile="gaussian", pml=8, cyl_radius_cells=12, src_cell=(20, 32),
+                            ndet=4, precision="fp32")
+    sim, orc = _pair(deck, 300)
+    _fields(sim, orc, (2, 3, 4), 1e-4)
+    for d in range(orc.num_detectors()):
+        v_o = orc.detector(d)[1]
+        v_g = sim.dtcArr()[d].series()[1]
+        err = rel_linf(v_g, v_o)
+        print(f"detector {d}: rel_linf={err:.3e}", flush=True)
+        assert err <= 1e-5
