@@ -20,6 +20,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -31,6 +32,30 @@ namespace chiml {
 namespace {
 
 thread_local std::string g_sim_err;  // last error of this host thread
+
+// Runs f(lo, hi, t) over [0, n) split into contiguous chunks, one per host
+// thread (up to 16); the first exception thrown by any chunk is rethrown.
+template <typename F>
+int parallel_chunks(int n, F f) {
+    int nt = static_cast<int>(std::thread::hardware_concurrency());
+    nt = std::max(1, std::min({nt, 16, n}));
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> err(nt);
+    for (int t = 0; t < nt; ++t) {
+        const int lo = static_cast<int>(static_cast<long long>(n) * t / nt), hi = static_cast<int>(static_cast<long long>(n) * (t + 1) / nt);
+        th.emplace_back([&, lo, hi, t] {
+            try {
+                f(lo, hi, t);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    }
+    for (auto& x : th) x.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+    return nt;
+}
 
 // Staggering of each Yee component along x, y, z (1 = half-integer position):
 // Ex (1,0,0) Ey (0,1,0) Ez (0,0,1) Hx (0,1,1) Hy (1,0,1) Hz (1,1,0)
@@ -150,6 +175,7 @@ class FDTDField {
 
   private:
     int objAt(std::array<double, 3> pt, bool ml) const;
+    void objAt2(const std::array<double, 3>& pt, int& withML, int& withoutML) const;
     void scanMatInPML();
 };
 
@@ -161,6 +187,20 @@ int FDTDField::objAt(std::array<double, 3> pt, bool ml) const {
         if (relevant && o.isObj(pt, IP.d[0], ml)) id = static_cast<int>(oo);
     }
     return id;
+}
+
+// objAt(pt, true) and objAt(pt, false) in one pass over the objects.
+void FDTDField::objAt2(const std::array<double, 3>& pt, int& withML, int& withoutML) const {
+    withML = withoutML = 0;
+    for (size_t oo = 1; oo < IP.objs.size(); ++oo) {
+        const Object& o = *IP.objs[oo];
+        bool relevant = o.ML || !o.pols.empty() || o.anyEpsNotOne() || o.mu != 1.0;
+        if (!relevant) continue;
+        const bool a = o.isObj(pt, IP.d[0], true);
+        const bool b = o.ML ? o.isObj(pt, IP.d[0], false) : a;
+        if (a) withML = static_cast<int>(oo);
+        if (b) withoutML = static_cast<int>(oo);
+    }
 }
 
 // Material-in-PML scan, src/FDTD_MANAGER/parallelFDTDField.hpp:274-370.  The
@@ -271,51 +311,112 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     int dispBox[3][6];
     for (int c : ecomps) {
         const int* st = kStag[c];
-        std::map<std::tuple<int, double, int>, int> ids;
         matE[c].assign(N, 0);
         physOf[c].assign(N, 0);
-        int bx[6] = {nx, -1, ny, -1, nz, -1};
-        for (int z = 0; z < nz; ++z)
-            for (int y = 0; y < ny; ++y)
-                for (int x = 0; x < nx; ++x) {
-                    std::array<double, 3> pt = {{coord(0, x, 0.5 * st[0]), coord(1, y, 0.5 * st[1]), is3D ? coord(2, z, 0.5 * st[2]) : 0.0}};
-                    if (!is3D) pt[2] = (-1 + 0.5 * st[2] - (nAll[2] - nAll[2] % 2) / 2.0) * IP.d[2];  // kk - 1 with kk = 0
-                    int phys = objAt(pt, true);
-                    int epsObj = objAt(pt, false);
-                    double eps = IP.objs[epsObj]->eps[c];
-                    const size_t i = x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z);
-                    physOf[c][i] = phys;
-                    bool pml = inPML(x, y, z);
+        // Material ids are numbered in order of first appearance in x-fastest
+        // scan order.  Chunks of z-planes (y-rows in 2D) are scanned in
+        // parallel with local numbering, then merged in chunk order, which
+        // gives the same numbering as one sequential scan.
+        using Key = std::tuple<int, double, int>;
+        const int outer = is3D ? nz : ny;
+        std::vector<std::vector<Key>> lkeys(16);
+        std::vector<std::array<int, 6>> lbox(16, std::array<int, 6>{{nx, -1, ny, -1, nz, -1}});
+        std::vector<std::pair<int, int>> lrange(16, {0, 0});
+        const int nt = parallel_chunks(outer, [&](int lo, int hi, int t) {
+            lrange[t] = {lo, hi};
+            std::map<Key, int> lid;
+            Key last{-1, 0.0, -1};
+            int lastId = -1;
+            auto& bx = lbox[t];
+            for (int o = lo; o < hi; ++o) {
+                const int z0 = is3D ? o : 0, z1 = is3D ? o + 1 : 1;
+                const int y0 = is3D ? 0 : o, y1 = is3D ? ny : o + 1;
+                for (int z = z0; z < z1; ++z)
+                    for (int y = y0; y < y1; ++y)
+                        for (int x = 0; x < nx; ++x) {
+                            std::array<double, 3> pt = {{coord(0, x, 0.5 * st[0]), coord(1, y, 0.5 * st[1]), is3D ? coord(2, z, 0.5 * st[2]) : 0.0}};
+                            if (!is3D) pt[2] = (-1 + 0.5 * st[2] - (nAll[2] - nAll[2] % 2) / 2.0) * IP.d[2];  // kk - 1 with kk = 0
+                            int phys, epsObj;
+                            objAt2(pt, phys, epsObj);
+                            const double eps = IP.objs[epsObj]->eps[c];
+                            const size_t i = x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z);
+                            physOf[c][i] = phys;
+                            const bool pml = inPML(x, y, z);
+                            const Object& po = *IP.objs[phys];
+                            const bool disp = !po.alpha.empty() || po.ML;
+                            if (pml && disp && !dielectricMatInPML)
+                                throw std::logic_error("dispersive or emitter material inside the PML without dielectric material in the PML is not supported");
+                            const Key key = std::make_tuple(phys, eps, pml ? 1 : 0);
+                            int id;
+                            if (key == last) {
+                                id = lastId;
+                            } else {
+                                auto it = lid.find(key);
+                                if (it == lid.end()) {
+                                    id = static_cast<int>(lkeys[t].size());
+                                    if (id >= CHIML_MAX_MAT) throw std::logic_error("too many distinct materials");
+                                    lid[key] = id;
+                                    lkeys[t].push_back(key);
+                                } else {
+                                    id = it->second;
+                                }
+                                last = key;
+                                lastId = id;
+                            }
+                            matE[c][i] = static_cast<uint8_t>(id);  // local id, remapped after the merge
+                            if (poleSetOf[phys] >= 0) {
+                                bx[0] = std::min(bx[0], x);
+                                bx[1] = std::max(bx[1], x + 1);
+                                bx[2] = std::min(bx[2], y);
+                                bx[3] = std::max(bx[3], y + 1);
+                                bx[4] = std::min(bx[4], z);
+                                bx[5] = std::max(bx[5], z + 1);
+                            }
+                        }
+            }
+        });
+        // merge: global ids in first-appearance order over the chunks
+        std::map<Key, int> ids;
+        std::vector<std::vector<uint8_t>> remap(nt);
+        for (int t = 0; t < nt; ++t) {
+            for (const Key& key : lkeys[t]) {
+                auto it = ids.find(key);
+                int id;
+                if (it == ids.end()) {
+                    id = static_cast<int>(ids.size());
+                    if (id >= CHIML_MAX_MAT) throw std::logic_error("too many distinct materials");
+                    ids[key] = id;
+                    const int phys = std::get<0>(key);
+                    const double eps = std::get<1>(key);
+                    const bool pml = std::get<2>(key) != 0;
                     const Object& po = *IP.objs[phys];
-                    bool disp = !po.alpha.empty() || po.ML;
-                    if (pml && disp && !dielectricMatInPML)
-                        throw std::logic_error("dispersive or emitter material inside the PML without dielectric material in the PML is not supported");
-                    auto key = std::make_tuple(phys, eps, pml ? 1 : 0);
-                    auto it = ids.find(key);
-                    int id;
-                    if (it == ids.end()) {
-                        id = static_cast<int>(ids.size());
-                        if (id >= CHIML_MAX_MAT) throw std::logic_error("too many distinct materials");
-                        ids[key] = id;
-                        double ce = (pml && !dielectricMatInPML) ? dt : dt / eps;
-                        ceT[c].push_back(ce);
-                        ievT[c].push_back(1.0 / eps);
-                        int fl = (disp || (pml && dielectricMatInPML)) ? CHIML_MAT_DCELL : 0;
-                        flT[c].push_back(fl);
-                        psT[c].push_back(poleSetOf[phys]);
-                    } else {
-                        id = it->second;
-                    }
-                    matE[c][i] = static_cast<uint8_t>(id);
-                    if (poleSetOf[phys] >= 0) {
-                        bx[0] = std::min(bx[0], x);
-                        bx[1] = std::max(bx[1], x + 1);
-                        bx[2] = std::min(bx[2], y);
-                        bx[3] = std::max(bx[3], y + 1);
-                        bx[4] = std::min(bx[4], z);
-                        bx[5] = std::max(bx[5], z + 1);
-                    }
+                    const bool disp = !po.alpha.empty() || po.ML;
+                    ceT[c].push_back((pml && !dielectricMatInPML) ? dt : dt / eps);
+                    ievT[c].push_back(1.0 / eps);
+                    flT[c].push_back((disp || (pml && dielectricMatInPML)) ? CHIML_MAT_DCELL : 0);
+                    psT[c].push_back(poleSetOf[phys]);
+                } else {
+                    id = it->second;
                 }
+                remap[t].push_back(static_cast<uint8_t>(id));
+            }
+        }
+        parallel_chunks(nt, [&](int lo, int hi, int) {
+            for (int t = lo; t < hi; ++t) {
+                const size_t plane = is3D ? static_cast<size_t>(nx) * ny : static_cast<size_t>(nx);
+                const size_t b = static_cast<size_t>(lrange[t].first) * plane, e = static_cast<size_t>(lrange[t].second) * plane;
+                for (size_t i = b; i < e; ++i) matE[c][i] = remap[t][matE[c][i]];
+            }
+        });
+        int bx[6] = {nx, -1, ny, -1, nz, -1};
+        for (int t = 0; t < nt; ++t) {
+            bx[0] = std::min(bx[0], lbox[t][0]);
+            bx[1] = std::max(bx[1], lbox[t][1]);
+            bx[2] = std::min(bx[2], lbox[t][2]);
+            bx[3] = std::max(bx[3], lbox[t][3]);
+            bx[4] = std::min(bx[4], lbox[t][4]);
+            bx[5] = std::max(bx[5], lbox[t][5]);
+        }
         if (bx[1] < 0) bx[0] = bx[1] = bx[2] = bx[3] = bx[4] = bx[5] = 0;
         for (int j = 0; j < 6; ++j) dispBox[c][j] = bx[j];
         P.mat_e[c] = matE[c].data();
@@ -335,13 +436,16 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         P.mat_h_ch[c] = &chv;
         const int* st = kStag[hc];
         physOf[hc].assign(N, 0);
-        for (int z = 0; z < nz; ++z)
-            for (int y = 0; y < ny; ++y)
+        parallel_chunks(nz * ny, [&](int lo, int hi, int) {
+            for (int r = lo; r < hi; ++r) {
+                const int z = r / ny, y = r % ny;
                 for (int x = 0; x < nx; ++x) {
                     std::array<double, 3> pt = {{coord(0, x, 0.5 * st[0]), coord(1, y, 0.5 * st[1]), is3D ? coord(2, z, 0.5 * st[2]) : 0.0}};
                     if (!is3D) pt[2] = (-1 + 0.5 * st[2] - (nAll[2] - nAll[2] % 2) / 2.0) * IP.d[2];  // kk - 1 with kk = 0
                     physOf[hc][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)] = objAt(pt, true);
                 }
+            }
+        });
     }
     (void)kOne;
     P.npoleset = static_cast<int>(pcount.size());

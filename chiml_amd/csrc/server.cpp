@@ -3,7 +3,7 @@
 //
 // Three kinds of threads touch a request, one at a time:
 //   caller threads   submit / poll / take (under the server mutex);
-//   the builder      parses the deck and builds the simulation (host tables,
+//   the builders     parse the deck and build the simulation (host tables,
 //                    uploads) for admitted requests, off the stepping thread;
 //   the stepper      polls every running simulation's stream and enqueues the
 //                    next chunk of steps on each one that has drained, so
@@ -66,7 +66,8 @@ struct chiml_server {
     double cell_updates = 0.0;
     long long next_id = 1;
     bool stop = false;
-    std::thread builder, stepper;
+    std::vector<std::thread> builders;  // host-side setup of admitted requests runs here
+    std::thread stepper;
 
     // Moves requests from the queue to the build list while they fit.
     void admit_locked() {
@@ -207,7 +208,10 @@ int chiml_server_create(size_t hbm_budget, int max_active, int chunk_steps, int 
     s->chunk = chunk_steps;
     s->graph = graph_steps;
     chiml_server* p = s.get();
-    p->builder = std::thread([p] { p->build_loop(); });
+    // a few builders: host-side setup of a large 3D deck takes about a second
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int nb = static_cast<int>(std::max(1u, std::min(4u, hc / 4)));
+    for (int b = 0; b < nb; ++b) p->builders.emplace_back([p] { p->build_loop(); });
     p->stepper = std::thread([p] { p->step_loop(); });
     *out = s.release();
     return CHIML_OK;
@@ -300,7 +304,7 @@ void chiml_server_destroy(chiml_server_t* s) {
         s->queue.clear();
     }
     s->cv_build.notify_all();
-    s->builder.join();
+    for (auto& b : s->builders) b.join();
     s->stepper.join();
     for (auto& kv : s->reqs)
         if (kv.second->sim) chiml_sim_destroy(kv.second->sim);
