@@ -150,6 +150,8 @@ struct FluxBox {
 
 }  // namespace
 
+void check_pml_axes(const ProgramInputs& IP);
+
 class FDTDField {
   public:
     explicit FDTDField(const std::string& deck);
@@ -256,6 +258,7 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     const size_t N = static_cast<size_t>(nx) * ny * nz;
     const double dt = IP.dt;
     const int nsteps = IP.nSteps();
+    check_pml_axes(IP);
     scanMatInPML();
     for (auto& o : IP.objs)
         if (o->mu != 1.0) throw std::logic_error("magnetic materials (mu != 1) are outside the supported time-stepping path");
@@ -1078,6 +1081,22 @@ std::vector<cplx> FDTDField::flux(int fi) {
     return total;
 }
 
+// A CPML on some axes only is rejected: with a zero-thickness axis a next to
+// CPML slabs on the others, the reference's slab lists for H reach one plane
+// further along a (parallelPML.hpp:590-626: mx -= lnpl - 1 with lnpl = 0) and
+// update the H components that carry an a-derivative on the last plane
+// against the zero ghost E, a boundary treatment this path does not carry.
+void check_pml_axes(const ProgramInputs& IP) {
+    const int na = IP.is3D() ? 3 : 2;
+    bool any = false, zero = false;
+    for (int a = 0; a < na; ++a) {
+        if (IP.pmlThickness[a] > 0) any = true;
+        else zero = true;
+    }
+    if (any && zero)
+        throw std::logic_error("a CPML on some axes only (zero thickness on another) is not supported; give every axis a CPML or none");
+}
+
 // Sets this thread's message for chiml_sim_last_error (used by the server).
 void set_sim_error(const std::string& m) { g_sim_err = m; }
 
@@ -1261,6 +1280,7 @@ int chiml_sim_estimate_bytes(const char* deck, size_t* bytes) {
     if (!deck || !bytes) return sim_fail("null argument");
     try {
         auto IP = chiml::parseInputs(deck);
+        chiml::check_pml_axes(IP);
         auto n = IP.nVec();
         if (!IP.is3D()) n[2] = 1;
         double cells = static_cast<double>(n[0]) * n[1] * n[2];

@@ -68,6 +68,7 @@ def test_estimate_monotone():
     (lambda d: d["CompCell"].__setitem__("dt", 1.0), "stable time step"),
     (lambda d: d["PML"].__setitem__("thickness", [5.0, 5.0, 0.0]), "PML size is larger"),
     (lambda d: d["CompCell"].__setitem__("pol", "Hz"), "TMz"),
+    (lambda d: d["PML"].__setitem__("thickness", [0.2, 0.0, 0.0]), "CPML on some axes only"),
     (lambda d: d["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "loc": [0, 0, 0],
                                        "PulseList": [{"profile": "sinc", "fcen": 1.0}]}), "Pulse profile undefined"),
     (lambda d: d["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "loc": [50.0, 0, 0],

@@ -97,8 +97,8 @@ def _split_deck(variant, precision):
     dx = 0.02
     if variant == "wide":  # nx = 40, ny = nz = 32
         d["CompCell"]["size"][0] = 39 * dx
-    elif variant == "zfree":  # no CPML along z: no z slabs
-        d["PML"]["thickness"][2] = 0.0
+    elif variant == "thinz":  # a 2-cell CPML along z: thin z slabs
+        d["PML"]["thickness"][2] = 2 * dx
     elif variant == "pmlsrc":  # extra H point sources in the low-x slab and in a corner
         # (E sources on CPML cells filled with material land on D and are dropped, as in the reference)
         for pol, loc in (("Hy", [-14 * dx, 0.0, 0.0]), ("Hz", [-13 * dx, 13 * dx, -14 * dx])):
@@ -107,7 +107,7 @@ def _split_deck(variant, precision):
     return json.dumps(d)
 
 
-@pytest.mark.parametrize("variant", ["cube", "wide", "zfree", "pmlsrc"])
+@pytest.mark.parametrize("variant", ["cube", "wide", "thinz", "pmlsrc"])
 @pytest.mark.parametrize("precision,tol_f,tol_flux", [("fp64", FP64_TOL, 1e-8), ("fp32", 1e-4, 1e-5)])
 def test_3d_split_kernels(variant, precision, tol_f, tol_flux):
     deck = _split_deck(variant, precision)
