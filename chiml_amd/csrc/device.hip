@@ -28,6 +28,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,11 @@ constexpr int kDftBlock = 32;  // samples staged before one DFT contraction
 constexpr int kMaxLevels = 8;
 
 thread_local std::string g_err;  // last error of this host thread
+// Stream capture of one context must not overlap calls that may synchronise
+// the device from other threads (allocation, free, blocking copies): those
+// would invalidate the capture.  Capture, create, destroy and the blocking
+// reads take this lock; plain step launches do not.
+std::mutex g_capture_mu;
 
 #define HIP_TRY(expr)                                                                                \
     do {                                                                                             \
@@ -1797,6 +1803,7 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
     if (p->precision != 32 && p->precision != 64) return fail(nullptr, CHIML_E_INVALID, "precision must be 32 or 64");
     if (p->nsteps < 0) return fail(nullptr, CHIML_E_INVALID, "nsteps must be >= 0");
     if (p->nsrc > CHIML_MAX_SRC) return fail(nullptr, CHIML_E_INVALID, "too many sources");
+    std::lock_guard<std::mutex> lk(g_capture_mu);
     auto c = std::make_unique<chiml_dev>();
     try {
         int ndev = 0;
@@ -1862,6 +1869,7 @@ void enqueue_steps(chiml_dev* c, int n) {
 // every chunk of the run.
 void ensure_graph(chiml_dev* c) {
     if (c->gexec) return;
+    std::lock_guard<std::mutex> lk(g_capture_mu);
     hipGraph_t g = nullptr;
     const int saved = c->steps_done;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1941,6 +1949,7 @@ int chiml_dev_sync(chiml_dev_t* c) {
 int chiml_dev_read_field(chiml_dev_t* c, int comp, double* host) {
     if (!c || !host || comp < 0 || comp > 5) return fail(c, CHIML_E_INVALID, "invalid argument");
     size_t N = static_cast<size_t>(c->nx) * c->ny * c->nz;
+    std::lock_guard<std::mutex> lk(g_capture_mu);
     try {
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (c->prec == 32) {
@@ -1966,6 +1975,7 @@ int chiml_dev_read_points(chiml_dev_t* c, int d, double* values, size_t cap, siz
     size_t n = static_cast<size_t>(ns) * c->det_npts[d];
     if (!values) return CHIML_OK;
     if (cap < n) return fail(c, CHIML_E_INVALID, "buffer too small");
+    std::lock_guard<std::mutex> lk(g_capture_mu);
     try {
         HIP_TRY(hipStreamSynchronize(c->stream));
         HIP_TRY(hipMemcpy(values, c->det_out + c->det_out_off[d], n * sizeof(double), hipMemcpyDeviceToHost));
@@ -1980,6 +1990,7 @@ int chiml_dev_read_dft(chiml_dev_t* c, double* reim, size_t cap) {
     size_t n = 2 * static_cast<size_t>(c->ndft) * c->nfreq;
     if (n == 0) return CHIML_OK;
     if (!reim || cap < n) return fail(c, CHIML_E_INVALID, "buffer too small");
+    std::lock_guard<std::mutex> lk(g_capture_mu);
     try {
         // Contract the partially filled staging block into a copy of F.
         int kmax = c->steps_done / c->dft_interval;
@@ -2025,6 +2036,7 @@ int chiml_dev_read_pops(chiml_dev_t* c, int q, int j, double* reim, size_t cap, 
     *nsamples = static_cast<size_t>(ns);
     if (!reim) return CHIML_OK;
     if (cap < 2 * static_cast<size_t>(ns)) return fail(c, CHIML_E_INVALID, "buffer too small");
+    std::lock_guard<std::mutex> lk(g_capture_mu);
     try {
         HIP_TRY(hipStreamSynchronize(c->stream));
         HIP_TRY(hipMemcpy(reim, d.pop_out + static_cast<size_t>(j) * d.pop_cap, 2 * ns * sizeof(double), hipMemcpyDeviceToHost));
@@ -2040,6 +2052,7 @@ int chiml_dev_read_density(chiml_dev_t* c, int q, int m, int e, double* reim) {
     if (m < 0 || m >= d.nmem || e < 0 || e >= d.nemit) return fail(c, CHIML_E_INVALID, "index out of range");
     const int S = d.N * d.N;
     const size_t ld = static_cast<size_t>(d.nmem) * d.nemit;
+    std::lock_guard<std::mutex> lk(g_capture_mu);
     try {
         HIP_TRY(hipStreamSynchronize(c->stream));
         for (int k = 0; k < S; ++k)
@@ -2094,6 +2107,7 @@ const char* chiml_dev_global_error(void) { return g_err.c_str(); }
 
 void chiml_dev_destroy(chiml_dev_t* c) {
     if (!c) return;
+    std::lock_guard<std::mutex> lk(g_capture_mu);
     (void)hipStreamSynchronize(c->stream);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (auto& pe : c->pending) {
