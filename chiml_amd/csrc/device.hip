@@ -23,6 +23,8 @@
 // src/PML/parallelPML.hpp:693-765 and src/PML/parallelPML.cpp:12-40.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -542,7 +544,63 @@ __device__ __forceinline__ Pk<uint8_t, W> load_ids(const uint8_t* m, long long i
     return *reinterpret_cast<const Pk<uint8_t, W>*>(m + i);
 }
 
-// E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.
+// E of the W cells of a group from their curl terms t1 - t2, materials,
+// sources and poles: e_new for a group, with each pole's P and J of the
+// group read and written as one aligned 16-byte access (the pole-state box
+// is 4-aligned in x); cells without poles keep their stored values.
+template <typename T, int W>
+__device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], int c, long long i, int x0,
+                                               int y, int z, unsigned valid, const T (&t1)[W], const T (&t2)[W], bool src, int step) {
+    const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[c], i);
+    Pk<T, W> e = ldv<T, W>(a.F[c] + i);
+    T jsum[W];
+    bool anyp = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        jsum[w] = T(0);
+        anyp |= (mt[c][id.v[w]].poleset >= 0) && (valid & (1u << w));
+    }
+    if (anyp) {  // rare: groups that touch a dispersive object
+        const int* bx = a.box[c];
+        const long long bi = (x0 - bx[0]) + static_cast<long long>(bx[1] - bx[0]) * ((y - bx[2]) + static_cast<long long>(bx[3] - bx[2]) * (z - bx[4]));
+        int np[W], npmax = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int ps = mt[c][id.v[w]].poleset;
+            np[w] = (ps >= 0 && (valid & (1u << w))) ? a.pole_count[ps] : 0;
+            npmax = max(npmax, np[w]);
+        }
+        for (int p = 0; p < npmax; ++p) {
+            T* Pp = a.P[c] + p * a.boxn[c] + bi;
+            T* Jp = a.J[c] + p * a.boxn[c] + bi;
+            Pk<T, W> P = ldv<T, W>(Pp), J = ldv<T, W>(Jp);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if (p >= np[w]) continue;
+                const int ps = mt[c][id.v[w]].poleset;
+                const T* co = a.pole_coef + (static_cast<long long>(ps) * CHIML_MAX_POLES + p) * 3;
+                const T sv = src ? src_value(a, c, x0 + w, y, z, step) : T(0);
+                const T Jn = co[0] * P.v[w] + co[1] * J.v[w] + co[2] * (e.v[w] + sv);
+                P.v[w] = P.v[w] + Jn;
+                J.v[w] = Jn;
+                jsum[w] += Jn;
+            }
+            stv<T, W>(Pp, P);
+            stv<T, W>(Jp, J);
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        if (!(valid & (1u << w))) continue;
+        const MatLds<T> m = mt[c][id.v[w]];
+        const T sv = src ? src_value(a, c, x0 + w, y, z, step) : T(0);
+        const T e_base = (m.flags & CHIML_MAT_DCELL) ? e.v[w] : e.v[w] + sv;
+        e.v[w] = e_base + m.ce * (t1[w] - t2[w]) - m.inveps * jsum[w];
+    }
+    stv<T, W>(a.F[c] + i, e);
+}
+
+// E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.// E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.
 template <typename T, int W, bool PML, bool SRC>
 __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], int x0, int y, int z) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -574,13 +632,7 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
                 t2[w] = d2[w] * idz;
             }
         }
-        const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[0], i);
-        Pk<T, W> e = ldv<T, W>(a.F[0] + i);
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-            if (vx & (1u << w))
-                e.v[w] = e_new(a, mt[0][id.v[w]], 0, x0 + w, y, z, e.v[w], t1[w] - t2[w], src ? src_value(a, 0, x0 + w, y, z, step) : T(0));
-        stv<T, W>(a.F[0] + i, e);
+        e_finish_group<T, W>(a, mt, 0, i, x0, y, z, vx, t1, t2, src, step);
     }
     if (y < ny - 1) {  // Ey += ce [dHx/dz - dHz/dx]
         const Pk<T, W> hxz = z > 0 ? ldv<T, W>(Hx + i - sz) : zpk<T, W>();
@@ -600,12 +652,7 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
                 t2[w] = d2[w] * idx;
             }
         }
-        const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[1], i);
-        Pk<T, W> e = ldv<T, W>(a.F[1] + i);
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-            e.v[w] = e_new(a, mt[1][id.v[w]], 1, x0 + w, y, z, e.v[w], t1[w] - t2[w], src ? src_value(a, 1, x0 + w, y, z, step) : T(0));
-        stv<T, W>(a.F[1] + i, e);
+        e_finish_group<T, W>(a, mt, 1, i, x0, y, z, all, t1, t2, src, step);
     }
     if (z < nz - 1) {  // Ez += ce [dHy/dx - dHx/dy]
         const Pk<T, W> hxy = y > 0 ? ldv<T, W>(Hx + i - sy) : zpk<T, W>();
@@ -625,12 +672,7 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
                 t2[w] = d2[w] * idy;
             }
         }
-        const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[2], i);
-        Pk<T, W> e = ldv<T, W>(a.F[2] + i);
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-            e.v[w] = e_new(a, mt[2][id.v[w]], 2, x0 + w, y, z, e.v[w], t1[w] - t2[w], src ? src_value(a, 2, x0 + w, y, z, step) : T(0));
-        stv<T, W>(a.F[2] + i, e);
+        e_finish_group<T, W>(a, mt, 2, i, x0, y, z, all, t1, t2, src, step);
     }
 }
 
@@ -806,12 +848,7 @@ __global__ void __launch_bounds__(256) k_e2v(StepArgs<T> a, int nm2) {
     }
     pml_group<T, W>(a.pml[2], 0, x0, x0, y, 0, nx, ny, d1, a.inv_d[0], all, t1);
     pml_group<T, W>(a.pml[2], 1, y, x0, y, 0, nx, ny, d2, a.inv_d[1], all, t2);
-    const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[2], i);
-    Pk<T, W> e = ldv<T, W>(a.F[2] + i);
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-        e.v[w] = e_new(a, mt[2][id.v[w]], 2, x0 + w, y, 0, e.v[w], t1[w] - t2[w], src ? src_value(a, 2, x0 + w, y, 0, step) : T(0));
-    stv<T, W>(a.F[2] + i, e);
+    e_finish_group<T, W>(a, mt, 2, i, x0, y, 0, all, t1, t2, src, step);
 }
 
 // ----------------------------------------------------- step counter ---------
@@ -942,6 +979,7 @@ struct QeDev {
     const int* gam_row;
     const int* gam_col;
     const double* gam_val;
+    const int* gam_colptr;  // [N + 1]: entries whose row lies in density column j
     double na, hbar, dt;
     int bmin[3], bn[3];     // emitter box
     double* Pbox[3];        // (bn+2)^3 node polarization, z-extent bn+1+zoff
@@ -1166,9 +1204,8 @@ __device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* H, 
         }
         out[i] = make_double2(-(s1.y - s2.y) * ih, (s1.x - s2.x) * ih);
     }
-    for (int g = 0; g < q.ngam; ++g) {
+    for (int g = q.gam_colptr[j]; g < q.gam_colptr[j + 1]; ++g) {
         const int row = q.gam_row[g];
-        if (row / N != j) continue;
         const double2 v = X[q.gam_col[g]];
         const double w = q.gam_val[g];
 #pragma unroll
@@ -1462,10 +1499,16 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
     for (int k = 0; k < p->npoleset; ++k) maxp = std::max(maxp, pc[k]);
     for (int k = 0; k < 3; ++k) {
         for (int j = 0; j < 6; ++j) a.box[k][j] = p->disp_box[k][j];
-        long long bn = 0;
-        if (p->disp_box[k][1] > p->disp_box[k][0] && p->disp_box[k][3] > p->disp_box[k][2] && p->disp_box[k][5] > p->disp_box[k][4])
-            bn = static_cast<long long>(p->disp_box[k][1] - p->disp_box[k][0]) * (p->disp_box[k][3] - p->disp_box[k][2]) *
-                 (p->disp_box[k][5] - p->disp_box[k][4]);
+        // pole-state storage x-range aligned to 4 cells, so that a W-cell group
+        // of the vector kernels is one aligned 16-byte access (the extra
+        // storage belongs to cells without poles and is never changed)
+        if (a.box[k][1] > a.box[k][0]) {
+            a.box[k][0] = a.box[k][0] / 4 * 4;
+            a.box[k][1] = std::min((a.box[k][1] + 3) / 4 * 4, nx);
+        }
+        long long bn = 0;  // cells of the (aligned) pole-state box
+        if (a.box[k][1] > a.box[k][0] && a.box[k][3] > a.box[k][2] && a.box[k][5] > a.box[k][4])
+            bn = static_cast<long long>(a.box[k][1] - a.box[k][0]) * (a.box[k][3] - a.box[k][2]) * (a.box[k][5] - a.box[k][4]);
         a.boxn[k] = bn;
         a.P[k] = (bn > 0 && maxp > 0) ? c->zeros<T>(bn * maxp) : nullptr;
         a.J[k] = (bn > 0 && maxp > 0) ? c->zeros<T>(bn * maxp) : nullptr;
@@ -1584,9 +1627,27 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         q.h0 = reinterpret_cast<const double2*>(c->upload<double>(qi.h0, 2 * static_cast<size_t>(qi.nmembers) * S));
         q.mu = reinterpret_cast<const double2*>(c->upload<double>(qi.mu, 2 * 3 * static_cast<size_t>(S)));
         q.ngam = qi.ngam;
-        q.gam_row = c->upload<int>(qi.gam_row, std::max(qi.ngam, 1));
-        q.gam_col = c->upload<int>(qi.gam_col, std::max(qi.ngam, 1));
-        q.gam_val = c->upload<double>(qi.gam_val, std::max(qi.ngam, 1));
+        {
+            // Lindblad entries grouped by the density column their row lies in
+            // (stable, so each row keeps its accumulation order); gam_colptr[j]
+            // .. gam_colptr[j+1] are the entries a team's lane j applies.
+            std::vector<int> order(qi.ngam);
+            for (int g = 0; g < qi.ngam; ++g) order[g] = g;
+            std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return qi.gam_row[u] / qi.nlevel < qi.gam_row[v] / qi.nlevel; });
+            std::vector<int> gr(std::max(qi.ngam, 1)), gc(std::max(qi.ngam, 1)), cp(qi.nlevel + 1, 0);
+            std::vector<double> gv(std::max(qi.ngam, 1));
+            for (int g = 0; g < qi.ngam; ++g) {
+                gr[g] = qi.gam_row[order[g]];
+                gc[g] = qi.gam_col[order[g]];
+                gv[g] = qi.gam_val[order[g]];
+                ++cp[gr[g] / qi.nlevel + 1];
+            }
+            for (int j = 0; j < qi.nlevel; ++j) cp[j + 1] += cp[j];
+            q.gam_row = c->upload<int>(gr.data(), gr.size());
+            q.gam_col = c->upload<int>(gc.data(), gc.size());
+            q.gam_val = c->upload<double>(gv.data(), gv.size());
+            q.gam_colptr = c->upload<int>(cp.data(), cp.size());
+        }
         q.na = qi.na;
         q.hbar = qi.hbar;
         q.dt = p->dt;

@@ -119,7 +119,7 @@ def test_3d_split_kernels(variant, precision, tol_f, tol_flux):
     assert err <= tol_flux
 
 
-@pytest.mark.parametrize("nlev", [3, 4])
+@pytest.mark.parametrize("nlev", [3, 4, 6])
 def test_3d_n_level_emitter_block_fp64(nlev):
     """A 3x3x3 block of N-level emitters (seeded Hermitian H0 and dipoles)
     driven by a point source: fields, populations and one density matrix."""
