@@ -400,6 +400,7 @@ __device__ __forceinline__ void load_mat(const StepArgs<T>& a, MatLds<T> (*mt)[C
 
 struct Box3 {
     int x0, x1, y0, y1, z0, z1;
+    int by;  // rows per y band of the interior walk (0: one band)
 };
 
 // CPML term of derivative axis ax for the W cells of a group (pml_term for a
@@ -676,17 +677,31 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
     }
 }
 
-// Interior box, flattened over groups (x fastest), one group per thread.
+// Interior box, one group per thread, walked band by band: x fastest, then
+// the by rows of a y band, then z, then the band.  With the XCD renumbering
+// each XCD marches one band through z, so the rows of the previous and next
+// planes a group reads are still in that XCD's L2 (a whole plane of six
+// fields is larger than an L2).
+template <typename T, int W>
+__device__ __forceinline__ long long int_groups(const Box3& b) {
+    const int gw = (b.x1 - b.x0) / W, ny = b.y1 - b.y0, nz = b.z1 - b.z0;
+    const int by = b.by > 0 ? b.by : ny, nb = (ny + by - 1) / by;
+    return static_cast<long long>(gw) * by * nz * nb;
+}
+
 template <typename T, int W>
 __device__ __forceinline__ bool int_group(const Box3& b, long long t, int& x0, int& y, int& z) {
-    const int gw = (b.x1 - b.x0) / W, ny = b.y1 - b.y0;
-    const long long ng = static_cast<long long>(gw) * ny * (b.z1 - b.z0);
-    if (t >= ng) return false;
-    const long long r = t / gw;
+    const int gw = (b.x1 - b.x0) / W, ny = b.y1 - b.y0, nz = b.z1 - b.z0;
+    const int by = b.by > 0 ? b.by : ny;
+    if (t >= int_groups<T, W>(b)) return false;
+    long long r = t / gw;
     x0 = b.x0 + static_cast<int>(t - r * gw) * W;
-    y = b.y0 + static_cast<int>(r % ny);
-    z = b.z0 + static_cast<int>(r / ny);
-    return true;
+    const int yy = static_cast<int>(r % by);
+    r /= by;
+    z = b.z0 + static_cast<int>(r % nz);
+    const int band = static_cast<int>(r / nz);
+    y = b.y0 + band * by + yy;
+    return y < b.y1;  // the last band may be short
 }
 
 template <typename T, int W, bool SRC>
@@ -1729,7 +1744,8 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
     if (DIM == 3 && c->split) {
         constexpr int W = 16 / sizeof(T);
         const Box3 b = c->ibox;
-        const long long ni = static_cast<long long>((b.x1 - b.x0) / W) * (b.y1 - b.y0) * (b.z1 - b.z0);
+        const int nyb = b.y1 - b.y0, by = b.by > 0 ? b.by : nyb;
+        const long long ni = static_cast<long long>((b.x1 - b.x0) / W) * by * (b.z1 - b.z0) * ((nyb + by - 1) / by);
         const long long ns = c->slab.nA + c->slab.nB + c->slab.nC;
         const dim3 gi(static_cast<unsigned>((ni + 255) / 256)), gs(static_cast<unsigned>((ns + 255) / 256)), blk(256);
         {
@@ -1887,7 +1903,8 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
         }
         if (c->dim == 3) {
             const int Tx = p->pml_T[0], Ty = p->pml_T[1], Tz = p->pml_T[2];
-            Box3 b{Tx, p->nx - Tx, Ty, p->ny - Ty, Tz, p->nz - Tz};
+            Box3 b{Tx, p->nx - Tx, Ty, p->ny - Ty, Tz, p->nz - Tz, 0};
+            b.by = (b.y1 - b.y0 + 7) / 8;  // one y band per XCD
             // the split kernels take W = 16 / sizeof(T) cells per thread along x:
             // x rows and x slabs must hold whole groups, and there is no magnetic map
             const int W = c->prec == 32 ? 4 : 2;
