@@ -107,6 +107,7 @@ struct StepArgs {
     const T* pole_coef;  // [set][kMaxPoles][3]
     T* P[3];
     T* J[3];
+    T* jsum[3];  // pole-box sized: sum of the new pole currents of this step (split 3D path)
     int box[3][6];
     long long boxn[3];
     PmlDev<T> pml[6];
@@ -549,7 +550,7 @@ __device__ __forceinline__ Pk<uint8_t, W> load_ids(const uint8_t* m, long long i
 // sources and poles: e_new for a group, with each pole's P and J of the
 // group read and written as one aligned 16-byte access (the pole-state box
 // is 4-aligned in x); cells without poles keep their stored values.
-template <typename T, int W>
+template <typename T, int W, bool PRE = false>
 __device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], int c, long long i, int x0,
                                                int y, int z, unsigned valid, const T (&t1)[W], const T (&t2)[W], bool src, int step) {
     const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[c], i);
@@ -561,7 +562,14 @@ __device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLd
         jsum[w] = T(0);
         anyp |= (mt[c][id.v[w]].poleset >= 0) && (valid & (1u << w));
     }
-    if (anyp) {  // rare: groups that touch a dispersive object
+    if (PRE && anyp) {  // the pole currents were advanced by k_ade3 this step
+        const int* bx = a.box[c];
+        const long long bi = (x0 - bx[0]) + static_cast<long long>(bx[1] - bx[0]) * ((y - bx[2]) + static_cast<long long>(bx[3] - bx[2]) * (z - bx[4]));
+        const Pk<T, W> js = ldv<T, W>(a.jsum[c] + bi);
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            if (mt[c][id.v[w]].poleset >= 0 && (valid & (1u << w))) jsum[w] = js.v[w];
+    } else if (anyp) {  // rare: groups that touch a dispersive object
         const int* bx = a.box[c];
         const long long bi = (x0 - bx[0]) + static_cast<long long>(bx[1] - bx[0]) * ((y - bx[2]) + static_cast<long long>(bx[3] - bx[2]) * (z - bx[4]));
         int np[W], npmax = 0;
@@ -602,7 +610,7 @@ __device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLd
 }
 
 // E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.// E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.
-template <typename T, int W, bool PML, bool SRC>
+template <typename T, int W, bool PML, bool SRC, bool PRE>
 __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], int x0, int y, int z) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const long long sy = nx, sz = static_cast<long long>(nx) * ny;
@@ -633,7 +641,7 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
                 t2[w] = d2[w] * idz;
             }
         }
-        e_finish_group<T, W>(a, mt, 0, i, x0, y, z, vx, t1, t2, src, step);
+        e_finish_group<T, W, PRE>(a, mt, 0, i, x0, y, z, vx, t1, t2, src, step);
     }
     if (y < ny - 1) {  // Ey += ce [dHx/dz - dHz/dx]
         const Pk<T, W> hxz = z > 0 ? ldv<T, W>(Hx + i - sz) : zpk<T, W>();
@@ -653,7 +661,7 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
                 t2[w] = d2[w] * idx;
             }
         }
-        e_finish_group<T, W>(a, mt, 1, i, x0, y, z, all, t1, t2, src, step);
+        e_finish_group<T, W, PRE>(a, mt, 1, i, x0, y, z, all, t1, t2, src, step);
     }
     if (z < nz - 1) {  // Ez += ce [dHy/dx - dHx/dy]
         const Pk<T, W> hxy = y > 0 ? ldv<T, W>(Hx + i - sy) : zpk<T, W>();
@@ -673,7 +681,7 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
                 t2[w] = d2[w] * idy;
             }
         }
-        e_finish_group<T, W>(a, mt, 2, i, x0, y, z, all, t1, t2, src, step);
+        e_finish_group<T, W, PRE>(a, mt, 2, i, x0, y, z, all, t1, t2, src, step);
     }
 }
 
@@ -720,7 +728,66 @@ __global__ void __launch_bounds__(256) k_e3_int(StepArgs<T> a, Box3 b, int nm0, 
     const long long blk = xcd_block(blockIdx.x, gridDim.x);
     int x0, y, z;
     if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z)) return;
-    e_group<T, W, false, SRC>(a, mt, x0, y, z);
+    e_group<T, W, false, SRC, true>(a, mt, x0, y, z);
+}
+
+// Pole update of the 3D split path, before the E kernels: one thread per
+// W-cell group of a component's pole box (blockIdx.y = component), so the
+// few dispersive cells are spread over the whole chip instead of loading the
+// XCDs whose part of the domain holds the object.  It advances P and J from
+// E before its update (plus the soft source, as updatePolE sees it) and
+// leaves the sum of the new currents per cell in jsum for the E kernels.
+template <typename T, int W, bool SRC>
+__global__ void __launch_bounds__(256) k_ade3(StepArgs<T> a) {
+    const int c = blockIdx.y;
+    const int* bx = a.box[c];
+    if (!a.jsum[c]) return;
+    const int gw = (bx[1] - bx[0]) / W, ny = bx[3] - bx[2], nz = bx[5] - bx[4];
+    const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (t >= static_cast<long long>(gw) * ny * nz) return;
+    const long long r = t / gw;
+    const int x0 = bx[0] + static_cast<int>(t - r * gw) * W;
+    const int y = bx[2] + static_cast<int>(r % ny), z = bx[4] + static_cast<int>(r / ny);
+    const long long bi = t * W;  // (x0 - bx0) + width * ((y - by0) + height * (z - bz0))
+    const long long i = lidx(x0, y, z, a.nx, a.ny);
+    const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[c], i);
+    // the cells the E kernels update: Ex x < nx - 1, Ey y < ny - 1, Ez z < nz - 1
+    unsigned valid = (1u << W) - 1;
+    if (c == 0 && x0 + W >= a.nx) valid >>= 1;
+    if ((c == 1 && y >= a.ny - 1) || (c == 2 && z >= a.nz - 1)) valid = 0;
+    int ps[W], np[W], npmax = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        ps[w] = (valid & (1u << w)) ? a.poleset[c][id.v[w]] : -1;
+        np[w] = ps[w] >= 0 ? a.pole_count[ps[w]] : 0;
+        npmax = max(npmax, np[w]);
+    }
+    Pk<T, W> js = zpk<T, W>();
+    if (npmax > 0) {
+        const bool src = SRC && group_has_src<T, W>(a, x0, y, z);
+        const int step = src ? *a.step : 0;
+        const Pk<T, W> e = ldv<T, W>(a.F[c] + i);
+        T ea[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) ea[w] = e.v[w] + (src ? src_value(a, c, x0 + w, y, z, step) : T(0));
+        for (int p = 0; p < npmax; ++p) {
+            T* Pp = a.P[c] + p * a.boxn[c] + bi;
+            T* Jp = a.J[c] + p * a.boxn[c] + bi;
+            Pk<T, W> P = ldv<T, W>(Pp), J = ldv<T, W>(Jp);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if (p >= np[w]) continue;
+                const T* co = a.pole_coef + (static_cast<long long>(ps[w]) * CHIML_MAX_POLES + p) * 3;
+                const T Jn = co[0] * P.v[w] + co[1] * J.v[w] + co[2] * ea[w];
+                P.v[w] = P.v[w] + Jn;
+                J.v[w] = Jn;
+                js.v[w] += Jn;
+            }
+            stv<T, W>(Pp, P);
+            stv<T, W>(Jp, J);
+        }
+    }
+    stv<T, W>(a.jsum[c] + bi, js);
 }
 
 // CPML slab groups: the union of the z slabs, the y slabs inside the z range
@@ -786,7 +853,7 @@ __global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int n
     if (t >= m.nA + m.nB + m.nC) return;
     int x0, y, z;
     slab_group(m, t, x0, y, z);
-    e_group<T, W, true, SRC>(a, mt, x0, y, z);
+    e_group<T, W, true, SRC, true>(a, mt, x0, y, z);
 }
 
 // 2D TMz on W-cell groups over the whole domain (CPML handled per group; the
@@ -1425,6 +1492,7 @@ struct chiml_dev {
     hipGraphExec_t gexec = nullptr;
     bool split = false;  // 3D interior + CPML-slab kernels
     bool vec2d = false;  // 2D TMz W-cell group kernels
+    long long ade_groups = 0;  // largest pole-box group count over the E components (split path)
     Box3 ibox{};
     SlabMap slab{};
     int ndet = 0, totpts = 0, ndft = 0, nfreq = 0, dft_interval = 1;
@@ -1527,6 +1595,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         a.boxn[k] = bn;
         a.P[k] = (bn > 0 && maxp > 0) ? c->zeros<T>(bn * maxp) : nullptr;
         a.J[k] = (bn > 0 && maxp > 0) ? c->zeros<T>(bn * maxp) : nullptr;
+        a.jsum[k] = (bn > 0 && maxp > 0 && nz > 1) ? c->zeros<T>(bn) : nullptr;
     }
     const int nn[3] = {nx, ny, nz};
     for (int comp = 0; comp < 6; ++comp) {
@@ -1758,6 +1827,12 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
             if (c->has_src) hipLaunchKernelGGL((k_h3_slab<T, W, true>), gs, blk, 0, c->stream, a, c->slab);
             else hipLaunchKernelGGL((k_h3_slab<T, W, false>), gs, blk, 0, c->stream, a, c->slab);
         }
+        if (c->ade_groups > 0) {
+            Timed t(c, "ade");
+            const dim3 ga(static_cast<unsigned>((c->ade_groups + 255) / 256), 3);
+            if (c->has_src) hipLaunchKernelGGL((k_ade3<T, W, true>), ga, blk, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_ade3<T, W, false>), ga, blk, 0, c->stream, a);
+        }
         {
             Timed t(c, "update_e");
             if (c->has_src) hipLaunchKernelGGL((k_e3_int<T, W, true>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
@@ -1919,6 +1994,12 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
                 m.nA = 2LL * Tz * gx * p->ny;
                 m.nB = 2LL * Ty * gx * (p->nz - 2 * Tz);
                 m.nC = 2LL * (Tx / W) * (p->ny - 2 * Ty) * (p->nz - 2 * Tz);
+                for (int k = 0; k < 3; ++k) {
+                    const int* bx = c->prec == 32 ? c->a32.box[k] : c->a64.box[k];
+                    const bool has = c->prec == 32 ? c->a32.jsum[k] != nullptr : c->a64.jsum[k] != nullptr;
+                    if (has)
+                        c->ade_groups = std::max(c->ade_groups, static_cast<long long>((bx[1] - bx[0]) / W) * (bx[3] - bx[2]) * (bx[5] - bx[4]));
+                }
             }
         }
         // t = 0 samples (taken by the reference's constructor, parallelFDTDField.cpp:832-837)

@@ -985,7 +985,14 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
                     for (int x = b[0]; x < b[1]; ++x) {
                         int ps = psT[c][matE[c][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)]];
                         if (ps < 0) continue;
-                        kbytes[inInt(x, y, z) ? "update_e" : "update_e_pml"] += pcount[ps] * 4.0 * s;
+                        if (split) {
+                            // pole kernel: P and J read and written, E read, current sum written;
+                            // the E kernel reads the current sum back
+                            kbytes["ade"] += pcount[ps] * 4.0 * s + 2.0 * s;
+                            kbytes[inInt(x, y, z) ? "update_e" : "update_e_pml"] += s;
+                        } else {
+                            kbytes[inInt(x, y, z) ? "update_e" : "update_e_pml"] += pcount[ps] * 4.0 * s;
+                        }
                     }
         }
         kbytes["ml_density"] = bytesML;
@@ -994,7 +1001,7 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         if (ndft > 0)
             kbytes["dft_flush"] = static_cast<double>(ndft) * (32.0 * s + 2.0 * 16.0 * nfreq) + 32.0 * 16.0 * nfreq;
         bytesH = kbytes["update_h"] + kbytes["update_h_pml"];
-        bytesE = kbytes["update_e"] + kbytes["update_e_pml"];
+        bytesE = kbytes["update_e"] + kbytes["update_e_pml"] + kbytes["ade"];
         estimate = static_cast<size_t>(cells * (6.0 * s + 3.0) + psiWords * s + bytesML / 7.0 * 5.0 +
                                        static_cast<double>(ndft) * (nfreq * 16.0 + 32 * s));
     }
