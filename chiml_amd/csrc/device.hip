@@ -307,9 +307,9 @@ __global__ void __launch_bounds__(256) k_update_e(StepArgs<T> a) {
 }
 
 // ------------------------------------------------ 3D split kernels ---------
-// The 3D field update is split by region: an interior kernel over the box of
-// cells outside every CPML slab (no CPML code) and a slab kernel over the
-// CPML cells only.  Both give each thread W consecutive x cells so that every
+// The 3D field update is split by region: an interior kernel over whole x
+// rows of the y/z interior (CPML terms only where a row enters an x slab)
+// and a slab kernel over the z and y slabs.  Both give each thread W consecutive x cells so that every
 // field, psi and material access of a lane is one 16-byte load or store
 // (W = 4 in FP32, 2 in FP64): on MI355X a plain copy reaches ~3.8 TB/s with
 // 4-byte lanes and ~6.3 TB/s with 16-byte lanes.  The caller guarantees
@@ -712,15 +712,16 @@ __device__ __forceinline__ bool int_group(const Box3& b, long long t, int& x0, i
     return y < b.y1;  // the last band may be short
 }
 
-template <typename T, int W, bool SRC>
+// ROWS: the walk covers whole x rows, so the x slabs' CPML is applied inline
+template <typename T, int W, bool SRC, bool ROWS>
 __global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b) {
     const long long blk = xcd_block(blockIdx.x, gridDim.x);
     int x0, y, z;
     if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z)) return;
-    h_group<T, W, false, SRC>(a, x0, y, z);
+    h_group<T, W, ROWS, SRC>(a, x0, y, z);
 }
 
-template <typename T, int W, bool SRC>
+template <typename T, int W, bool SRC, bool ROWS>
 __global__ void __launch_bounds__(256) k_e3_int(StepArgs<T> a, Box3 b, int nm0, int nm1, int nm2) {
     __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
     const int nm[3] = {nm0, nm1, nm2};
@@ -728,7 +729,7 @@ __global__ void __launch_bounds__(256) k_e3_int(StepArgs<T> a, Box3 b, int nm0, 
     const long long blk = xcd_block(blockIdx.x, gridDim.x);
     int x0, y, z;
     if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z)) return;
-    e_group<T, W, false, SRC, true>(a, mt, x0, y, z);
+    e_group<T, W, ROWS, SRC, true>(a, mt, x0, y, z);
 }
 
 // Pole update of the 3D split path, before the E kernels: one thread per
@@ -1276,16 +1277,22 @@ __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const 
 template <int N>
 __device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* H, const double2* X, int j, double2 (&out)[N]) {
     const double ih = 1.0 / q.hbar;
+    // outer-product order over k (not unrolled): one column of H and of X
+    // in flight at a time keeps the register footprint small
+    double2 s1[N], s2[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
+    for (int i = 0; i < N; ++i) s1[i] = s2[i] = make_double2(0.0, 0.0);
+#pragma unroll 1
+    for (int k = 0; k < N; ++k) {
+        const double2 xk = X[k + j * N], hk = H[k + j * N];
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            s1 = cadd(s1, cmul(H[i + k * N], X[k + j * N]));
-            s2 = cadd(s2, cmul(X[i + k * N], H[k + j * N]));
+        for (int i = 0; i < N; ++i) {
+            s1[i] = cadd(s1[i], cmul(H[i + k * N], xk));
+            s2[i] = cadd(s2[i], cmul(X[i + k * N], hk));
         }
-        out[i] = make_double2(-(s1.y - s2.y) * ih, (s1.x - s2.x) * ih);
     }
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = make_double2(-(s1[i].y - s2[i].y) * ih, (s1[i].x - s2[i].x) * ih);
     for (int g = q.gam_colptr[j]; g < q.gam_colptr[j + 1]; ++g) {
         const int row = q.gam_row[g];
         const double2 v = X[q.gam_col[g]];
@@ -1819,8 +1826,9 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
         const dim3 gi(static_cast<unsigned>((ni + 255) / 256)), gs(static_cast<unsigned>((ns + 255) / 256)), blk(256);
         {
             Timed t(c, "update_h");
-            if (c->has_src) hipLaunchKernelGGL((k_h3_int<T, W, true>), gi, blk, 0, c->stream, a, b);
-            else hipLaunchKernelGGL((k_h3_int<T, W, false>), gi, blk, 0, c->stream, a, b);
+            constexpr bool R = W == 4;  // whole-row interior walk in FP32
+            if (c->has_src) hipLaunchKernelGGL((k_h3_int<T, W, true, R>), gi, blk, 0, c->stream, a, b);
+            else hipLaunchKernelGGL((k_h3_int<T, W, false, R>), gi, blk, 0, c->stream, a, b);
         }
         if (ns > 0) {
             Timed t(c, "update_h_pml");
@@ -1835,8 +1843,9 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
         }
         {
             Timed t(c, "update_e");
-            if (c->has_src) hipLaunchKernelGGL((k_e3_int<T, W, true>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
-            else hipLaunchKernelGGL((k_e3_int<T, W, false>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
+            constexpr bool R = W == 4;
+            if (c->has_src) hipLaunchKernelGGL((k_e3_int<T, W, true, R>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
+            else hipLaunchKernelGGL((k_e3_int<T, W, false, R>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
         }
         if (ns > 0) {
             Timed t(c, "update_e_pml");
@@ -1978,7 +1987,13 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
         }
         if (c->dim == 3) {
             const int Tx = p->pml_T[0], Ty = p->pml_T[1], Tz = p->pml_T[2];
-            Box3 b{Tx, p->nx - Tx, Ty, p->ny - Ty, Tz, p->nz - Tz, 0};
+            // FP32 (W = 4): the interior walk takes whole x rows of the y/z
+            // interior (one wavefront per 256-cell row, the x slabs' CPML terms
+            // applied inline) and the slab kernel covers the z and y slabs.
+            // FP64 (W = 2): the x slabs stay in the slab kernel, which measured
+            // faster there.
+            const bool rows = c->prec == 32;
+            Box3 b{rows ? 0 : Tx, rows ? p->nx : p->nx - Tx, Ty, p->ny - Ty, Tz, p->nz - Tz, 0};
             b.by = (b.y1 - b.y0 + 7) / 8;  // one y band per XCD
             // the split kernels take W = 16 / sizeof(T) cells per thread along x:
             // x rows and x slabs must hold whole groups, and there is no magnetic map
@@ -1993,7 +2008,7 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
                 const long long gx = p->nx / W;
                 m.nA = 2LL * Tz * gx * p->ny;
                 m.nB = 2LL * Ty * gx * (p->nz - 2 * Tz);
-                m.nC = 2LL * (Tx / W) * (p->ny - 2 * Ty) * (p->nz - 2 * Tz);
+                m.nC = rows ? 0 : 2LL * (Tx / W) * (p->ny - 2 * Ty) * (p->nz - 2 * Tz);
                 for (int k = 0; k < 3; ++k) {
                     const int* bx = c->prec == 32 ? c->a32.box[k] : c->a64.box[k];
                     const bool has = c->prec == 32 ? c->a32.jsum[k] != nullptr : c->a64.jsum[k] != nullptr;

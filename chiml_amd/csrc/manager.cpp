@@ -954,10 +954,13 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         // interior box of the 3D split kernels (see device.hip); 2D uses one kernel each
         const int W = IP.fp32 ? 4 : 2;  // x cells per thread of the split kernels
         const bool split = is3D && T[0] < nx - T[0] && T[1] < ny - T[1] && T[2] < nz - T[2] && (T[0] % W) == 0 && (nx % W) == 0;
-        const double Ni = split ? static_cast<double>(nx - 2 * T[0]) * (ny - 2 * T[1]) * (nz - 2 * T[2]) : cells;
+        // interior kernel: whole x rows of the y/z interior in FP32 (x slabs
+        // included), the y/z interior without the x slabs in FP64
+        const bool rows = IP.fp32;
+        const double Ni = split ? static_cast<double>(rows ? nx : nx - 2 * T[0]) * (ny - 2 * T[1]) * (nz - 2 * T[2]) : cells;
         const double Ns = cells - Ni;
         auto inInt = [&](int x, int y, int z) {
-            return !split || (x >= T[0] && x < nx - T[0] && y >= T[1] && y < ny - T[1] && z >= T[2] && z < nz - T[2]);
+            return !split || ((rows || (x >= T[0] && x < nx - T[0])) && y >= T[1] && y < ny - T[1] && z >= T[2] && z < nz - T[2]);
         };
         kbytes["update_h"] = Ni * (2.0 * nH + nE) * s;
         kbytes["update_e"] = Ni * ((2.0 * nE + nH) * s + nE);
@@ -974,8 +977,16 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
                     for (int o = 0; o < 3; ++o)
                         if (o != a) other *= (o == 0 ? nx : (o == 1 ? ny : nz));
                     double w = S * other;
-                    const char* k = comp < 3 ? (split ? "update_e_pml" : "update_e") : (split ? "update_h_pml" : "update_h");
-                    kbytes[k] += 2.0 * s * w;
+                    const char* kint = comp < 3 ? "update_e" : "update_h";
+                    const char* kpml = comp < 3 ? (split ? "update_e_pml" : "update_e") : (split ? "update_h_pml" : "update_h");
+                    if (split && rows && a == 0) {
+                        // x psi of the y/z-interior rows is updated by the interior kernel
+                        const double wi = S * (ny - 2 * T[1]) * (nz - 2 * T[2]);
+                        kbytes[kint] += 2.0 * s * wi;
+                        kbytes[kpml] += 2.0 * s * (w - wi);
+                    } else {
+                        kbytes[kpml] += 2.0 * s * w;
+                    }
                     psiWords += w;
                 }
         for (int c : ecomps) {
