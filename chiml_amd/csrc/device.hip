@@ -456,14 +456,17 @@ __device__ __forceinline__ void pml_group(const PmlDev<T>& p, int ax, int pos0, 
 }
 
 // H update of the group (x0..x0+W-1, y, z): H += ch (curl E) (+ sources).
+// nb: the next lane of the wave holds the next group of the same row, so the
+// first value of that group comes by a lane shuffle instead of a load
 template <typename T, int W, bool PML, bool SRC>
-__device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int z) {
+__device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int z, bool nb = false) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const long long sy = nx, sz = static_cast<long long>(nx) * ny;
     const long long i = lidx(x0, y, z, nx, ny);
     const T idx = a.inv_d[0], idy = a.inv_d[1], idz = a.inv_d[2];
     const T *Ex = a.F[0], *Ey = a.F[1], *Ez = a.F[2];
     const Pk<T, W> ex = ldv<T, W>(Ex + i), ey = ldv<T, W>(Ey + i), ez = ldv<T, W>(Ez + i);
+    const T ez_nb = __shfl_down(ez.v[0], 1), ey_nb = __shfl_down(ey.v[0], 1);
     const bool last = x0 + W >= nx;
     const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;  // cells with x < nx - 1
     const bool src = SRC && group_has_src<T, W>(a, x0, y, z);
@@ -498,7 +501,7 @@ __device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int
     }
     if (z < nz - 1) {  // Hy += ch [dEz/dx - dEx/dz], x < nx - 1
         const Pk<T, W> exz = ldv<T, W>(Ex + i + sz);
-        const T ezn = last ? T(0) : Ez[i + W];
+        const T ezn = last ? T(0) : (nb ? ez_nb : Ez[i + W]);
         Pk<T, W> h = ldv_nt<T, W>(a.F[4] + i);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -527,7 +530,7 @@ __device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int
     }
     if (y < ny - 1) {  // Hz += ch [dEx/dy - dEy/dx], x < nx - 1
         const Pk<T, W> exy = ldv<T, W>(Ex + i + sy);
-        const T eyn = last ? T(0) : Ey[i + W];
+        const T eyn = last ? T(0) : (nb ? ey_nb : Ey[i + W]);
         Pk<T, W> h = ldv_nt<T, W>(a.F[5] + i);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -631,14 +634,17 @@ __device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLd
 }
 
 // E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.// E update of the group: the E-form of updateE/updateD + ADE + D2E per cell.
+// nb: the previous lane of the wave holds the previous group of the same row
 template <typename T, int W, bool PML, bool SRC, bool PRE>
-__device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], int x0, int y, int z) {
+__device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], int x0, int y, int z,
+                                        bool nb = false) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const long long sy = nx, sz = static_cast<long long>(nx) * ny;
     const long long i = lidx(x0, y, z, nx, ny);
     const T idx = a.inv_d[0], idy = a.inv_d[1], idz = a.inv_d[2];
     const T *Hx = a.F[3], *Hy = a.F[4], *Hz = a.F[5];
     const Pk<T, W> hx = ldv<T, W>(Hx + i), hy = ldv<T, W>(Hy + i), hz = ldv<T, W>(Hz + i);
+    const T hz_nb = __shfl_up(hz.v[W - 1], 1), hy_nb = __shfl_up(hy.v[W - 1], 1);
     const bool last = x0 + W >= nx;
     const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;
     const bool src = SRC && group_has_src<T, W>(a, x0, y, z);
@@ -666,7 +672,7 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
     }
     if (y < ny - 1) {  // Ey += ce [dHx/dz - dHz/dx]
         const Pk<T, W> hxz = z > 0 ? ldv<T, W>(Hx + i - sz) : zpk<T, W>();
-        const T hzm = x0 > 0 ? Hz[i - 1] : T(0);
+        const T hzm = x0 > 0 ? (nb ? hz_nb : Hz[i - 1]) : T(0);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             d1[w] = hx.v[w] - hxz.v[w];
@@ -686,7 +692,7 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
     }
     if (z < nz - 1) {  // Ez += ce [dHy/dx - dHx/dy]
         const Pk<T, W> hxy = y > 0 ? ldv<T, W>(Hx + i - sy) : zpk<T, W>();
-        const T hym = x0 > 0 ? Hy[i - 1] : T(0);
+        const T hym = x0 > 0 ? (nb ? hy_nb : Hy[i - 1]) : T(0);
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             d1[w] = hy.v[w] - (w > 0 ? hy.v[w - 1] : hym);
@@ -719,12 +725,13 @@ __device__ __forceinline__ long long int_groups(const Box3& b) {
 }
 
 template <typename T, int W>
-__device__ __forceinline__ bool int_group(const Box3& b, long long t, int& x0, int& y, int& z) {
+__device__ __forceinline__ bool int_group(const Box3& b, long long t, int& x0, int& y, int& z, int& col) {
     const int gw = (b.x1 - b.x0) / W, ny = b.y1 - b.y0, nz = b.z1 - b.z0;
     const int by = b.by > 0 ? b.by : ny;
     if (t >= int_groups<T, W>(b)) return false;
     long long r = t / gw;
-    x0 = b.x0 + static_cast<int>(t - r * gw) * W;
+    col = static_cast<int>(t - r * gw);
+    x0 = b.x0 + col * W;
     const int yy = static_cast<int>(r % by);
     r /= by;
     z = b.z0 + static_cast<int>(r % nz);
@@ -737,9 +744,12 @@ __device__ __forceinline__ bool int_group(const Box3& b, long long t, int& x0, i
 template <typename T, int W, bool SRC, bool ROWS>
 __global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b) {
     const long long blk = xcd_block(blockIdx.x, gridDim.x);
-    int x0, y, z;
-    if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z)) return;
-    h_group<T, W, ROWS, SRC>(a, x0, y, z);
+    int x0, y, z, col;
+    const bool in = int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z, col);
+    if (!in) return;
+    // consecutive threads walk a row: lane + 1 holds the next group unless this one ends the row
+    const int lane = threadIdx.x & 63, gw = (b.x1 - b.x0) / W;
+    h_group<T, W, ROWS, SRC>(a, x0, y, z, lane < 63 && col + 1 < gw);
 }
 
 template <typename T, int W, bool SRC, bool ROWS>
@@ -748,9 +758,10 @@ __global__ void __launch_bounds__(256) k_e3_int(StepArgs<T> a, Box3 b, int nm0, 
     const int nm[3] = {nm0, nm1, nm2};
     load_mat(a, mt, nm);
     const long long blk = xcd_block(blockIdx.x, gridDim.x);
-    int x0, y, z;
-    if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z)) return;
-    e_group<T, W, ROWS, SRC, true>(a, mt, x0, y, z);
+    int x0, y, z, col;
+    if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z, col)) return;
+    const int lane = threadIdx.x & 63;
+    e_group<T, W, ROWS, SRC, true>(a, mt, x0, y, z, lane > 0 && col > 0);
 }
 
 // Pole update of the 3D split path, before the E kernels: one thread per
