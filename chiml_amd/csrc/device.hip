@@ -1095,6 +1095,7 @@ struct QeDev {
     const int* gam_col;
     const double* gam_val;
     const int* gam_colptr;  // [N + 1]: entries whose row lies in density column j
+    const int* gam_rc;      // [ngam]: row | col << 16 of each entry, in the same order
     double na, hbar, dt;
     int bmin[3], bn[3];     // emitter box
     double* Pbox[3];        // (bn+2)^3 node polarization, z-extent bn+1+zoff
@@ -1305,9 +1306,26 @@ __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const 
 }
 
 // Column j of d rho/dt for the matrix X held in LDS (column-major):
-// (i/hbar)(H X - X H)[:, j] + the Lindblad rows that fall in column j.
+// (i/hbar)(H X - X H)[:, j] + the Lindblad rows that fall in column j, with
+// H = h0 - sum_c ef_c mu_c formed element by element from the block's LDS
+// copies of h0 and mu (the same for every emitter of the member).
 template <int N>
-__device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* H, const double2* X, int j, double2 (&out)[N]) {
+__device__ __forceinline__ double2 ham(const double2* h0, const double2 (*mu)[N * N], const double (&ef)[3], int a) {
+    double2 h = h0[a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        h.x -= ef[c] * mu[c][a].x;
+        h.y -= ef[c] * mu[c][a].y;
+    }
+    return h;
+}
+
+// grc / gv / gcp: the Lindblad entries (packed row | col << 16, value) and
+// column pointers, staged in LDS by the caller when they fit.
+template <int N>
+__device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* h0, const double2 (*mu)[N * N], const double (&ef)[3],
+                                              const double2* X, int j, const int* grc, const double* gv, const int* gcp,
+                                              double2 (&out)[N]) {
     const double ih = 1.0 / q.hbar;
     // outer-product order over k (not unrolled): one column of H and of X
     // in flight at a time keeps the register footprint small
@@ -1316,19 +1334,20 @@ __device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* H, 
     for (int i = 0; i < N; ++i) s1[i] = s2[i] = make_double2(0.0, 0.0);
 #pragma unroll 1
     for (int k = 0; k < N; ++k) {
-        const double2 xk = X[k + j * N], hk = H[k + j * N];
+        const double2 xk = X[k + j * N], hk = ham<N>(h0, mu, ef, k + j * N);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            s1[i] = cadd(s1[i], cmul(H[i + k * N], xk));
+            s1[i] = cadd(s1[i], cmul(ham<N>(h0, mu, ef, i + k * N), xk));
             s2[i] = cadd(s2[i], cmul(X[i + k * N], hk));
         }
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) out[i] = make_double2(-(s1[i].y - s2[i].y) * ih, (s1[i].x - s2[i].x) * ih);
-    for (int g = q.gam_colptr[j]; g < q.gam_colptr[j + 1]; ++g) {
-        const int row = q.gam_row[g];
-        const double2 v = X[q.gam_col[g]];
-        const double w = q.gam_val[g];
+    for (int g = gcp[j]; g < gcp[j + 1]; ++g) {
+        const int rc = grc[g];
+        const int row = rc & 0xffff;
+        const double2 v = X[rc >> 16];
+        const double w = gv[g];
 #pragma unroll
         for (int i = 0; i < N; ++i)
             if (row - j * N == i) {
@@ -1339,21 +1358,43 @@ __device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* H, 
 }
 
 // N >= 3: a team of N lanes per emitter, lane j owning column j of rho, its
-// history and its derivative; H and the matrix being differentiated are
-// shared through LDS.  Same arithmetic as k_ml_density, without the N^2
-// register arrays per lane that spill from N = 3 on.
+// history and its derivative (the same arithmetic as k_ml_density, without
+// the N^2 register arrays per lane that spill from N = 3 on).  Lanes are
+// grouped by column, lane = j * EB + team, so the lanes of one column hold
+// consecutive emitters and every rho / history access of a wave is a few
+// contiguous runs.  The matrix being differentiated is shared through LDS
+// with a per-emitter stride of N^2 + 1 elements, which spreads the emitters
+// of a wave over the LDS banks.
+constexpr int kMlThreads = 256;
+constexpr int kMlGamLds = 2048;  // Lindblad entries staged in LDS (N^2 x N^2 at most for N <= 6)
+
 template <typename T, int N>
-__global__ void __launch_bounds__(128) k_ml_team(QeDev q, const T* E0, const T* E1, const T* E2, int nx, int ny, int nz,
-                                                 const int* stepp) {
-    constexpr int S = N * N, EB = 128 / N;
-    __shared__ double2 Hs[EB][S];
-    __shared__ double2 Xs[EB][S];
-    __shared__ double2 red[128];
-    __shared__ double Pl[128][3];
-    const int team = threadIdx.x / N, j = threadIdx.x - team * N;
+__global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, const T* E1, const T* E2, int nx, int ny, int nz,
+                                                        const int* stepp) {
+    constexpr int S = N * N, EB = kMlThreads / N, XS = S + 1;
+    constexpr int XBYTES = EB * XS * 16, TAIL = kMlThreads * (3 * 8 + 16);
+    // Xs during the member loop; the polarisation and population reductions afterwards
+    __shared__ __align__(16) unsigned char lds[XBYTES > TAIL ? XBYTES : TAIL];
+    __shared__ double2 h0s[S];
+    __shared__ double2 mus[3][S];
+    __shared__ int grc_s[kMlGamLds];
+    __shared__ double gv_s[kMlGamLds];
+    __shared__ int gcp_s[N + 1];
+    double2* Xs = reinterpret_cast<double2*>(lds);
+    const int j = threadIdx.x / EB, team = threadIdx.x - j * EB;
     const int e = blockIdx.x * EB + team;
-    const bool act = team < EB && e < q.nemit;
+    const bool act = j < N && e < q.nemit;
     const int step = *stepp;
+    for (int k = threadIdx.x; k < 3 * S; k += blockDim.x) mus[k / S][k % S] = q.mu[k];
+    const bool gam_lds = q.ngam <= kMlGamLds;
+    if (gam_lds)
+        for (int g = threadIdx.x; g < q.ngam; g += blockDim.x) {
+            grc_s[g] = q.gam_rc[g];
+            gv_s[g] = q.gam_val[g];
+        }
+    for (int k = threadIdx.x; k <= N; k += blockDim.x) gcp_s[k] = q.gam_colptr[k];
+    const int* grc = gam_lds ? grc_s : q.gam_rc;
+    const double* gv = gam_lds ? gv_s : q.gam_val;
     double2 pop[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
     double P[3] = {0.0, 0.0, 0.0};
     double ef[3] = {0.0, 0.0, 0.0};
@@ -1380,23 +1421,15 @@ __global__ void __launch_bounds__(128) k_ml_team(QeDev q, const T* E0, const T* 
     const int s0 = (u + 3) & 3, s1 = (u + 2) & 3, s2 = (u + 1) & 3, s3 = u;
     const long long ld = static_cast<long long>(q.nmem) * q.nemit;
     const double dt = q.dt;
-    double2* Hm = Hs[act ? team : 0];
-    double2* Xm = Xs[act ? team : 0];
+    double2* Xm = Xs + (act ? team : 0) * XS;
     for (int m = 0; m < q.nmem; ++m) {
         const long long col = static_cast<long long>(m) * q.nemit + e;
+        for (int k = threadIdx.x; k < S; k += blockDim.x) h0s[k] = q.h0[m * S + k];
         double2 rc[N], fp[N];
         if (act) {
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 const int k = i + j * N;
-                double2 h = q.h0[m * S + k];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double2 mu = q.mu[c * S + k];
-                    h.x -= ef[c] * mu.x;
-                    h.y -= ef[c] * mu.y;
-                }
-                Hm[k] = h;
                 const double2 rk = q.rho[k * ld + col];
                 const double2 f0 = q.hist[s0 * S * ld + k * ld + col];
                 const double2 f1 = q.hist[s1 * S * ld + k * ld + col];
@@ -1416,7 +1449,7 @@ __global__ void __launch_bounds__(128) k_ml_team(QeDev q, const T* E0, const T* 
             }
         }
         __syncthreads();
-        if (act) den_deriv_col<N>(q, Hm, Xm, j, fp);
+        if (act) den_deriv_col<N>(q, h0s, mus, ef, Xm, j, grc, gv, gcp_s, fp);
         __syncthreads();
         if (act) {
 #pragma unroll
@@ -1429,7 +1462,7 @@ __global__ void __launch_bounds__(128) k_ml_team(QeDev q, const T* E0, const T* 
         }
         __syncthreads();
         if (act) {
-            den_deriv_col<N>(q, Hm, Xm, j, fp);
+            den_deriv_col<N>(q, h0s, mus, ef, Xm, j, grc, gv, gcp_s, fp);
 #pragma unroll
             for (int i = 0; i < N; ++i) q.hist[s3 * S * ld + (i + j * N) * ld + col] = fp[i];
 #pragma unroll
@@ -1446,15 +1479,17 @@ __global__ void __launch_bounds__(128) k_ml_team(QeDev q, const T* E0, const T* 
                 double sacc = 0.0;
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    const double2 mu = q.mu[c * S + i + j * N];
+                    const double2 mu = mus[c][i + j * N];
                     sacc += rc[i].x * mu.x + rc[i].y * mu.y;
                 }
                 P[c] += q.na * sacc;
             }
         }
-        __syncthreads();  // Hs / Xs are rewritten by the next member
+        __syncthreads();  // Xs and h0s are rewritten by the next member
     }
     // polarisation of the emitter: sum of its lanes' column parts
+    double (*Pl)[3] = reinterpret_cast<double (*)[3]>(lds);
+    double2* red = reinterpret_cast<double2*>(lds + kMlThreads * 3 * 8);
 #pragma unroll
     for (int c = 0; c < 3; ++c) Pl[threadIdx.x][c] = P[c];
     __syncthreads();
@@ -1464,7 +1499,7 @@ __global__ void __launch_bounds__(128) k_ml_team(QeDev q, const T* E0, const T* 
         for (int c = 0; c < 3; ++c) {
             if (!Es[c]) continue;
             double sum = 0.0;
-            for (int l = 0; l < N; ++l) sum += Pl[threadIdx.x + l][c];
+            for (int l = 0; l < N; ++l) sum += Pl[team + l * EB][c];
             q.Pbox[c][bx + PX * (by + PY * bz)] = sum;
         }
     }
@@ -1754,6 +1789,10 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
             // Lindblad entries grouped by the density column their row lies in
             // (stable, so each row keeps its accumulation order); gam_colptr[j]
             // .. gam_colptr[j+1] are the entries a team's lane j applies.
+            const int S = qi.nlevel * qi.nlevel;
+            for (int g = 0; g < qi.ngam; ++g)
+                if (qi.gam_row[g] < 0 || qi.gam_row[g] >= S || qi.gam_col[g] < 0 || qi.gam_col[g] >= S)
+                    throw std::runtime_error("Lindblad entry index outside nlevel^2");
             std::vector<int> order(qi.ngam);
             for (int g = 0; g < qi.ngam; ++g) order[g] = g;
             std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return qi.gam_row[u] / qi.nlevel < qi.gam_row[v] / qi.nlevel; });
@@ -1770,6 +1809,9 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
             q.gam_col = c->upload<int>(gc.data(), gc.size());
             q.gam_val = c->upload<double>(gv.data(), gv.size());
             q.gam_colptr = c->upload<int>(cp.data(), cp.size());
+            std::vector<int> grc(gr.size());
+            for (size_t g = 0; g < gr.size(); ++g) grc[g] = gr[g] | (gc[g] << 16);
+            q.gam_rc = c->upload<int>(grc.data(), grc.size());
         }
         q.na = qi.na;
         q.hbar = qi.hbar;
@@ -1797,7 +1839,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         q.pop_flat = c->upload<int>(qi.pop_flat, std::max(qi.npop, 1));
         q.pop_interval = qi.pop_interval > 0 ? qi.pop_interval : 1;
         // emitters per 128-thread block: one per thread for N = 2, one team of N lanes otherwise
-        const int per_block = q.N < 3 ? 128 : 128 / q.N;
+        const int per_block = q.N < 3 ? 128 : kMlThreads / q.N;
         qh.nblocks = (q.nemit + per_block - 1) / per_block;
         q.pop_partial = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(qh.nblocks)));
         q.pop_cap = p->nsteps / q.pop_interval + 1;
@@ -1935,7 +1977,7 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
 #define CHIML_ML_CASE(NN) \
     case NN: hipLaunchKernelGGL((k_ml_density<T, NN>), g, dim3(128), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
 #define CHIML_ML_TEAM(NN) \
-    case NN: hipLaunchKernelGGL((k_ml_team<T, NN>), g, dim3(128), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
+    case NN: hipLaunchKernelGGL((k_ml_team<T, NN>), g, dim3(kMlThreads), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
             CHIML_ML_CASE(2)
             CHIML_ML_TEAM(3)
             CHIML_ML_TEAM(4)
