@@ -1305,27 +1305,13 @@ __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const 
     }
 }
 
-// Column j of d rho/dt for the matrix X held in LDS (column-major):
-// (i/hbar)(H X - X H)[:, j] + the Lindblad rows that fall in column j, with
-// H = h0 - sum_c ef_c mu_c formed element by element from the block's LDS
-// copies of h0 and mu (the same for every emitter of the member).
-template <int N>
-__device__ __forceinline__ double2 ham(const double2* h0, const double2 (*mu)[N * N], const double (&ef)[3], int a) {
-    double2 h = h0[a];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        h.x -= ef[c] * mu[c][a].x;
-        h.y -= ef[c] * mu[c][a].y;
-    }
-    return h;
-}
-
+// Column j of d rho/dt for the matrices H and X held in LDS (column-major):
+// (i/hbar)(H X - X H)[:, j] + the Lindblad rows that fall in column j.
 // grc / gv / gcp: the Lindblad entries (packed row | col << 16, value) and
 // column pointers, staged in LDS by the caller when they fit.
 template <int N>
-__device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* h0, const double2 (*mu)[N * N], const double (&ef)[3],
-                                              const double2* X, int j, const int* grc, const double* gv, const int* gcp,
-                                              double2 (&out)[N]) {
+__device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* H, const double2* X, int j, const int* grc, const double* gv,
+                                              const int* gcp, double2 (&out)[N]) {
     const double ih = 1.0 / q.hbar;
     // outer-product order over k (not unrolled): one column of H and of X
     // in flight at a time keeps the register footprint small
@@ -1334,10 +1320,10 @@ __device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* h0,
     for (int i = 0; i < N; ++i) s1[i] = s2[i] = make_double2(0.0, 0.0);
 #pragma unroll 1
     for (int k = 0; k < N; ++k) {
-        const double2 xk = X[k + j * N], hk = ham<N>(h0, mu, ef, k + j * N);
+        const double2 xk = X[k + j * N], hk = H[k + j * N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            s1[i] = cadd(s1[i], cmul(ham<N>(h0, mu, ef, i + k * N), xk));
+            s1[i] = cadd(s1[i], cmul(H[i + k * N], xk));
             s2[i] = cadd(s2[i], cmul(X[i + k * N], hk));
         }
     }
@@ -1362,9 +1348,10 @@ __device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* h0,
 // the N^2 register arrays per lane that spill from N = 3 on).  Lanes are
 // grouped by column, lane = j * EB + team, so the lanes of one column hold
 // consecutive emitters and every rho / history access of a wave is a few
-// contiguous runs.  The matrix being differentiated is shared through LDS
-// with a per-emitter stride of N^2 + 1 elements, which spreads the emitters
-// of a wave over the LDS banks.
+// contiguous runs.  H = h0 - E.mu (formed once per member, each lane its
+// column) and the matrix being differentiated are shared through LDS with a
+// per-emitter stride of N^2 + 1 elements, which spreads the emitters of a
+// wave over the LDS banks.
 constexpr int kMlThreads = 256;
 constexpr int kMlGamLds = 2048;  // Lindblad entries staged in LDS (N^2 x N^2 at most for N <= 6)
 
@@ -1375,7 +1362,7 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
     constexpr int XBYTES = EB * XS * 16, TAIL = kMlThreads * (3 * 8 + 16);
     // Xs during the member loop; the polarisation and population reductions afterwards
     __shared__ __align__(16) unsigned char lds[XBYTES > TAIL ? XBYTES : TAIL];
-    __shared__ double2 h0s[S];
+    __shared__ double2 Hs[EB * XS];
     __shared__ double2 mus[3][S];
     __shared__ int grc_s[kMlGamLds];
     __shared__ double gv_s[kMlGamLds];
@@ -1422,14 +1409,22 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
     const long long ld = static_cast<long long>(q.nmem) * q.nemit;
     const double dt = q.dt;
     double2* Xm = Xs + (act ? team : 0) * XS;
+    double2* Hm = Hs + (act ? team : 0) * XS;
+    __syncthreads();  // mu and the Lindblad entries are staged
     for (int m = 0; m < q.nmem; ++m) {
         const long long col = static_cast<long long>(m) * q.nemit + e;
-        for (int k = threadIdx.x; k < S; k += blockDim.x) h0s[k] = q.h0[m * S + k];
         double2 rc[N], fp[N];
         if (act) {
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 const int k = i + j * N;
+                double2 h = q.h0[m * S + k];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    h.x -= ef[c] * mus[c][k].x;
+                    h.y -= ef[c] * mus[c][k].y;
+                }
+                Hm[k] = h;
                 const double2 rk = q.rho[k * ld + col];
                 const double2 f0 = q.hist[s0 * S * ld + k * ld + col];
                 const double2 f1 = q.hist[s1 * S * ld + k * ld + col];
@@ -1449,7 +1444,7 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
             }
         }
         __syncthreads();
-        if (act) den_deriv_col<N>(q, h0s, mus, ef, Xm, j, grc, gv, gcp_s, fp);
+        if (act) den_deriv_col<N>(q, Hm, Xm, j, grc, gv, gcp_s, fp);
         __syncthreads();
         if (act) {
 #pragma unroll
@@ -1462,7 +1457,7 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
         }
         __syncthreads();
         if (act) {
-            den_deriv_col<N>(q, h0s, mus, ef, Xm, j, grc, gv, gcp_s, fp);
+            den_deriv_col<N>(q, Hm, Xm, j, grc, gv, gcp_s, fp);
 #pragma unroll
             for (int i = 0; i < N; ++i) q.hist[s3 * S * ld + (i + j * N) * ld + col] = fp[i];
 #pragma unroll
@@ -1485,7 +1480,7 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
                 P[c] += q.na * sacc;
             }
         }
-        __syncthreads();  // Xs and h0s are rewritten by the next member
+        __syncthreads();  // Hs and Xs are rewritten by the next member
     }
     // polarisation of the emitter: sum of its lanes' column parts
     double (*Pl)[3] = reinterpret_cast<double (*)[3]>(lds);
@@ -1517,16 +1512,27 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
     }
 }
 
-__global__ void k_ml_pop_finish(QeDev q, int nblocks, const int* stepp) {
-    const int j = threadIdx.x;
+// One block per population detector: the density kernel's per-block partial
+// sums, each thread a fixed strided subset, then a tree in LDS (a fixed
+// order, so every run sums the same way).
+constexpr int kPopThreads = 256;
+
+__global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, int nblocks, const int* stepp) {
+    __shared__ double2 red[kPopThreads];
+    const int j = blockIdx.x;
     const int step = *stepp;
-    if (j >= q.npop) return;
     if (step % q.pop_interval != 0) return;
-    int slot = step / q.pop_interval;
+    const int slot = step / q.pop_interval;
     if (slot >= q.pop_cap) return;
     double2 s = make_double2(0.0, 0.0);
-    for (int b = 0; b < nblocks; ++b) s = cadd(s, q.pop_partial[j * nblocks + b]);
-    q.pop_out[static_cast<long long>(j) * q.pop_cap + slot] = make_double2(s.x / q.nemit, s.y / q.nemit);
+    for (int b = threadIdx.x; b < nblocks; b += kPopThreads) s = cadd(s, q.pop_partial[j * nblocks + b]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kPopThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) q.pop_out[static_cast<long long>(j) * q.pop_cap + slot] = make_double2(red[0].x / q.nemit, red[0].y / q.nemit);
 }
 
 __global__ void k_zero(double* p, long long n) {
@@ -1972,23 +1978,25 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
         hipLaunchKernelGGL(k_ml_addp<T>, dim3((n + 255) / 256, 3), dim3(256), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny,
                            c->nz);
         dim3 g(qh.nblocks);
-        Timed t(c, "ml_density");
-        switch (qh.N) {
+        {
+            Timed t(c, "ml_density");
+            switch (qh.N) {
 #define CHIML_ML_CASE(NN) \
     case NN: hipLaunchKernelGGL((k_ml_density<T, NN>), g, dim3(128), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
 #define CHIML_ML_TEAM(NN) \
     case NN: hipLaunchKernelGGL((k_ml_team<T, NN>), g, dim3(kMlThreads), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
-            CHIML_ML_CASE(2)
-            CHIML_ML_TEAM(3)
-            CHIML_ML_TEAM(4)
-            CHIML_ML_TEAM(5)
-            CHIML_ML_TEAM(6)
-            CHIML_ML_TEAM(7)
-            CHIML_ML_TEAM(8)
+                CHIML_ML_CASE(2)
+                CHIML_ML_TEAM(3)
+                CHIML_ML_TEAM(4)
+                CHIML_ML_TEAM(5)
+                CHIML_ML_TEAM(6)
+                CHIML_ML_TEAM(7)
+                CHIML_ML_TEAM(8)
 #undef CHIML_ML_CASE
 #undef CHIML_ML_TEAM
+            }
         }
-        if (q.npop > 0) hipLaunchKernelGGL(k_ml_pop_finish, dim3(1), dim3(64), 0, c->stream, q, qh.nblocks, a.step);
+        if (q.npop > 0) hipLaunchKernelGGL(k_ml_pop_finish, dim3(q.npop), dim3(kPopThreads), 0, c->stream, q, qh.nblocks, a.step);
     }
 }
 
