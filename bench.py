@@ -56,7 +56,7 @@ def parse():
     return p.parse_args()
 
 
-KERNELS = ("update_h", "update_h_pml", "ade", "update_e", "update_e_pml", "dft_flush", "ml_density")
+KERNELS = ("update_h", "update_h_pml", "update_e", "update_e_pml", "dft_flush", "ml_density")
 
 
 def run_gpu(n, steps, warmup, precision):

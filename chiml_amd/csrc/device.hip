@@ -6,7 +6,8 @@
 //   2. k_update_e  : soft E sources, Lorentz/Drude ADE, E += curl H with CPML psi
 //                    (addPul, updatePolE, updateE/updateD, updateEPML, D2E)
 //   3. emitters    : k_ml_addp then k_ml_density (parallelQEBase::addQE)
-//   4. k_advance   : t_step += 1
+//   4. k_advance   : t_step += 1 (inside captured graphs; plain launches are
+//                    given the step number by the host)
 //   5. k_sample    : point detectors and DFT staging (dtc->output, flux->fieldIn)
 //   6. k_dft_flush : every kDftBlock samples, F[s,f] += sum_k W[k,f] U[k,s]
 //
@@ -116,7 +117,12 @@ struct StepArgs {
     const int* src_box;
     const T* src_amp;
     const int* step;
+    int stepv;  // >= 0: the step number, passed by the host (plain launches); -1: read *step (captured graphs)
 };
+
+// The step number a launch works on: passed by the host for plain launches,
+// read from the device counter inside captured graphs.
+__device__ __forceinline__ int step_now(const int* p, int v) { return v >= 0 ? v : *p; }
 
 __device__ __forceinline__ long long lidx(int x, int y, int z, int nx, int ny) {
     return x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
@@ -167,7 +173,7 @@ __global__ void __launch_bounds__(256) k_update_h(StepArgs<T> a) {
     const bool inpml = (x < a.Tp[0]) || (x >= nx - a.Tp[0]) || (y < a.Tp[1]) || (y >= ny - a.Tp[1]) ||
                        (DIM == 3 && ((z < a.Tp[2]) || (z >= nz - a.Tp[2])));
     int step = 0;
-    if (SRC) step = *a.step;
+    if (SRC) step = step_now(a.step, a.stepv);
     T* Ex = a.F[0];
     T* Ey = a.F[1];
     T* Ez = a.F[2];
@@ -267,7 +273,7 @@ __global__ void __launch_bounds__(256) k_update_e(StepArgs<T> a) {
     const bool inpml = (x < a.Tp[0]) || (x >= nx - a.Tp[0]) || (y < a.Tp[1]) || (y >= ny - a.Tp[1]) ||
                        (DIM == 3 && ((z < a.Tp[2]) || (z >= nz - a.Tp[2])));
     int step = 0;
-    if (SRC) step = *a.step;
+    if (SRC) step = step_now(a.step, a.stepv);
     const T* Hx = a.F[3];
     const T* Hy = a.F[4];
     const T* Hz = a.F[5];
@@ -470,7 +476,7 @@ __device__ __forceinline__ void h_group(const StepArgs<T>& a, int x0, int y, int
     const bool last = x0 + W >= nx;
     const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;  // cells with x < nx - 1
     const bool src = SRC && group_has_src<T, W>(a, x0, y, z);
-    const int step = src ? *a.step : 0;
+    const int step = src ? step_now(a.step, a.stepv) : 0;
     T d1[W], d2[W], t1[W], t2[W];
     if (y < ny - 1 && z < nz - 1) {  // Hx += ch [dEy/dz - dEz/dy]
         const Pk<T, W> eyz = ldv<T, W>(Ey + i + sz), ezy = ldv<T, W>(Ez + i + sy);
@@ -586,7 +592,7 @@ __device__ __forceinline__ void e_finish_group(const StepArgs<T>& a, const MatLd
         jsum[w] = T(0);
         anyp |= (mt[c][id.v[w]].poleset >= 0) && (valid & (1u << w));
     }
-    if (PRE && anyp) {  // the pole currents were advanced by k_ade3 this step
+    if (PRE && anyp) {  // the pole currents were advanced by ade_block this step
         const int* bx = a.box[c];
         const long long bi = (x0 - bx[0]) + static_cast<long long>(bx[1] - bx[0]) * ((y - bx[2]) + static_cast<long long>(bx[3] - bx[2]) * (z - bx[4]));
         const Pk<T, W> js = ldv<T, W>(a.jsum[c] + bi);
@@ -648,7 +654,7 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
     const bool last = x0 + W >= nx;
     const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;
     const bool src = SRC && group_has_src<T, W>(a, x0, y, z);
-    const int step = src ? *a.step : 0;
+    const int step = src ? step_now(a.step, a.stepv) : 0;
     T d1[W], d2[W], t1[W], t2[W];
     {  // Ex += ce [dHz/dy - dHy/dz], x < nx - 1
         const Pk<T, W> hzy = y > 0 ? ldv<T, W>(Hz + i - sy) : zpk<T, W>();
@@ -712,6 +718,65 @@ __device__ __forceinline__ void e_group(const StepArgs<T>& a, const MatLds<T> (*
     }
 }
 
+// Pole update of the 3D split path, before the E kernels: one thread per
+// W-cell group of component c's pole box (blk: block within the component),
+// run by the first blocks of the H interior launch, so the few dispersive
+// cells are spread over the whole chip instead of loading the XCDs whose
+// part of the domain holds the object.  It advances P and J from
+// E before its update (plus the soft source, as updatePolE sees it) and
+// leaves the sum of the new currents per cell in jsum for the E kernels.
+template <typename T, int W, bool SRC>
+__device__ __forceinline__ void ade_block(const StepArgs<T>& a, int c, long long blk) {
+    const int* bx = a.box[c];
+    if (!a.jsum[c]) return;
+    const int gw = (bx[1] - bx[0]) / W, ny = bx[3] - bx[2], nz = bx[5] - bx[4];
+    const long long t = blk * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(gw) * ny * nz) return;
+    const long long r = t / gw;
+    const int x0 = bx[0] + static_cast<int>(t - r * gw) * W;
+    const int y = bx[2] + static_cast<int>(r % ny), z = bx[4] + static_cast<int>(r / ny);
+    const long long bi = t * W;  // (x0 - bx0) + width * ((y - by0) + height * (z - bz0))
+    const long long i = lidx(x0, y, z, a.nx, a.ny);
+    const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[c], i);
+    // the cells the E kernels update: Ex x < nx - 1, Ey y < ny - 1, Ez z < nz - 1
+    unsigned valid = (1u << W) - 1;
+    if (c == 0 && x0 + W >= a.nx) valid >>= 1;
+    if ((c == 1 && y >= a.ny - 1) || (c == 2 && z >= a.nz - 1)) valid = 0;
+    int ps[W], np[W], npmax = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        ps[w] = (valid & (1u << w)) ? a.poleset[c][id.v[w]] : -1;
+        np[w] = ps[w] >= 0 ? a.pole_count[ps[w]] : 0;
+        npmax = max(npmax, np[w]);
+    }
+    Pk<T, W> js = zpk<T, W>();
+    if (npmax > 0) {
+        const bool src = SRC && group_has_src<T, W>(a, x0, y, z);
+        const int step = src ? step_now(a.step, a.stepv) : 0;
+        const Pk<T, W> e = ldv<T, W>(a.F[c] + i);
+        T ea[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) ea[w] = e.v[w] + (src ? src_value(a, c, x0 + w, y, z, step) : T(0));
+        for (int p = 0; p < npmax; ++p) {
+            T* Pp = a.P[c] + p * a.boxn[c] + bi;
+            T* Jp = a.J[c] + p * a.boxn[c] + bi;
+            Pk<T, W> P = ldv_nt<T, W>(Pp), J = ldv_nt<T, W>(Jp);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if (p >= np[w]) continue;
+                const T* co = a.pole_coef + (static_cast<long long>(ps[w]) * CHIML_MAX_POLES + p) * 3;
+                const T Jn = co[0] * P.v[w] + co[1] * J.v[w] + co[2] * ea[w];
+                P.v[w] = P.v[w] + Jn;
+                J.v[w] = Jn;
+                js.v[w] += Jn;
+            }
+            stv_nt<T, W>(Pp, P);
+            stv_nt<T, W>(Jp, J);
+        }
+    }
+    stv<T, W>(a.jsum[c] + bi, js);
+}
+
 // Interior box, one group per thread, walked band by band: x fastest, then
 // the by rows of a y band, then z, then the band.  With the XCD renumbering
 // each XCD marches one band through z, so the rows of the previous and next
@@ -740,10 +805,23 @@ __device__ __forceinline__ bool int_group(const Box3& b, long long t, int& x0, i
     return y < b.y1;  // the last band may be short
 }
 
-// ROWS: the walk covers whole x rows, so the x slabs' CPML is applied inline
+// ROWS: the walk covers whole x rows, so the x slabs' CPML is applied inline.
+// The first ade_blocks(ade_bpc) blocks run the pole update (ade_bpc blocks
+// per E component, padded to a multiple of 8 so that the interior blocks
+// keep their XCD numbering): it reads only E, which this kernel does not
+// write, and its latency-bound blocks overlap the bandwidth-bound H sweep
+// (15 us less than a launch of its own on the 256^3 step).
+__host__ __device__ constexpr int ade_blocks(int ade_bpc) { return (3 * ade_bpc + 7) / 8 * 8; }
+
 template <typename T, int W, bool SRC, bool ROWS>
-__global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b) {
-    const long long blk = xcd_block(blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b, int nib, int ade_bpc) {
+    const int nade = ade_blocks(ade_bpc);
+    if (static_cast<int>(blockIdx.x) < nade) {
+        const int c = blockIdx.x / max(ade_bpc, 1);
+        if (c < 3) ade_block<T, W, SRC>(a, c, blockIdx.x - c * ade_bpc);
+        return;
+    }
+    const long long blk = xcd_block(blockIdx.x - nade, nib);
     int x0, y, z, col;
     const bool in = int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z, col);
     if (!in) return;
@@ -764,64 +842,6 @@ __global__ void __launch_bounds__(256) k_e3_int(StepArgs<T> a, Box3 b, int nm0, 
     e_group<T, W, ROWS, SRC, true>(a, mt, x0, y, z, lane > 0 && col > 0);
 }
 
-// Pole update of the 3D split path, before the E kernels: one thread per
-// W-cell group of a component's pole box (blockIdx.y = component), so the
-// few dispersive cells are spread over the whole chip instead of loading the
-// XCDs whose part of the domain holds the object.  It advances P and J from
-// E before its update (plus the soft source, as updatePolE sees it) and
-// leaves the sum of the new currents per cell in jsum for the E kernels.
-template <typename T, int W, bool SRC>
-__global__ void __launch_bounds__(256) k_ade3(StepArgs<T> a) {
-    const int c = blockIdx.y;
-    const int* bx = a.box[c];
-    if (!a.jsum[c]) return;
-    const int gw = (bx[1] - bx[0]) / W, ny = bx[3] - bx[2], nz = bx[5] - bx[4];
-    const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-    if (t >= static_cast<long long>(gw) * ny * nz) return;
-    const long long r = t / gw;
-    const int x0 = bx[0] + static_cast<int>(t - r * gw) * W;
-    const int y = bx[2] + static_cast<int>(r % ny), z = bx[4] + static_cast<int>(r / ny);
-    const long long bi = t * W;  // (x0 - bx0) + width * ((y - by0) + height * (z - bz0))
-    const long long i = lidx(x0, y, z, a.nx, a.ny);
-    const Pk<uint8_t, W> id = load_ids<W>(a.mat_e[c], i);
-    // the cells the E kernels update: Ex x < nx - 1, Ey y < ny - 1, Ez z < nz - 1
-    unsigned valid = (1u << W) - 1;
-    if (c == 0 && x0 + W >= a.nx) valid >>= 1;
-    if ((c == 1 && y >= a.ny - 1) || (c == 2 && z >= a.nz - 1)) valid = 0;
-    int ps[W], np[W], npmax = 0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-        ps[w] = (valid & (1u << w)) ? a.poleset[c][id.v[w]] : -1;
-        np[w] = ps[w] >= 0 ? a.pole_count[ps[w]] : 0;
-        npmax = max(npmax, np[w]);
-    }
-    Pk<T, W> js = zpk<T, W>();
-    if (npmax > 0) {
-        const bool src = SRC && group_has_src<T, W>(a, x0, y, z);
-        const int step = src ? *a.step : 0;
-        const Pk<T, W> e = ldv<T, W>(a.F[c] + i);
-        T ea[W];
-#pragma unroll
-        for (int w = 0; w < W; ++w) ea[w] = e.v[w] + (src ? src_value(a, c, x0 + w, y, z, step) : T(0));
-        for (int p = 0; p < npmax; ++p) {
-            T* Pp = a.P[c] + p * a.boxn[c] + bi;
-            T* Jp = a.J[c] + p * a.boxn[c] + bi;
-            Pk<T, W> P = ldv_nt<T, W>(Pp), J = ldv_nt<T, W>(Jp);
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                if (p >= np[w]) continue;
-                const T* co = a.pole_coef + (static_cast<long long>(ps[w]) * CHIML_MAX_POLES + p) * 3;
-                const T Jn = co[0] * P.v[w] + co[1] * J.v[w] + co[2] * ea[w];
-                P.v[w] = P.v[w] + Jn;
-                J.v[w] = Jn;
-                js.v[w] += Jn;
-            }
-            stv_nt<T, W>(Pp, P);
-            stv_nt<T, W>(Jp, J);
-        }
-    }
-    stv<T, W>(a.jsum[c] + bi, js);
-}
 
 // CPML slab groups: the union of the z slabs, the y slabs inside the z range
 // and the x slabs inside the y and z ranges, enumerated x fastest, in units
@@ -831,15 +851,19 @@ struct SlabMap {
     long long nA, nB, nC;  // group counts of the three regions
 };
 
-__device__ __forceinline__ void slab_group(const SlabMap& m, long long t, int& x0, int& y, int& z) {
+// col / rowlen: the group's place in a run of consecutive groups of one x
+// row (rowlen 0 where rows are not walked whole: the x slabs)
+__device__ __forceinline__ void slab_group(const SlabMap& m, long long t, int& x0, int& y, int& z, int& col, int& rowlen) {
     const int gx = m.nx / m.W;
+    rowlen = gx;
     if (t < m.nA) {  // z slabs: z in [0,Tz) U [nz-Tz,nz), all x, y
         const long long plane = static_cast<long long>(gx) * m.ny;
         const int k = static_cast<int>(t / plane);
         const long long r = t - k * plane;
         z = k < m.Tz ? k : m.nz - 2 * m.Tz + k;
         y = static_cast<int>(r / gx);
-        x0 = static_cast<int>(r - static_cast<long long>(y) * gx) * m.W;
+        col = static_cast<int>(r - static_cast<long long>(y) * gx);
+        x0 = col * m.W;
         return;
     }
     t -= m.nA;
@@ -849,11 +873,14 @@ __device__ __forceinline__ void slab_group(const SlabMap& m, long long t, int& x
         const long long r = t - k * plane;
         z = m.Tz + k;
         const int yy = static_cast<int>(r / gx);
-        x0 = static_cast<int>(r - static_cast<long long>(yy) * gx) * m.W;
+        col = static_cast<int>(r - static_cast<long long>(yy) * gx);
+        x0 = col * m.W;
         y = yy < m.Ty ? yy : m.ny - 2 * m.Ty + yy;
         return;
     }
     t -= m.nB;
+    col = 0;
+    rowlen = 0;
     {  // x slabs inside the y and z interiors
         const int gT = 2 * (m.Tx / m.W);
         const int nyi = m.ny - 2 * m.Ty;
@@ -872,9 +899,11 @@ template <typename T, int W, bool SRC>
 __global__ void __launch_bounds__(256) k_h3_slab(StepArgs<T> a, SlabMap m) {
     const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= m.nA + m.nB + m.nC) return;
-    int x0, y, z;
-    slab_group(m, t, x0, y, z);
-    h_group<T, W, true, SRC>(a, x0, y, z);
+    int x0, y, z, col, rowlen;
+    slab_group(m, t, x0, y, z, col, rowlen);
+    // the z- and y-slab regions start on row boundaries (nA is whole rows), so
+    // lane + 1 holds the next group of the row unless this one ends it
+    h_group<T, W, true, SRC>(a, x0, y, z, (threadIdx.x & 63) < 63 && col + 1 < rowlen);
 }
 
 template <typename T, int W, bool SRC>
@@ -884,9 +913,9 @@ __global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int n
     load_mat(a, mt, nm);
     const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= m.nA + m.nB + m.nC) return;
-    int x0, y, z;
-    slab_group(m, t, x0, y, z);
-    e_group<T, W, true, SRC, true>(a, mt, x0, y, z);
+    int x0, y, z, col, rowlen;
+    slab_group(m, t, x0, y, z, col, rowlen);
+    e_group<T, W, true, SRC, true>(a, mt, x0, y, z);  // (the lane shuffle measured slower here)
 }
 
 // 2D TMz on W-cell groups over the whole domain (CPML handled per group; the
@@ -903,7 +932,7 @@ __global__ void __launch_bounds__(256) k_h2v(StepArgs<T> a) {
     const bool last = x0 + W >= nx;
     const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;
     const bool src = SRC && group_has_src<T, W>(a, x0, y, 0);
-    const int step = src ? *a.step : 0;
+    const int step = src ? step_now(a.step, a.stepv) : 0;
     T d[W], term[W];
     if (y < ny - 1) {  // Hx += ch [-dEz/dy]
         const Pk<T, W> ezy = ldv<T, W>(Ez + i + nx);
@@ -954,7 +983,7 @@ __global__ void __launch_bounds__(256) k_e2v(StepArgs<T> a, int nm2) {
     const T hym = x0 > 0 ? Hy[i - 1] : T(0);
     const unsigned all = (1u << W) - 1;
     const bool src = SRC && group_has_src<T, W>(a, x0, y, 0);
-    const int step = src ? *a.step : 0;
+    const int step = src ? step_now(a.step, a.stepv) : 0;
     T d1[W], d2[W], t1[W], t2[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -968,12 +997,14 @@ __global__ void __launch_bounds__(256) k_e2v(StepArgs<T> a, int nm2) {
 
 // ----------------------------------------------------- step counter ---------
 __global__ void k_advance(int* step) { *step += 1; }
+__global__ void k_set_step(int* step, int v) { *step = v; }
 
 // ----------------------------------------------- detectors and DFT ----------
 template <typename T>
 struct SampleArgs {
     const T* F[6];
     const int* step;
+    int stepv;  // as in StepArgs
     int ndet;
     const int* det_interval;
     const int* det_npts;
@@ -996,7 +1027,7 @@ struct SampleArgs {
 template <typename T>
 __global__ void k_sample(SampleArgs<T> a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int step = *a.step;
+    const int step = step_now(a.step, a.stepv);
     if (t < a.totpts) {
         // locate the detector of point t
         int d = 0;
@@ -1189,9 +1220,9 @@ __device__ __forceinline__ void den_deriv(const QeDev& q, const double2* H, cons
 // instead of the reference's three copies per step (density.hpp:65-70).
 template <typename T, int N>
 __global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const T* E1, const T* E2, int nx, int ny, int nz,
-                                                    const int* stepp) {
+                                                    const int* stepp, int stepv) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    const int step = *stepp;  // number of earlier density updates
+    const int step = step_now(stepp, stepv);  // number of earlier density updates
     constexpr int S = N * N;
     __shared__ double2 red[128];
     double2 pop[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
@@ -1357,7 +1388,7 @@ constexpr int kMlGamLds = 2048;  // Lindblad entries staged in LDS (N^2 x N^2 at
 
 template <typename T, int N>
 __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, const T* E1, const T* E2, int nx, int ny, int nz,
-                                                        const int* stepp) {
+                                                        const int* stepp, int stepv) {
     constexpr int S = N * N, EB = kMlThreads / N, XS = S + 1;
     constexpr int XBYTES = EB * XS * 16, TAIL = kMlThreads * (3 * 8 + 16);
     // Xs during the member loop; the polarisation and population reductions afterwards
@@ -1371,7 +1402,7 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
     const int j = threadIdx.x / EB, team = threadIdx.x - j * EB;
     const int e = blockIdx.x * EB + team;
     const bool act = j < N && e < q.nemit;
-    const int step = *stepp;
+    const int step = step_now(stepp, stepv);
     for (int k = threadIdx.x; k < 3 * S; k += blockDim.x) mus[k / S][k % S] = q.mu[k];
     const bool gam_lds = q.ngam <= kMlGamLds;
     if (gam_lds)
@@ -1515,12 +1546,12 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
 // One block per population detector: the density kernel's per-block partial
 // sums, each thread a fixed strided subset, then a tree in LDS (a fixed
 // order, so every run sums the same way).
-constexpr int kPopThreads = 256;
+constexpr int kPopThreads = 1024;
 
-__global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, int nblocks, const int* stepp) {
+__global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, int nblocks, const int* stepp, int stepv) {
     __shared__ double2 red[kPopThreads];
     const int j = blockIdx.x;
-    const int step = *stepp;
+    const int step = step_now(stepp, stepv);
     if (step % q.pop_interval != 0) return;
     const int slot = step / q.pop_interval;
     if (slot >= q.pop_cap) return;
@@ -1568,6 +1599,7 @@ struct chiml_dev {
     int nmat[3] = {1, 1, 1};
     int enq_step = 0;    // step counter value after the step being enqueued
     int graph_steps = 0;  // > 0: replay captured graphs of this many steps
+    bool capturing = false;  // do_steps is being captured into a graph
     bool device_flush = false;  // launch the DFT flush every step; it decides on the device
     hipGraphExec_t gexec = nullptr;
     bool split = false;  // 3D interior + CPML-slab kernels
@@ -1727,9 +1759,11 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         a.src_amp = upload_cast<T>(c, p->src_amp, static_cast<size_t>(p->nsteps) * p->nsrc);
     }
     a.step = c->step_ctr.as<int>();
+    a.stepv = -1;
     // detectors and DFT
     for (int k = 0; k < 6; ++k) s.F[k] = a.F[k];
     s.step = a.step;
+    s.stepv = -1;
     c->ndet = p->ndet;
     std::vector<int> off(p->ndet + 1, 0);
     std::vector<long long> ooff(p->ndet + 1, 0);
@@ -1883,15 +1917,17 @@ struct Timed {
     }
 };
 
+// step: the step number once the step being enqueued has advanced it (-1:
+// read the device counter, inside captured graphs)
 template <typename T>
-void launch_flush(chiml_dev* c, int force) {
+void launch_flush(chiml_dev* c, int force, int step) {
     if (c->ndft == 0) return;
     // enq_step = the device step counter once the step being enqueued has advanced it
     const int st = c->enq_step;
     if (!force && !c->device_flush && (st % c->dft_interval != 0 || ((st / c->dft_interval) % kDftBlock) != kDftBlock - 1)) return;
     Timed t(c, "dft_flush");
     hipLaunchKernelGGL(k_dft_flush<T>, dim3((c->ndft + 255) / 256), dim3(256), 0, c->stream, static_cast<const T*>(c->dft_stage),
-                       c->dft_phasor, c->dft_F, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, force, -1);
+                       c->dft_phasor, c->dft_F, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, force, step);
 }
 
 
@@ -1904,22 +1940,19 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
         const long long ni = static_cast<long long>((b.x1 - b.x0) / W) * by * (b.z1 - b.z0) * ((nyb + by - 1) / by);
         const long long ns = c->slab.nA + c->slab.nB + c->slab.nC;
         const dim3 gi(static_cast<unsigned>((ni + 255) / 256)), gs(static_cast<unsigned>((ns + 255) / 256)), blk(256);
-        {
+        // the pole update rides on the H interior launch as its first blocks
+        const int nib = static_cast<int>(gi.x), ade_bpc = static_cast<int>((c->ade_groups + 255) / 256);
+        const dim3 gh(static_cast<unsigned>(nib + ade_blocks(ade_bpc)));
+        if (gh.x > 0) {
             Timed t(c, "update_h");
             constexpr bool R = W == 4;  // whole-row interior walk in FP32
-            if (c->has_src) hipLaunchKernelGGL((k_h3_int<T, W, true, R>), gi, blk, 0, c->stream, a, b);
-            else hipLaunchKernelGGL((k_h3_int<T, W, false, R>), gi, blk, 0, c->stream, a, b);
+            if (c->has_src) hipLaunchKernelGGL((k_h3_int<T, W, true, R>), gh, blk, 0, c->stream, a, b, nib, ade_bpc);
+            else hipLaunchKernelGGL((k_h3_int<T, W, false, R>), gh, blk, 0, c->stream, a, b, nib, ade_bpc);
         }
         if (ns > 0) {
             Timed t(c, "update_h_pml");
             if (c->has_src) hipLaunchKernelGGL((k_h3_slab<T, W, true>), gs, blk, 0, c->stream, a, c->slab);
             else hipLaunchKernelGGL((k_h3_slab<T, W, false>), gs, blk, 0, c->stream, a, c->slab);
-        }
-        if (c->ade_groups > 0) {
-            Timed t(c, "ade");
-            const dim3 ga(static_cast<unsigned>((c->ade_groups + 255) / 256), 3);
-            if (c->has_src) hipLaunchKernelGGL((k_ade3<T, W, true>), ga, blk, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_ade3<T, W, false>), ga, blk, 0, c->stream, a);
         }
         {
             Timed t(c, "update_e");
@@ -1982,9 +2015,9 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
             Timed t(c, "ml_density");
             switch (qh.N) {
 #define CHIML_ML_CASE(NN) \
-    case NN: hipLaunchKernelGGL((k_ml_density<T, NN>), g, dim3(128), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
+    case NN: hipLaunchKernelGGL((k_ml_density<T, NN>), g, dim3(128), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step, a.stepv); break;
 #define CHIML_ML_TEAM(NN) \
-    case NN: hipLaunchKernelGGL((k_ml_team<T, NN>), g, dim3(kMlThreads), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step); break;
+    case NN: hipLaunchKernelGGL((k_ml_team<T, NN>), g, dim3(kMlThreads), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step, a.stepv); break;
                 CHIML_ML_CASE(2)
                 CHIML_ML_TEAM(3)
                 CHIML_ML_TEAM(4)
@@ -1996,22 +2029,33 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
 #undef CHIML_ML_TEAM
             }
         }
-        if (q.npop > 0) hipLaunchKernelGGL(k_ml_pop_finish, dim3(q.npop), dim3(kPopThreads), 0, c->stream, q, qh.nblocks, a.step);
+        if (q.npop > 0) hipLaunchKernelGGL(k_ml_pop_finish, dim3(q.npop), dim3(kPopThreads), 0, c->stream, q, qh.nblocks, a.step, a.stepv);
     }
 }
 
+// Plain launches carry the step number from the host, so no counter kernel
+// runs between steps and the device counter is set once at the end.  Steps
+// being captured into a graph read and advance the device counter instead,
+// so that one graph serves every replay.
 template <typename T>
 void do_steps(chiml_dev* c, StepArgs<T>& a, SampleArgs<T>& s, int n) {
+    const bool host = !c->capturing;
     for (int k = 0; k < n; ++k) {
+        const int st = c->steps_done + k;
+        a.stepv = host ? st : -1;
+        s.stepv = host ? st + 1 : -1;
         if (c->dim == 3) launch_fields<T, 3>(c, a);
         else launch_fields<T, 2>(c, a);
         if (!c->qes.empty()) launch_ml<T>(c, a);
-        hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->step_ctr.as<int>());
-        c->enq_step = c->steps_done + k + 1;
+        if (!host) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->step_ctr.as<int>());
+        c->enq_step = st + 1;
         launch_sample<T>(c, s);
-        launch_flush<T>(c, 0);
+        launch_flush<T>(c, 0, host ? st + 1 : -1);
         HIP_TRY(hipGetLastError());
     }
+    if (host && n > 0) hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, c->stream, c->step_ctr.as<int>(), c->steps_done + n);
+    a.stepv = -1;  // launches outside the step loop (the t = 0 sample) read the counter
+    s.stepv = -1;
     c->steps_done += n;
 }
 
@@ -2129,14 +2173,17 @@ void ensure_graph(chiml_dev* c) {
     hipGraph_t g = nullptr;
     const int saved = c->steps_done;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
     try {
         enqueue_steps(c, c->graph_steps);
     } catch (...) {
+        c->capturing = false;
         (void)hipStreamEndCapture(c->stream, &g);
         if (g) (void)hipGraphDestroy(g);
         c->steps_done = saved;
         throw;
     }
+    c->capturing = false;
     HIP_TRY(hipStreamEndCapture(c->stream, &g));
     c->steps_done = saved;  // capture enqueued nothing
     hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);

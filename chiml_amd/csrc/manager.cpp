@@ -1011,8 +1011,11 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         // samples and the block's phasors, read and write every accumulator
         if (ndft > 0)
             kbytes["dft_flush"] = static_cast<double>(ndft) * (32.0 * s + 2.0 * 16.0 * nfreq) + 32.0 * 16.0 * nfreq;
+        // the pole update runs as extra blocks of the H interior launch
+        kbytes["update_h"] += kbytes["ade"];
+        kbytes.erase("ade");
         bytesH = kbytes["update_h"] + kbytes["update_h_pml"];
-        bytesE = kbytes["update_e"] + kbytes["update_e_pml"] + kbytes["ade"];
+        bytesE = kbytes["update_e"] + kbytes["update_e_pml"];
         estimate = static_cast<size_t>(cells * (6.0 * s + 3.0) + psiWords * s + bytesML / 7.0 * 5.0 +
                                        static_cast<double>(ndft) * (nfreq * 16.0 + 32 * s));
     }
