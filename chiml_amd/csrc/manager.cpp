@@ -1011,11 +1011,22 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         // samples and the block's phasors, read and write every accumulator
         if (ndft > 0)
             kbytes["dft_flush"] = static_cast<double>(ndft) * (32.0 * s + 2.0 * 16.0 * nfreq) + 32.0 * 16.0 * nfreq;
-        // the pole update runs as extra blocks of the H interior launch
+        // the pole update runs as extra blocks of the H interior launch, and in
+        // FP32 each field's slab and interior regions are one launch
         kbytes["update_h"] += kbytes["ade"];
         kbytes.erase("ade");
-        bytesH = kbytes["update_h"] + kbytes["update_h_pml"];
-        bytesE = kbytes["update_e"] + kbytes["update_e_pml"];
+        if (split && rows) {
+            kbytes["update_h"] += kbytes["update_h_pml"];
+            kbytes["update_e"] += kbytes["update_e_pml"];
+            kbytes.erase("update_h_pml");
+            kbytes.erase("update_e_pml");
+        }
+        auto kb = [&](const char* k) {
+            auto it = kbytes.find(k);
+            return it == kbytes.end() ? 0.0 : it->second;
+        };
+        bytesH = kb("update_h") + kb("update_h_pml");
+        bytesE = kb("update_e") + kb("update_e_pml");
         estimate = static_cast<size_t>(cells * (6.0 * s + 3.0) + psiWords * s + bytesML / 7.0 * 5.0 +
                                        static_cast<double>(ndft) * (nfreq * 16.0 + 32 * s));
     }
