@@ -50,6 +50,9 @@ def parse():
     p.add_argument("--no-sweep-baseline", action="store_true", help="skip the one-at-a-time replay")
     p.add_argument("--no-configs", action="store_true",
                    help="skip the other workloads (2D TMz, 2D and 3D Maxwell-Liouville)")
+    p.add_argument("--only", choices=["tmz", "ml2d", "ml3d"],
+                   help="run one of the other workloads alone and print its line (for profiling)")
+    p.add_argument("--only-steps", type=int, default=0, help="step count for --only (0: the workload's own)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a plain 256 MiB device copy three times (the known-size kernel that scales "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -185,17 +188,28 @@ def run_workload(deck, graph_steps, profile_kernel=None):
     return out
 
 
+def other_workload(name, steps=0):
+    """(deck, graph steps, profiled kernel) of one of the other workloads at its
+    full size; steps > 0 overrides its step count."""
+    from chiml_amd import decks
+
+    if name == "tmz":
+        return decks.config1_2d(n=256, steps=steps or 2000, profile="gaussian"), 64, None
+    if name == "ml2d":
+        return decks.config3_ml2d(n=1024, steps=steps or 20000), 64, None
+    return decks.config4_ml3d(n=192, steps=steps or 5000), 0, "ml_density"
+
+
 def run_configs():
     """The other workloads of the path, each at its full size: 2D TMz 256^2
     (2000 steps, FP64), 2D Maxwell-Liouville 1024^2 with 32,768 2-level
     emitters (20,000 steps, FP32 fields / FP64 rho) and 3D Maxwell-Liouville
-    192^3 with 368,640 6-level emitters over a Drude film (5000 steps)."""
-    from chiml_amd import decks
-
+    192^3 with 368,640 (192 x 192 x 10) 6-level emitters over a 20-plane
+    Drude film (5000 steps)."""
     res = {}
-    res["tmz_256_cylinder_fp64"] = run_workload(decks.config1_2d(n=256, steps=2000, profile="gaussian"), 64)
-    res["ml2d_1024_2level_fp32"] = run_workload(decks.config3_ml2d(n=1024, steps=20000), 64)
-    res["ml3d_192_6level_fp32"] = run_workload(decks.config4_ml3d(n=192, steps=5000), 0, "ml_density")
+    res["tmz_256_cylinder_fp64"] = run_workload(*other_workload("tmz"))
+    res["ml2d_1024_2level_fp32"] = run_workload(*other_workload("ml2d"))
+    res["ml3d_192_6level_fp32"] = run_workload(*other_workload("ml3d"))
     return res
 
 
@@ -247,6 +261,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU slabs are a design note only)"}))
         sys.exit(2)
+    if args.only:
+        print(json.dumps({"workload": args.only, **run_workload(*other_workload(args.only, args.only_steps))}), flush=True)
+        return
     if args.copy_calibration:
         import torch
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")

@@ -258,16 +258,22 @@ def n_level_emitter(nlev, size3, loc, seed, mol_den=1e25):
 
 
 def config4_ml3d(n=192, steps=5000, pml=16, d=0.02, layer_cells=10, film_cells=20, nlev=6, seed=4, precision="fp32"):
-    """3D Maxwell-Liouville: a layer of N-level emitters above a Drude film."""
+    """3D Maxwell-Liouville: a layer of N-level emitters (layer_cells planes,
+    n * n * layer_cells emitters) one plane above a Drude film of film_cells
+    planes.  Each block is centred between grid planes with a z size of whole
+    cells, so both of its faces fall between planes and it holds exactly that
+    many planes."""
     deck, size, dt = _cell((n, n, n), d, 0.5, steps, pml, precision)
     full = (n - 1) * d
-    z_film = _pos(n // 2 - film_cells // 2, n, d)
+    film_lo = n // 2 - film_cells // 2
+    z_film = _pos(film_lo, n, d) + (film_cells - 1) * d / 2.0
     deck["ObjectList"].append({
         "shape": "block", "material": "custom", "eps": 1.0, "pols": gold_like_pols()[:1],
-        "size": [full, full, (film_cells - 1) * d], "loc": [0.0, 0.0, z_film], "Basis_Set": [],
+        "size": [full, full, film_cells * d], "loc": [0.0, 0.0, z_film], "Basis_Set": [],
     })
-    z_layer = z_film + (film_cells // 2 + 1 + layer_cells // 2) * d
-    deck["ObjectList"].append(n_level_emitter(nlev, [full, full, (layer_cells - 1) * d], (0.0, 0.0, z_layer), seed))
+    layer_lo = film_lo + film_cells + 1
+    z_layer = _pos(layer_lo, n, d) + (layer_cells - 1) * d / 2.0
+    deck["ObjectList"].append(n_level_emitter(nlev, [full, full, layer_cells * d], (0.0, 0.0, z_layer), seed))
     fcen = A_UNIT / 600e-9
     inner = (n - 2 * pml - 2) * d
     deck["SourceList"].append({
