@@ -69,9 +69,17 @@ def test_graph_steps_equal_plain_launches(graph):
         _same(_alone(deck), _alone(deck, graph), f"deck {i}")
 
 
+def _served_mix():
+    # a long request first: it is still stepping while the next ones are built,
+    # so the server holds at least two at once however fast the short ones run
+    long = decks.config2_3d(n=32, steps=1500, pml=8, d=0.02, sphere_cells=5, flux_cells=12, nfreq=4,
+                            precision="fp64")
+    return [long] + _mix()
+
+
 @pytest.mark.parametrize("graph", [0, 16])
 def test_server_results_equal_alone(graph):
-    mix = _mix()
+    mix = _served_mix()
     alone = [_alone(d) for d in mix]
     # budget that holds about two of the larger requests at once: the rest queue
     budget = 2 * max(chiml_amd.estimate_bytes(d) for d in mix)
