@@ -2,10 +2,14 @@
 //
 // One context = one device-resident domain.  Per step (the order of
 // parallelFDTDFieldBase::step(), src/FDTD_MANAGER/parallelFDTDField.hpp:1228-1303):
-//   1. k_update_h  : H += curl E with CPML psi (updateH + updateHPML) and H sources
-//   2. k_update_e  : soft E sources, Lorentz/Drude ADE, E += curl H with CPML psi
-//                    (addPul, updatePolE, updateE/updateD, updateEPML, D2E)
-//   3. emitters    : k_ml_addp then k_ml_density (parallelQEBase::addQE)
+//   1. H sweep     : H += curl E with CPML psi (updateH + updateHPML) and H
+//                    sources; in 3D its first blocks advance the poles (updatePolE)
+//   2. E sweep     : soft E sources, pole currents, E += curl H with CPML psi
+//                    (addPul, updateE/updateD, updateEPML, D2E)
+//                    3D FP32: k_h3_all / k_e3_all; 3D FP64: k_h3_int + k_h3_slab /
+//                    k_e3_int + k_e3_slab; 2D: k_h2v / k_e2v; else k_update_h / k_update_e
+//   3. emitters    : k_ml_addp then k_ml_density / k_ml_team and k_ml_pop_finish
+//                    (parallelQEBase::addQE)
 //   4. k_advance   : t_step += 1 (inside captured graphs; plain launches are
 //                    given the step number by the host)
 //   5. k_sample    : point detectors and DFT staging (dtc->output, flux->fieldIn)
@@ -2187,8 +2191,8 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
             const int Tx = p->pml_T[0], Ty = p->pml_T[1], Tz = p->pml_T[2];
             // FP32 (W = 4): the interior walk takes whole x rows of the y/z
             // interior (one wavefront per 256-cell row, the x slabs' CPML terms
-            // applied inline) and the slab kernel covers the z and y slabs.
-            // FP64 (W = 2): the x slabs stay in the slab kernel, which measured
+            // applied inline) and the slab region covers the z and y slabs.
+            // FP64 (W = 2): the x slabs stay in the slab region, which measured
             // faster there.
             const bool rows = c->prec == 32;
             Box3 b{rows ? 0 : Tx, rows ? p->nx : p->nx - Tx, Ty, p->ny - Ty, Tz, p->nz - Tz, 0};
