@@ -40,7 +40,9 @@
 //   denDeriv + PCABAM4                T1 decay rate, T2 decay + precession
 //                                     frequency, Rabi frequency and amplitude
 //                                     (SI dipole units), trace and hermiticity
-//   wigner3j / expectation (mu_z)     textbook 3j values, orthogonality, Rabi
+//   wigner3j / expectation            textbook 3j values, orthogonality, Rabi;
+//                                     mu_x, mu_y, mu_z equal <Y_i| x/r, y/r, z/r |Y_j>
+//                                     by quadrature over the sphere (scipy Y_lm)
 //   QEPopDtc flat index               pops equal rho[0], rho[3] (trap T3)
 //   flux                              box-size independence, ~0 for an outside
 //                                     source, quadratic in amplitude (loose)
@@ -50,8 +52,8 @@
 //   plane averages in material-filled PMLs, beyond the reflection bound;
 //   detectMatInPML's scan; Obj::isObj for sphere / cylinder / ellipsoid
 //   rasterisation; flux makeParamIn / combineField offset averaging and the
-//   Simpson weights beyond the invariants; expectation for mu_x / mu_y; the
-//   addQE polarisation back-coupling E -= P/(2 eps) at the two Yee edges.
+//   Simpson weights beyond the invariants; the addQE polarisation
+//   back-coupling E -= P/(2 eps) at the two Yee edges.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -1195,7 +1197,7 @@ static double sphInt(std::array<int, 2> a, std::array<int, 2> b, std::array<int,
     return sph;
 }
 // BasisSet::expectationVals, src/ML/BasisSet.cpp:17-43.
-static std::vector<cplx> expectation(const std::vector<std::array<int, 2>>& B, int dir) {
+std::vector<cplx> expectation(const std::vector<std::array<int, 2>>& B, int dir) {
     int N = static_cast<int>(B.size());
     std::vector<cplx> r;
     std::array<int, 2> op0 = {{1, 0}}, op1 = {{1, 1}}, op2 = {{1, -1}};
@@ -1672,6 +1674,19 @@ int oracle_qe_gamma(void* hp, int q, double* gam) {
 double oracle_hbar(void* hp, int q) { return static_cast<OracleHandle*>(hp)->sim->qes.at(q).hbar; }
 
 double oracle_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) { return orc::wigner3j(j1, j2, j3, m1, m2, m3); }
+
+// Test hook: the dipole operator matrix of axis dir in the basis (l, m) pairs
+// lm[2 * i], lm[2 * i + 1] (BasisSet::expectationVals), flat index i * N + j.
+int oracle_basis_dipole(int N, const int* lm, int dir, double* re, double* im) {
+    std::vector<std::array<int, 2>> B(N);
+    for (int i = 0; i < N; ++i) B[i] = {{lm[2 * i], lm[2 * i + 1]}};
+    auto r = orc::expectation(B, dir);
+    for (int k = 0; k < N * N; ++k) {
+        re[k] = r[k].real();
+        im[k] = r[k].imag();
+    }
+    return N;
+}
 
 double oracle_pulse(const char* json, int src, double t) {
     try {

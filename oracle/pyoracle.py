@@ -46,6 +46,8 @@ def lib():
         L.oracle_hbar.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_wigner3j.restype = ctypes.c_double
         L.oracle_wigner3j.argtypes = [ctypes.c_int] * 6
+        L.oracle_basis_dipole.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.oracle_pulse.restype = ctypes.c_double
         L.oracle_pulse.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
         _LIB = L
@@ -149,6 +151,17 @@ class Oracle:
 
 def wigner3j(j1, j2, j3, m1, m2, m3):
     return lib().oracle_wigner3j(j1, j2, j3, m1, m2, m3)
+
+
+def basis_dipole(basis, axis):
+    """Dipole operator matrix (x, y, z for axis 0, 1, 2) in a basis of (l, m)
+    spherical harmonics, as the emitter setup builds it."""
+    n = len(basis)
+    lm = (ctypes.c_int * (2 * n))(*[v for b in basis for v in b])
+    re = np.zeros(n * n)
+    im = np.zeros(n * n)
+    lib().oracle_basis_dipole(n, lm, axis, _dp(re), _dp(im))
+    return (re + 1j * im).reshape(n, n)
 
 
 def pulse(deck_json, src, t):

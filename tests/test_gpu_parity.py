@@ -144,6 +144,37 @@ def test_3d_n_level_emitter_block_fp64(nlev):
     assert rel_linf(rg, ro) <= 1e-10
 
 
+def test_3d_basis_set_emitters_x_y_dipoles_fp64():
+    """A 3x3x3 block of 4-level emitters whose dipoles come from the
+    spherical-harmonic basis {(0,0), (1,-1), (1,0), (1,1)} (x and y matrix
+    elements between m = +-1 states), driven by Ex and Ey point sources: the
+    product's own basis-dipole construction against the oracle's, which is
+    pinned to quadrature."""
+    nlev = 4
+    d = decks.empty_deck((20, 20, 20), 0.02, 90, 4, precision="fp64")
+    em = decks.n_level_emitter(nlev, [0.04, 0.04, 0.04], (0.01, 0.01, 0.01), seed=77)
+    del em["H0_eV"], em["dipole_Debye"]
+    em["Basis_Set"] = [{"l": 0, "m": 0}, {"l": 1, "m": -1}, {"l": 1, "m": 0}, {"l": 1, "m": 1}]
+    em["couplings"] = [12.0] * (nlev * nlev)
+    d["ObjectList"].append(em)
+    fc = decks.ev2fdtd(2.0)
+    pul = {"profile": "gaussian", "fcen": fc, "fwidth": fc, "cutoff": 3.0, "t_0": 0.3, "Field_Intensity": 1e10}
+    d["SourceList"].append({"pol": "Ex", "size": [0.0, 0.0, 0.0], "loc": [-0.09, 0.01, 0.01], "PulseList": [pul]})
+    d["SourceList"].append({"pol": "Ey", "size": [0.0, 0.0, 0.0], "loc": [0.01, -0.09, 0.03], "PulseList": [pul]})
+    deck = json.dumps(d)
+    sim, orc = _pair(deck, 90)
+    _fields(sim, orc, range(6), FP64_TOL)
+    q = sim.qeArr()[0]
+    for j in range(2):
+        pg, po = q.population(j), orc.pops(0, j)
+        assert pg.shape == po.shape
+        assert rel_linf(pg, po) <= 1e-10, (j, rel_linf(pg, po))
+    rg = q.density(0, 13, nlev)
+    ro = orc.density(0, 0, 13, nlev)
+    assert np.max(np.abs(ro - np.diag(np.diag(ro)))) > 0.0  # coherences were driven
+    assert rel_linf(rg, ro) <= 1e-10
+
+
 def test_2d_tmz_fp32_close_to_fp64_oracle():
     """FP32 fields against the FP64 oracle: detector series within 1e-5 of the
     series' peak, fields within 1e-4."""

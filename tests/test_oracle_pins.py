@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from chiml_amd import decks
-from oracle.pyoracle import Oracle, pulse, wigner3j
+from oracle.pyoracle import Oracle, basis_dipole, pulse, wigner3j
 
 
 # ------------------------------------------------------------------ 3j ------
@@ -30,6 +30,37 @@ from oracle.pyoracle import Oracle, pulse, wigner3j
 ])
 def test_wigner3j_textbook_values(args, value):
     assert wigner3j(*args) == pytest.approx(value, abs=1e-14)
+
+
+def _sph_y(l, m, polar, azim):
+    """Complex spherical harmonic with the Condon-Shortley phase (scipy)."""
+    from scipy import special
+    return special.sph_harm_y(l, m, polar, azim)
+
+
+def test_dipole_matrices_match_quadrature():
+    """The emitter dipole operators in a spherical-harmonic basis equal
+    <Y_i| x/r, y/r, z/r |Y_j> integrated over the sphere.  A Gauss-Legendre
+    rule in cos(theta) times a uniform rule in phi is exact for these
+    polynomial integrands, so this pins the sign, the phase (the factor i of
+    the y operator) and the m = +-1 combination independently of the 3j
+    route the oracle takes."""
+    basis = [(0, 0), (1, -1), (1, 0), (1, 1), (2, -2), (2, 1)]
+    xg, wg = np.polynomial.legendre.leggauss(24)
+    polar = np.arccos(xg)
+    azim = np.arange(48) * (2.0 * np.pi / 48)
+    TH, PH = np.meshgrid(polar, azim, indexing="ij")
+    W = np.outer(wg, np.full(48, 2.0 * np.pi / 48))
+    unit = [np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)]
+    Y = [_sph_y(l, m, TH, PH) for (l, m) in basis]
+    for axis in range(3):
+        ref = np.array([[np.sum(W * np.conj(Y[i]) * unit[axis] * Y[j]) for j in range(len(basis))]
+                        for i in range(len(basis))])
+        got = basis_dipole(basis, axis)
+        assert np.max(np.abs(ref)) > 0.1
+        assert np.max(np.abs(got - ref)) < 1e-12, (axis, np.max(np.abs(got - ref)))
+        # Hermitian, as every Cartesian dipole component must be
+        assert np.max(np.abs(got - got.conj().T)) < 1e-12
 
 
 def test_wigner3j_orthogonality():
