@@ -40,6 +40,9 @@
 //   denDeriv + PCABAM4                T1 decay rate, T2 decay + precession
 //                                     frequency, Rabi frequency and amplitude
 //                                     (SI dipole units), trace and hermiticity
+//   addQE back-coupling               a weakly driven 2-level slab transmits what
+//                                     the equivalent Lorentz slab transmits (linear
+//                                     response), to 5% of the slab's effect
 //   wigner3j / expectation            textbook 3j values, orthogonality, Rabi;
 //                                     mu_x, mu_y, mu_z equal <Y_i| x/r, y/r, z/r |Y_j>
 //                                     by quadrature over the sphere (scipy Y_lm)
@@ -52,8 +55,7 @@
 //   plane averages in material-filled PMLs, beyond the reflection bound;
 //   detectMatInPML's scan; Obj::isObj for sphere / cylinder / ellipsoid
 //   rasterisation; flux makeParamIn / combineField offset averaging and the
-//   Simpson weights beyond the invariants; the addQE polarisation
-//   back-coupling E -= P/(2 eps) at the two Yee edges.
+//   Simpson weights beyond the invariants.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -1674,6 +1676,21 @@ int oracle_qe_gamma(void* hp, int q, double* gam) {
 double oracle_hbar(void* hp, int q) { return static_cast<OracleHandle*>(hp)->sim->qes.at(q).hbar; }
 
 double oracle_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) { return orc::wigner3j(j1, j2, j3, m1, m2, m3); }
+
+// Test hook: an emitter group's number density, hbar (code units), and the
+// real parts of member 0's H0 and of its dipole matrix along axis c (the
+// Hamiltonian is H0 - E_c mu_c), flat index i * N + j.
+int oracle_qe_params(void* hp, int q, int c, double* na, double* hbar, double* h0re, double* mure) {
+    auto& Q = static_cast<OracleHandle*>(hp)->sim->qes.at(q);
+    auto& m = Q.members.at(0);
+    *na = Q.na;
+    *hbar = Q.hbar;
+    for (int k = 0; k < Q.N * Q.N; ++k) {
+        h0re[k] = m.h0[k].real();
+        mure[k] = -m.negmu[c][k].real();
+    }
+    return Q.N;
+}
 
 // Test hook: the dipole operator matrix of axis dir in the basis (l, m) pairs
 // lm[2 * i], lm[2 * i + 1] (BasisSet::expectationVals), flat index i * N + j.

@@ -46,6 +46,9 @@ def lib():
         L.oracle_hbar.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_wigner3j.restype = ctypes.c_double
         L.oracle_wigner3j.argtypes = [ctypes.c_int] * 6
+        L.oracle_qe_params.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]
         L.oracle_basis_dipole.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.oracle_pulse.restype = ctypes.c_double
@@ -141,6 +144,16 @@ class Oracle:
 
     def hbar(self, q):
         return lib().oracle_hbar(self._h, q)
+
+    def qe_params(self, q, axis, nlev):
+        """(number density, hbar, H0 real part, dipole real part along axis) of
+        emitter group q in code units."""
+        na = ctypes.c_double()
+        hb = ctypes.c_double()
+        h0 = np.zeros(nlev * nlev)
+        mu = np.zeros(nlev * nlev)
+        lib().oracle_qe_params(self._h, q, axis, ctypes.byref(na), ctypes.byref(hb), _dp(h0), _dp(mu))
+        return na.value, hb.value, h0.reshape(nlev, nlev), mu.reshape(nlev, nlev)
 
     def density(self, q, member, emitter, nlev):
         re = np.zeros(nlev * nlev)

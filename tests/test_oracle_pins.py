@@ -503,3 +503,47 @@ def test_source_lands_on_its_cell_after_the_h_update():
     assert np.count_nonzero(ez) == 1
     assert ez[cy, cx] == pytest.approx(o.dt * pulse(js, 0, 0.0), rel=1e-14)
     assert np.all(o.field(3) == 0) and np.all(o.field(4) == 0)
+
+
+# ------------------------------------------------ emitter back-coupling -----
+def test_emitter_slab_linear_response_equals_lorentz_slab():
+    """In linear response a 2-level emitter with transition frequency w0,
+    coherence decay g and dipole mu is a Lorentz oscillator:
+    P'' + 2g P' + (w0^2 + g^2) P = (2 na mu^2 w0 / hbar) E.  A weakly driven
+    slab of emitters (addQE back-coupling E -= P/eps) must therefore transmit
+    what a Lorentz slab with omega^2 = w0^2 + g^2, gamma = g and sigma =
+    2 na mu^2 w0 / (hbar omega^2) transmits.  A wrong sign of the coupling
+    shows as twice the effect, a lost factor 2 (the two-edge average) as half.
+    Odd point counts: the reference centres emitter points on (n - 1)/2 and
+    material cells on (n - n % 2)/2, which differ by half a cell on even grids."""
+    nx, ny, d, pml, steps = 161, 141, 0.05, 12, 3500
+    deck = decks.empty_deck((nx, ny, 1), d, steps, pml, precision="fp64")
+    fc = decks.ev2fdtd(2.0)
+    xw = (nx - 2 * pml - 4) * d
+    deck["SourceList"].append({"pol": "Ez", "size": [xw, 0.0, 0.0], "loc": [0.0, decks._pos(ny // 2 + 25, ny, d), 0.0],
+                               "PulseList": [decks.gaussian_pulse(fc, fc, 3.0, 1e2)]})
+    deck["DetectorList"].append({"type": "Ez", "dtc_class": "cout", "fname": "t", "size": [0.0, 0.0, 0.0],
+                                 "loc": [0.0, decks._pos(ny // 2 - 25, ny, d), 0.0]})
+    slab = {"size": [xw, 10 * d, 0.0], "loc": [0.0, decks._pos(ny // 2, ny, d) + 0.5 * d, 0.0]}
+
+    def transmitted(objs):
+        dd = json.loads(json.dumps(deck))
+        dd["ObjectList"].extend(objs)
+        o = Oracle(json.dumps(dd))
+        o.step(steps)
+        return o, o.detector(0)[1][:, 0]
+
+    o_ml, v_ml = transmitted([decks.two_level_emitter(slab["size"][0], slab["size"][1], slab["loc"])])
+    na, hb, h0, mu = o_ml.qe_params(0, 2, 2)
+    g = -o_ml.qe_gamma(0, 2)[2, 2]  # decay rate of rho_01 (flat index 0 + 1 * 2)
+    w0 = (h0[1, 1] - h0[0, 0]) / hb
+    w2 = w0 * w0 + g * g
+    sig = 2.0 * na * mu[0, 1] ** 2 * w0 / (hb * w2)
+    lor = {"shape": "block", "material": "custom", "eps": 1.0, "Basis_Set": [],
+           "pols": [{"omega": math.sqrt(w2) / (2 * math.pi), "gamma": g / math.pi, "sigma_p": sig}], **slab}
+    _, v_l = transmitted([lor])
+    _, v_v = transmitted([])
+    effect = np.abs(v_ml - v_v).max()
+    assert effect > 0.01 * np.abs(v_v).max()  # the slab matters
+    assert np.abs(v_ml - v_l).max() <= 0.05 * effect, (np.abs(v_ml - v_l).max(), effect)
+
