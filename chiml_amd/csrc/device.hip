@@ -10,8 +10,9 @@
 //                    k_e3_int + k_e3_slab; 2D: k_h2v / k_e2v; else k_update_h / k_update_e
 //   3. emitters    : k_ml_addp then k_ml_density / k_ml_team and k_ml_pop_finish
 //                    (parallelQEBase::addQE)
-//   4. k_advance   : t_step += 1 (inside captured graphs; plain launches are
-//                    given the step number by the host)
+//   4. t_step += 1 : plain launches are given the step number by the host;
+//                    inside captured graphs the H launch advances a two-slot
+//                    device counter (advance_counter)
 //   5. k_sample    : point detectors and DFT staging (dtc->output, flux->fieldIn)
 //   6. k_dft_flush : every kDftBlock samples, F[s,f] += sum_k W[k,f] U[k,s]
 //
@@ -122,11 +123,23 @@ struct StepArgs {
     const T* src_amp;
     const int* step;
     int stepv;  // >= 0: the step number, passed by the host (plain launches); -1: read *step (captured graphs)
+    int* step_next;  // captured graphs: the other counter slot, set to *step + 1 by the H launch
 };
 
 // The step number a launch works on: passed by the host for plain launches,
 // read from the device counter inside captured graphs.
 __device__ __forceinline__ int step_now(const int* p, int v) { return v >= 0 ? v : *p; }
+
+// Inside captured graphs the step counter has two slots: step k's launches
+// read slot k % 2, and the first block of its H launch writes slot (k+1) % 2,
+// which nothing reads before the next step's launches (the sample and DFT
+// launches of step k read it after the H launch, as the advanced step).
+template <typename A>
+__device__ __forceinline__ void advance_counter(const A& a) {
+    if (a.step_next && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0 &&
+        threadIdx.z == 0)
+        *a.step_next = *a.step + 1;
+}
 
 __device__ __forceinline__ long long lidx(int x, int y, int z, int nx, int ny) {
     return x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
@@ -167,6 +180,7 @@ __device__ __forceinline__ T src_value(const StepArgs<T>& a, int comp, int x, in
 // the Ez terms of Hx and Hy (OneCompCurlK/J, src/UTIL/FDTD_up_eq.cpp:10-35).
 template <typename T, int DIM, bool HMAP, bool SRC>
 __global__ void __launch_bounds__(256) k_update_h(StepArgs<T> a) {
+    advance_counter(a);
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z;
@@ -821,6 +835,7 @@ __host__ __device__ constexpr int ade_blocks(int ade_bpc) { return (3 * ade_bpc 
 
 template <typename T, int W, bool SRC>
 __global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b, int nib, int ade_bpc) {
+    advance_counter(a);
     const int nade = ade_blocks(ade_bpc);
     if (static_cast<int>(blockIdx.x) < nade) {
         const int c = blockIdx.x / max(ade_bpc, 1);
@@ -934,6 +949,7 @@ __global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int n
 // kernel would.
 template <typename T, int W, bool SRC>
 __global__ void __launch_bounds__(256) k_h3_all(StepArgs<T> a, Box3 b, SlabMap m, int nib, int nsbp, int ade_bpc) {
+    advance_counter(a);
     const int nade = ade_blocks(ade_bpc);
     int bx = blockIdx.x;
     if (bx < nade) {
@@ -984,6 +1000,7 @@ __global__ void __launch_bounds__(256) k_e3_all(StepArgs<T> a, Box3 b, SlabMap m
 // x-slab membership is uniform within a group as in 3D).
 template <typename T, int W, bool SRC>
 __global__ void __launch_bounds__(256) k_h2v(StepArgs<T> a) {
+    advance_counter(a);
     const int nx = a.nx, ny = a.ny, gx = nx / W;
     const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= static_cast<long long>(gx) * ny) return;
@@ -1058,7 +1075,6 @@ __global__ void __launch_bounds__(256) k_e2v(StepArgs<T> a, int nm2) {
 }
 
 // ----------------------------------------------------- step counter ---------
-__global__ void k_advance(int* step) { *step += 1; }
 __global__ void k_set_step(int* step, int v) { *step = v; }
 
 // ----------------------------------------------- detectors and DFT ----------
@@ -1660,7 +1676,8 @@ struct chiml_dev {
     bool hmap = false, has_src = false;
     int nmat[3] = {1, 1, 1};
     int enq_step = 0;    // step counter value after the step being enqueued
-    int graph_steps = 0;  // > 0: replay captured graphs of this many steps
+    int graph_steps = 0;  // > 0: replay captured graphs of this many steps (even)
+    int graph_parity = 0;  // counter slot (steps_done & 1) the captured graph starts on
     bool capturing = false;  // do_steps is being captured into a graph
     bool device_flush = false;  // launch the DFT flush every step; it decides on the device
     hipGraphExec_t gexec = nullptr;
@@ -1822,6 +1839,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
     }
     a.step = c->step_ctr.as<int>();
     a.stepv = -1;
+    a.step_next = nullptr;
     // detectors and DFT
     for (int k = 0; k < 6; ++k) s.F[k] = a.F[k];
     s.step = a.step;
@@ -1982,14 +2000,14 @@ struct Timed {
 // step: the step number once the step being enqueued has advanced it (-1:
 // read the device counter, inside captured graphs)
 template <typename T>
-void launch_flush(chiml_dev* c, int force, int step) {
+void launch_flush(chiml_dev* c, int force, int step, const int* stepp) {
     if (c->ndft == 0) return;
     // enq_step = the device step counter once the step being enqueued has advanced it
     const int st = c->enq_step;
     if (!force && !c->device_flush && (st % c->dft_interval != 0 || ((st / c->dft_interval) % kDftBlock) != kDftBlock - 1)) return;
     Timed t(c, "dft_flush");
     hipLaunchKernelGGL(k_dft_flush<T>, dim3((c->ndft + 255) / 256), dim3(256), 0, c->stream, static_cast<const T*>(c->dft_stage),
-                       c->dft_phasor, c->dft_F, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, force, step);
+                       c->dft_phasor, c->dft_F, stepp, c->ndft, c->nfreq, c->dft_interval, force, step);
 }
 
 
@@ -2109,29 +2127,37 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
     }
 }
 
-// Plain launches carry the step number from the host, so no counter kernel
-// runs between steps and the device counter is set once at the end.  Steps
-// being captured into a graph read and advance the device counter instead,
-// so that one graph serves every replay.
+// Plain launches carry the step number from the host, so no counter work
+// runs between steps, and the counter slot of the next step is set once at
+// the end.  Steps being captured into a graph read the two-slot device
+// counter instead (advance_counter), so that one graph serves every replay
+// that starts on the same slot.
 template <typename T>
 void do_steps(chiml_dev* c, StepArgs<T>& a, SampleArgs<T>& s, int n) {
     const bool host = !c->capturing;
+    int* ctr = c->step_ctr.as<int>();
     for (int k = 0; k < n; ++k) {
-        const int st = c->steps_done + k;
+        const int st = c->steps_done + k, p = st & 1;
         a.stepv = host ? st : -1;
         s.stepv = host ? st + 1 : -1;
+        a.step = ctr + p;
+        a.step_next = host ? nullptr : ctr + (1 - p);
+        s.step = ctr + (1 - p);
         if (c->dim == 3) launch_fields<T, 3>(c, a);
         else launch_fields<T, 2>(c, a);
         if (!c->qes.empty()) launch_ml<T>(c, a);
-        if (!host) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->step_ctr.as<int>());
         c->enq_step = st + 1;
         launch_sample<T>(c, s);
-        launch_flush<T>(c, 0, host ? st + 1 : -1);
+        launch_flush<T>(c, 0, host ? st + 1 : -1, ctr + (1 - p));
         HIP_TRY(hipGetLastError());
     }
-    if (host && n > 0) hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, c->stream, c->step_ctr.as<int>(), c->steps_done + n);
-    a.stepv = -1;  // launches outside the step loop (the t = 0 sample) read the counter
+    if (host && n > 0) {
+        const int end = c->steps_done + n;
+        hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, c->stream, ctr + (end & 1), end);
+    }
+    a.stepv = -1;
     s.stepv = -1;
+    a.step_next = nullptr;
     c->steps_done += n;
 }
 
@@ -2180,7 +2206,7 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
         c->nz = p->nz;
         c->dim = p->nz > 1 ? 3 : 2;
         c->nsteps = p->nsteps;
-        c->step_ctr.alloc(sizeof(int), c->stream, true);
+        c->step_ctr.alloc(2 * sizeof(int), c->stream, true);
         if (c->prec == 32) build<float>(c.get(), p, c->a32, c->s32);
         else build<double>(c.get(), p, c->a64, c->s64);
         if (c->dim == 2) {
@@ -2248,6 +2274,7 @@ void ensure_graph(chiml_dev* c) {
     std::lock_guard<std::mutex> lk(g_capture_mu);
     hipGraph_t g = nullptr;
     const int saved = c->steps_done;
+    c->graph_parity = c->steps_done & 1;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
     try {
@@ -2275,6 +2302,11 @@ int chiml_dev_step(chiml_dev_t* c, int n) {
         if (c->graph_steps > 0) {
             while (n >= c->graph_steps) {
                 ensure_graph(c);
+                if ((c->steps_done & 1) != c->graph_parity) {  // realign the counter slot with one plain step
+                    enqueue_steps(c, 1);
+                    --n;
+                    continue;
+                }
                 HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
                 c->steps_done += c->graph_steps;
                 n -= c->graph_steps;
@@ -2294,7 +2326,8 @@ int chiml_dev_set_graph(chiml_dev_t* c, int steps_per_graph) {
         (void)hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
     }
-    c->graph_steps = steps_per_graph;
+    // even, so that every replay starts on the counter slot the graph was captured on
+    c->graph_steps = steps_per_graph + (steps_per_graph & 1);
     c->device_flush = steps_per_graph > 0;
     return CHIML_OK;
 }

@@ -153,9 +153,9 @@ int chiml_dev_sync(chiml_dev_t* ctx);
  * (the serving loop polls this instead of blocking on one context). */
 int chiml_dev_query(chiml_dev_t* ctx, int* idle);
 /* steps_per_graph > 0: chiml_dev_step replays a HIP graph of that many
- * captured steps for each whole chunk (the DFT flush then decides on the
- * device every step); 0 restores plain launches.  Per-launch profiling is
- * off while graphs are in use. */
+ * captured steps (rounded up to even) for each whole chunk (the DFT flush
+ * then decides on the device every step); 0 restores plain launches.
+ * Per-launch profiling is off while graphs are in use. */
 int chiml_dev_set_graph(chiml_dev_t* ctx, int steps_per_graph);
 /* Copies one field component to host memory as float64 (n = nx*ny*nz). */
 int chiml_dev_read_field(chiml_dev_t* ctx, int comp, double* host);
