@@ -54,6 +54,28 @@ def test_2d_tmz_cylinder_fp64(profile):
         assert rel_linf(v_g, v_o) <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", 1e-4)])
+def test_2d_detectors_on_every_component(precision, tol):
+    """A small 2D deck whose detectors read Ez, Hx, Hy and E power (2D decks
+    this small sample their point detectors inside the E kernel)."""
+    d = json.loads(decks.config1_2d(n=64, steps=300, pml=10, cyl_radius_cells=12, src_cell=(20, 32), ndet=2,
+                                    precision=precision))
+    base = d["DetectorList"][0]
+    for typ in ("Hx", "Hy", "Epow"):
+        det = dict(base)
+        det["type"] = typ
+        d["DetectorList"].append(det)
+    deck = json.dumps(d)
+    sim, orc = _pair(deck, 300)
+    assert orc.num_detectors() == 5
+    for k in range(orc.num_detectors()):
+        t_o, v_o = orc.detector(k)
+        t_g, v_g = sim.dtcArr()[k].series()
+        assert v_g.shape == v_o.shape, k
+        assert np.abs(v_o).max() > 0.0, k
+        assert rel_linf(v_g, v_o) <= tol, (k, rel_linf(v_g, v_o))
+
+
 def test_3d_aniso_drude_lorentz_flux_fp64():
     deck = decks.config2_3d(n=28, steps=120, pml=6, d=0.02, sphere_cells=6, flux_cells=12, nfreq=8, precision="fp64")
     sim, orc = _pair(deck, 120)
