@@ -38,6 +38,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/chiml/chiml_dev.h"
@@ -134,6 +135,11 @@ __device__ __forceinline__ int step_now(const int* p, int v) { return v >= 0 ? v
 // read slot k % 2, and the first block of its H launch writes slot (k+1) % 2,
 // which nothing reads before the next step's launches (the sample and DFT
 // launches of step k read it after the H launch, as the advanced step).
+// Only the H kernels' ADV instantiations, launched by steps being captured,
+// carry it, after their sweep: a store ahead of the sweep's loads keeps the
+// compiler from proving the uniform ones unclobbered, and they become
+// per-lane vector loads (the 256^3 FP32 H sweep took 8% longer), and even
+// at the end the store cost plain launches 3%.
 template <typename A>
 __device__ __forceinline__ void advance_counter(const A& a) {
     if (a.step_next && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0 &&
@@ -179,8 +185,7 @@ __device__ __forceinline__ T src_value(const StepArgs<T>& a, int comp, int x, in
 // (Ex(z+1)-Ex)/dz], Hz += ch[(Ex(y+1)-Ex)/dy - (Ey(x+1)-Ey)/dx]; 2D TMz keeps
 // the Ez terms of Hx and Hy (OneCompCurlK/J, src/UTIL/FDTD_up_eq.cpp:10-35).
 template <typename T, int DIM, bool HMAP, bool SRC>
-__global__ void __launch_bounds__(256) k_update_h(StepArgs<T> a) {
-    advance_counter(a);
+__device__ __forceinline__ void update_h_cells(const StepArgs<T>& a) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z;
@@ -242,6 +247,12 @@ __global__ void __launch_bounds__(256) k_update_h(StepArgs<T> a) {
         if (SRC) h += src_value(a, 5, x, y, z, step);
         a.F[5][i] = h;
     }
+}
+
+template <typename T, int DIM, bool HMAP, bool SRC, bool ADV>
+__global__ void __launch_bounds__(256) k_update_h(StepArgs<T> a) {
+    update_h_cells<T, DIM, HMAP, SRC>(a);
+    if (ADV) advance_counter(a);
 }
 
 // ----------------------------------------------------------- E update -------
@@ -835,8 +846,7 @@ __device__ __forceinline__ bool int_group(const Box3& b, long long t, int& x0, i
 __host__ __device__ constexpr int ade_blocks(int ade_bpc) { return (3 * ade_bpc + 7) / 8 * 8; }
 
 template <typename T, int W, bool SRC>
-__global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b, int nib, int ade_bpc) {
-    advance_counter(a);
+__device__ __forceinline__ void h3_int_sweep(const StepArgs<T>& a, const Box3& b, int nib, int ade_bpc) {
     const int nade = ade_blocks(ade_bpc);
     if (static_cast<int>(blockIdx.x) < nade) {
         const int c = blockIdx.x / max(ade_bpc, 1);
@@ -850,6 +860,12 @@ __global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b, int nib, 
     // consecutive threads walk a row: lane + 1 holds the next group unless this one ends the row
     const int lane = threadIdx.x & 63, gw = (b.x1 - b.x0) / W;
     h_group<T, W, false, SRC>(a, x0, y, z, lane < 63 && col + 1 < gw);
+}
+
+template <typename T, int W, bool SRC, bool ADV>
+__global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b, int nib, int ade_bpc) {
+    h3_int_sweep<T, W, SRC>(a, b, nib, ade_bpc);
+    if (ADV) advance_counter(a);
 }
 
 template <typename T, int W, bool SRC>
@@ -949,8 +965,8 @@ __global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int n
 // kernel carries no CPML code and runs at a higher occupancy than a merged
 // kernel would.
 template <typename T, int W, bool SRC>
-__global__ void __launch_bounds__(256) k_h3_all(StepArgs<T> a, Box3 b, SlabMap m, int nib, int nsbp, int ade_bpc) {
-    advance_counter(a);
+__device__ __forceinline__ void h3_all_sweep(const StepArgs<T>& a, const Box3& b, const SlabMap& m, int nib, int nsbp,
+                                             int ade_bpc) {
     const int nade = ade_blocks(ade_bpc);
     int bx = blockIdx.x;
     if (bx < nade) {
@@ -973,6 +989,12 @@ __global__ void __launch_bounds__(256) k_h3_all(StepArgs<T> a, Box3 b, SlabMap m
     if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z, col)) return;
     const int lane = threadIdx.x & 63, gw = (b.x1 - b.x0) / W;
     h_group<T, W, true, SRC>(a, x0, y, z, lane < 63 && col + 1 < gw);
+}
+
+template <typename T, int W, bool SRC, bool ADV>
+__global__ void __launch_bounds__(256) k_h3_all(StepArgs<T> a, Box3 b, SlabMap m, int nib, int nsbp, int ade_bpc) {
+    h3_all_sweep<T, W, SRC>(a, b, m, nib, nsbp, ade_bpc);
+    if (ADV) advance_counter(a);
 }
 
 template <typename T, int W, bool SRC>
@@ -1045,8 +1067,7 @@ __device__ __forceinline__ void store_point(const SampleArgs<T>& a, int t, doubl
 // 2D TMz on W-cell groups over the whole domain (CPML handled per group; the
 // x-slab membership is uniform within a group as in 3D).
 template <typename T, int W, bool SRC>
-__global__ void __launch_bounds__(256) k_h2v(StepArgs<T> a) {
-    advance_counter(a);
+__device__ __forceinline__ void h2v_sweep(const StepArgs<T>& a) {
     const int nx = a.nx, ny = a.ny, gx = nx / W;
     const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= static_cast<long long>(gx) * ny) return;
@@ -1090,6 +1111,12 @@ __global__ void __launch_bounds__(256) k_h2v(StepArgs<T> a) {
         }
         stv<T, W>(a.F[4] + i, h);
     }
+}
+
+template <typename T, int W, bool SRC, bool ADV>
+__global__ void __launch_bounds__(256) k_h2v(StepArgs<T> a) {
+    h2v_sweep<T, W, SRC>(a);
+    if (ADV) advance_counter(a);
 }
 
 // DET: the deck's point detectors (single-cell, Ez / Hx / Hy) are sampled
@@ -2036,8 +2063,24 @@ SampleArgs<float>& sample_args<float>(chiml_dev* c) { return c->s32; }
 template <>
 SampleArgs<double>& sample_args<double>(chiml_dev* c) { return c->s64; }
 
+// f(S, A) with S, A std::integral_constant<bool> for the two runtime flags
+template <typename F>
+void pick2(bool s, bool adv, F&& f) {
+    using Y = std::true_type;
+    using N = std::false_type;
+    if (s) {
+        if (adv) f(Y{}, Y{});
+        else f(Y{}, N{});
+    } else {
+        if (adv) f(N{}, Y{});
+        else f(N{}, N{});
+    }
+}
+
+// adv: the H launch advances the two-slot step counter (steps being captured)
 template <typename T, int DIM>
 void launch_fields(chiml_dev* c, StepArgs<T>& a) {
+    const bool adv = a.step_next != nullptr;
     if (DIM == 3 && c->split) {
         constexpr int W = 16 / sizeof(T);
         const Box3 b = c->ibox;
@@ -2051,8 +2094,10 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
             const dim3 gh(static_cast<unsigned>(ade_blocks(ade_bpc) + nsbp + nib)), ge(static_cast<unsigned>(nsbp + nib));
             if (gh.x > 0) {
                 Timed t(c, "update_h");
-                if (c->has_src) hipLaunchKernelGGL((k_h3_all<T, W, true>), gh, blk, 0, c->stream, a, b, c->slab, nib, nsbp, ade_bpc);
-                else hipLaunchKernelGGL((k_h3_all<T, W, false>), gh, blk, 0, c->stream, a, b, c->slab, nib, nsbp, ade_bpc);
+                pick2(c->has_src, adv, [&](auto S, auto A) {
+                    hipLaunchKernelGGL((k_h3_all<T, W, decltype(S)::value, decltype(A)::value>), gh, blk, 0, c->stream, a, b, c->slab,
+                                       nib, nsbp, ade_bpc);
+                });
             }
             if (ge.x > 0) {
                 Timed t(c, "update_e");
@@ -2065,8 +2110,10 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
             const dim3 gh(static_cast<unsigned>(nib + ade_blocks(ade_bpc)));
             if (gh.x > 0) {
                 Timed t(c, "update_h");
-                if (c->has_src) hipLaunchKernelGGL((k_h3_int<T, W, true>), gh, blk, 0, c->stream, a, b, nib, ade_bpc);
-                else hipLaunchKernelGGL((k_h3_int<T, W, false>), gh, blk, 0, c->stream, a, b, nib, ade_bpc);
+                pick2(c->has_src, adv, [&](auto S, auto A) {
+                    hipLaunchKernelGGL((k_h3_int<T, W, decltype(S)::value, decltype(A)::value>), gh, blk, 0, c->stream, a, b, nib,
+                                       ade_bpc);
+                });
             }
             if (ns > 0) {
                 Timed t(c, "update_h_pml");
@@ -2092,8 +2139,9 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
         const dim3 g(static_cast<unsigned>((ng + 255) / 256)), blk(256);
         {
             Timed t(c, "update_h");
-            if (c->has_src) hipLaunchKernelGGL((k_h2v<T, W, true>), g, blk, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_h2v<T, W, false>), g, blk, 0, c->stream, a);
+            pick2(c->has_src, adv, [&](auto S, auto A) {
+                hipLaunchKernelGGL((k_h2v<T, W, decltype(S)::value, decltype(A)::value>), g, blk, 0, c->stream, a);
+            });
         }
         {
             Timed t(c, "update_e");
@@ -2112,13 +2160,10 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
     dim3 grd((c->nx + 63) / 64, (c->ny + 3) / 4, c->nz);
     {
         Timed t(c, "update_h");
-        if (c->hmap) {
-            if (c->has_src) hipLaunchKernelGGL((k_update_h<T, DIM, true, true>), grd, blk, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_update_h<T, DIM, true, false>), grd, blk, 0, c->stream, a);
-        } else {
-            if (c->has_src) hipLaunchKernelGGL((k_update_h<T, DIM, false, true>), grd, blk, 0, c->stream, a);
-            else hipLaunchKernelGGL((k_update_h<T, DIM, false, false>), grd, blk, 0, c->stream, a);
-        }
+        pick2(c->has_src, adv, [&](auto S, auto A) {
+            if (c->hmap) hipLaunchKernelGGL((k_update_h<T, DIM, true, decltype(S)::value, decltype(A)::value>), grd, blk, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_update_h<T, DIM, false, decltype(S)::value, decltype(A)::value>), grd, blk, 0, c->stream, a);
+        });
     }
     {
         Timed t(c, "update_e");
