@@ -18,6 +18,11 @@ untimed pass of up to 200 steps after the warmup, where every launch is timed
 (the "kernels" breakdown).  traffic
 comes from the committed rocprofv3 --pmc summary
 (profiles/traffic_config2_<precision>.json) when it is present.
+step_roofline: the whole step against the bytes it would move if every state
+word were read once and written once (fields, one material byte per cell,
+psi, D and two P words per pole), over the summed field-kernel time per step
+and over the wall-clock step.  The split H and E sweeps read each field twice,
+so this fraction sits below the per-kernel one.
 cpu_baseline: the FP64 CPU restatement of the reference step loop (the
 oracle) timed on this host for a few steps of the same 256^3 deck.
 """
@@ -107,6 +112,7 @@ def run_gpu(n, steps, warmup, precision):
         "dominant_ms": dom_ms / max(dom_n, 1),
         "dominant_launches": dom_n,
         "dominant_bytes": sim.kernel_bytes(dominant),
+        "yee_step_min": sim.kernel_bytes("yee_step_min"),
         "footprint": sim.footprint(),
         "flux0": None if flux is None else [float(flux[0].real), float(flux[0].imag)],
     }
@@ -144,6 +150,28 @@ def roofline(r, precision):
         "ms_per_launch": round(ms, 5),
         "launches": r["dominant_launches"],
         "algorithmic_bytes_per_launch": b,
+    }
+
+
+def step_roofline(r):
+    """The whole Yee + CPML + ADE step against the compulsory bytes of a step
+    in which every state word is read once and written once (a fused H and E
+    update): over the field kernels' time per step (the per-launch events of
+    the untimed profiled pass) and over the timed wall-clock step."""
+    b = r["yee_step_min"]
+    ms = sum(r["kernels"][k]["ms_per_step"] for k in ("update_h", "update_h_pml", "update_e", "update_e_pml")
+             if k in r["kernels"])
+    if b <= 0 or ms <= 0:
+        return None
+    achieved = b / (ms * 1e-3) / 1e9
+    return {
+        "bytes_min_per_step": b,
+        "field_kernels_ms_per_step": round(ms, 5),
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "frac_of_wall_step": round(b / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
 
 
@@ -295,6 +323,7 @@ def main():
             "parallelism": "single GPU, one device-resident domain",
         },
         "roofline": roof,
+        "step_roofline": step_roofline(r32),
         "kernels": kernel_table(r32),
     }
     if not args.no_fp64 and args.precision == "fp32":
@@ -304,6 +333,7 @@ def main():
             "value": round(r64["mcells"], 1),
             "ms_per_step": round(r64["ms_per_step"], 4),
             "roofline": roof64,
+            "step_roofline": step_roofline(r64),
             "kernels": kernel_table(r64),
         }
         out["fp32_over_fp64"] = round(r32["mcells"] / r64["mcells"], 3)

@@ -968,7 +968,7 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
             kbytes["update_h_pml"] = Ns * (2.0 * nH + nE) * s;
             kbytes["update_e_pml"] = Ns * ((2.0 * nE + nH) * s + nE);
         }
-        double psiWords = 0.0;
+        double psiWords = 0.0, dispMin = 0.0;
         for (int comp = 0; comp < 6; ++comp)
             for (int a = 0; a < 3; ++a)
                 if (P.pml_b[comp][a]) {
@@ -996,6 +996,7 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
                     for (int x = b[0]; x < b[1]; ++x) {
                         int ps = psT[c][matE[c][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)]];
                         if (ps < 0) continue;
+                        dispMin += 2.0 * s + pcount[ps] * 4.0 * s;
                         if (split) {
                             // pole kernel: P and J read and written, E read, current sum written;
                             // the E kernel reads the current sum back
@@ -1007,6 +1008,13 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
                     }
         }
         kbytes["ml_density"] = bytesML;
+        // The compulsory bytes of one whole Yee + CPML + ADE step if every state
+        // word were read once and written once (a fused H and E update): per
+        // cell 2 F words (F = 6 in 3D, 3 in 2D TMz) and one material byte, 2
+        // words per psi word, and per dispersive cell-component the D word and
+        // P, P_prev per pole, read and written.  The split H and E sweeps read
+        // E in the H sweep and H in the E sweep on top of this.
+        kbytes["yee_step_min"] = cells * (2.0 * (nE + nH) * s + 1.0) + psiWords * 2.0 * s + dispMin;
         // DFT flush of one block of 32 staged samples: read the staged
         // samples and the block's phasors, read and write every accumulator
         if (ndft > 0)

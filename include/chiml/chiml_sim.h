@@ -48,8 +48,10 @@ int chiml_sim_density(chiml_sim_t* s, int q, int member, int emitter, double* re
  * kernels (update_h + update_e), used for the roofline figure. */
 int chiml_sim_cost(chiml_sim_t* s, double* cells, double* bytes_h, double* bytes_e, double* bytes_ml);
 /* Algorithmic HBM bytes per launch of one kernel ("update_h", "update_e",
- * "update_h_pml", "update_e_pml", "ade", "ml_density", "dft_flush"); 0 for
- * unknown names. */
+ * "update_h_pml", "update_e_pml", "ml_density", "dft_flush"; the pole update
+ * is counted in "update_h"), or with "yee_step_min" the compulsory bytes of a
+ * whole Yee + CPML + ADE step if each state word were read and written once;
+ * 0 for unknown names. */
 int chiml_sim_kernel_bytes(chiml_sim_t* s, const char* kernel, double* bytes);
 /* Per-kernel HIP-event timing (chiml_dev_profile / chiml_dev_profile_only). */
 int chiml_sim_profile(chiml_sim_t* s, int enable);
