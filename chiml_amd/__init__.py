@@ -63,6 +63,11 @@ def load_library():
     L.chiml_server_drain.argtypes = [ctypes.c_void_p]
     L.chiml_server_stats.argtypes = [ctypes.c_void_p, c_dbl_p, c_size_p, c_int_p, c_int_p, c_int_p]
     L.chiml_server_destroy.argtypes = [ctypes.c_void_p]
+    L.chiml_write_detector_txt.argtypes = [ctypes.c_char_p, c_dbl_p, c_dbl_p, ctypes.c_size_t, ctypes.c_int, c_dbl_p]
+    L.chiml_write_detector_bin.argtypes = [ctypes.c_char_p, c_dbl_p, c_dbl_p, ctypes.c_size_t, c_int_p, c_int_p]
+    L.chiml_write_flux.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int]
+    L.chiml_write_pops.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double]
+    L.chiml_sim_write_outputs.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     L.chiml_sim_last_error.restype = ctypes.c_char_p
     L.chiml_sim_destroy.argtypes = [ctypes.c_void_p]
     _LIB = L
@@ -76,6 +81,44 @@ def _check(rc):
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def write_detector_txt(path, times, values, corner):
+    """The reference's TXT point-detector file (parallelDTC_TXT.cpp) from
+    times[ns] and values[ns, npts] (points y, then z, then x fastest)."""
+    t, v, c = _f64(times), _f64(values), _f64(corner)
+    ns = t.size
+    npts = v.size // ns if ns else 0
+    _check(load_library().chiml_write_detector_txt(str(path).encode(), _dp(t), _dp(v), ns, npts, _dp(c)))
+
+
+def write_detector_bin(path, times, values, sz, loc):
+    """The reference's BIN point-detector file (parallelDTC_BIN.cpp)."""
+    t, v = _f64(times), _f64(values)
+    s = (ctypes.c_int * 3)(*[int(x) for x in sz])
+    lo = (ctypes.c_int * 3)(*[int(x) for x in loc])
+    _check(load_library().chiml_write_detector_bin(str(path).encode(), _dp(t), _dp(v), t.size, s, lo))
+
+
+def write_flux(path, freqs, faces, front_back=False):
+    """The reference's flux table (parallelFlux.hpp getFlux) from the
+    per-face complex fluxes faces[nfaces, nfreq] (no incident field)."""
+    f = _f64(freqs)
+    fc = np.asarray(faces, dtype=np.complex128).reshape(-1, f.size)
+    reim = _f64(np.stack([fc.real, fc.imag], axis=-1))
+    _check(load_library().chiml_write_flux(str(path).encode(), _dp(f), f.size, fc.shape[0], _dp(reim),
+                                           1 if front_back else 0))
+
+
+def write_pops(path, pops, interval, dt):
+    """The reference's population file (QEPopDtc::toFile)."""
+    p = np.asarray(pops, dtype=np.complex128)
+    reim = _f64(np.stack([p.real, p.imag], axis=-1))
+    _check(load_library().chiml_write_pops(str(path).encode(), _dp(reim), p.size, int(interval), float(dt)))
 
 
 class Detector:
@@ -199,6 +242,12 @@ class FDTDField:
         """The reference's main loop: ceil(tLim/dt) steps (src/main.cpp:54-57)."""
         self.step(self.nsteps)
         self.sync()
+
+    def toFile(self, prefix=""):
+        """Writes every output file the deck names (TXT/BIN detectors, flux
+        tables, population files) as the reference's run leaves them;
+        prefix goes in front of relative paths."""
+        _check(load_library().chiml_sim_write_outputs(self._h, str(prefix).encode()))
 
     def field(self, comp):
         if isinstance(comp, str):

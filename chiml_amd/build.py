@@ -30,7 +30,7 @@ def build_product(jobs=None):
     _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", *DEVICE_FLAGS,
           "-c", os.path.join(CSRC, "device.hip"), "-o", dev_obj])
     host_objs = []
-    for src in ("manager.cpp", "setup.cpp", "server.cpp"):
+    for src in ("manager.cpp", "setup.cpp", "server.cpp", "writers.cpp"):
         obj = os.path.join(objdir, src.replace(".cpp", ".o"))
         _run([HIPCC, "-O2", "-std=c++17", "-fPIC", "-pthread", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
               "-c", os.path.join(CSRC, src), "-o", obj])

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <filesystem>
 #include <functional>
 #include <map>
 #include <memory>
@@ -25,6 +26,7 @@
 #include <vector>
 
 #include "../../include/chiml/chiml_dev.h"
+#include "../../include/chiml/chiml_out.h"
 #include "../../include/chiml/chiml_sim.h"
 #include "setup.hpp"
 
@@ -146,6 +148,11 @@ struct FluxBox {
     std::vector<Face> faces;
     std::vector<double> freqs;
     int timeInt = 1;
+    std::string name;
+    // fluxConv_ and freqConv_ of parallelFluxDTC (parallelFlux.hpp:126-151):
+    // the deck's weight, or with SI the unit conversions
+    double conv = 1.0, freqConv = 1.0;
+    bool frontBack = false;  // the box extends in all three directions
 };
 
 }  // namespace
@@ -174,6 +181,7 @@ class FDTDField {
     size_t estimate = 0;
 
     std::vector<cplx> flux(int f);
+    std::vector<std::vector<cplx>> fluxFaces(int f);
 
   private:
     int objAt(std::array<double, 3> pt, bool ml) const;
@@ -710,6 +718,13 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         FluxBox fb;
         fb.freqs = F.freqs;
         fb.timeInt = F.timeInt;
+        fb.name = F.name;
+        fb.conv = F.weight;
+        if (F.SI) {
+            fb.freqConv = SPEED_OF_LIGHT / IP.a;
+            fb.conv = IP.I0 / IP.a * IP.I0 / (EPS0 * IP.a * SPEED_OF_LIGHT);
+        }
+        fb.frontBack = F.sz[0] > 1 && F.sz[1] > 1 && F.sz[2] > 1;
         if (!fluxes.empty() && (fluxes[0].freqs != F.freqs || fluxes[0].timeInt != F.timeInt))
             throw std::logic_error("flux boxes of one simulation must share their frequency list and time interval");
         std::array<int, 3> fsz = F.sz;
@@ -1059,7 +1074,17 @@ static cplx simpson(const cplx* f, int n, double d) {
     return r;
 }
 
+// Total over the faces in their order (flux += tempFlux, parallelFlux.hpp:522-526).
 std::vector<cplx> FDTDField::flux(int fi) {
+    auto faces = fluxFaces(fi);
+    std::vector<cplx> total(nfreq, 0.0);
+    for (auto& v : faces)
+        for (int k = 0; k < nfreq; ++k) total[k] += v[k];
+    return total;
+}
+
+// Per face and frequency: tempFlux = fluxConv_ * weight_ * simps2D(...).
+std::vector<std::vector<cplx>> FDTDField::fluxFaces(int fi) {
     FluxBox& fb = fluxes.at(fi);
     std::vector<double> F(2 * static_cast<size_t>(ndft) * nfreq);
     if (ndft > 0 && chiml_dev_read_dft(dev_, F.data(), F.size()) != CHIML_OK)
@@ -1067,8 +1092,9 @@ std::vector<cplx> FDTDField::flux(int fi) {
     int steps = 0;
     chiml_dev_steps_done(dev_, &steps);
     const double nt = steps / fb.timeInt + 1;
-    std::vector<cplx> total(nfreq, 0.0);
+    std::vector<std::vector<cplx>> out;
     for (auto& f : fb.faces) {
+        std::vector<cplx>& fv = out.emplace_back(nfreq, 0.0);
         const int n1 = f.sz[f.transCor1], n2 = f.sz[f.cor];
         auto combine = [&](std::vector<Store>& st, int pad) {
             std::vector<cplx> out;
@@ -1115,10 +1141,10 @@ std::vector<cplx> FDTDField::flux(int fi) {
             } else {
                 sres = simpson(g.data(), n1, d1);
             }
-            total[k] += f.weight * sres;
+            fv[k] = fb.conv * f.weight * sres;
         }
     }
-    return total;
+    return out;
 }
 
 // A CPML on some axes only is rejected: with a zero-thickness axis a next to
@@ -1267,6 +1293,92 @@ int chiml_sim_pops(chiml_sim_t* s, int q, int j, double* reim, size_t cap, size_
     int rc = chiml_dev_read_pops(s->f->dev_, q, j, reim, cap, ns);
     if (rc) chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
     return rc;
+}
+
+// The files the reference's run leaves (detector output calls during the
+// run, getFlux and the population detectors' toFile at the end of main(),
+// src/main.cpp:67-118), written from the device's results.  Directories are
+// created as the reference's input parser creates them
+// (parallelInputs.cpp:650-655, 680-682, 760-763).
+int chiml_sim_write_outputs(chiml_sim_t* s, const char* prefix) {
+    if (!s) return sim_fail("null simulation");
+    auto& F = *s->f;
+    const auto& IP = F.IP;
+    const std::string pre = prefix ? prefix : "";
+    auto place = [&](const std::string& name) {
+        std::filesystem::path p = name;
+        if (!pre.empty() && p.is_relative()) p = std::filesystem::path(pre) / p;
+        std::error_code ec;
+        if (p.has_parent_path()) std::filesystem::create_directories(p.parent_path(), ec);
+        if (ec) throw std::runtime_error("cannot create " + p.parent_path().string() + ": " + ec.message());
+        return p.string();
+    };
+    try {
+        const auto nv = IP.nVec();
+        for (size_t d = 0; d < IP.dtcs.size(); ++d) {
+            const auto& D = IP.dtcs[d];
+            if (D.cls != chiml::DtcClass::TXT && D.cls != chiml::DtcClass::BIN) continue;
+            size_t ns = 0;
+            int npts = 0;
+            int rc = chiml_sim_detector(s, static_cast<int>(d), nullptr, nullptr, 0, &ns, &npts);
+            if (rc) return rc;
+            std::vector<double> t(ns), v(ns * static_cast<size_t>(npts));
+            rc = chiml_sim_detector(s, static_cast<int>(d), t.data(), v.data(), ns, &ns, &npts);
+            if (rc) return rc;
+            const std::string path = place(D.name);
+            if (D.cls == chiml::DtcClass::BIN) {
+                rc = chiml_write_detector_bin(path.c_str(), t.data(), v.data(), ns, D.sz.data(), D.loc.data());
+            } else {
+                // realSpaceLoc_ (parallelDTC.hpp:58-80): the field grid holds n + 2
+                // points per axis (1 along z in 2D), and with SI the corner is
+                // scaled by a twice (dscal_, then the loop)
+                double corner[3];
+                for (int i = 0; i < 3; ++i) {
+                    const int ng = (i < 2 || F.is3D) ? nv[i] + 2 : 1;
+                    corner[i] = IP.d[i] * (D.loc[i] - (ng - 2 - ng % 2) / 2);
+                    if (D.SI) {
+                        corner[i] *= IP.a;
+                        corner[i] *= IP.a;
+                    }
+                }
+                rc = chiml_write_detector_txt(path.c_str(), t.data(), v.data(), ns, npts, corner);
+            }
+            if (rc) return rc;
+        }
+        for (size_t f = 0; f < F.fluxes.size(); ++f) {
+            const auto& fb = F.fluxes[f];
+            auto faces = F.fluxFaces(static_cast<int>(f));
+            std::vector<double> freqs(fb.freqs.size()), reim;
+            for (size_t k = 0; k < freqs.size(); ++k) freqs[k] = fb.freqConv * fb.freqs[k];
+            for (auto& fv : faces)
+                for (auto& c : fv) {
+                    reim.push_back(c.real());
+                    reim.push_back(c.imag());
+                }
+            const std::string path = place(fb.name + ".dat");
+            int rc = chiml_write_flux(path.c_str(), freqs.data(), static_cast<int>(freqs.size()),
+                                      static_cast<int>(faces.size()), reim.data(), fb.frontBack ? 1 : 0);
+            if (rc) return rc;
+        }
+        for (size_t q = 0; q < IP.qes.size(); ++q) {
+            const auto& Q = IP.qes[q];
+            for (size_t j = 0; j < Q.dtcLevs.size(); ++j) {
+                size_t ns = 0;
+                int rc = chiml_sim_pops(s, static_cast<int>(q), static_cast<int>(j), nullptr, 0, &ns);
+                if (rc) return rc;
+                std::vector<double> reim(2 * ns);
+                rc = chiml_sim_pops(s, static_cast<int>(q), static_cast<int>(j), reim.data(), reim.size(), &ns);
+                if (rc) return rc;
+                const std::string path =
+                    place(Q.popFnameBase + std::to_string(q) + "_level_" + std::to_string(Q.dtcLevs[j]) + ".dat");
+                rc = chiml_write_pops(path.c_str(), reim.data(), ns, Q.popTimeInt, IP.dt);
+                if (rc) return rc;
+            }
+        }
+        return CHIML_OK;
+    } catch (std::exception& e) {
+        return sim_fail(e.what());
+    }
 }
 
 int chiml_sim_density(chiml_sim_t* s, int q, int m, int e, double* reim, int* nlevel) {

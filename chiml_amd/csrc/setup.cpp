@@ -531,6 +531,7 @@ ProgramInputs parseInputs(const std::string& text) {
             const Value& DL = json::get_child_or_empty(it, "dtc_levs");
             for (size_t k = 0; k < DL.size(); ++k) q.dtcLevs.push_back(static_cast<int>(DL.child(k).as_num()));
             q.popTimeInt = json::get_int(it, "levDTC_timeInt", 1);
+            q.popFnameBase = json::get_str(it, "dtc_pop_fname_base", "output_data/qe_");
             // Extension: explicit Hermitian H0 (eV) and dipole matrices (Debye).
             const Value* H0 = it.path("H0_eV");
             if (H0) q.h0Full = readCplxMatrix(*H0, q.nlevel, e_conv);
@@ -609,6 +610,10 @@ ProgramInputs parseInputs(const std::string& text) {
         }
         F.SI = json::get_bool(it, "SI", false);
         F.crossSec = json::get_bool(it, "cross_sec", false);
+        // save / load write and read the frequency fields of a TFSF
+        // normalisation run (parallelFlux.hpp:616-722)
+        if (json::get_bool(it, "save", false) || json::get_bool(it, "load", false))
+            throw std::logic_error("flux save/load (frequency fields of a TFSF normalisation run) are not supported on this path");
         P.fluxes.push_back(F);
     }
     for (auto& o : P.objs) o->setUpConsts(P.dt);

@@ -121,6 +121,7 @@ struct QESpec {
     std::vector<RelaxSpec> relax;
     std::vector<int> dtcLevs;
     int popTimeInt = 1;
+    std::string popFnameBase = "output_data/qe_";  // dtc_pop_fname_base
     std::vector<std::array<int, 3>> locs;  // grid points (0-based)
     // Extension (not in the reference): full Hermitian H0 per member and
     // arbitrary dipole matrices, flat column-major like the reference's

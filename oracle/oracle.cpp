@@ -959,6 +959,9 @@ void Sim::buildFlux() {
         Flux fx;
         fx.weight = F.weight;
         fx.timeInt = F.timeInt;
+        // fluxConv_ is the deck's weight, replaced by the unit conversion with
+        // SI (parallelFlux.hpp:126, 146-151)
+        fx.fluxConv = F.SI ? IP.I0 / IP.a * IP.I0 / (EPS0 * IP.a * SPEED_OF_LIGHT) : F.weight;
         fx.freqs = F.freqs;
         fx.loc = F.loc;
         fx.sz = F.sz;

@@ -80,6 +80,10 @@ def test_estimate_monotone():
      "chiral"),
     (lambda d: d["ObjectList"].append({"shape": "sphere", "material": "unobtainium", "eps": 2.0, "pols": [],
                                        "radius": 0.2, "loc": [0, 0, 0], "Basis_Set": []}), "not supported"),
+    (lambda d: d["FluxList"].append({"name": "f", "size": [0.4, 0.4, 0], "loc": [0, 0, 0], "fcen": 1.0,
+                                     "fwidth": 0.5, "nfreq": 3, "save": True}), "save/load"),
+    (lambda d: d["FluxList"].append({"name": "f", "size": [0.4, 0.4, 0], "loc": [0, 0, 0], "fcen": 1.0,
+                                     "fwidth": 0.5, "nfreq": 3, "load": True}), "save/load"),
 ])
 def test_deck_errors(mutate, message):
     deck = _base()

@@ -433,7 +433,7 @@ def test_population_detector_reads_flat_index():
 
 
 # ------------------------------------------------------------- flux --------
-def _flux_run(boxes, src_off, n=36, pml=6, d=0.02, steps=400, amp=1.0):
+def _flux_run(boxes, src_off, n=36, pml=6, d=0.02, steps=400, amp=1.0, extra=None, dets=None):
     deck = decks.empty_deck((n, n, n), d, steps, pml)
     fcen = 1.0 / (10 * d)
     deck["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "loc": [src_off[0] * d, src_off[1] * d, src_off[2] * d],
@@ -441,10 +441,16 @@ def _flux_run(boxes, src_off, n=36, pml=6, d=0.02, steps=400, amp=1.0):
     for k, (cells, off) in enumerate(boxes):
         s = (cells - 1) * d
         deck["FluxList"].append({"name": f"f{k}", "size": [s, s, s], "loc": [off[0] * d, off[1] * d, off[2] * d],
-                                 "lamL": 7 * d, "lamR": 16 * d, "nfreq": 6})
+                                 "lamL": 7 * d, "lamR": 16 * d, "nfreq": 6, **((extra or [{}] * len(boxes))[k])})
+    for t, si in dets or []:
+        deck["DetectorList"].append({"type": t, "dtc_class": "cout", "fname": "d", "SI": si, "size": [0, 0, 0],
+                                     "loc": [2 * d, d, 0.0]})
     o = Oracle(json.dumps(deck))
     o.step(steps)
-    return [o.flux(k) for k in range(len(boxes))]
+    out = [o.flux(k) for k in range(len(boxes))]
+    if dets:
+        out.append([o.detector(k)[1][:, 0] for k in range(len(dets))])
+    return out
 
 
 def test_flux_box_invariants():
@@ -459,6 +465,27 @@ def test_flux_box_invariants():
     assert np.max(np.abs(fc / fb)) < 0.15
     (f2,) = _flux_run([(11, (0, 0, 0))], (-1, 0, 0), amp=2.0)
     assert np.allclose(f2, 4.0 * fa, rtol=1e-12, atol=0.0)
+
+
+def test_flux_weight_once_and_si_scale_is_product_of_field_scales():
+    """fluxConv_ (parallelFlux.hpp:126, 146-151).  Without SI the deck's
+    weight multiplies the flux once: weight 2 gives exactly twice the weight-1
+    flux (applied twice it would give four times).  With SI the weight is
+    replaced, not multiplied: SI with weight 3 equals SI with weight 1.  And
+    since the flux is bilinear in E and H, its SI factor is the product of the
+    SI factors the point detectors apply to E and to H (a second code path)."""
+    box = [(9, (0, 0, 0))] * 4
+    extra = [{}, {"weight": 2.0}, {"SI": True}, {"SI": True, "weight": 3.0}]
+    f1, f2, fs, fs3, (ez, ez_si, hx, hx_si) = _flux_run(
+        box, (0, 0, 0), n=24, pml=5, steps=160, extra=extra,
+        dets=[("Ez", False), ("Ez", True), ("Hx", False), ("Hx", True)])
+    assert np.abs(f1).max() > 0
+    assert np.array_equal(f2, 2.0 * f1)
+    assert np.array_equal(fs3, fs)
+    k = np.argmax(np.abs(ez))
+    j = np.argmax(np.abs(hx))
+    scale = (ez_si[k] / ez[k]) * (hx_si[j] / hx[j])
+    assert np.allclose(fs, scale * f1, rtol=1e-12, atol=0.0)
 
 
 # -------------------------------------------------------- detectors --------
