@@ -48,14 +48,21 @@
 //                                     by quadrature over the sphere (scipy Y_lm)
 //   QEPopDtc flat index               pops equal rho[0], rho[3] (trap T3)
 //   flux                              box-size independence, ~0 for an outside
-//                                     source, quadratic in amplitude (loose)
+//                                     source, quadratic in amplitude (loose);
+//                                     the weight applied once, SI factor = the
+//                                     product of the detectors' E and H factors
+//   Obj::isObj + setupPhysFields      eps_c read back through one E update equals
+//                                     exact geometry at the Yee samples of Ex, Ey,
+//                                     Ez for an off-centre sphere (odd and even
+//                                     grid), cylinder and ellipsoid
+//   detectMatInPML                    interior / upper-slab / near-slab cases and
+//                                     the lower slab probed on its corner line only
 // Parity unpinned (no closed form; the restatement follows the reference
 // line by line and only the golden fixtures freeze it):
 //   getPsiUpList / getGridUpList corner ownership and end trims and genEtaEff
 //   plane averages in material-filled PMLs, beyond the reflection bound;
-//   detectMatInPML's scan; Obj::isObj for sphere / cylinder / ellipsoid
-//   rasterisation; flux makeParamIn / combineField offset averaging and the
-//   Simpson weights beyond the invariants.
+//   block rasterisation under a rotated frame; flux makeParamIn / combineField
+//   offset averaging and the Simpson weights beyond the invariants.
 #include <algorithm>
 #include <cmath>
 #include <complex>

@@ -574,3 +574,134 @@ def test_emitter_slab_linear_response_equals_lorentz_slab():
     assert effect > 0.01 * np.abs(v_v).max()  # the slab matters
     assert np.abs(v_ml - v_l).max() <= 0.05 * effect, (np.abs(v_ml - v_l).max(), effect)
 
+
+
+# ------------------------------------------------------ rasterisation ------
+def _eps_maps(objects, n, d=0.05):
+    """eps seen by Ex, Ey, Ez at every sample, read back through one E update:
+    with E = 0 and H set so that every curl component is 1 (Hz = (j+1) d,
+    Hx = (k+1) d, Hy = (i+1) d), one step leaves E_c = dt / eps_c."""
+    deck = decks.empty_deck((n, n, n), d, 2, 0)
+    deck["ObjectList"] += objects
+    o = Oracle(json.dumps(deck))
+    k, j, i = np.meshgrid(*(np.arange(s) for s in o.shape), indexing="ij")
+    o.set_field(3, (k + 1.0) * d)
+    o.set_field(4, (i + 1.0) * d)
+    o.set_field(5, (j + 1.0) * d)
+    o.step(1)
+    with np.errstate(divide="ignore"):  # edge samples without a full stencil stay 0
+        return [o.dt / o.field(c) for c in range(3)], (i, j, k)
+
+
+def _expected(inside, eps_in, n, d, ijk):
+    """eps_c = eps_in[c] where the Yee sample of component c (its own half-cell
+    offset along c, lattice centred at (n - n % 2) / 2) lies in the shape."""
+    i, j, k = ijk
+    c0 = (n - n % 2) / 2.0
+    out, margins = [], []
+    for c in range(3):
+        off = [0.0, 0.0, 0.0]
+        off[c] = 0.5
+        p = np.stack([(i + off[0] - c0) * d, (j + off[1] - c0) * d, (k + off[2] - c0) * d])
+        ins, margin = inside(p)
+        out.append(np.where(ins, eps_in[c], 1.0))
+        margins.append(margin)
+    return out, min(margins)
+
+
+def _check_raster(objects, inside, eps_in, n=23, d=0.05):
+    got, ijk = _eps_maps(objects, n, d)
+    want, margin = _expected(inside, eps_in, n, d, ijk)
+    assert margin > 1e-4 * d  # no sample within rounding of the surface
+    sl = (slice(1, n - 1),) * 3  # the outermost samples have no full stencil
+    for c in range(3):
+        assert np.count_nonzero(want[c][sl] != 1.0) > 50, c
+        assert np.allclose(got[c][sl], want[c][sl], rtol=1e-12, atol=0.0), c
+
+
+def _obj(shape, eps_diag, loc, **geo):
+    o = {"shape": shape, "material": "custom", "eps": eps_diag[0], "eps_diag": list(eps_diag), "pols": [],
+         "loc": list(loc), "Basis_Set": [],
+         "unit_vectors": [{"uvec": [1, 0, 0]}, {"uvec": [0, 1, 0]}, {"uvec": [0, 0, 1]}]}
+    o.update(geo)
+    return o
+
+
+@pytest.mark.parametrize("n", [23, 24])
+def test_sphere_rasterisation_is_the_yee_samples_inside_the_ball(n):
+    """Obj::isObj for a sphere (src/OBJECTS/Obj.cpp:373-380) as placed by
+    setupPhysFields (parallelFDTDField.hpp:867-947): eps_c applies exactly at
+    the Ex/Ey/Ez sample points within the radius, for an off-centre ball with
+    a radius between lattice distances, on an odd and an even grid (the even
+    grid puts the lattice centre half a cell off the cell's middle)."""
+    d = 0.05
+    loc, r = np.array([0.7, -1.3, 0.4]) * d, 5.37 * d
+    eps = (2.0, 3.0, 4.0)
+
+    def inside(p):
+        dist = np.sqrt(sum((p[a] - loc[a]) ** 2 for a in range(3)))
+        return dist <= r, float(np.min(np.abs(dist - r)))
+
+    _check_raster([_obj("sphere", eps, loc, radius=r)], inside, eps, n, d)
+
+
+def test_cylinder_rasterisation():
+    """Cylinder along the object y axis (Obj.cpp:523-533): radius in object
+    x-z, half length along y."""
+    d = 0.05
+    loc, r, length = np.array([-0.6, 0.3, 1.2]) * d, 4.41 * d, 11.3 * d
+    eps = (2.5, 1.5, 3.5)
+
+    def inside(p):
+        t = [p[a] - loc[a] for a in range(3)]
+        rad = np.sqrt(t[0] ** 2 + t[2] ** 2)
+        ins = (rad <= r) & (np.abs(t[1]) <= length / 2)
+        margin = min(float(np.min(np.abs(rad - r))), float(np.min(np.abs(np.abs(t[1]) - length / 2))))
+        return ins, margin
+
+    _check_raster([_obj("cylinder", eps, loc, radius=r, length=length)], inside, eps, 23, d)
+
+
+def test_ellipsoid_rasterisation():
+    """Ellipsoid with full axes `size` (Obj.cpp:437-457, no cuts): the samples
+    with sum (t_i / (size_i / 2))^2 <= 1."""
+    d = 0.05
+    loc, size = np.array([0.35, -0.8, 0.6]) * d, np.array([15.3, 9.7, 12.1]) * d
+    eps = (4.0, 2.0, 3.0)
+
+    def inside(p):
+        s = sum(((p[a] - loc[a]) / (size[a] / 2)) ** 2 for a in range(3))
+        return s <= 1.0, float(np.min(np.abs(s - 1.0)))
+
+    _check_raster([_obj("ellipsoid", eps, loc, size=list(size))], inside, eps, 23, d)
+
+
+@pytest.mark.parametrize("where,expect", [
+    ("interior", False), ("high_x", True), ("high_y", True), ("high_z", True), ("edge_of_interior", False),
+    ("low_x_corner", True), ("low_x_off_corner", False),
+])
+def test_detect_mat_in_pml(where, expect):
+    """detectMatInPML (parallelFDTDField.hpp:274-370) over a 25^3 cell with a
+    5-cell CPML: a dielectric inside the interior, or stopping a cell short of
+    a slab, leaves the flag unset; one inside an upper slab of any axis sets
+    it.  The lower slabs are scanned only along the first (jj, kk) line, the
+    domain's corner edge, because the loop overwrites the lower range with the
+    upper one after that line: a block in the low-x slab is seen at the corner
+    and missed away from it (its mirror image in the high-x slab is seen)."""
+    n, d, pml = 25, 0.05, 5
+    half = (n - 1) * d / 2
+    size = [2 * d, 4 * d, 4 * d]
+    edge = half - 1.5 * d
+    loc = {"interior": [0.0, 0.0, 0.0],
+           "high_x": [edge, 0.0, 0.0],
+           "high_y": [0.0, edge, 0.0],
+           "high_z": [0.0, 0.0, edge],
+           "edge_of_interior": [-half + (pml + 2) * d, 0.0, 0.0],
+           "low_x_corner": [-edge, -edge, -edge],
+           "low_x_off_corner": [-edge, 0.0, 0.0]}[where]
+    if where in ("high_y", "high_z"):
+        size = [4 * d, 2 * d, 2 * d] if where == "high_y" else [4 * d, 4 * d, 2 * d]
+    deck = decks.empty_deck((n, n, n), d, 2, pml)
+    deck["ObjectList"].append({"shape": "block", "material": "custom", "eps": 3.0, "pols": [], "size": size,
+                               "loc": loc, "Basis_Set": []})
+    assert Oracle(json.dumps(deck)).dielectric_mat_in_pml is expect
