@@ -289,6 +289,38 @@ def _cpml_run(n, pml, steps, det_offset):
     return o.detector(0)[1][:, 0]
 
 
+def _corner_run(n, pml, steps, src, det):
+    d = 0.05
+    deck = decks.empty_deck((n, n, 1), d, steps, pml)
+    fcen = 1.0 / (20 * d)
+    deck["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "loc": [src[0] * d, src[1] * d, 0.0],
+                               "PulseList": [decks.gaussian_pulse(fcen, 2.0 * fcen, 4.0, 1.0)]})
+    for det_cells in det:
+        deck["DetectorList"].append({"type": "Ez", "dtc_class": "cout", "fname": "d", "size": [0, 0, 0],
+                                     "loc": [det_cells[0] * d, det_cells[1] * d, 0.0]})
+    o = Oracle(json.dumps(deck))
+    o.step(steps)
+    return np.stack([o.detector(k)[1][:, 0] for k in range(len(det))])
+
+
+def test_cpml_corner_absorbs_like_the_faces():
+    """CPML corner ownership (getPsiUpList / getGridUpList, parallelPML.hpp:321-668):
+    a source and detectors six cells from both the x and the y slab of a
+    101^2 cell with a 20-cell CPML see what a 281^2 cell shows before any of
+    its reflections return.  Where both slabs overlap every psi term must be
+    present; one missing or owned twice in the corner reflects at the 1e-1
+    level (the bare PEC wall: 0.74).  Measured: 3.7e-6 (20 cells), 3.2e-5
+    (10 cells)."""
+    steps = 450
+    src, det = (24, 24), [(22, 26), (26, 22), (19, 19)]
+    small = _corner_run(101, 20, steps, src, det)
+    big = _corner_run(281, 20, steps, src, det)
+    ref = np.max(np.abs(big))
+    assert ref > 0
+    err = np.max(np.abs(small - big)) / ref
+    assert err < 1e-4, err
+
+
 def test_cpml_reflection_is_small():
     """Point source, detector 13 cells from a 20-cell CPML: the signal matches a
     domain large enough that no reflection returns within the window."""
@@ -705,3 +737,27 @@ def test_detect_mat_in_pml(where, expect):
     deck["ObjectList"].append({"shape": "block", "material": "custom", "eps": 3.0, "pols": [], "size": size,
                                "loc": loc, "Basis_Set": []})
     assert Oracle(json.dumps(deck)).dielectric_mat_in_pml is expect
+
+
+def test_rotated_block_rasterisation():
+    """Block in an object frame rotated by 30 degrees about z (unit_vectors,
+    RealSpace2ObjectSpace Obj.cpp:13-21, Obj.cpp:400-410): the samples whose
+    object coordinates t_r = u_r . (p - loc) satisfy |t_r| <= size_r / 2."""
+    d = 0.05
+    loc, size = np.array([0.4, -0.9, 0.3]) * d, np.array([13.3, 6.7, 9.1]) * d
+    c30, s30 = np.cos(np.pi / 6), np.sin(np.pi / 6)
+    u = np.array([[c30, s30, 0.0], [-s30, c30, 0.0], [0.0, 0.0, 1.0]])
+    eps = (3.0, 2.0, 2.5)
+
+    def inside(p):
+        t = [sum(u[r][a] * (p[a] - loc[a]) for a in range(3)) for r in range(3)]
+        ins = np.ones_like(t[0], dtype=bool)
+        margin = np.inf
+        for r in range(3):
+            ins &= np.abs(t[r]) <= size[r] / 2
+            margin = min(margin, float(np.min(np.abs(np.abs(t[r]) - size[r] / 2))))
+        return ins, margin
+
+    obj = _obj("block", eps, loc, size=list(size))
+    obj["unit_vectors"] = [{"uvec": list(row)} for row in u]
+    _check_raster([obj], inside, eps, 23, d)
