@@ -31,7 +31,9 @@
 //   CPML as a whole                   reflection < 2e-3 for 20 cells; PEC wall
 //                                     without it reflects > 0.1; at a corner
 //                                     (both slabs overlapping) < 1e-4 against a
-//                                     large domain (3.7e-6 measured, PEC 0.74)
+//                                     large domain (3.7e-6 measured, PEC 0.74);
+//                                     a 3D eps = 4 filled cell absorbs < 1e-4
+//                                     (genEtaEff), the 2D one is never flagged
 //   pulses                            Gaussian envelope / turn-off, default t_0,
 //                                     Ricker identically zero (trap T1)
 //   grid / time step                  n = floor(size/d + 0.5) + 1, 2D dt with the
@@ -63,7 +65,7 @@
 // Parity unpinned (no closed form; the restatement follows the reference
 // line by line and only the golden fixtures freeze it):
 //   getPsiUpList / getGridUpList end trims and genEtaEff plane averages in
-//   material-filled PMLs, beyond the face and corner absorption bounds; flux
+//   partly filled PMLs, beyond the absorption bounds; flux
 //   makeParamIn / combineField offset averaging and the Simpson weights beyond
 //   the invariants.
 #include <algorithm>
