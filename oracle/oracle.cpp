@@ -54,7 +54,10 @@
 //   flux                              box-size independence, ~0 for an outside
 //                                     source, quadratic in amplitude (loose);
 //                                     the weight applied once, SI factor = the
-//                                     product of the detectors' E and H factors
+//                                     product of the detectors' E and H factors;
+//                                     a uniform static E, H through an x or z plane
+//                                     gives s e0 h0 A |sum exp(i w k dt)|^2 / nt^2
+//                                     (sign, sample count, nt = K + 1, area)
 //   Obj::isObj + setupPhysFields      eps_c read back through one E update equals
 //                                     exact geometry at the Yee samples of Ex, Ey,
 //                                     Ez for an off-centre sphere (odd and even
@@ -66,8 +69,8 @@
 // line by line and only the golden fixtures freeze it):
 //   getPsiUpList / getGridUpList end trims and genEtaEff plane averages in
 //   partly filled PMLs, beyond the absorption bounds; flux
-//   makeParamIn / combineField offset averaging and the Simpson weights beyond
-//   the invariants.
+//   makeParamIn / combineField face-centre offset averaging (a uniform field
+//   cannot see it) beyond the box-size invariance.
 #include <algorithm>
 #include <cmath>
 #include <complex>

@@ -800,3 +800,35 @@ def test_material_in_pml_never_detected_in_2d():
     big, f_big = _filled_run((281, 281, 1), 20, 700, 4.0, (18, 9, 0))
     assert not f_small and not f_big
     assert np.max(np.abs(small - big)) / np.max(np.abs(big)) > 0.1
+
+
+@pytest.mark.parametrize("axis", ["x", "z"])
+def test_flux_of_a_static_field_is_the_dirichlet_kernel(axis):
+    """Flux through one plane (a one-face box) of a uniform static field, set
+    after the constructor's t = 0 sample (parallelFDTDField.cpp:837, fields
+    still zero there): away from the walls the field stays constant for the
+    whole run, so with K = 40 steps and nt = K + 1 calls (getFlux,
+    parallelFlux.hpp:418-531)
+        flux = s * e0 * h0 * A * |sum_{k=1..K} exp(i w k dt)|^2 / nt^2,
+    s = -1 for Ez, Hy through an x plane (Ey Hz* - Ez Hy*) and +1 for Ex, Hy
+    through a z plane (Ex Hy* - Ey Hx*), with A the exact area (Simpson is
+    exact on a constant) and no phase between the E and H samples.  A sign,
+    sample-count, normalisation or area mistake fails it."""
+    n, d, steps = 61, 0.05, 40
+    deck = decks.empty_deck((n, n, n), d, steps, 0)
+    size = [0.0, 10 * d, 10 * d] if axis == "x" else [10 * d, 10 * d, 0.0]
+    fc = 1.0 / (10 * d)
+    deck["FluxList"].append({"name": "p", "size": size, "loc": [0.0, 0.0, 0.0], "fcen": fc, "fwidth": fc,
+                             "nfreq": 5})
+    o = Oracle(json.dumps(deck))
+    e0, h0 = 0.7, -0.3
+    o.set_field(2 if axis == "x" else 0, np.full(o.shape, e0))
+    o.set_field(4, np.full(o.shape, h0))
+    o.step(steps)
+    got = o.flux(0)
+    w = 2 * np.pi * (fc - fc / 2 + np.arange(5) * fc / 4)
+    kern = np.abs(np.exp(1j * np.outer(w, np.arange(1, steps + 1)) * o.dt).sum(axis=1)) ** 2
+    sign = -1.0 if axis == "x" else 1.0
+    want = sign * e0 * h0 * (10 * d) ** 2 * kern / (steps + 1) ** 2
+    assert np.allclose(got.real, want, rtol=1e-10, atol=0.0)
+    assert np.max(np.abs(got.imag)) <= 1e-12 * np.max(np.abs(want))
