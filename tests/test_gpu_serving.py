@@ -28,6 +28,17 @@ def _mix():
                                 precision="fp64"))
     out.append(decks.config1_2d(n=64, steps=75, pml=10, cyl_radius_cells=9, src_cell=(20, 32), ndet=3,
                                 precision="fp64"))
+    # an odd-size grid: the generic per-cell kernels instead of the vector ones
+    out.append(decks.config1_2d(n=45, steps=64, pml=6, cyl_radius_cells=5, src_cell=(14, 22), ndet=2,
+                                precision="fp32"))
+    # N-level emitters in 3D: the team density kernel
+    d = decks.empty_deck((24, 24, 24), 0.02, 60, 4, precision="fp32")
+    d["ObjectList"].append(decks.n_level_emitter(4, [0.04, 0.04, 0.04], (0.01, 0.01, 0.01), seed=44))
+    fc = decks.ev2fdtd(2.0)
+    d["SourceList"].append({"pol": "Ez", "size": [0.0, 0.0, 0.0], "loc": [-0.09, 0.01, 0.01],
+                            "PulseList": [{"profile": "gaussian", "fcen": fc, "fwidth": fc, "cutoff": 3.0, "t_0": 0.3,
+                                           "Field_Intensity": 1e10}]})
+    out.append(json.dumps(d))
     return out
 
 
