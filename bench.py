@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--n", type=int, default=256, help="points per axis")
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--cpu-steps", type=int, default=24)  # about 12 s of one core on 256^3
     p.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     p.add_argument("--sweep-requests", type=int, default=32,
                    help="requests in the serving-sweep sample (0 skips the sweep leg)")
@@ -280,7 +280,7 @@ def cpu_baseline(n, steps):
         "cores": 1,
         "kind": "port",
         "sample": f"FP64 oracle (reference step loop restated, BLAS-1 per x-run), same {n}^3 deck, "
-                  f"{steps} timed steps after 1 untimed step; setup {setup:.1f} s excluded",
+                  f"{steps} timed steps ({t2 - t1:.1f} s) after 1 untimed step; setup {setup:.1f} s excluded",
     }
 
 
