@@ -8,7 +8,7 @@
 //                    (addPul, updateE/updateD, updateEPML, D2E)
 //                    3D FP32: k_h3_all / k_e3_all; 3D FP64: k_h3_int + k_h3_slab /
 //                    k_e3_int + k_e3_slab; 2D: k_h2v / k_e2v; else k_update_h / k_update_e
-//   3. emitters    : k_ml_addp then k_ml_density / k_ml_team and k_ml_pop_finish
+//   3. emitters    : k_ml_addp then k_ml_team and k_ml_pop_finish
 //                    (parallelQEBase::addQE)
 //   4. t_step += 1 : plain launches are given the step number by the host;
 //                    inside captured graphs the H launch advances a two-slot
@@ -1301,163 +1301,6 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 
-// d rho/dt = (i/hbar)(H rho - rho H) + sum_j Gamma[i][j] rho_j, column-major
-// (denDeriv, src/ML/parallelQE.hpp:727-744).
-template <int N>
-__device__ __forceinline__ void den_deriv(const QeDev& q, const double2* H, const double2* r, double2* out) {
-    const double ih = 1.0 / q.hbar;
-#pragma unroll
-    for (int j = 0; j < N; ++j)
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int k = 0; k < N; ++k) {
-                s1 = cadd(s1, cmul(H[i + k * N], r[k + j * N]));
-                s2 = cadd(s2, cmul(r[i + k * N], H[k + j * N]));
-            }
-            // (i/hbar) s1 - (i/hbar) s2
-            double dr = s1.x - s2.x, di = s1.y - s2.y;
-            out[i + j * N] = make_double2(-di * ih, dr * ih);
-        }
-    for (int g = 0; g < q.ngam; ++g) {
-        const int row = q.gam_row[g], cl = q.gam_col[g];
-        const double w = q.gam_val[g];
-        double2 v = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int k = 0; k < N * N; ++k)
-            if (cl == k) v = r[k];
-#pragma unroll
-        for (int k = 0; k < N * N; ++k)
-            if (row == k) {
-                out[k].x += v.x * w;
-                out[k].y += v.y * w;
-            }
-    }
-}
-
-// One thread per emitter: every broadening member advanced with the
-// Adams-Bashforth-Moulton predictor-corrector (PCABAM4, parallelQE.hpp:751-770),
-// cold-started from zero history; the derivative history is a 4-slot ring
-// instead of the reference's three copies per step (density.hpp:65-70).
-template <typename T, int N>
-__global__ void __launch_bounds__(128) k_ml_density(QeDev q, const T* E0, const T* E1, const T* E2, int nx, int ny, int nz,
-                                                    const int* stepp, int stepv) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    const int step = step_now(stepp, stepv);  // number of earlier density updates
-    constexpr int S = N * N;
-    __shared__ double2 red[128];
-    double2 pop[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-    if (e < q.nemit) {
-        const int px = q.xyz[3 * e], py = q.xyz[3 * e + 1], pz = q.xyz[3 * e + 2];
-        double ef[3] = {0.0, 0.0, 0.0};
-        const T* Es[3] = {E0, E1, E2};
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (!Es[c]) continue;
-            long long i0 = px + static_cast<long long>(nx) * (py + static_cast<long long>(ny) * pz);
-            if (q.dim == 2) ef[c] = static_cast<double>(Es[c][i0]);
-            else {
-                // mean of the two Yee edges E_c(p) and E_c(p - e_c) (getE_TE)
-                const int qx = px - (c == 0), qy = py - (c == 1), qz = pz - (c == 2);
-                double v1 = (qx >= 0 && qy >= 0 && qz >= 0) ? static_cast<double>(Es[c][qx + static_cast<long long>(nx) * (qy + static_cast<long long>(ny) * qz)]) : 0.0;
-                ef[c] = 0.5 * static_cast<double>(Es[c][i0]) + 0.5 * v1;
-            }
-        }
-        // Ring slots: the derivative from update j lives in slot j & 3, so at
-        // update u: f_n -> (u-1)&3, f_{n-1} -> (u-2)&3, f_{n-2} -> (u-3)&3,
-        // f_{n-3} -> u&3, and the new f_n overwrites slot u&3.
-        const int u = step & 3;
-        const int s0 = (u + 3) & 3, s1 = (u + 2) & 3, s2 = (u + 1) & 3, s3 = u;
-        const long long ld = static_cast<long long>(q.nmem) * q.nemit;
-        double P[3] = {0.0, 0.0, 0.0};
-        const double dt = q.dt;
-        for (int m = 0; m < q.nmem; ++m) {
-            const long long col = static_cast<long long>(m) * q.nemit + e;
-            double2 H[S], pred[S], r[S], fp[S];
-            // H = h0 - sum_c E_c mu_c (Hamiltonian::getHam)
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                double2 h = q.h0[m * S + k];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    double2 mu = q.mu[c * S + k];
-                    h.x -= ef[c] * mu.x;
-                    h.y -= ef[c] * mu.y;
-                }
-                H[k] = h;
-            }
-            // Predictor rho + dt/24 (55 f_n - 59 f_{n-1} + 37 f_{n-2} - 9 f_{n-3})
-            // and the f_pred-free part of the corrector
-            // rho + dt/24 (19 f_n - 5 f_{n-1} + f_{n-2}).
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                double2 rk = q.rho[k * ld + col];
-                double2 f0 = q.hist[s0 * S * ld + k * ld + col];
-                double2 f1 = q.hist[s1 * S * ld + k * ld + col];
-                double2 f2 = q.hist[s2 * S * ld + k * ld + col];
-                double2 f3 = q.hist[s3 * S * ld + k * ld + col];
-                double2 p = rk;
-                p = cadd(p, cscale(f0, 55.0 * dt / 24.0));
-                p = cadd(p, cscale(f1, -59.0 * dt / 24.0));
-                p = cadd(p, cscale(f2, 37.0 * dt / 24.0));
-                p = cadd(p, cscale(f3, -9.0 * dt / 24.0));
-                pred[k] = p;
-                double2 cr = rk;
-                cr = cadd(cr, cscale(f0, 19.0 * dt / 24.0));
-                cr = cadd(cr, cscale(f1, -5.0 * dt / 24.0));
-                cr = cadd(cr, cscale(f2, dt / 24.0));
-                r[k] = cr;
-            }
-            den_deriv<N>(q, H, pred, fp);
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                r[k] = cadd(r[k], cscale(fp[k], 9.0 * dt / 24.0));
-                q.rho[k * ld + col] = r[k];
-            }
-            den_deriv<N>(q, H, r, fp);
-#pragma unroll
-            for (int k = 0; k < S; ++k) q.hist[s3 * S * ld + k * ld + col] = fp[k];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (j >= q.npop) break;
-                const int kf = q.pop_flat[j];
-#pragma unroll
-                for (int k = 0; k < S; ++k)
-                    if (kf == k) pop[j] = cadd(pop[j], r[k]);
-            }
-            // updateQEPol: P_c += na Re(sum conj(rho_k) mu_k)
-            for (int c = 0; c < 3; ++c) {
-                if (!Es[c]) continue;
-                double s = 0.0;
-    #pragma unroll
-            for (int k = 0; k < S; ++k) {
-                    double2 mu = q.mu[c * S + k];
-                    s += r[k].x * mu.x + r[k].y * mu.y;
-                }
-                P[c] += q.na * s;
-            }
-        }
-        const int bx = px - q.bmin[0] + 1, by = py - q.bmin[1] + 1, bz = (q.dim == 3) ? pz - q.bmin[2] + 1 : 0;
-        const int PX = q.bn[0] + 2, PY = q.bn[1] + 2;
-        for (int c = 0; c < 3; ++c)
-            if (Es[c]) q.Pbox[c][bx + PX * (by + PY * bz)] = P[c];
-    }
-    // deterministic per-block population partial sums
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (j >= q.npop) break;
-        red[threadIdx.x] = pop[j];
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) q.pop_partial[j * gridDim.x + blockIdx.x] = red[0];
-        __syncthreads();
-    }
-}
-
 // Column j of d rho/dt for the matrices H and X held in LDS (column-major):
 // (i/hbar)(H X - X H)[:, j] + the Lindblad rows that fall in column j.
 // grc / gv / gcp: the Lindblad entries (packed row | col << 16, value) and
@@ -1496,9 +1339,13 @@ __device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* H, 
     }
 }
 
-// N >= 3: a team of N lanes per emitter, lane j owning column j of rho, its
-// history and its derivative (the same arithmetic as k_ml_density, without
-// the N^2 register arrays per lane that spill from N = 3 on).  Lanes are
+// Every emitter group (N >= 2): a team of N lanes per emitter, lane j owning
+// column j of rho, its history and its derivative, advanced by the
+// Adams-Bashforth-Moulton predictor-corrector (PCABAM4, parallelQE.hpp:751-770)
+// cold-started from zero history; the derivative history is a 4-slot ring
+// instead of the reference's three copies per step (density.hpp:65-70).
+// One thread per emitter would hold N^2 complex words per lane and spill
+// from N = 3 on; for N = 2 it leaves too few waves to hide latency.  Lanes are
 // grouped by column, lane = j * EB + team, so the lanes of one column hold
 // consecutive emitters and every rho / history access of a wave is a few
 // contiguous runs.  H = h0 - E.mu (formed once per member, each lane its
@@ -2003,8 +1850,8 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         if (qi.npop > 4) throw std::runtime_error("at most 4 population detectors per emitter group");
         q.pop_flat = c->upload<int>(qi.pop_flat, std::max(qi.npop, 1));
         q.pop_interval = qi.pop_interval > 0 ? qi.pop_interval : 1;
-        // emitters per 128-thread block: one per thread for N = 2, one team of N lanes otherwise
-        const int per_block = q.N < 3 ? 128 : kMlThreads / q.N;
+        // emitters per block of kMlThreads lanes
+        const int per_block = kMlThreads / q.N;  // one team of N lanes per emitter
         qh.nblocks = (q.nemit + per_block - 1) / per_block;
         q.pop_partial = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(qh.nblocks)));
         q.pop_cap = p->nsteps / q.pop_interval + 1;
@@ -2184,18 +2031,15 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
         {
             Timed t(c, "ml_density");
             switch (qh.N) {
-#define CHIML_ML_CASE(NN) \
-    case NN: hipLaunchKernelGGL((k_ml_density<T, NN>), g, dim3(128), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step, a.stepv); break;
 #define CHIML_ML_TEAM(NN) \
     case NN: hipLaunchKernelGGL((k_ml_team<T, NN>), g, dim3(kMlThreads), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step, a.stepv); break;
-                CHIML_ML_CASE(2)
+                CHIML_ML_TEAM(2)
                 CHIML_ML_TEAM(3)
                 CHIML_ML_TEAM(4)
                 CHIML_ML_TEAM(5)
                 CHIML_ML_TEAM(6)
                 CHIML_ML_TEAM(7)
                 CHIML_ML_TEAM(8)
-#undef CHIML_ML_CASE
 #undef CHIML_ML_TEAM
             }
         }
