@@ -141,18 +141,22 @@ def test_3d_split_kernels(variant, precision, tol_f, tol_flux):
     assert err <= tol_flux
 
 
-@pytest.mark.parametrize("nlev", [3, 4, 6])
-def test_3d_n_level_emitter_block_fp64(nlev):
-    """A 3x3x3 block of N-level emitters (seeded Hermitian H0 and dipoles)
-    driven by a point source: fields, populations and one density matrix."""
-    d = decks.empty_deck((20, 20, 20), 0.02, 90, 4, precision="fp64")
+def _n_level_block(nlev, precision):
+    d = decks.empty_deck((20, 20, 20), 0.02, 90, 4, precision=precision)
     em = decks.n_level_emitter(nlev, [0.04, 0.04, 0.04], (0.01, 0.01, 0.01), seed=40 + nlev)
     d["ObjectList"].append(em)
     fc = decks.ev2fdtd(2.0)
     d["SourceList"].append({"pol": "Ez", "size": [0.0, 0.0, 0.0], "loc": [-0.09, 0.01, 0.01],
                             "PulseList": [{"profile": "gaussian", "fcen": fc, "fwidth": fc, "cutoff": 3.0,
                                            "t_0": 0.3, "Field_Intensity": 1e10}]})
-    deck = json.dumps(d)
+    return json.dumps(d)
+
+
+@pytest.mark.parametrize("nlev", [3, 4, 6])
+def test_3d_n_level_emitter_block_fp64(nlev):
+    """A 3x3x3 block of N-level emitters (seeded Hermitian H0 and dipoles)
+    driven by a point source: fields, populations and one density matrix."""
+    deck = _n_level_block(nlev, "fp64")
     print(f"n-level block N={nlev}", flush=True)
     sim, orc = _pair(deck, 90)
     _fields(sim, orc, range(6), FP64_TOL)
@@ -164,6 +168,30 @@ def test_3d_n_level_emitter_block_fp64(nlev):
     rg = q.density(0, 13, nlev)
     ro = orc.density(0, 0, 13, nlev)
     assert rel_linf(rg, ro) <= 1e-10
+
+
+@pytest.mark.parametrize("nlev", [2, 4, 6])
+def test_3d_n_level_emitter_block_fp32_fields(nlev):
+    """The same block with FP32 fields and FP64 density matrices (the 3D
+    emitter workload's precision): fields within the FP32 field bound of the
+    FP64 oracle, populations within 1e-4 of theirs."""
+    deck = _n_level_block(nlev, "fp32")
+    print(f"n-level block N={nlev} fp32 fields", flush=True)
+    sim, orc = _pair(deck, 90)
+    # FP32 rounding is relative to the field's scale: Hz, nearly zero by
+    # symmetry here, carries the rounding of Hx and Hy, 1e4 times larger
+    for comps in ((0, 1, 2), (3, 4, 5)):
+        ref = [orc.field(c) for c in comps]
+        scale = max(np.abs(o).max() for o in ref)
+        for c, o in zip(comps, ref):
+            err = np.abs(sim.field(c) - o).max() / scale
+            print(f"field {c}: max|ref|={np.abs(o).max():.3e} err/scale={err:.3e}", flush=True)
+            assert err <= 1e-4, (c, err)
+    q = sim.qeArr()[0]
+    for j in range(2):
+        pg, po = q.population(j), orc.pops(0, j)
+        assert pg.shape == po.shape
+        assert rel_linf(pg, po) <= 1e-4, (j, rel_linf(pg, po))
 
 
 def test_3d_basis_set_emitters_x_y_dipoles_fp64():
