@@ -1032,6 +1032,8 @@ struct SampleArgs {
     const long long* det_cell;
     const double* det_conv;
     const int* det_pow;
+    const int* det_nf;             // fields per point
+    const long long* det_eoff;     // per detector offset into det_comp / det_cell (pairs)
     double* det_out;
     const long long* det_out_off;  // per detector offset into det_out
     const int* det_cap;            // per detector max samples
@@ -1170,8 +1172,30 @@ __global__ void k_sample(SampleArgs<T> a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int step = step_now(a.step, a.stepv);
     if (t < a.totpts) {
-        store_point(a, t, static_cast<double>(a.F[a.det_comp[2 * t]][a.det_cell[2 * t]]),
-                    static_cast<double>(a.F[a.det_comp[2 * t + 1]][a.det_cell[2 * t + 1]]), step);
+        int d = 0;  // the detector of point t
+        while (d + 1 < a.ndet && a.det_off[d + 1] <= t) ++d;
+        if (step % a.det_interval[d] != 0) return;
+        const int slot = step / a.det_interval[d];
+        if (slot >= a.det_cap[d]) return;
+        // the point's fields in order, each as the pair (point, offset cell)
+        const int nf = a.det_nf[d];
+        const long long e0 = a.det_eoff[d] + static_cast<long long>(t - a.det_off[d]) * nf;
+        const double c = a.det_conv[d];
+        double point = 0.0;
+        for (int f = 0; f < nf; ++f) {
+            const long long e = 2 * (e0 + f);
+            const long long cu = a.det_cell[e], cv = a.det_cell[e + 1];
+            const double u = cu >= 0 ? static_cast<double>(a.F[a.det_comp[e]][cu]) : 0.0;
+            const double v = cv >= 0 ? static_cast<double>(a.F[a.det_comp[e + 1]][cv]) : 0.0;
+            if (a.det_pow[d]) {
+                point = point + c / 2.0 * (u * u);
+                point = point + c / 2.0 * (v * v);
+            } else {
+                point = point + c / 2.0 * u;
+                point = point + c / 2.0 * v;
+            }
+        }
+        a.det_out[a.det_out_off[d] + static_cast<long long>(slot) * a.det_npts[d] + (t - a.det_off[d])] = point;
     } else if (t < a.totpts + a.ndft) {
         int s = t - a.totpts;
         if (step % a.dft_interval == 0) {
@@ -1575,6 +1599,7 @@ struct chiml_dev {
     bool split = false;  // 3D interior + CPML-slab kernels
     bool vec2d = false;  // 2D TMz W-cell group kernels
     bool fuse_sample = false;  // 2D: the E kernel takes the point detectors (no DFT, few single-cell points)
+    bool det_single_field = true;  // every point detector reads one field (two cells) per point
     long long ade_groups = 0;  // largest pole-box group count over the E components (split path)
     Box3 ibox{};
     SlabMap slab{};
@@ -1742,8 +1767,13 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
     c->det_npts.assign(p->det_npts, p->det_npts + p->ndet);
     c->det_interval.assign(p->det_interval, p->det_interval + p->ndet);
     c->det_cap.resize(p->ndet);
+    std::vector<int> nf(p->ndet, 1);
+    std::vector<long long> eoff(p->ndet + 1, 0);
     for (int d = 0; d < p->ndet; ++d) {
         if (p->det_interval[d] <= 0) throw std::runtime_error("detector interval must be positive");
+        if (p->det_nfield) nf[d] = p->det_nfield[d];
+        if (nf[d] < 1 || nf[d] > 3) throw std::runtime_error("detector fields per point must be 1..3");
+        eoff[d + 1] = eoff[d] + static_cast<long long>(p->det_npts[d]) * nf[d];
         off[d + 1] = off[d] + p->det_npts[d];
         c->det_cap[d] = p->nsteps / p->det_interval[d] + 1;
         ooff[d + 1] = ooff[d] + static_cast<long long>(c->det_cap[d]) * p->det_npts[d];
@@ -1756,8 +1786,11 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         s.det_interval = c->upload<int>(p->det_interval, p->ndet);
         s.det_npts = c->upload<int>(p->det_npts, p->ndet);
         s.det_off = c->upload<int>(off.data(), p->ndet + 1);
-        s.det_comp = c->upload<int>(p->det_comp, 2 * static_cast<size_t>(c->totpts));
-        s.det_cell = c->upload<long long>(p->det_cell, 2 * static_cast<size_t>(c->totpts));
+        s.det_comp = c->upload<int>(p->det_comp, 2 * static_cast<size_t>(eoff[p->ndet]));
+        s.det_cell = c->upload<long long>(p->det_cell, 2 * static_cast<size_t>(eoff[p->ndet]));
+        s.det_nf = c->upload<int>(nf.data(), p->ndet);
+        s.det_eoff = c->upload<long long>(eoff.data(), p->ndet + 1);
+        c->det_single_field = std::all_of(nf.begin(), nf.end(), [](int v) { return v == 1; });
         s.det_conv = c->upload<double>(p->det_conv, p->ndet);
         s.det_pow = c->upload<int>(p->det_pow, p->ndet);
         c->det_out = c->zeros<double>(static_cast<size_t>(ooff[p->ndet]));
@@ -2138,7 +2171,7 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
             // changes E after the E kernel (the back-coupling runs later), and
             // the grid is small enough for the saved launch to outweigh the scan
             // (256^2: 9.9 -> 8.6 us per step; 1024^2: 12.7 -> 13.5 us, so not there)
-            bool single = c->totpts > 0 && c->totpts <= 32 && c->ndft == 0 && c->qes.empty() &&
+            bool single = c->det_single_field && c->totpts > 0 && c->totpts <= 32 && c->ndft == 0 && c->qes.empty() &&
                           static_cast<long long>(p->nx) * p->ny <= 512LL * 512;
             for (int t = 0; single && t < c->totpts; ++t) {
                 const int cu = p->det_comp[2 * t];

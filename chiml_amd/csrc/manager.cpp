@@ -606,7 +606,7 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     P.src_amp = srcAmp.data();
 
     // ---- point detectors -----------------------------------------------------
-    std::vector<int> dInt, dNp, dComp, dPow;
+    std::vector<int> dInt, dNp, dComp, dPow, dNf;
     std::vector<long long> dCell;
     std::vector<double> dConv;
     auto cellIdx = [&](int x, int y, int z) -> long long {
@@ -632,7 +632,6 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
             else if (is3D) fl = {{3, {{1, 0, 0}}}, {4, {{0, 1, 0}}}, {5, {{0, 0, 1}}}};
             else if (D.type == DtcType::EPOW) fl = {{2, {{0, 0, 0}}}};
             else fl = {{3, {{1, 0, 0}}}, {4, {{0, 1, 0}}}};
-            if (fl.size() != 1) throw std::logic_error("power detectors over several components are not supported");
         } else {
             int comp = static_cast<int>(D.type);
             bool present = is3D || comp == 2 || comp == 3 || comp == 4;
@@ -642,21 +641,28 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         std::array<int, 3> sz = D.sz;
         if (!is3D) sz[2] = 1;
         int np = 0;
+        // an offset cell beyond the domain is the reference's zero ghost (-1)
+        auto ghostOr = [&](int x, int y, int z) -> long long {
+            if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return -1;
+            return x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
+        };
         for (int jj = 0; jj < sz[1]; ++jj)
             for (int kk = 0; kk < sz[2]; ++kk)
                 for (int ii = 0; ii < sz[0]; ++ii) {
                     int x = D.loc[0] + ii, y = D.loc[1] + jj, z = is3D ? D.loc[2] + kk : 0;
-                    auto& f = fl[0];
-                    dComp.push_back(f.first);
-                    dComp.push_back(f.first);
-                    dCell.push_back(cellIdx(x, y, z));
-                    dCell.push_back(cellIdx(x + f.second[0], y + f.second[1], z + f.second[2]));
+                    for (auto& f : fl) {
+                        dComp.push_back(f.first);
+                        dComp.push_back(f.first);
+                        dCell.push_back(cellIdx(x, y, z));
+                        dCell.push_back(ghostOr(x + f.second[0], y + f.second[1], z + f.second[2]));
+                    }
                     ++np;
                 }
         dInt.push_back(iv);
         dNp.push_back(np);
         dConv.push_back(conv);
         dPow.push_back(pw ? 1 : 0);
+        dNf.push_back(static_cast<int>(fl.size()));
         detNpts.push_back(np);
         detTConv.push_back(tconv);
         detInterval.push_back(iv);
@@ -668,6 +674,7 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     P.det_cell = dCell.data();
     P.det_conv = dConv.data();
     P.det_pow = dPow.data();
+    P.det_nfield = dNf.data();
 
     // ---- flux boxes -> DFT cell set ------------------------------------------
     std::unordered_map<long long, int> cellId;

@@ -121,15 +121,20 @@ typedef struct {
     const int* src_box;   /* [nsrc][6] */
     const double* src_amp; /* [nsteps][nsrc] */
 
-    /* Point detectors: npt sampled values per output, each the mean of two
-     * cells times conv; sampled when t_step % interval == 0, t_step >= 0. */
+    /* Point detectors: npt sampled values per output, each summed over the
+     * detector's nfield fields of conv/2 (a + b), or conv/2 (a^2 + b^2) for
+     * power, with a, b the field at the point and at its offset cell
+     * (fieldOutputFunction / pwrOutputFunction, parallelDTCOutputFxn.cpp);
+     * sampled when t_step % interval == 0, t_step >= 0.  A cell index of -1
+     * is the zero ghost beyond the domain. */
     int ndet;
     const int* det_interval; /* [ndet] */
     const int* det_npts;     /* [ndet] */
-    const int* det_comp;     /* [sum npts][2] component of the two cells */
-    const long long* det_cell; /* [sum npts][2] interior index of the two cells */
+    const int* det_comp;     /* [sum npts * nfield][2] component of the two cells */
+    const long long* det_cell; /* [sum npts * nfield][2] interior index of the two cells, or -1 */
     const double* det_conv;  /* [ndet] */
     const int* det_pow;      /* [ndet] 1: conv/2 (|a|^2 + |b|^2) */
+    const int* det_nfield;   /* [ndet] fields per point (NULL: 1 each) */
 
     /* DFT accumulators: nsamp cells, nfreq frequencies, phasor table for
      * every sampling instant k (sample k taken at t_step = k*interval). */

@@ -194,6 +194,41 @@ def test_3d_n_level_emitter_block_fp32_fields(nlev):
         assert rel_linf(pg, po) <= 1e-4, (j, rel_linf(pg, po))
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_power_detectors_over_several_fields_fp64(dim):
+    """Epow / Hpow detectors sum conv/2 (a^2 + b^2) over the E (H) components,
+    each with its Yee offset (parallelFDTDField.cpp:796-826; three fields in
+    3D, two for Hpow in 2D): interior points, a box, and points on the domain
+    edge whose offset cell is the zero ghost beyond it."""
+    n, d = (24, 0.05) if dim == 3 else (48, 0.05)
+    n3 = (n, n, n) if dim == 3 else (n, n, 1)
+    half = (n - 1) * d / 2
+    z0 = 0.0
+    deck = decks.empty_deck(n3, d, 80, 4, precision="fp64")
+    fc = 1.0 / (8 * d)
+    for pol, loc in (("Ez", [d, 0.0, z0]), ("Ex", [-d, 2 * d, z0]) if dim == 3 else ("Ez", [-3 * d, 2 * d, z0])):
+        deck["SourceList"].append({"pol": pol, "size": [0, 0, 0], "loc": loc,
+                                   "PulseList": [decks.gaussian_pulse(fc, fc, 3.0, 1.0)]})
+    sz_box = [2 * d, d, d if dim == 3 else 0.0]
+    for t in ("Epow", "Hpow"):
+        for loc, size in (([2 * d, -d, z0], [0, 0, 0]), ([0.0, 3 * d, z0], sz_box),
+                          ([-half, 0.0, z0], [0, 0, 0]), ([half, 0.0, z0], [0, 0, 0])):
+            deck["DetectorList"].append({"type": t, "dtc_class": "cout", "fname": "p", "SI": False,
+                                         "size": size, "loc": loc})
+    deck = json.dumps(deck)
+    print(f"power detectors {dim}D", flush=True)
+    sim, orc = _pair(deck, 80)
+    for k, dtc in enumerate(sim.dtcArr()):
+        tg, vg = dtc.series()
+        to, vo = orc.detector(k)
+        assert vg.shape == vo.shape, (k, vg.shape, vo.shape)
+        assert np.array_equal(tg, to), k
+        assert np.abs(vo).max() > 0.0, k
+        err = rel_linf(vg, vo)
+        print(f"detector {k}: max|ref|={np.abs(vo).max():.3e} rel_linf={err:.3e}", flush=True)
+        assert err <= FP64_TOL, (k, err)
+
+
 def test_3d_basis_set_emitters_x_y_dipoles_fp64():
     """A 3x3x3 block of 4-level emitters whose dipoles come from the
     spherical-harmonic basis {(0,0), (1,-1), (1,0), (1,1)} (x and y matrix
