@@ -142,7 +142,7 @@ bool Object::isObj(const std::array<double, 3>& v, double dx, bool useML) const 
             if (std::sqrt(t[0] * t[0] + t[2] * t[2]) > g[0] + 1.0e-6 * dx) return false;
             return true;
         }
-        case Shape::ELLIPSOID: {  // src/OBJECTS/Obj.cpp:437-457 with no cuts
+        case Shape::ELLIPSOID: {  // src/OBJECTS/Obj.cpp:437-457
             auto t = toObjectSpace(*this, v);
             double s = 0.0;
             int nax = (g[2] != 0.0) ? 3 : 2;
@@ -150,10 +150,25 @@ bool Object::isObj(const std::array<double, 3>& v, double dx, bool useML) const 
             if (1.0 < s) return false;
             for (int i = 0; i < 3; ++i) {
                 if (g[i] == 0.0) continue;
-                if ((2.0 * t[i] / g[i] < -1.0) || (2.0 * t[i] / g[i] > 1.0)) return false;
-                if ((t[i] + g[i] / 2.0) / g[i] >= 1.0) return false;
+                if ((2.0 * t[i] / g[i] < cutNeg[i]) || (2.0 * t[i] / g[i] > cutPos[i])) return false;
+                if (cutGlobal[i] >= 0.0 && (t[i] + g[i] / 2.0) / g[i] >= cutGlobal[i]) return false;
+                else if (cutGlobal[i] <= 0.0 && (t[i] - g[i] / 2.0) / g[i] <= cutGlobal[i]) return false;
             }
             return true;
+        }
+        case Shape::HEMISPHERE: {  // src/OBJECTS/Obj.cpp:382-398: the ball's half at object x <= 0
+            double d2 = 0.0;
+            for (int i = 0; i < 3; ++i) d2 += (v[i] - loc[i]) * (v[i] - loc[i]);
+            if (std::sqrt(d2) > g[0] + dx / 1.0e6) return false;
+            return !(toObjectSpace(*this, v)[0] > dx / 1e6);
+        }
+        case Shape::HEMIELLIPSOID: {  // src/OBJECTS/Obj.cpp:459-471
+            auto t = toObjectSpace(*this, v);
+            if (t[0] > dx / 1e6) return false;
+            double s = 0.0;
+            int nax = (g[2] != 0.0) ? 3 : 2;
+            for (int i = 0; i < nax; ++i) s += std::pow((t[i] - dx / 1.0e6) / (g[i] / 2.0), 2.0);
+            return !(1.0 < s);
         }
     }
     return false;
@@ -283,7 +298,26 @@ static std::shared_ptr<Object> ptreeToObject(const Value& item, const ProgramInp
         o->shape = Shape::CYLINDER;
         o->geo = {json::get_num(item, "radius", 0.0), json::get_num(item, "length", 0.0)};
     } else if (s == "ellipsoid") {
+        // src/INPUTS/parallelInputs.cpp:1479-1494
         o->shape = Shape::ELLIPSOID;
+        o->geo = json::get_vec(item, "size");
+        o->geo.resize(3, 0.0);
+        double cn[3], cp[3], cg[3];
+        json::get_arr3(item, "cut_neg", cn, -1.0);
+        json::get_arr3(item, "cut_pos", cp, 1.0);
+        json::get_arr3(item, "cut_global", cg, 1.0);
+        for (int i = 0; i < 3; ++i) {
+            if (std::abs(cn[i]) > 1.0 || std::abs(cp[i]) > 1.0 || std::abs(cg[i]) > 1.0)
+                throw std::logic_error("abs(ellipsoid cutoff) must be <= 1.0");
+            o->cutNeg[i] = cn[i];
+            o->cutPos[i] = cp[i];
+            o->cutGlobal[i] = cg[i];
+        }
+    } else if (s == "hemisphere") {  // parallelInputs.cpp:1506-1510
+        o->shape = Shape::HEMISPHERE;
+        o->geo = {json::get_num(item, "radius", 0.0)};
+    } else if (s == "hemiellipsoid") {  // parallelInputs.cpp:1496-1500
+        o->shape = Shape::HEMIELLIPSOID;
         o->geo = json::get_vec(item, "size");
         o->geo.resize(3, 0.0);
     } else {
@@ -294,12 +328,14 @@ static std::shared_ptr<Object> ptreeToObject(const Value& item, const ProgramInp
 }
 
 static void addMLBuff(Object& o, double d) {
-    // addMLBuff per shape, src/OBJECTS/Obj.hpp:537, 653, 776, 949.
+    // addMLBuff per shape, src/OBJECTS/Obj.hpp:537, 595, 653, 776, 833, 949.
     switch (o.shape) {
-        case Shape::SPHERE: o.geoML[0] += 3.0 * d; break;
+        case Shape::SPHERE:
+        case Shape::HEMISPHERE: o.geoML[0] += 3.0 * d; break;
         case Shape::CYLINDER: o.geoML[0] += 3.0 * d; o.geoML[1] += 3.0 * d; break;
         case Shape::BLOCK:
         case Shape::ELLIPSOID:
+        case Shape::HEMIELLIPSOID:
             for (int i = 0; i < 3; ++i) o.geoML[i] += 3.0 * d;
             break;
     }

@@ -30,7 +30,7 @@ enum class Pol { EX = 0, EY, EZ, HX, HY, HZ };
 enum class PulseShape { GAUSSIAN, BH, RECT, CONTINUOUS, RICKER, RAMP_CONT };
 enum class DtcType { EX = 0, EY, EZ, HX, HY, HZ, EPOW, HPOW };
 enum class DtcClass { BIN, TXT, COUT, FREQ };
-enum class Shape { BLOCK, SPHERE, CYLINDER, ELLIPSOID };
+enum class Shape { BLOCK, SPHERE, CYLINDER, ELLIPSOID, HEMISPHERE, HEMIELLIPSOID };
 
 Pol string2pol(const std::string& p);
 PulseShape string2prof(const std::string& p);
@@ -54,6 +54,8 @@ struct Object {
     std::vector<double> geo, geoML;  // geoML carries the emitter buffer (addMLBuff)
     std::array<double, 3> loc{{0, 0, 0}};
     std::array<double, 9> T{};  // coordTransform_: row r = unit vector r (normalised)
+    // ellipsoid cuts (cut_neg, cut_pos, cut_global), fractions of the half axes
+    std::array<double, 3> cutNeg{{-1, -1, -1}}, cutPos{{1, 1, 1}}, cutGlobal{{1, 1, 1}};
     std::array<double, 3> eps{{1, 1, 1}};  // eps_infty per Yee component (x,y,z)
     double epsScalar = 1.0;  // the deck's scalar "eps" (reference epsInfty())
     double mu = 1.0;
@@ -63,7 +65,7 @@ struct Object {
     std::vector<double> alpha, xi, gamma;
 
     void setUpConsts(double dt);
-    // isObj for the four supported shapes; useML selects geoML (the emitter
+    // isObj for the supported shapes; useML selects geoML (the emitter
     // map) instead of geo (the permittivity map).
     bool isObj(const std::array<double, 3>& v, double dx, bool useML) const;
     bool dielectric() const;  // eps != 1 or poles or emitters

@@ -229,6 +229,26 @@ def test_power_detectors_over_several_fields_fp64(dim):
         assert err <= FP64_TOL, (k, err)
 
 
+def test_3d_hemisphere_hemiellipsoid_and_cut_ellipsoid_fp64():
+    """The half shapes and ellipsoid cuts through the whole product path
+    (material ids, dispersive boxes, split kernels) against the oracle."""
+    d = decks.empty_deck((24, 24, 24), 0.02, 80, 4, precision="fp64")
+    base = {"material": "custom", "Basis_Set": [],
+            "unit_vectors": [{"uvec": [0.8, 0.6, 0.0]}, {"uvec": [-0.6, 0.8, 0.0]}, {"uvec": [0.0, 0.0, 1.0]}]}
+    d["ObjectList"].append(dict(base, shape="hemisphere", eps=3.0, pols=[], radius=0.11, loc=[-0.07, 0.05, 0.0]))
+    d["ObjectList"].append(dict(base, shape="hemiellipsoid", eps=1.0, pols=decks.gold_like_pols()[:2],
+                                size=[0.16, 0.12, 0.1], loc=[0.08, -0.06, 0.03]))
+    d["ObjectList"].append(dict(base, shape="ellipsoid", eps=2.2, pols=[], size=[0.18, 0.1, 0.14],
+                                loc=[0.02, 0.1, -0.08], cut_pos=[0.6, 1.0, 1.0], cut_global=[1.0, 1.0, 0.7]))
+    fc = 1.0 / (8 * 0.02)
+    d["SourceList"].append({"pol": "Ez", "size": [0.0, 0.0, 0.0], "loc": [0.0, 0.0, 0.05],
+                            "PulseList": [decks.gaussian_pulse(fc, fc, 3.0, 1.0)]})
+    deck = json.dumps(d)
+    print("half shapes and cut ellipsoid", flush=True)
+    sim, orc = _pair(deck, 80)
+    _fields(sim, orc, range(6), FP64_TOL)
+
+
 def test_3d_basis_set_emitters_x_y_dipoles_fp64():
     """A 3x3x3 block of 4-level emitters whose dipoles come from the
     spherical-harmonic basis {(0,0), (1,-1), (1,0), (1,1)} (x and y matrix

@@ -832,3 +832,68 @@ def test_flux_of_a_static_field_is_the_dirichlet_kernel(axis):
     want = sign * e0 * h0 * (10 * d) ** 2 * kern / (steps + 1) ** 2
     assert np.allclose(got.real, want, rtol=1e-10, atol=0.0)
     assert np.max(np.abs(got.imag)) <= 1e-12 * np.max(np.abs(want))
+
+
+def test_hemisphere_rasterisation_rotated():
+    """Hemisphere (Obj.cpp:382-398): the ball's samples with object x <= 0,
+    in an object frame rotated by 30 degrees about z."""
+    d = 0.05
+    loc, r = np.array([0.37, -0.61, 0.23]) * d, 6.29 * d
+    c30, s30 = np.cos(np.pi / 6), np.sin(np.pi / 6)
+    u = np.array([[c30, s30, 0.0], [-s30, c30, 0.0], [0.0, 0.0, 1.0]])
+    eps = (2.0, 3.5, 2.5)
+
+    def inside(p):
+        rel = [p[a] - loc[a] for a in range(3)]
+        dist = np.sqrt(sum(x ** 2 for x in rel))
+        t0 = sum(u[0][a] * rel[a] for a in range(3))
+        ins = (dist <= r) & (t0 <= 0.0)
+        return ins, min(float(np.min(np.abs(dist - r))), float(np.min(np.abs(t0))))
+
+    obj = _obj("hemisphere", eps, loc, radius=r)
+    obj["unit_vectors"] = [{"uvec": list(row)} for row in u]
+    _check_raster([obj], inside, eps, 23, d)
+
+
+def test_hemiellipsoid_rasterisation():
+    """Hemiellipsoid (Obj.cpp:459-471): the ellipsoid's samples with object
+    x <= 0."""
+    d = 0.05
+    loc, size = np.array([0.41, -0.27, 0.33]) * d, np.array([14.3, 10.7, 12.9]) * d
+    eps = (3.0, 2.5, 4.5)
+
+    def inside(p):
+        t = [p[a] - loc[a] for a in range(3)]
+        s = sum((t[a] / (size[a] / 2)) ** 2 for a in range(3))
+        ins = (s <= 1.0) & (t[0] <= 0.0)
+        return ins, min(float(np.min(np.abs(s - 1.0))), float(np.min(np.abs(t[0]))))
+
+    _check_raster([_obj("hemiellipsoid", eps, loc, size=list(size))], inside, eps, 23, d)
+
+
+def test_ellipsoid_cuts_rasterisation():
+    """Ellipsoid cuts (parallelInputs.cpp:1479-1494, Obj.cpp:437-457): cut_pos
+    and cut_neg bound 2 t / size per axis; a positive cut_global c keeps
+    (t + size/2) / size < c, a negative one keeps (t - size/2) / size > c."""
+    d = 0.05
+    loc, size = np.array([-0.148, -0.034, -0.236]) * d, np.array([12.431, 14.376, 11.787]) * d
+    cpos, cneg, cglob = [0.57, 1.0, 1.0], [-1.0, -0.54, -1.0], [1.0, 1.0, 0.75]
+    eps = (2.5, 3.0, 2.0)
+
+    def inside(p):
+        t = [p[a] - loc[a] for a in range(3)]
+        s = sum((t[a] / (size[a] / 2)) ** 2 for a in range(3))
+        ins = s <= 1.0
+        margin = float(np.min(np.abs(s - 1.0)))
+        for a in range(3):
+            q = 2 * t[a] / size[a]
+            ins &= (q >= cneg[a]) & (q <= cpos[a])
+            g = (t[a] + size[a] / 2) / size[a]
+            ins &= g < cglob[a]
+            margin = min(margin, float(np.min(np.abs(q - cpos[a]))), float(np.min(np.abs(q - cneg[a]))),
+                         float(np.min(np.abs(g - cglob[a]))))
+        return ins, margin * d
+
+    obj = _obj("ellipsoid", eps, loc, size=list(size))
+    obj.update({"cut_pos": cpos, "cut_neg": cneg, "cut_global": cglob})
+    _check_raster([obj], inside, eps, 23, d)
