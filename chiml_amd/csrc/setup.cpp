@@ -162,6 +162,20 @@ bool Object::isObj(const std::array<double, 3>& v, double dx, bool useML) const 
             if (std::sqrt(d2) > g[0] + dx / 1.0e6) return false;
             return !(toObjectSpace(*this, v)[0] > dx / 1e6);
         }
+        case Shape::PARABOLOID: {  // src/OBJECTS/Obj.cpp:550-560: geo = a, b, length
+            auto t = toObjectSpace(*this, v);
+            t[1] += g[2] / 2.0;
+            if (t[1] > g[2]) return false;
+            return !(std::pow(t[0] / g[0], 2.0) + std::pow(t[2] / g[1], 2.0) > t[1]);
+        }
+        case Shape::TORUS: {  // src/OBJECTS/Obj.cpp:562-575: geo = ring radius, cross radius, phi range
+            auto t = toObjectSpace(*this, v);
+            if (std::pow(std::sqrt(t[0] * t[0] + t[1] * t[1]) - g[0], 2.0) > g[1] * g[1] - t[2] * t[2]) return false;
+            double ph = t[0] != 0 ? std::atan(t[1] / t[0]) : t[1] / std::abs(t[1]) * kPi / 2.0;
+            if (t[0] < 0) ph += kPi;
+            else if (t[1] < 0) ph += 2.0 * kPi;
+            return !(ph < g[2] || ph > g[3]);
+        }
         case Shape::HEMIELLIPSOID: {  // src/OBJECTS/Obj.cpp:459-471
             auto t = toObjectSpace(*this, v);
             if (t[0] > dx / 1e6) return false;
@@ -316,6 +330,15 @@ static std::shared_ptr<Object> ptreeToObject(const Value& item, const ProgramInp
     } else if (s == "hemisphere") {  // parallelInputs.cpp:1506-1510
         o->shape = Shape::HEMISPHERE;
         o->geo = {json::get_num(item, "radius", 0.0)};
+    } else if (s == "paraboloid") {  // parallelInputs.cpp:1575-1579
+        o->shape = Shape::PARABOLOID;
+        const double rc = json::get_num(item, "rad_curve", 0.0);
+        o->geo = {json::get_num(item, "a", 1.0 / (2.0 * rc)), json::get_num(item, "b", 1.0 / (2.0 * rc)),
+                  json::get_num(item, "length", 0.0)};
+    } else if (s == "torus") {  // parallelInputs.cpp:1580-1584
+        o->shape = Shape::TORUS;
+        o->geo = {json::get_num(item, "rad_Ring", 0.0), json::get_num(item, "rad_CrossSec", 0.0),
+                  json::get_num(item, "phiMin", 0.0) * kPi / 180.0, json::get_num(item, "phiMax", 360.0) * kPi / 180.0};
     } else if (s == "hemiellipsoid") {  // parallelInputs.cpp:1496-1500
         o->shape = Shape::HEMIELLIPSOID;
         o->geo = json::get_vec(item, "size");
@@ -328,10 +351,12 @@ static std::shared_ptr<Object> ptreeToObject(const Value& item, const ProgramInp
 }
 
 static void addMLBuff(Object& o, double d) {
-    // addMLBuff per shape, src/OBJECTS/Obj.hpp:537, 595, 653, 776, 833, 949.
+    // addMLBuff per shape, src/OBJECTS/Obj.hpp:537, 595, 653, 776, 833, 949, 1147, 1205.
     switch (o.shape) {
         case Shape::SPHERE:
         case Shape::HEMISPHERE: o.geoML[0] += 3.0 * d; break;
+        case Shape::PARABOLOID:
+        case Shape::TORUS: o.geoML[0] += 3.0 * d; o.geoML[1] += 3.0 * d; break;
         case Shape::CYLINDER: o.geoML[0] += 3.0 * d; o.geoML[1] += 3.0 * d; break;
         case Shape::BLOCK:
         case Shape::ELLIPSOID:

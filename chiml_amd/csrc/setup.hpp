@@ -30,7 +30,7 @@ enum class Pol { EX = 0, EY, EZ, HX, HY, HZ };
 enum class PulseShape { GAUSSIAN, BH, RECT, CONTINUOUS, RICKER, RAMP_CONT };
 enum class DtcType { EX = 0, EY, EZ, HX, HY, HZ, EPOW, HPOW };
 enum class DtcClass { BIN, TXT, COUT, FREQ };
-enum class Shape { BLOCK, SPHERE, CYLINDER, ELLIPSOID, HEMISPHERE, HEMIELLIPSOID };
+enum class Shape { BLOCK, SPHERE, CYLINDER, ELLIPSOID, HEMISPHERE, HEMIELLIPSOID, PARABOLOID, TORUS };
 
 Pol string2pol(const std::string& p);
 PulseShape string2prof(const std::string& p);

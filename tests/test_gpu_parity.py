@@ -230,7 +230,7 @@ def test_power_detectors_over_several_fields_fp64(dim):
 
 
 def test_3d_hemisphere_hemiellipsoid_and_cut_ellipsoid_fp64():
-    """The half shapes and ellipsoid cuts through the whole product path
+    """The half shapes, ellipsoid cuts, a paraboloid and a torus section through the whole product path
     (material ids, dispersive boxes, split kernels) against the oracle."""
     d = decks.empty_deck((24, 24, 24), 0.02, 80, 4, precision="fp64")
     base = {"material": "custom", "Basis_Set": [],
@@ -240,6 +240,10 @@ def test_3d_hemisphere_hemiellipsoid_and_cut_ellipsoid_fp64():
                                 size=[0.16, 0.12, 0.1], loc=[0.08, -0.06, 0.03]))
     d["ObjectList"].append(dict(base, shape="ellipsoid", eps=2.2, pols=[], size=[0.18, 0.1, 0.14],
                                 loc=[0.02, 0.1, -0.08], cut_pos=[0.6, 1.0, 1.0], cut_global=[1.0, 1.0, 0.7]))
+    d["ObjectList"].append(dict(base, shape="paraboloid", eps=2.6, pols=[], a=0.05, b=0.04, length=0.12,
+                                loc=[-0.1, -0.1, 0.1]))
+    d["ObjectList"].append(dict(base, shape="torus", eps=1.8, pols=[], rad_Ring=0.12, rad_CrossSec=0.03,
+                                phiMin=30.0, phiMax=250.0, loc=[0.0, 0.0, -0.02]))
     fc = 1.0 / (8 * 0.02)
     d["SourceList"].append({"pol": "Ez", "size": [0.0, 0.0, 0.0], "loc": [0.0, 0.0, 0.05],
                             "PulseList": [decks.gaussian_pulse(fc, fc, 3.0, 1.0)]})

@@ -897,3 +897,45 @@ def test_ellipsoid_cuts_rasterisation():
     obj = _obj("ellipsoid", eps, loc, size=list(size))
     obj.update({"cut_pos": cpos, "cut_neg": cneg, "cut_global": cglob})
     _check_raster([obj], inside, eps, 23, d)
+
+
+def test_paraboloid_rasterisation():
+    """Paraboloid (Obj.cpp:550-560, parse parallelInputs.cpp:1575-1579): with
+    s = t_y + length / 2, the samples with s <= length and
+    (t_x / a)^2 + (t_z / b)^2 <= s."""
+    d = 0.05
+    loc, a, b, length = np.array([0.335, 0.49, 0.443]) * d, 2.511 * d, 2.013 * d, 12.912 * d
+    eps = (2.0, 2.5, 3.0)
+
+    def inside(p):
+        t = [p[k] - loc[k] for k in range(3)]
+        s = t[1] + length / 2
+        q = (t[0] / a) ** 2 + (t[2] / b) ** 2
+        ins = (s <= length) & (q <= s)
+        return ins, min(float(np.min(np.abs(s - length))), float(np.min(np.abs(q - s))) * d)
+
+    _check_raster([_obj("paraboloid", eps, loc, a=a, b=b, length=length)], inside, eps, 23, d)
+
+
+def test_partial_torus_rasterisation():
+    """Torus section (Obj.cpp:562-575, parse parallelInputs.cpp:1580-1584):
+    (rho - R)^2 <= r^2 - t_z^2 with rho the distance from the object z axis,
+    and the azimuth atan2(t_y, t_x) taken in [0, 2 pi) within [phiMin, phiMax]."""
+    d = 0.05
+    loc, R, r = np.array([0.31, 0.17, -0.23]) * d, 6.13 * d, 2.41 * d
+    phimin, phimax = 20.0, 290.0
+    eps = (3.5, 2.0, 2.5)
+
+    def inside(p):
+        t = [p[k] - loc[k] for k in range(3)]
+        rho = np.sqrt(t[0] ** 2 + t[1] ** 2)
+        lhs, rhs = (rho - R) ** 2, r ** 2 - t[2] ** 2
+        ph = np.mod(np.arctan2(t[1], t[0]), 2 * np.pi)
+        lo, hi = np.deg2rad(phimin), np.deg2rad(phimax)
+        ins = (lhs <= rhs) & (ph >= lo) & (ph <= hi)
+        margin = min(float(np.min(np.abs(lhs - rhs))) / d, float(np.min(np.abs(ph - lo))) * d,
+                     float(np.min(np.abs(ph - hi))) * d)
+        return ins, margin
+
+    _check_raster([_obj("torus", eps, loc, rad_Ring=R, rad_CrossSec=r, phiMin=phimin, phiMax=phimax)],
+                  inside, eps, 23, d)
