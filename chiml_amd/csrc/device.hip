@@ -1034,6 +1034,7 @@ struct SampleArgs {
     const int* det_pow;
     const int* det_nf;             // fields per point
     const long long* det_eoff;     // per detector offset into det_comp / det_cell (pairs)
+    int det_all1;                  // every detector reads one field: point t's cells are entries 2t, 2t + 1
     double* det_out;
     const long long* det_out_off;  // per detector offset into det_out
     const int* det_cap;            // per detector max samples
@@ -1171,7 +1172,11 @@ template <typename T>
 __global__ void k_sample(SampleArgs<T> a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int step = step_now(a.step, a.stepv);
-    if (t < a.totpts) {
+    if (t < a.totpts && a.det_all1) {
+        // the cells do not depend on the detector: their loads overlap its search
+        store_point(a, t, static_cast<double>(a.F[a.det_comp[2 * t]][a.det_cell[2 * t]]),
+                    static_cast<double>(a.F[a.det_comp[2 * t + 1]][a.det_cell[2 * t + 1]]), step);
+    } else if (t < a.totpts) {
         int d = 0;  // the detector of point t
         while (d + 1 < a.ndet && a.det_off[d + 1] <= t) ++d;
         if (step % a.det_interval[d] != 0) return;
@@ -1791,6 +1796,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         s.det_nf = c->upload<int>(nf.data(), p->ndet);
         s.det_eoff = c->upload<long long>(eoff.data(), p->ndet + 1);
         c->det_single_field = std::all_of(nf.begin(), nf.end(), [](int v) { return v == 1; });
+        s.det_all1 = c->det_single_field ? 1 : 0;
         s.det_conv = c->upload<double>(p->det_conv, p->ndet);
         s.det_pow = c->upload<int>(p->det_pow, p->ndet);
         c->det_out = c->zeros<double>(static_cast<size_t>(ooff[p->ndet]));
