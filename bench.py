@@ -9,22 +9,21 @@ is Mcell-updates/s = cells * steps / seconds / 1e6 with the fields resident in
 HBM.  The FP32 path is the headline (`value`); the FP64 path is measured on
 the same deck and reported beside it.
 
-roofline: the dominant kernel's algorithmic bytes per launch (every state
-word it must read and write once: fields, material ids, CPML psi, pole
-currents) divided by its mean launch time, measured with HIP events on the
-simulation's stream around each of its launches in the timed region, against
-8 TB/s.  The dominant kernel is the one with the largest time per step in an
-untimed pass of up to 200 steps after the warmup, where every launch is timed
-(the "kernels" breakdown).  traffic
-comes from the committed rocprofv3 --pmc summary
-(profiles/traffic_config2_<precision>.json) when it is present.
-step_roofline: the whole step against the bytes it would move if every state
-word were read once and written once (fields, one material byte per cell,
-psi, D and two P words per pole), over the summed field-kernel time per step
-and over the wall-clock step.  The split H and E sweeps read each field twice,
-so this fraction sits below the per-kernel one.
+roofline: the whole Yee + CPML + ADE step (the field kernels) against the
+bytes a step must move when every state word is read once and written once
+(fields, one material byte per cell and E component, CPML psi, D and two P
+words per pole), over the field kernels' mean time per step.  That time is
+measured with HIP events on the simulation's stream around each field-kernel
+launch in the timed region.  traffic is the field kernels' HBM bytes per step
+from the committed rocprofv3 --pmc summary (profiles/traffic_config2_<precision>.json).
+kernel_roofline: the dominant kernel alone, against its own per-launch bytes.
+accuracy: FP32 against FP64 on fixed runs, whatever --steps is: the 256^3
+deck with six point detectors for 1000 steps (detector series and flux
+spectrum), and a 2D Maxwell-Liouville deck for 4000 steps (populations).
 cpu_baseline: the FP64 CPU restatement of the reference step loop (the
-oracle) timed on this host for a few steps of the same 256^3 deck.
+oracle) on this host for a few steps of the same 256^3 deck, with its update
+lists split over y slabs across P threads (P = the host's CPU share), and on
+one thread beside it.
 """
 import argparse
 import json
@@ -46,7 +45,9 @@ def parse():
     p.add_argument("--n", type=int, default=256, help="points per axis")
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-steps", type=int, default=24)  # about 12 s of one core on 256^3
+    p.add_argument("--cpu-steps", type=int, default=16)  # about 8 s of one core on 256^3
+    p.add_argument("--cpu-steps-par", type=int, default=48, help="timed oracle steps on the threaded CPU leg")
+    p.add_argument("--no-accuracy", action="store_true", help="skip the fixed FP32-vs-FP64 accuracy runs")
     p.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     p.add_argument("--sweep-requests", type=int, default=32,
                    help="requests in the serving-sweep sample (0 skips the sweep leg)")
@@ -64,7 +65,8 @@ def parse():
     return p.parse_args()
 
 
-KERNELS = ("update_h", "update_h_pml", "update_e", "update_e_pml", "dft_flush", "ml_density")
+KERNELS = ("update_fused", "update_ade", "update_h", "update_h_pml", "update_e", "update_e_pml", "dft_flush", "ml_density")
+FIELD_KERNELS = ("update_fused", "update_ade", "update_h", "update_h_pml", "update_e", "update_e_pml")
 
 
 def run_gpu(n, steps, warmup, precision):
@@ -90,15 +92,17 @@ def run_gpu(n, steps, warmup, precision):
                        "bytes_per_launch": sim.kernel_bytes(k)}
     cand = [k for k in kern if kern[k]["bytes_per_launch"] > 0]
     dominant = max(cand, key=lambda k: kern[k]["ms_per_step"]) if cand else "update_e"
-    # timed region: only the dominant kernel carries events (two records per step)
-    sim.profile_only(dominant)
+    # timed region: only the field kernels carry events (two records per launch)
+    field = [k for k in FIELD_KERNELS if k in kern]
+    sim.profile_only(",".join(field))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.step(steps)
     sim.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    dom_ms, dom_n = sim.profile_read(dominant)
+    timed = {k: sim.profile_read(k) for k in field}
+    dom_ms, dom_n = timed.get(dominant, (0.0, 0))
     sim.profile(False)
     flux = sim.fluxArr()[0].getFlux() if sim.fluxArr() else None
     wall = t1 - t0
@@ -112,6 +116,8 @@ def run_gpu(n, steps, warmup, precision):
         "dominant_ms": dom_ms / max(dom_n, 1),
         "dominant_launches": dom_n,
         "dominant_bytes": sim.kernel_bytes(dominant),
+        "field_ms_per_step": sum(ms for ms, _ in timed.values()) / steps,
+        "field_launches": {k: n for k, (_, n) in timed.items()},
         "yee_step_min": sim.kernel_bytes("yee_step_min"),
         "footprint": sim.footprint(),
         "flux0": None if flux is None else [float(flux[0].real), float(flux[0].imag)],
@@ -130,7 +136,9 @@ def load_traffic(precision):
         return json.load(f)
 
 
-def roofline(r, precision):
+def kernel_roofline(r, precision):
+    """The dominant kernel alone: its algorithmic bytes per launch over its
+    mean launch time in the timed region."""
     name = r["dominant"]
     ms = r["dominant_ms"]
     b = r["dominant_bytes"]
@@ -139,7 +147,7 @@ def roofline(r, precision):
     t = load_traffic(precision)
     if t and name in t.get("kernels", {}):
         traffic = t["kernels"][name]["hbm_bytes_per_launch"]
-    return name, {
+    return {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
@@ -153,24 +161,32 @@ def roofline(r, precision):
     }
 
 
-def step_roofline(r):
-    """The whole Yee + CPML + ADE step against the compulsory bytes of a step
-    in which every state word is read once and written once (a fused H and E
-    update): over the field kernels' time per step (the per-launch events of
-    the untimed profiled pass) and over the timed wall-clock step."""
+def step_roofline(r, precision):
+    """The Yee + CPML + ADE step against the compulsory bytes of a step in
+    which every state word is read once and written once, over the field
+    kernels' mean time per step in the timed region (HIP events around each
+    launch); traffic sums the field kernels' measured HBM bytes per step."""
     b = r["yee_step_min"]
-    ms = sum(r["kernels"][k]["ms_per_step"] for k in ("update_h", "update_h_pml", "update_e", "update_e_pml")
-             if k in r["kernels"])
+    ms = r["field_ms_per_step"]
     if b <= 0 or ms <= 0:
         return None
     achieved = b / (ms * 1e-3) / 1e9
+    traffic = None
+    t = load_traffic(precision)
+    if t:
+        ks = t.get("kernels", {})
+        if all(k in ks for k in r["field_launches"]):
+            traffic = int(sum(ks[k]["hbm_bytes_per_launch"] for k in r["field_launches"]))
     return {
-        "bytes_min_per_step": b,
-        "field_kernels_ms_per_step": round(ms, 5),
+        "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "kernels": sorted(r["field_launches"]),
+        "algorithmic_bytes_per_step": b,
+        "field_kernels_ms_per_step": round(ms, 5),
         "frac_of_wall_step": round(b / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
 
@@ -261,26 +277,112 @@ def run_sweep(args):
     return res
 
 
-def cpu_baseline(n, steps):
-    from chiml_amd import decks
-    from oracle.pyoracle import Oracle
+def rel_linf(a, b):
+    import numpy as np
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = float(np.max(np.abs(b))) if b.size else 0.0
+    return float(np.max(np.abs(a - b)) / den) if den > 0 else None
 
+
+def accuracy_leg(n=256):
+    """FP32 fields against FP64 fields on fixed runs (independent of --steps):
+    config 2 at n^3 with six point detectors (Ex..Hz) for 1000 steps, and a
+    2D Maxwell-Liouville deck (256^2, 128 x 16 two-level emitters, resonant
+    pulse) for 4000 steps.  Errors are normalised L-inf per series."""
+    import chiml_amd
+    from chiml_amd import decks
+
+    out = {}
+    res = {}
+    for prec in ("fp32", "fp64"):
+        sim = chiml_amd.FDTDField(decks.config2_3d(n=n, steps=1000, precision=prec, ndet=6))
+        sim.step(sim.nsteps)
+        sim.sync()
+        res[prec] = ([d.series()[1] for d in sim.dtcArr()], sim.fluxArr()[0].getFlux())
+        sim.close()
+    det = [rel_linf(a, b) for a, b in zip(res["fp32"][0], res["fp64"][0])]
+    out["config2_1000_steps"] = {
+        "detectors_rel_linf": {c: None if e is None else float(f"{e:.3e}") for c, e in zip(["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"], det)},
+        "flux_spectrum_rel_linf": float(f"{rel_linf(res['fp32'][1], res['fp64'][1]):.3e}"),
+    }
+    pops = {}
+    for prec in ("fp32", "fp64"):
+        sim = chiml_amd.FDTDField(decks.config3_ml2d(n=256, steps=4000, pml=20, slab=(128, 16), precision=prec))
+        sim.step(sim.nsteps)
+        sim.sync()
+        q = sim.qeArr()[0]
+        pops[prec] = [q.population(0), q.population(1)]
+        sim.close()
+    import numpy as np
+    p32 = np.concatenate(pops["fp32"])
+    p64 = np.concatenate(pops["fp64"])
+    # the excited population is the small one: its error is also stated
+    # against its own peak
+    out["ml2d_256_4000_steps"] = {
+        "populations_abs_linf": float(f"{float(np.max(np.abs(p32 - p64))):.3e}"),
+        "excited_population_rel_linf": float(f"{rel_linf(pops['fp32'][1], pops['fp64'][1]):.3e}"),
+        "excited_population_peak": float(f"{float(np.max(np.abs(pops['fp64'][1]))):.3e}"),
+    }
+    out["bound"] = 1e-5
+    return out
+
+
+def cpu_share():
+    """Threads for the CPU baseline: the CPUs this process may run on, capped
+    by OMP_NUM_THREADS when it is set (the host's CPU share for one GPU)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_time(n, steps, threads):
+    from chiml_amd import decks
+    from oracle import pyoracle
+
+    pyoracle.set_threads(threads)
     deck = decks.config2_3d(n=n, steps=steps + 1, precision="fp64")
     t0 = time.perf_counter()
-    o = Oracle(deck)
+    o = pyoracle.Oracle(deck)
     setup = time.perf_counter() - t0
     o.step(1)  # first touch
     t1 = time.perf_counter()
     o.step(steps)
     t2 = time.perf_counter()
-    cells = n ** 3
+    del o
+    pyoracle.set_threads(1)
+    return n ** 3 * steps / (t2 - t1) / 1e6, t2 - t1, setup
+
+
+def cpu_baseline(n, steps, steps_par):
+    p = cpu_share()
+    v1, s1, setup = cpu_time(n, steps, 1)
+    vp, sp, _ = cpu_time(n, steps_par, p)
     return {
-        "value": round(cells * steps / (t2 - t1) / 1e6, 3),
+        "value": round(vp, 3),
         "unit": "Mcell-updates/s",
-        "cores": 1,
+        "cores": p,
         "kind": "port",
-        "sample": f"FP64 oracle (reference step loop restated, BLAS-1 per x-run), same {n}^3 deck, "
-                  f"{steps} timed steps ({t2 - t1:.1f} s) after 1 untimed step; setup {setup:.1f} s excluded",
+        "sample": f"FP64 oracle (reference step loop restated, BLAS-1 per x-run), same {n}^3 deck; update lists "
+                  f"split over y slabs across {p} threads: {steps_par} timed steps ({sp:.1f} s) after 1 untimed step; "
+                  f"setup {setup:.1f} s excluded",
+        "one_thread": {"value": round(v1, 3), "steps": steps, "seconds": round(s1, 1)},
+        "nproc": os.cpu_count(),
+        "cpu_share": p,
+        "cpu_model": cpu_model(),
     }
 
 
@@ -301,7 +403,6 @@ def main():
         torch.cuda.synchronize()
         del a, b
     r32 = run_gpu(args.n, args.steps, args.warmup, args.precision)
-    kname, roof = roofline(r32, args.precision)
     out = {
         "metric": "Mcell-updates/s + % HBM roofline (FP32 vs FP64); sweep p50/p99 latency",
         "value": round(r32["mcells"], 1),
@@ -322,31 +423,28 @@ def main():
             "precision": args.precision,
             "parallelism": "single GPU, one device-resident domain",
         },
-        "roofline": roof,
-        "step_roofline": step_roofline(r32),
+        "roofline": step_roofline(r32, args.precision),
+        "kernel_roofline": kernel_roofline(r32, args.precision),
         "kernels": kernel_table(r32),
     }
     if not args.no_fp64 and args.precision == "fp32":
         r64 = run_gpu(args.n, args.steps, args.warmup, "fp64")
-        k64, roof64 = roofline(r64, "fp64")
         out["fp64"] = {
             "value": round(r64["mcells"], 1),
             "ms_per_step": round(r64["ms_per_step"], 4),
-            "roofline": roof64,
-            "step_roofline": step_roofline(r64),
+            "roofline": step_roofline(r64, "fp64"),
+            "kernel_roofline": kernel_roofline(r64, "fp64"),
             "kernels": kernel_table(r64),
         }
         out["fp32_over_fp64"] = round(r32["mcells"] / r64["mcells"], 3)
-        if r32["flux0"] and r64["flux0"]:
-            a = complex(*r32["flux0"])
-            b = complex(*r64["flux0"])
-            out["flux_f0_rel_err_fp32_vs_fp64"] = abs(a - b) / abs(b) if abs(b) > 0 else None
+    if not args.no_accuracy and args.n == 256:
+        out["accuracy_fp32_vs_fp64"] = accuracy_leg(args.n)
     if not args.no_configs:
         out["other_workloads"] = run_configs()
     if args.sweep_requests > 0:
         out["sweep"] = run_sweep(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.n, args.cpu_steps, args.cpu_steps_par)
     print(json.dumps(out), flush=True)
 
 

@@ -33,6 +33,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -94,12 +95,14 @@ struct PmlDev {
     const T* cd[3];
     const T* kinv[3];
     T* psi[3];
+    const T* psi_in[3];  // fused step: psi before the step (ping-pong for H); psi itself elsewhere
     int lo[3], hi0[3], hi1[3], S[3];
 };
 
 template <typename T>
 struct StepArgs {
     T* F[6];  // Ex, Ey, Ez, Hx, Hy, Hz
+    const T* G[6];  // fused step: the fields before the step (ping-pong); F itself elsewhere
     int nx, ny, nz;
     T inv_d[3];
     int Tp[3];
@@ -1019,6 +1022,562 @@ __global__ void __launch_bounds__(256) k_e3_all(StepArgs<T> a, Box3 b, SlabMap m
     e_group<T, W, true, SRC, true>(a, mt, x0, y, z, lane > 0 && col > 0);
 }
 
+// ------------------------------------------------------- fused 3D step ----
+// One launch advances H and then E over the whole 3D domain, so that each
+// field word is read once and written once per step (the split sweeps read
+// E again in the H sweep and H again in the E sweep).  The fields are
+// ping-ponged: old values come from a.G, new ones go to a.F, and so does the
+// CPML psi of the H components (pml[3..5].psi_in -> psi); the E psi is
+// updated in place.  A block owns a tile of `ty` whole x rows for `lz`
+// consecutive planes and marches it through z.  Per plane it computes H for
+// its rows and, without storing it, for the row below the tile; shares the
+// plane through LDS; then computes E, which needs H at x-1 and y-1 (LDS) and
+// at z-1 (registers, from the previous plane).  A segment's first plane
+// takes H at z-1 from the same recomputation.  The reference's phase order
+// (H, sources on H, pole update, E, parallelFDTDField.hpp:1228-1303) holds
+// per cell: a cell's E is formed from H values that are final for the step.
+constexpr int kFusedMaxThreads = 640;  // (ty + 1) * gx at most: 9 + 1 rows of 64 groups
+
+struct FuseMap {
+    int gx;        // W-cell groups per x row (nx / W)
+    int ty;        // rows per tile
+    int ntile;     // tiles along y
+    int nseg;      // segments along z
+    int lz;        // planes per segment
+    int nthr;      // (ty + 1) * gx working threads of a block
+    int ys0, ys1;  // rows outside [ys0, ys1) carry y CPML psi of some component
+    int zs0, zs1;  // planes outside [zs0, zs1) carry z CPML psi of some component
+    unsigned cstride;  // bytes between components of a field buffer (a.F[k] = a.F[0] + k N)
+    int xpsi;          // the x-CPML psi of Hy, Hz, Ey and Ez exist (all four or none)
+};
+
+// CPML profiles of the fused kernel in LDS, by slab-compact position s
+// (the psi array's own index along its axis): b, c/d and 1/kappa for each
+// component and derivative axis that carries psi.  Staged at kernel entry,
+// so the per-cell CPML terms read LDS instead of waiting on global loads.
+constexpr int kFusedSlab = 64;  // psi positions per axis (both slabs) at most
+
+template <typename T>
+struct ProfLds {
+    T v[6][3][3][kFusedSlab];  // [comp][axis][b, c/d, 1/kappa][s]
+};
+
+template <typename T>
+__device__ __forceinline__ void load_prof_lds(const StepArgs<T>& a, ProfLds<T>& P) {
+    const int tid = threadIdx.x;
+    for (int k = tid; k < 6 * 3 * kFusedSlab; k += blockDim.x) {
+        const int comp = k / (3 * kFusedSlab), ax = (k / kFusedSlab) % 3, sidx = k % kFusedSlab;
+        const PmlDev<T>& p = a.pml[comp];
+        T b = 0, cd = 0, ki = 1;
+        if (p.psi[ax] && sidx < p.S[ax]) {
+            const int pos = sidx < p.lo[ax] ? sidx : p.hi0[ax] + (sidx - p.lo[ax]);
+            b = p.b[ax][pos];
+            cd = p.cd[ax][pos];
+            ki = p.kinv[ax][pos];
+        }
+        P.v[comp][ax][0][sidx] = b;
+        P.v[comp][ax][1][sidx] = cd;
+        P.v[comp][ax][2][sidx] = ki;
+    }
+}
+
+// Slab-compact position of pos on axis ax, or -1 outside the psi region.
+template <typename T>
+__device__ __forceinline__ int psi_pos(const PmlDev<T>& p, int ax, int pos) {
+    if (pos < p.lo[ax]) return pos;
+    if (pos >= p.hi0[ax] && pos < p.hi1[ax]) return p.lo[ax] + pos - p.hi0[ax];
+    return -1;
+}
+
+// Element index of the psi of the group at slab position s (s >= 0).
+template <typename T>
+__device__ __forceinline__ unsigned psi_index(const PmlDev<T>& p, int ax, int s, int x0, int y, int z, int nx, int ny) {
+    if (ax == 0) return static_cast<unsigned>(s) + static_cast<unsigned>(p.S[0]) * (static_cast<unsigned>(y) + static_cast<unsigned>(ny) * z);
+    if (ax == 1) return static_cast<unsigned>(x0) + static_cast<unsigned>(nx) * (static_cast<unsigned>(s) + static_cast<unsigned>(p.S[1]) * z);
+    return static_cast<unsigned>(x0) + static_cast<unsigned>(nx) * (static_cast<unsigned>(y) + static_cast<unsigned>(ny) * s);
+}
+
+// CPML term of (comp, ax) for a group at slab position sp (-1: none) whose
+// psi ps was loaded ahead, profiles from LDS; ps becomes the new psi (cells
+// outside `valid` keep the old one).  pml_group's arithmetic.
+template <typename T, int W>
+__device__ __forceinline__ void psi_term_l(const ProfLds<T>& P, int comp, int ax, int sp, const T (&dV)[W], T invd, unsigned valid,
+                                           Pk<T, W>& ps, T (&term)[W]) {
+    if (sp < 0) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) term[w] = dV[w] * invd;
+        return;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const int q = ax == 0 ? sp + w : sp;
+        const T psi = P.v[comp][ax][0][q] * ps.v[w] + P.v[comp][ax][1][q] * dV[w];
+        if (valid & (1u << w)) ps.v[w] = psi;
+        term[w] = P.v[comp][ax][2][q] * (dV[w] * invd) + psi;
+    }
+}
+
+// A y or z CPML term (rows / planes of the slabs, wave-uniform): load psi,
+// update, store.
+template <typename T, int W>
+__device__ __forceinline__ void psi_yz_l(const StepArgs<T>& a, const ProfLds<T>& P, int comp, int ax, int pos0, int x0, int y, int z,
+                                         const T (&dV)[W], T invd, unsigned valid, bool store, T (&term)[W]) {
+    const PmlDev<T>& p = a.pml[comp];
+    const int sp = psi_pos(p, ax, pos0);
+    Pk<T, W> ps = zpk<T, W>();
+    unsigned pi = 0;
+    if (sp >= 0) {
+        pi = psi_index(p, ax, sp, x0, y, z, a.nx, a.ny);
+        ps = ldv_nt<T, W>(p.psi_in[ax] + pi);
+    }
+    psi_term_l<T, W>(P, comp, ax, sp, dV, invd, valid, ps, term);
+    if (store && sp >= 0) stv_nt<T, W>(p.psi[ax] + pi, ps);
+}
+
+// The soft sources of one launch, staged in LDS at kernel entry with this
+// step's amplitudes: inside the fused kernel's loop, which stores to global
+// memory, uniform loads of the source table would become per-lane loads,
+// each waited for.
+template <typename T>
+struct SrcLds {
+    int n;
+    int comp[CHIML_MAX_SRC];
+    int box[CHIML_MAX_SRC][6];
+    T amp[CHIML_MAX_SRC];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_src_lds(const StepArgs<T>& a, SrcLds<T>& S, int step) {
+    const int tid = threadIdx.x;
+    if (tid == 0) S.n = min(a.nsrc, CHIML_MAX_SRC);
+    for (int k = tid; k < min(a.nsrc, CHIML_MAX_SRC); k += blockDim.x) {
+        S.comp[k] = a.src_comp[k];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) S.box[k][j] = a.src_box[6 * k + j];
+        S.amp[k] = a.src_amp[static_cast<long long>(step) * a.nsrc + k];
+    }
+}
+
+// group_has_src / src_value on the staged table
+template <typename T, int W>
+__device__ __forceinline__ bool group_has_src_l(const SrcLds<T>& S, int x0, int y, int z) {
+    for (int s = 0; s < S.n; ++s) {
+        const int* b = S.box[s];
+        if (x0 < b[1] && b[0] < x0 + W && y >= b[2] && y < b[3] && z >= b[4] && z < b[5]) return true;
+    }
+    return false;
+}
+template <typename T>
+__device__ __forceinline__ T src_value_l(const SrcLds<T>& S, int comp, int x, int y, int z) {
+    T v = 0;
+    for (int s = 0; s < S.n; ++s) {
+        if (S.comp[s] != comp) continue;
+        const int* b = S.box[s];
+        if (x >= b[0] && x < b[1] && y >= b[2] && y < b[3] && z >= b[4] && z < b[5]) v += S.amp[s];
+    }
+    return v;
+}
+
+// The group's new E of component c from its curl terms (e_finish_group's
+// arithmetic), with its material ids loaded ahead; the stores are the
+// caller's.  PRE: the pole-current sum was left in jsum by the pole-update
+// launch; otherwise the pole update runs here.  Cells outside `valid` keep
+// their old value.
+template <typename T, int W, bool PRE>
+__device__ __forceinline__ Pk<T, W> e_new_group(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], const SrcLds<T>& S, int c,
+                                                int x0, int y, int z, unsigned valid, const T (&t1)[W], const T (&t2)[W], bool src,
+                                                Pk<T, W> e, const Pk<uint8_t, W>& id) {
+    T jsum[W];
+    bool anyp = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        jsum[w] = T(0);
+        anyp |= (mt[c][id.v[w]].poleset >= 0) && (valid & (1u << w));
+    }
+    if (PRE && anyp) {
+        const int* bx = a.box[c];
+        const long long bi = (x0 - bx[0]) + static_cast<long long>(bx[1] - bx[0]) * ((y - bx[2]) + static_cast<long long>(bx[3] - bx[2]) * (z - bx[4]));
+        const Pk<T, W> js = ldv<T, W>(a.jsum[c] + bi);
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            if (mt[c][id.v[w]].poleset >= 0 && (valid & (1u << w))) jsum[w] = js.v[w];
+    } else if (!PRE && anyp) {
+        const int* bx = a.box[c];
+        const long long bi = (x0 - bx[0]) + static_cast<long long>(bx[1] - bx[0]) * ((y - bx[2]) + static_cast<long long>(bx[3] - bx[2]) * (z - bx[4]));
+        int np[W], npmax = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int ps = mt[c][id.v[w]].poleset;
+            np[w] = (ps >= 0 && (valid & (1u << w))) ? a.pole_count[ps] : 0;
+            npmax = max(npmax, np[w]);
+        }
+        for (int p = 0; p < npmax; ++p) {
+            T* Pp = a.P[c] + p * a.boxn[c] + bi;
+            T* Jp = a.J[c] + p * a.boxn[c] + bi;
+            Pk<T, W> Pv = ldv_nt<T, W>(Pp), J = ldv_nt<T, W>(Jp);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if (p >= np[w]) continue;
+                const int ps = mt[c][id.v[w]].poleset;
+                const T* co = a.pole_coef + (static_cast<long long>(ps) * CHIML_MAX_POLES + p) * 3;
+                const T sv = src ? src_value_l<T>(S, c, x0 + w, y, z) : T(0);
+                const T Jn = co[0] * Pv.v[w] + co[1] * J.v[w] + co[2] * (e.v[w] + sv);
+                Pv.v[w] = Pv.v[w] + Jn;
+                J.v[w] = Jn;
+                jsum[w] += Jn;
+            }
+            stv_nt<T, W>(Pp, Pv);
+            stv_nt<T, W>(Jp, J);
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        if (!(valid & (1u << w))) continue;
+        const MatLds<T> m = mt[c][id.v[w]];
+        const T sv = src ? src_value_l<T>(S, c, x0 + w, y, z) : T(0);
+        const T e_base = (m.flags & CHIML_MAT_DCELL) ? e.v[w] : e.v[w] + sv;
+        e.v[w] = e_base + m.ce * (t1[w] - t2[w]) - m.inveps * jsum[w];
+    }
+    return e;
+}
+
+// Loads and stores at a byte offset from a base held in scalar registers
+// (the 32-bit offset keeps one vector register per address).
+template <typename T, int W>
+__device__ __forceinline__ Pk<T, W> ldo(const T* base, unsigned off) {
+    return *reinterpret_cast<const Pk<T, W>*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename T, int W>
+__device__ __forceinline__ Pk<T, W> ldo_nt(const T* base, unsigned off) {
+    return ldv_nt<T, W>(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off));
+}
+template <typename T, int W>
+__device__ __forceinline__ void sto_nt(T* base, unsigned off, const Pk<T, W>& v) {
+    stv_nt<T, W>(reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off), v);
+}
+template <typename T>
+__device__ __forceinline__ T ld1(const T* base, unsigned off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+
+template <typename T, int W>
+struct FuseCarry {
+    Pk<T, W> ex, ey, ez;  // E at the plane being updated
+    Pk<T, W> hxp, hyp;    // new Hx, Hy one plane down
+};
+
+// New Hx and Hy of the group at a plane, not stored: the plane below a
+// segment (one value per own row).  h_group's arithmetic.
+template <typename T, int W, bool SRC>
+__device__ __forceinline__ void fused_h_below(const StepArgs<T>& a, const SrcLds<T>& S, const ProfLds<T>& P, const T (&chv)[3], unsigned cs,
+                                              int x0, int y, int z, Pk<T, W>& hx, Pk<T, W>& hy) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const unsigned ob = static_cast<unsigned>(lidx(x0, y, z, nx, ny)) * sizeof(T);
+    const unsigned sy = nx * sizeof(T), sz = static_cast<unsigned>(nx) * ny * sizeof(T);
+    const T* G = a.G[0];
+    const bool last = x0 + W >= nx;
+    const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;
+    const Pk<T, W> ex = ldo<T, W>(G, ob), ey = ldo<T, W>(G, ob + cs), ez = ldo<T, W>(G, ob + 2 * cs);
+    const Pk<T, W> exz = ldo<T, W>(G, ob + sz), eyz = ldo<T, W>(G, ob + cs + sz);
+    const Pk<T, W> ezy = y < ny - 1 ? ldo<T, W>(G, ob + 2 * cs + sy) : zpk<T, W>();
+    const T ezn = last ? T(0) : ld1<T>(G, ob + 2 * cs + W * sizeof(T));
+    hx = ldo_nt<T, W>(G, ob + 3 * cs);
+    hy = ldo_nt<T, W>(G, ob + 4 * cs);
+    const bool src = SRC && group_has_src_l<T, W>(S, x0, y, z);
+    T d1[W], d2[W], t1[W], t2[W];
+    if (y < ny - 1 && z < nz - 1) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = eyz.v[w] - ey.v[w];
+            d2[w] = ezy.v[w] - ez.v[w];
+        }
+        psi_yz_l<T, W>(a, P, 3, 2, z, x0, y, z, d1, a.inv_d[2], all, false, t1);
+        psi_yz_l<T, W>(a, P, 3, 1, y, x0, y, z, d2, a.inv_d[1], all, false, t2);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            T hv = hx.v[w] + chv[0] * (t1[w] - t2[w]);
+            if (SRC && src) hv += src_value_l<T>(S, 3, x0 + w, y, z);
+            hx.v[w] = hv;
+        }
+    }
+    if (z < nz - 1) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = (w + 1 < W ? ez.v[w + 1] : ezn) - ez.v[w];
+            d2[w] = exz.v[w] - ex.v[w];
+        }
+        psi_yz_l<T, W>(a, P, 4, 0, x0, x0, y, z, d1, a.inv_d[0], vx, false, t1);
+        psi_yz_l<T, W>(a, P, 4, 2, z, x0, y, z, d2, a.inv_d[2], vx, false, t2);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            if (!(vx & (1u << w))) continue;
+            T hv = hy.v[w] + chv[1] * (t1[w] - t2[w]);
+            if (SRC && src) hv += src_value_l<T>(S, 4, x0 + w, y, z);
+            hy.v[w] = hv;
+        }
+    }
+}
+
+// One plane of the fused step for one thread's group (row y, cells x0..).
+// Its memory traffic is in three phases, so that it waits for memory once:
+// every load it needs (E one plane up, E of the row above and of the next
+// group, old H, material ids, x-CPML psi) is issued first, from clamped
+// addresses and without branches; then H is formed and shared through LDS,
+// and after the block barrier E; every store comes last.  (One in-order
+// counter tracks loads and stores alike, so a load issued after a store
+// could not be waited for without waiting for the store.)  slab: the row
+// or plane lies in a y or z CPML slab (wave-uniform), whose psi terms load
+// and store in place; elsewhere those terms are plain differences.  The
+// field buffers hold the six components cstride bytes apart (a.G[0], a.F[0]).
+template <typename T, int W, bool SRC, bool PRE>
+__device__ __forceinline__ void fused_plane(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], const SrcLds<T>& S,
+                                            const ProfLds<T>& P, const T (&chv)[3], T* L, int plane, unsigned cs, int r, int x0,
+                                            int y, int z, bool row, bool own, bool work, bool slab, bool m_xpsi, FuseCarry<T, W>& st) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const unsigned sy = nx * sizeof(T), sz = static_cast<unsigned>(nx) * ny * sizeof(T);
+    const int yc = min(max(y, 0), ny - 1);  // a row of the domain for the (unused) loads of rows outside it
+    const long long i = lidx(x0, yc, z, nx, ny);
+    const unsigned ob = static_cast<unsigned>(i) * sizeof(T);
+    const T* G = a.G[0];
+    T* F = a.F[0];
+    const T idx = a.inv_d[0], idy = a.inv_d[1], idz = a.inv_d[2];
+    const bool last = x0 + W >= nx;
+    const unsigned all = (1u << W) - 1, vx = last ? (all >> 1) : all;  // cells with x < nx - 1
+    const bool yin = y < ny - 1, zin = z < nz - 1;
+    // ---- every load of the plane ----
+    const unsigned oz = zin ? sz : 0u, oy = yin ? sy : 0u, ox = last ? 0u : W * sizeof(T);
+    Pk<T, W> exz = ldo<T, W>(G, ob + oz), eyz = ldo<T, W>(G, ob + cs + oz), ezz = ldo<T, W>(G, ob + 2 * cs + oz);
+    const Pk<T, W> exy = ldo<T, W>(G, ob + oy), ezy = ldo<T, W>(G, ob + 2 * cs + oy);
+    T ezn = ld1<T>(G, ob + 2 * cs + ox), eyn = ld1<T>(G, ob + cs + ox);
+    Pk<T, W> hx = ldo_nt<T, W>(G, ob + 3 * cs), hy = ldo_nt<T, W>(G, ob + 4 * cs), hz = ldo_nt<T, W>(G, ob + 5 * cs);
+    const Pk<uint8_t, W> id0 = load_ids<W>(a.mat_e[0], i), id1 = load_ids<W>(a.mat_e[1], i), id2 = load_ids<W>(a.mat_e[2], i);
+    // x psi of Hy, Hz, Ey, Ez (slab-compact positions, -1 outside the x slabs)
+    const int sHy = zin ? psi_pos(a.pml[4], 0, x0) : -1, sHz = yin ? psi_pos(a.pml[5], 0, x0) : -1;
+    const int sEy = yin ? psi_pos(a.pml[1], 0, x0) : -1, sEz = zin ? psi_pos(a.pml[2], 0, x0) : -1;
+    const unsigned iHy = psi_index(a.pml[4], 0, max(sHy, 0), x0, yc, z, nx, ny), iHz = psi_index(a.pml[5], 0, max(sHz, 0), x0, yc, z, nx, ny);
+    const unsigned iEy = psi_index(a.pml[1], 0, max(sEy, 0), x0, yc, z, nx, ny), iEz = psi_index(a.pml[2], 0, max(sEz, 0), x0, yc, z, nx, ny);
+    const bool xpsi = m_xpsi;
+    Pk<T, W> psHy = zpk<T, W>(), psHz = zpk<T, W>(), psEy = zpk<T, W>(), psEz = zpk<T, W>();
+    if (xpsi) {
+        psHy = ldv_nt<T, W>(a.pml[4].psi_in[0] + iHy);
+        psHz = ldv_nt<T, W>(a.pml[5].psi_in[0] + iHz);
+        psEy = ldv_nt<T, W>(a.pml[1].psi[0] + iEy);
+        psEz = ldv_nt<T, W>(a.pml[2].psi[0] + iEz);
+    }
+    if (!zin) exz = eyz = ezz = zpk<T, W>();
+    if (last) ezn = eyn = T(0);
+    const bool src = SRC && row && group_has_src_l<T, W>(S, x0, y, z);
+    // ---- H ----
+    T d1[W], d2[W], t1[W], t2[W];
+    const Pk<T, W>&ex = st.ex, &ey = st.ey, &ez = st.ez;
+    if (yin && zin) {  // Hx += ch [dEy/dz - dEz/dy]
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = eyz.v[w] - ey.v[w];
+            d2[w] = ezy.v[w] - ez.v[w];
+        }
+        if (slab) {
+            psi_yz_l<T, W>(a, P, 3, 2, z, x0, yc, z, d1, idz, all, own, t1);
+            psi_yz_l<T, W>(a, P, 3, 1, y, x0, yc, z, d2, idy, all, own, t2);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                t1[w] = d1[w] * idz;
+                t2[w] = d2[w] * idy;
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            T hv = hx.v[w] + chv[0] * (t1[w] - t2[w]);
+            if (SRC && src) hv += src_value_l<T>(S, 3, x0 + w, y, z);
+            hx.v[w] = hv;
+        }
+    }
+    if (own && zin) {  // Hy += ch [dEz/dx - dEx/dz], x < nx - 1
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = (w + 1 < W ? ez.v[w + 1] : ezn) - ez.v[w];
+            d2[w] = exz.v[w] - ex.v[w];
+        }
+        psi_term_l<T, W>(P, 4, 0, sHy, d1, idx, vx, psHy, t1);
+        if (slab) {
+            psi_yz_l<T, W>(a, P, 4, 2, z, x0, yc, z, d2, idz, vx, true, t2);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) t2[w] = d2[w] * idz;
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            if (!(vx & (1u << w))) continue;
+            T hv = hy.v[w] + chv[1] * (t1[w] - t2[w]);
+            if (SRC && src) hv += src_value_l<T>(S, 4, x0 + w, y, z);
+            hy.v[w] = hv;
+        }
+    }
+    if (yin) {  // Hz += ch [dEx/dy - dEy/dx], x < nx - 1
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            d1[w] = exy.v[w] - ex.v[w];
+            d2[w] = (w + 1 < W ? ey.v[w + 1] : eyn) - ey.v[w];
+        }
+        if (slab) {
+            psi_yz_l<T, W>(a, P, 5, 1, y, x0, yc, z, d1, idy, vx, own, t1);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) t1[w] = d1[w] * idy;
+        }
+        psi_term_l<T, W>(P, 5, 0, sHz, d2, idx, vx, psHz, t2);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            if (!(vx & (1u << w))) continue;
+            T hv = hz.v[w] + chv[2] * (t1[w] - t2[w]);
+            if (SRC && src) hv += src_value_l<T>(S, 5, x0 + w, y, z);
+            hz.v[w] = hv;
+        }
+    }
+    if (!row) hx = hy = hz = zpk<T, W>();  // the zero ghost below the domain / past its last row
+    if (work) {
+        const int o = r * nx + x0;
+        stv<T, W>(L + o, hx);
+        stv<T, W>(L + plane + o, hy);
+        stv<T, W>(L + 2 * plane + o, hz);
+    }
+    __syncthreads();
+    // ---- E ----
+    Pk<T, W> en0 = st.ex, en1 = st.ey, en2 = st.ez;
+    if (own) {
+        const int o = r * nx + x0;
+        const Pk<T, W> hxy = ldv<T, W>(L + o - nx), hzy = ldv<T, W>(L + 2 * plane + o - nx);
+        const T hym = x0 > 0 ? L[plane + o - 1] : T(0), hzm = x0 > 0 ? L[2 * plane + o - 1] : T(0);
+        {  // Ex += ce [dHz/dy - dHy/dz], x < nx - 1
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                d1[w] = hz.v[w] - hzy.v[w];
+                d2[w] = hy.v[w] - st.hyp.v[w];
+            }
+            if (slab) {
+                psi_yz_l<T, W>(a, P, 0, 1, y, x0, y, z, d1, idy, vx, true, t1);
+                psi_yz_l<T, W>(a, P, 0, 2, z, x0, y, z, d2, idz, vx, true, t2);
+            } else {
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    t1[w] = d1[w] * idy;
+                    t2[w] = d2[w] * idz;
+                }
+            }
+            en0 = e_new_group<T, W, PRE>(a, mt, S, 0, x0, y, z, vx, t1, t2, src, st.ex, id0);
+        }
+        if (yin) {  // Ey += ce [dHx/dz - dHz/dx]
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                d1[w] = hx.v[w] - st.hxp.v[w];
+                d2[w] = hz.v[w] - (w > 0 ? hz.v[w - 1] : hzm);
+            }
+            if (slab) {
+                psi_yz_l<T, W>(a, P, 1, 2, z, x0, y, z, d1, idz, all, true, t1);
+            } else {
+#pragma unroll
+                for (int w = 0; w < W; ++w) t1[w] = d1[w] * idz;
+            }
+            psi_term_l<T, W>(P, 1, 0, sEy, d2, idx, all, psEy, t2);
+            en1 = e_new_group<T, W, PRE>(a, mt, S, 1, x0, y, z, all, t1, t2, src, st.ey, id1);
+        }
+        if (zin) {  // Ez += ce [dHy/dx - dHx/dy]
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                d1[w] = hy.v[w] - (w > 0 ? hy.v[w - 1] : hym);
+                d2[w] = hx.v[w] - hxy.v[w];
+            }
+            psi_term_l<T, W>(P, 2, 0, sEz, d1, idx, all, psEz, t1);
+            if (slab) {
+                psi_yz_l<T, W>(a, P, 2, 1, y, x0, y, z, d2, idy, all, true, t2);
+            } else {
+#pragma unroll
+                for (int w = 0; w < W; ++w) t2[w] = d2[w] * idy;
+            }
+            en2 = e_new_group<T, W, PRE>(a, mt, S, 2, x0, y, z, all, t1, t2, src, st.ez, id2);
+        }
+    }
+    // ---- every store of the plane ----
+    if (own) {
+        if (yin && zin) sto_nt<T, W>(F, ob + 3 * cs, hx);
+        if (zin) {
+            sto_nt<T, W>(F, ob + 4 * cs, hy);
+            if (sHy >= 0) stv_nt<T, W>(a.pml[4].psi[0] + iHy, psHy);
+        }
+        if (yin) {
+            sto_nt<T, W>(F, ob + 5 * cs, hz);
+            if (sHz >= 0) stv_nt<T, W>(a.pml[5].psi[0] + iHz, psHz);
+        }
+        sto_nt<T, W>(F, ob, en0);
+        if (yin) {
+            sto_nt<T, W>(F, ob + cs, en1);
+            if (sEy >= 0) stv_nt<T, W>(a.pml[1].psi[0] + iEy, psEy);
+        }
+        if (zin) {
+            sto_nt<T, W>(F, ob + 2 * cs, en2);
+            if (sEz >= 0) stv_nt<T, W>(a.pml[2].psi[0] + iEz, psEz);
+        }
+    }
+    st.hxp = hx;
+    st.hyp = hy;
+    st.ex = exz;
+    st.ey = eyz;
+    st.ez = ezz;
+}
+
+template <typename T, int W, bool SRC, bool PRE, bool ADV>
+__global__ void __launch_bounds__(kFusedMaxThreads) k_fused3(StepArgs<T> a, FuseMap m, int nm0, int nm1, int nm2) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    __shared__ SrcLds<T> S;
+    __shared__ ProfLds<T> P;
+    extern __shared__ __align__(16) unsigned char fused_lds[];
+    T* lds = reinterpret_cast<T*>(fused_lds);  // [2][3][(ty + 1) * nx]
+    const int nm[3] = {nm0, nm1, nm2};
+    const int step = SRC ? step_now(a.step, a.stepv) : 0;
+    if (SRC) load_src_lds<T>(a, S, step);
+    else if (threadIdx.x == 0) S.n = 0;
+    load_prof_lds<T>(a, P);
+    load_mat(a, mt, nm);  // (its barrier also publishes the source and profile tables)
+    const T chv[3] = {a.ch[0][0], a.ch[1][0], a.ch[2][0]};
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int item = static_cast<int>(xcd_block(blockIdx.x, gridDim.x));
+    const int tile = item / m.nseg, seg = item - tile * m.nseg;
+    const int z0 = seg * m.lz, z1 = min(z0 + m.lz, nz);
+    const int tid = threadIdx.x;
+    const bool work = tid < m.nthr;
+    const int r = work ? tid / m.gx : 0, g = work ? tid - r * m.gx : 0;
+    const int x0 = g * W, y = tile * m.ty - 1 + r;
+    const bool row = work && y >= 0 && y < ny;  // a row of the domain
+    const bool own = row && r > 0;              // a row this block updates (r = 0: the row below the tile)
+    const bool yslab = y < m.ys0 || y >= m.ys1;
+    const int plane = (m.ty + 1) * nx;
+    const unsigned cs = m.cstride;
+    FuseCarry<T, W> st;
+    st.ex = st.ey = st.ez = st.hxp = st.hyp = zpk<T, W>();
+    if (row) {
+        const unsigned ob = static_cast<unsigned>(lidx(x0, y, z0, nx, ny)) * sizeof(T);
+        st.ex = ldo<T, W>(a.G[0], ob);
+        st.ey = ldo<T, W>(a.G[0], ob + cs);
+        st.ez = ldo<T, W>(a.G[0], ob + 2 * cs);
+        if (own && z0 > 0) fused_h_below<T, W, SRC>(a, S, P, chv, cs, x0, y, z0 - 1, st.hxp, st.hyp);
+    }
+    for (int z = z0; z < z1; ++z) {
+        T* L = lds + (z & 1) * 3 * plane;
+        fused_plane<T, W, SRC, PRE>(a, mt, S, P, chv, L, plane, cs, r, x0, y, z, row, own, work, yslab || z < m.zs0 || z >= m.zs1,
+                                    m.xpsi != 0, st);
+    }
+    if (ADV) advance_counter(a);
+}
+
+// The pole update of the fused step, as a launch of its own before it
+// (ade_block over the pole boxes; it reads E before the step, a.F = a.G).
+template <typename T, int W, bool SRC>
+__global__ void __launch_bounds__(256) k_ade3(StepArgs<T> a, int ade_bpc) {
+    const int c = blockIdx.x / max(ade_bpc, 1);
+    if (c < 3) ade_block<T, W, SRC>(a, c, blockIdx.x - c * ade_bpc);
+}
+
 template <typename T>
 struct SampleArgs {
     const T* F[6];
@@ -1173,9 +1732,11 @@ __global__ void k_sample(SampleArgs<T> a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int step = step_now(a.step, a.stepv);
     if (t < a.totpts && a.det_all1) {
-        // the cells do not depend on the detector: their loads overlap its search
-        store_point(a, t, static_cast<double>(a.F[a.det_comp[2 * t]][a.det_cell[2 * t]]),
-                    static_cast<double>(a.F[a.det_comp[2 * t + 1]][a.det_cell[2 * t + 1]]), step);
+        // the cells do not depend on the detector: their loads overlap its
+        // search; a cell of -1 is the zero ghost beyond the domain
+        const long long cu = a.det_cell[2 * t], cv = a.det_cell[2 * t + 1];
+        store_point(a, t, cu >= 0 ? static_cast<double>(a.F[a.det_comp[2 * t]][cu]) : 0.0,
+                    cv >= 0 ? static_cast<double>(a.F[a.det_comp[2 * t + 1]][cv]) : 0.0, step);
     } else if (t < a.totpts) {
         int d = 0;  // the detector of point t
         while (d + 1 < a.ndet && a.det_off[d + 1] <= t) ++d;
@@ -1602,6 +2163,11 @@ struct chiml_dev {
     bool device_flush = false;  // launch the DFT flush every step; it decides on the device
     hipGraphExec_t gexec = nullptr;
     bool split = false;  // 3D interior + CPML-slab kernels
+    bool fused = false;  // 3D: one fused H + E launch per step on ping-pong buffers
+    FuseMap fmap{};
+    size_t fused_lds = 0;  // dynamic LDS of the fused launch (bytes)
+    void* fbuf[2][6] = {};      // fused: the two field buffers (fbuf[0] is the build's F)
+    void* fpsi[2][3][3] = {};   // fused: the two psi buffers of each H component and axis
     bool vec2d = false;  // 2D TMz W-cell group kernels
     bool fuse_sample = false;  // 2D: the E kernel takes the point detectors (no DFT, few single-cell points)
     bool det_single_field = true;  // every point detector reads one field (two cells) per point
@@ -1657,9 +2223,15 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         a.Tp[k] = p->pml_T[k];
     }
     const bool is3 = nz > 1;
-    for (int k = 0; k < 6; ++k) {
-        bool present = is3 || k == 2 || k == 3 || k == 4;
-        a.F[k] = present ? c->zeros<T>(N) : nullptr;
+    {  // the present components in one allocation, x + nx (y + ny z) each, in component order
+        int nf = 0;
+        for (int k = 0; k < 6; ++k) nf += (is3 || k == 2 || k == 3 || k == 4) ? 1 : 0;
+        T* base = c->zeros<T>(N * nf);
+        for (int k = 0; k < 6; ++k) {
+            bool present = is3 || k == 2 || k == 3 || k == 4;
+            a.F[k] = present ? base : nullptr;
+            if (present) base += N;
+        }
     }
     for (int k = 0; k < 3; ++k) {
         bool present = is3 || k == 2;
@@ -1759,6 +2331,9 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         a.src_box = c->upload<int>(p->src_box, static_cast<size_t>(p->nsrc) * 6);
         a.src_amp = upload_cast<T>(c, p->src_amp, static_cast<size_t>(p->nsteps) * p->nsrc);
     }
+    for (int k = 0; k < 6; ++k) a.G[k] = a.F[k];
+    for (int comp = 0; comp < 6; ++comp)
+        for (int ax = 0; ax < 3; ++ax) a.pml[comp].psi_in[ax] = a.pml[comp].psi[ax];
     a.step = c->step_ctr.as<int>();
     a.stepv = -1;
     a.step_next = nullptr;
@@ -1907,13 +2482,26 @@ void launch_sample(chiml_dev* c, SampleArgs<T>& s) {
     hipLaunchKernelGGL(k_sample<T>, dim3((tot + 255) / 256), dim3(256), 0, c->stream, s);
 }
 
+// Is kernel n one of the comma-separated names in `list`?
+bool in_name_list(const std::string& list, const char* n) {
+    const size_t len = std::strlen(n);
+    size_t p = 0;
+    while (p <= list.size()) {
+        size_t q = list.find(',', p);
+        if (q == std::string::npos) q = list.size();
+        if (q - p == len && list.compare(p, len, n) == 0) return true;
+        p = q + 1;
+    }
+    return false;
+}
+
 struct Timed {
     chiml_dev* c;
     const char* name;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool on;
     Timed(chiml_dev* c_, const char* n)
-        : c(c_), name(n), on(c_->profile && !c_->device_flush && (c_->profile_only.empty() || c_->profile_only == n)) {
+        : c(c_), name(n), on(c_->profile && !c_->device_flush && (c_->profile_only.empty() || in_name_list(c_->profile_only, n))) {
         if (on) {
             (void)hipEventCreate(&e0);
             (void)hipEventCreate(&e1);
@@ -1967,6 +2555,32 @@ void pick2(bool s, bool adv, F&& f) {
 template <typename T, int DIM>
 void launch_fields(chiml_dev* c, StepArgs<T>& a) {
     const bool adv = a.step_next != nullptr;
+    if (DIM == 3 && c->fused) {
+        constexpr int W = 16 / sizeof(T);
+        const int ade_bpc = static_cast<int>((c->ade_groups + 255) / 256);
+        const bool pre = ade_bpc > 0;
+        if (pre) {  // pole update from E before the step
+            StepArgs<T> aa = a;
+            for (int k = 0; k < 3; ++k) aa.F[k] = const_cast<T*>(a.G[k]);
+            Timed t(c, "update_ade");
+            if (c->has_src) hipLaunchKernelGGL((k_ade3<T, W, true>), dim3(3 * ade_bpc), dim3(256), 0, c->stream, aa, ade_bpc);
+            else hipLaunchKernelGGL((k_ade3<T, W, false>), dim3(3 * ade_bpc), dim3(256), 0, c->stream, aa, ade_bpc);
+        }
+        const FuseMap& m = c->fmap;
+        const dim3 g(static_cast<unsigned>(m.ntile * m.nseg)), blk(static_cast<unsigned>((m.nthr + 63) / 64 * 64));
+        Timed t(c, "update_fused");
+        auto go = [&](auto S, auto P, auto A) {
+            hipLaunchKernelGGL((k_fused3<T, W, decltype(S)::value, decltype(P)::value, decltype(A)::value>), g, blk, c->fused_lds,
+                               c->stream, a, m, c->nmat[0], c->nmat[1], c->nmat[2]);
+        };
+        using Y = std::true_type;
+        using N = std::false_type;
+        pick2(c->has_src, adv, [&](auto S, auto A) {
+            if (pre) go(S, Y{}, A);
+            else go(S, N{}, A);
+        });
+        return;
+    }
     if (DIM == 3 && c->split) {
         constexpr int W = 16 / sizeof(T);
         const Box3 b = c->ibox;
@@ -2091,12 +2705,32 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
 // the end.  Steps being captured into a graph read the two-slot device
 // counter instead (advance_counter), so that one graph serves every replay
 // that starts on the same slot.
+// Fused 3D step: step st reads the buffers of parity st & 1 and writes the
+// other ones (fields and H psi); everything launched after the field update
+// (emitters, detectors, DFT) and every read between steps sees the written
+// ones.  The current buffers after k steps are those of parity k & 1.
+template <typename T>
+void set_fused_buffers(chiml_dev* c, StepArgs<T>& a, SampleArgs<T>& s, int st) {
+    const int p = st & 1;
+    for (int k = 0; k < 6; ++k) {
+        a.G[k] = static_cast<const T*>(c->fbuf[p][k]);
+        a.F[k] = static_cast<T*>(c->fbuf[p ^ 1][k]);
+        s.F[k] = a.F[k];
+    }
+    for (int h = 0; h < 3; ++h)
+        for (int ax = 0; ax < 3; ++ax) {
+            a.pml[3 + h].psi_in[ax] = static_cast<const T*>(c->fpsi[p][h][ax]);
+            a.pml[3 + h].psi[ax] = static_cast<T*>(c->fpsi[p ^ 1][h][ax]);
+        }
+}
+
 template <typename T>
 void do_steps(chiml_dev* c, StepArgs<T>& a, SampleArgs<T>& s, int n) {
     const bool host = !c->capturing;
     int* ctr = c->step_ctr.as<int>();
     for (int k = 0; k < n; ++k) {
         const int st = c->steps_done + k, p = st & 1;
+        if (c->fused) set_fused_buffers<T>(c, a, s, st);
         a.stepv = host ? st : -1;
         s.stepv = host ? st + 1 : -1;
         a.step = ctr + p;
@@ -2118,6 +2752,8 @@ void do_steps(chiml_dev* c, StepArgs<T>& a, SampleArgs<T>& s, int n) {
     s.stepv = -1;
     a.step_next = nullptr;
     c->steps_done += n;
+    // between steps F (and the sample kernel's fields) are the current buffers
+    if (c->fused) set_fused_buffers<T>(c, a, s, c->steps_done - 1);
 }
 
 void collect_profile(chiml_dev* c) {
@@ -2134,6 +2770,95 @@ void collect_profile(chiml_dev* c) {
     c->pending.clear();
 }
 
+}  // namespace
+
+namespace {
+// Integer knob from the environment (tuning sweeps of the fused launch
+// shape; 0 or unset keeps the default).
+int env_int(const char* name) {
+    const char* v = std::getenv(name);
+    return v ? std::atoi(v) : 0;
+}
+}  // namespace
+
+// Fused 3D step (k_fused3): tiles of ty whole x rows (plus the recomputed
+// row below), segments of lz planes, one tile-segment per block; enough
+// segments for about two blocks per CU.  Needs the whole x row of a tile
+// plane in one block and a second buffer for every field and H psi array.
+// It moves the compulsory bytes only (1.10x of them on the 256^3 deck) but
+// runs at one 9-wave block per CU with a barrier per plane, and measured
+// 0.39 ms per step against 0.29 ms for the split sweeps, so it is opt-in:
+// CHIML_FUSED=1 selects it (both do the whole step);
+// CHIML_FUSED_TY / CHIML_FUSED_LZ set the tile shape.
+template <typename T>
+void enable_fused(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>& s) {
+    constexpr int W = 16 / sizeof(T);
+    const char* on = std::getenv("CHIML_FUSED");
+    if (!on || std::atoi(on) != 1) return;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    if (nx % W != 0 || p->pml_T[0] % W != 0 || c->hmap) return;
+    FuseMap m{};
+    m.gx = nx / W;
+    int ty = env_int("CHIML_FUSED_TY");
+    if (ty <= 0) ty = 8;
+    ty = std::min(ty, kFusedMaxThreads / m.gx - 1);
+    if (ty < 1) return;
+    m.ty = ty;
+    m.ntile = (ny + ty - 1) / ty;
+    int lz = env_int("CHIML_FUSED_LZ");
+    if (lz <= 0) {
+        const int nseg = std::max(1, std::min(nz, (512 + m.ntile - 1) / m.ntile));
+        lz = (nz + nseg - 1) / nseg;
+    }
+    m.lz = std::min(lz, nz);
+    m.nseg = (nz + m.lz - 1) / m.lz;
+    m.nthr = (ty + 1) * m.gx;
+    // rows and planes inside every component's y / z psi-free range
+    m.ys0 = 0; m.ys1 = ny; m.zs0 = 0; m.zs1 = nz;
+    for (int comp = 0; comp < 6; ++comp) {
+        const PmlDev<T>& pm = a.pml[comp];
+        if (pm.psi[1]) { m.ys0 = std::max(m.ys0, pm.lo[1]); m.ys1 = std::min(m.ys1, pm.hi0[1]); }
+        if (pm.psi[2]) { m.zs0 = std::max(m.zs0, pm.lo[2]); m.zs1 = std::min(m.zs1, pm.hi0[2]); }
+    }
+    for (int comp = 0; comp < 6; ++comp)
+        for (int ax = 0; ax < 3; ++ax)
+            if (a.pml[comp].psi[ax] && a.pml[comp].S[ax] > kFusedSlab) return;  // profiles must fit the LDS table
+    const int nxp = (a.pml[4].psi[0] != nullptr) + (a.pml[5].psi[0] != nullptr) + (a.pml[1].psi[0] != nullptr) + (a.pml[2].psi[0] != nullptr);
+    if (nxp != 0 && nxp != 4) return;
+    m.xpsi = nxp == 4 ? 1 : 0;
+    const size_t lds = 2ull * 3 * (ty + 1) * nx * sizeof(T);
+    int lds_max = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    if (lds + sizeof(MatLds<T>) * 3 * CHIML_MAX_MAT + sizeof(SrcLds<T>) + sizeof(ProfLds<T>) > static_cast<size_t>(lds_max)) return;
+    const size_t N = static_cast<size_t>(nx) * ny * nz;
+    // byte offsets within a buffer of six components are 32-bit in the kernel
+    if (6 * N * sizeof(T) >= (1ull << 32)) return;
+    T* second = c->zeros<T>(6 * N);
+    for (int k = 0; k < 6; ++k) {
+        c->fbuf[0][k] = a.F[k];
+        c->fbuf[1][k] = second + k * N;
+    }
+    m.cstride = static_cast<unsigned>(N * sizeof(T));
+    const int nn[3] = {nx, ny, nz};
+    for (int h = 0; h < 3; ++h)
+        for (int ax = 0; ax < 3; ++ax) {
+            const PmlDev<T>& pm = a.pml[3 + h];
+            c->fpsi[0][h][ax] = pm.psi[ax];
+            c->fpsi[1][h][ax] = nullptr;
+            if (pm.psi[ax]) {
+                size_t ps = static_cast<size_t>(pm.S[ax]);
+                for (int o = 0; o < 3; ++o)
+                    if (o != ax) ps *= nn[o];
+                c->fpsi[1][h][ax] = c->zeros<T>(ps);
+            }
+        }
+    c->fmap = m;
+    c->fused_lds = lds;
+    c->fused = true;
+    set_fused_buffers<T>(c, a, s, -1);
+}
+
+namespace {
 int fail(chiml_dev* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     g_err = msg;
@@ -2201,6 +2926,7 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
             if (b.x1 > b.x0 && b.y1 > b.y0 && b.z1 > b.z0 && (Tx % W) == 0 && (p->nx % W) == 0 && !c->hmap) {
                 c->split = true;
                 c->ibox = b;
+                if (c->prec == 32) enable_fused<float>(c.get(), p, c->a32, c->s32);
                 SlabMap& m = c->slab;
                 m.nx = p->nx; m.ny = p->ny; m.nz = p->nz;
                 m.Tx = Tx; m.Ty = Ty; m.Tz = Tz;

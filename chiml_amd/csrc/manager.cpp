@@ -609,12 +609,6 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     std::vector<int> dInt, dNp, dComp, dPow, dNf;
     std::vector<long long> dCell;
     std::vector<double> dConv;
-    auto cellIdx = [&](int x, int y, int z) -> long long {
-        x = std::max(0, std::min(x, nx - 1));
-        y = std::max(0, std::min(y, ny - 1));
-        z = std::max(0, std::min(z, nz - 1));
-        return x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
-    };
     for (auto& D : IP.dtcs) {
         int iv = static_cast<int>(std::floor(D.timeInterval / dt));
         if (iv == 0) throw std::logic_error("The time step of a detector is less than the main grid or set to 0.");
@@ -641,7 +635,9 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         std::array<int, 3> sz = D.sz;
         if (!is3D) sz[2] = 1;
         int np = 0;
-        // an offset cell beyond the domain is the reference's zero ghost (-1)
+        // a point or offset cell beyond the domain is the reference's zero
+        // ghost (-1): a detector whose rounded extent runs one cell past the
+        // upper face reads 0 there, as the ghost border of parallelGrid does
         auto ghostOr = [&](int x, int y, int z) -> long long {
             if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return -1;
             return x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
@@ -653,7 +649,7 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
                     for (auto& f : fl) {
                         dComp.push_back(f.first);
                         dComp.push_back(f.first);
-                        dCell.push_back(cellIdx(x, y, z));
+                        dCell.push_back(ghostOr(x, y, z));
                         dCell.push_back(ghostOr(x + f.second[0], y + f.second[1], z + f.second[2]));
                     }
                     ++np;
@@ -1028,6 +1024,24 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
                             kbytes[inInt(x, y, z) ? "update_e" : "update_e_pml"] += pcount[ps] * 4.0 * s;
                         }
                     }
+        }
+        // the opt-in fused step (3D FP32 with the split layout): one pass that
+        // reads and writes every field word once, three material bytes per
+        // cell, every psi word; the pole update as a launch of its own before it
+        if (split && rows) {
+            kbytes["update_fused"] = cells * (2.0 * (nE + nH) * s + nE) + psiWords * 2.0 * s;
+            kbytes["update_ade"] = 0.0;
+            for (int c : ecomps) {
+                const int* b = dispBox[c];
+                for (int z = b[4]; z < b[5]; ++z)
+                    for (int y = b[2]; y < b[3]; ++y)
+                        for (int x = b[0]; x < b[1]; ++x) {
+                            int ps = psT[c][matE[c][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)]];
+                            if (ps < 0) continue;
+                            kbytes["update_ade"] += pcount[ps] * 4.0 * s + 2.0 * s;
+                            kbytes["update_fused"] += s;  // the pole-current sum read back
+                        }
+            }
         }
         kbytes["ml_density"] = bytesML;
         // The compulsory bytes of one whole Yee + CPML + ADE step if every state

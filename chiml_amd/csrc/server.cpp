@@ -74,8 +74,10 @@ struct chiml_server {
         for (auto it = queue.begin(); it != queue.end();) {
             Request* r = *it;
             if (admitted >= max_active) break;
-            if (held + r->bytes <= budget || (admitted == 0 && held == 0)) {
-                // an oversized request still runs when it would be alone
+            if (held + r->bytes <= budget || admitted == 0) {
+                // a request still runs when nothing else does: an oversized
+                // one, or one that waits only for finished results the caller
+                // has not taken yet (so drain() always completes)
                 for (auto jt = queue.begin(); jt != it; ++jt) ++(*jt)->passed;
                 held += r->bytes;
                 ++admitted;

@@ -132,9 +132,11 @@ def gold_like_pols(a=A_UNIT):
 
 
 def config2_3d(n=256, steps=1000, pml=16, d=0.02, sphere_cells=40, flux_cells=120, nfreq=40,
-               eps_bg=(2.0, 2.25, 2.5), precision="fp32", with_flux=True, with_source=True):
+               eps_bg=(2.0, 2.25, 2.5), precision="fp32", with_flux=True, with_source=True, ndet=0):
     """3D Yee with anisotropic background, Drude+2 Lorentz sphere, soft
-    plane-wave sheet and a DFT flux box."""
+    plane-wave sheet and a DFT flux box; ndet > 0 adds point detectors of Ex,
+    Ey, Ez, Hx, Hy, Hz in turn on the sheet side of the sphere, between the
+    sheet and the sphere's surface."""
     deck, size, dt = _cell((n, n, n), d, 0.5, steps, pml, precision)
     deck["ObjectList"].append({
         "shape": "block", "material": "custom", "eps": eps_bg[1], "eps_diag": list(eps_bg), "pols": [],
@@ -151,6 +153,13 @@ def config2_3d(n=256, steps=1000, pml=16, d=0.02, sphere_cells=40, flux_cells=12
             "pol": "Ex", "size": [inner, inner, 0.0],
             "loc": [0.0, 0.0, _pos(pml + 4, n, d)],
             "PulseList": [gaussian_pulse(fcen, 2.0 * fcen, 3.0, 1.0)],
+        })
+    comps = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]
+    zc = max(pml + 6, n // 2 - sphere_cells - 3)
+    for k in range(ndet):
+        deck["DetectorList"].append({
+            "type": comps[k % 6], "dtc_class": "cout", "fname": "dtc", "size": [0.0, 0.0, 0.0],
+            "loc": [_pos(n // 2 + 3 * (k // 6) + 1, n, d), _pos(n // 2 + 2 * (k % 6) - 5, n, d), _pos(zc, n, d)],
         })
     if with_flux:
         fs = (flux_cells - 1) * d

@@ -177,8 +177,9 @@ int chiml_dev_steps_done(chiml_dev_t* ctx, int* steps);
 /* Kernel timing: HIP events on the context's stream around every launch
  * (enable = 1), accumulated per kernel name until read; resets the totals. */
 int chiml_dev_profile(chiml_dev_t* ctx, int enable);
-/* As chiml_dev_profile(ctx, 1) but timing only the launches of one kernel,
- * so that a timed run carries two event records per step at most. */
+/* As chiml_dev_profile(ctx, 1) but timing only the launches of the named
+ * kernels (one name, or several separated by commas), so that a timed run
+ * carries few event records per step. */
 int chiml_dev_profile_only(chiml_dev_t* ctx, const char* kernel);
 int chiml_dev_profile_read(chiml_dev_t* ctx, const char* kernel, double* total_ms, int* launches);
 /* Device memory the context holds (bytes). */

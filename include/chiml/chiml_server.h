@@ -45,7 +45,10 @@ int chiml_server_poll(chiml_server_t* s, long long id, int* state, double* admit
 int chiml_server_take(chiml_server_t* s, long long id, chiml_sim_t** sim);
 /* Failure message of a request (state CHIML_REQ_FAILED). */
 int chiml_server_error(chiml_server_t* s, long long id, char* buf, size_t cap);
-/* Blocks until nothing is queued, building or running. */
+/* Blocks until nothing is queued, building or running.  Finished results
+ * count against the budget until they are taken; when nothing is running the
+ * next queued request is admitted regardless, so drain() completes even when
+ * untaken results fill the budget (the device then holds more than it). */
 int chiml_server_drain(chiml_server_t* s);
 /* Totals: cell-updates completed, bytes held, requests active/queued now and
  * the largest number of simulations that were stepping at the same time. */

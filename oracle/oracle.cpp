@@ -96,6 +96,19 @@ static void dcopy(int n, const double* x, int incx, double* y, int incy) {
     for (int i = 0; i < n; ++i) y[i * incy] = x[i * incx];
 }
 
+// Worker threads for the update lists (oracle_set_threads).  Every entry of
+// one list writes its own cells, and the lists keep their order, so each
+// cell sees the same operations in the same order as a serial pass: results
+// are bit-identical for any thread count.  The lists are built y-outermost,
+// so a static split of a list gives each thread a y slab, as the reference's
+// MPI decomposition does (without the halo copies: the threads share memory).
+static int g_threads = 1;
+template <typename F>
+static void par_for(size_t n, F&& f) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1 && n > 16)
+    for (long long i = 0; i < static_cast<long long>(n); ++i) f(static_cast<size_t>(i));
+}
+
 // A reference parallelGrid for one rank: ghost border of one point,
 // index x + lx*(z + y*lz) (src/GRID/parallelGrid.hpp:563).
 template <typename T>
@@ -406,25 +419,28 @@ struct CPML {
 
     // addPsi / addGridOnly / updatePsiField, src/PML/parallelPML.cpp:12-40.
     static void updatePsiField(const std::vector<PsiParam>& l, DGrid& psi, DGrid& grid) {
-        for (auto& q : l) {
+        par_for(l.size(), [&](size_t e) {
+            const PsiParam& q = l[e];
             dscal(q.transSz, q.b, psi.p(q.ind), q.stride);
             daxpy(q.transSz, q.c, grid.p(q.ind), q.stride, psi.p(q.ind), q.stride);
             daxpy(q.transSz, -1.0 * q.c, grid.p(q.indOff), q.stride, psi.p(q.ind), q.stride);
-        }
+        });
     }
     static void addPsi(const std::vector<GridParam>& gl, const std::vector<PsiParam>& pl, DGrid& gi, DGrid& psi, DGrid& grid) {
         updatePsiField(pl, psi, grid);
-        for (auto& q : gl) {
+        par_for(gl.size(), [&](size_t e) {
+            const GridParam& q = gl[e];
             daxpy(q.nAx, q.DbField, grid.p(q.ind), q.stride, gi.p(q.ind), q.stride);
             daxpy(q.nAx, -1.0 * q.DbField, grid.p(q.indOff), q.stride, gi.p(q.ind), q.stride);
             daxpy(q.nAx, q.Db, psi.p(q.ind), q.stride, gi.p(q.ind), q.stride);
-        }
+        });
     }
     static void addGridOnly(const std::vector<GridParam>& gl, DGrid& gi, DGrid& grid) {
-        for (auto& q : gl) {
+        par_for(gl.size(), [&](size_t e) {
+            const GridParam& q = gl[e];
             daxpy(q.nAx, q.DbField, grid.p(q.ind), q.stride, gi.p(q.ind), q.stride);
             daxpy(q.nAx, -1.0 * q.DbField, grid.p(q.indOff), q.stride, gi.p(q.ind), q.stride);
-        }
+        });
     }
     void updateGrid() {
         if (psi_j.ok()) addPsi(gridK, psiJ, *gi, psi_j, *gk);
@@ -1070,16 +1086,18 @@ void Sim::fluxFieldIn() {
         std::vector<cplx> fft(nf);
         for (int f = 0; f < nf; ++f) fft[f] = std::exp(cplx(0.0, -1.0 * tcur * fx.freqs[f]));
         auto acc = [&](FreqStore& s) {
-            int idx = 0;
-            for (int kk = 0; kk < s.lsz[2]; ++kk)
-                for (int jj = 0; jj < s.lsz[1]; ++jj)
-                    for (int ii = 0; ii < s.lsz[0]; ++ii, ++idx) {
-                        double u = s.grid->at(s.lloc[0] + ii, s.lloc[1] + jj, s.lloc[2] + kk);
-                        for (int f = 0; f < nf; ++f) {
-                            s.fre[idx * nf + f] += fft[f].real() * u;
-                            s.fim[idx * nf + f] += fft[f].imag() * u;
-                        }
+            // one (kk, jj) line of the store per work item; idx runs x fastest
+            par_for(static_cast<size_t>(s.lsz[2]) * s.lsz[1], [&](size_t line) {
+                const int kk = static_cast<int>(line / s.lsz[1]), jj = static_cast<int>(line % s.lsz[1]);
+                int idx = static_cast<int>(line) * s.lsz[0];
+                for (int ii = 0; ii < s.lsz[0]; ++ii, ++idx) {
+                    double u = s.grid->at(s.lloc[0] + ii, s.lloc[1] + jj, s.lloc[2] + kk);
+                    for (int f = 0; f < nf; ++f) {
+                        s.fre[idx * nf + f] += fft[f].real() * u;
+                        s.fim[idx * nf + f] += fft[f].imag() * u;
                     }
+                }
+            });
         };
         for (auto& face : fx.faces) {
             for (auto& s : face.Ej) acc(s);
@@ -1451,9 +1469,9 @@ void Sim::step() {
         if (!H[ii].ok()) continue;
         DGrid* gj = E[(ii + 1) % 3].ok() ? &E[(ii + 1) % 3] : nullptr;
         DGrid* gk = E[(ii + 2) % 3].ok() ? &E[(ii + 2) % 3] : nullptr;
-        for (auto& r : upH[ii]) {
-            auto& u = r.up;
-            auto& pf = r.pref;
+        par_for(upH[ii].size(), [&](size_t e) {
+            auto& u = upH[ii][e].up;
+            auto& pf = upH[ii][e].pref;
             if (gj) {
                 daxpy(u[0], pf[2], gj->p(u[1]), 1, H[ii].p(u[1]), 1);
                 daxpy(u[0], -1.0 * pf[2], gj->p(u[4]), 1, H[ii].p(u[1]), 1);
@@ -1462,7 +1480,7 @@ void Sim::step() {
                 daxpy(u[0], -1.0 * pf[1], gk->p(u[1]), 1, H[ii].p(u[1]), 1);
                 daxpy(u[0], pf[1], gk->p(u[3]), 1, H[ii].p(u[1]), 1);
             }
-        }
+        });
     }
     for (int ii = 0; ii < 3; ++ii)
         if (HPML[ii] && H[ii].ok()) HPML[ii]->updateGrid();
@@ -1482,26 +1500,29 @@ void Sim::step() {
     // updatePolE: Lorentz ADE on the upLorD runs (E before the D update).
     for (int ii = 0; ii < 3; ++ii) {
         if (!E[ii].ok()) continue;
-        for (auto& r : upLorD[ii]) {
+        par_for(upLorD[ii].size(), [&](size_t e) {
+            const Run& r = upLorD[ii][e];
             const Object& o = *IP.objs[r.up[5]];
+            thread_local std::vector<double> scr;  // the reference's scratch row, one per thread
+            if (scr.size() < scratch.size()) scr.resize(scratch.size());
             for (size_t pp = 0; pp < o.alpha.size(); ++pp) {
                 double* P = lorP[ii][pp].p(r.up[1]);
                 double* Pp = prevLorP[ii][pp].p(r.up[1]);
-                dcopy(r.up[0], P, 1, scratch.data(), 1);
+                dcopy(r.up[0], P, 1, scr.data(), 1);
                 dscal(r.up[0], o.alpha[pp], P, 1);
                 daxpy(r.up[0], o.xi[pp], Pp, 1, P, 1);
                 daxpy(r.up[0], o.gamma[pp], E[ii].p(r.up[1]), 1, P, 1);
-                dcopy(r.up[0], scratch.data(), 1, Pp, 1);
+                dcopy(r.up[0], scr.data(), 1, Pp, 1);
             }
-        }
+        });
     }
     // updateD and updateE (TwoCompCurl / OneCompCurlJ / OneCompCurlK).
     auto curlE = [&](std::vector<Run>& lst, DGrid& U, int ii) {
         DGrid* gj = H[(ii + 1) % 3].ok() ? &H[(ii + 1) % 3] : nullptr;
         DGrid* gk = H[(ii + 2) % 3].ok() ? &H[(ii + 2) % 3] : nullptr;
-        for (auto& r : lst) {
-            auto& u = r.up;
-            auto& pf = r.pref;
+        par_for(lst.size(), [&](size_t e) {
+            auto& u = lst[e].up;
+            auto& pf = lst[e].pref;
             if (gj) {
                 daxpy(u[0], pf[2], gj->p(u[1]), 1, U.p(u[1]), 1);
                 daxpy(u[0], -1.0 * pf[2], gj->p(u[4]), 1, U.p(u[1]), 1);
@@ -1510,7 +1531,7 @@ void Sim::step() {
                 daxpy(u[0], -1.0 * pf[1], gk->p(u[1]), 1, U.p(u[1]), 1);
                 daxpy(u[0], pf[1], gk->p(u[3]), 1, U.p(u[1]), 1);
             }
-        }
+        });
     };
     for (int ii = 0; ii < 3; ++ii)
         if (D[ii].ok()) curlE(upD[ii], D[ii], ii);
@@ -1521,12 +1542,13 @@ void Sim::step() {
     // D2E: E = D/eps - sum_p P_p/eps on the upLorD runs.
     for (int ii = 0; ii < 3; ++ii) {
         if (!E[ii].ok()) continue;
-        for (auto& r : upLorD[ii]) {
+        par_for(upLorD[ii].size(), [&](size_t e) {
+            const Run& r = upLorD[ii][e];
             double ep = r.pref[3];
             dcopy(r.up[0], D[ii].p(r.up[1]), 1, E[ii].p(r.up[1]), 1);
             dscal(r.up[0], 1.0 / ep, E[ii].p(r.up[1]), 1);
             for (size_t pp = 0; pp < lorP[ii].size(); ++pp) daxpy(r.up[0], -1.0 / ep, lorP[ii][pp].p(r.up[1]), 1, E[ii].p(r.up[1]), 1);
-        }
+        });
     }
     addQE();
     tcur += dt;
@@ -1561,6 +1583,9 @@ void* oracle_create(const char* json) {
 }
 const char* oracle_last_error() { return g_oracle_err.c_str(); }
 void oracle_destroy(void* h) { delete static_cast<OracleHandle*>(h); }
+// Worker threads of every oracle in the process (1 = serial); results do not
+// depend on it.
+void oracle_set_threads(int n) { orc::g_threads = n < 1 ? 1 : n; }
 
 int oracle_step(void* hp, int nsteps) {
     auto* h = static_cast<OracleHandle*>(hp);

@@ -51,10 +51,16 @@ def lib():
                                        ctypes.POINTER(ctypes.c_double)]
         L.oracle_basis_dipole.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_pulse.restype = ctypes.c_double
         L.oracle_pulse.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    """Worker threads of the oracle's update lists (results do not depend on it)."""
+    lib().oracle_set_threads(int(n))
 
 
 def _dp(a):

@@ -141,6 +141,27 @@ def test_3d_split_kernels(variant, precision, tol_f, tol_flux):
     assert err <= tol_flux
 
 
+@pytest.mark.parametrize("variant", ["cube", "wide", "thinz", "pmlsrc"])
+def test_3d_fused_step_fp32(variant, monkeypatch):
+    """The opt-in fused H + E kernel (CHIML_FUSED=1, read when the simulation
+    is created) on the split-kernel decks, against the FP64 oracle and against
+    the split sweeps of the same FP32 deck."""
+    deck = _split_deck(variant, "fp32")
+    print(f"fused variant={variant}", flush=True)
+    from chiml_amd import FDTDField
+    split = FDTDField(deck)
+    split.step(100)
+    split.sync()
+    monkeypatch.setenv("CHIML_FUSED", "1")
+    sim, orc = _pair(deck, 100)
+    _fields(sim, orc, range(6), 1e-4)
+    for c in range(6):
+        assert rel_linf(sim.field(c), split.field(c)) <= 1e-5, c
+    err = rel_linf(sim.fluxArr()[0].getFlux(), orc.flux(0))
+    print("flux rel", err, flush=True)
+    assert err <= 1e-5
+
+
 def _n_level_block(nlev, precision):
     d = decks.empty_deck((20, 20, 20), 0.02, 90, 4, precision=precision)
     em = decks.n_level_emitter(nlev, [0.04, 0.04, 0.04], (0.01, 0.01, 0.01), seed=40 + nlev)

@@ -113,3 +113,25 @@ def test_server_results_equal_alone(graph):
         assert srv.stats()["bytes_held"] == 0
     finally:
         srv.close()
+
+
+def test_server_drain_completes_with_untaken_results_over_budget():
+    # results not yet taken fill the budget: each later request is admitted
+    # once nothing runs, so drain() returns and every result can be taken after
+    mix = [decks.config1_2d(n=32, steps=20 + 5 * k, pml=6, cyl_radius_cells=4, src_cell=(10, 16), ndet=1,
+                            precision="fp32") for k in range(4)]
+    budget = min(chiml_amd.estimate_bytes(d) for d in mix)  # room for one request at a time
+    srv = chiml_amd.Server(budget, max_active=2, chunk_steps=7, graph_steps=0)
+    try:
+        ids = [srv.submit(d) for d in mix]
+        srv.drain()
+        for k, rid in enumerate(ids):
+            state, _, _ = srv.poll(rid)
+            assert state == chiml_amd.Server.DONE, srv.error(rid)
+        for k, rid in enumerate(ids):
+            sim = srv.take(rid)
+            _same(_alone(mix[k]), _results(sim), f"request {k}")
+            sim.close()
+        assert srv.stats()["bytes_held"] == 0
+    finally:
+        srv.close()
