@@ -149,6 +149,11 @@ __device__ __forceinline__ void advance_counter(const A& a) {
         threadIdx.z == 0)
         *a.step_next = *a.step + 1;
 }
+// The same for a block bx of one simulation inside a batched launch.
+template <typename A>
+__device__ __forceinline__ void advance_counter_at(const A& a, long long bx) {
+    if (a.step_next && bx == 0 && threadIdx.x == 0) *a.step_next = *a.step + 1;
+}
 
 __device__ __forceinline__ long long lidx(int x, int y, int z, int nx, int ny) {
     return x + static_cast<long long>(nx) * (y + static_cast<long long>(ny) * z);
@@ -849,14 +854,14 @@ __device__ __forceinline__ bool int_group(const Box3& b, long long t, int& x0, i
 __host__ __device__ constexpr int ade_blocks(int ade_bpc) { return (3 * ade_bpc + 7) / 8 * 8; }
 
 template <typename T, int W, bool SRC>
-__device__ __forceinline__ void h3_int_sweep(const StepArgs<T>& a, const Box3& b, int nib, int ade_bpc) {
+__device__ __forceinline__ void h3_int_sweep(const StepArgs<T>& a, const Box3& b, int nib, int ade_bpc, int bx) {
     const int nade = ade_blocks(ade_bpc);
-    if (static_cast<int>(blockIdx.x) < nade) {
-        const int c = blockIdx.x / max(ade_bpc, 1);
-        if (c < 3) ade_block<T, W, SRC>(a, c, blockIdx.x - c * ade_bpc);
+    if (bx < nade) {
+        const int c = bx / max(ade_bpc, 1);
+        if (c < 3) ade_block<T, W, SRC>(a, c, bx - c * ade_bpc);
         return;
     }
-    const long long blk = xcd_block(blockIdx.x - nade, nib);
+    const long long blk = xcd_block(bx - nade, nib);
     int x0, y, z, col;
     const bool in = int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z, col);
     if (!in) return;
@@ -867,8 +872,17 @@ __device__ __forceinline__ void h3_int_sweep(const StepArgs<T>& a, const Box3& b
 
 template <typename T, int W, bool SRC, bool ADV>
 __global__ void __launch_bounds__(256) k_h3_int(StepArgs<T> a, Box3 b, int nib, int ade_bpc) {
-    h3_int_sweep<T, W, SRC>(a, b, nib, ade_bpc);
+    h3_int_sweep<T, W, SRC>(a, b, nib, ade_bpc, blockIdx.x);
     if (ADV) advance_counter(a);
+}
+
+template <typename T, int W, bool SRC>
+__device__ __forceinline__ void e3_int_sweep(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], const Box3& b, int bx, int nb) {
+    const long long blk = xcd_block(bx, nb);
+    int x0, y, z, col;
+    if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z, col)) return;
+    const int lane = threadIdx.x & 63;
+    e_group<T, W, false, SRC, true>(a, mt, x0, y, z, lane > 0 && col > 0);
 }
 
 template <typename T, int W, bool SRC>
@@ -876,11 +890,7 @@ __global__ void __launch_bounds__(256) k_e3_int(StepArgs<T> a, Box3 b, int nm0, 
     __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
     const int nm[3] = {nm0, nm1, nm2};
     load_mat(a, mt, nm);
-    const long long blk = xcd_block(blockIdx.x, gridDim.x);
-    int x0, y, z, col;
-    if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z, col)) return;
-    const int lane = threadIdx.x & 63;
-    e_group<T, W, false, SRC, true>(a, mt, x0, y, z, lane > 0 && col > 0);
+    e3_int_sweep<T, W, SRC>(a, mt, b, blockIdx.x, gridDim.x);
 }
 
 
@@ -937,8 +947,8 @@ __device__ __forceinline__ void slab_group(const SlabMap& m, long long t, int& x
 }
 
 template <typename T, int W, bool SRC>
-__global__ void __launch_bounds__(256) k_h3_slab(StepArgs<T> a, SlabMap m) {
-    const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void h3_slab_sweep(const StepArgs<T>& a, const SlabMap& m, int bx, int nb) {
+    const long long t = xcd_block(bx, nb) * blockDim.x + threadIdx.x;
     if (t >= m.nA + m.nB + m.nC) return;
     int x0, y, z, col, rowlen;
     slab_group(m, t, x0, y, z, col, rowlen);
@@ -948,15 +958,25 @@ __global__ void __launch_bounds__(256) k_h3_slab(StepArgs<T> a, SlabMap m) {
 }
 
 template <typename T, int W, bool SRC>
-__global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int nm0, int nm1, int nm2) {
-    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
-    const int nm[3] = {nm0, nm1, nm2};
-    load_mat(a, mt, nm);
-    const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_h3_slab(StepArgs<T> a, SlabMap m) {
+    h3_slab_sweep<T, W, SRC>(a, m, blockIdx.x, gridDim.x);
+}
+
+template <typename T, int W, bool SRC>
+__device__ __forceinline__ void e3_slab_sweep(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], const SlabMap& m, int bx, int nb) {
+    const long long t = xcd_block(bx, nb) * blockDim.x + threadIdx.x;
     if (t >= m.nA + m.nB + m.nC) return;
     int x0, y, z, col, rowlen;
     slab_group(m, t, x0, y, z, col, rowlen);
     e_group<T, W, true, SRC, true>(a, mt, x0, y, z);  // (the lane shuffle measured slower here)
+}
+
+template <typename T, int W, bool SRC>
+__global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int nm0, int nm1, int nm2) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int nm[3] = {nm0, nm1, nm2};
+    load_mat(a, mt, nm);
+    e3_slab_sweep<T, W, SRC>(a, mt, m, blockIdx.x, gridDim.x);
 }
 
 // FP32: the whole 3D sweep of a field in one launch, [pole update | CPML
@@ -969,9 +989,8 @@ __global__ void __launch_bounds__(256) k_e3_slab(StepArgs<T> a, SlabMap m, int n
 // kernel would.
 template <typename T, int W, bool SRC>
 __device__ __forceinline__ void h3_all_sweep(const StepArgs<T>& a, const Box3& b, const SlabMap& m, int nib, int nsbp,
-                                             int ade_bpc) {
+                                             int ade_bpc, int bx) {
     const int nade = ade_blocks(ade_bpc);
-    int bx = blockIdx.x;
     if (bx < nade) {
         const int c = bx / max(ade_bpc, 1);
         if (c < 3) ade_block<T, W, SRC>(a, c, bx - c * ade_bpc);
@@ -996,16 +1015,13 @@ __device__ __forceinline__ void h3_all_sweep(const StepArgs<T>& a, const Box3& b
 
 template <typename T, int W, bool SRC, bool ADV>
 __global__ void __launch_bounds__(256) k_h3_all(StepArgs<T> a, Box3 b, SlabMap m, int nib, int nsbp, int ade_bpc) {
-    h3_all_sweep<T, W, SRC>(a, b, m, nib, nsbp, ade_bpc);
+    h3_all_sweep<T, W, SRC>(a, b, m, nib, nsbp, ade_bpc, blockIdx.x);
     if (ADV) advance_counter(a);
 }
 
 template <typename T, int W, bool SRC>
-__global__ void __launch_bounds__(256) k_e3_all(StepArgs<T> a, Box3 b, SlabMap m, int nib, int nsbp, int nm0, int nm1, int nm2) {
-    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
-    const int nm[3] = {nm0, nm1, nm2};
-    load_mat(a, mt, nm);
-    int bx = blockIdx.x;
+__device__ __forceinline__ void e3_all_sweep(const StepArgs<T>& a, const MatLds<T> (*mt)[CHIML_MAX_MAT], const Box3& b, const SlabMap& m,
+                                             int nib, int nsbp, int bx) {
     if (bx < nsbp) {
         const long long t = xcd_block(bx, nsbp) * blockDim.x + threadIdx.x;
         if (t >= m.nA + m.nB + m.nC) return;
@@ -1020,6 +1036,14 @@ __global__ void __launch_bounds__(256) k_e3_all(StepArgs<T> a, Box3 b, SlabMap m
     if (!int_group<T, W>(b, blk * blockDim.x + threadIdx.x, x0, y, z, col)) return;
     const int lane = threadIdx.x & 63;
     e_group<T, W, true, SRC, true>(a, mt, x0, y, z, lane > 0 && col > 0);
+}
+
+template <typename T, int W, bool SRC>
+__global__ void __launch_bounds__(256) k_e3_all(StepArgs<T> a, Box3 b, SlabMap m, int nib, int nsbp, int nm0, int nm1, int nm2) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int nm[3] = {nm0, nm1, nm2};
+    load_mat(a, mt, nm);
+    e3_all_sweep<T, W, SRC>(a, mt, b, m, nib, nsbp, blockIdx.x);
 }
 
 // ------------------------------------------------------- fused 3D step ----
@@ -1629,9 +1653,9 @@ __device__ __forceinline__ void store_point(const SampleArgs<T>& a, int t, doubl
 // 2D TMz on W-cell groups over the whole domain (CPML handled per group; the
 // x-slab membership is uniform within a group as in 3D).
 template <typename T, int W, bool SRC>
-__device__ __forceinline__ void h2v_sweep(const StepArgs<T>& a) {
+__device__ __forceinline__ void h2v_sweep(const StepArgs<T>& a, int bx, int nb) {
     const int nx = a.nx, ny = a.ny, gx = nx / W;
-    const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const long long t = xcd_block(bx, nb) * blockDim.x + threadIdx.x;
     if (t >= static_cast<long long>(gx) * ny) return;
     const int y = static_cast<int>(t / gx), x0 = static_cast<int>(t - static_cast<long long>(y) * gx) * W;
     const long long i = x0 + static_cast<long long>(nx) * y;
@@ -1677,7 +1701,7 @@ __device__ __forceinline__ void h2v_sweep(const StepArgs<T>& a) {
 
 template <typename T, int W, bool SRC, bool ADV>
 __global__ void __launch_bounds__(256) k_h2v(StepArgs<T> a) {
-    h2v_sweep<T, W, SRC>(a);
+    h2v_sweep<T, W, SRC>(a, blockIdx.x, gridDim.x);
     if (ADV) advance_counter(a);
 }
 
@@ -1685,12 +1709,9 @@ __global__ void __launch_bounds__(256) k_h2v(StepArgs<T> a) {
 // here, by the thread that owns each detector cell, at the advanced step:
 // no separate sample launch (2D decks without flux boxes and with few points).
 template <typename T, int W, bool SRC, bool DET>
-__global__ void __launch_bounds__(256) k_e2v(StepArgs<T> a, SampleArgs<T> sa, int nm2) {
-    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
-    const int nm[3] = {0, 0, nm2};
-    load_mat(a, mt, nm);
+__device__ __forceinline__ void e2v_sweep(const StepArgs<T>& a, const SampleArgs<T>& sa, const MatLds<T> (*mt)[CHIML_MAX_MAT], int bx, int nb) {
     const int nx = a.nx, ny = a.ny, gx = nx / W;
-    const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const long long t = xcd_block(bx, nb) * blockDim.x + threadIdx.x;
     if (t >= static_cast<long long>(gx) * ny) return;
     const int y = static_cast<int>(t / gx), x0 = static_cast<int>(t - static_cast<long long>(y) * gx) * W;
     const long long i = x0 + static_cast<long long>(nx) * y;
@@ -1722,14 +1743,21 @@ __global__ void __launch_bounds__(256) k_e2v(StepArgs<T> a, SampleArgs<T> sa, in
     }
 }
 
+template <typename T, int W, bool SRC, bool DET>
+__global__ void __launch_bounds__(256) k_e2v(StepArgs<T> a, SampleArgs<T> sa, int nm2) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int nm[3] = {0, 0, nm2};
+    load_mat(a, mt, nm);
+    e2v_sweep<T, W, SRC, DET>(a, sa, mt, blockIdx.x, gridDim.x);
+}
+
 // ----------------------------------------------------- step counter ---------
 __global__ void k_set_step(int* step, int v) { *step = v; }
 
 // ----------------------------------------------- detectors and DFT ----------
 
 template <typename T>
-__global__ void k_sample(SampleArgs<T> a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sample_at(const SampleArgs<T>& a, int t) {
     const int step = step_now(a.step, a.stepv);
     if (t < a.totpts && a.det_all1) {
         // the cells do not depend on the detector: their loads overlap its
@@ -1769,6 +1797,11 @@ __global__ void k_sample(SampleArgs<T> a) {
             a.stage[static_cast<long long>(k) * a.ndft + s] = a.F[a.dft_comp[s]][a.dft_cell[s]];
         }
     }
+}
+
+template <typename T>
+__global__ void k_sample(SampleArgs<T> a) {
+    sample_at<T>(a, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // F[f][s] += sum_k exp(-i w_f t_k) U[k][s] over the staged samples k of the
@@ -2105,6 +2138,133 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
 // One block per population detector: the density kernel's per-block partial
 // sums, each thread a fixed strided subset, then a tree in LDS (a fixed
 // order, so every run sums the same way).
+// ------------------------------------------------------- batched launches ----
+// One launch per kernel class covers many simulations (the serving loop's
+// batches): slot k of a device-side descriptor table holds simulation k's
+// parameter blocks, and a block finds its slot from the per-launch prefix
+// table begin[] (blocks [begin[k], begin[k+1]) belong to slot k, each range
+// starting on a multiple of 8 so that the XCD-aware block numbering of the
+// sweeps holds within it).  The sweeps are the ones of the single
+// simulation's launches, so each simulation's arithmetic is unchanged.
+// Batched steps read the step number from each simulation's device counter
+// (as captured graphs do), so the table is the same every step of a given
+// counter parity.
+template <typename T>
+struct BatchSlot {
+    StepArgs<T> a;
+    SampleArgs<T> s;
+    Box3 b;
+    SlabMap m;
+    int nib, nsbp, ade_bpc;
+    int nm[3];
+    int nblk[5];  // blocks of each phase launch (unpadded)
+};
+
+enum { kPhH = 0, kPhHs = 1, kPhE = 2, kPhEs = 3, kPhS = 4, kPhases = 5 };
+
+__device__ __forceinline__ int batch_slot(const int* __restrict__ begin, int n, int b) {
+    int lo = 0, hi = n;  // begin[lo] <= b < begin[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (begin[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(256) k_bh2v(const BatchSlot<T>* __restrict__ sl, const int* __restrict__ begin, int n) {
+    const int k = batch_slot(begin, n, blockIdx.x);
+    const int bx = blockIdx.x - begin[k];
+    const BatchSlot<T>& q = sl[k];
+    if (bx >= q.nblk[kPhH]) return;
+    h2v_sweep<T, W, true>(q.a, bx, q.nblk[kPhH]);
+    advance_counter_at(q.a, bx);
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(256) k_be2v(const BatchSlot<T>* __restrict__ sl, const int* __restrict__ begin, int n) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int k = batch_slot(begin, n, blockIdx.x);
+    const int bx = blockIdx.x - begin[k];
+    const BatchSlot<T>& q = sl[k];
+    if (bx >= q.nblk[kPhE]) return;
+    const int nm[3] = {0, 0, q.nm[2]};
+    load_mat(q.a, mt, nm);
+    e2v_sweep<T, W, true, false>(q.a, q.s, mt, bx, q.nblk[kPhE]);
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(256) k_bh3_all(const BatchSlot<T>* __restrict__ sl, const int* __restrict__ begin, int n) {
+    const int k = batch_slot(begin, n, blockIdx.x);
+    const int bx = blockIdx.x - begin[k];
+    const BatchSlot<T>& q = sl[k];
+    if (bx >= q.nblk[kPhH]) return;
+    h3_all_sweep<T, W, true>(q.a, q.b, q.m, q.nib, q.nsbp, q.ade_bpc, bx);
+    advance_counter_at(q.a, bx);
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(256) k_be3_all(const BatchSlot<T>* __restrict__ sl, const int* __restrict__ begin, int n) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int k = batch_slot(begin, n, blockIdx.x);
+    const int bx = blockIdx.x - begin[k];
+    const BatchSlot<T>& q = sl[k];
+    if (bx >= q.nblk[kPhE]) return;
+    load_mat(q.a, mt, q.nm);
+    e3_all_sweep<T, W, true>(q.a, mt, q.b, q.m, q.nib, q.nsbp, bx);
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(256) k_bh3_int(const BatchSlot<T>* __restrict__ sl, const int* __restrict__ begin, int n) {
+    const int k = batch_slot(begin, n, blockIdx.x);
+    const int bx = blockIdx.x - begin[k];
+    const BatchSlot<T>& q = sl[k];
+    if (bx >= q.nblk[kPhH]) return;
+    h3_int_sweep<T, W, true>(q.a, q.b, q.nib, q.ade_bpc, bx);
+    advance_counter_at(q.a, bx);
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(256) k_bh3_slab(const BatchSlot<T>* __restrict__ sl, const int* __restrict__ begin, int n) {
+    const int k = batch_slot(begin, n, blockIdx.x);
+    const int bx = blockIdx.x - begin[k];
+    const BatchSlot<T>& q = sl[k];
+    if (bx >= q.nblk[kPhHs]) return;
+    h3_slab_sweep<T, W, true>(q.a, q.m, bx, q.nblk[kPhHs]);
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(256) k_be3_int(const BatchSlot<T>* __restrict__ sl, const int* __restrict__ begin, int n) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int k = batch_slot(begin, n, blockIdx.x);
+    const int bx = blockIdx.x - begin[k];
+    const BatchSlot<T>& q = sl[k];
+    if (bx >= q.nblk[kPhE]) return;
+    load_mat(q.a, mt, q.nm);
+    e3_int_sweep<T, W, true>(q.a, mt, q.b, bx, q.nblk[kPhE]);
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(256) k_be3_slab(const BatchSlot<T>* __restrict__ sl, const int* __restrict__ begin, int n) {
+    __shared__ MatLds<T> mt[3][CHIML_MAX_MAT];
+    const int k = batch_slot(begin, n, blockIdx.x);
+    const int bx = blockIdx.x - begin[k];
+    const BatchSlot<T>& q = sl[k];
+    if (bx >= q.nblk[kPhEs]) return;
+    load_mat(q.a, mt, q.nm);
+    e3_slab_sweep<T, W, true>(q.a, mt, q.m, bx, q.nblk[kPhEs]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bsample(const BatchSlot<T>* __restrict__ sl, const int* __restrict__ begin, int n) {
+    const int k = batch_slot(begin, n, blockIdx.x);
+    const int bx = blockIdx.x - begin[k];
+    const BatchSlot<T>& q = sl[k];
+    if (bx >= q.nblk[kPhS]) return;
+    sample_at<T>(q.s, bx * blockDim.x + threadIdx.x);
+}
+
 constexpr int kPopThreads = 1024;
 
 __global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, int nblocks, const int* stepp, int stepv) {
@@ -2143,6 +2303,7 @@ struct QeHost {
 struct chiml_dev {
     std::string err;
     hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;  // joins this context's stream with a batch's stream
     int device = 0;
     int prec = 64;
     int nx = 0, ny = 0, nz = 0, dim = 3;
@@ -2772,6 +2933,220 @@ void collect_profile(chiml_dev* c) {
 
 }  // namespace
 
+// ----------------------------------------------------------- batched steps ----
+constexpr int kBatchMaxSlots = 64;  // simulations per group of shared launches
+constexpr size_t kBatchHostBytes = 2 * kBatchMaxSlots * sizeof(BatchSlot<double>) + kPhases * (kBatchMaxSlots + 1) * sizeof(int) + 4096;
+
+namespace {
+void enqueue_steps(chiml_dev* c, int n);
+int fail(chiml_dev* c, int code, const std::string& msg);
+unsigned char g_batch_host[kBatchHostBytes];  // host image of one call's tables (under g_batch_mu)
+std::mutex g_batch_mu;
+
+// Kernel class of a context for batched launches: 1 = 2D W-cell kernels,
+// 2 = 3D split sweeps; 0 = not batched (stepped on its own stream).
+int batch_class(const chiml_dev* c) {
+    if (c->fused || c->capturing) return 0;
+    if (c->dim == 2 && c->vec2d) return 1;
+    if (c->dim == 3 && c->split) return 2;
+    return 0;
+}
+
+template <typename T>
+StepArgs<T>& step_args(chiml_dev* c);
+template <>
+StepArgs<float>& step_args<float>(chiml_dev* c) { return c->a32; }
+template <>
+StepArgs<double>& step_args<double>(chiml_dev* c) { return c->a64; }
+
+// The slot of context c for the steps whose counter parity is p.
+template <typename T>
+BatchSlot<T> make_slot(chiml_dev* c, int p) {
+    constexpr int W = 16 / sizeof(T);
+    BatchSlot<T> q{};
+    q.a = step_args<T>(c);
+    q.s = sample_args<T>(c);
+    int* ctr = c->step_ctr.as<int>();
+    q.a.step = ctr + p;
+    q.a.step_next = ctr + (1 - p);
+    q.a.stepv = -1;
+    q.s.step = ctr + (1 - p);
+    q.s.stepv = -1;
+    for (int k = 0; k < 3; ++k) q.nm[k] = c->nmat[k];
+    if (c->dim == 2) {
+        const long long ng = static_cast<long long>(c->nx / W) * c->ny;
+        q.nblk[kPhH] = q.nblk[kPhE] = static_cast<int>((ng + 255) / 256);
+    } else {
+        const Box3 b = c->ibox;
+        const int nyb = b.y1 - b.y0, by = b.by > 0 ? b.by : nyb;
+        const long long ni = static_cast<long long>((b.x1 - b.x0) / W) * by * (b.z1 - b.z0) * ((nyb + by - 1) / by);
+        const long long ns = c->slab.nA + c->slab.nB + c->slab.nC;
+        q.b = b;
+        q.m = c->slab;
+        q.nib = static_cast<int>((ni + 255) / 256);
+        q.ade_bpc = static_cast<int>((c->ade_groups + 255) / 256);
+        const int gs = static_cast<int>((ns + 255) / 256);
+        if (W == 4) {  // FP32: one launch per field, as launch_fields
+            q.nsbp = (gs + 7) / 8 * 8;
+            q.nblk[kPhH] = ade_blocks(q.ade_bpc) + q.nsbp + q.nib;
+            q.nblk[kPhE] = q.nsbp + q.nib;
+        } else {  // FP64: interior (+ pole update) and slab launches per field
+            q.nblk[kPhH] = q.nib + ade_blocks(q.ade_bpc);
+            q.nblk[kPhHs] = gs;
+            q.nblk[kPhE] = q.nib;
+            q.nblk[kPhEs] = gs;
+        }
+    }
+    q.nblk[kPhS] = (c->totpts + c->ndft + 255) / 256;
+    return q;
+}
+
+// Enqueues nsteps batched steps of the contexts g[0..n) (one class, type T)
+// on stream bs, with their tables at device address dev (host image host).
+template <typename T>
+void batch_group(chiml_dev** g, int n, int nsteps, hipStream_t bs, unsigned char* dev, unsigned char* host) {
+    constexpr int W = 16 / sizeof(T);
+    BatchSlot<T>* hs = reinterpret_cast<BatchSlot<T>*>(host);
+    int* hb = reinterpret_cast<int*>(host + 2 * n * sizeof(BatchSlot<T>));
+    int total[kPhases] = {};
+    for (int i = 0; i < n; ++i) {
+        const int p = g[i]->steps_done & 1;
+        hs[i] = make_slot<T>(g[i], p);
+        hs[n + i] = make_slot<T>(g[i], 1 - p);
+    }
+    for (int ph = 0; ph < kPhases; ++ph) {
+        int acc = 0;
+        for (int i = 0; i < n; ++i) {
+            hb[ph * (n + 1) + i] = acc;
+            acc += (hs[i].nblk[ph] + 7) / 8 * 8;  // each simulation's range starts on a multiple of 8 blocks
+        }
+        hb[ph * (n + 1) + n] = acc;
+        total[ph] = acc;
+    }
+    const size_t bytes = 2 * n * sizeof(BatchSlot<T>) + kPhases * (n + 1) * sizeof(int);
+    HIP_TRY(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, bs));
+    const BatchSlot<T>* ds = reinterpret_cast<const BatchSlot<T>*>(dev);
+    const int* db = reinterpret_cast<const int*>(dev + 2 * n * sizeof(BatchSlot<T>));
+    const dim3 blk(256);
+    for (int j = 0; j < nsteps; ++j) {
+        const BatchSlot<T>* t = ds + (j & 1) * n;  // every slot's counter parity flips each step
+        const int* bH = db + kPhH * (n + 1);
+        const int* bHs = db + kPhHs * (n + 1);
+        const int* bE = db + kPhE * (n + 1);
+        const int* bEs = db + kPhEs * (n + 1);
+        const int* bS = db + kPhS * (n + 1);
+        if (g[0]->dim == 2) {
+            if (total[kPhH]) hipLaunchKernelGGL((k_bh2v<T, W>), dim3(total[kPhH]), blk, 0, bs, t, bH, n);
+            if (total[kPhE]) hipLaunchKernelGGL((k_be2v<T, W>), dim3(total[kPhE]), blk, 0, bs, t, bE, n);
+        } else if (W == 4) {
+            if (total[kPhH]) hipLaunchKernelGGL((k_bh3_all<T, W>), dim3(total[kPhH]), blk, 0, bs, t, bH, n);
+            if (total[kPhE]) hipLaunchKernelGGL((k_be3_all<T, W>), dim3(total[kPhE]), blk, 0, bs, t, bE, n);
+        } else {
+            if (total[kPhH]) hipLaunchKernelGGL((k_bh3_int<T, W>), dim3(total[kPhH]), blk, 0, bs, t, bH, n);
+            if (total[kPhHs]) hipLaunchKernelGGL((k_bh3_slab<T, W>), dim3(total[kPhHs]), blk, 0, bs, t, bHs, n);
+            if (total[kPhE]) hipLaunchKernelGGL((k_be3_int<T, W>), dim3(total[kPhE]), blk, 0, bs, t, bE, n);
+            if (total[kPhEs]) hipLaunchKernelGGL((k_be3_slab<T, W>), dim3(total[kPhEs]), blk, 0, bs, t, bEs, n);
+        }
+        // emitters per simulation, on the batch stream, between the E update and the sampling
+        for (int i = 0; i < n; ++i) {
+            chiml_dev* c = g[i];
+            if (c->qes.empty()) continue;
+            StepArgs<T>& a = step_args<T>(c);
+            const int p = (c->steps_done + j) & 1;
+            int* ctr = c->step_ctr.as<int>();
+            a.step = ctr + p;
+            a.stepv = -1;
+            a.step_next = nullptr;
+            const hipStream_t own = c->stream;
+            c->stream = bs;
+            launch_ml<T>(c, a);
+            c->stream = own;
+        }
+        if (total[kPhS]) hipLaunchKernelGGL((k_bsample<T>), dim3(total[kPhS]), blk, 0, bs, t, bS, n);
+        // DFT flushes decide on the device whether their block of samples is complete
+        for (int i = 0; i < n; ++i) {
+            chiml_dev* c = g[i];
+            if (c->ndft == 0) continue;
+            const int p = (c->steps_done + j) & 1;
+            int* ctr = c->step_ctr.as<int>();
+            hipLaunchKernelGGL(k_dft_flush<T>, dim3((c->ndft + 255) / 256), blk, 0, bs, static_cast<const T*>(c->dft_stage), c->dft_phasor,
+                               c->dft_F, ctr + (1 - p), c->ndft, c->nfreq, c->dft_interval, 0, -1);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    for (int i = 0; i < n; ++i) {
+        StepArgs<T>& a = step_args<T>(g[i]);
+        a.stepv = -1;
+        a.step_next = nullptr;
+        g[i]->steps_done += nsteps;
+        g[i]->enq_step = g[i]->steps_done;
+    }
+}
+
+// chiml_dev_step_batch: contexts of one kernel class and
+// precision share every field and sample launch; the others step on their
+// own streams.  The batch runs on the first batched context's stream, joined
+// with the other contexts' streams by events before and after.
+int step_batch(chiml_dev** ctxs, int n, int nsteps) {
+    if (!ctxs || n < 0 || nsteps < 0) return fail(nullptr, CHIML_E_INVALID, "invalid argument");
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return fail(nullptr, CHIML_E_INVALID, "null context in batch");
+        for (int k = 0; k < i; ++k)
+            if (ctxs[k] == ctxs[i]) return fail(nullptr, CHIML_E_INVALID, "a context appears twice in one batch");
+        if (ctxs[i]->steps_done + nsteps > ctxs[i]->nsteps) return fail(ctxs[i], CHIML_E_STATE, "step count exceeds the per-step tables built at create");
+        if (ctxs[i]->device != ctxs[0]->device) return fail(nullptr, CHIML_E_INVALID, "a batch must share one device");
+    }
+    if (n == 0 || nsteps == 0) return CHIML_OK;
+    try {
+        chiml_dev* grp[4][kBatchMaxSlots];
+        int ng[4] = {0, 0, 0, 0};
+        size_t off = 0;  // the groups of one call take consecutive parts of the host image
+        // the shared launches of one group (class x precision)
+        auto run_group = [&](int gi) {
+            const int m = ng[gi];
+            ng[gi] = 0;
+            if (m == 0) return;
+            std::lock_guard<std::mutex> lk(g_batch_mu);
+            chiml_dev** g = grp[gi];
+            const hipStream_t bs = g[0]->stream;
+            for (int i = 1; i < m; ++i) {
+                HIP_TRY(hipEventRecord(g[i]->ev, g[i]->stream));
+                HIP_TRY(hipStreamWaitEvent(bs, g[i]->ev, 0));
+            }
+            const bool fp32 = (gi & 1) == 0;
+            const size_t bytes = 2 * m * (fp32 ? sizeof(BatchSlot<float>) : sizeof(BatchSlot<double>)) + kPhases * (m + 1) * sizeof(int);
+            unsigned char* dev = nullptr;
+            size_t at = (off + 255) / 256 * 256;
+            if (at + bytes > kBatchHostBytes) at = 0;  // (more than 64 per call: the host image is consumed by the copy)
+            if (bytes > kBatchHostBytes) throw std::runtime_error("batch tables exceed the host image size");
+            off = at + bytes;
+            HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&dev), bytes, bs));
+            if (fp32) batch_group<float>(g, m, nsteps, bs, dev, g_batch_host + at);
+            else batch_group<double>(g, m, nsteps, bs, dev, g_batch_host + at);
+            HIP_TRY(hipFreeAsync(dev, bs));
+            HIP_TRY(hipEventRecord(g[0]->ev, bs));
+            for (int i = 1; i < m; ++i) HIP_TRY(hipStreamWaitEvent(g[i]->stream, g[0]->ev, 0));
+        };
+        // groups: (class 1 or 2) x (FP32, FP64)
+        for (int i = 0; i < n; ++i) {
+            chiml_dev* c = ctxs[i];
+            const int cls = batch_class(c);
+            if (cls == 0) {  // stepped alone
+                enqueue_steps(c, nsteps);
+                continue;
+            }
+            const int gi = (cls - 1) * 2 + (c->prec == 32 ? 0 : 1);
+            if (ng[gi] == kBatchMaxSlots) run_group(gi);  // a full group: step it and start another
+            grp[gi][ng[gi]++] = c;
+        }
+        for (int gi = 0; gi < 4; ++gi) run_group(gi);
+    } catch (std::exception& e) {
+        return fail(nullptr, CHIML_E_HIP, e.what());
+    }
+    return CHIML_OK;
+}
+}  // namespace
+
 namespace {
 // Integer knob from the environment (tuning sweeps of the fused launch
 // shape; 0 or unset keeps the default).
@@ -2884,6 +3259,7 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
         if (ndev < 1) throw std::runtime_error("no HIP device");
         HIP_TRY(hipGetDevice(&c->device));
         HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev, hipEventDisableTiming));
         c->prec = p->precision;
         c->nx = p->nx;
         c->ny = p->ny;
@@ -3039,15 +3415,7 @@ int chiml_dev_query(chiml_dev_t* c, int* idle) {
     return CHIML_OK;
 }
 
-int chiml_dev_step_batch(chiml_dev_t** ctxs, int n, int nsteps) {
-    if (!ctxs || n < 0) return fail(nullptr, CHIML_E_INVALID, "invalid argument");
-    for (int s = 0; s < nsteps; ++s)
-        for (int i = 0; i < n; ++i) {
-            int rc = chiml_dev_step(ctxs[i], 1);
-            if (rc) return rc;
-        }
-    return CHIML_OK;
-}
+int chiml_dev_step_batch(chiml_dev_t** ctxs, int n, int nsteps) { return step_batch(ctxs, n, nsteps); }
 
 int chiml_dev_sync(chiml_dev_t* c) {
     if (!c) return fail(nullptr, CHIML_E_INVALID, "null context");
@@ -3226,6 +3594,7 @@ void chiml_dev_destroy(chiml_dev_t* c) {
     }
     c->bufs.clear();
     c->qes.clear();
+    if (c->ev) (void)hipEventDestroy(c->ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

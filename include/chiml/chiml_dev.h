@@ -188,8 +188,19 @@ const char* chiml_dev_last_error(const chiml_dev_t* ctx);
 const char* chiml_dev_global_error(void);
 void chiml_dev_destroy(chiml_dev_t* ctx);
 
-/* Batched stepping: advances several contexts of the same class by nsteps,
- * interleaving their launches on their own streams. */
+/* Batched stepping: advances every context by nsteps.  Contexts of one
+ * kernel class (2D TMz, or 3D with CPML slabs) and one precision share their
+ * launches: each field, pole-update and sample launch of a step covers all of
+ * them, through a device-side table of per-simulation parameter blocks that
+ * the call allocates (stream-ordered) and frees.  Emitter and DFT launches
+ * stay per simulation, on the batch's stream; contexts of other classes step
+ * alone on their own streams.  The batch runs on the stream of its first
+ * context, joined by events with the others' streams before and after, so the
+ * call is asynchronous like chiml_dev_step.  Every simulation's arithmetic
+ * is that of chiml_dev_step, so results are identical bit for bit.  A context
+ * may appear once per call; all must be on one device.  No counterpart in the
+ * reference, which runs one simulation per job (src/main.cpp:40-118): this is
+ * the sweep server's shared launch. */
 int chiml_dev_step_batch(chiml_dev_t** ctxs, int n, int nsteps);
 
 #ifdef __cplusplus

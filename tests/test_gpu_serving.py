@@ -135,3 +135,46 @@ def test_server_drain_completes_with_untaken_results_over_budget():
         assert srv.stats()["bytes_held"] == 0
     finally:
         srv.close()
+
+
+def _batch_run(mix, chunks):
+    """Steps every deck of `mix` to its end through chiml_sim_step_batch, in
+    batches of the given chunk lengths (each batch holds the simulations with
+    at least that many steps left; the rest finish alone)."""
+    sims = [chiml_amd.FDTDField(d) for d in mix]
+    done = [0] * len(sims)
+    k = 0
+    while True:
+        left = [s.nsteps - done[i] for i, s in enumerate(sims)]
+        if max(left) == 0:
+            break
+        ch = chunks[k % len(chunks)]
+        k += 1
+        grp = [i for i in range(len(sims)) if left[i] >= ch]
+        if grp:
+            chiml_amd.step_batch([sims[i] for i in grp], ch)
+            for i in grp:
+                done[i] += ch
+        else:
+            for i in range(len(sims)):
+                if left[i]:
+                    sims[i].step(left[i])
+                    done[i] += left[i]
+    out = []
+    for s in sims:
+        s.sync()
+        out.append(_results(s))
+        s.close()
+    return out
+
+
+def test_batched_steps_equal_alone():
+    """Shared launches (2D and 3D, FP32 and FP64, emitters, flux boxes, and an
+    odd-size deck that steps alone inside the batch call) give every
+    simulation exactly its results alone."""
+    mix = _mix()
+    alone = [_alone(d) for d in mix]
+    got = _batch_run(mix, [7, 13, 1, 20])
+    for k, (a, b) in enumerate(zip(alone, got)):
+        print(f"batched deck {k}", flush=True)
+        _same(a, b, f"deck {k}")
