@@ -54,6 +54,7 @@ def load_library():
     L.chiml_sim_query.argtypes = [ctypes.c_void_p, c_int_p]
     L.chiml_sim_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.chiml_sim_step_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int]
+    L.chiml_sim_step_batch_noalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int]
     c_ll_p = ctypes.POINTER(ctypes.c_longlong)
     L.chiml_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_void_p)]
@@ -303,12 +304,14 @@ class FDTDField:
         return b.value
 
 
-def step_batch(sims, nsteps):
+def step_batch(sims, nsteps, noalloc=False):
     """Advances every simulation of `sims` by nsteps with shared launches
-    (chiml_sim_step_batch).  Asynchronous, like FDTDField.step."""
+    (chiml_sim_step_batch; noalloc: chiml_sim_step_batch_noalloc, at most 64
+    simulations, no device allocation).  Asynchronous, like FDTDField.step."""
     L = load_library()
     arr = (ctypes.c_void_p * len(sims))(*[sim._h for sim in sims])
-    _check(L.chiml_sim_step_batch(arr, len(sims), int(nsteps)))
+    fn = L.chiml_sim_step_batch_noalloc if noalloc else L.chiml_sim_step_batch
+    _check(fn(arr, len(sims), int(nsteps)))
 
 
 def estimate_bytes(deck_json):

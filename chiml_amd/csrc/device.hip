@@ -2934,14 +2934,40 @@ void collect_profile(chiml_dev* c) {
 }  // namespace
 
 // ----------------------------------------------------------- batched steps ----
-constexpr int kBatchMaxSlots = 64;  // simulations per group of shared launches
-constexpr size_t kBatchHostBytes = 2 * kBatchMaxSlots * sizeof(BatchSlot<double>) + kPhases * (kBatchMaxSlots + 1) * sizeof(int) + 4096;
+// Scratch of the allocation-free batched step: the slot-descriptor tables
+// and prefix tables of one call, at most kBatchMaxSlots simulations.  Each GPU
+// holds its own copy; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device.
+constexpr int kBatchMaxSlots = 64;
+constexpr size_t kBatchScratchBytes = 2 * kBatchMaxSlots * sizeof(BatchSlot<double>) + kPhases * (kBatchMaxSlots + 1) * sizeof(int) + 4096;
+static __device__ __align__(256) unsigned char scratch[kBatchScratchBytes];
 
 namespace {
 void enqueue_steps(chiml_dev* c, int n);
 int fail(chiml_dev* c, int code, const std::string& msg);
-unsigned char g_batch_host[kBatchHostBytes];  // host image of one call's tables (under g_batch_mu)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+unsigned char g_batch_host[kBatchScratchBytes];  // host image of one call's tables (under g_batch_mu)
 std::mutex g_batch_mu;
+
+// The scratch array's address on device dev (looked up once per device,
+// with that device current).
+void* scratch_on(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur != dev) HIP_TRY(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) HIP_TRY(hipSetDevice(cur));
+        HIP_TRY(e);
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
 
 // Kernel class of a context for batched launches: 1 = 2D W-cell kernels,
 // 2 = 3D split sweeps; 0 = not batched (stepped on its own stream).
@@ -3083,12 +3109,14 @@ void batch_group(chiml_dev** g, int n, int nsteps, hipStream_t bs, unsigned char
     }
 }
 
-// chiml_dev_step_batch: contexts of one kernel class and
+// chiml_dev_step_batch(_noalloc): contexts of one kernel class and
 // precision share every field and sample launch; the others step on their
 // own streams.  The batch runs on the first batched context's stream, joined
 // with the other contexts' streams by events before and after.
-int step_batch(chiml_dev** ctxs, int n, int nsteps) {
+int step_batch(chiml_dev** ctxs, int n, int nsteps, bool noalloc) {
     if (!ctxs || n < 0 || nsteps < 0) return fail(nullptr, CHIML_E_INVALID, "invalid argument");
+    if (noalloc && n > kBatchMaxSlots)
+        return fail(nullptr, CHIML_E_INVALID, "chiml_dev_step_batch_noalloc: more than 64 simulations in one call");
     for (int i = 0; i < n; ++i) {
         if (!ctxs[i]) return fail(nullptr, CHIML_E_INVALID, "null context in batch");
         for (int k = 0; k < i; ++k)
@@ -3100,7 +3128,7 @@ int step_batch(chiml_dev** ctxs, int n, int nsteps) {
     try {
         chiml_dev* grp[4][kBatchMaxSlots];
         int ng[4] = {0, 0, 0, 0};
-        size_t off = 0;  // the groups of one call take consecutive parts of the host image
+        size_t off = 0;  // the groups of one call take consecutive parts of the scratch and host image
         // the shared launches of one group (class x precision)
         auto run_group = [&](int gi) {
             const int m = ng[gi];
@@ -3117,13 +3145,23 @@ int step_batch(chiml_dev** ctxs, int n, int nsteps) {
             const size_t bytes = 2 * m * (fp32 ? sizeof(BatchSlot<float>) : sizeof(BatchSlot<double>)) + kPhases * (m + 1) * sizeof(int);
             unsigned char* dev = nullptr;
             size_t at = (off + 255) / 256 * 256;
-            if (at + bytes > kBatchHostBytes) at = 0;  // (more than 64 per call: the host image is consumed by the copy)
-            if (bytes > kBatchHostBytes) throw std::runtime_error("batch tables exceed the host image size");
+            if (at + bytes > kBatchScratchBytes) {
+                // only the allocating form gets here (more than 64 per call): its
+                // tables have their own device buffer, and the host image is
+                // consumed by the copy before the call returns
+                if (noalloc) throw std::runtime_error("batch tables exceed the scratch size");
+                at = 0;
+            }
+            if (bytes > kBatchScratchBytes) throw std::runtime_error("batch tables exceed the host image size");
             off = at + bytes;
-            HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&dev), bytes, bs));
+            if (noalloc) {
+                dev = static_cast<unsigned char*>(scratch_on(g[0]->device)) + at;
+            } else {
+                HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&dev), bytes, bs));
+            }
             if (fp32) batch_group<float>(g, m, nsteps, bs, dev, g_batch_host + at);
             else batch_group<double>(g, m, nsteps, bs, dev, g_batch_host + at);
-            HIP_TRY(hipFreeAsync(dev, bs));
+            if (!noalloc) HIP_TRY(hipFreeAsync(dev, bs));
             HIP_TRY(hipEventRecord(g[0]->ev, bs));
             for (int i = 1; i < m; ++i) HIP_TRY(hipStreamWaitEvent(g[i]->stream, g[0]->ev, 0));
         };
@@ -3415,7 +3453,9 @@ int chiml_dev_query(chiml_dev_t* c, int* idle) {
     return CHIML_OK;
 }
 
-int chiml_dev_step_batch(chiml_dev_t** ctxs, int n, int nsteps) { return step_batch(ctxs, n, nsteps); }
+int chiml_dev_step_batch(chiml_dev_t** ctxs, int n, int nsteps) { return step_batch(ctxs, n, nsteps, false); }
+
+int chiml_dev_step_batch_noalloc(chiml_dev_t** ctxs, int n, int nsteps) { return step_batch(ctxs, n, nsteps, true); }
 
 int chiml_dev_sync(chiml_dev_t* c) {
     if (!c) return fail(nullptr, CHIML_E_INVALID, "null context");

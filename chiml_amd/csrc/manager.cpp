@@ -1226,18 +1226,21 @@ int chiml_sim_step(chiml_sim_t* s, int n) {
     return rc;
 }
 
-int chiml_sim_step_batch(chiml_sim_t** sims, int n, int nsteps) {
+static int sim_step_batch(chiml_sim_t** sims, int n, int nsteps, bool noalloc) {
     if (!sims || n < 0 || n > 4096) return sim_fail("invalid argument");
     chiml_dev_t* devs[4096];
     for (int i = 0; i < n; ++i) {
         if (!sims[i]) return sim_fail("null simulation in batch");
         devs[i] = sims[i]->f->dev_;
     }
-    int rc = chiml_dev_step_batch(devs, n, nsteps);
+    int rc = noalloc ? chiml_dev_step_batch_noalloc(devs, n, nsteps) : chiml_dev_step_batch(devs, n, nsteps);
     if (rc) chiml::g_sim_err = chiml_dev_global_error();
     return rc;
 }
 
+int chiml_sim_step_batch(chiml_sim_t** sims, int n, int nsteps) { return sim_step_batch(sims, n, nsteps, false); }
+
+int chiml_sim_step_batch_noalloc(chiml_sim_t** sims, int n, int nsteps) { return sim_step_batch(sims, n, nsteps, true); }
 
 int chiml_sim_sync(chiml_sim_t* s) {
     if (!s) return sim_fail("null simulation");

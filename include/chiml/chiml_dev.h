@@ -202,6 +202,15 @@ void chiml_dev_destroy(chiml_dev_t* ctx);
  * reference, which runs one simulation per job (src/main.cpp:40-118): this is
  * the sweep server's shared launch. */
 int chiml_dev_step_batch(chiml_dev_t** ctxs, int n, int nsteps);
+/* As chiml_dev_step_batch, allocating nothing: the parameter tables live in a
+ * static device array of the library (at most 64 simulations per call,
+ * 250 KB on each GPU, held until the process ends).  For callers that cannot
+ * allocate device memory (serving loops that must not call hipMalloc, code
+ * that a caller may capture).  Calls that overlap on one GPU share that array
+ * and race: a call must be enqueued after the previous call's launches on a
+ * stream that orders them (the same first context), or after a sync.
+ * No counterpart in the reference. */
+int chiml_dev_step_batch_noalloc(chiml_dev_t** ctxs, int n, int nsteps);
 
 #ifdef __cplusplus
 }

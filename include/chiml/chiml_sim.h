@@ -29,9 +29,10 @@ typedef struct chiml_sim chiml_sim_t;
 int chiml_sim_create(const char* deck_json, chiml_sim_t** out);
 /* Advance n steps (asynchronous on the device). */
 int chiml_sim_step(chiml_sim_t* s, int n);
-/* chiml_dev_step_batch over the simulations' device contexts (at most 4096
- * per call). */
+/* chiml_dev_step_batch / chiml_dev_step_batch_noalloc over the simulations'
+ * device contexts (at most 4096 per call). */
 int chiml_sim_step_batch(chiml_sim_t** sims, int n, int nsteps);
+int chiml_sim_step_batch_noalloc(chiml_sim_t** sims, int n, int nsteps);
 int chiml_sim_sync(chiml_sim_t* s);
 /* chiml_dev_query / chiml_dev_set_graph for the simulation's context. */
 int chiml_sim_query(chiml_sim_t* s, int* idle);
