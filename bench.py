@@ -54,6 +54,9 @@ def parse():
     p.add_argument("--sweep-steps", type=int, nargs=2, default=[100, 1000],
                    help="step range of the sweep sample (the full sweep uses 1000 10000)")
     p.add_argument("--no-sweep-baseline", action="store_true", help="skip the one-at-a-time replay")
+    p.add_argument("--sweep-active", type=int, default=16, help="simulations the server holds at once")
+    p.add_argument("--sweep-progress", type=int, default=0, help="print a line every this many finished requests")
+    p.add_argument("--only-sweep", action="store_true", help="run the serving sweep alone and print its line")
     p.add_argument("--no-configs", action="store_true",
                    help="skip the other workloads (2D TMz, 2D and 3D Maxwell-Liouville)")
     p.add_argument("--only", choices=["tmz", "ml2d", "ml3d"],
@@ -269,10 +272,14 @@ def run_sweep(args):
     total = torch.cuda.get_device_properties(0).total_memory
     res = {"sample": f"{args.sweep_requests} requests, steps {args.sweep_steps[0]}-{args.sweep_steps[1]}, "
                      f"seed 5, Poisson 20/s",
-           "server": sweep.run_server(reqs, hbm_budget=int(0.6 * total), max_active=16, chunk_steps=64,
-                                      graph_steps=32)}
+           "server": sweep.run_server(reqs, hbm_budget=int(0.6 * total), max_active=args.sweep_active,
+                                      chunk_steps=64, graph_steps=0, batch=True, progress=args.sweep_progress),
+           "server_streams": sweep.run_server(reqs, hbm_budget=int(0.6 * total), max_active=args.sweep_active,
+                                              chunk_steps=64, graph_steps=32, progress=args.sweep_progress)}
+    res["server"]["mode"] = "shared launches (chiml_dev_step_batch), 64-step chunks"
+    res["server_streams"]["mode"] = "one stream per simulation, 32-step captured graphs"
     if not args.no_sweep_baseline:
-        res["one_at_a_time"] = sweep.run_one_at_a_time(reqs)
+        res["one_at_a_time"] = sweep.run_one_at_a_time(reqs, progress=args.sweep_progress)
         res["throughput_ratio"] = round(res["server"]["mcells_per_s"] / res["one_at_a_time"]["mcells_per_s"], 3)
     return res
 
@@ -393,6 +400,9 @@ def main():
         sys.exit(2)
     if args.only:
         print(json.dumps({"workload": args.only, **run_workload(*other_workload(args.only, args.only_steps))}), flush=True)
+        return
+    if args.only_sweep:
+        print(json.dumps({"workload": "serving sweep", **run_sweep(args)}), flush=True)
         return
     if args.copy_calibration:
         import torch

@@ -59,6 +59,7 @@ def load_library():
     L.chiml_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_void_p)]
     L.chiml_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_char_p, c_ll_p]
+    L.chiml_server_set_batch.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.chiml_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_longlong, c_int_p, c_dbl_p, c_dbl_p]
     L.chiml_server_take.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)]
     L.chiml_server_error.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_char_p, ctypes.c_size_t]
@@ -326,12 +327,14 @@ class Server:
 
     QUEUED, BUILDING, RUNNING, DONE, FAILED = range(5)
 
-    def __init__(self, hbm_budget, max_active=16, chunk_steps=64, graph_steps=0):
+    def __init__(self, hbm_budget, max_active=16, chunk_steps=64, graph_steps=0, batch=False):
         L = load_library()
         h = ctypes.c_void_p()
         _check(L.chiml_server_create(int(hbm_budget), int(max_active), int(chunk_steps), int(graph_steps),
                                      ctypes.byref(h)))
         self._h = h
+        if batch:
+            _check(L.chiml_server_set_batch(h, 1))
 
     def submit(self, deck_json):
         i = ctypes.c_longlong()

@@ -54,6 +54,7 @@ double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::d
 struct chiml_server {
     size_t budget = 0, held = 0;
     int max_active = 1, chunk = 1, graph = 0;
+    bool batch = false;  // idle simulations step together through chiml_sim_step_batch
     std::mutex mu;
     std::condition_variable cv_build, cv_idle;
     std::map<long long, std::unique_ptr<Request>> reqs;
@@ -135,8 +136,18 @@ struct chiml_server {
         }
     }
 
+    // Fails every request of a batch whose shared launch could not be enqueued.
+    void fail_all(const std::vector<Request*>& rs, std::vector<Request*>& finished) {
+        const std::string e = chiml_sim_last_error();
+        for (Request* r : rs) {
+            r->err = e.empty() ? "batched step failed" : e;
+            finished.push_back(r);
+        }
+    }
+
     void step_loop() {
-        std::vector<Request*> finished;
+        std::vector<Request*> finished, ready, batched;
+        std::vector<chiml_sim_t*> sims;
         for (;;) {
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -147,6 +158,7 @@ struct chiml_server {
             }
             bool progressed = false;
             finished.clear();
+            ready.clear();
             for (Request* r : running) {
                 int idle = 0;
                 if (chiml_sim_query(r->sim, &idle) != CHIML_OK) {
@@ -159,7 +171,19 @@ struct chiml_server {
                     finished.push_back(r);
                     continue;
                 }
+                ready.push_back(r);
+            }
+            // every drained simulation with a whole chunk left joins one shared
+            // batch (batch mode); the others take their next chunk alone
+            batched.clear();
+            sims.clear();
+            for (Request* r : ready) {
                 const int k = std::min(chunk, r->nsteps - r->done_steps);
+                if (batch && k == chunk) {
+                    batched.push_back(r);
+                    sims.push_back(r->sim);
+                    continue;
+                }
                 if (chiml_sim_step(r->sim, k) != CHIML_OK) {
                     r->err = chiml_sim_last_error();
                     finished.push_back(r);
@@ -167,6 +191,14 @@ struct chiml_server {
                 }
                 r->done_steps += k;
                 progressed = true;
+            }
+            if (!batched.empty()) {
+                if (chiml_sim_step_batch(sims.data(), static_cast<int>(sims.size()), chunk) != CHIML_OK) {
+                    fail_all(batched, finished);
+                } else {
+                    for (Request* r : batched) r->done_steps += chunk;
+                    progressed = true;
+                }
             }
             if (!finished.empty()) {
                 const auto now = Clock::now();
@@ -216,6 +248,13 @@ int chiml_server_create(size_t hbm_budget, int max_active, int chunk_steps, int 
     for (int b = 0; b < nb; ++b) p->builders.emplace_back([p] { p->build_loop(); });
     p->stepper = std::thread([p] { p->step_loop(); });
     *out = s.release();
+    return CHIML_OK;
+}
+
+int chiml_server_set_batch(chiml_server_t* s, int enable) {
+    if (!s) return srv_fail("invalid argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->batch = enable != 0;
     return CHIML_OK;
 }
 

@@ -29,9 +29,10 @@ def _summary(lat, cell_updates, span, n):
             "mean_s": round(float(lat.mean()), 4)}
 
 
-def run_server(requests, hbm_budget, max_active=16, chunk_steps=64, graph_steps=32, poll_s=0.002):
-    """requests: [(arrival_s, deck_json)] sorted by arrival."""
-    srv = Server(hbm_budget, max_active=max_active, chunk_steps=chunk_steps, graph_steps=graph_steps)
+def run_server(requests, hbm_budget, max_active=16, chunk_steps=64, graph_steps=32, poll_s=0.002, batch=False, progress=0):
+    """requests: [(arrival_s, deck_json)] sorted by arrival.  batch: drained
+    simulations take their chunks together through shared launches."""
+    srv = Server(hbm_budget, max_active=max_active, chunk_steps=chunk_steps, graph_steps=graph_steps, batch=batch)
     try:
         t0 = time.perf_counter()
         pending = {}
@@ -60,6 +61,8 @@ def run_server(requests, hbm_budget, max_active=16, chunk_steps=64, graph_steps=
                     done_at = t_sub + fin  # the server's clock: last step finished on the device
                     lat.append(done_at - arr)
                     last_finish = max(last_finish, t0 + done_at)
+                    if progress and len(lat) % progress == 0:
+                        print(f"server: {len(lat)}/{len(requests)} done at {time.perf_counter() - t0:.1f} s", flush=True)
             time.sleep(poll_s)
         if failed:
             raise RuntimeError(f"{len(failed)} sweep requests failed: {failed[0]}")
@@ -70,7 +73,7 @@ def run_server(requests, hbm_budget, max_active=16, chunk_steps=64, graph_steps=
         srv.close()
 
 
-def run_one_at_a_time(requests):
+def run_one_at_a_time(requests, progress=0):
     """The plain GPU path: each request runs alone, in arrival order."""
     t0 = time.perf_counter()
     lat = []
@@ -85,5 +88,7 @@ def run_one_at_a_time(requests):
         cell_updates += _cells(deck) * sim.nsteps
         sim.close()
         lat.append(time.perf_counter() - t0 - arr)
+        if progress and len(lat) % progress == 0:
+            print(f"one at a time: {len(lat)}/{len(requests)} done at {time.perf_counter() - t0:.1f} s", flush=True)
     span = time.perf_counter() - t0 - requests[0][0]
     return _summary(lat, cell_updates, span, len(requests))

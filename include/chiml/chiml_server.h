@@ -35,6 +35,11 @@ enum { CHIML_REQ_QUEUED = 0, CHIML_REQ_BUILDING = 1, CHIML_REQ_RUNNING = 2, CHIM
  * chunk_steps: steps enqueued per simulation per refill; graph_steps: steps
  * per captured HIP graph (0 = plain launches). */
 int chiml_server_create(size_t hbm_budget, int max_active, int chunk_steps, int graph_steps, chiml_server_t** out);
+/* enable = 1: the simulations whose last chunk has finished take their next
+ * chunk together, through chiml_sim_step_batch (one launch per kernel class
+ * and phase for all of them; per-simulation graphs are then not used);
+ * 0 (the default): each simulation enqueues its chunks on its own stream. */
+int chiml_server_set_batch(chiml_server_t* s, int enable);
 /* Queues a deck (the reference's JSON input) to run for its ceil(tLim/dt) steps. */
 int chiml_server_submit(chiml_server_t* s, const char* deck_json, long long* id);
 /* State of a request and its times in seconds since submission: admitted

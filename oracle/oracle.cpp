@@ -28,6 +28,11 @@
 //                                     3D, anisotropic eps per component (1e-12)
 //   ADE (Lorentz + Drude, D form)     characteristic polynomial of the mode
 //                                     recurrence from P'' + 2g P' + w^2 P = s w^2 E
+//   CPML psi / grid lists             cell by cell (1e-11) against an independent
+//                                     textbook Roden-Gedney CPML in numpy, 3D, every
+//                                     component, kappa_max = 1 / a_max = 0 and
+//                                     kappa_max = 3 / a_max = 0.2 (end trims, profile
+//                                     positions of E and H nodes)
 //   CPML as a whole                   reflection < 2e-3 for 20 cells; PEC wall
 //                                     without it reflects > 0.1; at a corner
 //                                     (both slabs overlapping) < 1e-4 against a
@@ -57,7 +62,12 @@
 //                                     product of the detectors' E and H factors;
 //                                     a uniform static E, H through an x or z plane
 //                                     gives s e0 h0 A |sum exp(i w k dt)|^2 / nt^2
-//                                     (sign, sample count, nt = K + 1, area)
+//                                     (sign, sample count, nt = K + 1, area);
+//                                     makeParamIn / combineField offsets: a seeded
+//                                     oblique discrete plane wave through one plane of
+//                                     each normal and through a closed box equals the
+//                                     closed form with every tangential component
+//                                     interpolated to the face's integer points (1e-10)
 //   Obj::isObj + setupPhysFields      eps_c read back through one E update equals
 //                                     exact geometry at the Yee samples of Ex, Ey,
 //                                     Ez for an off-centre sphere (odd and even
@@ -67,10 +77,8 @@
 //                                     the lower slab probed on its corner line only
 // Parity unpinned (no closed form; the restatement follows the reference
 // line by line and only the golden fixtures freeze it):
-//   getPsiUpList / getGridUpList end trims and genEtaEff plane averages in
-//   partly filled PMLs, beyond the absorption bounds; flux
-//   makeParamIn / combineField face-centre offset averaging (a uniform field
-//   cannot see it) beyond the box-size invariance.
+//   genEtaEff plane averages in partly filled PMLs, beyond the absorption
+//   bounds.
 #include <algorithm>
 #include <cmath>
 #include <complex>

@@ -88,13 +88,13 @@ def _served_mix():
     return [long] + _mix()
 
 
-@pytest.mark.parametrize("graph", [0, 16])
-def test_server_results_equal_alone(graph):
+@pytest.mark.parametrize("graph,batch", [(0, False), (16, False), (0, True)])
+def test_server_results_equal_alone(graph, batch):
     mix = _served_mix()
     alone = [_alone(d) for d in mix]
     # budget that holds about two of the larger requests at once: the rest queue
     budget = 2 * max(chiml_amd.estimate_bytes(d) for d in mix)
-    srv = chiml_amd.Server(budget, max_active=3, chunk_steps=10, graph_steps=graph)
+    srv = chiml_amd.Server(budget, max_active=3, chunk_steps=10, graph_steps=graph, batch=batch)
     try:
         ids = [srv.submit(d) for d in mix[:3]]
         time.sleep(0.05)  # the rest arrive while the first ones run

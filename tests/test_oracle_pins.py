@@ -939,3 +939,307 @@ def test_partial_torus_rasterisation():
 
     _check_raster([_obj("torus", eps, loc, rad_Ring=R, rad_CrossSec=r, phiMin=phimin, phiMax=phimax)],
                   inside, eps, 23, d)
+
+
+# ------------------------------------------- flux faces: a discrete plane wave ----
+# Yee positions (in cells) of each component relative to its array index and
+# the axes along which it sits half a cell off the integer points.
+_STAGGER = {0: (0,), 1: (1,), 2: (2,), 3: (1, 2), 4: (0, 2), 5: (0, 1)}
+
+
+def _discrete_plane_wave(kvec, aux, d, dt):
+    """An exact solution of the Yee leapfrog in vacuum (c = eps = mu = 1):
+    E_c = E0_c cos(k.r_c - W t + phi) at t = n dt, H_c likewise at
+    (n - 1/2) dt, with kt_a = (2/d) sin(k_a d/2), |kt| = (2/dt) sin(W dt/2),
+    kt . E0 = 0 and H0 = kt x E0 / |kt|."""
+    k = np.asarray(kvec, float)
+    kt = 2.0 / d * np.sin(k * d / 2.0)
+    wt = float(np.linalg.norm(kt))
+    w = 2.0 / dt * math.asin(wt * dt / 2.0)
+    e0 = np.cross(kt, np.asarray(aux, float))
+    e0 /= np.linalg.norm(e0)
+    h0 = np.cross(kt, e0) / wt
+    return k, w, e0, h0
+
+
+def _seed_plane_wave(o, n, d, k, w, e0, h0, phi):
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    idx = (x, y, z)
+    for c in range(6):
+        pos = [idx[a] + (0.5 if a in _STAGGER[c] else 0.0) for a in range(3)]
+        ph = sum(k[a] * pos[a] * d for a in range(3)) + phi
+        if c < 3:
+            o.set_field(c, e0[c] * np.cos(ph))
+        else:
+            o.set_field(c, h0[c - 3] * np.cos(ph + w * o.dt / 2.0))
+
+
+def _face_dft(amp, comp, k, w, d, dt, r0, steps, omegas):
+    """DFT sums over the samples n = 1..steps (t = n dt) of a component
+    interpolated to the integer points r0 (cells; arrays broadcast): its
+    complex amplitude at r0 is amp e^{i k.r0 d} times cos(k_a d/2) for each
+    staggered axis a (the average of the two samples around r0), with H
+    half a step behind E."""
+    a = amp * np.exp(1j * sum(k[ax] * r0[ax] * d for ax in range(3)))
+    for ax in _STAGGER[comp]:
+        a = a * math.cos(k[ax] * d / 2.0)
+    if comp >= 3:
+        a = a * np.exp(1j * w * dt / 2.0)
+    t = np.arange(1, steps + 1) * dt
+    out = []
+    for om in omegas:
+        sp = np.exp(-1j * (om + w) * t).sum()
+        sm = np.exp(-1j * (om - w) * t).sum()
+        out.append(0.5 * a * sp + 0.5 * np.conj(a) * sm)
+    return np.array(out)
+
+
+def _plane_flux_expected(normal, plane, lo, npts, k, w, e0, h0, phi, d, dt, steps, omegas, shift=None):
+    """Flux through one face (normal axis, integer plane index, transverse
+    lower corner lo and point counts npts) with weight +1: Simpson's rule
+    (scipy) over the face of (Ej Hk* - Ek Hj*) / nt^2, nt = steps + 1.
+    shift = (comp, axis): move that component's samples by one cell."""
+    from scipy.integrate import simpson
+    j_ax, k_ax = {0: (1, 2), 1: (2, 0), 2: (0, 1)}[normal]
+    cj, ck = j_ax, k_ax
+    hj, hk = 3 + j_ax, 3 + k_ax
+    gj, gk = np.meshgrid(np.arange(npts[0]) + lo[0], np.arange(npts[1]) + lo[1], indexing="ij")
+    vals = {}
+    for comp in (cj, ck, hj, hk):
+        r0 = [None, None, None]
+        r0[normal] = np.full(gj.shape, float(plane))
+        r0[j_ax], r0[k_ax] = gj.astype(float), gk.astype(float)
+        if shift and shift[0] == comp:
+            r0[shift[1]] = r0[shift[1]] + 1.0
+        amp = (e0[comp] if comp < 3 else h0[comp - 3]) * np.exp(1j * phi)
+        vals[comp] = _face_dft(amp, comp, k, w, d, dt, r0, steps, omegas)
+    g = (vals[cj] * np.conj(vals[hk]) - vals[ck] * np.conj(vals[hj])) / (steps + 1) ** 2
+    # integrate over the face: along the J points then the K points (both d)
+    return simpson(simpson(g, dx=d, axis=1), dx=d, axis=1)
+
+
+@pytest.mark.parametrize("normal", [0, 1, 2])
+def test_flux_plane_of_an_oblique_discrete_plane_wave(normal):
+    """One flux plane (a one-face box) in an oblique discrete plane wave,
+    seeded through the oracle's field hook: every step's E and H at the plane
+    and its neighbours stay the exact plane wave for as many steps as the
+    plane is cells away from the walls (the leapfrog's domain of
+    dependence).  The reference samples Ek, Hj (twice), Hk (twice) and Ej
+    on boxes extended one cell down along a transverse or the normal axis
+    and averages the copies (parallelFlux.cpp:416-440, combineField
+    parallelFlux.hpp:325-395): every tangential component ends on the
+    integer points of the face, so its DFT there is known in closed form
+    (_face_dft), and so is the flux.  An oblique wave makes every sample
+    offset a phase: moving any one store by a cell changes the flux by far
+    more than the tolerance (checked on the closed form itself)."""
+    n, d, steps = 49, 0.05, 12
+    deck = decks.empty_deck((n, n, n), d, steps, 0)
+    size = [10 * d, 10 * d, 10 * d]
+    size[normal] = 0.0
+    o0 = Oracle(json.dumps(deck))
+    k, w, e0, h0 = _discrete_plane_wave([2 * math.pi / (9.1 * d), 2 * math.pi / (13.3 * d), -2 * math.pi / (11.7 * d)],
+                                        [0.3, -0.8, 0.5], d, o0.dt)
+    fc = w / (2 * math.pi)
+    deck["FluxList"].append({"name": "p", "size": size, "loc": [0.0, 0.0, 0.0], "fcen": fc, "fwidth": 0.6 * fc,
+                             "nfreq": 5})
+    o = Oracle(json.dumps(deck))
+    phi = 0.37
+    _seed_plane_wave(o, n, d, k, w, e0, h0, phi)
+    o.step(steps)
+    # the plane and its neighbours still hold the exact wave
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    sl = (slice(18, 31),) * 3
+    for c in range(6):
+        pos = [(x, y, z)[a] + (0.5 if a in _STAGGER[c] else 0.0) for a in range(3)]
+        ph = sum(k[a] * pos[a] * d for a in range(3)) + phi - w * steps * o.dt
+        ref = (e0[c] * np.cos(ph)) if c < 3 else (h0[c - 3] * np.cos(ph + w * o.dt / 2.0))
+        assert np.max(np.abs(o.field(c)[sl] - ref[sl])) < 1e-12
+    omegas = 2 * np.pi * (fc - 0.3 * fc + np.arange(5) * 0.6 * fc / 4)
+    # the plane of points 19..29 (11 points, area (10 d)^2) through the centre point 24
+    got = o.flux(0)
+    want = _plane_flux_expected(normal, 24, (19, 19), (11, 11), k, w, e0, h0, phi, d, o.dt, steps, omegas)
+    scale = np.max(np.abs(want))
+    assert scale > 0
+    assert np.max(np.abs(got - want)) <= 1e-10 * scale, (got, want)
+    # a one-cell shift of any one stored component along any axis is visible
+    j_ax, k_ax = {0: (1, 2), 1: (2, 0), 2: (0, 1)}[normal]
+    for comp in (j_ax, k_ax, 3 + j_ax, 3 + k_ax):
+        for ax in range(3):
+            bad = _plane_flux_expected(normal, 24, (19, 19), (11, 11), k, w, e0, h0, phi, d, o.dt, steps, omegas,
+                                       shift=(comp, ax))
+            assert np.max(np.abs(bad - want)) > 1e-3 * scale, (comp, ax)
+
+
+def test_flux_box_of_an_oblique_discrete_plane_wave():
+    """The six faces of a closed box (parallelFlux.cpp:46-55: upper faces with
+    weight +1, lower faces -1) in the seeded oblique plane wave: the total is
+    the sum of the six closed-form face fluxes."""
+    n, d, steps = 49, 0.05, 12
+    deck = decks.empty_deck((n, n, n), d, steps, 0)
+    o0 = Oracle(json.dumps(deck))
+    k, w, e0, h0 = _discrete_plane_wave([2 * math.pi / (10.3 * d), -2 * math.pi / (8.9 * d), 2 * math.pi / (12.1 * d)],
+                                        [-0.4, 0.2, 0.9], d, o0.dt)
+    fc = w / (2 * math.pi)
+    deck["FluxList"].append({"name": "b", "size": [10 * d] * 3, "loc": [0.0, 0.0, 0.0], "fcen": fc, "fwidth": 0.6 * fc,
+                             "nfreq": 5})
+    o = Oracle(json.dumps(deck))
+    phi = -0.81
+    _seed_plane_wave(o, n, d, k, w, e0, h0, phi)
+    o.step(steps)
+    omegas = 2 * np.pi * (fc - 0.3 * fc + np.arange(5) * 0.6 * fc / 4)
+    want = 0.0
+    for normal in range(3):
+        for plane, sign in ((29, 1.0), (19, -1.0)):
+            want = want + sign * _plane_flux_expected(normal, plane, (19, 19), (11, 11), k, w, e0, h0, phi, d, o.dt,
+                                                      steps, omegas)
+    got = o.flux(0)
+    scale = max(np.max(np.abs(want)), 1e-300)
+    # the faces nearly cancel (a plane wave through a box): compare against the
+    # size of one face's flux
+    one = np.max(np.abs(_plane_flux_expected(0, 29, (19, 19), (11, 11), k, w, e0, h0, phi, d, o.dt, steps, omegas)))
+    assert np.max(np.abs(got - want)) <= 1e-10 * one, (got, want, scale)
+
+
+# --------------------------------------- CPML against a textbook implementation ----
+def _cpml_profiles(n, T, d, dt, m=3.0, ma=1.0, kmax=1.0, amax=0.0, half=False):
+    """Roden-Gedney CPML constants along one axis of n points with T-point
+    PMLs at both ends (Taflove & Hagness ch. 7): the outermost point of each
+    side is depth L = (T - 1) d into the layer, the layer's inner edge is
+    point T - 1 (depth 0).  sigma = sigma_max (depth / L)^m,
+    sigma_max = 0.8 (m + 1) / d, kappa = 1 + (kappa_max - 1) (depth / L)^m,
+    a = a_max (1 - depth / L)^ma; half: the nodes half a cell up.  Returns
+    b, c, kappa per node (b = 1, c = 0, kappa = 1 outside the layers)."""
+    p = np.arange(n) + (0.5 if half else 0.0)
+    dist = np.minimum(p, (n - 1) - p)  # distance from the nearer outermost point
+    L = T - 1
+    inside = (dist >= 0) & (dist <= L)
+    depth = np.where(inside, (L - dist) / L, 0.0)
+    sig = np.where(inside, 0.8 * (m + 1) / d * depth ** m, 0.0)
+    kap = np.where(inside, 1.0 + (kmax - 1.0) * depth ** m, 1.0)
+    a = np.where(inside, amax * (1.0 - depth) ** ma, 0.0)
+    b = np.exp(-(sig / kap + a) * dt)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        c = np.where(sig > 0, sig * (b - 1.0) / (kap * (sig + kap * a)), 0.0)
+    return b, c, kap
+
+
+class _TextbookYee3D:
+    """Vacuum Yee leapfrog with a CPML on every face, written from the
+    textbook update (not from the reference's run/psi lists): H first, then
+    E; each derivative term along axis a is dV/(kappa_a d) + psi_a with
+    psi_a <- b_a psi_a + c_a dV/d; arrays [z, y, x], zero ghost beyond the
+    walls, edge components that have no neighbour on one side not updated."""
+
+    def __init__(self, n, T, d, dt, **kw):
+        self.n, self.d, self.dt = n, d, dt
+        self.F = [np.zeros((n, n, n)) for _ in range(6)]
+        self.psi = {}
+        prof = {False: _cpml_profiles(n, T, d, dt, **kw), True: _cpml_profiles(n, T, d, dt, half=True, **kw)}
+        self.prof = prof
+
+    def _term(self, key, dv, axis, half):
+        b, c, k = (np.asarray(v) for v in self.prof[half])
+        shape = [1, 1, 1]
+        shape[2 - axis] = -1  # axis 0 = x = last array axis
+        b, c, k = (v.reshape(shape) for v in (b, c, k))
+        ps = self.psi.get(key)
+        if ps is None:
+            ps = np.zeros_like(dv)
+        ps = b * ps + c * dv / self.d
+        self.psi[key] = ps
+        return dv / (k * self.d) + ps
+
+    @staticmethod
+    def _fwd(V, axis):  # V[r + e_a] - V[r], zero past the wall
+        ax = 2 - axis
+        out = -V.copy()
+        sl = [slice(None)] * 3
+        sl2 = [slice(None)] * 3
+        sl[ax] = slice(0, -1)
+        sl2[ax] = slice(1, None)
+        out[tuple(sl)] += V[tuple(sl2)]
+        return out
+
+    @staticmethod
+    def _bwd(V, axis):  # V[r] - V[r - e_a], zero before the wall
+        ax = 2 - axis
+        out = V.copy()
+        sl = [slice(None)] * 3
+        sl2 = [slice(None)] * 3
+        sl[ax] = slice(1, None)
+        sl2[ax] = slice(0, -1)
+        out[tuple(sl)] -= V[tuple(sl2)]
+        return out
+
+    def _mask(self, axes):
+        m = np.ones((self.n,) * 3, bool)
+        for a in axes:
+            sl = [slice(None)] * 3
+            sl[2 - a] = -1
+            m[tuple(sl)] = False
+        return m
+
+    def step(self):
+        Ex, Ey, Ez, Hx, Hy, Hz = self.F
+        dt = self.dt
+        # H: component c updated where it has both neighbours along its staggered axes
+        hx = dt * (self._term("hx_z", self._fwd(Ey, 2), 2, True) - self._term("hx_y", self._fwd(Ez, 1), 1, True))
+        hy = dt * (self._term("hy_x", self._fwd(Ez, 0), 0, True) - self._term("hy_z", self._fwd(Ex, 2), 2, True))
+        hz = dt * (self._term("hz_y", self._fwd(Ex, 1), 1, True) - self._term("hz_x", self._fwd(Ey, 0), 0, True))
+        Hx += np.where(self._mask((1, 2)), hx, 0.0)
+        Hy += np.where(self._mask((0, 2)), hy, 0.0)
+        Hz += np.where(self._mask((0, 1)), hz, 0.0)
+        ex = dt * (self._term("ex_y", self._bwd(Hz, 1), 1, False) - self._term("ex_z", self._bwd(Hy, 2), 2, False))
+        ey = dt * (self._term("ey_z", self._bwd(Hx, 2), 2, False) - self._term("ey_x", self._bwd(Hz, 0), 0, False))
+        ez = dt * (self._term("ez_x", self._bwd(Hy, 0), 0, False) - self._term("ez_y", self._bwd(Hx, 1), 1, False))
+        Ex += np.where(self._mask((0,)), ex, 0.0)
+        Ey += np.where(self._mask((1,)), ey, 0.0)
+        Ez += np.where(self._mask((2,)), ez, 0.0)
+
+
+@pytest.mark.parametrize("kw", [{}, {"kmax": 3.0, "amax": 0.2, "m": 2.0, "ma": 1.5}], ids=["kappa1_a0", "kappa3_a02"])
+def test_cpml_3d_equals_textbook_roden_gedney(kw):
+    """The oracle's 3D CPML (getPsiUpList / getGridUpList lists with their end
+    trims, parallelPML.hpp:321-668, and the addPsi kernels, parallelPML.cpp:
+    12-40) against the textbook CPML above, cell by cell for every component,
+    after a pulse has crossed every slab, edge and corner several times.  A
+    psi term on a wrong cell, a half-cell error in a profile or a missing
+    trim shows at 1e-3 or more; rounding stays below 1e-11."""
+    n, T, d, steps = 22, 6, 0.05, 70
+    deck = decks.empty_deck((n, n, n), d, steps, T)
+    if kw:
+        deck["PML"].update({"kappaMax": kw["kmax"], "aMax": kw["amax"], "m": kw["m"], "ma": kw["ma"]})
+    o = Oracle(json.dumps(deck))
+    ref = _TextbookYee3D(n, T, d, o.dt, **kw)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    bump = np.exp(-((x - 5.3) ** 2 + (y - 7.1) ** 2 + (z - 15.2) ** 2) / 6.0)
+    for c, amp in ((0, 0.4), (2, 1.0), (4, -0.3)):
+        o.set_field(c, amp * bump)
+        ref.F[c][:] = amp * bump
+    for s in range(steps):
+        o.step(1)
+        ref.step()
+    scale = max(np.max(np.abs(ref.F[c])) for c in range(6))
+    for c in range(6):
+        err = np.max(np.abs(o.field(c) - ref.F[c])) / scale
+        assert err < 1e-11, (c, err)
+
+
+def test_cpml_textbook_sensitivity():
+    """The comparison above has teeth: the textbook run with its H-node
+    profiles moved half a cell (a plausible grading mistake) differs from
+    the oracle by far more than the tolerance."""
+    n, T, d, steps = 22, 6, 0.05, 70
+    deck = decks.empty_deck((n, n, n), d, steps, T)
+    o = Oracle(json.dumps(deck))
+    ref = _TextbookYee3D(n, T, d, o.dt)
+    ref.prof[True] = _cpml_profiles(n, T, d, o.dt, half=False)  # H nodes graded as if on the E nodes
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    bump = np.exp(-((x - 5.3) ** 2 + (y - 7.1) ** 2 + (z - 15.2) ** 2) / 6.0)
+    o.set_field(2, bump)
+    ref.F[2][:] = bump
+    for s in range(steps):
+        o.step(1)
+        ref.step()
+    scale = max(np.max(np.abs(ref.F[c])) for c in range(6))
+    assert max(np.max(np.abs(o.field(c) - ref.F[c])) for c in range(6)) / scale > 1e-4
