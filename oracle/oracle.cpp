@@ -75,10 +75,13 @@
 //                                     rotated object frame
 //   detectMatInPML                    interior / upper-slab / near-slab cases and
 //                                     the lower slab probed on its corner line only
-// Parity unpinned (no closed form; the restatement follows the reference
-// line by line and only the golden fixtures freeze it):
-//   genEtaEff plane averages in partly filled PMLs, beyond the absorption
-//   bounds.
+//   genEtaEff (eta_eff of a layer)    a dielectric slab ending inside the lower or
+//                                     the upper z layer and filling part of every x
+//                                     and y layer plane: the textbook CPML with
+//                                     sigma_max / eta from numpy plane averages of the
+//                                     slab at each component's samples equals the
+//                                     oracle cell by cell (1e-11), the upper list read
+//                                     mirrored as the reference reads it
 #include <algorithm>
 #include <cmath>
 #include <complex>

@@ -1243,3 +1243,112 @@ def test_cpml_textbook_sensitivity():
         ref.step()
     scale = max(np.max(np.abs(ref.F[c])) for c in range(6))
     assert max(np.max(np.abs(o.field(c) - ref.F[c])) for c in range(6)) / scale > 1e-4
+
+
+def _eta_profile(n, T, d, dt, eps_planes, half, m=3.0, mirrored=True):
+    """sigma of the textbook CPML with sigma_max divided by the layer's
+    effective eta (genEtaEff, parallelPML.hpp:196-279): eta of a plane is
+    the mean eps_inf over the plane of the component's samples (times the
+    mean mu_inf, 1 here).  The planes of the lower layer are listed from the
+    outer wall inwards and a node at distance dist from its wall reads entry
+    int(dist); the upper layer's list runs from its inner plane outwards and
+    is read with the same distance, so it is mirrored (as the reference reads
+    it).  eps_planes: the plane means along the axis.  Returns b, c."""
+    sig = np.zeros(n)
+    L = T - 1
+    for p in range(n):
+        pos = p + (0.5 if half else 0.0)
+        lo, hi = pos, (n - 1) - pos
+        if 0 <= lo <= L:
+            eta = eps_planes[int(lo)]
+            sig[p] = 0.8 * (m + 1) / d / eta * ((L - lo) / L) ** m
+        elif 0 <= hi <= L:
+            eta = eps_planes[n - T + int(hi)] if mirrored else eps_planes[n - 1 - int(hi)]
+            sig[p] = 0.8 * (m + 1) / d / eta * ((L - hi) / L) ** m
+    b = np.exp(-sig * dt)
+    c = np.where(sig > 0, b - 1.0, 0.0)
+    return b, c
+
+
+def _eta_run(slab, mirrored):
+    """Oracle and textbook fields after the run of the partly filled layer
+    test; slab = (z_from, z_to) in cells."""
+    # an odd grid: the material samples are centred on (n - 1) / 2 like the
+    # field samples (on an even grid the reference centres materials half a
+    # cell off, which the rasterisation tests pin)
+    n, T, d, steps, eps_s = 21, 6, 0.05, 70, 2.5
+    deck = decks.empty_deck((n, n, n), d, steps, T)
+    half_cell = (n - 1) * d / 2.0
+    z_lo = slab[0] * d - half_cell
+    z_hi = slab[1] * d - half_cell
+    deck["ObjectList"].append({"shape": "block", "material": "custom", "eps": eps_s, "pols": [],
+                               "size": [3 * (n - 1) * d, 3 * (n - 1) * d, z_hi - z_lo],
+                               "loc": [0.0, 0.0, (z_hi + z_lo) / 2.0], "Basis_Set": []})
+    o = Oracle(json.dumps(deck))
+    ref = _TextbookYee3D(n, T, d, o.dt)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    eps = {}
+    for c in range(6):
+        zpos = z + (0.5 if 2 in _STAGGER[c] else 0.0)
+        eps[c] = np.where((zpos > slab[0]) & (zpos < slab[1]), eps_s, 1.0)
+
+    def planes(c, ax):
+        e = eps[c]
+        sl = [slice(None)] * 3
+        for b in range(3):
+            if b != ax and b in _STAGGER[c]:
+                sl[2 - b] = slice(0, n - 1)
+        e = e[tuple(sl)]
+        other = tuple(2 - b for b in range(3) if b != ax)
+        return e.mean(axis=other)
+    names = {0: "ex", 1: "ey", 2: "ez", 3: "hx", 4: "hy", 5: "hz"}
+    profs = {}
+    for c in range(6):
+        for ax in range(3):
+            if ax != (c % 3):
+                profs[f"{names[c]}_{'xyz'[ax]}"] = _eta_profile(n, T, d, o.dt, planes(c, ax), c >= 3, mirrored=mirrored)
+
+    def term(key, dv, axis, half):
+        b, c_ = profs[key]
+        shape = [1, 1, 1]
+        shape[2 - axis] = -1
+        b, c_ = b.reshape(shape), c_.reshape(shape)
+        ps = ref.psi.get(key)
+        if ps is None:
+            ps = np.zeros_like(dv)
+        ps = b * ps + c_ * dv / d
+        ref.psi[key] = ps
+        return dv / d + ps
+    ref._term = term
+    # the pulse is seeded in H only: with material inside a layer the reference
+    # advances D there and forms E = D / eps, so E seeded directly would not be
+    # carried into D
+    bump = np.exp(-((x - 6.3) ** 2 + (y - 14.1) ** 2 + (z - 10.2) ** 2) / 6.0)
+    for c, amp in ((3, 0.6), (4, -0.5)):
+        o.set_field(c, amp * bump)
+        ref.F[c][:] = amp * bump
+    for s in range(steps):
+        o.step(1)
+        old = [f.copy() for f in ref.F[:3]]
+        ref.step()
+        # E was advanced with dt: scale the increments by 1 / eps at each sample
+        for c in range(3):
+            ref.F[c][:] = old[c] + (ref.F[c] - old[c]) / eps[c]
+    scale = max(np.max(np.abs(ref.F[c])) for c in range(6))
+    return max(float(np.max(np.abs(o.field(c) - ref.F[c]))) for c in range(6)) / scale
+
+
+@pytest.mark.parametrize("slab", [(-3.0, 8.25), (17.25, 24.0)], ids=["lower", "upper"])
+def test_cpml_eta_eff_in_partly_filled_layers(slab):
+    """A dielectric slab (eps = 2.5) across the whole x-y extent, either
+    filling the lower z layer and the bottom of every x and y layer plane, or
+    ending inside the upper z layer (planes 17-20 filled, 15-16 empty).  The
+    textbook CPML above, with sigma_max / eta per plane (_eta_profile) from
+    numpy rasterisations of the slab at each component's samples and E
+    updated with dt / eps, equals the oracle cell by cell.  This pins
+    genEtaEff's plane averages (which samples, which planes) where the
+    material changes across a layer, and in the upper case the reference's
+    mirrored reading of its upper list: the plane at the node instead fails."""
+    assert _eta_run(slab, True) < 1e-11
+    if slab[0] > 0:
+        assert _eta_run(slab, False) > 1e-4
