@@ -40,7 +40,16 @@
 //                                     a 3D eps = 4 filled cell absorbs < 1e-4
 //                                     (genEtaEff), the 2D one is never flagged
 //   pulses                            Gaussian envelope / turn-off, default t_0,
-//                                     Ricker identically zero (trap T1)
+//                                     Ricker identically zero (trap T1); continuous
+//                                     and ramped envelopes; BH = scipy's
+//                                     Blackman-Harris (and Hann) window with its
+//                                     support edges; rectangle = scipy's analog
+//                                     Butterworth power response, off at |t-t0| >= tau
+//   named metals (Au, Ag)             plane-wave mode of a filled cell follows the
+//                                     central-difference Rakic Lorentz-Drude terms
+//                                     from the published table (1e-12)
+//   broadening members                member (i, j) = w_i w_j x single-member run;
+//                                     pops = weighted sum (1e-14)
 //   grid / time step                  n = floor(size/d + 0.5) + 1, 2D dt with the
 //                                     z term, ceil(tLim/dt) steps
 //   sources                           cell placement, added after the H update

@@ -120,6 +120,100 @@ def test_gaussian_default_t0_is_cutoff_over_fwidth_squared():
     assert env[0] == pytest.approx(env[1], rel=1e-10)
 
 
+def test_continuous_pulse_is_a_cosine_from_time_zero():
+    """continuous: E0 exp(-2 pi i f t), phase referenced to t = 0, never off;
+    with the default I0 the amplitude is the deck's Field_Intensity."""
+    fcen, e0 = 0.23, 1.7
+    deck = _source_deck({"profile": "continuous", "fcen": fcen, "Field_Intensity": e0})
+    for t in np.linspace(0.0, 400.0, 2001):
+        assert pulse(deck, 0, t) == pytest.approx(e0 * math.cos(2 * math.pi * fcen * t), rel=1e-12, abs=1e-12)
+
+
+def test_ramped_continuous_envelope_is_min_of_ramp_and_one():
+    fcen, ramp, e0 = 0.31, 0.05, 2.0
+    deck = _source_deck({"profile": "ramped_cont", "fcen": fcen, "ramp_val": ramp, "Field_Intensity": e0})
+    for t in np.linspace(0.0, 3.0 / ramp, 1201):
+        carrier = math.cos(2 * math.pi * fcen * t)
+        if abs(carrier) < 0.2:
+            continue
+        assert pulse(deck, 0, t) / carrier == pytest.approx(e0 * min(ramp * t, 1.0), rel=1e-12, abs=1e-14)
+    # continuous at the end of the ramp
+    t1 = 1.0 / ramp
+    assert pulse(deck, 0, t1 - 1e-9) == pytest.approx(pulse(deck, 0, t1 + 1e-9), rel=1e-6)
+
+
+def _bh_times(t0, tau, m):
+    return t0 - tau / 2 + tau * np.arange(m) / (m - 1)
+
+
+def test_blackman_harris_pulse_is_the_scipy_window():
+    """BH with fcen = 0 is E0 times the 4-term Blackman-Harris window centred at
+    t_0 with total width tau (scipy.signal.windows.blackmanharris, symmetric),
+    zero outside [t_0 - tau/2, t_0 + tau/2]; tau defaults to 1/fwidth (fwidth
+    is read even when tau is given, as the reference's default argument is
+    evaluated first, parallelInputs.cpp:143)."""
+    from scipy.signal import windows
+    t0, tau, e0, m = 40.0, 25.0, 1.3, 101
+    for spec in ({"tau": tau, "fwidth": 0.3}, {"fwidth": 1.0 / tau}):
+        deck = _source_deck(dict({"profile": "BH", "fcen": 0.0, "t_0": t0, "Field_Intensity": e0}, **spec))
+        got = np.array([pulse(deck, 0, t) for t in _bh_times(t0, tau, m)])
+        want = e0 * windows.blackmanharris(m)
+        assert np.max(np.abs(got - want)) < 1e-13
+        for t in (t0 - tau / 2 - 1e-7, t0 + tau / 2 + 1e-7, 0.0, t0 + tau):
+            assert pulse(deck, 0, t) == 0.0
+        assert pulse(deck, 0, t0 + tau / 2) == pytest.approx(e0 * 6.0e-5, rel=1e-6)  # the window's edge value
+
+
+def test_blackman_harris_coefficients_and_carrier():
+    """BH1..BH4 replace the window's coefficients (0.5, 0.5, 0, 0 is the Hann
+    window) and the carrier's phase is referenced to t_0."""
+    from scipy.signal import windows
+    t0, tau, m, fcen = 30.0, 20.0, 81, 0.4
+    deck = _source_deck({"profile": "BH", "fcen": 0.0, "t_0": t0, "tau": tau, "fwidth": 1.0, "BH1": 0.5, "BH2": 0.5,
+                         "BH3": 0.0, "BH4": 0.0, "Field_Intensity": 1.0})
+    got = np.array([pulse(deck, 0, t) for t in _bh_times(t0, tau, m)])
+    assert np.max(np.abs(got - windows.hann(m))) < 1e-13
+    deck = _source_deck({"profile": "BH", "fcen": fcen, "t_0": t0, "tau": tau, "fwidth": 1.0, "Field_Intensity": 1.0})
+    ts = _bh_times(t0, tau, m)
+    w = windows.blackmanharris(m)
+    for t, wt in zip(ts[5:-5], w[5:-5]):
+        assert pulse(deck, 0, t) == pytest.approx(wt * math.cos(2 * math.pi * fcen * (t - t0)), rel=1e-10, abs=1e-13)
+
+
+@pytest.mark.parametrize("n", [1, 3, None])
+def test_rectangle_pulse_is_a_butterworth_power_response(n):
+    """rectangle with fcen = 0 is E0 / ((2 (t - t_0)/tau)^(2n) + 1): the power
+    response |H(j x)|^2 of an analog Butterworth filter of order n (scipy) at
+    x = 2 |t - t_0| / tau, half maximum at t_0 +- tau/2, switched off for
+    |t - t_0| >= tau; n defaults to 30."""
+    from scipy.signal import butter, freqs_zpk
+    t0, tau, e0 = 50.0, 16.0, 0.9
+    spec = {"profile": "rectangle", "fcen": 0.0, "t_0": t0, "tau": tau, "Field_Intensity": e0}
+    if n is not None:
+        spec["n"] = n
+    order = 30 if n is None else n
+    deck = _source_deck(spec)
+    ts = np.linspace(t0 - tau + 1e-6, t0 + tau - 1e-6, 401)
+    z, p, k = butter(order, 1.0, analog=True, output="zpk")
+    _, h = freqs_zpk(z, p, k, worN=np.abs(2 * (ts - t0) / tau))
+    got = np.array([pulse(deck, 0, t) for t in ts])
+    assert np.max(np.abs(got - e0 * np.abs(h) ** 2)) < 1e-12
+    assert pulse(deck, 0, t0 + tau / 2) == pytest.approx(e0 / 2, rel=1e-14)
+    assert pulse(deck, 0, t0 - tau / 2) == pytest.approx(e0 / 2, rel=1e-14)
+    for t in (t0 - tau, t0 + tau, t0 + tau + 1.0, 0.0):
+        assert pulse(deck, 0, t) == 0.0
+    if order == 1:  # the turn-off is a step from E0/5 to zero
+        assert pulse(deck, 0, t0 + tau - 1e-9) == pytest.approx(e0 / 5, rel=1e-7)
+
+
+def test_rectangle_pulse_carrier_phase_at_t0():
+    t0, tau, fcen = 50.0, 16.0, 0.37
+    deck = _source_deck({"profile": "rectangle", "fcen": fcen, "t_0": t0, "tau": tau, "n": 2, "Field_Intensity": 1.0})
+    for t in np.linspace(t0 - 0.9 * tau, t0 + 0.9 * tau, 97):
+        env = 1.0 / ((2 * (t - t0) / tau) ** 4 + 1.0)
+        assert pulse(deck, 0, t) == pytest.approx(env * math.cos(2 * math.pi * fcen * (t - t0)), rel=1e-10, abs=1e-14)
+
+
 # ---------------------------------------------------- grid and time step ----
 def test_time_step_2d_includes_z_term_and_point_count():
     d = 0.05
@@ -212,12 +306,18 @@ def _char_poly(eps_inf, poles, s):
     (z - 1)^2 (eps_inf prod Q + sum_p c_p z prod_{r!=p} Q_r) + s z prod Q = 0,
     Q_p = z^2 - a_p z - x_p.
     """
+    axc = []
+    for (w, g, sig, dt) in poles:
+        axc.append(((2 - (w * dt) ** 2) / (1 + g * dt), (g * dt - 1) / (1 + g * dt), sig * (w * dt) ** 2 / (1 + g * dt)))
+    return _char_poly_axc(eps_inf, axc, s)
+
+
+def _char_poly_axc(eps_inf, axc, s):
+    """_char_poly for poles given as recurrence coefficients (a, x, c)."""
     Q = []
     C = []
-    for (w, g, sig, dt) in poles:
-        a = (2 - (w * dt) ** 2) / (1 + g * dt)
-        x = (g * dt - 1) / (1 + g * dt)
-        C.append(sig * (w * dt) ** 2 / (1 + g * dt))
+    for (a, x, c) in axc:
+        C.append(c)
         Q.append(np.array([1.0, -a, -x]))
     prodQ = np.array([1.0])
     for q in Q:
@@ -273,6 +373,112 @@ def test_ade_drude_lorentz_mode_recurrence():
     k = len(wrong)
     window = amps[10:10 + k][::-1]
     assert abs(float(wrong @ window)) > 1e-6 * float(np.abs(wrong) @ np.abs(window))
+
+
+# Lorentz-Drude parameters of Rakic et al., Appl. Opt. 37, 5271 (1998), in eV:
+# plasma frequency, Drude (f0, Gamma0), oscillators (f_j, Gamma_j, omega_j).
+_RAKIC = {
+    "Au": (9.03, (0.760, 0.053), [(0.024, 0.241, 0.415), (0.010, 0.345, 0.830), (0.071, 0.870, 2.969),
+                                  (0.601, 2.494, 4.304), (4.384, 2.214, 13.32)]),
+    "Ag": (9.01, (0.845, 0.048), [(0.065, 3.886, 0.816), (0.124, 0.452, 4.481), (0.011, 0.065, 8.185),
+                                  (0.840, 0.916, 9.083), (5.646, 2.419, 20.29)]),
+}
+# eV -> angular frequency in units of c/a with the reference's Planck constant
+# (parallelInputs.cpp:1182-1185) and speed of light
+_H_EV, _C_REF = 4.135666e-15, 2.9979245e8
+
+
+def _ev_to_w(ev):
+    return 2 * math.pi * ev / _H_EV * A_UNIT / _C_REF
+
+
+def _ld_eps(metal, w):
+    """Textbook Lorentz-Drude permittivity (exp(-i w t) convention):
+    eps = 1 - f0 wp^2 / (w (w + i G0)) + sum_j f_j wp^2 / (w_j^2 - w^2 - i w G_j)."""
+    wp, (f0, g0), osc = _RAKIC[metal]
+    wp = _ev_to_w(wp)
+    eps = 1.0 - f0 * wp ** 2 / (w * (w + 1j * _ev_to_w(g0)))
+    for f, g, w0 in osc:
+        eps += f * wp ** 2 / (_ev_to_w(w0) ** 2 - w ** 2 - 1j * w * _ev_to_w(g))
+    return eps
+
+
+def _ld_recurrence(metal, dt, gamma_scale=1.0):
+    """Central-difference recurrence P^{n+1} = a P^n + x P^{n-1} + c E^n of
+    each Lorentz-Drude term P'' + G P' + w0^2 P = f wp^2 E (Drude: w0 = 0)."""
+    wp, (f0, g0), osc = _RAKIC[metal]
+    wp = _ev_to_w(wp)
+    out = []
+    for f, g, w0 in [(f0, g0, 0.0)] + list(osc):
+        h = gamma_scale * _ev_to_w(g) * dt / 2
+        w0 = _ev_to_w(w0)
+        out.append(((2 - (w0 * dt) ** 2) / (1 + h), (h - 1) / (1 + h), f * wp ** 2 * dt ** 2 / (1 + h)))
+    return out
+
+
+def _discrete_chi(axc, w, dt):
+    z = np.exp(-1j * w * dt)
+    return sum(c / (z - a - x / z) for (a, x, c) in axc)
+
+
+def _mode_sequence(eps_inf, axc, s, steps):
+    """E amplitude of one grid mode (curl curl -> s / dt^2) of a medium with
+    D = eps_inf E + sum P after E^0 = 1 is written with D, H and P at zero:
+    D^{n+1} = 2 D^n - D^{n-1} - s E^n (D^{-1} = D^0 = 0, H starts at zero),
+    P^{n+1} = a P^n + x P^{n-1} + c E^n, E^{n+1} = (D^{n+1} - sum P^{n+1}) / eps_inf."""
+    e, d0, d1 = 1.0, 0.0, 0.0
+    p = [(0.0, 0.0) for _ in axc]
+    out = [e]
+    for _ in range(steps):
+        d0, d1 = 2 * d0 - d1 - s * e, d0
+        p = [(a * p0 + x * p1 + c * e, p0) for (a, x, c), (p0, p1) in zip(axc, p)]
+        e = (d0 - sum(q[0] for q in p)) / eps_inf
+        out.append(e)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("metal,eps_inf", [("Au", 1.0 + 1.0e-14), ("Ag", 1.0 + 1.1e-14)])
+def test_named_metal_is_the_rakic_lorentz_drude_medium(metal, eps_inf):
+    """A cell filled with the named metal carries a plane-wave E mode whose
+    amplitude follows the central-difference discretisation of the Rakic
+    Lorentz-Drude terms (Drude at w0 = 0), built here from the published table;
+    that discretisation's susceptibility approaches the textbook eps(w) as
+    (w dt)^2 at 500 and 800 nm.  A damping rate off by a factor 2 (Gamma versus
+    Gamma/2 in P') or a dropped oscillator moves the sequence by far more than
+    the tolerance."""
+    n, pml, d, steps = 120, 6, 0.05, 40
+    deck = decks.empty_deck((n, n, 1), d, steps + 1, pml)
+    deck["ObjectList"].append({"shape": "block", "material": metal, "size": [(n - 1) * d] * 2 + [0.0],
+                               "loc": [0, 0, 0], "Basis_Set": []})
+    o = Oracle(json.dumps(deck))
+    dt = o.dt
+    axc = _ld_recurrence(metal, dt)
+    for lam in (500e-9, 800e-9):
+        w = 2 * math.pi * A_UNIT / lam
+        chi_d = _discrete_chi(axc, w, dt)
+        eps = _ld_eps(metal, w)
+        assert abs(eps_inf + chi_d - eps) < 2 * (w * dt) ** 2 * abs(eps - 1.0)
+        assert eps.real < -2.0  # a metal in the visible
+    kx, ky = 2 * math.pi / (10.1 * d), 2 * math.pi / (17.9 * d)
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ez0 = np.cos(kx * i * d + ky * j * d + 0.4).reshape(1, n, n)
+    o.set_field(2, ez0)
+    s = dt ** 2 * (4 * math.sin(kx * d / 2) ** 2 + 4 * math.sin(ky * d / 2) ** 2) / d ** 2
+    m = pml + steps + 3
+    sl = (0, slice(m, n - m), slice(m, n - m))
+    amps = [1.0]
+    for _ in range(steps):
+        o.step(1)
+        r, res = _center_ratio(ez0, o.field(2), sl)
+        assert res < 1e-11
+        amps.append(r)
+    amps = np.array(amps)
+    want = _mode_sequence(eps_inf, axc, s, steps)
+    assert np.max(np.abs(amps - want)) < 1e-12
+    vac = _mode_sequence(1.0, [], s, steps)
+    assert np.max(np.abs(want - vac)) > 0.1  # the metal's response is visible
+    for bad in (_ld_recurrence(metal, dt, gamma_scale=2.0), axc[:-1], axc[1:]):
+        assert np.max(np.abs(_mode_sequence(eps_inf, bad, s, steps) - want)) > 1e-6
 
 
 # --------------------------------------------------------- CPML ---------
@@ -462,6 +668,47 @@ def test_population_detector_reads_flat_index():
     rho = o.density(0, 0, 0, 2)
     assert o.pops(0, 0)[-1] == pytest.approx(rho[0, 0], abs=1e-15)
     assert o.pops(0, 1)[-1] == pytest.approx(rho[1, 1], abs=1e-15)
+
+
+def _broadened_oracle(levels):
+    deck = decks.empty_deck((9, 9, 1), 0.01, 10, 2)
+    em = decks.two_level_emitter(0.0, 0.0, (0.0, 0.0, 0.0), e_ev=2.0, dipole_debye=10.0, t1=2e-15, t2=1e-15)
+    em["Energy_Levels"] = levels
+    deck["ObjectList"].append(em)
+    return Oracle(json.dumps(deck))
+
+
+def test_broadening_members_are_the_weighted_single_member_runs():
+    """Energy_Levels with several E_cen per level make one member per
+    combination (first level outermost, weights multiplied), each starting in
+    its ground state with its weight as population.  The Liouville equation is
+    linear in rho, so under a prescribed field member (i, j) of the ensemble
+    equals w_i w_j times the single-member run at (E_i, E_j), and the
+    population detector reads the weighted sum of those runs."""
+    e0s, w0s = [0.0, 0.05], [0.6, 0.4]
+    e1s, w1s = [1.9, 2.0, 2.2], [0.2, 0.5, 0.3]
+    ens = _broadened_oracle([{"E_cen": e0s, "weights": w0s}, {"E_cen": e1s, "weights": w1s}])
+    w0 = 2.0 * Q_E / HBAR_SI * A_UNIT / C_SI
+    field = (0.0, 0.0, 0.3 * w0 * HBAR_SI / (2 * 10.0 * DEBYE) * C_SI / A_UNIT)  # Rabi rate 0.3 w0
+    steps = 600
+    ens.qe_drive(0, field, steps)
+    pops_sum = [np.zeros(steps, dtype=complex), np.zeros(steps, dtype=complex)]
+    for i, (ea, wa) in enumerate(zip(e0s, w0s)):
+        for j, (eb, wb) in enumerate(zip(e1s, w1s)):
+            one = _broadened_oracle([{"E_cen": [ea]}, {"E_cen": [eb]}])
+            one.qe_drive(0, field, steps)
+            rho_one = one.density(0, 0, 0, 2)
+            assert abs(rho_one[1, 1]) > 1e-3  # the field moved population
+            got = ens.density(0, i * len(e1s) + j, 0, 2)
+            assert np.max(np.abs(got - wa * wb * rho_one)) < 1e-14
+            for lev in (0, 1):
+                pops_sum[lev] += wa * wb * one.pops(0, lev)
+    for lev in (0, 1):
+        assert np.max(np.abs(ens.pops(0, lev) - pops_sum[lev])) < 1e-13
+    # the members differ: a detuned member is not a copy of another
+    a = ens.density(0, 0, 0, 2) / (w0s[0] * w1s[0])
+    b = ens.density(0, 2, 0, 2) / (w0s[0] * w1s[2])
+    assert np.max(np.abs(a - b)) > 1e-3
 
 
 # ------------------------------------------------------------- flux --------
