@@ -112,3 +112,27 @@ def test_oracle_is_test_infrastructure_only():
                 assert "oracle" not in text, os.path.join(dirpath, fn)
     with open(os.path.join(ROOT, "oracle", "oracle.cpp")) as f:
         assert "test infrastructure only" in f.read(400)
+
+
+def _build_c_consumer(out_dir):
+    """Compiles tests/c/abi_consumer.c, a C99 program that includes every
+    public header and builds a chiml_problem_t by hand, against the library."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    lib_dir = os.path.dirname(os.path.abspath(chiml_amd.load_library()._name))
+    exe = os.path.join(str(out_dir), "abi_consumer")
+    cmd = [gcc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "c", "abi_consumer.c"), "-L", lib_dir, "-lchiml_amd",
+           f"-Wl,-rpath,{lib_dir}", "-lm", "-o", exe]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return exe
+
+
+def test_c_consumer_compiles_strict_c99_and_links(tmp_path):
+    import subprocess
+    exe = _build_c_consumer(tmp_path)
+    out = subprocess.run([exe, "--link-only"], check=True, capture_output=True, text=True).stdout
+    assert "linked 16 entry points" in out

@@ -73,6 +73,7 @@ def test_2d_detectors_on_every_component(precision, tol):
         t_g, v_g = sim.dtcArr()[k].series()
         assert v_g.shape == v_o.shape, k
         assert np.abs(v_o).max() > 0.0, k
+        print(f"{precision} detector {k}: rel_linf={rel_linf(v_g, v_o):.3e}", flush=True)
         assert rel_linf(v_g, v_o) <= tol, (k, rel_linf(v_g, v_o))
 
 
@@ -212,6 +213,7 @@ def test_3d_n_level_emitter_block_fp32_fields(nlev):
     for j in range(2):
         pg, po = q.population(j), orc.pops(0, j)
         assert pg.shape == po.shape
+        print(f"population {j}: rel_linf={rel_linf(pg, po):.3e}", flush=True)
         assert rel_linf(pg, po) <= 1e-4, (j, rel_linf(pg, po))
 
 
