@@ -92,7 +92,8 @@ struct DetectorSpec {
     std::array<int, 3> loc{{0, 0, 0}}, sz{{1, 1, 1}};
     double timeInterval = 0.0;
     std::string name;
-    std::vector<double> freqs;
+    std::vector<double> freqs;  // FREQ class: angular frequencies (c/a units)
+    bool outputMaps = false;    // FREQ class: one map file per frequency
 };
 
 struct FluxSpec {

@@ -140,10 +140,29 @@ def test_config2_scaled_fp64(config2_oracle):
     sim.close()
 
 
+def _family_detectors(sim, orc, tol, label):
+    """Detectors k = 6 j + c read component c (Ex..Hz) at location j.  FP32
+    rounding is relative to the field's magnitude, so a component that is
+    nearly zero there (Ey and Hx on the sheet's axis, set by scattering only)
+    carries the rounding of the strong ones: each series' error is scaled by
+    the largest peak of its family (E or H) at the same location.  The
+    error relative to its own peak is printed beside it."""
+    n = orc.num_detectors()
+    ser = [(orc.detector(k)[1], sim.dtcArr()[k].series()[1]) for k in range(n)]
+    for k, (vo, vg) in enumerate(ser):
+        assert vg.shape == vo.shape, (label, k)
+        fam = [j for j in range(n) if j // 6 == k // 6 and (j % 6 < 3) == (k % 6 < 3)]
+        scale = max(float(np.abs(ser[j][0]).max()) for j in fam)
+        err = float(np.abs(vg - vo).max()) / scale
+        print(f"{label}: detector {k} (comp {k % 6}) err/family peak {err:.3e}, "
+              f"err/own peak {rel_linf(vg, vo):.3e}", flush=True)
+        assert err <= tol, (label, k, err)
+
+
 def test_config2_scaled_fp32_flux_and_detectors(config2_oracle):
     orc = config2_oracle
     sim = _gpu(_config2_128("fp32"), 400)
-    _detectors(sim, orc, FP32_TOL, "config 2 fp32")
+    _family_detectors(sim, orc, FP32_TOL, "config 2 fp32")
     err = rel_linf(sim.fluxArr()[0].getFlux(), orc.flux(0))
     print(f"config 2 fp32: flux rel_linf {err:.3e}", flush=True)
     assert err <= FP32_TOL

@@ -16,6 +16,9 @@ from tests.helpers import rel_linf
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-9
+# FP32 fields, detector series, flux and populations against the FP64 oracle
+# (the bound the FP32 path is held to; measured 1e-7..5e-6 on these decks)
+FP32_TOL = 1e-5
 
 
 def _pair(deck, steps):
@@ -54,7 +57,7 @@ def test_2d_tmz_cylinder_fp64(profile):
         assert rel_linf(v_g, v_o) <= FP64_TOL
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", 1e-4)])
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
 def test_2d_detectors_on_every_component(precision, tol):
     """A small 2D deck whose detectors read Ez, Hx, Hy and E power (2D decks
     this small sample their point detectors inside the E kernel)."""
@@ -90,7 +93,7 @@ def test_3d_aniso_drude_lorentz_flux_fp64():
 def test_3d_fp32_close_to_fp64_oracle():
     deck = decks.config2_3d(n=28, steps=120, pml=6, d=0.02, sphere_cells=6, flux_cells=12, nfreq=8, precision="fp32")
     sim, orc = _pair(deck, 120)
-    _fields(sim, orc, range(6), 1e-4)
+    _fields(sim, orc, range(6), FP32_TOL)
     fg = sim.fluxArr()[0].getFlux()
     fo = orc.flux(0)
     err = rel_linf(fg, fo)
@@ -131,7 +134,7 @@ def _split_deck(variant, precision):
 
 
 @pytest.mark.parametrize("variant", ["cube", "wide", "thinz", "pmlsrc"])
-@pytest.mark.parametrize("precision,tol_f,tol_flux", [("fp64", FP64_TOL, 1e-8), ("fp32", 1e-4, 1e-5)])
+@pytest.mark.parametrize("precision,tol_f,tol_flux", [("fp64", FP64_TOL, 1e-8), ("fp32", FP32_TOL, FP32_TOL)])
 def test_3d_split_kernels(variant, precision, tol_f, tol_flux):
     deck = _split_deck(variant, precision)
     print(f"split variant={variant} precision={precision}", flush=True)
@@ -155,7 +158,7 @@ def test_3d_fused_step_fp32(variant, monkeypatch):
     split.sync()
     monkeypatch.setenv("CHIML_FUSED", "1")
     sim, orc = _pair(deck, 100)
-    _fields(sim, orc, range(6), 1e-4)
+    _fields(sim, orc, range(6), FP32_TOL)
     for c in range(6):
         assert rel_linf(sim.field(c), split.field(c)) <= 1e-5, c
     err = rel_linf(sim.fluxArr()[0].getFlux(), orc.flux(0))
@@ -196,7 +199,7 @@ def test_3d_n_level_emitter_block_fp64(nlev):
 def test_3d_n_level_emitter_block_fp32_fields(nlev):
     """The same block with FP32 fields and FP64 density matrices (the 3D
     emitter workload's precision): fields within the FP32 field bound of the
-    FP64 oracle, populations within 1e-4 of theirs."""
+    FP64 oracle, populations within 1e-5 of theirs."""
     deck = _n_level_block(nlev, "fp32")
     print(f"n-level block N={nlev} fp32 fields", flush=True)
     sim, orc = _pair(deck, 90)
@@ -208,13 +211,13 @@ def test_3d_n_level_emitter_block_fp32_fields(nlev):
         for c, o in zip(comps, ref):
             err = np.abs(sim.field(c) - o).max() / scale
             print(f"field {c}: max|ref|={np.abs(o).max():.3e} err/scale={err:.3e}", flush=True)
-            assert err <= 1e-4, (c, err)
+            assert err <= FP32_TOL, (c, err)
     q = sim.qeArr()[0]
     for j in range(2):
         pg, po = q.population(j), orc.pops(0, j)
         assert pg.shape == po.shape
         print(f"population {j}: rel_linf={rel_linf(pg, po):.3e}", flush=True)
-        assert rel_linf(pg, po) <= 1e-4, (j, rel_linf(pg, po))
+        assert rel_linf(pg, po) <= FP32_TOL, (j, rel_linf(pg, po))
 
 
 @pytest.mark.parametrize("dim", [2, 3])
@@ -309,11 +312,11 @@ def test_3d_basis_set_emitters_x_y_dipoles_fp64():
 
 def test_2d_tmz_fp32_close_to_fp64_oracle():
     """FP32 fields against the FP64 oracle: detector series within 1e-5 of the
-    series' peak, fields within 1e-4."""
+    series' peak, fields within 1e-5."""
     deck = decks.config1_2d(n=64, steps=300, profile="gaussian", pml=8, cyl_radius_cells=12, src_cell=(20, 32),
                             ndet=4, precision="fp32")
     sim, orc = _pair(deck, 300)
-    _fields(sim, orc, (2, 3, 4), 1e-4)
+    _fields(sim, orc, (2, 3, 4), FP32_TOL)
     for d in range(orc.num_detectors()):
         v_o = orc.detector(d)[1]
         v_g = sim.dtcArr()[d].series()[1]
