@@ -42,6 +42,10 @@ def load_library():
     L.chiml_sim_field.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p]
     L.chiml_sim_detector.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, c_dbl_p, ctypes.c_size_t, c_size_p, c_int_p]
     L.chiml_sim_flux.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t, c_int_p]
+    L.chiml_sim_num_freq_detectors.argtypes = [ctypes.c_void_p, c_int_p]
+    L.chiml_sim_freq_info.argtypes = [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_dbl_p]
+    L.chiml_sim_freq_fields.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t]
+    L.chiml_sim_freq_values.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t]
     L.chiml_sim_pops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_size_t, c_size_p]
     L.chiml_sim_density.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_dbl_p, c_int_p]
     L.chiml_sim_cost.argtypes = [ctypes.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]
@@ -69,6 +73,9 @@ def load_library():
     L.chiml_write_detector_txt.argtypes = [ctypes.c_char_p, c_dbl_p, c_dbl_p, ctypes.c_size_t, ctypes.c_int, c_dbl_p]
     L.chiml_write_detector_bin.argtypes = [ctypes.c_char_p, c_dbl_p, c_dbl_p, ctypes.c_size_t, c_int_p, c_int_p]
     L.chiml_write_flux.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int]
+    L.chiml_write_freq_table.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int]
+    L.chiml_write_freq_maps.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        c_dbl_p, ctypes.c_int]
     L.chiml_write_pops.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double]
     L.chiml_sim_write_outputs.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     L.chiml_sim_last_error.restype = ctypes.c_char_p
@@ -117,6 +124,27 @@ def write_flux(path, freqs, faces, front_back=False):
                                            1 if front_back else 0))
 
 
+def write_freq_table(path, freqs, values, nincd):
+    """The reference's frequency-detector table (parallelDetectorFREQ_Base::toFile
+    with incident fields) from values[grids, nfreq]; the nincd incident
+    fields are zero."""
+    f = _f64(freqs)
+    v = np.asarray(values, dtype=np.complex128).reshape(-1, f.size)
+    reim = _f64(np.stack([v.real, v.imag], axis=-1))
+    _check(load_library().chiml_write_freq_table(str(path).encode(), _dp(f), f.size, v.shape[0], _dp(reim), int(nincd)))
+
+
+def write_freq_maps(path, freqs, sz, values, nincd):
+    """The reference's frequency-detector maps (toMap with incident fields),
+    one file per frequency, from values[grids, nfreq, box points]."""
+    f = _f64(freqs)
+    npt = int(sz[0]) * int(sz[1]) * int(sz[2])
+    v = np.asarray(values, dtype=np.complex128).reshape(-1, f.size, npt)
+    reim = _f64(np.stack([v.real, v.imag], axis=-1))
+    s3 = (ctypes.c_int * 3)(*[int(x) for x in sz])
+    _check(load_library().chiml_write_freq_maps(str(path).encode(), _dp(f), f.size, v.shape[0], s3, _dp(reim), int(nincd)))
+
+
 def write_pops(path, pops, interval, dt):
     """The reference's population file (QEPopDtc::toFile)."""
     p = np.asarray(pops, dtype=np.complex128)
@@ -141,6 +169,42 @@ class Detector:
         v = np.zeros(ns.value * npts.value)
         _check(L.chiml_sim_detector(self._sim._h, self.index, _dp(t), _dp(v), ns.value, ctypes.byref(ns), ctypes.byref(npts)))
         return t, v.reshape(ns.value, npts.value)
+
+
+class FreqDetector:
+    """A frequency detector (parallelDetectorFREQ_Base): running DFTs of the
+    fields over a box of points, read out per grid and frequency."""
+
+    def __init__(self, sim, index):
+        self._sim = sim
+        self.index = index
+
+    def info(self):
+        """(printed frequencies, grids, sampled points per grid, samples taken)."""
+        L = load_library()
+        nf, ng, npts, nt = (ctypes.c_int() for _ in range(4))
+        _check(L.chiml_sim_freq_info(self._sim._h, self.index, ctypes.byref(nf), ctypes.byref(ng), ctypes.byref(npts),
+                                     ctypes.byref(nt), None))
+        fr = np.zeros(nf.value)
+        _check(L.chiml_sim_freq_info(self._sim._h, self.index, ctypes.byref(nf), ctypes.byref(ng), ctypes.byref(npts),
+                                     ctypes.byref(nt), _dp(fr)))
+        return fr, ng.value, npts.value, nt.value
+
+    def fields(self):
+        """Accumulated DFT, shape (grids, points, nfreq); points x fastest, then z, then y."""
+        fr, ng, npts, _ = self.info()
+        buf = np.zeros(max(2 * ng * npts * fr.size, 2))
+        if ng * npts * fr.size > 0:
+            _check(load_library().chiml_sim_freq_fields(self._sim._h, self.index, _dp(buf), buf.size))
+        n = ng * npts * fr.size
+        return (buf[0:2 * n:2] + 1j * buf[1:2 * n:2]).reshape(ng, npts, fr.size)
+
+    def values(self):
+        """The table's values, shape (grids, nfreq)."""
+        fr, ng, _, _ = self.info()
+        buf = np.zeros(2 * ng * fr.size)
+        _check(load_library().chiml_sim_freq_values(self._sim._h, self.index, _dp(buf), buf.size))
+        return (buf[0::2] + 1j * buf[1::2]).reshape(ng, fr.size)
 
 
 class Flux:
@@ -267,6 +331,11 @@ class FDTDField:
 
     def dtcArr(self):
         return [Detector(self, i) for i in range(self._ndet)]
+
+    def dtcFreqArr(self):
+        n = ctypes.c_int()
+        _check(load_library().chiml_sim_num_freq_detectors(self._h, ctypes.byref(n)))
+        return [FreqDetector(self, i) for i in range(n.value)]
 
     def fluxArr(self):
         return [Flux(self, i) for i in range(self._nflux)]

@@ -8,7 +8,7 @@
 //                    (addPul, updateE/updateD, updateEPML, D2E)
 //                    3D FP32: k_h3_all / k_e3_all; 3D FP64: k_h3_int + k_h3_slab /
 //                    k_e3_int + k_e3_slab; 2D: k_h2v / k_e2v; else k_update_h / k_update_e
-//   3. emitters    : k_ml_addp then k_ml_team and k_ml_pop_finish
+//   3. emitters    : k_ml_addp then k_ml_team (its last block finishes the populations)
 //                    (parallelQEBase::addQE)
 //   4. t_step += 1 : plain launches are given the step number by the host;
 //                    inside captured graphs the H launch advances a two-slot
@@ -1602,6 +1602,17 @@ __global__ void __launch_bounds__(256) k_ade3(StepArgs<T> a, int ade_bpc) {
     if (c < 3) ade_block<T, W, SRC>(a, c, blockIdx.x - c * ade_bpc);
 }
 
+// One set of DFT accumulators: its sampled cells, sampling interval and
+// staging buffer [kDftBlock][n].
+constexpr int kMaxDftGroups = 8;
+template <typename T>
+struct DftSample {
+    int n, interval;
+    const int* comp;
+    const long long* cell;
+    T* stage;
+};
+
 template <typename T>
 struct SampleArgs {
     const T* F[6];
@@ -1622,11 +1633,10 @@ struct SampleArgs {
     const long long* det_out_off;  // per detector offset into det_out
     const int* det_cap;            // per detector max samples
     int totpts;
-    int ndft;
-    const int* dft_comp;
-    const long long* dft_cell;
-    int dft_interval;
-    T* stage;  // [kDftBlock][ndft]
+    int ndft;  // sampled DFT cells over all groups
+    int ndg;   // DFT groups (0: the flux boxes; then the frequency detectors)
+    int gofs[kMaxDftGroups + 1];  // first sample thread of each group
+    DftSample<T> dg[kMaxDftGroups];
 };
 
 // Stores detector point t's sample at `step` from its two field values u, v
@@ -1791,10 +1801,18 @@ __device__ __forceinline__ void sample_at(const SampleArgs<T>& a, int t) {
         }
         a.det_out[a.det_out_off[d] + static_cast<long long>(slot) * a.det_npts[d] + (t - a.det_off[d])] = point;
     } else if (t < a.totpts + a.ndft) {
-        int s = t - a.totpts;
-        if (step % a.dft_interval == 0) {
-            int k = (step / a.dft_interval) % kDftBlock;
-            a.stage[static_cast<long long>(k) * a.ndft + s] = a.F[a.dft_comp[s]][a.dft_cell[s]];
+        const int s = t - a.totpts;
+        // constant group indices keep the group table in kernel arguments
+#pragma unroll
+        for (int g = 0; g < kMaxDftGroups; ++g) {
+            if (g >= a.ndg) break;
+            if (s < a.gofs[g] || s >= a.gofs[g + 1]) continue;
+            const DftSample<T>& d = a.dg[g];
+            const int j = s - a.gofs[g];
+            if (step % d.interval == 0) {
+                const int k = (step / d.interval) % kDftBlock;
+                d.stage[static_cast<long long>(k) * d.n + j] = a.F[d.comp[j]][d.cell[j]];
+            }
         }
     }
 }
@@ -1862,6 +1880,7 @@ __global__ void __launch_bounds__(256) k_dft_flush(const T* stage, const double*
 
 // ------------------------------------------------------------ emitters ------
 struct QeDev {
+    int packed;             // rho and its history hold the upper triangle only (Hermitian)
     int N, nmem, nemit;
     const int* xyz;         // [nemit][3]
     const double2* h0;      // [nmem][N*N]
@@ -1882,6 +1901,7 @@ struct QeDev {
     const int* pop_flat;
     int pop_interval;
     double2* pop_partial;   // [npop][nblocks]
+    int* pop_ticket;        // blocks of the density launch done (the last one finishes the populations)
     double2* pop_out;       // [npop][cap]
     int pop_cap;
     int dim;
@@ -1924,64 +1944,84 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 
-// Column j of d rho/dt for the matrices H and X held in LDS (column-major):
-// (i/hbar)(H X - X H)[:, j] + the Lindblad rows that fall in column j.
-// grc / gv / gcp: the Lindblad entries (packed row | col << 16, value) and
-// column pointers, staged in LDS by the caller when they fit.
-template <int N>
-__device__ __forceinline__ void den_deriv_col(const QeDev& q, const double2* H, const double2* X, int j, const int* grc, const double* gv,
-                                              const int* gcp, double2 (&out)[N]) {
+// Rows i0 .. i0 + R - 1 of column j of d rho/dt for the matrices H and X
+// held in LDS (column-major): (i/hbar)(H X - X H)[i, j] + the Lindblad rows
+// among them.  grc / gv / gcp: the Lindblad entries (packed row | col << 16,
+// value) and column pointers, staged in LDS by the caller when they fit.
+// PK: only the stored rows i <= j are needed.
+template <int N, int R, bool PK>
+__device__ __forceinline__ void den_deriv_rows(const QeDev& q, const double2* H, const double2* X, int j, int i0, const int* grc,
+                                               const double* gv, const int* gcp, double2 (&out)[R]) {
     const double ih = 1.0 / q.hbar;
     // outer-product order over k (not unrolled): one column of H and of X
     // in flight at a time keeps the register footprint small
-    double2 s1[N], s2[N];
+    double2 s1[R], s2[R];
 #pragma unroll
-    for (int i = 0; i < N; ++i) s1[i] = s2[i] = make_double2(0.0, 0.0);
+    for (int r = 0; r < R; ++r) s1[r] = s2[r] = make_double2(0.0, 0.0);
 #pragma unroll 1
     for (int k = 0; k < N; ++k) {
         const double2 xk = X[k + j * N], hk = H[k + j * N];
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            s1[i] = cadd(s1[i], cmul(H[i + k * N], xk));
-            s2[i] = cadd(s2[i], cmul(X[i + k * N], hk));
+        for (int r = 0; r < R; ++r) {
+            const int i = i0 + r;
+            if (i >= N || (PK && i > j)) continue;
+            s1[r] = cadd(s1[r], cmul(H[i + k * N], xk));
+            s2[r] = cadd(s2[r], cmul(X[i + k * N], hk));
         }
     }
 #pragma unroll
-    for (int i = 0; i < N; ++i) out[i] = make_double2(-(s1[i].y - s2[i].y) * ih, (s1[i].x - s2[i].x) * ih);
+    for (int r = 0; r < R; ++r) out[r] = make_double2(-(s1[r].y - s2[r].y) * ih, (s1[r].x - s2[r].x) * ih);
     for (int g = gcp[j]; g < gcp[j + 1]; ++g) {
         const int rc = grc[g];
-        const int row = rc & 0xffff;
+        const int rr = (rc & 0xffff) - j * N - i0;
+        if (rr < 0 || rr >= R) continue;
         const double2 v = X[rc >> 16];
         const double w = gv[g];
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-            if (row - j * N == i) {
-                out[i].x += v.x * w;
-                out[i].y += v.y * w;
+        for (int r = 0; r < R; ++r)
+            if (rr == r && !(PK && i0 + r > j)) {
+                out[r].x += v.x * w;
+                out[r].y += v.y * w;
             }
     }
 }
 
-// Every emitter group (N >= 2): a team of N lanes per emitter, lane j owning
-// column j of rho, its history and its derivative, advanced by the
+// Every emitter group (N >= 2): a team of lanes per emitter, each lane owning
+// R consecutive rows of one column j of rho, its history and its derivative
+// (R = N for N <= 3, else about N / 2: two lanes per column), advanced by the
 // Adams-Bashforth-Moulton predictor-corrector (PCABAM4, parallelQE.hpp:751-770)
 // cold-started from zero history; the derivative history is a 4-slot ring
 // instead of the reference's three copies per step (density.hpp:65-70).
-// One thread per emitter would hold N^2 complex words per lane and spill
-// from N = 3 on; for N = 2 it leaves too few waves to hide latency.  Lanes are
-// grouped by column, lane = j * EB + team, so the lanes of one column hold
-// consecutive emitters and every rho / history access of a wave is a few
-// contiguous runs.  H = h0 - E.mu (formed once per member, each lane its
-// column) and the matrix being differentiated are shared through LDS with a
-// per-emitter stride of N^2 + 1 elements, which spreads the emitters of a
-// wave over the LDS banks.
+// Splitting a column over two lanes halves the per-lane registers (the
+// 5 R complex words of the ABM4 update are loaded at once), which doubles the
+// waves per SIMD for N >= 4; for N <= 3 a whole column per lane already
+// leaves at least 3.  Lanes are grouped by (column, row chunk), lane =
+// c * EB + team, so the lanes of one chunk hold consecutive emitters and every
+// rho / history access of a wave is a few contiguous runs.  H = h0 - E.mu
+// (formed once per member, each lane its rows) and the matrix being
+// differentiated are shared through LDS with a per-emitter stride of N^2 + 1
+// elements, which spreads the emitters of a wave over the LDS banks.
+//
+// PK (Hermitian packing): rho and the derivative history keep only the upper
+// triangle, entry (i, j) with i <= j stored at j (j + 1) / 2 + i, so a lane
+// loads and stores its rows i <= j; the lower entries are the conjugates of
+// the other lanes' entries, written to the shared matrices by those lanes.
+// rho stays exactly Hermitian, which the full form keeps up to rounding (the
+// Lindblad sums of mirrored entries may add in another order).  Used when H0,
+// the dipoles and the Lindblad operator preserve hermiticity; the derivative
+// of a Hermitian rho is then Hermitian.
 constexpr int kMlThreads = 256;
-constexpr int kMlGamLds = 2048;  // Lindblad entries staged in LDS (N^2 x N^2 at most for N <= 6)
+constexpr int kMlGamLds = 256;  // Lindblad entries staged in LDS (more are read from global memory)
 
-template <typename T, int N>
+__host__ __device__ constexpr int ml_store_words(int n, bool pk) { return pk ? n * (n + 1) / 2 : n * n; }
+__host__ __device__ constexpr int ml_rows(int n) { return n <= 3 ? n : 2; }
+__host__ __device__ constexpr int ml_lanes(int n) { return n * ((n + ml_rows(n) - 1) / ml_rows(n)); }  // lanes per emitter
+
+template <typename T, int N, bool PK>
 __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, const T* E1, const T* E2, int nx, int ny, int nz,
                                                         const int* stepp, int stepv) {
-    constexpr int S = N * N, EB = kMlThreads / N, XS = S + 1;
+    constexpr int S = N * N, R = ml_rows(N), NL = (N + R - 1) / R, LN = N * NL, EB = kMlThreads / LN, XS = S + 1,
+                  SP = ml_store_words(N, PK);
     constexpr int XBYTES = EB * XS * 16, TAIL = kMlThreads * (3 * 8 + 16);
     // Xs during the member loop; the polarisation and population reductions afterwards
     __shared__ __align__(16) unsigned char lds[XBYTES > TAIL ? XBYTES : TAIL];
@@ -1991,9 +2031,10 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
     __shared__ double gv_s[kMlGamLds];
     __shared__ int gcp_s[N + 1];
     double2* Xs = reinterpret_cast<double2*>(lds);
-    const int j = threadIdx.x / EB, team = threadIdx.x - j * EB;
+    const int cl = threadIdx.x / EB, team = threadIdx.x - cl * EB;  // chunk lane (column j, rows i0..) and emitter
+    const int j = cl / NL, i0 = (cl - j * NL) * R;
     const int e = blockIdx.x * EB + team;
-    const bool act = j < N && e < q.nemit;
+    const bool act = cl < LN && e < q.nemit;
     const int step = step_now(stepp, stepv);
     for (int k = threadIdx.x; k < 3 * S; k += blockDim.x) mus[k / S][k % S] = q.mu[k];
     const bool gam_lds = q.ngam <= kMlGamLds;
@@ -2017,13 +2058,13 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             if (!Es[c]) continue;
-            long long i0 = px + static_cast<long long>(nx) * (py + static_cast<long long>(ny) * pz);
-            if (q.dim == 2) ef[c] = static_cast<double>(Es[c][i0]);
+            long long ip = px + static_cast<long long>(nx) * (py + static_cast<long long>(ny) * pz);
+            if (q.dim == 2) ef[c] = static_cast<double>(Es[c][ip]);
             else {
                 // mean of the two Yee edges E_c(p) and E_c(p - e_c) (getE_TE)
                 const int qx = px - (c == 0), qy = py - (c == 1), qz = pz - (c == 2);
                 double v1 = (qx >= 0 && qy >= 0 && qz >= 0) ? static_cast<double>(Es[c][qx + static_cast<long long>(nx) * (qy + static_cast<long long>(ny) * qz)]) : 0.0;
-                ef[c] = 0.5 * static_cast<double>(Es[c][i0]) + 0.5 * v1;
+                ef[c] = 0.5 * static_cast<double>(Es[c][ip]) + 0.5 * v1;
             }
         }
     }
@@ -2033,13 +2074,18 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
     const double dt = q.dt;
     double2* Xm = Xs + (act ? team : 0) * XS;
     double2* Hm = Hs + (act ? team : 0) * XS;
+    // row i of this lane is kept in memory (upper triangle only when packed)
+    auto kept = [&](int i) { return i < N && !(PK && i > j); };
+    auto stored = [&](int i) -> long long { return PK ? j * (j + 1) / 2 + i : i + j * N; };
     __syncthreads();  // mu and the Lindblad entries are staged
     for (int m = 0; m < q.nmem; ++m) {
         const long long col = static_cast<long long>(m) * q.nemit + e;
-        double2 rc[N], fp[N];
+        double2 rc[R], fp[R];
         if (act) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
+            for (int r = 0; r < R; ++r) {
+                const int i = i0 + r;
+                if (i >= N) continue;
                 const int k = i + j * N;
                 double2 h = q.h0[m * S + k];
 #pragma unroll
@@ -2048,76 +2094,95 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
                     h.y -= ef[c] * mus[c][k].y;
                 }
                 Hm[k] = h;
-                const double2 rk = q.rho[k * ld + col];
-                const double2 f0 = q.hist[s0 * S * ld + k * ld + col];
-                const double2 f1 = q.hist[s1 * S * ld + k * ld + col];
-                const double2 f2 = q.hist[s2 * S * ld + k * ld + col];
-                const double2 f3 = q.hist[s3 * S * ld + k * ld + col];
+                if (!kept(i)) continue;
+                const long long ks = stored(i);
+                const double2 rk = q.rho[ks * ld + col];
+                const double2 f0 = q.hist[s0 * SP * ld + ks * ld + col];
+                const double2 f1 = q.hist[s1 * SP * ld + ks * ld + col];
+                const double2 f2 = q.hist[s2 * SP * ld + ks * ld + col];
+                const double2 f3 = q.hist[s3 * SP * ld + ks * ld + col];
                 double2 pr = rk;
                 pr = cadd(pr, cscale(f0, 55.0 * dt / 24.0));
                 pr = cadd(pr, cscale(f1, -59.0 * dt / 24.0));
                 pr = cadd(pr, cscale(f2, 37.0 * dt / 24.0));
                 pr = cadd(pr, cscale(f3, -9.0 * dt / 24.0));
                 Xm[k] = pr;
+                if (PK && i < j) Xm[j + i * N] = make_double2(pr.x, -pr.y);
                 double2 cr = rk;
                 cr = cadd(cr, cscale(f0, 19.0 * dt / 24.0));
                 cr = cadd(cr, cscale(f1, -5.0 * dt / 24.0));
                 cr = cadd(cr, cscale(f2, dt / 24.0));
-                rc[i] = cr;
+                rc[r] = cr;
             }
         }
         __syncthreads();
-        if (act) den_deriv_col<N>(q, Hm, Xm, j, grc, gv, gcp_s, fp);
+        if (act) den_deriv_rows<N, R, PK>(q, Hm, Xm, j, i0, grc, gv, gcp_s, fp);
         __syncthreads();
         if (act) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
+            for (int r = 0; r < R; ++r) {
+                const int i = i0 + r;
+                if (!kept(i)) continue;
                 const int k = i + j * N;
-                rc[i] = cadd(rc[i], cscale(fp[i], 9.0 * dt / 24.0));
-                q.rho[k * ld + col] = rc[i];
-                Xm[k] = rc[i];
+                rc[r] = cadd(rc[r], cscale(fp[r], 9.0 * dt / 24.0));
+                q.rho[stored(i) * ld + col] = rc[r];
+                Xm[k] = rc[r];
+                if (PK && i < j) Xm[j + i * N] = make_double2(rc[r].x, -rc[r].y);
             }
         }
         __syncthreads();
         if (act) {
-            den_deriv_col<N>(q, Hm, Xm, j, grc, gv, gcp_s, fp);
+            den_deriv_rows<N, R, PK>(q, Hm, Xm, j, i0, grc, gv, gcp_s, fp);
 #pragma unroll
-            for (int i = 0; i < N; ++i) q.hist[s3 * S * ld + (i + j * N) * ld + col] = fp[i];
+            for (int r = 0; r < R; ++r)
+                if (kept(i0 + r)) q.hist[s3 * SP * ld + stored(i0 + r) * ld + col] = fp[r];
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) {
                 if (jj >= q.npop) break;
                 const int k = q.pop_flat[jj];
+                const int ri = k % N, cj = k / N;
 #pragma unroll
-                for (int i = 0; i < N; ++i)
-                    if (k == i + j * N) pop[jj] = cadd(pop[jj], rc[i]);
+                for (int r = 0; r < R; ++r) {
+                    const int i = i0 + r;
+                    if (!kept(i)) continue;
+                    // entry (ri, cj) of the full matrix: stored (ri, cj), or with
+                    // packing below the diagonal the conjugate of (cj, ri)
+                    if ((!PK || ri <= cj) && i == ri && j == cj) pop[jj] = cadd(pop[jj], rc[r]);
+                    if (PK && ri > cj && i == cj && j == ri) pop[jj] = cadd(pop[jj], make_double2(rc[r].x, -rc[r].y));
+                }
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 if (!Es[c]) continue;
                 double sacc = 0.0;
 #pragma unroll
-                for (int i = 0; i < N; ++i) {
+                for (int r = 0; r < R; ++r) {
+                    const int i = i0 + r;
+                    if (!kept(i)) continue;
+                    // Re(conj(rho_ij) mu_ij); with packing the mirrored entry
+                    // (j, i) adds the same again (rho and mu Hermitian)
                     const double2 mu = mus[c][i + j * N];
-                    sacc += rc[i].x * mu.x + rc[i].y * mu.y;
+                    const double t = rc[r].x * mu.x + rc[r].y * mu.y;
+                    sacc += (PK && i < j) ? 2.0 * t : t;
                 }
                 P[c] += q.na * sacc;
             }
         }
         __syncthreads();  // Hs and Xs are rewritten by the next member
     }
-    // polarisation of the emitter: sum of its lanes' column parts
+    // polarisation of the emitter: sum of its lanes' parts
     double (*Pl)[3] = reinterpret_cast<double (*)[3]>(lds);
     double2* red = reinterpret_cast<double2*>(lds + kMlThreads * 3 * 8);
 #pragma unroll
     for (int c = 0; c < 3; ++c) Pl[threadIdx.x][c] = P[c];
     __syncthreads();
-    if (act && j == 0) {
+    if (act && cl == 0) {
         const int bx = px - q.bmin[0] + 1, by = py - q.bmin[1] + 1, bz = (q.dim == 3) ? pz - q.bmin[2] + 1 : 0;
         const int PX = q.bn[0] + 2, PY = q.bn[1] + 2;
         for (int c = 0; c < 3; ++c) {
             if (!Es[c]) continue;
             double sum = 0.0;
-            for (int l = 0; l < N; ++l) sum += Pl[team + l * EB][c];
+            for (int l = 0; l < LN; ++l) sum += Pl[team + l * EB][c];
             q.Pbox[c][bx + PX * (by + PY * bz)] = sum;
         }
     }
@@ -2133,11 +2198,36 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
         if (threadIdx.x == 0) q.pop_partial[jj * gridDim.x + blockIdx.x] = red[0];
         __syncthreads();
     }
+    // The last block to finish sums the blocks' partials (each thread a fixed
+    // strided subset, then a tree in LDS: the same order every run, whichever
+    // block is last) and stores the populations of this step (QEPopDtc, a
+    // sample every pop_interval steps, averaged over the emitters).
+    if (q.npop == 0) return;
+    __shared__ int is_last;
+    __threadfence();  // this block's partials are visible device-wide before its ticket
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(q.pop_ticket, 1) == static_cast<int>(gridDim.x) - 1;
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    const int nb = static_cast<int>(gridDim.x);
+    const bool keep = step % q.pop_interval == 0 && step / q.pop_interval < q.pop_cap;
+    for (int jj = 0; jj < q.npop; ++jj) {
+        double2 sum = make_double2(0.0, 0.0);
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) sum = cadd(sum, q.pop_partial[jj * nb + b]);
+        red[threadIdx.x] = sum;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && keep)
+            q.pop_out[static_cast<long long>(jj) * q.pop_cap + step / q.pop_interval] = make_double2(red[0].x / q.nemit, red[0].y / q.nemit);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *q.pop_ticket = 0;  // for the next launch (stream-ordered)
 }
 
-// One block per population detector: the density kernel's per-block partial
-// sums, each thread a fixed strided subset, then a tree in LDS (a fixed
-// order, so every run sums the same way).
 // ------------------------------------------------------- batched launches ----
 // One launch per kernel class covers many simulations (the serving loop's
 // batches): slot k of a device-side descriptor table holds simulation k's
@@ -2265,25 +2355,6 @@ __global__ void __launch_bounds__(256) k_bsample(const BatchSlot<T>* __restrict_
     sample_at<T>(q.s, bx * blockDim.x + threadIdx.x);
 }
 
-constexpr int kPopThreads = 1024;
-
-__global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, int nblocks, const int* stepp, int stepv) {
-    __shared__ double2 red[kPopThreads];
-    const int j = blockIdx.x;
-    const int step = step_now(stepp, stepv);
-    if (step % q.pop_interval != 0) return;
-    const int slot = step / q.pop_interval;
-    if (slot >= q.pop_cap) return;
-    double2 s = make_double2(0.0, 0.0);
-    for (int b = threadIdx.x; b < nblocks; b += kPopThreads) s = cadd(s, q.pop_partial[j * nblocks + b]);
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = kPopThreads / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) q.pop_out[static_cast<long long>(j) * q.pop_cap + slot] = make_double2(red[0].x / q.nemit, red[0].y / q.nemit);
-}
 
 __global__ void k_zero(double* p, long long n) {
     long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
@@ -2335,14 +2406,18 @@ struct chiml_dev {
     long long ade_groups = 0;  // largest pole-box group count over the E components (split path)
     Box3 ibox{};
     SlabMap slab{};
-    int ndet = 0, totpts = 0, ndft = 0, nfreq = 0, dft_interval = 1;
+    int ndet = 0, totpts = 0, ndft = 0;
     std::vector<int> det_npts, det_cap, det_interval;
     std::vector<long long> det_out_off;
     double* det_out = nullptr;
-    double* dft_F = nullptr;
-    double* dft_tmp = nullptr;
-    void* dft_stage = nullptr;
-    const double* dft_phasor = nullptr;
+    struct DftGroup {  // accumulators F[f][cell] of one DFT group, device memory
+        int n = 0, nfreq = 0, interval = 1;
+        double* F = nullptr;
+        double* tmp = nullptr;
+        void* stage = nullptr;
+        const double* phasor = nullptr;
+    };
+    std::vector<DftGroup> dgs;
     std::vector<QeHost> qes;
     size_t footprint = 0;
     bool profile = false;
@@ -2370,6 +2445,37 @@ T* upload_cast(chiml_dev* c, const double* src, size_t n) {
     std::vector<T> tmp(n);
     for (size_t i = 0; i < n; ++i) tmp[i] = static_cast<T>(src[i]);
     return c->upload<T>(tmp.data(), n);
+}
+
+// Does the emitter group keep rho Hermitian?  True when every member's H0 and
+// the three dipole matrices are Hermitian and the Lindblad operator maps
+// Hermitian to Hermitian: G[(a,b),(c,d)] = G[(b,a),(d,c)] for its real
+// entries (column-major flat indices).  The density kernel then stores the
+// upper triangle only.
+bool hermitian_preserving(const chiml_qe_t& qi) {
+    const int N = qi.nlevel, S = N * N;
+    auto herm = [&](const double* m) {
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                const double* a = m + 2 * (i + j * N);
+                const double* b = m + 2 * (j + i * N);
+                if (a[0] != b[0] || a[1] != -b[1]) return false;
+            }
+        return true;
+    };
+    for (int mm = 0; mm < qi.nmembers; ++mm)
+        if (!herm(qi.h0 + 2 * static_cast<size_t>(mm) * S)) return false;
+    for (int c = 0; c < 3; ++c)
+        if (!herm(qi.mu + 2 * static_cast<size_t>(c) * S)) return false;
+    std::map<std::pair<int, int>, double> g;
+    for (int k = 0; k < qi.ngam; ++k) g[{qi.gam_row[k], qi.gam_col[k]}] += qi.gam_val[k];
+    auto mirror = [&](int f) { return f / N + (f % N) * N; };
+    for (const auto& kv : g) {
+        auto it = g.find({mirror(kv.first.first), mirror(kv.first.second)});
+        const double v = it == g.end() ? 0.0 : it->second;
+        if (v != kv.second) return false;
+    }
+    return true;
 }
 
 template <typename T>
@@ -2540,21 +2646,45 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         s.det_out_off = c->upload<long long>(ooff.data(), p->ndet + 1);
         s.det_cap = c->upload<int>(c->det_cap.data(), p->ndet);
     }
-    c->ndft = p->ndft;
-    c->nfreq = p->nfreq;
-    c->dft_interval = p->dft_interval > 0 ? p->dft_interval : 1;
-    s.ndft = p->ndft;
-    s.dft_interval = c->dft_interval;
-    if (p->ndft > 0) {
-        s.dft_comp = c->upload<int>(p->dft_comp, p->ndft);
-        s.dft_cell = c->upload<long long>(p->dft_cell, p->ndft);
-        T* st = c->zeros<T>(static_cast<size_t>(kDftBlock) * p->ndft);
-        s.stage = st;
-        c->dft_stage = st;
-        c->dft_F = c->zeros<double>(2 * static_cast<size_t>(p->ndft) * p->nfreq);
-        c->dft_tmp = c->zeros<double>(2 * static_cast<size_t>(p->ndft) * p->nfreq);
-        c->dft_phasor = c->upload<double>(p->dft_phasor, 2 * static_cast<size_t>(p->nsteps / c->dft_interval + 1) * p->nfreq);
+    // DFT groups: 0 = the flux boxes (ndft / dft_*), then p->dftg[0..ndftg)
+    if (p->ndftg < 0 || 1 + p->ndftg > kMaxDftGroups) throw std::runtime_error("too many DFT groups (at most 7 frequency detectors)");
+    c->dgs.clear();
+    c->ndft = 0;
+    s.ndg = 1 + p->ndftg;
+    s.gofs[0] = 0;
+    for (int g = 0; g < s.ndg; ++g) {
+        const int n = g == 0 ? p->ndft : p->dftg[g - 1].ncell;
+        const int nf = g == 0 ? p->nfreq : p->dftg[g - 1].nfreq;
+        const int iv = g == 0 ? p->dft_interval : p->dftg[g - 1].interval;
+        const int* comp = g == 0 ? p->dft_comp : p->dftg[g - 1].comp;
+        const long long* cell = g == 0 ? p->dft_cell : p->dftg[g - 1].cell;
+        const double* ph = g == 0 ? p->dft_phasor : p->dftg[g - 1].phasor;
+        if (n < 0 || (n > 0 && nf < 1)) throw std::runtime_error("invalid DFT group");
+        chiml_dev::DftGroup d;
+        d.n = n;
+        d.nfreq = nf;
+        d.interval = iv > 0 ? iv : 1;
+        DftSample<T>& ds = s.dg[g];
+        ds.n = n;
+        ds.interval = d.interval;
+        ds.comp = nullptr;
+        ds.cell = nullptr;
+        ds.stage = nullptr;
+        if (n > 0) {
+            ds.comp = c->upload<int>(comp, n);
+            ds.cell = c->upload<long long>(cell, n);
+            T* st = c->zeros<T>(static_cast<size_t>(kDftBlock) * n);
+            ds.stage = st;
+            d.stage = st;
+            d.F = c->zeros<double>(2 * static_cast<size_t>(n) * nf);
+            d.tmp = c->zeros<double>(2 * static_cast<size_t>(n) * nf);
+            d.phasor = c->upload<double>(ph, 2 * static_cast<size_t>(p->nsteps / d.interval + 1) * nf);
+        }
+        c->dgs.push_back(d);
+        c->ndft += n;
+        s.gofs[g + 1] = c->ndft;
     }
+    s.ndft = c->ndft;
     // emitters
     const double* epsn = nullptr;
     if (p->nqe > 0) epsn = c->upload<double>(p->eps_node, N);
@@ -2616,19 +2746,22 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
             q.Aprev[k] = c->zeros<double>(ab);
         }
         const size_t ld = static_cast<size_t>(q.nmem) * q.nemit;
-        std::vector<double> rho0(2 * S * ld, 0.0);
+        q.packed = hermitian_preserving(qi) && std::getenv("CHIML_ML_UNPACKED") == nullptr ? 1 : 0;
+        const size_t SW = static_cast<size_t>(ml_store_words(q.N, q.packed != 0));
+        std::vector<double> rho0(2 * SW * ld, 0.0);  // rho_00 = the member's weight (entry 0 in both layouts)
         for (int m = 0; m < q.nmem; ++m)
             for (int e = 0; e < q.nemit; ++e) rho0[2 * (0 * ld + static_cast<size_t>(m) * q.nemit + e)] = qi.weight[m];
         q.rho = reinterpret_cast<double2*>(c->upload<double>(rho0.data(), rho0.size()));
-        q.hist = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * S * ld));
+        q.hist = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * SW * ld));
         q.npop = std::min(qi.npop, 4);
         if (qi.npop > 4) throw std::runtime_error("at most 4 population detectors per emitter group");
         q.pop_flat = c->upload<int>(qi.pop_flat, std::max(qi.npop, 1));
         q.pop_interval = qi.pop_interval > 0 ? qi.pop_interval : 1;
         // emitters per block of kMlThreads lanes
-        const int per_block = kMlThreads / q.N;  // one team of N lanes per emitter
+        const int per_block = kMlThreads / ml_lanes(q.N);  // one team of lanes per emitter
         qh.nblocks = (q.nemit + per_block - 1) / per_block;
         q.pop_partial = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(qh.nblocks)));
+        q.pop_ticket = c->zeros<int>(1);
         q.pop_cap = p->nsteps / q.pop_interval + 1;
         q.pop_out = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(q.pop_cap)));
         q.eps_node = epsn;
@@ -2681,13 +2814,15 @@ struct Timed {
 // read the device counter, inside captured graphs)
 template <typename T>
 void launch_flush(chiml_dev* c, int force, int step, const int* stepp) {
-    if (c->ndft == 0) return;
     // enq_step = the device step counter once the step being enqueued has advanced it
     const int st = c->enq_step;
-    if (!force && !c->device_flush && (st % c->dft_interval != 0 || ((st / c->dft_interval) % kDftBlock) != kDftBlock - 1)) return;
-    Timed t(c, "dft_flush");
-    hipLaunchKernelGGL(k_dft_flush<T>, dim3((c->ndft + 255) / 256), dim3(256), 0, c->stream, static_cast<const T*>(c->dft_stage),
-                       c->dft_phasor, c->dft_F, stepp, c->ndft, c->nfreq, c->dft_interval, force, step);
+    for (const chiml_dev::DftGroup& d : c->dgs) {
+        if (d.n == 0) continue;
+        if (!force && !c->device_flush && (st % d.interval != 0 || ((st / d.interval) % kDftBlock) != kDftBlock - 1)) continue;
+        Timed t(c, "dft_flush");
+        hipLaunchKernelGGL(k_dft_flush<T>, dim3((d.n + 255) / 256), dim3(256), 0, c->stream, static_cast<const T*>(d.stage), d.phasor,
+                           d.F, stepp, d.n, d.nfreq, d.interval, force, step);
+    }
 }
 
 
@@ -2845,8 +2980,15 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
         {
             Timed t(c, "ml_density");
             switch (qh.N) {
-#define CHIML_ML_TEAM(NN) \
-    case NN: hipLaunchKernelGGL((k_ml_team<T, NN>), g, dim3(kMlThreads), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, c->nz, a.step, a.stepv); break;
+#define CHIML_ML_TEAM(NN)                                                                                                       \
+    case NN:                                                                                                                    \
+        if (q.packed)                                                                                                           \
+            hipLaunchKernelGGL((k_ml_team<T, NN, true>), g, dim3(kMlThreads), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, \
+                               c->nz, a.step, a.stepv);                                                                         \
+        else                                                                                                                    \
+            hipLaunchKernelGGL((k_ml_team<T, NN, false>), g, dim3(kMlThreads), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny, \
+                               c->nz, a.step, a.stepv);                                                                         \
+        break;
                 CHIML_ML_TEAM(2)
                 CHIML_ML_TEAM(3)
                 CHIML_ML_TEAM(4)
@@ -2857,7 +2999,6 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
 #undef CHIML_ML_TEAM
             }
         }
-        if (q.npop > 0) hipLaunchKernelGGL(k_ml_pop_finish, dim3(q.npop), dim3(kPopThreads), 0, c->stream, q, qh.nblocks, a.step, a.stepv);
     }
 }
 
@@ -2897,8 +3038,12 @@ void do_steps(chiml_dev* c, StepArgs<T>& a, SampleArgs<T>& s, int n) {
         a.step = ctr + p;
         a.step_next = host ? nullptr : ctr + (1 - p);
         s.step = ctr + (1 - p);
-        if (c->dim == 3) launch_fields<T, 3>(c, a);
-        else launch_fields<T, 2>(c, a);
+        {
+            // one event pair around all field launches of the step ("field_step")
+            Timed t(c, "field_step");
+            if (c->dim == 3) launch_fields<T, 3>(c, a);
+            else launch_fields<T, 2>(c, a);
+        }
         if (!c->qes.empty()) launch_ml<T>(c, a);
         c->enq_step = st + 1;
         if (!c->fuse_sample) launch_sample<T>(c, s);
@@ -3092,11 +3237,13 @@ void batch_group(chiml_dev** g, int n, int nsteps, hipStream_t bs, unsigned char
         // DFT flushes decide on the device whether their block of samples is complete
         for (int i = 0; i < n; ++i) {
             chiml_dev* c = g[i];
-            if (c->ndft == 0) continue;
             const int p = (c->steps_done + j) & 1;
             int* ctr = c->step_ctr.as<int>();
-            hipLaunchKernelGGL(k_dft_flush<T>, dim3((c->ndft + 255) / 256), blk, 0, bs, static_cast<const T*>(c->dft_stage), c->dft_phasor,
-                               c->dft_F, ctr + (1 - p), c->ndft, c->nfreq, c->dft_interval, 0, -1);
+            for (const chiml_dev::DftGroup& d : c->dgs) {
+                if (d.n == 0) continue;
+                hipLaunchKernelGGL(k_dft_flush<T>, dim3((d.n + 255) / 256), blk, 0, bs, static_cast<const T*>(d.stage), d.phasor, d.F,
+                                   ctr + (1 - p), d.n, d.nfreq, d.interval, 0, -1);
+            }
         }
         HIP_TRY(hipGetLastError());
     }
@@ -3349,6 +3496,7 @@ int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out) {
                 m.nA = 2LL * Tz * gx * p->ny;
                 m.nB = 2LL * Ty * gx * (p->nz - 2 * Tz);
                 m.nC = rows ? 0 : 2LL * (Tx / W) * (p->ny - 2 * Ty) * (p->nz - 2 * Tz);
+
                 for (int k = 0; k < 3; ++k) {
                     const int* bx = c->prec == 32 ? c->a32.box[k] : c->a64.box[k];
                     const bool has = c->prec == 32 ? c->a32.jsum[k] != nullptr : c->a64.jsum[k] != nullptr;
@@ -3503,46 +3651,48 @@ int chiml_dev_read_points(chiml_dev_t* c, int d, double* values, size_t cap, siz
     return CHIML_OK;
 }
 
-int chiml_dev_read_dft(chiml_dev_t* c, double* reim, size_t cap) {
+int chiml_dev_read_dft_group(chiml_dev_t* c, int g, double* reim, size_t cap) {
     if (!c) return fail(nullptr, CHIML_E_INVALID, "null context");
-    size_t n = 2 * static_cast<size_t>(c->ndft) * c->nfreq;
+    if (g < 0 || g >= static_cast<int>(c->dgs.size())) return fail(c, CHIML_E_INVALID, "DFT group index out of range");
+    const chiml_dev::DftGroup& d = c->dgs[g];
+    size_t n = 2 * static_cast<size_t>(d.n) * d.nfreq;
     if (n == 0) return CHIML_OK;
     if (!reim || cap < n) return fail(c, CHIML_E_INVALID, "buffer too small");
     std::lock_guard<std::mutex> lk(g_capture_mu);
     try {
-        // Contract the partially filled staging block into a copy of F.
-        int kmax = c->steps_done / c->dft_interval;
+        int kmax = c->steps_done / d.interval;
         bool partial = (kmax % kDftBlock) != kDftBlock - 1;
         std::vector<double> host(n);
         if (partial) {
             // Contract samples kfirst..kmax of the open block into a copy of F.
-            double* tmp = c->dft_tmp;
-            HIP_TRY(hipMemcpyAsync(tmp, c->dft_F, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-            int last = kmax * c->dft_interval;
+            HIP_TRY(hipMemcpyAsync(d.tmp, d.F, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            int last = kmax * d.interval;
             if (c->prec == 32)
-                hipLaunchKernelGGL(k_dft_flush<float>, dim3((c->ndft + 255) / 256), dim3(256), 0, c->stream, static_cast<const float*>(c->dft_stage),
-                                   c->dft_phasor, tmp, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, 1, last);
+                hipLaunchKernelGGL(k_dft_flush<float>, dim3((d.n + 255) / 256), dim3(256), 0, c->stream, static_cast<const float*>(d.stage),
+                                   d.phasor, d.tmp, c->step_ctr.as<int>(), d.n, d.nfreq, d.interval, 1, last);
             else
-                hipLaunchKernelGGL(k_dft_flush<double>, dim3((c->ndft + 255) / 256), dim3(256), 0, c->stream, static_cast<const double*>(c->dft_stage),
-                                   c->dft_phasor, tmp, c->step_ctr.as<int>(), c->ndft, c->nfreq, c->dft_interval, 1, last);
+                hipLaunchKernelGGL(k_dft_flush<double>, dim3((d.n + 255) / 256), dim3(256), 0, c->stream, static_cast<const double*>(d.stage),
+                                   d.phasor, d.tmp, c->step_ctr.as<int>(), d.n, d.nfreq, d.interval, 1, last);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(c->stream));
-            HIP_TRY(hipMemcpy(host.data(), tmp, n * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(host.data(), d.tmp, n * sizeof(double), hipMemcpyDeviceToHost));
         } else {
             HIP_TRY(hipStreamSynchronize(c->stream));
-            HIP_TRY(hipMemcpy(host.data(), c->dft_F, n * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(host.data(), d.F, n * sizeof(double), hipMemcpyDeviceToHost));
         }
         // device layout [f][s] -> ABI layout [s][f]
-        for (int f = 0; f < c->nfreq; ++f)
-            for (int sm = 0; sm < c->ndft; ++sm) {
-                reim[2 * (static_cast<size_t>(sm) * c->nfreq + f)] = host[2 * (static_cast<size_t>(f) * c->ndft + sm)];
-                reim[2 * (static_cast<size_t>(sm) * c->nfreq + f) + 1] = host[2 * (static_cast<size_t>(f) * c->ndft + sm) + 1];
+        for (int f = 0; f < d.nfreq; ++f)
+            for (int sm = 0; sm < d.n; ++sm) {
+                reim[2 * (static_cast<size_t>(sm) * d.nfreq + f)] = host[2 * (static_cast<size_t>(f) * d.n + sm)];
+                reim[2 * (static_cast<size_t>(sm) * d.nfreq + f) + 1] = host[2 * (static_cast<size_t>(f) * d.n + sm) + 1];
             }
     } catch (std::exception& e) {
         return fail(c, CHIML_E_HIP, e.what());
     }
     return CHIML_OK;
 }
+
+int chiml_dev_read_dft(chiml_dev_t* c, double* reim, size_t cap) { return chiml_dev_read_dft_group(c, 0, reim, cap); }
 
 int chiml_dev_read_pops(chiml_dev_t* c, int q, int j, double* reim, size_t cap, size_t* nsamples) {
     if (!c || q < 0 || q >= static_cast<int>(c->qes.size()) || !nsamples) return fail(c, CHIML_E_INVALID, "invalid argument");
@@ -3573,8 +3723,13 @@ int chiml_dev_read_density(chiml_dev_t* c, int q, int m, int e, double* reim) {
     std::lock_guard<std::mutex> lk(g_capture_mu);
     try {
         HIP_TRY(hipStreamSynchronize(c->stream));
-        for (int k = 0; k < S; ++k)
-            HIP_TRY(hipMemcpy(reim + 2 * k, d.rho + k * ld + static_cast<size_t>(m) * d.nemit + e, sizeof(double2), hipMemcpyDeviceToHost));
+        for (int k = 0; k < S; ++k) {
+            const int i = k % d.N, j = k / d.N;
+            // packed: entry (i, j) stored for i <= j; below the diagonal the conjugate of (j, i)
+            const int ks = !d.packed ? k : (i <= j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j);
+            HIP_TRY(hipMemcpy(reim + 2 * k, d.rho + ks * ld + static_cast<size_t>(m) * d.nemit + e, sizeof(double2), hipMemcpyDeviceToHost));
+            if (d.packed && i > j) reim[2 * k + 1] = -reim[2 * k + 1];
+        }
     } catch (std::exception& ex) {
         return fail(c, CHIML_E_HIP, ex.what());
     }

@@ -13,6 +13,7 @@
 // kernels read directly.
 #include <algorithm>
 #include <cmath>
+#include <fstream>
 #include <complex>
 #include <cstring>
 #include <filesystem>
@@ -153,6 +154,29 @@ struct FluxBox {
     // the deck's weight, or with SI the unit conversions
     double conv = 1.0, freqConv = 1.0;
     bool frontBack = false;  // the box extends in all three directions
+    std::array<int, 3> sz{};  // sz_ of the box
+    bool save = false;        // write <name>_fields.dat with the output files
+    // loaded incident fields times -1, the starting values of each face's
+    // combined Ej, Ek, Hj, Hk [freq][transCor1][cor] (loadFields(-1.0), parallelFlux.hpp:664-722)
+    std::vector<std::array<std::vector<cplx>, 4>> loaded;
+};
+
+// Frequency detector (parallelDetectorFREQ_Base, src/DTC/parallelDTC_FREQ.hpp:94-141,
+// built by coustructDTC, parallelFDTDField.cpp:1727-1728), sampling through
+// parallelStorageFreqDTC with DIRECTION::NONE (parallelStorageFreqDTC.hpp:200-320).
+struct FreqDet {
+    int deck = 0;  // index in the deck's DetectorList
+    std::string name;
+    bool pow = false, maps = false;
+    int timeInt = 1;
+    double conv = 1.0, freqConv = 1.0;  // convFactor_ and freqConv_
+    std::vector<double> freqs;           // angular, c/a units
+    std::array<int, 3> sz{};             // the detector box (sz_), x y z
+    std::array<int, 3> lsz{};            // sampled extent x y z (0 along z in 2D)
+    std::vector<int> comps;              // field component of each grid
+    int group = 0;                       // DFT group on the device
+    int npts() const { return lsz[0] * lsz[1] * lsz[2]; }
+    int szFreq() const { return sz[0] * sz[1] * sz[2]; }
 };
 
 }  // namespace
@@ -172,6 +196,8 @@ class FDTDField {
     chiml_dev_t* dev_ = nullptr;
     std::vector<FluxBox> fluxes;
     int ndft = 0, nfreq = 0, dftInterval = 1;
+    std::vector<FreqDet> freqDets;
+    std::vector<int> pointDtc;  // deck index of each point detector (dtcArr order)
     std::vector<int> detNpts;
     std::vector<double> detTConv;
     std::vector<int> detInterval;
@@ -182,6 +208,17 @@ class FDTDField {
 
     std::vector<cplx> flux(int f);
     std::vector<std::vector<cplx>> fluxFaces(int f);
+    // per face the combined Ej, Ek, Hj, Hk [freq][transCor1][cor] (empty where absent)
+    std::vector<std::array<std::vector<cplx>, 4>> fluxFields(int f);
+    void saveFluxFields(int f, const std::string& path);
+    void loadFluxFields(FluxBox& fb, const std::string& path);
+    // accumulated DFT of frequency detector d: [grid][point][freq]
+    std::vector<cplx> freqFields(int d);
+    int freqNt(int d);
+    // toFile values: [grid][freq] (conv/2 sum F / nt, or conv/2 sum |F|^2 / nt^2)
+    std::vector<cplx> freqValues(int d);
+    // toMap values: [grid][freq][point jj] over the detector box
+    std::vector<cplx> freqMapValues(int d);
 
   private:
     int objAt(std::array<double, 3> pt, bool ml) const;
@@ -609,7 +646,10 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     std::vector<int> dInt, dNp, dComp, dPow, dNf;
     std::vector<long long> dCell;
     std::vector<double> dConv;
-    for (auto& D : IP.dtcs) {
+    for (size_t di = 0; di < IP.dtcs.size(); ++di) {
+        const auto& D = IP.dtcs[di];
+        if (D.cls == DtcClass::FREQ) continue;
+        pointDtc.push_back(static_cast<int>(di));
         int iv = static_cast<int>(std::floor(D.timeInterval / dt));
         if (iv == 0) throw std::logic_error("The time step of a detector is less than the main grid or set to 0.");
         double conv = 1.0, tconv = 1.0;
@@ -728,6 +768,8 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
             fb.conv = IP.I0 / IP.a * IP.I0 / (EPS0 * IP.a * SPEED_OF_LIGHT);
         }
         fb.frontBack = F.sz[0] > 1 && F.sz[1] > 1 && F.sz[2] > 1;
+        fb.sz = F.sz;
+        fb.save = F.save;
         if (!fluxes.empty() && (fluxes[0].freqs != F.freqs || fluxes[0].timeInt != F.timeInt))
             throw std::logic_error("flux boxes of one simulation must share their frequency list and time interval");
         std::array<int, 3> fsz = F.sz;
@@ -800,6 +842,7 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
             else if (fsz[1] == 1) makeFace(1, true);
             else if (fsz[2] == 1) makeFace(2, true);
         }
+        if (F.load) loadFluxFields(fb, F.incdFile);
         fluxes.push_back(std::move(fb));
     }
     std::vector<double> phasor;
@@ -830,6 +873,88 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     P.nfreq = nfreq;
     P.dft_interval = dftInterval;
     P.dft_phasor = phasor.data();
+
+    // ---- frequency detectors -> DFT groups 1.. ------------------------------------
+    std::vector<std::vector<int>> fdComp;
+    std::vector<std::vector<long long>> fdCell;
+    std::vector<std::vector<double>> fdPhasor;
+    std::vector<chiml_dft_group_t> fdGroups;
+    for (size_t di = 0; di < IP.dtcs.size(); ++di) {
+        const auto& D = IP.dtcs[di];
+        if (D.cls != DtcClass::FREQ) continue;
+        FreqDet fd;
+        fd.deck = static_cast<int>(di);
+        fd.name = D.name;
+        fd.maps = D.outputMaps;
+        fd.freqs = D.freqs;
+        fd.timeInt = static_cast<int>(std::floor(D.timeInterval / dt + 0.5));  // coustructDTC, parallelFDTDField.cpp:1728
+        if (fd.timeInt <= 0) throw std::logic_error("The time interval of a frequency detector is less than half a time step.");
+        // convFactor_ / freqConv_, parallelDTC_FREQ.hpp:123-139
+        if (D.SI) {
+            fd.conv = IP.I0 / IP.a;
+            if (D.type == DtcType::EX || D.type == DtcType::EY || D.type == DtcType::EZ) fd.conv /= EPS0 * SPEED_OF_LIGHT;
+            fd.freqConv = SPEED_OF_LIGHT / IP.a;
+        }
+        fd.pow = D.type == DtcType::EPOW || D.type == DtcType::HPOW;
+        if (fd.pow) fd.conv = std::pow(fd.conv, 2.0);
+        fd.freqConv /= kPi * 2.0;
+        // grids, parallelFDTDField.cpp:688-826 (their offsets are not used by the frequency storage)
+        if (D.type == DtcType::EPOW) fd.comps = is3D ? std::vector<int>{0, 1, 2} : std::vector<int>{2};
+        else if (D.type == DtcType::HPOW) fd.comps = is3D ? std::vector<int>{3, 4, 5} : std::vector<int>{3, 4};
+        else {
+            const int comp = static_cast<int>(D.type);
+            if (!(is3D || comp == 2 || comp == 3 || comp == 4)) throw std::logic_error("detector on a field component that does not exist");
+            fd.comps = {comp};
+        }
+        fd.sz = D.sz;
+        // getLocalSzEl on one process (parallelStorageFreqDTC.hpp:187-193): the
+        // box clipped to the domain.  Along z of a 2D grid (one point, no
+        // ghosts) it gives 0 points, so a 2D frequency detector samples nothing.
+        const int nn[3] = {nx, ny, nz};
+        for (int a = 0; a < 3; ++a) fd.lsz[a] = (D.sz[a] + D.loc[a] > nn[a]) ? nn[a] - D.loc[a] : D.sz[a];
+        if (!is3D) fd.lsz[2] = 0;
+        std::vector<int> gc;
+        std::vector<long long> cell;
+        for (int comp : fd.comps)
+            for (int y = 0; y < fd.lsz[1]; ++y)
+                for (int z = 0; z < fd.lsz[2]; ++z)
+                    for (int x = 0; x < fd.lsz[0]; ++x) {
+                        gc.push_back(comp);
+                        cell.push_back((D.loc[0] + x) + static_cast<long long>(nx) * ((D.loc[1] + y) + static_cast<long long>(ny) * (D.loc[2] + z)));
+                    }
+        const int nf = static_cast<int>(fd.freqs.size());
+        std::vector<double> ph(2 * static_cast<size_t>(nsteps / fd.timeInt + 1) * nf, 0.0);
+        double tcur = 0.0;
+        for (int st = 0; st <= nsteps; ++st) {
+            if (st % fd.timeInt == 0) {
+                const int k = st / fd.timeInt;
+                for (int f = 0; f < nf; ++f) {
+                    const cplx w = std::exp(cplx(0.0, -1.0 * tcur * fd.freqs[f]));
+                    ph[2 * (static_cast<size_t>(k) * nf + f)] = w.real();
+                    ph[2 * (static_cast<size_t>(k) * nf + f) + 1] = w.imag();
+                }
+            }
+            tcur += dt;
+        }
+        fd.group = static_cast<int>(freqDets.size()) + 1;
+        fdComp.push_back(std::move(gc));
+        fdCell.push_back(std::move(cell));
+        fdPhasor.push_back(std::move(ph));
+        freqDets.push_back(std::move(fd));
+    }
+    for (size_t k = 0; k < freqDets.size(); ++k) {
+        chiml_dft_group_t g;
+        g.ncell = static_cast<int>(fdCell[k].size());
+        g.comp = fdComp[k].data();
+        g.cell = fdCell[k].data();
+        g.nfreq = static_cast<int>(freqDets[k].freqs.size());
+        g.interval = freqDets[k].timeInt;
+        g.phasor = fdPhasor[k].data();
+        fdGroups.push_back(g);
+    }
+    if (fdGroups.size() > 7) throw std::logic_error("at most 7 frequency detectors per simulation");
+    P.ndftg = static_cast<int>(fdGroups.size());
+    P.dftg = fdGroups.empty() ? nullptr : fdGroups.data();
 
     // ---- emitters --------------------------------------------------------------
     std::vector<chiml_qe_t> qdesc;
@@ -1071,8 +1196,11 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         };
         bytesH = kb("update_h") + kb("update_h_pml");
         bytesE = kb("update_e") + kb("update_e_pml");
+        double fdBytes = 0.0;  // frequency detectors: F and its read copy, staging, phasors
+        for (size_t k = 0; k < freqDets.size(); ++k)
+            fdBytes += static_cast<double>(fdCell[k].size()) * (freqDets[k].freqs.size() * 32.0 + 32 * s) + fdPhasor[k].size() * 8.0;
         estimate = static_cast<size_t>(cells * (6.0 * s + 3.0) + psiWords * s + bytesML / 7.0 * 5.0 +
-                                       static_cast<double>(ndft) * (nfreq * 16.0 + 32 * s));
+                                       static_cast<double>(ndft) * (nfreq * 16.0 + 32 * s) + fdBytes);
     }
 
     int rc = chiml_dev_create(&P, &dev_);
@@ -1095,6 +1223,67 @@ static cplx simpson(const cplx* f, int n, double d) {
     return r;
 }
 
+int FDTDField::freqNt(int d) {
+    const FreqDet& fd = freqDets.at(d);
+    int steps = 0;
+    chiml_dev_steps_done(dev_, &steps);
+    return steps / fd.timeInt + 1;  // t_step_: the t = 0 output and one per timeInt steps
+}
+
+std::vector<cplx> FDTDField::freqFields(int d) {
+    const FreqDet& fd = freqDets.at(d);
+    const size_t nf = fd.freqs.size(), ng = fd.comps.size(), np = static_cast<size_t>(fd.npts());
+    std::vector<cplx> out(ng * np * nf, cplx(0.0, 0.0));
+    if (np == 0) return out;
+    std::vector<double> F(2 * ng * np * nf);
+    if (chiml_dev_read_dft_group(dev_, fd.group, F.data(), F.size()) != CHIML_OK) throw std::runtime_error(chiml_dev_last_error(dev_));
+    for (size_t i = 0; i < out.size(); ++i) out[i] = cplx(F[2 * i], F[2 * i + 1]);
+    return out;
+}
+
+// toFile (parallelDTC_FREQ.hpp:398-444): per grid and frequency,
+// fieldOutFreqFunction = conv/2 * sum_p F / nt or pwrOutFreqFunction =
+// conv/2 * sum_p F conj(F) / nt^2 (parallelDTCOutputFxn.cpp:9-18), the sum
+// over the detector box's points in the storage order, the points past the
+// sampled ones (a clipped box, or any 2D box) contributing zero.
+std::vector<cplx> FDTDField::freqValues(int d) {
+    const FreqDet& fd = freqDets.at(d);
+    const auto F = freqFields(d);
+    const int nf = static_cast<int>(fd.freqs.size()), ng = static_cast<int>(fd.comps.size()), np = fd.npts(), sf = fd.szFreq();
+    const double nt = static_cast<double>(freqNt(d));
+    std::vector<cplx> out(static_cast<size_t>(ng) * nf);
+    for (int g = 0; g < ng; ++g)
+        for (int f = 0; f < nf; ++f) {
+            cplx sum(0.0, 0.0);
+            for (int p = 0; p < sf; ++p) {
+                const cplx v = p < np ? F[(static_cast<size_t>(g) * np + p) * nf + f] : cplx(0.0, 0.0);
+                sum += fd.pow ? v * std::conj(v) : v;
+            }
+            out[static_cast<size_t>(g) * nf + f] = fd.pow ? fd.conv / 2.0 * sum / std::pow(nt, 2.0) : fd.conv / 2.0 * sum / std::pow(nt, 1.0);
+        }
+    return out;
+}
+
+// toMap (parallelDTC_FREQ.hpp:506-580): per grid, frequency and box point jj
+// = x + z sz_x + y sz_x sz_z, the value of point jj of the collected storage
+// (whose layout follows the sampled extent; they agree unless the box is
+// clipped), through the same output functions with one point.
+std::vector<cplx> FDTDField::freqMapValues(int d) {
+    const FreqDet& fd = freqDets.at(d);
+    const auto F = freqFields(d);
+    const int nf = static_cast<int>(fd.freqs.size()), ng = static_cast<int>(fd.comps.size()), np = fd.npts(), sf = fd.szFreq();
+    const double nt = static_cast<double>(freqNt(d));
+    std::vector<cplx> out(static_cast<size_t>(ng) * nf * sf);
+    for (int g = 0; g < ng; ++g)
+        for (int f = 0; f < nf; ++f)
+            for (int jj = 0; jj < sf; ++jj) {
+                const cplx v = jj < np ? F[(static_cast<size_t>(g) * np + jj) * nf + f] : cplx(0.0, 0.0);
+                out[(static_cast<size_t>(g) * nf + f) * sf + jj] =
+                    fd.pow ? fd.conv / 2.0 * (v * std::conj(v)) / std::pow(nt, 2.0) : fd.conv / 2.0 * v / std::pow(nt, 1.0);
+            }
+    return out;
+}
+
 // Total over the faces in their order (flux += tempFlux, parallelFlux.hpp:522-526).
 std::vector<cplx> FDTDField::flux(int fi) {
     auto faces = fluxFaces(fi);
@@ -1105,22 +1294,23 @@ std::vector<cplx> FDTDField::flux(int fi) {
 }
 
 // Per face and frequency: tempFlux = fluxConv_ * weight_ * simps2D(...).
-std::vector<std::vector<cplx>> FDTDField::fluxFaces(int fi) {
+// Per face the combined frequency fields (combineField, parallelFlux.hpp:325-395):
+// each stored copy of a component averaged onto the face's points with weight
+// 1 / (copies x offsets), added to the loaded incident fields (times -1).
+std::vector<std::array<std::vector<cplx>, 4>> FDTDField::fluxFields(int fi) {
     FluxBox& fb = fluxes.at(fi);
     std::vector<double> F(2 * static_cast<size_t>(ndft) * nfreq);
     if (ndft > 0 && chiml_dev_read_dft(dev_, F.data(), F.size()) != CHIML_OK)
         throw std::runtime_error(chiml_dev_last_error(dev_));
-    int steps = 0;
-    chiml_dev_steps_done(dev_, &steps);
-    const double nt = steps / fb.timeInt + 1;
-    std::vector<std::vector<cplx>> out;
-    for (auto& f : fb.faces) {
-        std::vector<cplx>& fv = out.emplace_back(nfreq, 0.0);
+    std::vector<std::array<std::vector<cplx>, 4>> out;
+    for (size_t v = 0; v < fb.faces.size(); ++v) {
+        Face& f = fb.faces[v];
         const int n1 = f.sz[f.transCor1], n2 = f.sz[f.cor];
-        auto combine = [&](std::vector<Store>& st, int pad) {
-            std::vector<cplx> out;
-            if (st.empty()) return out;
-            out.assign(static_cast<size_t>(nfreq) * n1 * n2, 0.0);
+        auto combine = [&](std::vector<Store>& st, int pad, int which) {
+            std::vector<cplx> acc;
+            if (st.empty()) return acc;
+            if (!fb.loaded.empty()) acc = fb.loaded[v][which];
+            else acc.assign(static_cast<size_t>(nfreq) * n1 * n2, 0.0);
             const int nOff = pad >= 0 ? 2 : 1;
             const double w = 1.0 / static_cast<double>(st.size() * nOff);
             for (auto& s : st)
@@ -1135,21 +1325,41 @@ std::vector<std::vector<cplx>> FDTDField::fluxFaces(int fi) {
                             int id = s.ids[q[0] + static_cast<size_t>(s.n[0]) * (q[1] + static_cast<size_t>(s.n[1]) * q[2])];
                             if (id < 0) continue;
                             for (int k = 0; k < nfreq; ++k)
-                                out[(static_cast<size_t>(k) * n1 + i1) * n2 + i2] +=
+                                acc[(static_cast<size_t>(k) * n1 + i1) * n2 + i2] +=
                                     w * cplx(F[2 * (static_cast<size_t>(id) * nfreq + k)], F[2 * (static_cast<size_t>(id) * nfreq + k) + 1]);
                         }
-            return out;
+            return acc;
         };
-        auto Ej = combine(f.Ej, f.padEj), Ek = combine(f.Ek, f.padEk), Hj = combine(f.Hj, f.padHj), Hk = combine(f.Hk, f.padHk);
+        out.push_back({combine(f.Ej, f.padEj, 0), combine(f.Ek, f.padEk, 1), combine(f.Hj, f.padHj, 2), combine(f.Hk, f.padHk, 3)});
+    }
+    return out;
+}
+
+// Per face and frequency: tempFlux = fluxConv_ * weight_ * simps2D(...).
+std::vector<std::vector<cplx>> FDTDField::fluxFaces(int fi) {
+    FluxBox& fb = fluxes.at(fi);
+    const auto fields = fluxFields(fi);
+    int steps = 0;
+    chiml_dev_steps_done(dev_, &steps);
+    const double nt = steps / fb.timeInt + 1;
+    std::vector<std::vector<cplx>> out;
+    for (size_t v = 0; v < fb.faces.size(); ++v) {
+        const Face& f = fb.faces[v];
+        std::vector<cplx>& fv = out.emplace_back(nfreq, 0.0);
+        const int n1 = f.sz[f.transCor1], n2 = f.sz[f.cor];
+        const auto& Ej = fields[v][0];
+        const auto& Ek = fields[v][1];
+        const auto& Hj = fields[v][2];
+        const auto& Hk = fields[v][3];
         std::vector<cplx> g(static_cast<size_t>(n1) * n2);
         for (int k = 0; k < nfreq; ++k) {
             for (int i1 = 0; i1 < n1; ++i1)
                 for (int i2 = 0; i2 < n2; ++i2) {
                     size_t idx = (static_cast<size_t>(k) * n1 + i1) * n2 + i2;
-                    cplx v(0.0, 0.0);
-                    if (!Ej.empty()) v = Ej[idx] * std::conj(Hk[idx]) / (nt * nt);
-                    if (!Ek.empty()) v += -1.0 * (Ek[idx] * std::conj(Hj[idx]) / (nt * nt));
-                    g[i2 + static_cast<size_t>(n2) * i1] = v;
+                    cplx val(0.0, 0.0);
+                    if (!Ej.empty()) val = Ej[idx] * std::conj(Hk[idx]) / (nt * nt);
+                    if (!Ek.empty()) val += -1.0 * (Ek[idx] * std::conj(Hj[idx]) / (nt * nt));
+                    g[i2 + static_cast<size_t>(n2) * i1] = val;
                 }
             cplx sres;
             const double d1 = IP.d[f.transCor1], d2 = IP.d[f.cor];
@@ -1166,6 +1376,79 @@ std::vector<std::vector<cplx>> FDTDField::fluxFaces(int fi) {
         }
     }
     return out;
+}
+
+// saveFields (parallelFlux.hpp:616-660): nfreq, sz_[0..2], the number of Ej,
+// Ek, Hj, Hk grids (one per face), then for each grid, Ej of every face first,
+// its n_vec (nfreq, points along transCor1, along cor) and its values,
+// frequency fastest, then cor, then transCor1; native ints and complex doubles.
+void FDTDField::saveFluxFields(int fi, const std::string& path) {
+    const FluxBox& fb = fluxes.at(fi);
+    const auto fields = fluxFields(fi);
+    std::ofstream f(path, std::ios::binary | std::ios::out);
+    if (!f) throw std::runtime_error("cannot open " + path + " for writing");
+    auto wint = [&](int v) { f.write(reinterpret_cast<const char*>(&v), sizeof(v)); };
+    wint(nfreq);
+    for (int a = 0; a < 3; ++a) wint(fb.sz[a]);
+    const int nface = static_cast<int>(fb.faces.size());
+    for (int w = 0; w < 4; ++w) wint(nface);
+    for (int w = 0; w < 4; ++w)
+        for (int v = 0; v < nface; ++v) {
+            const Face& fc = fb.faces[v];
+            const int n1 = fc.sz[fc.transCor1], n2 = fc.sz[fc.cor];
+            wint(nfreq);
+            wint(n1);
+            wint(n2);
+            const auto& src = fields[v][w];
+            std::vector<cplx> buf(static_cast<size_t>(nfreq) * n1 * n2, cplx(0.0, 0.0));
+            for (int i1 = 0; i1 < n1; ++i1)
+                for (int i2 = 0; i2 < n2; ++i2)
+                    for (int k = 0; k < nfreq; ++k)
+                        buf[k + static_cast<size_t>(nfreq) * (i2 + static_cast<size_t>(n2) * i1)] = src[(static_cast<size_t>(k) * n1 + i1) * n2 + i2];
+            f.write(reinterpret_cast<const char*>(buf.data()), static_cast<std::streamsize>(buf.size() * sizeof(cplx)));
+        }
+    if (!f) throw std::runtime_error("write to " + path + " failed");
+}
+
+// loadFields(-1.0) (parallelFlux.hpp:664-722), with its checks and messages.
+void FDTDField::loadFluxFields(FluxBox& fb, const std::string& path) {
+    std::ifstream f(path, std::ios::binary | std::ios::in);
+    if (!f) throw std::logic_error("cannot open the incident fields file " + path);
+    auto rint = [&]() {
+        int v = 0;
+        f.read(reinterpret_cast<char*>(&v), sizeof(v));
+        if (!f) throw std::logic_error("the incident fields file " + path + " is truncated");
+        return v;
+    };
+    const int nf = rint(), nx = rint(), ny = rint(), nzz = rint();
+    int cnt[4];
+    for (int w = 0; w < 4; ++w) cnt[w] = rint();
+    const int nface = static_cast<int>(fb.faces.size());
+    if (static_cast<int>(fb.freqs.size()) != nf || fb.sz[0] != nx || fb.sz[1] != ny || fb.sz[2] != nzz)
+        throw std::logic_error("Given incident fields do not match size and frequency numbers for the current calculations");
+    for (int w = 0; w < 4; ++w)
+        if (cnt[w] != nface)
+            throw std::logic_error("The size of the field vectors of the saved fields are not the same as those in the current calculation");
+    static const char* names[4] = {"Ej_freq_", "Ek_freq_", "Hj_freq_", "Hk_freq_"};
+    fb.loaded.assign(nface, {});
+    for (int w = 0; w < 4; ++w)
+        for (int v = 0; v < nface; ++v) {
+            const Face& fc = fb.faces[v];
+            const int n1 = fc.sz[fc.transCor1], n2 = fc.sz[fc.cor];
+            const int a = rint(), b = rint(), c = rint();
+            if (a != nf || b != n1 || c != n2)
+                throw std::logic_error(std::string("When loading in a ") + names[w] +
+                                       " field in a flux region, one of the size elements did not agree with the file.");
+            std::vector<cplx> buf(static_cast<size_t>(nf) * n1 * n2);
+            f.read(reinterpret_cast<char*>(buf.data()), static_cast<std::streamsize>(buf.size() * sizeof(cplx)));
+            if (!f) throw std::logic_error("the incident fields file " + path + " is truncated");
+            std::vector<cplx>& dst = fb.loaded[v][w];
+            dst.assign(buf.size(), cplx(0.0, 0.0));
+            for (int i1 = 0; i1 < n1; ++i1)
+                for (int i2 = 0; i2 < n2; ++i2)
+                    for (int k = 0; k < nf; ++k)
+                        dst[(static_cast<size_t>(k) * n1 + i1) * n2 + i2] = -1.0 * buf[k + static_cast<size_t>(nf) * (i2 + static_cast<size_t>(n2) * i1)];
+        }
 }
 
 // A CPML on some axes only is rejected: with a zero-thickness axis a next to
@@ -1309,6 +1592,56 @@ int chiml_sim_detector(chiml_sim_t* s, int d, double* times, double* values, siz
     return CHIML_OK;
 }
 
+int chiml_sim_num_freq_detectors(chiml_sim_t* s, int* n) {
+    if (!s || !n) return sim_fail("null argument");
+    *n = static_cast<int>(s->f->freqDets.size());
+    return CHIML_OK;
+}
+
+int chiml_sim_freq_info(chiml_sim_t* s, int d, int* nfreq, int* ngrid, int* npts, int* nt, double* freqs_out) {
+    if (!s) return sim_fail("null simulation");
+    auto& F = *s->f;
+    if (d < 0 || d >= static_cast<int>(F.freqDets.size())) return sim_fail("frequency detector index out of range");
+    const auto& fd = F.freqDets[d];
+    if (nfreq) *nfreq = static_cast<int>(fd.freqs.size());
+    if (ngrid) *ngrid = static_cast<int>(fd.comps.size());
+    if (npts) *npts = fd.npts();
+    if (nt) *nt = F.freqNt(d);
+    if (freqs_out)
+        for (size_t k = 0; k < fd.freqs.size(); ++k) freqs_out[k] = fd.freqConv * fd.freqs[k];
+    return CHIML_OK;
+}
+
+int chiml_sim_freq_fields(chiml_sim_t* s, int d, double* reim, size_t cap) {
+    if (!s) return sim_fail("null simulation");
+    try {
+        auto v = s->f->freqFields(d);
+        if (!reim || cap < 2 * v.size()) return sim_fail("buffer too small");
+        for (size_t k = 0; k < v.size(); ++k) {
+            reim[2 * k] = v[k].real();
+            reim[2 * k + 1] = v[k].imag();
+        }
+    } catch (std::exception& e) {
+        return sim_fail(e.what());
+    }
+    return CHIML_OK;
+}
+
+int chiml_sim_freq_values(chiml_sim_t* s, int d, double* reim, size_t cap) {
+    if (!s) return sim_fail("null simulation");
+    try {
+        auto v = s->f->freqValues(d);
+        if (!reim || cap < 2 * v.size()) return sim_fail("buffer too small");
+        for (size_t k = 0; k < v.size(); ++k) {
+            reim[2 * k] = v[k].real();
+            reim[2 * k + 1] = v[k].imag();
+        }
+    } catch (std::exception& e) {
+        return sim_fail(e.what());
+    }
+    return CHIML_OK;
+}
+
 int chiml_sim_flux(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq) {
     if (!s || !nfreq) return sim_fail("null argument");
     try {
@@ -1352,15 +1685,15 @@ int chiml_sim_write_outputs(chiml_sim_t* s, const char* prefix) {
     };
     try {
         const auto nv = IP.nVec();
-        for (size_t d = 0; d < IP.dtcs.size(); ++d) {
-            const auto& D = IP.dtcs[d];
+        for (size_t pd = 0; pd < F.pointDtc.size(); ++pd) {
+            const auto& D = IP.dtcs[F.pointDtc[pd]];
             if (D.cls != chiml::DtcClass::TXT && D.cls != chiml::DtcClass::BIN) continue;
             size_t ns = 0;
             int npts = 0;
-            int rc = chiml_sim_detector(s, static_cast<int>(d), nullptr, nullptr, 0, &ns, &npts);
+            int rc = chiml_sim_detector(s, static_cast<int>(pd), nullptr, nullptr, 0, &ns, &npts);
             if (rc) return rc;
             std::vector<double> t(ns), v(ns * static_cast<size_t>(npts));
-            rc = chiml_sim_detector(s, static_cast<int>(d), t.data(), v.data(), ns, &ns, &npts);
+            rc = chiml_sim_detector(s, static_cast<int>(pd), t.data(), v.data(), ns, &ns, &npts);
             if (rc) return rc;
             const std::string path = place(D.name);
             if (D.cls == chiml::DtcClass::BIN) {
@@ -1382,6 +1715,24 @@ int chiml_sim_write_outputs(chiml_sim_t* s, const char* prefix) {
             }
             if (rc) return rc;
         }
+        for (size_t d = 0; d < F.freqDets.size(); ++d) {
+            const auto& fd = F.freqDets[d];
+            const chiml::DtcType ty = IP.dtcs[fd.deck].type;
+            const int nincd = (ty == chiml::DtcType::EPOW || ty == chiml::DtcType::HPOW) ? 3 : 1;  // main.cpp:79-94
+            std::vector<double> freqs(fd.freqs.size()), reim;
+            for (size_t k = 0; k < freqs.size(); ++k) freqs[k] = fd.freqConv * fd.freqs[k];
+            const auto vals = fd.maps ? F.freqMapValues(static_cast<int>(d)) : F.freqValues(static_cast<int>(d));
+            for (auto& c : vals) {
+                reim.push_back(c.real());
+                reim.push_back(c.imag());
+            }
+            const std::string path = place(fd.name);
+            int rc = fd.maps ? chiml_write_freq_maps(path.c_str(), freqs.data(), static_cast<int>(freqs.size()), static_cast<int>(fd.comps.size()),
+                                                     fd.sz.data(), reim.data(), nincd)
+                             : chiml_write_freq_table(path.c_str(), freqs.data(), static_cast<int>(freqs.size()),
+                                                      static_cast<int>(fd.comps.size()), reim.data(), nincd);
+            if (rc) return rc;
+        }
         for (size_t f = 0; f < F.fluxes.size(); ++f) {
             const auto& fb = F.fluxes[f];
             auto faces = F.fluxFaces(static_cast<int>(f));
@@ -1396,6 +1747,7 @@ int chiml_sim_write_outputs(chiml_sim_t* s, const char* prefix) {
             int rc = chiml_write_flux(path.c_str(), freqs.data(), static_cast<int>(freqs.size()),
                                       static_cast<int>(faces.size()), reim.data(), fb.frontBack ? 1 : 0);
             if (rc) return rc;
+            if (fb.save) F.saveFluxFields(static_cast<int>(f), place(fb.name + "_fields.dat"));  // getFlux's saveFields
         }
         for (size_t q = 0; q < IP.qes.size(); ++q) {
             const auto& Q = IP.qes[q];

@@ -632,7 +632,28 @@ ProgramInputs parseInputs(const std::string& text) {
                 throw std::logic_error("A detector is outside the FDTD cell.");
             D.loc[c] = find_pt(tloc[c] + P.size[c] / 2.0 - tsz[c] / 2.0, P.d[c]);
         }
-        if (D.cls == DtcClass::FREQ) throw std::logic_error("frequency detectors: use a FluxList entry on this path");
+        D.outputMaps = json::get_bool(it, "output_map", false);
+        if (D.cls == DtcClass::FREQ) {
+            // frequency list, parallelInputs.cpp:718-752
+            double fCen = json::get_num(it, "fcen", -1.0), fWidth = json::get_num(it, "fwidth", -1.0);
+            double lamL = json::get_num(it, "lamL", -1.0), lamR = json::get_num(it, "lamR", -1.0);
+            int nFreq = json::get_int(it, "nfreq", -1);
+            if (nFreq < 1) throw std::logic_error("The freq detector regions need to have a number of frequencies specified");
+            D.freqs.assign(nFreq, 0.0);
+            if (fCen != -1.0 && fWidth != -1.0) {
+                if (lamL != -1.0 && lamR != -1.0)
+                    throw std::logic_error("Both a freq and wavelength range is defined, please select one to define for the freq detector");
+                double dOmg = fWidth / static_cast<double>(nFreq - 1);
+                for (int ii = 0; ii < nFreq; ++ii) D.freqs[ii] = (fCen - fWidth / 2.0 + ii * dOmg) * 2.0 * kPi;
+            } else if (lamL != -1.0 && lamR != -1.0) {
+                double dLam = (lamR - lamL) / static_cast<double>(nFreq - 1);
+                for (int ii = 0; ii < nFreq; ++ii) D.freqs[ii] = 2.0 * kPi / (lamL + ii * dLam);
+            } else {
+                throw std::logic_error("All frequency detectors must either have fcen and fwidth defined or lamL and lamR defined");
+            }
+            // the header's equal-spacing test reads freqList[2] (parallelDTC_FREQ.hpp:115-116)
+            if (nFreq < 3) throw std::logic_error("frequency detectors need at least 3 frequencies");
+        }
         P.dtcs.push_back(D);
     }
 
@@ -671,10 +692,16 @@ ProgramInputs parseInputs(const std::string& text) {
         }
         F.SI = json::get_bool(it, "SI", false);
         F.crossSec = json::get_bool(it, "cross_sec", false);
-        // save / load write and read the frequency fields of a TFSF
-        // normalisation run (parallelFlux.hpp:616-722)
-        if (json::get_bool(it, "save", false) || json::get_bool(it, "load", false))
-            throw std::logic_error("flux save/load (frequency fields of a TFSF normalisation run) are not supported on this path");
+        // save / load the face frequency fields (parallelInputs.cpp:827-835)
+        F.incdFile = json::get_str(it, "incd_fileds", "");
+        F.save = json::get_bool(it, "save", false);
+        F.load = json::get_bool(it, "load", false);
+        if (F.load && F.incdFile == "")
+            throw std::logic_error("Trying to load in file without a valid path, in the " + std::to_string(dd) + " flux detector");
+        if ((F.save || F.load) && !P.is3D())
+            // saveFields / loadFields dereference every face's Ej, Ek, Hj and Hk
+            // grids (parallelFlux.hpp:634-659), half of which a 2D box lacks
+            throw std::logic_error("flux save/load needs a 3D flux box (a 2D box lacks half of the saved field grids)");
         P.fluxes.push_back(F);
     }
     for (auto& o : P.objs) o->setUpConsts(P.dt);

@@ -104,6 +104,8 @@ struct FluxSpec {
     std::vector<double> freqs;
     bool SI = false;
     bool crossSec = false;
+    bool save = false, load = false;  // <name>_fields.dat out / incident fields in (parallelFlux.hpp:616-722)
+    std::string incdFile;
 };
 
 struct RelaxSpec {

@@ -124,6 +124,103 @@ int chiml_write_flux(const char* path, const double* freqs, int nfreq, int nface
     return close_out(f, path);
 }
 
+// parallelDetectorFREQ_Base::toFile(incd, dt) (parallelDTC_FREQ.hpp:398-444)
+// on one process: nincd incident fields, each zero without a TFSF surface.
+int chiml_write_freq_table(const char* path, const double* freqs, int nfreq, int ngrid, const double* vals_reim, int nincd) {
+    if (nfreq < 0 || ngrid < 1 || nincd < 0 || (nfreq > 0 && (!freqs || !vals_reim))) return fail("null argument");
+    using cplx = std::complex<double>;
+    std::ofstream f;
+    int rc = open_out(f, path, false);
+    if (rc) return rc;
+    f << "#" << std::setw(16) << "freq\tabs(incd)\treal(incd)\timag(incd)";
+    for (int nn = 1; nn < ngrid; ++nn) f << "\tabs(field " << nn << ")\treal(field " << nn << ")\timag(field " << nn << ")";
+    if (ngrid > 1) f << "\tabs(total)\treal(total)\timag(total)";
+    f << std::endl;
+    const cplx zero(0.0, 0.0);
+    for (int ii = 0; ii < nfreq; ++ii) {
+        f << std::setw(16) << std::setprecision(12) << freqs[ii];
+        cplx incd(0.0, 0.0);
+        for (int k = 0; k < nincd; ++k) {
+            f << "\t" << std::setw(16) << std::setprecision(12) << std::real(zero) << "\t" << std::setw(16) << std::setprecision(12)
+              << std::imag(zero) << "\t" << std::setw(16) << std::setprecision(12) << std::abs(zero);
+            incd += zero;
+        }
+        if (nincd > 2)
+            f << "\t" << std::setw(16) << std::setprecision(12) << std::real(incd) << "\t" << std::setw(16) << std::setprecision(12)
+              << std::imag(incd) << "\t" << std::setw(16) << std::setprecision(12) << std::abs(incd);
+        cplx total(0.0, 0.0);
+        for (int g = 0; g < ngrid; ++g) {
+            const size_t i = (static_cast<size_t>(g) * nfreq + ii) * 2;
+            const cplx pt(vals_reim[i], vals_reim[i + 1]);
+            f << "\t" << std::setw(16) << std::setprecision(12) << std::real(pt) << "\t" << std::setw(16) << std::setprecision(12)
+              << std::imag(pt) << "\t" << std::setw(16) << std::setprecision(12) << std::abs(pt);
+            total += pt;
+        }
+        if (ngrid > 1)
+            f << '\t' << std::setw(16) << std::setprecision(12) << std::real(total) << "\t" << std::setw(16) << std::setprecision(12)
+              << std::imag(total) << "\t" << std::setw(16) << std::setprecision(12) << std::abs(total) << std::endl;
+        else
+            f << "\n";
+    }
+    return close_out(f, path);
+}
+
+// parallelDetectorFREQ_Base::toMap(incd, dt) (parallelDTC_FREQ.hpp:506-580) on
+// one process: one file per frequency, <path>.<frequency as %f>.
+int chiml_write_freq_maps(const char* path, const double* freqs, int nfreq, int ngrid, const int sz[3], const double* vals_reim,
+                          int nincd) {
+    if (!path || !sz || nfreq < 0 || ngrid < 1 || nincd < 0 || (nfreq > 0 && (!freqs || !vals_reim))) return fail("null argument");
+    using cplx = std::complex<double>;
+    const size_t npt = static_cast<size_t>(sz[0]) * sz[1] * sz[2];
+    for (int ii = 0; ii < nfreq; ++ii) {
+        const std::string name = std::string(path) + "." + std::to_string(freqs[ii]);
+        std::ofstream f;
+        int rc = open_out(f, name.c_str(), false);
+        if (rc) return rc;
+        f << "#" << std::setw(16) << "x\ty\tz\tabs(incd)\treal(incd)\timag(incd)";
+        for (int nn = 1; nn < ngrid; ++nn) f << "\tabs(field " << nn << ")\treal(field " << nn << ")\timag(field " << nn << ")";
+        if (ngrid > 1) f << "\tabs(total)\treal(total)\timag(total)";
+        f << std::endl;
+        std::vector<double> incdOut;
+        cplx incd(0.0, 0.0);
+        const cplx zero(0.0, 0.0);
+        for (int k = 0; k < nincd; ++k) {
+            incdOut.push_back(std::real(zero));
+            incdOut.push_back(std::imag(zero));
+            incdOut.push_back(std::abs(zero));
+            incd += zero;
+        }
+        if (nincd > 2) {
+            incdOut.push_back(std::real(incd));
+            incdOut.push_back(std::imag(incd));
+            incdOut.push_back(std::abs(incd));
+        }
+        for (int yy = 0; yy < sz[1]; ++yy)
+            for (int zz = 0; zz < sz[2]; ++zz)
+                for (int xx = 0; xx < sz[0]; ++xx) {
+                    f << std::setw(16) << xx << '\t' << yy << '\t' << zz;
+                    for (double v : incdOut) f << "\t" << std::setw(16) << std::setprecision(12) << v;
+                    const size_t jj = static_cast<size_t>(xx) + static_cast<size_t>(zz) * sz[0] + static_cast<size_t>(yy) * sz[0] * sz[2];
+                    cplx total(0.0, 0.0);
+                    for (int g = 0; g < ngrid; ++g) {
+                        const size_t i = ((static_cast<size_t>(g) * nfreq + ii) * npt + jj) * 2;
+                        const cplx pt(vals_reim[i], vals_reim[i + 1]);
+                        f << "\t" << std::setw(16) << std::setprecision(12) << std::abs(pt) << "\t" << std::real(pt) << "\t" << std::imag(pt);
+                        total += pt;
+                    }
+                    if (ngrid > 1)
+                        f << '\t' << std::setw(16) << std::setprecision(12) << std::real(total) << "\t" << std::setw(16)
+                          << std::setprecision(12) << std::imag(total) << "\t" << std::setw(16) << std::setprecision(12) << std::abs(total)
+                          << std::endl;
+                    else
+                        f << "\n";
+                }
+        rc = close_out(f, name.c_str());
+        if (rc) return rc;
+    }
+    return CHIML_OK;
+}
+
 // QEPopDtc::toFile (QEPopDtc.cpp:36-61) on one process.
 int chiml_write_pops(const char* path, const double* reim, size_t ns, int interval, double dt) {
     if (ns > 0 && !reim) return fail("null argument");

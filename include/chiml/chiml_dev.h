@@ -78,6 +78,20 @@ typedef struct {
     int pop_interval;      /* levDTC_timeInt */
 } chiml_qe_t;
 
+/* A further set of DFT accumulators with its own frequencies and sampling
+ * interval: a frequency detector (parallelDetectorFREQ_Base::output and
+ * parallelStorageFreqDTCReal::fieldIn, src/DTC/parallelDTC_FREQ.hpp:258-263,
+ * src/DTC/parallelStorageFreqDTC.cpp:21-30): F[cell][f] += exp(-i w_f t_k) U(cell)
+ * at every step k * interval, t = 0 included. */
+typedef struct {
+    int ncell;
+    const int* comp;          /* [ncell] component 0..5 */
+    const long long* cell;    /* [ncell] interior index */
+    int nfreq;
+    int interval;
+    const double* phasor;     /* [nsteps/interval + 1][nfreq][2] exp(-i w t) */
+} chiml_dft_group_t;
+
 typedef struct {
     int nx, ny, nz;       /* interior points per axis; nz == 1 selects 2D TMz */
     int precision;        /* 32 or 64: arithmetic type of fields, ADE and CPML */
@@ -149,6 +163,10 @@ typedef struct {
     int nqe;
     const chiml_qe_t* qe;
     const double* eps_node;  /* (nx*ny*nz) eps at the nodes, used by the emitter back-coupling */
+
+    /* DFT groups 1..ndftg (group 0 is ndft / dft_* above), at most 7. */
+    int ndftg;
+    const chiml_dft_group_t* dftg;
 } chiml_problem_t;
 
 int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out);
@@ -168,6 +186,10 @@ int chiml_dev_read_field(chiml_dev_t* ctx, int comp, double* host);
 int chiml_dev_read_points(chiml_dev_t* ctx, int d, double* values, size_t cap, size_t* nsamples);
 /* DFT accumulators as [ndft][nfreq] complex (re, im interleaved). */
 int chiml_dev_read_dft(chiml_dev_t* ctx, double* reim, size_t cap);
+/* DFT group g (0 = the ndft cells, as chiml_dev_read_dft; 1.. = dftg[g - 1]):
+ * reim[(cell * nfreq + f) * 2 + {0,1}].  Replaces the accumulators read by
+ * parallelDetectorFREQ_Base::collectFreqFields (src/DTC/parallelDTC_FREQ.hpp:270-296). */
+int chiml_dev_read_dft_group(chiml_dev_t* ctx, int g, double* reim, size_t cap);
 /* Population series of detector j of emitter group q: [ns] complex. */
 int chiml_dev_read_pops(chiml_dev_t* ctx, int q, int j, double* reim, size_t cap, size_t* nsamples);
 /* Density matrix of (group q, member m, emitter e): N*N complex column-major. */
@@ -179,7 +201,8 @@ int chiml_dev_steps_done(chiml_dev_t* ctx, int* steps);
 int chiml_dev_profile(chiml_dev_t* ctx, int enable);
 /* As chiml_dev_profile(ctx, 1) but timing only the launches of the named
  * kernels (one name, or several separated by commas), so that a timed run
- * carries few event records per step. */
+ * carries few event records per step.  "field_step" names one event pair
+ * around all field-update launches of each step. */
 int chiml_dev_profile_only(chiml_dev_t* ctx, const char* kernel);
 int chiml_dev_profile_read(chiml_dev_t* ctx, const char* kernel, double* total_ms, int* launches);
 /* Device memory the context holds (bytes). */
@@ -204,7 +227,7 @@ void chiml_dev_destroy(chiml_dev_t* ctx);
 int chiml_dev_step_batch(chiml_dev_t** ctxs, int n, int nsteps);
 /* As chiml_dev_step_batch, allocating nothing: the parameter tables live in a
  * static device array of the library (at most 64 simulations per call,
- * 250 KB on each GPU, held until the process ends).  For callers that cannot
+ * 280 KB on each GPU, held until the process ends).  For callers that cannot
  * allocate device memory (serving loops that must not call hipMalloc, code
  * that a caller may capture).  Calls that overlap on one GPU share that array
  * and race: a call must be enqueued after the previous call's launches on a

@@ -40,6 +40,23 @@ int chiml_write_detector_bin(const char* path, const double* times, const double
 int chiml_write_flux(const char* path, const double* freqs, int nfreq, int nfaces, const double* face_reim,
                      int front_back);
 
+/* Frequency detector table (parallelDetectorFREQ_Base::toFile with incident
+ * fields, src/DTC/parallelDTC_FREQ.hpp:398-444): a header, then per frequency
+ * the frequency, real / imag / abs of each of the nincd incident fields (zero:
+ * there is no TFSF surface) and of their sum when nincd > 2, real / imag / abs
+ * of each grid's value and, with more than one grid, of their sum.
+ * vals_reim[(grid * nfreq + f) * 2 + {0,1}]. */
+int chiml_write_freq_table(const char* path, const double* freqs, int nfreq, int ngrid, const double* vals_reim, int nincd);
+
+/* Frequency detector maps (parallelDetectorFREQ_Base::toMap with incident
+ * fields, src/DTC/parallelDTC_FREQ.hpp:506-580): one file per frequency,
+ * <path>.<frequency printed as %f>, a line per box point (y, then z, x
+ * fastest) with its x y z, the incident columns and abs / real / imag of each
+ * grid's value.  vals_reim[((grid * nfreq + f) * npts + jj) * 2 + {0,1}],
+ * jj = x + z sz[0] + y sz[0] sz[2]. */
+int chiml_write_freq_maps(const char* path, const double* freqs, int nfreq, int ngrid, const int sz[3], const double* vals_reim,
+                          int nincd);
+
 /* Population detector (QEPopDtc::toFile, src/ML/QEPopDtc.cpp:36-61): per
  * sample tt the time tt * interval * dt and real / imag / abs of the mean
  * population.  reim[2 * ns]. */
@@ -47,7 +64,9 @@ int chiml_write_pops(const char* path, const double* reim, size_t ns, int interv
 
 /* Writes every output file of the simulation's deck as the reference's run
  * would leave them at its end: TXT and BIN point detectors
- * (<fname>_field_<d>.dat), flux tables (<name>.dat) and population files
+ * (<fname>_field_<d>.dat), frequency detector tables or maps (the same name),
+ * flux tables (<name>.dat) with, for boxes with "save", their face fields
+ * (<name>_fields.dat, parallelFluxDTC::saveFields) and population files
  * (<dtc_pop_fname_base><q>_level_<l>.dat), creating their directories.
  * prefix (may be NULL or "") is put in front of every relative path.  COUT
  * detectors print to standard output in the reference and write no file. */

@@ -46,6 +46,18 @@ int chiml_sim_detector(chiml_sim_t* s, int d, double* times, double* values, siz
 /* Total flux of flux box f per frequency (complex, re/im interleaved), as
  * parallelFluxDTC::getFlux sums faces (src/DTC/parallelFlux.hpp:406-534). */
 int chiml_sim_flux(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq);
+/* Frequency detectors (DetectorList entries with dtc_class "freq", in deck
+ * order; the reference's dtcFreqArr(), parallelFDTDField.hpp:1773).  Info of
+ * detector d: its frequencies as its table prints them (freqs_out, may be
+ * NULL), the grids it reads, the points sampled per grid and the samples taken
+ * so far (t = 0 included). */
+int chiml_sim_num_freq_detectors(chiml_sim_t* s, int* n);
+int chiml_sim_freq_info(chiml_sim_t* s, int d, int* nfreq, int* ngrid, int* npts, int* nt, double* freqs_out);
+/* Accumulated DFT, reim[((grid * npts + p) * nfreq + f) * 2 + {0,1}], points
+ * in storage order (x fastest, then z, then y). */
+int chiml_sim_freq_fields(chiml_sim_t* s, int d, double* reim, size_t cap);
+/* The table's value of each grid at each frequency, reim[(grid * nfreq + f) * 2 + {0,1}]. */
+int chiml_sim_freq_values(chiml_sim_t* s, int d, double* reim, size_t cap);
 int chiml_sim_pops(chiml_sim_t* s, int q, int j, double* reim, size_t cap, size_t* nsamples);
 int chiml_sim_density(chiml_sim_t* s, int q, int member, int emitter, double* reim, int* nlevel);
 /* Cell-updates per step and the algorithmic HBM bytes per step of the field

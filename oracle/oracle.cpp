@@ -50,6 +50,13 @@
 //                                     from the published table (1e-12)
 //   broadening members                member (i, j) = w_i w_j x single-member run;
 //                                     pops = weighted sum (1e-14)
+//   flux save / load                  saved file read back (reference layout) and
+//                                     integrated by scipy gives the flux (1e-12);
+//                                     loading one's own file gives 0; loading the
+//                                     empty run's file saves the difference
+//   frequency detectors               per-point DFT of a seeded oblique discrete plane
+//                                     wave in closed form (1e-11), point order, table
+//                                     values and SI factors; 2D samples nothing
 //   grid / time step                  n = floor(size/d + 0.5) + 1, 2D dt with the
 //                                     z term, ceil(tLim/dt) steps
 //   sources                           cell placement, added after the H update
@@ -95,6 +102,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -491,8 +499,24 @@ struct PointDetector {
     std::vector<double> values;  // per sample: sz product values
 };
 
-// Frequency-domain storage box (parallelStorageFreqDTC): F[f, p] accumulates
-// Re/Im(exp(-i w t)) * U(p) for the points of a box.
+// Frequency detector, parallelDetectorFREQ_Base (parallelDTC_FREQ.hpp:94-141)
+// with one storage per grid, parallelStorageFreqDTCReal with DIRECTION::NONE
+// (genDatStruct, parallelStorageFreqDTC.hpp:200-320): the storage samples
+// rows of ssz[0] points along x at (loc_x, loc_y + b, loc_z + a), a < ssz[1]
+// (z), b < ssz[2] (y), and keeps Re and Im of sum exp(-i w t) U per point and
+// frequency, point index c = a + b ssz[1], x within it, frequency fastest.
+struct FreqDetector {
+    DtcType type;
+    int timeInt;
+    bool pow = false;
+    double conv = 1.0, freqConv = 1.0;
+    std::vector<double> freqs;
+    std::array<int, 3> loc, sz;
+    std::array<int, 3> lloc, ssz;  // local corner (ghost included) and the storage's sz_ (x, z, y)
+    std::vector<DGrid*> grids;
+    std::vector<std::vector<double>> fInReal, fInCplx;  // per grid
+    int t_step = 0;
+};
 struct FreqStore {
     DGrid* grid;
     std::array<int, 3> loc, sz;  // global point coords (may start at -1)
@@ -518,6 +542,8 @@ struct Flux {
     std::vector<FluxFace> faces;
     int t_step = 0;
     double fluxConv = 1.0;
+    // loadFields(-1.0): per face the starting Ej, Ek, Hj, Hk (parallelFlux.hpp:664-722)
+    std::vector<std::array<std::vector<cplx>, 4>> loaded;
 };
 
 // -------------------------------------------------------------- emitters ----
@@ -565,6 +591,7 @@ struct Sim {
     std::unique_ptr<CPML> EPML[3], HPML[3];
     std::vector<Source> srcs;
     std::vector<PointDetector> dtcs;
+    std::vector<FreqDetector> freqDtcs;
     std::vector<Flux> fluxes;
     std::vector<QE> qes;
     std::vector<double> scratch;
@@ -587,6 +614,8 @@ struct Sim {
     void buildQE();
     void step();
     void outputDetectors();
+    void freqOutput();
+    std::vector<cplx> freqValues(int d);
     void fluxFieldIn();
     void addQE();
     void updateDensity(QE& q, bool gather = true);
@@ -594,6 +623,9 @@ struct Sim {
     void denDeriv(QE& q, const std::vector<cplx>& H, const cplx* den, cplx* out);
     void getHam(QE::Member& mem, const cplx Ex, const cplx Ey, const cplx Ez, std::vector<cplx>& H);
     std::vector<cplx> getFlux(int fi);
+    std::vector<std::array<std::vector<cplx>, 4>> faceFields(int fi);
+    void saveFields(int fi, const std::string& path);
+    void loadFields(Flux& fx, const std::string& path);
 };
 
 // setupPhysFields, src/FDTD_MANAGER/parallelFDTDField.hpp:867-947.
@@ -854,6 +886,7 @@ Sim::Sim(const std::string& json) : IP(parseInputs(json)) {
     // Detectors and flux take their t = 0 sample in the constructor
     // (parallelFDTDField.cpp:832-837).
     outputDetectors();
+    freqOutput();
     fluxFieldIn();
 }
 
@@ -919,6 +952,57 @@ static DGrid* dtcField(Sim& S, DtcType t) {
 // Point detectors, parallelDTC.hpp:44-95 and parallelFDTDField.cpp:690-830.
 void Sim::buildDetectors() {
     for (auto& D : IP.dtcs) {
+        if (D.cls == DtcClass::FREQ) {
+            // coustructDTC, parallelFDTDField.cpp:1727-1728; fields :688-826
+            FreqDetector Q;
+            Q.type = D.type;
+            Q.timeInt = static_cast<int>(std::floor(D.timeInterval / dt + 0.5));
+            if (Q.timeInt <= 0) throw std::logic_error("The time interval of a frequency detector is less than half a time step.");
+            Q.freqs = D.freqs;
+            Q.loc = D.loc;
+            Q.sz = D.sz;
+            if (D.SI) {  // parallelDTC_FREQ.hpp:123-134
+                Q.conv = IP.I0 / IP.a;
+                if (D.type == DtcType::EX || D.type == DtcType::EY || D.type == DtcType::EZ) Q.conv /= EPS0 * SPEED_OF_LIGHT;
+                Q.freqConv = SPEED_OF_LIGHT / IP.a;
+            }
+            if (D.type == DtcType::EPOW || D.type == DtcType::HPOW) {
+                Q.conv = std::pow(Q.conv, 2.0);
+                Q.pow = true;
+            }
+            Q.freqConv /= (kPi * 2.0);
+            if (D.type == DtcType::EPOW) Q.grids = is3D ? std::vector<DGrid*>{&E[0], &E[1], &E[2]} : std::vector<DGrid*>{&E[2]};
+            else if (D.type == DtcType::HPOW) Q.grids = is3D ? std::vector<DGrid*>{&H[0], &H[1], &H[2]} : std::vector<DGrid*>{&H[0], &H[1]};
+            else {
+                DGrid* g = dtcField(*this, D.type);
+                if (!g || !g->ok()) throw std::logic_error("detector on a field component that does not exist");
+                Q.grids = {g};
+            }
+            // getLocalLocEl / getLocalSzEl on one process (parallelStorageFreqDTC.hpp:168-193)
+            const DGrid& g0 = *Q.grids[0];
+            for (int i = 0; i < 3; ++i) {
+                if (Q.loc[i] >= 0 && Q.loc[i] < g0.ln(i) - 2) Q.lloc[i] = Q.loc[i] + 1;
+                else if (Q.loc[i] == -1) Q.lloc[i] = 0;
+                else Q.lloc[i] = -1;
+            }
+            if (g0.ln(2) == 1) Q.lloc[2] = 0;
+            auto localSz = [&](int ind, int localLoc) {
+                return (Q.sz[ind] + Q.loc[ind] > g0.ln(ind) - 2) ? g0.ln(ind) - localLoc - 1 : Q.loc[ind] + Q.sz[ind] - (localLoc - 1);
+            };
+            Q.ssz = {{0, 0, 0}};
+            if (Q.lloc[0] != -1 && Q.lloc[1] != -1 && Q.lloc[2] != -1) {
+                Q.ssz[0] = localSz(0, Q.lloc[0]);
+                Q.ssz[1] = localSz(2, Q.lloc[2]);
+                Q.ssz[2] = localSz(1, Q.lloc[1]);
+            }
+            const size_t n = static_cast<size_t>(std::max(0, Q.ssz[0] * Q.ssz[1] * Q.ssz[2])) * Q.freqs.size();
+            for (size_t k = 0; k < Q.grids.size(); ++k) {
+                Q.fInReal.emplace_back(n, 0.0);
+                Q.fInCplx.emplace_back(n, 0.0);
+            }
+            freqDtcs.push_back(std::move(Q));
+            continue;
+        }
         PointDetector P;
         P.type = D.type;
         P.timeInt = static_cast<int>(std::floor(D.timeInterval / dt));
@@ -948,6 +1032,57 @@ void Sim::buildDetectors() {
         }
         dtcs.push_back(P);
     }
+}
+
+// output(tt) (parallelDTC_FREQ.hpp:258-263) every timeInt steps and at t = 0:
+// fftFact = exp(-i w tt), then per storage two rank-1 updates (dger_ of
+// Re and of Im of fftFact with the row of field values, parallelStorageFreqDTC.cpp:21-30).
+void Sim::freqOutput() {
+    for (auto& Q : freqDtcs) {
+        if (t_step % Q.timeInt != 0) continue;
+        const int nf = static_cast<int>(Q.freqs.size());
+        std::vector<cplx> fft(nf);
+        for (int f = 0; f < nf; ++f) fft[f] = std::exp(cplx(0.0, -1.0 * tcur * Q.freqs[f]));
+        for (size_t g = 0; g < Q.grids.size(); ++g) {
+            DGrid& G = *Q.grids[g];
+            for (int b = 0; b < Q.ssz[2]; ++b)
+                for (int a = 0; a < Q.ssz[1]; ++a) {
+                    const size_t c = static_cast<size_t>(a) + static_cast<size_t>(b) * Q.ssz[1];
+                    for (int x = 0; x < Q.ssz[0]; ++x) {
+                        const double u = G.at(Q.lloc[0] + x, Q.lloc[1] + b, Q.lloc[2] + a);
+                        const size_t base = (c * Q.ssz[0] + x) * nf;
+                        for (int f = 0; f < nf; ++f) {
+                            Q.fInReal[g][base + f] += fft[f].real() * u;
+                            Q.fInCplx[g][base + f] += fft[f].imag() * u;
+                        }
+                    }
+                }
+        }
+        ++Q.t_step;
+    }
+}
+
+// The table's value per grid and frequency (toFile, parallelDTC_FREQ.hpp:398-444):
+// the storage copied to the front of a (nfreq, prod sz_) grid, zero elsewhere,
+// then fieldOutFreqFunction conv/2 sum F / nt or pwrOutFreqFunction
+// conv/2 sum F conj(F) / nt^2 over all prod sz_ points (parallelDTCOutputFxn.cpp:9-18).
+std::vector<cplx> Sim::freqValues(int d) {
+    FreqDetector& Q = freqDtcs.at(d);
+    const int nf = static_cast<int>(Q.freqs.size());
+    const size_t np = static_cast<size_t>(std::max(0, Q.ssz[0] * Q.ssz[1] * Q.ssz[2]));
+    const size_t szFreq = static_cast<size_t>(Q.sz[0]) * Q.sz[1] * Q.sz[2];
+    std::vector<cplx> out;
+    for (size_t g = 0; g < Q.grids.size(); ++g)
+        for (int f = 0; f < nf; ++f) {
+            cplx sum(0.0, 0.0);
+            for (size_t p = 0; p < szFreq; ++p) {
+                const cplx v = p < np ? cplx(Q.fInReal[g][p * nf + f], Q.fInCplx[g][p * nf + f]) : cplx(0.0, 0.0);
+                sum += Q.pow ? v * std::conj(v) : v;
+            }
+            const double cf = Q.conv / 2.0;
+            out.push_back(Q.pow ? cf * sum / std::pow(static_cast<double>(Q.t_step), 2.0) : cf * sum / std::pow(static_cast<double>(Q.t_step), 1.0));
+        }
+    return out;
 }
 
 void Sim::outputDetectors() {
@@ -1093,8 +1228,75 @@ void Sim::buildFlux() {
             else if (fx.sz[1] == 1) make(1, true);
             else if (fx.sz[2] == 1) make(2, true);
         }
+        if (F.load) loadFields(fx, F.incdFile);
         fluxes.push_back(fx);
     }
+}
+
+// Face frequency grids of saveFields / loadFields: Grid<cplx> of n_vec
+// (nfreq, sz[transCor1], sz[cor]), index f + nfreq (i2 + n2 i1)
+// (Grid::getInd, src/GRID/Grid.hpp:201-205).
+void Sim::saveFields(int fi, const std::string& path) {
+    Flux& fx = fluxes.at(fi);
+    auto fields = faceFields(fi);
+    const int nf = static_cast<int>(fx.freqs.size());
+    std::ofstream f(path, std::ios::binary | std::ios::out);
+    auto put = [&](int v) { f.write(reinterpret_cast<char*>(&v), sizeof(v)); };
+    put(nf);
+    put(fx.sz[0]);
+    put(fx.sz[1]);
+    put(fx.sz[2]);
+    const int nface = static_cast<int>(fx.faces.size());
+    for (int w = 0; w < 4; ++w) put(nface);
+    for (int w = 0; w < 4; ++w)
+        for (int v = 0; v < nface; ++v) {
+            const FluxFace& fc = fx.faces[v];
+            const int n1 = fc.sz[fc.transCor1], n2 = fc.sz[fc.cor];
+            put(nf);
+            put(n1);
+            put(n2);
+            for (int i1 = 0; i1 < n1; ++i1)
+                for (int i2 = 0; i2 < n2; ++i2)
+                    for (int k = 0; k < nf; ++k) {
+                        cplx val = fields[v][w][(static_cast<size_t>(k) * n1 + i1) * n2 + i2];
+                        f.write(reinterpret_cast<char*>(&val), sizeof(val));
+                    }
+        }
+}
+
+void Sim::loadFields(Flux& fx, const std::string& path) {
+    std::ifstream f(path, std::ios::binary | std::ios::in);
+    if (!f) throw std::logic_error("cannot open " + path);
+    auto get = [&]() {
+        int v = 0;
+        f.read(reinterpret_cast<char*>(&v), sizeof(v));
+        return v;
+    };
+    const int nf = get(), nx = get(), ny = get(), nz = get();
+    int cnt[4] = {get(), get(), get(), get()};
+    if (static_cast<int>(fx.freqs.size()) != nf || fx.sz[0] != nx || fx.sz[1] != ny || fx.sz[2] != nz)
+        throw std::logic_error("Given incident fields do not match size and frequency numbers for the current calculations");
+    const int nface = static_cast<int>(fx.faces.size());
+    for (int w = 0; w < 4; ++w)
+        if (cnt[w] != nface)
+            throw std::logic_error("The size of the field vectors of the saved fields are not the same as those in the current calculation");
+    fx.loaded.assign(nface, {});
+    for (int w = 0; w < 4; ++w)
+        for (int v = 0; v < nface; ++v) {
+            const FluxFace& fc = fx.faces[v];
+            const int n1 = fc.sz[fc.transCor1], n2 = fc.sz[fc.cor];
+            if (get() != nf || get() != n1 || get() != n2) throw std::logic_error("a saved flux field's size does not agree with the file");
+            std::vector<cplx>& dst = fx.loaded[v][w];
+            dst.assign(static_cast<size_t>(nf) * n1 * n2, 0.0);
+            for (int i1 = 0; i1 < n1; ++i1)
+                for (int i2 = 0; i2 < n2; ++i2)
+                    for (int k = 0; k < nf; ++k) {
+                        cplx val;
+                        f.read(reinterpret_cast<char*>(&val), sizeof(val));
+                        dst[(static_cast<size_t>(k) * n1 + i1) * n2 + i2] = -1.0 * val;  // zscal_(weight = -1)
+                    }
+        }
+    if (!f) throw std::logic_error(path + " is truncated");
 }
 
 // fieldIn: fftFact = exp(-i w t) then a rank-1 update per box row
@@ -1145,19 +1347,20 @@ static cplx simps(const cplx* f, int n, double d) {
 
 // combineField + getFlux (no incident normalisation),
 // parallelFlux.hpp:325-395, 406-534.  Returns the total flux per frequency.
-std::vector<cplx> Sim::getFlux(int fi) {
+// combineField per face (parallelFlux.hpp:325-395): the face-centre frequency
+// fields [f][i1][i2], added onto the loaded ones.
+std::vector<std::array<std::vector<cplx>, 4>> Sim::faceFields(int fi) {
     Flux& fx = fluxes[fi];
     int nf = static_cast<int>(fx.freqs.size());
-    int nt = fx.t_step;
-    std::vector<cplx> total(nf, 0.0);
-    for (auto& face : fx.faces) {
+    std::vector<std::array<std::vector<cplx>, 4>> res;
+    for (size_t v = 0; v < fx.faces.size(); ++v) {
+        FluxFace& face = fx.faces[v];
         int n1 = face.sz[face.transCor1], n2 = face.sz[face.cor];
-        double d1 = IP.d[face.transCor1], d2 = IP.d[face.cor];
-        // Face-centre frequency fields, [f][i1][i2].
-        auto combine = [&](std::vector<FreqStore>& st, int pad) {
+        auto combine = [&](std::vector<FreqStore>& st, int pad, int which) {
             std::vector<cplx> out;
             if (st.empty()) return out;
-            out.assign(static_cast<size_t>(nf) * n1 * n2, 0.0);
+            if (!fx.loaded.empty()) out = fx.loaded[v][which];
+            else out.assign(static_cast<size_t>(nf) * n1 * n2, 0.0);
             int nOff = (pad >= 0) ? 2 : 1;
             double w = 1.0 / static_cast<double>(st.size() * nOff);
             for (auto& s : st) {
@@ -1177,8 +1380,26 @@ std::vector<cplx> Sim::getFlux(int fi) {
             }
             return out;
         };
-        std::vector<cplx> Ej = combine(face.Ej, face.padEj), Ek = combine(face.Ek, face.padEk);
-        std::vector<cplx> Hj = combine(face.Hj, face.padHj), Hk = combine(face.Hk, face.padHk);
+        res.push_back({combine(face.Ej, face.padEj, 0), combine(face.Ek, face.padEk, 1), combine(face.Hj, face.padHj, 2),
+                       combine(face.Hk, face.padHk, 3)});
+    }
+    return res;
+}
+
+std::vector<cplx> Sim::getFlux(int fi) {
+    Flux& fx = fluxes[fi];
+    int nf = static_cast<int>(fx.freqs.size());
+    int nt = fx.t_step;
+    std::vector<cplx> total(nf, 0.0);
+    auto fields = faceFields(fi);
+    for (size_t v = 0; v < fx.faces.size(); ++v) {
+        FluxFace& face = fx.faces[v];
+        int n1 = face.sz[face.transCor1], n2 = face.sz[face.cor];
+        double d1 = IP.d[face.transCor1], d2 = IP.d[face.cor];
+        const std::vector<cplx>& Ej = fields[v][0];
+        const std::vector<cplx>& Ek = fields[v][1];
+        const std::vector<cplx>& Hj = fields[v][2];
+        const std::vector<cplx>& Hk = fields[v][3];
         for (int f = 0; f < nf; ++f) {
             // flux_grid(x = i2 along cor, y = i1 along transCor1)
             std::vector<cplx> g(static_cast<size_t>(n1) * n2, 0.0);
@@ -1574,6 +1795,7 @@ void Sim::step() {
     tcur += dt;
     ++t_step;
     outputDetectors();
+    freqOutput();
     fluxFieldIn();
 }
 
@@ -1649,6 +1871,44 @@ int oracle_field(void* hp, int comp, double* out) {
 
 int oracle_num_detectors(void* hp) { return static_cast<int>(static_cast<OracleHandle*>(hp)->sim->dtcs.size()); }
 
+int oracle_num_freq_detectors(void* hp) { return static_cast<int>(static_cast<OracleHandle*>(hp)->sim->freqDtcs.size()); }
+
+// Frequency detector d: writes nfreq, grids, sampled points per grid and
+// samples taken; returns 0.
+int oracle_freq_info(void* hp, int d, int* nfreq, int* ngrid, int* npts, int* nt) {
+    auto& Q = static_cast<OracleHandle*>(hp)->sim->freqDtcs.at(d);
+    *nfreq = static_cast<int>(Q.freqs.size());
+    *ngrid = static_cast<int>(Q.grids.size());
+    *npts = std::max(0, Q.ssz[0] * Q.ssz[1] * Q.ssz[2]);
+    *nt = Q.t_step;
+    return 0;
+}
+
+// Accumulated DFT [grid][point][freq] (point c * ssz_x + x), cap complex values.
+int oracle_freq_fields(void* hp, int d, double* re, double* im, int cap) {
+    auto& Q = static_cast<OracleHandle*>(hp)->sim->freqDtcs.at(d);
+    int k = 0;
+    for (size_t g = 0; g < Q.grids.size(); ++g)
+        for (size_t i = 0; i < Q.fInReal[g].size() && k < cap; ++i, ++k) {
+            re[k] = Q.fInReal[g][i];
+            im[k] = Q.fInCplx[g][i];
+        }
+    return k;
+}
+
+// The table's values [grid][freq] and the printed frequencies.
+int oracle_freq_values(void* hp, int d, double* re, double* im, double* freqs_out) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    auto v = S.freqValues(d);
+    for (size_t i = 0; i < v.size(); ++i) {
+        re[i] = v[i].real();
+        im[i] = v[i].imag();
+    }
+    auto& Q = S.freqDtcs.at(d);
+    for (size_t f = 0; f < Q.freqs.size(); ++f) freqs_out[f] = Q.freqConv * Q.freqs[f];
+    return static_cast<int>(v.size());
+}
+
 // Returns the number of samples; writes times[ns] and values[ns*npts].
 int oracle_detector(void* hp, int d, double* times, double* values, int cap_samples, int* npts) {
     auto& P = static_cast<OracleHandle*>(hp)->sim->dtcs.at(d);
@@ -1659,6 +1919,18 @@ int oracle_detector(void* hp, int d, double* times, double* values, int cap_samp
     for (int i = 0; i < nc; ++i) times[i] = P.times[i];
     for (int i = 0; i < nc * np; ++i) values[i] = P.values[i];
     return ns;
+}
+
+// Writes flux f's face fields as saveFields does (getFlux with save, parallelFlux.hpp:535-536).
+int oracle_flux_save(void* hp, int f, const char* path) {
+    auto& h = *static_cast<OracleHandle*>(hp);
+    try {
+        h.sim->saveFields(f, path);
+    } catch (std::exception& e) {
+        g_oracle_err = e.what();
+        return -1;
+    }
+    return 0;
 }
 
 int oracle_flux(void* hp, int f, double* re, double* im, int cap) {

@@ -52,6 +52,13 @@ def lib():
         L.oracle_basis_dipole.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_flux_save.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]
+        L.oracle_num_freq_detectors.argtypes = [ctypes.c_void_p]
+        L.oracle_freq_info.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4
+        L.oracle_freq_fields.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.oracle_freq_values.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.oracle_pulse.restype = ctypes.c_double
         L.oracle_pulse.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double]
         _LIB = L
@@ -114,6 +121,38 @@ class Oracle:
         vals = np.zeros(ns * max(npts.value, 1), dtype=np.float64)
         lib().oracle_detector(self._h, d, _dp(times), _dp(vals), ns, ctypes.byref(npts))
         return times, vals.reshape(ns, npts.value)
+
+    def num_freq_detectors(self):
+        return lib().oracle_num_freq_detectors(self._h)
+
+    def freq_info(self, d):
+        """(nfreq, grids, sampled points per grid, samples taken) of frequency detector d."""
+        v = [ctypes.c_int() for _ in range(4)]
+        lib().oracle_freq_info(self._h, d, *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def freq_fields(self, d):
+        """Accumulated DFT of frequency detector d, shape (grids, points, nfreq)."""
+        nf, ng, npts, _ = self.freq_info(d)
+        n = ng * npts * nf
+        re = np.zeros(max(n, 1))
+        im = np.zeros(max(n, 1))
+        lib().oracle_freq_fields(self._h, d, _dp(re), _dp(im), n)
+        return (re[:n] + 1j * im[:n]).reshape(ng, npts, nf)
+
+    def freq_values(self, d):
+        """(printed frequencies, values of shape (grids, nfreq)) of frequency detector d's table."""
+        nf, ng, _, _ = self.freq_info(d)
+        re = np.zeros(ng * nf)
+        im = np.zeros(ng * nf)
+        fr = np.zeros(nf)
+        lib().oracle_freq_values(self._h, d, _dp(re), _dp(im), _dp(fr))
+        return fr, (re + 1j * im).reshape(ng, nf)
+
+    def flux_save(self, f, path):
+        """Writes flux f's face fields as the reference's saveFields does."""
+        if lib().oracle_flux_save(self._h, f, str(path).encode()) != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
 
     def flux(self, f, nmax=4096):
         re = np.zeros(nmax)

@@ -83,7 +83,7 @@ def test_estimate_monotone():
     (lambda d: d["FluxList"].append({"name": "f", "size": [0.4, 0.4, 0], "loc": [0, 0, 0], "fcen": 1.0,
                                      "fwidth": 0.5, "nfreq": 3, "save": True}), "save/load"),
     (lambda d: d["FluxList"].append({"name": "f", "size": [0.4, 0.4, 0], "loc": [0, 0, 0], "fcen": 1.0,
-                                     "fwidth": 0.5, "nfreq": 3, "load": True}), "save/load"),
+                                     "fwidth": 0.5, "nfreq": 3, "load": True}), "without a valid path"),
 ])
 def test_deck_errors(mutate, message):
     deck = _base()

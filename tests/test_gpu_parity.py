@@ -323,3 +323,76 @@ def test_2d_tmz_fp32_close_to_fp64_oracle():
         err = rel_linf(v_g, v_o)
         print(f"detector {d}: rel_linf={err:.3e}", flush=True)
         assert err <= 1e-5
+
+
+def _freq_deck(precision, dim=3):
+    """A small 3D deck (aniso background, dispersive sphere, plane-wave sheet)
+    with frequency detectors of every kind the path reads: a field over a box,
+    the E and H powers, a wavelength list and a sampling interval of 2 steps;
+    a point detector listed after them keeps its own index."""
+    if dim == 3:
+        d = json.loads(decks.config2_3d(n=28, steps=160, pml=6, d=0.02, sphere_cells=5, flux_cells=12, nfreq=6,
+                                        precision=precision))
+        n, dd = 28, 0.02
+    else:
+        d = json.loads(decks.config1_2d(n=48, steps=160, pml=8, cyl_radius_cells=6, src_cell=(16, 24), ndet=0,
+                                        precision=precision))
+        n, dd = 48, 0.05
+    p = lambda c: decks._pos(c, n, dd)  # noqa: E731
+    fc = 1.0 / (12 * dd)
+    z = p(17) if dim == 3 else 0.0
+    d["DetectorList"] = [
+        {"type": "Ex" if dim == 3 else "Ez", "dtc_class": "freq", "fname": "fq/e", "size": [3 * dd, 2 * dd, dd if dim == 3 else 0.0],
+         "loc": [p(15), p(12), z], "fcen": fc, "fwidth": fc, "nfreq": 5},
+        {"type": "Epow", "dtc_class": "freq", "fname": "fq/ep", "size": [2 * dd, 2 * dd, 0.0], "loc": [p(13), p(15), z],
+         "lamL": 1.0 / (1.4 * fc), "lamR": 1.0 / (0.7 * fc), "nfreq": 4, "Time_Interval": 2 * decks.dt_of(dd)},
+        {"type": "Hpow", "dtc_class": "freq", "fname": "fq/hp", "size": [dd, 3 * dd, 0.0], "loc": [p(14), p(14), z],
+         "fcen": fc, "fwidth": 0.5 * fc, "nfreq": 3, "SI": False},
+        {"type": "Ez", "dtc_class": "cout", "fname": "pt", "size": [0, 0, 0], "loc": [p(15), p(13), z]},
+    ]
+    return json.dumps(d)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_freq_detectors_3d(precision):
+    """Running DFTs of the frequency detectors (parallelDetectorFREQ_Base::output,
+    src/DTC/parallelDTC_FREQ.hpp:258-263) on the GPU against the oracle: the
+    accumulators per grid, point and frequency, the samples taken, and the
+    table's values; FP32 fields keep FP64 accumulators."""
+    deck = _freq_deck(precision)
+    print(f"freq detectors 3D {precision}", flush=True)
+    sim, orc = _pair(deck, 160)
+    tol = FP64_TOL if precision == "fp64" else FP32_TOL
+    fds = sim.dtcFreqArr()
+    assert len(fds) == orc.num_freq_detectors() == 3
+    for k, fd in enumerate(fds):
+        fr, ng, npts, nt = fd.info()
+        nf, ng_o, npts_o, nt_o = orc.freq_info(k)
+        assert (fr.size, ng, npts, nt) == (nf, ng_o, npts_o, nt_o), k
+        fr_o, vals_o = orc.freq_values(k)
+        np.testing.assert_allclose(fr, fr_o, rtol=1e-15)
+        F, F_o = fd.fields(), orc.freq_fields(k)
+        assert np.abs(F_o).max() > 0.0
+        err_f = rel_linf(F, F_o)
+        err_v = rel_linf(fd.values(), vals_o)
+        print(f"freq detector {k}: {ng} grids x {npts} points x {nf} freqs, nt {nt}: fields {err_f:.3e}, values {err_v:.3e}",
+              flush=True)
+        assert err_f <= tol and err_v <= tol, (k, err_f, err_v)
+    # the point detector after them is point detector 0
+    t_o, v_o = orc.detector(0)
+    t_g, v_g = sim.dtcArr()[0].series()
+    assert rel_linf(v_g, v_o) <= tol
+
+
+def test_freq_detector_2d_is_zero():
+    """In 2D the reference's frequency storage samples no point (its z extent
+    is 0), so the table is zero; the product keeps that."""
+    deck = _freq_deck("fp64", dim=2)
+    sim, orc = _pair(deck, 160)
+    for k, fd in enumerate(sim.dtcFreqArr()):
+        fr, ng, npts, nt = fd.info()
+        assert npts == 0 and orc.freq_info(k)[2] == 0
+        assert np.all(fd.values() == 0.0)
+    t_o, v_o = orc.detector(0)
+    t_g, v_g = sim.dtcArr()[0].series()
+    assert np.abs(v_o).max() > 0.0 and rel_linf(v_g, v_o) <= FP64_TOL

@@ -7,6 +7,7 @@ population file src/ML/QEPopDtc.cpp:36-61; file names
 src/INPUTS/parallelInputs.cpp:650, 680, 760.
 """
 import json
+import math
 import struct
 
 import numpy as np
@@ -131,3 +132,88 @@ def test_population_files(tmp_path):
             assert float(r[3]) == pytest.approx(abs(p[tt]), rel=1e-15)
     assert q.population(0)[0] == pytest.approx(1.0)
     sim.close()
+
+
+def test_freq_detector_files(tmp_path):
+    """Frequency-detector tables and maps written after a 3D run read back as
+    the values the API returns (12 significant digits), with the file names
+    <fname>_field_<d>.dat and, for maps, one file per frequency."""
+    from chiml_amd import FDTDField
+    d = json.loads(decks.config2_3d(n=24, steps=80, pml=4, d=0.02, sphere_cells=4, flux_cells=10, nfreq=4,
+                                    precision="fp64"))
+    p = lambda c: decks._pos(c, 24, 0.02)  # noqa: E731
+    fc = 1.0 / (12 * 0.02)
+    d["DetectorList"] = [
+        {"type": "Epow", "dtc_class": "freq", "fname": "fq/tab", "size": [0.04, 0.02, 0.0], "loc": [p(12), p(12), p(10)],
+         "fcen": fc, "fwidth": fc, "nfreq": 3},
+        {"type": "Hy", "dtc_class": "freq", "fname": "fq/map", "size": [0.02, 0.02, 0.02], "loc": [p(12), p(12), p(11)],
+         "fcen": fc, "fwidth": fc, "nfreq": 3, "output_map": True},
+    ]
+    d["FluxList"] = []
+    print("3D 24^3 freq detector files", flush=True)
+    sim = FDTDField(json.dumps(d))
+    sim.run()
+    sim.toFile(tmp_path)
+    tab, mp = sim.dtcFreqArr()
+    fr, ng, npts, nt = tab.info()
+    vals = tab.values()
+    rows = _rows(tmp_path / "fq" / "tab_field_0.dat")
+    assert rows[0][0].startswith("#")
+    assert len(rows) == 1 + fr.size
+    for k, r in enumerate(rows[1:]):
+        x = [float(v) for v in r]
+        assert x[0] == pytest.approx(fr[k], rel=1e-11)
+        assert x[1:13] == [0.0] * 12  # three incident fields and their total
+        for gi in range(ng):
+            re, im, ab = x[13 + 3 * gi: 16 + 3 * gi]
+            assert re == pytest.approx(vals[gi, k].real, rel=1e-11, abs=1e-300)
+            assert im == pytest.approx(vals[gi, k].imag, rel=1e-11, abs=1e-300)
+            assert ab == pytest.approx(abs(vals[gi, k]), rel=1e-11, abs=1e-300)
+    frm, _, npm, ntm = mp.info()
+    F = mp.fields()
+    for k in range(frm.size):
+        rows = _rows(tmp_path / "fq" / ("map_field_1.dat." + "%f" % frm[k]))
+        assert len(rows) == 1 + 8
+        for jj, r in enumerate(rows[1:]):
+            # H component, SI: conv = I0 / a; value conv/2 F / nt per point
+            c = 2.9979245e8  # the reference's speed of light
+            conv = 1.0 / (4e-7 * math.pi * c * c) * c  # I0 / a = eps0 c at the default I0
+            want = conv / 2.0 * F[0, jj, k] / ntm
+            ab, re, im = (float(v) for v in r[6:9])
+            assert re == pytest.approx(want.real, rel=1e-9, abs=1e-18)
+            assert im == pytest.approx(want.imag, rel=1e-9, abs=1e-18)
+
+
+def test_flux_save_and_load(tmp_path):
+    """A flux box with save writes <name>_fields.dat with the face fields
+    (the reference's saveFields layout); a second run that loads it starts
+    from those fields negated: on the GPU as in the oracle, and the files of
+    the two agree."""
+    from chiml_amd import FDTDField
+    from oracle.pyoracle import Oracle
+    from tests.test_oracle_pins import _save_load_deck, read_flux_fields
+    print("flux save / load 30^3", flush=True)
+    a = FDTDField(_save_load_deck(save=True))
+    a.run()
+    a.toFile(tmp_path / "gpu")
+    pa = tmp_path / "gpu" / "f_fields.dat"
+    oa = Oracle(_save_load_deck(save=True))
+    oa.step(60)
+    po = tmp_path / "oracle_fields.dat"
+    oa.flux_save(0, po)
+    nf, sz, fg = read_flux_fields(pa)
+    nf_o, sz_o, fo = read_flux_fields(po)
+    assert (nf, sz) == (nf_o, sz_o)
+    scale = max(np.abs(fo[w][v]).max() for w in range(4) for v in range(6))
+    for w in range(4):
+        for v in range(6):
+            assert fg[w][v].shape == fo[w][v].shape
+            assert np.abs(fg[w][v] - fo[w][v]).max() <= 1e-9 * scale, (w, v)
+    b = FDTDField(_save_load_deck(sphere=True, load=pa))
+    b.run()
+    ob = Oracle(_save_load_deck(sphere=True, load=pa))
+    ob.step(60)
+    got, want = b.fluxArr()[0].getFlux(), ob.flux(0)
+    print("scattered flux", np.abs(want).max(), "rel err", np.abs(got - want).max() / np.abs(want).max(), flush=True)
+    assert np.abs(want).max() > 0
+    assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()

@@ -1599,3 +1599,215 @@ def test_cpml_eta_eff_in_partly_filled_layers(slab):
     assert _eta_run(slab, True) < 1e-11
     if slab[0] > 0:
         assert _eta_run(slab, False) > 1e-4
+
+
+# ------------------------------------------------- frequency detectors ------
+C_SI_REF = 2.9979245e8  # the reference's speed of light
+EPS0_REF = 1.0 / (4e-7 * math.pi * C_SI_REF ** 2)
+
+def _box_corner(n, d, size):
+    """find_pt(0 + size_cell/2 - size/2): the lower corner of a box centred
+    on the cell centre, in points."""
+    return int(math.floor(((n - 1) * d / 2.0 - size / 2.0) / d + 0.5))
+
+
+@pytest.mark.parametrize("kind", ["Ey", "Hz", "Epow", "Hpow"])
+def test_freq_detector_of_an_oblique_discrete_plane_wave(kind):
+    """A frequency detector in the seeded oblique plane wave: each grid's
+    running DFT at every point of its box is the closed-form sum over the
+    samples after each of the K steps (the t = 0 sample was taken on zero
+    fields, before the hook), the component read at its own Yee position,
+    points x fastest, then z, then y; the table's value is
+    conv/2 sum_p F / nt (power: conv/2 sum_p |F|^2 / nt^2), nt = K + 1, with
+    the SI factors (E components: 1 at the default I0; H and both powers:
+    I0/a, squared for a power) and the printed frequencies w c / (2 pi a)."""
+    n, d, steps = 41, 0.05, 10
+    deck = decks.empty_deck((n, n, n), d, steps, 0)
+    o0 = Oracle(json.dumps(deck))
+    k, w, e0, h0 = _discrete_plane_wave([2 * math.pi / (9.7 * d), -2 * math.pi / (12.9 * d), 2 * math.pi / (10.9 * d)],
+                                        [0.5, 0.3, -0.7], d, o0.dt)
+    fc = w / (2 * math.pi)
+    size = [3 * d, 2 * d, 1 * d]  # 4 x 3 x 2 points
+    deck["DetectorList"].append({"type": kind, "dtc_class": "freq", "fname": "fq", "size": size, "loc": [0.0, 0.0, 0.0],
+                                 "fcen": fc, "fwidth": 0.8 * fc, "nfreq": 4})
+    o = Oracle(json.dumps(deck))
+    phi = -0.61
+    _seed_plane_wave(o, n, d, k, w, e0, h0, phi)
+    o.step(steps)
+    comps = {"Ey": [1], "Hz": [5], "Epow": [0, 1, 2], "Hpow": [3, 4, 5]}[kind]
+    nf, ng, npts, nt = o.freq_info(0)
+    assert (nf, ng, npts, nt) == (4, len(comps), 24, steps + 1)
+    lo = [_box_corner(n, d, s) for s in size]
+    omegas = 2 * np.pi * (fc - 0.4 * fc + np.arange(4) * 0.8 * fc / 3)
+    t = np.cumsum(np.full(steps, o.dt))  # tcur after each step
+    # storage order: y outermost, then z, x fastest
+    yy, zz, xx = np.meshgrid(np.arange(3), np.arange(2), np.arange(4), indexing="ij")
+    pts = [lo[0] + xx.ravel(), lo[1] + yy.ravel(), lo[2] + zz.ravel()]
+    want = np.zeros((ng, npts, nf), dtype=complex)
+    for g, c in enumerate(comps):
+        pos = [pts[a] + (0.5 if a in _STAGGER[c] else 0.0) for a in range(3)]
+        ph = sum(k[a] * pos[a] * d for a in range(3)) + phi
+        amp = e0[c] if c < 3 else h0[c - 3]
+        if c >= 3:
+            ph = ph + w * o.dt / 2.0
+        u = amp * np.cos(ph[:, None] - w * t[None, :])  # (points, steps)
+        want[g] = u @ np.exp(-1j * np.outer(t, omegas))
+    got = o.freq_fields(0)
+    scale = np.abs(want).max()
+    assert scale > 0
+    assert np.abs(got - want).max() <= 1e-11 * scale
+    # the same samples in x-then-y-then-z order (a transposed storage) differ
+    alt = want[:, np.arange(npts).reshape(3, 2, 4).transpose(1, 0, 2).ravel(), :]
+    assert np.abs(alt - want).max() > 1e-3 * scale
+    fr, vals = o.freq_values(0)
+    np.testing.assert_allclose(fr, omegas * C_SI_REF / A_UNIT / (2 * np.pi), rtol=1e-14)
+    # convFactor_ = I0/a, divided by eps0 c for the E components only: the
+    # E power keeps I0/a (parallelDTC_FREQ.hpp:123-130), then squared
+    conv = 1.0 if kind == "Ey" else (A_UNIT * EPS0_REF * C_SI_REF) / A_UNIT
+    if kind.endswith("pow"):
+        expect = conv ** 2 / 2.0 * (np.abs(want) ** 2).sum(axis=1) / nt ** 2
+    else:
+        expect = conv / 2.0 * want.sum(axis=1) / nt
+    assert np.abs(vals - expect).max() <= 1e-10 * np.abs(expect).max()
+
+
+def test_freq_detector_in_2d_samples_nothing():
+    """The reference's storage sizes its z extent as ln_z - loc_z - 1 = 0 on a
+    2D grid (one z point, no ghosts; parallelStorageFreqDTC.hpp:187-193), so a
+    2D frequency detector accumulates nothing and its table is zero."""
+    deck = decks.empty_deck((32, 32, 1), 0.05, 40, 4)
+    deck["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "loc": [0.0, 0.0, 0.0],
+                               "PulseList": [decks.gaussian_pulse(1.0, 1.0, 3.0, 1.0)]})
+    deck["DetectorList"].append({"type": "Ez", "dtc_class": "freq", "fname": "fq", "size": [0.2, 0.2, 0.0],
+                                 "loc": [0.0, 0.0, 0.0], "fcen": 1.0, "fwidth": 0.5, "nfreq": 3})
+    deck["DetectorList"].append({"type": "Ez", "dtc_class": "cout", "fname": "pt", "size": [0, 0, 0], "loc": [0.0, 0.0, 0.0]})
+    o = Oracle(json.dumps(deck))
+    o.step(40)
+    assert o.freq_info(0) == (3, 1, 0, 41)
+    _, vals = o.freq_values(0)
+    assert np.all(vals == 0.0)
+    assert np.abs(o.detector(0)[1]).max() > 0.0  # the field is there
+
+
+# ------------------------------------------------------ flux save / load ----
+def read_flux_fields(path):
+    """The saveFields file (parallelFlux.hpp:616-660): nfreq, sz[3], the
+    counts of Ej, Ek, Hj, Hk grids, then per grid n_vec[3] = (nfreq, n1, n2)
+    and its values, frequency fastest.  Returns (nfreq, sz, {w: [arrays
+    (n1, n2, nfreq)]}) with w in Ej, Ek, Hj, Hk order."""
+    import struct
+    b = open(path, "rb").read()
+    nf, sx, sy, sz = struct.unpack_from("4i", b, 0)
+    cnt = struct.unpack_from("4i", b, 16)
+    off = 32
+    out = {}
+    for w in range(4):
+        out[w] = []
+        for _ in range(cnt[w]):
+            a, n1, n2 = struct.unpack_from("3i", b, off)
+            off += 12
+            assert a == nf
+            v = np.frombuffer(b, dtype=np.complex128, count=nf * n1 * n2, offset=off).reshape(n1, n2, nf)
+            off += 16 * nf * n1 * n2
+            out[w].append(v.copy())
+    assert off == len(b)
+    return nf, (sx, sy, sz), out
+
+
+def _flux_from_fields(fields, nt, d, weights):
+    """sum_faces weight * simpson2D((Ej Hk* - Ek Hj*) / nt^2), odd point counts."""
+    from scipy.integrate import simpson
+    tot = 0.0
+    for v, wgt in enumerate(weights):
+        Ej, Ek, Hj, Hk = (fields[w][v] for w in range(4))
+        g = (Ej * np.conj(Hk) - Ek * np.conj(Hj)) / nt ** 2  # (n1, n2, nf)
+        tot = tot + wgt * simpson(simpson(g, dx=d, axis=1), dx=d, axis=0)
+    return tot
+
+
+def _save_load_deck(n=30, steps=60, sphere=False, load=None, save=False):
+    d = 0.05
+    deck = decks.empty_deck((n, n, n), d, steps, 6)
+    deck["SourceList"].append({"pol": "Ez", "size": [0, 0, 0], "loc": [-3 * d, 2 * d, 0.0],
+                               "PulseList": [decks.gaussian_pulse(1.0 / (10 * d), 1.0 / (10 * d), 3.0, 1.0)]})
+    if sphere:
+        deck["ObjectList"].append({"shape": "sphere", "material": "custom", "eps": 3.0, "pols": [],
+                                   "radius": 2.5 * d, "loc": [d, 0.0, 0.0], "Basis_Set": []})
+    # 11 x 9 x 7 points: every face has two different odd extents
+    fl = {"name": "f", "size": [10 * d, 8 * d, 6 * d], "loc": [0.0, 0.0, 0.0], "fcen": 1.0 / (10 * d), "fwidth": 0.5 / (10 * d),
+          "nfreq": 3}
+    if save:
+        fl["save"] = True
+    if load:
+        fl["load"] = True
+        fl["incd_fileds"] = str(load)
+    deck["FluxList"].append(fl)
+    return json.dumps(deck)
+
+
+def test_flux_saved_fields_reproduce_the_flux(tmp_path):
+    """The saved face fields, read back with the reference's layout and
+    integrated with scipy's Simpson rule (odd point counts), give the flux
+    the oracle reports: the file holds the combined face fields, each face's
+    grid (nfreq, sz[transCor1], sz[cor]) frequency fastest, faces in the
+    box's order (x+, x-, y+, y-, z+, z-; weights +1, -1)."""
+    o = Oracle(_save_load_deck(save=True))
+    o.step(60)
+    p = tmp_path / "a_fields.dat"
+    o.flux_save(0, p)
+    nf, sz, fields = read_flux_fields(p)
+    assert nf == 3 and sz == (11, 9, 7) and all(len(fields[w]) == 6 for w in range(4))
+    # x faces: (sz[transCor1] = y, sz[cor] = z); y faces (z, x); z faces (y, x)
+    assert [fields[0][v].shape for v in (0, 2, 4)] == [(9, 7, 3), (7, 11, 3), (9, 11, 3)]
+    want = o.flux(0)
+    got = _flux_from_fields(fields, 61, 0.05, [1, -1, 1, -1, 1, -1])
+    assert np.abs(want).max() > 0
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_flux_load_subtracts_the_saved_fields(tmp_path):
+    """loadFields(-1.0): the loaded fields start each face's sums negated, so
+    a run that loads its own saved fields reports zero, and a run with a
+    scatterer that loads the empty run's fields reports the flux of the
+    difference fields, whose own saved file is the difference of the two
+    runs' files."""
+    pa = tmp_path / "a_fields.dat"
+    a = Oracle(_save_load_deck(save=True))
+    a.step(60)
+    a.flux_save(0, pa)
+    same = Oracle(_save_load_deck(load=pa))
+    same.step(60)
+    assert np.abs(same.flux(0)).max() <= 1e-20 * np.abs(a.flux(0)).max()
+    full = Oracle(_save_load_deck(sphere=True, save=True))
+    full.step(60)
+    pf = tmp_path / "full_fields.dat"
+    full.flux_save(0, pf)
+    scat = Oracle(_save_load_deck(sphere=True, load=pa, save=True))
+    scat.step(60)
+    ps = tmp_path / "scat_fields.dat"
+    scat.flux_save(0, ps)
+    _, _, fa = read_flux_fields(pa)
+    _, _, ff = read_flux_fields(pf)
+    _, _, fs = read_flux_fields(ps)
+    for w in range(4):
+        for v in range(6):
+            diff = ff[w][v] - fa[w][v]
+            assert np.abs(fs[w][v] - diff).max() <= 1e-14 * max(np.abs(ff[w][v]).max(), 1e-300)
+    want = _flux_from_fields(fs, 61, 0.05, [1, -1, 1, -1, 1, -1])
+    got = scat.flux(0)
+    assert np.abs(got).max() > 1e-6 * np.abs(full.flux(0)).max()  # the sphere scatters
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_flux_load_checks_the_file(tmp_path):
+    pa = tmp_path / "a_fields.dat"
+    a = Oracle(_save_load_deck(save=True))
+    a.flux_save(0, pa)
+    bad = json.loads(_save_load_deck(load=pa))
+    bad["FluxList"][0]["nfreq"] = 4
+    with pytest.raises(Exception, match="do not match size and frequency"):
+        Oracle(json.dumps(bad))
+    bad = json.loads(_save_load_deck())
+    bad["FluxList"][0]["load"] = True
+    with pytest.raises(Exception, match="without a valid path"):
+        Oracle(json.dumps(bad))

@@ -136,3 +136,80 @@ def test_writer_errors(tmp_path):
         chiml_amd.write_pops(tmp_path / "missing_dir" / "x.dat", np.ones(2, complex), 1, 0.1)
     with pytest.raises(chiml_amd.ChimlError, match="size must be positive"):
         chiml_amd.write_detector_bin(tmp_path / "b.dat", np.zeros(1), np.zeros(1), (0, 1, 1), (0, 0, 0))
+
+
+def _freq_header(lead, ngrid):
+    h = "#" + lead
+    for nn in range(1, ngrid):
+        h += f"\tabs(field {nn})\treal(field {nn})\timag(field {nn})"
+    if ngrid > 1:
+        h += "\tabs(total)\treal(total)\timag(total)"
+    return h + "\n"
+
+
+@pytest.mark.parametrize("ngrid,nincd", [(1, 1), (3, 3), (2, 3)])
+def test_freq_table_bytes(tmp_path, ngrid, nincd):
+    """parallelDetectorFREQ_Base::toFile with incident fields
+    (src/DTC/parallelDTC_FREQ.hpp:398-444): the header names one grid fewer
+    than there are; every value is setw(16) at precision 12, real / imag /
+    abs; zero incident columns (and their total with more than two)."""
+    rng = np.random.default_rng(8)
+    nfreq = 6
+    freqs = np.linspace(2.2e14, 4.4e14, nfreq)
+    vals = (rng.standard_normal((ngrid, nfreq)) + 1j * rng.standard_normal((ngrid, nfreq))) * 1e-7
+    vals[0, 0] = 0.0
+    vals[-1, 1] = -3.5 + 0.0j
+    p = tmp_path / "fq.dat"
+    chiml_amd.write_freq_table(p, freqs, vals, nincd)
+    # setw(16) on a longer header string pads nothing
+    want = _freq_header("freq\tabs(incd)\treal(incd)\timag(incd)", ngrid)
+    for k in range(nfreq):
+        want += g(freqs[k], 12, 16)
+        for _ in range(nincd):
+            want += "\t" + g(0.0, 12, 16) + "\t" + g(0.0, 12, 16) + "\t" + g(0.0, 12, 16)
+        if nincd > 2:
+            want += "\t" + g(0.0, 12, 16) + "\t" + g(0.0, 12, 16) + "\t" + g(0.0, 12, 16)
+        tot = 0j
+        for gi in range(ngrid):
+            x = vals[gi, k]
+            tot += x
+            want += "\t" + g(x.real, 12, 16) + "\t" + g(x.imag, 12, 16) + "\t" + g(abs(x), 12, 16)
+        if ngrid > 1:
+            want += "\t" + g(tot.real, 12, 16) + "\t" + g(tot.imag, 12, 16) + "\t" + g(abs(tot), 12, 16)
+        want += "\n"
+    assert p.read_bytes() == want.encode()
+
+
+@pytest.mark.parametrize("ngrid,nincd", [(1, 1), (2, 3)])
+def test_freq_maps_bytes(tmp_path, ngrid, nincd):
+    """parallelDetectorFREQ_Base::toMap with incident fields
+    (src/DTC/parallelDTC_FREQ.hpp:506-580): one file per frequency named
+    <path>.<frequency as %f>; a line per point, y outermost, then z, x
+    fastest, with x at setw(16), the incident values, and per grid abs at
+    setw(16) then real and imag unpadded."""
+    rng = np.random.default_rng(9)
+    nfreq, sz = 3, (3, 2, 2)
+    npt = sz[0] * sz[1] * sz[2]
+    freqs = np.array([1.5e14, 2.25e14, 3.0e14])
+    vals = (rng.standard_normal((ngrid, nfreq, npt)) + 1j * rng.standard_normal((ngrid, nfreq, npt))) * 1e-5
+    base = tmp_path / "map"
+    chiml_amd.write_freq_maps(base, freqs, sz, vals, nincd)
+    incd = [0.0] * (3 * nincd + (3 if nincd > 2 else 0))
+    for k in range(nfreq):
+        p = tmp_path / ("map." + "%f" % freqs[k])
+        want = _freq_header("x\ty\tz\tabs(incd)\treal(incd)\timag(incd)", ngrid)
+        for yy in range(sz[1]):
+            for zz in range(sz[2]):
+                for xx in range(sz[0]):
+                    jj = xx + zz * sz[0] + yy * sz[0] * sz[2]
+                    want += str(xx).rjust(16) + "\t" + str(yy) + "\t" + str(zz)
+                    want += "".join("\t" + g(v, 12, 16) for v in incd)
+                    tot = 0j
+                    for gi in range(ngrid):
+                        x = vals[gi, k, jj]
+                        tot += x
+                        want += "\t" + g(abs(x), 12, 16) + "\t" + g(x.real, 12) + "\t" + g(x.imag, 12)
+                    if ngrid > 1:
+                        want += "\t" + g(tot.real, 12, 16) + "\t" + g(tot.imag, 12, 16) + "\t" + g(abs(tot), 12, 16)
+                    want += "\n"
+        assert p.read_bytes() == want.encode(), k
