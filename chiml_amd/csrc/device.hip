@@ -8,7 +8,7 @@
 //                    (addPul, updateE/updateD, updateEPML, D2E)
 //                    3D FP32: k_h3_all / k_e3_all; 3D FP64: k_h3_int + k_h3_slab /
 //                    k_e3_int + k_e3_slab; 2D: k_h2v / k_e2v; else k_update_h / k_update_e
-//   3. emitters    : k_ml_addp then k_ml_team (its last block finishes the populations)
+//   3. emitters    : k_ml_addp, k_ml_team, then k_ml_pop_finish on recording steps
 //                    (parallelQEBase::addQE)
 //   4. t_step += 1 : plain launches are given the step number by the host;
 //                    inside captured graphs the H launch advances a two-slot
@@ -28,6 +28,8 @@
 // small).  The CPML uses the same b, c, kappa profiles and psi recursions as
 // src/PML/parallelPML.hpp:693-765 and src/PML/parallelPML.cpp:12-40.
 #include <hip/hip_runtime.h>
+
+#include "qe_layout.hpp"
 
 #include <algorithm>
 
@@ -1901,7 +1903,6 @@ struct QeDev {
     const int* pop_flat;
     int pop_interval;
     double2* pop_partial;   // [npop][nblocks]
-    int* pop_ticket;        // blocks of the density launch done (the last one finishes the populations)
     double2* pop_out;       // [npop][cap]
     int pop_cap;
     int dim;
@@ -1988,19 +1989,20 @@ __device__ __forceinline__ void den_deriv_rows(const QeDev& q, const double2* H,
 
 // Every emitter group (N >= 2): a team of lanes per emitter, each lane owning
 // R consecutive rows of one column j of rho, its history and its derivative
-// (R = N for N <= 3, else about N / 2: two lanes per column), advanced by the
-// Adams-Bashforth-Moulton predictor-corrector (PCABAM4, parallelQE.hpp:751-770)
-// cold-started from zero history; the derivative history is a 4-slot ring
-// instead of the reference's three copies per step (density.hpp:65-70).
-// Splitting a column over two lanes halves the per-lane registers (the
-// 5 R complex words of the ABM4 update are loaded at once), which doubles the
-// waves per SIMD for N >= 4; for N <= 3 a whole column per lane already
-// leaves at least 3.  Lanes are grouped by (column, row chunk), lane =
-// c * EB + team, so the lanes of one chunk hold consecutive emitters and every
-// rho / history access of a wave is a few contiguous runs.  H = h0 - E.mu
-// (formed once per member, each lane its rows) and the matrix being
-// differentiated are shared through LDS with a per-emitter stride of N^2 + 1
-// elements, which spreads the emitters of a wave over the LDS banks.
+// (R = N: the whole column), advanced by the Adams-Bashforth-Moulton
+// predictor-corrector (PCABAM4, parallelQE.hpp:751-770) cold-started from
+// zero history; the derivative history is a 4-slot ring instead of the
+// reference's three copies per step (density.hpp:65-70).  The kernel takes
+// any R; splitting a column over lanes (R = 2 or 3 for N = 6) cuts the VGPRs
+// from 236 to 138-162 (3 waves per SIMD instead of 2) but measured slower,
+// 0.59 / 0.49 ms against 0.45 ms per launch for 368,640 6-level emitters:
+// every extra lane repeats the shared-matrix reads of the commutator.
+// Lanes are grouped by (column, row chunk), lane = c * EB + team, so the
+// lanes of one chunk hold consecutive emitters and every rho / history
+// access of a wave is a few contiguous runs.  H = h0 - E.mu (formed once per
+// member, each lane its rows) and the matrix being differentiated are shared
+// through LDS with a per-emitter stride of N^2 + 1 elements, which spreads
+// the emitters of a wave over the LDS banks.
 //
 // PK (Hermitian packing): rho and the derivative history keep only the upper
 // triangle, entry (i, j) with i <= j stored at j (j + 1) / 2 + i, so a lane
@@ -2014,7 +2016,7 @@ constexpr int kMlThreads = 256;
 constexpr int kMlGamLds = 256;  // Lindblad entries staged in LDS (more are read from global memory)
 
 __host__ __device__ constexpr int ml_store_words(int n, bool pk) { return pk ? n * (n + 1) / 2 : n * n; }
-__host__ __device__ constexpr int ml_rows(int n) { return n <= 3 ? n : 2; }
+__host__ __device__ constexpr int ml_rows(int n) { return n; }
 __host__ __device__ constexpr int ml_lanes(int n) { return n * ((n + ml_rows(n) - 1) / ml_rows(n)); }  // lanes per emitter
 
 template <typename T, int N, bool PK>
@@ -2186,6 +2188,13 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
             q.Pbox[c][bx + PX * (by + PY * bz)] = sum;
         }
     }
+    // Per-block population sums, only on the steps that record a sample
+    // (QEPopDtc: every pop_interval steps while slots remain); k_ml_pop_finish
+    // adds the blocks' sums in a fixed order.  A finish inside this launch
+    // (last block by an atomic ticket) needs a device-scope release per block,
+    // which writes back the XCD's L2 each time and more than doubled the
+    // launch.
+    if (q.npop == 0 || step % q.pop_interval != 0 || step / q.pop_interval >= q.pop_cap) return;
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
         if (jj >= q.npop) break;
@@ -2198,34 +2207,28 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
         if (threadIdx.x == 0) q.pop_partial[jj * gridDim.x + blockIdx.x] = red[0];
         __syncthreads();
     }
-    // The last block to finish sums the blocks' partials (each thread a fixed
-    // strided subset, then a tree in LDS: the same order every run, whichever
-    // block is last) and stores the populations of this step (QEPopDtc, a
-    // sample every pop_interval steps, averaged over the emitters).
-    if (q.npop == 0) return;
-    __shared__ int is_last;
-    __threadfence();  // this block's partials are visible device-wide before its ticket
+}
+
+constexpr int kPopThreads = 1024;
+
+// Populations of this step: block j sums population j's per-block partials
+// (each thread a fixed strided subset, then a tree in LDS, so the order is
+// the same every run) and stores the mean over the emitters.
+__global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, int nblocks, const int* stepp, int stepv) {
+    __shared__ double2 red[kPopThreads];
+    const int jj = blockIdx.x;
+    const int step = step_now(stepp, stepv);
+    if (step % q.pop_interval != 0 || step / q.pop_interval >= q.pop_cap) return;
+    double2 sum = make_double2(0.0, 0.0);
+    for (int b = threadIdx.x; b < nblocks; b += kPopThreads) sum = cadd(sum, q.pop_partial[jj * nblocks + b]);
+    red[threadIdx.x] = sum;
     __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(q.pop_ticket, 1) == static_cast<int>(gridDim.x) - 1;
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    const int nb = static_cast<int>(gridDim.x);
-    const bool keep = step % q.pop_interval == 0 && step / q.pop_interval < q.pop_cap;
-    for (int jj = 0; jj < q.npop; ++jj) {
-        double2 sum = make_double2(0.0, 0.0);
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) sum = cadd(sum, q.pop_partial[jj * nb + b]);
-        red[threadIdx.x] = sum;
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0 && keep)
-            q.pop_out[static_cast<long long>(jj) * q.pop_cap + step / q.pop_interval] = make_double2(red[0].x / q.nemit, red[0].y / q.nemit);
+    for (int w = kPopThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) *q.pop_ticket = 0;  // for the next launch (stream-ordered)
+    if (threadIdx.x == 0)
+        q.pop_out[static_cast<long long>(jj) * q.pop_cap + step / q.pop_interval] = make_double2(red[0].x / q.nemit, red[0].y / q.nemit);
 }
 
 // ------------------------------------------------------- batched launches ----
@@ -2438,21 +2441,12 @@ struct chiml_dev {
     T* zeros(size_t n) { return upload<T>(nullptr, n); }
 };
 
-namespace {
-
-template <typename T>
-T* upload_cast(chiml_dev* c, const double* src, size_t n) {
-    std::vector<T> tmp(n);
-    for (size_t i = 0; i < n; ++i) tmp[i] = static_cast<T>(src[i]);
-    return c->upload<T>(tmp.data(), n);
-}
-
 // Does the emitter group keep rho Hermitian?  True when every member's H0 and
 // the three dipole matrices are Hermitian and the Lindblad operator maps
 // Hermitian to Hermitian: G[(a,b),(c,d)] = G[(b,a),(d,c)] for its real
 // entries (column-major flat indices).  The density kernel then stores the
 // upper triangle only.
-bool hermitian_preserving(const chiml_qe_t& qi) {
+static bool hermitian_preserving(const chiml_qe_t& qi) {
     const int N = qi.nlevel, S = N * N;
     auto herm = [&](const double* m) {
         for (int i = 0; i < N; ++i)
@@ -2476,6 +2470,17 @@ bool hermitian_preserving(const chiml_qe_t& qi) {
         if (v != kv.second) return false;
     }
     return true;
+}
+
+bool chiml_qe_packed(const chiml_qe_t& qi) { return hermitian_preserving(qi) && std::getenv("CHIML_ML_UNPACKED") == nullptr; }
+
+namespace {
+
+template <typename T>
+T* upload_cast(chiml_dev* c, const double* src, size_t n) {
+    std::vector<T> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = static_cast<T>(src[i]);
+    return c->upload<T>(tmp.data(), n);
 }
 
 template <typename T>
@@ -2746,7 +2751,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
             q.Aprev[k] = c->zeros<double>(ab);
         }
         const size_t ld = static_cast<size_t>(q.nmem) * q.nemit;
-        q.packed = hermitian_preserving(qi) && std::getenv("CHIML_ML_UNPACKED") == nullptr ? 1 : 0;
+        q.packed = chiml_qe_packed(qi) ? 1 : 0;
         const size_t SW = static_cast<size_t>(ml_store_words(q.N, q.packed != 0));
         std::vector<double> rho0(2 * SW * ld, 0.0);  // rho_00 = the member's weight (entry 0 in both layouts)
         for (int m = 0; m < q.nmem; ++m)
@@ -2761,7 +2766,6 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         const int per_block = kMlThreads / ml_lanes(q.N);  // one team of lanes per emitter
         qh.nblocks = (q.nemit + per_block - 1) / per_block;
         q.pop_partial = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(qh.nblocks)));
-        q.pop_ticket = c->zeros<int>(1);
         q.pop_cap = p->nsteps / q.pop_interval + 1;
         q.pop_out = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(q.pop_cap)));
         q.eps_node = epsn;
@@ -2999,6 +3003,7 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
 #undef CHIML_ML_TEAM
             }
         }
+        if (q.npop > 0) hipLaunchKernelGGL(k_ml_pop_finish, dim3(q.npop), dim3(kPopThreads), 0, c->stream, q, qh.nblocks, a.step, a.stepv);
     }
 }
 

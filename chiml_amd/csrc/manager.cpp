@@ -29,6 +29,7 @@
 #include "../../include/chiml/chiml_dev.h"
 #include "../../include/chiml/chiml_out.h"
 #include "../../include/chiml/chiml_sim.h"
+#include "qe_layout.hpp"
 #include "setup.hpp"
 
 namespace chiml {
@@ -1079,7 +1080,9 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         q.pop_flat = qpop[g].data();
         q.pop_interval = qs.popTimeInt;
         qeN.push_back(L);
-        bytesML += static_cast<double>(q.nemit) * q.nmembers * (7.0 * 16.0 * L * L);
+        // rho + 4 history slots read, rho + 1 slot written, per stored word
+        const double words = chiml_qe_packed(q) ? L * (L + 1) / 2.0 : static_cast<double>(L) * L;
+        bytesML += static_cast<double>(q.nemit) * q.nmembers * (7.0 * 16.0 * words);
     }
     P.nqe = static_cast<int>(qdesc.size());
     P.qe = qdesc.data();
