@@ -47,6 +47,8 @@ def load_library():
     L.chiml_sim_freq_fields.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t]
     L.chiml_sim_freq_values.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t]
     L.chiml_sim_pops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_size_t, c_size_p]
+    L.chiml_sim_incd.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t, c_size_p]
+    L.chiml_sim_flux_incd.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
     L.chiml_sim_density.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_dbl_p, c_int_p]
     L.chiml_sim_cost.argtypes = [ctypes.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]
     L.chiml_sim_kernel_bytes.argtypes = [ctypes.c_void_p, ctypes.c_char_p, c_dbl_p]
@@ -73,6 +75,7 @@ def load_library():
     L.chiml_write_detector_txt.argtypes = [ctypes.c_char_p, c_dbl_p, c_dbl_p, ctypes.c_size_t, ctypes.c_int, c_dbl_p]
     L.chiml_write_detector_bin.argtypes = [ctypes.c_char_p, c_dbl_p, c_dbl_p, ctypes.c_size_t, c_int_p, c_int_p]
     L.chiml_write_flux.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int]
+    L.chiml_write_flux_incd.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int, c_dbl_p]
     L.chiml_write_freq_table.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int]
     L.chiml_write_freq_maps.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                         c_dbl_p, ctypes.c_int]
@@ -114,14 +117,21 @@ def write_detector_bin(path, times, values, sz, loc):
     _check(load_library().chiml_write_detector_bin(str(path).encode(), _dp(t), _dp(v), t.size, s, lo))
 
 
-def write_flux(path, freqs, faces, front_back=False):
+def write_flux(path, freqs, faces, front_back=False, incd=None):
     """The reference's flux table (parallelFlux.hpp getFlux) from the
-    per-face complex fluxes faces[nfaces, nfreq] (no incident field)."""
+    per-face complex fluxes faces[nfaces, nfreq] and the incident flux per
+    frequency (None: no incident field)."""
     f = _f64(freqs)
     fc = np.asarray(faces, dtype=np.complex128).reshape(-1, f.size)
     reim = _f64(np.stack([fc.real, fc.imag], axis=-1))
-    _check(load_library().chiml_write_flux(str(path).encode(), _dp(f), f.size, fc.shape[0], _dp(reim),
-                                           1 if front_back else 0))
+    if incd is None:
+        _check(load_library().chiml_write_flux(str(path).encode(), _dp(f), f.size, fc.shape[0], _dp(reim),
+                                               1 if front_back else 0))
+        return
+    ic = np.asarray(incd, dtype=np.complex128).reshape(f.size)
+    ir = _f64(np.stack([ic.real, ic.imag], axis=-1))
+    _check(load_library().chiml_write_flux_incd(str(path).encode(), _dp(f), f.size, fc.shape[0], _dp(reim),
+                                                1 if front_back else 0, _dp(ir)))
 
 
 def write_freq_table(path, freqs, values, nincd):
@@ -221,6 +231,16 @@ class Flux:
         _check(L.chiml_sim_flux(self._sim._h, self.index, None, 0, ctypes.byref(nf)))
         buf = np.zeros(2 * nf.value)
         _check(L.chiml_sim_flux(self._sim._h, self.index, _dp(buf), buf.size, ctypes.byref(nf)))
+        return buf[0::2] + 1j * buf[1::2]
+
+    def getFluxIncd(self):
+        """Incident flux per frequency from the TFSF incident records (the
+        incident columns of the table; zero without a TFSF surface)."""
+        L = load_library()
+        nf = ctypes.c_int()
+        _check(L.chiml_sim_flux_incd(self._sim._h, self.index, None, 0, ctypes.byref(nf)))
+        buf = np.zeros(2 * nf.value)
+        _check(L.chiml_sim_flux_incd(self._sim._h, self.index, _dp(buf), buf.size, ctypes.byref(nf)))
         return buf[0::2] + 1j * buf[1::2]
 
 
@@ -325,6 +345,19 @@ class FDTDField:
 
     def E(self, c):
         return self.field(c)
+
+    def incd(self, comp):
+        """Incident records of component comp (0..5 or 'Ex'..'Hz') from the TFSF
+        surfaces: two complex values per step, the point and its offset
+        (FF.ExIncd() .. FF.HzIncd(), src/main.cpp:70)."""
+        if isinstance(comp, str):
+            comp = FIELD_NAMES[comp]
+        L = load_library()
+        n = ctypes.c_size_t()
+        _check(L.chiml_sim_incd(self._h, int(comp), None, 0, ctypes.byref(n)))
+        buf = np.zeros(2 * n.value)
+        _check(L.chiml_sim_incd(self._h, int(comp), _dp(buf), buf.size, ctypes.byref(n)))
+        return buf[0::2] + 1j * buf[1::2]
 
     def H(self, c):
         return self.field(3 + c)

@@ -2231,6 +2231,137 @@ __global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, int nblo
         q.pop_out[static_cast<long long>(jj) * q.pop_cap + step / q.pop_interval] = make_double2(red[0].x / q.nemit, red[0].y / q.nemit);
 }
 
+// ----------------------------------------------------------------- TFSF ----
+// Total-field / scattered-field surfaces (parallelTFSFBase,
+// src/SOURCE/parallelTFSF.hpp:1058-1174).  Each surface owns twelve complex
+// FP64 incident lines (six components, set 0 read by the surface adds, set 1
+// the shorter pulse line whose values are recorded for the flux
+// normalisation).  They are a few hundred to a few thousand points long, so
+// one block steps all of them (k_tfsf_line) between the H and E updates of
+// the main grid; k_tfsf_add then adds pf * Re(line) to the surface cells.
+struct TfsfLineArgs {
+    int len[2], present[6], margin, pulse_at, pulse_n, psi_n, rec_idx, rec_off[6];
+    int nterm[6], tsrc[6][2], tpos[6][2], tneg[6][2];
+    double tcoef[6][2];
+    int nseg[6], ssrc[6][4], shigh[6][4], soff1[6][4], soff2[6][4];
+    double sdb[6][4];
+    const double* scoef;  // [6][4][psi_n][3]
+    double2* line[2];     // set s: [6][len[s]]
+    double2* psi;         // [2][6][4][psi_n]
+    const double2* amp;   // [nsteps][6][pulse_n]
+    double2* rec;         // [6][nrec]
+    long long nrec;
+};
+
+// One line point: pulse, the derivative terms in the reference's order, then
+// the CPML segments that cover it (updateIncdField*, parallelTFSF.cpp:11-45;
+// addPsi / updatePsiField, src/PML/PML.cpp:12-29).
+__device__ void tfsf_point(const TfsfLineArgs& t, int s, int c, int n, int step) {
+    const int len = t.len[s];
+    double2* L = t.line[s];
+    double2 v = L[c * len + n];
+    const int pr = n - t.pulse_at;
+    if (pr >= 0 && pr < t.pulse_n) v = cadd(v, t.amp[(static_cast<long long>(step) * 6 + c) * t.pulse_n + pr]);
+    if (n >= t.margin && n < len - t.margin)
+        for (int k = 0; k < t.nterm[c]; ++k) {
+            const double2* u = L + t.tsrc[c][k] * len;
+            const double a = t.tcoef[c][k];
+            v = cadd(v, cscale(u[n + t.tpos[c][k]], a));
+            v = cadd(v, cscale(u[n + t.tneg[c][k]], -1.0 * a));
+        }
+    for (int g = 0; g < t.nseg[c]; ++g) {
+        const int gl = t.shigh[c][g] ? len - 1 - t.psi_n - t.margin : t.margin;
+        const int i = n - gl;
+        if (i < 0 || i >= t.psi_n) continue;
+        const double* co = t.scoef + ((c * 4 + g) * static_cast<long long>(t.psi_n) + i) * 3;
+        const double2* u = L + t.ssrc[c][g] * len;
+        double2& ps = t.psi[((s * 6 + c) * 4 + g) * static_cast<long long>(t.psi_n) + i];
+        double2 p = cscale(ps, co[0]);
+        p = cadd(p, cscale(u[gl + t.soff1[c][g] + i], co[1]));
+        p = cadd(p, cscale(u[gl + t.soff2[c][g] + i], co[2]));
+        ps = p;
+        v = cadd(v, cscale(p, t.sdb[c][g]));
+    }
+    L[c * len + n] = v;
+}
+
+// parallelTFSFBase::step (parallelTFSF.hpp:1148-1174) for every line, with
+// the incident H recorded before it and the incident E after it
+// (parallelFDTDField.hpp:1240-1254).  One block.
+__global__ void __launch_bounds__(1024) k_tfsf_line(TfsfLineArgs t, const int* stepp, int stepv) {
+    const int step = step_now(stepp, stepv);
+    auto record = [&](int c0) {
+        const int c = c0 + static_cast<int>(threadIdx.x);
+        if (threadIdx.x < 3 && t.present[c]) {
+            const double2* P = t.line[1] + c * t.len[1];
+            double2* r = t.rec + c * t.nrec + 2LL * step;
+            r[0] = P[t.rec_idx];
+            r[1] = P[t.rec_idx - t.rec_off[c]];
+        }
+    };
+    record(3);
+    for (int ph = 0; ph < 2; ++ph) {  // H lines (3..5), then E lines (0..2)
+        const int c0 = ph == 0 ? 3 : 0;
+        const int tot = 3 * (t.len[0] + t.len[1]);
+        for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+            int r = q, s = 0;
+            if (r >= 3 * t.len[0]) {
+                r -= 3 * t.len[0];
+                s = 1;
+            }
+            const int c = c0 + r / t.len[s], n = r % t.len[s];
+            if (t.present[c]) tfsf_point(t, s, c, n, step);
+        }
+        __syncthreads();
+    }
+    record(0);
+}
+
+struct TfsfSurfD {
+    int comp, src;
+    double pf;
+    int o[3], a1, n1, a2, n2, incd0, s1, s2;
+    long long start;  // first flat point of this surface in its launch
+};
+
+__device__ __forceinline__ bool tfsf_in(const TfsfSurfD& q, const int* x, int& i1, int& i2) {
+    const int a3 = 3 - q.a1 - q.a2;
+    i1 = x[q.a1] - q.o[q.a1];
+    i2 = x[q.a2] - q.o[q.a2];
+    return x[a3] == q.o[a3] && i1 >= 0 && i1 < q.n1 && i2 >= 0 && i2 < q.n2;
+}
+
+// U += pf * Re(line[src][incd0 + i1 s1 + i2 s2]) over the surfaces of one
+// family (addIncdFields, parallelTFSF.cpp:78-84).  A cell on several surfaces
+// of its component (an edge of the box) takes all their adds in order from
+// the thread of its first surface, so the sums keep the reference's order.
+template <typename T>
+__global__ void __launch_bounds__(256) k_tfsf_add(const TfsfSurfD* __restrict__ sf, int ns, long long total, T* F0, T* F1, T* F2,
+                                                  T* F3, T* F4, T* F5, const double2* __restrict__ line, int len, int nx, int ny) {
+    const long long gid = blockIdx.x * 256LL + threadIdx.x;
+    if (gid >= total) return;
+    int s = 0;
+    while (s + 1 < ns && sf[s + 1].start <= gid) ++s;
+    const TfsfSurfD& q = sf[s];
+    const long long pnt = gid - q.start;
+    int x[3] = {q.o[0], q.o[1], q.o[2]};
+    x[q.a1] += static_cast<int>(pnt % q.n1);
+    x[q.a2] += static_cast<int>(pnt / q.n1);
+    int i1, i2;
+    for (int k = 0; k < s; ++k)
+        if (sf[k].comp == q.comp && tfsf_in(sf[k], x, i1, i2)) return;
+    T* const Fs[6] = {F0, F1, F2, F3, F4, F5};
+    T* F = Fs[q.comp];
+    const long long idx = x[0] + static_cast<long long>(nx) * (x[1] + static_cast<long long>(ny) * x[2]);
+    T v = F[idx];
+    for (int k = s; k < ns; ++k) {
+        const TfsfSurfD& r = sf[k];
+        if (r.comp != q.comp || !tfsf_in(r, x, i1, i2)) continue;
+        v += static_cast<T>(r.pf * line[r.src * len + r.incd0 + i1 * r.s1 + i2 * r.s2].x);
+    }
+    F[idx] = v;
+}
+
 // ------------------------------------------------------- batched launches ----
 // One launch per kernel class covers many simulations (the serving loop's
 // batches): slot k of a device-side descriptor table holds simulation k's
@@ -2422,6 +2553,15 @@ struct chiml_dev {
     };
     std::vector<DftGroup> dgs;
     std::vector<QeHost> qes;
+    struct TfsfHost {  // one TFSF surface: its lines and its surface table (H adds, then E adds)
+        TfsfLineArgs line{};
+        TfsfSurfD* surf = nullptr;
+        int nsh = 0, nse = 0;
+        long long toth = 0, tote = 0;
+    };
+    std::vector<TfsfHost> tfsf;
+    double2* incd_rec = nullptr;  // [6][nrec] incident records (E_incd_, H_incd_)
+    long long nrec = 0;
     size_t footprint = 0;
     bool profile = false;
     std::string profile_only;  // non-empty: time only launches of this kernel
@@ -2481,6 +2621,133 @@ T* upload_cast(chiml_dev* c, const double* src, size_t n) {
     std::vector<T> tmp(n);
     for (size_t i = 0; i < n; ++i) tmp[i] = static_cast<T>(src[i]);
     return c->upload<T>(tmp.data(), n);
+}
+
+// TFSF tables: every index a kernel will form is checked here, on the host.
+template <typename T>
+void build_tfsf(chiml_dev* c, const chiml_problem_t* p, const StepArgs<T>& a) {
+    if (p->ntfsf < 0 || (p->ntfsf > 0 && !p->tfsf)) throw std::runtime_error("invalid TFSF table");
+    if (p->ntfsf == 0) return;
+    c->nrec = 2LL * (p->nsteps + 1) + 2;
+    c->incd_rec = reinterpret_cast<double2*>(c->zeros<double>(2 * 6 * static_cast<size_t>(c->nrec)));
+    auto need = [](bool ok, const char* what) {
+        if (!ok) throw std::runtime_error(std::string("TFSF table: ") + what);
+    };
+    for (int k = 0; k < p->ntfsf; ++k) {
+        const chiml_tfsf_t& tf = p->tfsf[k];
+        chiml_dev::TfsfHost h;
+        TfsfLineArgs& t = h.line;
+        t.len[0] = tf.len0;
+        t.len[1] = tf.len1;
+        need(tf.len0 > 0 && tf.len1 > 0 && tf.margin >= 0 && tf.psi_n >= 0 && tf.pulse_n >= 0, "sizes");
+        t.margin = tf.margin;
+        t.pulse_at = tf.pulse_at;
+        t.pulse_n = tf.pulse_n;
+        t.psi_n = tf.psi_n;
+        t.rec_idx = tf.rec_idx;
+        need(tf.pulse_n == 0 || tf.amp, "pulse table");
+        need(tf.psi_n == 0 || tf.seg_coef, "CPML coefficients");
+        for (int cc = 0; cc < 6; ++cc) {
+            t.present[cc] = tf.present[cc] ? 1 : 0;
+            t.rec_off[cc] = tf.rec_off[cc];
+            t.nterm[cc] = tf.nterm[cc];
+            t.nseg[cc] = tf.nseg[cc];
+            need(tf.nterm[cc] >= 0 && tf.nterm[cc] <= 2 && tf.nseg[cc] >= 0 && tf.nseg[cc] <= 4, "term or segment count");
+            if (!t.present[cc]) {
+                t.nterm[cc] = t.nseg[cc] = 0;
+                continue;
+            }
+            for (int s = 0; s < 2; ++s) {
+                const int len = t.len[s];
+                need(tf.pulse_at >= 0 && tf.pulse_at + tf.pulse_n <= len, "pulse window");
+                need(tf.rec_idx < tf.len1 && tf.rec_idx - tf.rec_off[cc] >= 0 && tf.rec_off[cc] >= 0, "record point");
+                for (int j = 0; j < tf.nterm[cc]; ++j) {
+                    const int lo = std::min(tf.term_pos[cc][j], tf.term_neg[cc][j]), hi = std::max(tf.term_pos[cc][j], tf.term_neg[cc][j]);
+                    need(tf.term_src[cc][j] >= 0 && tf.term_src[cc][j] < 6 && tf.present[tf.term_src[cc][j]], "term line");
+                    need(len - 2 * tf.margin <= 0 || (tf.margin + lo >= 0 && len - tf.margin - 1 + hi < len), "term offsets");
+                }
+                for (int g = 0; g < tf.nseg[cc]; ++g) {
+                    const int gl = tf.seg_high[cc][g] ? len - 1 - tf.psi_n - tf.margin : tf.margin;
+                    need(tf.seg_src[cc][g] >= 0 && tf.seg_src[cc][g] < 6 && tf.present[tf.seg_src[cc][g]], "segment line");
+                    need(gl >= 0 && gl + tf.psi_n <= len, "segment position");
+                    for (int off : {tf.seg_off1[cc][g], tf.seg_off2[cc][g]})
+                        need(gl + off >= 0 && gl + off + tf.psi_n <= len, "segment offsets");
+                }
+            }
+            for (int j = 0; j < 2; ++j) {
+                t.tsrc[cc][j] = tf.term_src[cc][j];
+                t.tpos[cc][j] = tf.term_pos[cc][j];
+                t.tneg[cc][j] = tf.term_neg[cc][j];
+                t.tcoef[cc][j] = tf.term_coef[cc][j];
+            }
+            for (int g = 0; g < 4; ++g) {
+                t.ssrc[cc][g] = tf.seg_src[cc][g];
+                t.shigh[cc][g] = tf.seg_high[cc][g];
+                t.soff1[cc][g] = tf.seg_off1[cc][g];
+                t.soff2[cc][g] = tf.seg_off2[cc][g];
+                t.sdb[cc][g] = tf.seg_db[cc][g];
+            }
+        }
+        t.line[0] = reinterpret_cast<double2*>(c->zeros<double>(2 * 6 * static_cast<size_t>(tf.len0)));
+        t.line[1] = reinterpret_cast<double2*>(c->zeros<double>(2 * 6 * static_cast<size_t>(tf.len1)));
+        t.psi = reinterpret_cast<double2*>(c->zeros<double>(2 * 2 * 6 * 4 * static_cast<size_t>(std::max(tf.psi_n, 1))));
+        t.scoef = c->upload<double>(tf.seg_coef, 6 * 4 * static_cast<size_t>(tf.psi_n) * 3);
+        t.amp = reinterpret_cast<const double2*>(c->upload<double>(tf.amp, 2 * static_cast<size_t>(p->nsteps) * 6 * tf.pulse_n));
+        t.rec = c->incd_rec;
+        t.nrec = c->nrec;
+        // surfaces: H adds first, then E adds, each with flat point offsets
+        need(tf.nsurf_h >= 0 && tf.nsurf_e >= 0 && (tf.nsurf_h + tf.nsurf_e == 0 || tf.surf), "surface table");
+        std::vector<TfsfSurfD> sv;
+        long long acc = 0;
+        const int nn[3] = {p->nx, p->ny, p->nz};
+        for (int i = 0; i < tf.nsurf_h + tf.nsurf_e; ++i) {
+            if (i == tf.nsurf_h) {
+                h.toth = acc;
+                acc = 0;
+            }
+            const chiml_tfsf_surface_t& u = tf.surf[i];
+            const bool hfam = i < tf.nsurf_h;
+            need(u.comp >= (hfam ? 3 : 0) && u.comp < (hfam ? 6 : 3) && a.F[u.comp] != nullptr, "surface component");
+            need(u.src >= 0 && u.src < 6 && tf.present[u.src], "surface line");
+            need(u.a1 >= 0 && u.a1 < 3 && u.a2 >= 0 && u.a2 < 3 && u.a1 != u.a2 && u.n1 >= 0 && u.n2 >= 0, "surface axes");
+            TfsfSurfD d{};
+            d.comp = u.comp;
+            d.src = u.src;
+            d.pf = u.pf;
+            d.o[0] = u.x0;
+            d.o[1] = u.y0;
+            d.o[2] = u.z0;
+            d.a1 = u.a1;
+            d.n1 = u.n1;
+            d.a2 = u.a2;
+            d.n2 = u.n2;
+            d.incd0 = u.incd0;
+            d.s1 = u.incd_s1;
+            d.s2 = u.incd_s2;
+            d.start = acc;
+            if (u.n1 > 0 && u.n2 > 0) {
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int ext = ax == u.a1 ? u.n1 - 1 : (ax == u.a2 ? u.n2 - 1 : 0);
+                    need(d.o[ax] >= 0 && d.o[ax] + ext < nn[ax], "surface cells");
+                }
+                for (int e1 : {0, u.n1 - 1})
+                    for (int e2 : {0, u.n2 - 1}) {
+                        const long long li = u.incd0 + static_cast<long long>(e1) * u.incd_s1 + static_cast<long long>(e2) * u.incd_s2;
+                        need(li >= 0 && li < tf.len0, "surface line index");
+                    }
+            }
+            acc += static_cast<long long>(u.n1) * u.n2;
+            sv.push_back(d);
+        }
+        if (tf.nsurf_e == 0) h.toth = acc;
+        else h.tote = acc;
+        h.nsh = tf.nsurf_h;
+        h.nse = tf.nsurf_e;
+        if (!sv.empty()) {
+            h.surf = c->upload<TfsfSurfD>(sv.data(), sv.size());
+        }
+        c->tfsf.push_back(h);
+    }
 }
 
 template <typename T>
@@ -2771,6 +3038,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         q.eps_node = epsn;
         c->qes.push_back(std::move(qh));
     }
+    build_tfsf(c, p, a);
 }
 
 template <typename T>
@@ -2852,6 +3120,22 @@ void pick2(bool s, bool adv, F&& f) {
 }
 
 // adv: the H launch advances the two-slot step counter (steps being captured)
+// The TFSF part of a step, between the H and E updates of the main grid:
+// H surface adds, the incident lines, E surface adds (parallelFDTDField.hpp:1236-1256).
+template <typename T>
+void launch_tfsf(chiml_dev* c, StepArgs<T>& a) {
+    for (const auto& h : c->tfsf) {
+        const TfsfLineArgs& t = h.line;
+        if (h.toth > 0)
+            hipLaunchKernelGGL(k_tfsf_add<T>, dim3(static_cast<unsigned>((h.toth + 255) / 256)), dim3(256), 0, c->stream, h.surf, h.nsh,
+                               h.toth, a.F[0], a.F[1], a.F[2], a.F[3], a.F[4], a.F[5], t.line[0], t.len[0], c->nx, c->ny);
+        hipLaunchKernelGGL(k_tfsf_line, dim3(1), dim3(1024), 0, c->stream, t, a.step, a.stepv);
+        if (h.tote > 0)
+            hipLaunchKernelGGL(k_tfsf_add<T>, dim3(static_cast<unsigned>((h.tote + 255) / 256)), dim3(256), 0, c->stream, h.surf + h.nsh,
+                               h.nse, h.tote, a.F[0], a.F[1], a.F[2], a.F[3], a.F[4], a.F[5], t.line[0], t.len[0], c->nx, c->ny);
+    }
+}
+
 template <typename T, int DIM>
 void launch_fields(chiml_dev* c, StepArgs<T>& a) {
     const bool adv = a.step_next != nullptr;
@@ -2899,6 +3183,7 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
                                        nib, nsbp, ade_bpc);
                 });
             }
+            launch_tfsf<T>(c, a);
             if (ge.x > 0) {
                 Timed t(c, "update_e");
                 if (c->has_src)
@@ -2920,6 +3205,7 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
                 if (c->has_src) hipLaunchKernelGGL((k_h3_slab<T, W, true>), gs, blk, 0, c->stream, a, c->slab);
                 else hipLaunchKernelGGL((k_h3_slab<T, W, false>), gs, blk, 0, c->stream, a, c->slab);
             }
+            launch_tfsf<T>(c, a);
             {
                 Timed t(c, "update_e");
                 if (c->has_src) hipLaunchKernelGGL((k_e3_int<T, W, true>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
@@ -2943,6 +3229,7 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
                 hipLaunchKernelGGL((k_h2v<T, W, decltype(S)::value, decltype(A)::value>), g, blk, 0, c->stream, a);
             });
         }
+        launch_tfsf<T>(c, a);
         {
             Timed t(c, "update_e");
             const SampleArgs<T>& sa = sample_args<T>(c);
@@ -2965,6 +3252,7 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
             else hipLaunchKernelGGL((k_update_h<T, DIM, false, decltype(S)::value, decltype(A)::value>), grd, blk, 0, c->stream, a);
         });
     }
+    launch_tfsf<T>(c, a);
     {
         Timed t(c, "update_e");
         if (c->has_src) hipLaunchKernelGGL((k_update_e<T, DIM, true>), grd, blk, 0, c->stream, a);
@@ -3122,7 +3410,7 @@ void* scratch_on(int dev) {
 // Kernel class of a context for batched launches: 1 = 2D W-cell kernels,
 // 2 = 3D split sweeps; 0 = not batched (stepped on its own stream).
 int batch_class(const chiml_dev* c) {
-    if (c->fused || c->capturing) return 0;
+    if (c->fused || c->capturing || !c->tfsf.empty()) return 0;  // TFSF steps stay on their own stream
     if (c->dim == 2 && c->vec2d) return 1;
     if (c->dim == 3 && c->split) return 2;
     return 0;
@@ -3360,6 +3648,7 @@ void enable_fused(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, Sample
     constexpr int W = 16 / sizeof(T);
     const char* on = std::getenv("CHIML_FUSED");
     if (!on || std::atoi(on) != 1) return;
+    if (!c->tfsf.empty()) return;  // the TFSF adds sit between the H and E updates
     const int nx = p->nx, ny = p->ny, nz = p->nz;
     if (nx % W != 0 || p->pml_T[0] % W != 0 || c->hmap) return;
     FuseMap m{};
@@ -3698,6 +3987,26 @@ int chiml_dev_read_dft_group(chiml_dev_t* c, int g, double* reim, size_t cap) {
 }
 
 int chiml_dev_read_dft(chiml_dev_t* c, double* reim, size_t cap) { return chiml_dev_read_dft_group(c, 0, reim, cap); }
+
+int chiml_dev_read_incd(chiml_dev_t* c, int comp, double* reim, size_t cap, size_t* n) {
+    if (!c || comp < 0 || comp > 5 || !n) return fail(c, CHIML_E_INVALID, "invalid argument");
+    const long long nrec = c->incd_rec ? c->nrec : 2LL * (c->nsteps + 1) + 2;
+    *n = static_cast<size_t>(nrec);
+    if (!reim) return CHIML_OK;
+    if (cap < 2 * static_cast<size_t>(nrec)) return fail(c, CHIML_E_INVALID, "buffer too small");
+    if (!c->incd_rec) {
+        std::fill(reim, reim + 2 * nrec, 0.0);
+        return CHIML_OK;
+    }
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    try {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipMemcpy(reim, c->incd_rec + static_cast<size_t>(comp) * nrec, 2 * nrec * sizeof(double), hipMemcpyDeviceToHost));
+    } catch (std::exception& e) {
+        return fail(c, CHIML_E_HIP, e.what());
+    }
+    return CHIML_OK;
+}
 
 int chiml_dev_read_pops(chiml_dev_t* c, int q, int j, double* reim, size_t cap, size_t* nsamples) {
     if (!c || q < 0 || q >= static_cast<int>(c->qes.size()) || !nsamples) return fail(c, CHIML_E_INVALID, "invalid argument");

@@ -184,6 +184,341 @@ struct FreqDet {
 
 void check_pml_axes(const ProgramInputs& IP);
 
+// ---- TFSF surfaces -----------------------------------------------------------
+// Host tables of the TFSF surfaces (parallelTFSFBase's constructor,
+// src/SOURCE/parallelTFSF.hpp:268-467, and its incident CPMLs, src/PML/PML.hpp:104-257)
+// for an incident medium of vacuum: every far-side plane that genEpMu reads
+// (parallelTFSF.hpp:572-686) must hold the background (object 0) at the
+// component's own Yee samples, and no E surface cell may be a D cell.
+struct TfsfTables {
+    std::vector<chiml_tfsf_t> t;
+    std::vector<std::vector<double>> amp, coef;
+    std::vector<std::vector<chiml_tfsf_surface_t>> surf;
+};
+
+// phys(c, x, y, z): the reference's object map of component c at global point
+// (x, y, z): -1 on the ghost border and on the last row of a staggered axis
+// (setupPhysFields, parallelFDTDField.hpp:867-947), else the object id.
+TfsfTables buildTfsfTables(const ProgramInputs& IP, const std::array<int, 3>& n, bool is3D,
+                           const std::function<int(int, int, int, int)>& phys, const std::function<bool(int, int, int, int)>& dcell) {
+    TfsfTables out;
+    const double dt = IP.dt;
+    const int nsteps = IP.nSteps();
+    const auto& d = IP.d;
+    const double PI = kPi;
+    const bool present[6] = {is3D, is3D, true, true, true, is3D};
+    const int lv[3] = {n[0] + 2, n[1] + 2, is3D ? n[2] + 2 : 1};  // n_vec / ln_vec of one rank
+    for (const TfsfSpec& sp : IP.tfsfs) {
+        const auto& m = sp.m;
+        const auto& sz = sp.sz;
+        const auto& loc = sp.loc;
+        if (m[0] < 0 || m[1] < 0 || m[2] < 0) throw std::logic_error("TFSF surfaces with a negative m component are not supported");
+        const int mTot = m[0] + m[1] + m[2];
+        const int mMax = std::max(m[0], std::max(m[1], m[2]));
+        const int P = sp.pmlThick * mTot;
+        const int len0 = 2 * mMax + mTot * (sz[0] + sz[1] + sz[2] + 10 + 2 * sp.pmlThick);
+        const int len1 = 2 * P + 20 * mTot;
+        int mInd = -1;
+        if ((m[0] == 0 && m[1] == 0) || (m[2] == 0 && m[1] == 0) || (m[0] == 0 && m[2] == 0))
+            for (int a = 0; a < 3; ++a)
+                if (m[a] != 0) mInd = a;
+        if ((is3D && ((sz[0] <= 1 && sz[1] <= 1) || (sz[0] <= 1 && sz[2] <= 1) || (sz[1] <= 1 && sz[2] <= 1))) ||
+            (!is3D && sz[0] <= 1 && sz[1] <= 1 && lv[0] != 3 && lv[1] != 3))
+            throw std::logic_error("TFSF sources are not point/line sources sources");
+        if (mTot == 0) throw std::logic_error("The m array must have at least one non-zero number");
+        // grid step along the propagation line: d_a cos/sin terms / m_a, equal on every axis with m_a != 0
+        const double drA[3] = {d[0] * std::cos(sp.phi) * std::sin(sp.theta) / static_cast<double>(m[0]),
+                               d[1] * std::sin(sp.phi) * std::sin(sp.theta) / static_cast<double>(m[1]),
+                               d[2] * std::cos(sp.theta) / static_cast<double>(m[2])};
+        double dr = -1;
+        for (int a = 0; a < 3; ++a)
+            if (m[a] != 0) dr = drA[a];
+        for (int a = 0; a < 3; ++a)
+            if (m[a] != 0 && std::abs(drA[a] - dr) > 1e-13) throw std::logic_error("p[ii]*d[ii]/m[ii] must equal dr for all ii in x, y, z");
+        // origin corner by the quadrant of phi (and the sign of cos theta)
+        std::array<int, 3> org = loc;
+        int quad = 0;
+        if (sp.phi <= PI / 2.0 && sp.phi > 0) quad = 1;
+        else if (sp.phi <= PI && sp.phi > PI / 2.0) {
+            quad = 2;
+            org[0] += sz[0] - 1;
+        } else if (sp.phi <= 3 * PI / 2.0 && sp.phi > PI) {
+            quad = 3;
+            org[1] += sz[1] - 1;
+            org[0] += sz[0] - 1;
+        } else if ((sp.phi <= 2 * PI && sp.phi > 3.0 * PI / 2.0) || sp.phi == 0) {
+            quad = 4;
+            org[1] += sz[1] - 1;
+        }
+        if (sp.theta > PI / 2.0 || sp.theta < 0.0) {
+            quad *= -1;
+            org[2] += sz[2] - 1;
+        }
+        if ((sz[1] == 1 && m[0] != 0 && m[2] != 0) || (sz[0] == 1 && m[2] != 0 && m[1] != 0) || (sz[2] == 1 && is3D && m[0] != 0 && m[1] != 0))
+            throw std::logic_error("Angle of propgation has a component along the TFSF surface without any other surfaces to cancel it.");
+        const double phiP = (sp.theta == 0 || sp.theta == PI) ? PI / 2.0 : sp.phi;
+        double psiP = sp.psi, alpha = 0.0;
+        if (sp.circ != 0) {  // elliptical light: polarisation angle and phase of the two axes
+            const double cr = std::pow(sp.kRatio, 2.0);
+            const double s2 = std::sin(2 * sp.psi), c2 = std::cos(2 * sp.psi);
+            psiP = 0.5 * std::asin(std::sqrt((std::pow(c2, 2) * 4 * cr + std::pow((1 + cr) * s2, 2.0)) / std::pow(1.0 + cr, 2.0)));
+            alpha = std::acos(((cr - 1.0) * std::sin(2 * sp.psi)) /
+                              std::sqrt(std::pow(std::cos(2.0 * sp.psi), 2.0) * 4.0 * cr + std::pow((1.0 + cr) * std::sin(2.0 * sp.psi), 2.0)));
+            if (std::abs(std::tan(sp.psi)) > 1) psiP = PI / 2.0 - psiP;
+            if (sp.circ > 0) alpha *= -1.0;
+        }
+        // incident medium: the far-side planes of the box, per component
+        for (int c = 0; c < 6; ++c) {
+            if (!present[c]) continue;
+            bool defined = false;
+            for (int ii = is3D ? 1 : 3; ii < 6; ii += 2) {  // far planes z, y, x (the near ones are never read)
+                const int sI = 2 - ii / 2;
+                const int q0 = (sI != 2) ? (sI + 2) % 3 : 0, q1 = (sI != 2) ? (sI + 1) % 3 : 1;
+                const int pl0 = loc[q0], pl1 = loc[q1], ps0 = sz[q0], ps1 = sz[q1];
+                const int f0 = lv[q0], f1 = lv[q1];
+                if ((ps0 <= 1 && (is3D || (ps1 <= 1 && (f0 > 3 || f1 > 3)))) || (ps1 <= 1 && (is3D || (ps0 <= 1 && (f0 > 3 || f1 > 3)))))
+                    continue;
+                const int at = loc[sI] + sz[sI];
+                std::vector<std::array<int, 3>> pts;  // the plane points read (planes normal to x and z clip y)
+                if (sI == 1) {
+                    for (int b = 0; b < ps1; ++b)
+                        for (int a = 0; a < ps0; ++a) pts.push_back({{pl0 + a, at, pl1 + b}});
+                } else {
+                    int ySz = 0, y0 = 0;
+                    const int ny = lv[1] - 2;
+                    if (pl1 > 0 && pl1 <= ny) {
+                        ySz = (pl1 + ps1 > ny) ? ny - pl1 : ps1;
+                        y0 = pl1;
+                    } else if (pl1 <= 0 && pl1 + ps1 > 0) {
+                        ySz = (ny > pl1 + ps1) ? ps1 + pl1 : ny;
+                        y0 = 0;
+                    }
+                    for (int b = 0; b < ySz; ++b)
+                        for (int a = 0; a < ps0; ++a)
+                            pts.push_back(sI == 0 ? std::array<int, 3>{{at, y0 + b, pl0 + a}} : std::array<int, 3>{{pl0 + a, y0 + b, at}});
+                }
+                for (const auto& g : pts)
+                    if (phys(c, g[0], g[1], is3D ? g[2] : 0) != 0)
+                        throw std::logic_error("a TFSF surface whose far-side planes hold an object or the grid edge (layered incident medium) is not supported");
+                if (sI != mInd || sz[0] == 1 || sz[1] == 1 || sz[2] == 1) defined = true;
+            }
+            if (!defined) throw std::logic_error("no TFSF surface plane defines the incident medium");
+        }
+        chiml_tfsf_t t;
+        std::memset(&t, 0, sizeof(t));
+        t.len0 = len0;
+        t.len1 = len1;
+        t.margin = mTot;
+        t.pulse_at = P + 3 * mTot;
+        t.pulse_n = mTot;
+        t.psi_n = P;
+        t.rec_idx = P + mMax + 5 * mTot;
+        // polarisation prefactors (the incident eps and mu are 1)
+        const cplx psiJ = std::cos(psiP), psiK = std::sin(psiP) * std::exp(cplx(0.0, alpha));
+        const double ct = std::cos(sp.theta), st = std::sin(sp.theta), cp = std::cos(phiP), sp_ = std::sin(phiP);
+        cplx pref[6];
+        pref[3] = (psiK * sp_ + psiJ * ct * cp) * dt / (mTot * dr);
+        pref[4] = (-1.0 * psiK * cp + psiJ * ct * sp_) * dt / (mTot * dr);
+        pref[5] = (-1.0 * psiJ * st) * dt / (mTot * dr);
+        pref[0] = (psiJ * sp_ - psiK * ct * cp) * dt / (mTot * dr);
+        pref[1] = (-1.0 * psiJ * cp - psiK * ct * sp_) * dt / (mTot * dr);
+        pref[2] = (psiK * st) * dt / (mTot * dr);
+        std::vector<double> amp(2 * static_cast<size_t>(nsteps) * 6 * mTot, 0.0);
+        std::vector<double> coef(6 * 4 * static_cast<size_t>(P) * 3, 0.0);
+        for (int c = 0; c < 6; ++c) {
+            if (!present[c]) continue;
+            t.present[c] = 1;
+            const bool E = c < 3;
+            const int i = c % 3, j = (i + 1) % 3, k = (i + 2) % 3;
+            const int opp = E ? 3 : 0;  // the other family's line index offset
+            const int distBase = E ? m[i] : m[j] + m[k];
+            t.rec_off[c] = distBase % 2;
+            // derivative terms (updateIncdFieldJ / K / JK), J first
+            int nt = 0;
+            auto term = [&](int src, int pos, int neg, double cf) {
+                t.term_src[c][nt] = src;
+                t.term_pos[c][nt] = pos;
+                t.term_neg[c][nt] = neg;
+                t.term_coef[c][nt] = cf;
+                ++nt;
+            };
+            if (present[opp + j]) term(opp + j, E ? -1 * (m[k] + m[k] % 2) / 2 : (m[k] + m[k] % 2) / 2, E ? m[k] / 2 : -1 * m[k] / 2, dt / d[k]);
+            if (present[opp + k]) term(opp + k, E ? m[j] / 2 : -1 * m[j] / 2, E ? -1 * (m[j] + m[j] % 2) / 2 : (m[j] + m[j] % 2) / 2, dt / d[j]);
+            t.nterm[c] = nt;
+            // pulse: prefactor times the sum of pulses at t_step dt - dist
+            for (int s = 0; s < nsteps; ++s)
+                for (int r = 0; r < mTot; ++r) {
+                    const double dist = static_cast<double>(distBase + 2 * r) / 2.0 * dr;
+                    cplx pv(0.0, 0.0);
+                    for (const auto& pul : sp.pulses) pv = pv + pulseValue(pul, s * dt - dist);
+                    const cplx v = pref[c] * pv;
+                    const size_t at = ((static_cast<size_t>(s) * 6 + c) * mTot + r) * 2;
+                    amp[at] = v.real();
+                    amp[at + 1] = v.imag();
+                }
+            // CPML segments: derivative along j (reads line k), then along k (reads line j); high end first
+            const double mOffHalf = (E ? m[i] % 2 : m[j] % 2 + m[k] % 2) / 2.0;
+            int ns = 0;
+            auto seg = [&](int dir, int src, bool high) {
+                const int mc = m[dir];
+                const bool dirJ = (dir == j);
+                int o1, o2;
+                if (dirJ ^ !E) {
+                    o1 = E ? mc / 2 : -1 * (mc / 2);
+                    o2 = E ? -1 * (mc + mc % 2) / 2 : (mc + mc % 2) / 2;
+                } else {
+                    o1 = E ? -1 * (mc + mc % 2) / 2 : (mc + mc % 2) / 2;
+                    o2 = E ? mc / 2 : -1 * (mc / 2);
+                }
+                t.seg_src[c][ns] = src;
+                t.seg_high[c][ns] = high ? 1 : 0;
+                t.seg_off1[c][ns] = o1;
+                t.seg_off2[c][ns] = o2;
+                t.seg_db[c][ns] = dt / (d[dir] * 1.0);
+                const double iiMax = static_cast<double>(P - 1);
+                const double sMax = 0.8 * (sp.pmlM + 1) / std::abs(dr) / std::sqrt(1.0 * 1.0);
+                for (int dd = 0; dd < P; ++dd) {
+                    const double x = high ? P - 1 - (static_cast<double>(dd) + mOffHalf) : static_cast<double>(dd) + mOffHalf;
+                    const bool in = 0.0 <= x && x <= iiMax;
+                    const double sg = in ? sMax * std::pow((iiMax - x) / iiMax, sp.pmlM) : 0.0;
+                    const double kp = in ? 1.0 + (1.0 - 1.0) * std::pow((iiMax - x) / iiMax, sp.pmlM) : 1.0;
+                    const double av = in ? sp.pmlAMax * std::pow(x / iiMax, sp.pmlMa) : 0.0;
+                    const double bv = std::exp(-1.0 * (sg / kp + av) * dt);
+                    const double cv = (sg == 0 && av == 0) ? 0 : sg / (sg * kp + std::pow(kp, 2.0) * av) * (std::exp(-1.0 * (sg / kp + av) * dt) - 1.0);
+                    double* q = &coef[((static_cast<size_t>(c) * 4 + ns) * P + dd) * 3];
+                    q[0] = bv;
+                    q[1] = E ? cv : -1.0 * cv;
+                    q[2] = E ? -1.0 * cv : cv;
+                }
+                ++ns;
+            };
+            const bool haveK = present[opp + k], haveJ = present[opp + j];
+            if (haveK && m[j] != 0) {
+                if (!(is3D || j != 2)) throw std::logic_error("incident CPML list without its psi field");
+                seg(j, opp + k, true);
+                seg(j, opp + k, false);
+            }
+            if (haveJ && m[k] != 0) {
+                if (!(is3D || k != 2)) throw std::logic_error("incident CPML list without its psi field");
+                seg(k, opp + j, true);
+                seg(k, opp + j, false);
+            }
+            t.nseg[c] = ns;
+        }
+        // surfaces (initializeIncdAddStructs / genSurface, parallelTFSF.hpp:739-1000)
+        const bool pl = (sp.theta == 0 || ((sp.theta == PI / 2.0 && std::abs(quad) == 4) || std::abs(quad) == 1)) ? false : true;
+        std::vector<chiml_tfsf_surface_t> sv;
+        // the box face with normal dir, on the high side if high, of component c read from incident line src
+        auto surface = [&](int c, int src, int dir, bool high) {
+            const bool E = c < 3;
+            // a main component sits at half-integer positions along the axes its stagger names
+            const int* stg = kStag[c];
+            std::array<int, 3> fe{{stg[0], stg[1], is3D ? stg[2] : 0}};
+            double off = 0, half = 0;
+            for (int a = 0; a < 3; ++a)
+                if (kStag[src][a]) {
+                    off += static_cast<double>(m[a]) / 2.0;
+                    half += static_cast<double>(m[a] % 2) / 2.0;
+                }
+            // the axes a1 (points) and a2 (rows) of the face
+            int t1, t2;
+            if (dir == 0) {
+                t1 = (is3D && mInd != 2) ? 2 : 1;
+                t2 = 3 - 0 - t1;
+            } else if (dir == 1) {
+                t1 = (mInd != 0) ? 0 : 2;
+                t2 = 3 - 1 - t1;
+            } else {
+                t1 = (mInd != 0) ? 0 : 1;
+                t2 = 3 - 2 - t1;
+            }
+            // sign: the face's curl term is the j or the k term of component c
+            const bool isJ = !((dir == 0 && (c == 4 || c == 1)) || (dir == 1 && (c == 5 || c == 2)) || (dir == 2 && (c == 3 || c == 0)));
+            const double pf = ((E ^ isJ ^ high) ? 1.0 : -1.0) * dt / d[dir];
+            std::array<int, 3> lt = loc, szT = sz;
+            lt[dir] = high ? loc[dir] + sz[dir] - 1 : (E ? loc[dir] : loc[dir] - 1);
+            szT[dir] = 1;
+            std::array<int, 3> sl{{-1, -1, -1}};  // 1-based first point inside the grid, -1: none
+            for (int a = 0; a < 3; ++a) {
+                if (lt[a] >= 0 && lt[a] < lv[a] - 2) sl[a] = lt[a] + 1;
+                else if (lt[a] < 0 && lt[a] + szT[a] > 0) sl[a] = 1;
+            }
+            if (!is3D) sl[2] = 0;
+            if (sl[0] == -1 || sl[1] == -1 || sl[2] == -1) return;
+            auto extent = [&](int a) { return (szT[a] + loc[a] > lv[a] - 2) ? lv[a] - 2 - sl[a] + 1 : loc[a] + szT[a] - (sl[a] - 1) - fe[a]; };
+            const int n1 = extent(t1);
+            const int n2 = is3D ? extent(t2) : 1;
+            std::array<int, 3> rel = sl;
+            if (!high && !E) rel[dir] += 1;
+            for (int a = 0; a < 3; ++a) rel[a] = (rel[a] - 1) - org[a];
+            if (E && m[dir] > 0 && rel[dir] == 0) off -= m[dir];
+            const int mOff = static_cast<int>(std::floor(off - half));
+            chiml_tfsf_surface_t u;
+            u.comp = c;
+            u.src = src;
+            u.pf = pf;
+            u.x0 = sl[0] - 1;
+            u.y0 = sl[1] - 1;
+            u.z0 = is3D ? sl[2] - 1 : 0;
+            u.a1 = t1;
+            u.n1 = n1;
+            u.a2 = t2;
+            u.n2 = n2;
+            u.incd0 = std::abs(m[0] * rel[0]) + std::abs(m[1] * rel[1]) + std::abs(m[2] * rel[2]) + P + mOff + 6 * mTot;
+            u.incd_s1 = m[t1];
+            u.incd_s2 = m[t2];
+            if (E && n1 > 0 && n2 > 0)
+                for (int b = 0; b < n2; ++b)
+                    for (int a = 0; a < n1; ++a) {
+                        int x[3] = {u.x0, u.y0, u.z0};
+                        x[t1] += a;
+                        x[t2] += b;
+                        if (dcell(c, x[0], x[1], x[2]))
+                            throw std::logic_error("a TFSF surface on a dispersive, emitter or material-in-PML cell is not supported");
+                    }
+            sv.push_back(u);
+        };
+        int nh = 0;
+        for (int fam = 0; fam < 2; ++fam) {  // H components, then E components
+            for (int i = 0; i < 3; ++i) {
+                const int c = fam == 0 ? 3 + i : i;
+                if (!present[c]) continue;
+                const bool E = c < 3;
+                const int j = (i + 1) % 3, k = (i + 2) % 3;
+                const int opp = E ? 3 : 0;
+                const size_t before = sv.size();
+                if (sz[i] > 1 || lv[i] <= 3) {
+                    if (sz[j] > 1 || !is3D) {
+                        if (present[opp + j]) {
+                            surface(c, opp + j, k, pl);
+                            if (sz[k] > 1) surface(c, opp + j, k, !pl);
+                        }
+                    }
+                    if (sz[k] > 1 || !is3D) {
+                        if (present[opp + k]) {
+                            surface(c, opp + k, j, pl);
+                            if (sz[j] > 1) surface(c, opp + k, j, !pl);
+                        }
+                    }
+                }
+                if (fam == 0) nh += static_cast<int>(sv.size() - before);
+            }
+        }
+        t.nsurf_h = nh;
+        t.nsurf_e = static_cast<int>(sv.size()) - nh;
+        out.t.push_back(t);
+        out.amp.push_back(std::move(amp));
+        out.coef.push_back(std::move(coef));
+        out.surf.push_back(std::move(sv));
+    }
+    for (size_t k = 0; k < out.t.size(); ++k) {
+        out.t[k].amp = out.amp[k].data();
+        out.t[k].seg_coef = out.coef[k].data();
+        out.t[k].surf = out.surf[k].data();
+    }
+    return out;
+}
+
 class FDTDField {
   public:
     explicit FDTDField(const std::string& deck);
@@ -203,11 +538,14 @@ class FDTDField {
     std::vector<double> detTConv;
     std::vector<int> detInterval;
     std::vector<int> qeN;
+    int ntfsf = 0;
     double cells = 0, bytesH = 0, bytesE = 0, bytesML = 0;
     std::map<std::string, double> kbytes;  // algorithmic HBM bytes per launch
     size_t estimate = 0;
 
     std::vector<cplx> flux(int f);
+    // incident flux per frequency from the TFSF incident records (getFlux's incident column)
+    std::vector<cplx> fluxIncd(int f);
     std::vector<std::vector<cplx>> fluxFaces(int f);
     // per face the combined Ej, Ek, Hj, Hk [freq][transCor1][cor] (empty where absent)
     std::vector<std::array<std::vector<cplx>, 4>> fluxFields(int f);
@@ -642,6 +980,22 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     P.src_comp = srcComp.data();
     P.src_box = srcBox.data();
     P.src_amp = srcAmp.data();
+
+    // ---- TFSF surfaces ---------------------------------------------------------
+    auto physAt = [&](int c, int x, int y, int z) -> int {
+        if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return -1;  // ghost border
+        const int* st = kStag[c];
+        if ((st[0] && x == nx - 1 && nx + 2 > 3) || (st[1] && y == ny - 1 && ny + 2 > 3) || (is3D && st[2] && z == nz - 1)) return -1;
+        return physOf[c][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)];
+    };
+    auto dcellAt = [&](int c, int x, int y, int z) -> bool {
+        const size_t i = x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z);
+        return (flT[c][matE[c][i]] & CHIML_MAT_DCELL) != 0;
+    };
+    TfsfTables tfsfTab = buildTfsfTables(IP, n, is3D, physAt, dcellAt);
+    P.ntfsf = static_cast<int>(tfsfTab.t.size());
+    P.tfsf = tfsfTab.t.empty() ? nullptr : tfsfTab.t.data();
+    ntfsf = P.ntfsf;
 
     // ---- point detectors -----------------------------------------------------
     std::vector<int> dInt, dNp, dComp, dPow, dNf;
@@ -1296,6 +1650,70 @@ std::vector<cplx> FDTDField::flux(int fi) {
     return total;
 }
 
+// Incident flux of box fi (getFlux, parallelFlux.hpp:455-483): every
+// incident record (the point and its offset at each step) halved, its real
+// part phased at the record's step, summed per component; then the three
+// axis cross products E_inc x conj(H_inc) over (nt timeInt)^2, each scaled by
+// fluxIncdConv_ (the SI factor and, without cross_sec, the Simpson area of
+// the box faces normal to that axis, parallelFlux.hpp:147-173), squared,
+// summed and square-rooted.
+std::vector<cplx> FDTDField::fluxIncd(int fi) {
+    FluxBox& fb = fluxes.at(fi);
+    const FluxSpec& F = IP.fluxes.at(fi);
+    const int nf = static_cast<int>(fb.freqs.size());
+    std::vector<cplx> out(nf, cplx(0.0, 0.0));
+    std::vector<std::vector<cplx>> rec(6);
+    for (int c = 0; c < 6; ++c) {
+        size_t n = 0;
+        if (chiml_dev_read_incd(dev_, c, nullptr, 0, &n) != CHIML_OK) throw std::runtime_error(chiml_dev_last_error(dev_));
+        std::vector<double> b(2 * n);
+        if (chiml_dev_read_incd(dev_, c, b.data(), b.size(), &n) != CHIML_OK) throw std::runtime_error(chiml_dev_last_error(dev_));
+        rec[c].resize(n);
+        for (size_t i = 0; i < n; ++i) rec[c][i] = cplx(b[2 * i], b[2 * i + 1]);
+    }
+    double conv[3] = {1.0, 1.0, 1.0};
+    if (F.SI) conv[0] = conv[1] = conv[2] = IP.I0 / IP.a * IP.I0 / (EPS0 * IP.a * SPEED_OF_LIGHT);
+    if (!F.crossSec) {
+        const auto& sz = fb.sz;
+        const auto& d = IP.d;
+        auto line = [&](int n1, double d1) {
+            std::vector<cplx> ones(n1, cplx(1.0, 0.0));
+            return simpson(ones.data(), n1, d1);
+        };
+        auto area = [&](int n1, int n2, double d1, double d2) {
+            std::vector<cplx> r(n2, line(n1, d1));
+            return std::real(simpson(r.data(), n2, d2));
+        };
+        if (is3D) {
+            conv[0] *= (sz[2] > 1 && sz[1] > 1) ? area(sz[1], sz[2], d[1], d[2]) : 0.0;
+            conv[1] *= (sz[0] > 1 && sz[2] > 1) ? area(sz[0], sz[2], d[0], d[2]) : 0.0;
+            conv[2] *= (sz[0] > 1 && sz[1] > 1) ? area(sz[0], sz[1], d[0], d[1]) : 0.0;
+        } else {
+            conv[1] *= std::real(line(sz[0], d[0]));
+            conv[0] *= std::real(line(sz[1], d[1]));
+            conv[2] = 0.0;
+        }
+    }
+    int steps = 0;
+    chiml_dev_steps_done(dev_, &steps);
+    const int nt = steps / fb.timeInt + 1;
+    const double den = std::pow(static_cast<double>(nt * fb.timeInt), 2.0);
+    const size_t nrec = rec[0].size();
+    for (int k = 0; k < nf; ++k) {
+        cplx inc[6];
+        for (int par = 0; par < 2; ++par)  // the points, then their offsets
+            for (size_t tt = par; tt < nrec; tt += 2) {
+                const cplx ph = std::exp(cplx(0.0, fb.freqs[k] * static_cast<double>(tt / 2) * (IP.dt / static_cast<double>(fb.timeInt))));
+                for (int c = 0; c < 6; ++c) inc[c] += cplx(std::real(0.5 * rec[c][tt]), 0.0) * ph;
+            }
+        cplx v = std::pow(conv[0] * (inc[1] * std::conj(inc[5]) - inc[2] * std::conj(inc[4])) / den, 2.0);
+        v += std::pow(conv[1] * (inc[2] * std::conj(inc[3]) - inc[0] * std::conj(inc[5])) / den, 2.0);
+        v += std::pow(conv[2] * (inc[0] * std::conj(inc[4]) - inc[1] * std::conj(inc[3])) / den, 2.0);
+        out[k] = std::sqrt(v);
+    }
+    return out;
+}
+
 // Per face and frequency: tempFlux = fluxConv_ * weight_ * simps2D(...).
 // Per face the combined frequency fields (combineField, parallelFlux.hpp:325-395):
 // each stored copy of a component averaged onto the face's points with weight
@@ -1661,6 +2079,31 @@ int chiml_sim_flux(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq) 
     }
 }
 
+int chiml_sim_flux_incd(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq) {
+    if (!s || !nfreq) return sim_fail("null argument");
+    try {
+        if (f < 0 || f >= static_cast<int>(s->f->fluxes.size())) return sim_fail("flux index out of range");
+        *nfreq = static_cast<int>(s->f->fluxes[f].freqs.size());
+        if (!reim) return CHIML_OK;
+        if (cap < 2 * static_cast<size_t>(*nfreq)) return sim_fail("buffer too small");
+        auto v = s->f->fluxIncd(f);
+        for (size_t k = 0; k < v.size(); ++k) {
+            reim[2 * k] = v[k].real();
+            reim[2 * k + 1] = v[k].imag();
+        }
+    } catch (std::exception& e) {
+        return sim_fail(e.what(), CHIML_E_HIP);
+    }
+    return CHIML_OK;
+}
+
+int chiml_sim_incd(chiml_sim_t* s, int comp, double* reim, size_t cap, size_t* n) {
+    if (!s) return sim_fail("null simulation");
+    int rc = chiml_dev_read_incd(s->f->dev_, comp, reim, cap, n);
+    if (rc) chiml::g_sim_err = chiml_dev_last_error(s->f->dev_);
+    return rc;
+}
+
 int chiml_sim_pops(chiml_sim_t* s, int q, int j, double* reim, size_t cap, size_t* ns) {
     if (!s) return sim_fail("null simulation");
     int rc = chiml_dev_read_pops(s->f->dev_, q, j, reim, cap, ns);
@@ -1747,8 +2190,15 @@ int chiml_sim_write_outputs(chiml_sim_t* s, const char* prefix) {
                     reim.push_back(c.imag());
                 }
             const std::string path = place(fb.name + ".dat");
-            int rc = chiml_write_flux(path.c_str(), freqs.data(), static_cast<int>(freqs.size()),
-                                      static_cast<int>(faces.size()), reim.data(), fb.frontBack ? 1 : 0);
+            std::vector<double> incd;
+            if (F.ntfsf > 0)
+                for (const chiml::cplx& v : F.fluxIncd(static_cast<int>(f))) {
+                    incd.push_back(v.real());
+                    incd.push_back(v.imag());
+                }
+            int rc = chiml_write_flux_incd(path.c_str(), freqs.data(), static_cast<int>(freqs.size()),
+                                           static_cast<int>(faces.size()), reim.data(), fb.frontBack ? 1 : 0,
+                                           incd.empty() ? nullptr : incd.data());
             if (rc) return rc;
             if (fb.save) F.saveFluxFields(static_cast<int>(f), place(fb.name + "_fields.dat"));  // getFlux's saveFields
         }

@@ -376,6 +376,63 @@ static std::vector<cplx> readCplxMatrix(const Value& v, int n, double scale) {
     return m;
 }
 
+// A PulseList (parallelInputs.cpp:126-196 for sources, :338-392 for TFSF
+// surfaces).  The peak field Emax is appended last: real for a soft source
+// (parallelFDTDField.cpp:474), imaginary for a TFSF surface
+// (parallelFDTDField.cpp:647), whose main-grid add keeps the real part.
+static std::vector<PulseSpec> parsePulses(const Value& PL, const ProgramInputs& P, bool tfsf) {
+    std::vector<PulseSpec> out;
+    for (size_t pp = 0; pp < PL.size(); ++pp) {
+        const Value& pul = PL.child(pp);
+        PulseSpec ps;
+        ps.shape = string2prof(json::get_str(pul, "profile"));
+        double Emax = json::get_num(pul, "Field_Intensity", 1.0) * P.a * EPS0 * SPEED_OF_LIGHT / P.I0;
+        std::vector<cplx> f;
+        switch (ps.shape) {
+            case PulseShape::GAUSSIAN:
+                f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                f.push_back(1.0 / json::get_num(pul, "fwidth"));
+                f.push_back(json::get_num(pul, "cutoff") * f[1]);
+                f.push_back(json::get_num(pul, "t_0", std::real(f[1] * f[2])));
+                f[2] += f[3];
+                break;
+            case PulseShape::BH:
+                f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                f.push_back(json::get_num(pul, "tau", 1.0 / json::get_num(pul, "fwidth")));
+                // The reference's t_0 default reads past the end of its
+                // parameter list (parallelInputs.cpp:145), so t_0 is required.
+                f.push_back(json::get_num(pul, "t_0"));
+                f.push_back(json::get_num(pul, "BH1", 0.35875));
+                f.push_back(json::get_num(pul, "BH2", 0.48829));
+                f.push_back(json::get_num(pul, "BH3", 0.14128));
+                f.push_back(json::get_num(pul, "BH4", 0.01168));
+                break;
+            case PulseShape::RECT:
+                f.push_back(json::get_num(pul, "tau"));
+                f.push_back(json::get_num(pul, "t_0"));
+                f.push_back(json::get_num(pul, "n", 30) * 2.0);
+                f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                break;
+            case PulseShape::CONTINUOUS:
+                f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                break;
+            case PulseShape::RAMP_CONT:
+                f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                f.push_back(json::get_num(pul, "ramp_val"));
+                break;
+            case PulseShape::RICKER:
+                f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
+                f.push_back(json::get_num(pul, "fwidth"));
+                f.push_back(json::get_num(pul, "cutoff"));
+                break;
+        }
+        f.push_back(tfsf ? cplx(0.0, Emax) : cplx(Emax));
+        ps.param = f;
+        out.push_back(ps);
+    }
+    return out;
+}
+
 ProgramInputs parseInputs(const std::string& text) {
     Value IP = json::parse(text);
     ProgramInputs P;
@@ -420,8 +477,66 @@ ProgramInputs parseInputs(const std::string& text) {
         if (tmp[i] * 2 > P.size[i]) throw std::logic_error("PML size is larger than the cell size, this will lead to infinte fields.");
         P.pmlThickness[i] = find_pt(tmp[i], P.d[i]);
     }
-    if (json::get_child_or_empty(IP, "TFSF").size() > 0)
-        throw std::logic_error("TFSF surfaces are not supported on this path; use a soft source sheet");
+    // TFSF surfaces, parallelInputs.cpp:219-408.
+    const Value& TL = json::get_child_or_empty(IP, "TFSF");
+    for (size_t ii = 0; ii < TL.size(); ++ii) {
+        const Value& it = TL.child(ii);
+        TfsfSpec t;
+        double tsz[3], tloc[3];
+        json::get_arr3(it, "size", tsz);
+        for (int i = 0; i < 3; ++i) t.sz[i] = find_pt(tsz[i], P.d[i]) + 1;
+        const bool anyOne = t.sz[0] == 1 || t.sz[1] == 1 || t.sz[2] == 1;
+        if ((P.size[2] == 0 && t.sz[0] != 1 && t.sz[1] != 1) || (P.size[2] != 0.0 && !anyOne))
+            for (auto& v : t.sz) v -= (v - 1) % 2;  // odd sizes
+        json::get_arr3(it, "loc", tloc);
+        for (int i = 0; i < 3; ++i)
+            t.loc[i] = (P.size[i] != 0.0) ? find_pt(tloc[i] + P.size[i] / 2.0, P.d[i]) - (t.sz[i] - (t.sz[i] % 2)) / 2 : 0;
+        for (int i = 0; i < 3; ++i)
+            if (P.size[i] > 0.0 && (t.loc[i] + t.sz[i] / 2 > find_pt(P.size[i], P.d[i]) || t.loc[i] < 0))
+                if (t.loc[i] + t.sz[i] / 2.0 != 1) throw std::logic_error("A TFSF surface is outside the FDTD cell.");
+        const Value& mv = json::get_child_or_empty(it, "m");
+        if (mv.size() > 0) {
+            for (size_t i = 0; i < mv.size() && i < 3; ++i) t.m[i] = static_cast<int>(mv.child(i).as_num());
+            auto& m = t.m;
+            if ((m[0] == 0 && m[1] == 0) || (m[2] == 0 && m[1] == 0) || (m[0] == 0 && m[2] == 0)) {
+                int mInd = -1;
+                for (int i = 0; i < 3; ++i)
+                    if (m[i] != 0) mInd = i;
+                if (mInd >= 0 && std::abs(m[mInd]) != 1) m[mInd] /= std::abs(m[mInd]);
+            }
+            t.phi = (m[0] == 0) ? ((m[1] > 0) ? kPi / 2.0 : -1.0 * kPi / 2.0) : std::atan(m[1] * P.d[0] / (m[0] * P.d[1]));
+            if (m[0] < 0) t.phi += kPi;
+            else if (m[1] < 0) t.phi += 2.0 * kPi;
+            t.theta = (m[2] == 0) ? kPi / 2.0
+                                  : std::atan(std::sqrt(std::pow(m[1] / P.d[1], 2.0) + std::pow(m[0] / P.d[0], 2.0)) / (m[2] / P.d[2]));
+            if (m[2] < 0) t.theta += kPi;
+            t.psi = kPi * json::get_num(it, "psi", 90.0) / 180.0;
+        } else {
+            t.theta = kPi * json::get_num(it, "theta", 90.0) / 180.0;
+            if (t.theta < 0 || t.theta > kPi) throw std::logic_error("The theta value for the TFSF surfaces must be between 0 and 180 degrees.");
+            t.phi = kPi * json::get_num(it, "phi", 90.0) / 180.0;
+            t.psi = kPi * json::get_num(it, "psi", 90.0) / 180.0;
+            if (P.size[2] == 0.0) {
+                t.theta = kPi / 2.0;
+                t.psi = (P.pol == Pol::EZ || P.pol == Pol::HX || P.pol == Pol::HY) ? kPi / 2.0 : 0.0;
+            }
+            if (std::fmod(t.theta, kPi) == 0.0) t.m[2] = t.theta == 0 ? 1 : -1;
+            else if (std::fmod(t.phi, kPi) == 0.0) t.m[0] = (std::fmod(t.phi, 2 * kPi) == 0.0) ? 1 : -1;
+            else if (std::fmod(t.phi, kPi / 2.0) == 0.0) t.m[1] = (std::fmod(t.phi, 2 * kPi) == kPi / 2.0) ? 1 : -1;
+        }
+        if (t.m[0] == 0 && t.m[1] == 0 && t.m[2] == 0) throw std::logic_error("TFSF surface must have a direction of propgation.");
+        t.pulses = parsePulses(json::get_child_or_empty(it, "PulseList"), P, true);
+        std::string cp = json::get_str(it, "circPol", "Ex");
+        if (cp == "R") t.circ = 1;
+        else if (cp == "L") t.circ = -1;
+        else string2pol(cp);  // any other polarisation name is linear; an unknown one is an error
+        t.kRatio = json::get_num(it, "ellpiticalKRat", 1.0);
+        t.pmlThick = json::get_int(it, "pmlThick", 20);
+        t.pmlAMax = json::get_num(it, "pmlAMax", 0.0);
+        t.pmlM = json::get_num(it, "pmlM", P.pmlM);
+        t.pmlMa = json::get_num(it, "pmlMa", P.pmlMa);
+        P.tfsfs.push_back(t);
+    }
 
     // Soft sources, parallelInputs.cpp:115-216.
     const Value& SL = json::get_child_or_empty(IP, "SourceList");
@@ -429,55 +544,7 @@ ProgramInputs parseInputs(const std::string& text) {
         const Value& it = SL.child(ii);
         SourceSpec s;
         s.pol = string2pol(json::get_str(it, "pol"));
-        const Value& PL = json::get_child_or_empty(it, "PulseList");
-        for (size_t pp = 0; pp < PL.size(); ++pp) {
-            const Value& pul = PL.child(pp);
-            PulseSpec ps;
-            ps.shape = string2prof(json::get_str(pul, "profile"));
-            double Emax = json::get_num(pul, "Field_Intensity", 1.0) * P.a * EPS0 * SPEED_OF_LIGHT / P.I0;
-            std::vector<cplx> f;
-            switch (ps.shape) {
-                case PulseShape::GAUSSIAN:
-                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
-                    f.push_back(1.0 / json::get_num(pul, "fwidth"));
-                    f.push_back(json::get_num(pul, "cutoff") * f[1]);
-                    f.push_back(json::get_num(pul, "t_0", std::real(f[1] * f[2])));
-                    f[2] += f[3];
-                    break;
-                case PulseShape::BH:
-                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
-                    f.push_back(json::get_num(pul, "tau", 1.0 / json::get_num(pul, "fwidth")));
-                    // The reference's t_0 default reads past the end of its
-                    // parameter list (parallelInputs.cpp:145), so t_0 is required.
-                    f.push_back(json::get_num(pul, "t_0"));
-                    f.push_back(json::get_num(pul, "BH1", 0.35875));
-                    f.push_back(json::get_num(pul, "BH2", 0.48829));
-                    f.push_back(json::get_num(pul, "BH3", 0.14128));
-                    f.push_back(json::get_num(pul, "BH4", 0.01168));
-                    break;
-                case PulseShape::RECT:
-                    f.push_back(json::get_num(pul, "tau"));
-                    f.push_back(json::get_num(pul, "t_0"));
-                    f.push_back(json::get_num(pul, "n", 30) * 2.0);
-                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
-                    break;
-                case PulseShape::CONTINUOUS:
-                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
-                    break;
-                case PulseShape::RAMP_CONT:
-                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
-                    f.push_back(json::get_num(pul, "ramp_val"));
-                    break;
-                case PulseShape::RICKER:
-                    f.push_back(cplx(0.0, -1.0 * json::get_num(pul, "fcen") * 2 * kPi));
-                    f.push_back(json::get_num(pul, "fwidth"));
-                    f.push_back(json::get_num(pul, "cutoff"));
-                    break;
-            }
-            f.push_back(Emax);  // parallelFDTDField.cpp:474
-            ps.param = f;
-            s.pulses.push_back(ps);
-        }
+        s.pulses = parsePulses(json::get_child_or_empty(it, "PulseList"), P, false);
         double tsz[3], tloc[3];
         json::get_arr3(it, "size", tsz);
         for (int c = 0; c < 3; ++c) s.sz[c] = find_pt(tsz[c], P.d[c]) + 1;

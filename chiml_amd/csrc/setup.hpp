@@ -85,6 +85,20 @@ struct SourceSpec {
     double phi = 90.0;
 };
 
+// A total-field / scattered-field surface (parallelInputs.cpp:219-408): the
+// box in grid points, the propagation vector m (integers), the angles it
+// implies, the polarisation angle psi and the incident line's CPML.
+struct TfsfSpec {
+    std::array<int, 3> loc{{0, 0, 0}}, sz{{1, 1, 1}};
+    std::array<int, 3> m{{0, 0, 0}};
+    double theta = 0.0, phi = 0.0, psi = 0.0;  // radians
+    int circ = 0;                 // 0 linear, +1 "R", -1 "L" (circPol)
+    double kRatio = 1.0;          // ellpiticalKRat
+    int pmlThick = 20;            // nominal; the line's CPML is pmlThick * sum|m| points
+    double pmlAMax = 0.0, pmlM = 3.0, pmlMa = 1.0;
+    std::vector<PulseSpec> pulses;
+};
+
 struct DetectorSpec {
     DtcType type = DtcType::EZ;
     DtcClass cls = DtcClass::COUT;
@@ -146,6 +160,7 @@ struct ProgramInputs {
     double pmlSigOptRat = 1.0, pmlKappaMax = 1.0, pmlAMax = 0.0, pmlMa = 1.0, pmlM = 3.0;
     std::array<int, 3> pmlThickness{{0, 0, 0}};
     std::vector<SourceSpec> srcs;
+    std::vector<TfsfSpec> tfsfs;
     std::vector<std::shared_ptr<Object>> objs;  // objs[0] is the vacuum background
     std::vector<DetectorSpec> dtcs;
     std::vector<FluxSpec> fluxes;

@@ -85,10 +85,15 @@ int chiml_write_detector_bin(const char* path, const double* times, const double
     return close_out(f, path);
 }
 
-// parallelFluxDTC::getFlux (parallelFlux.hpp:445-534) with the incident field
-// absent: flux_incd = sqrt(0) = 0 in every row.
+// parallelFluxDTC::getFlux (parallelFlux.hpp:445-534); without an incident
+// field flux_incd = sqrt(0) = 0 in every row.
 int chiml_write_flux(const char* path, const double* freqs, int nfreq, int nfaces, const double* face_reim,
                      int front_back) {
+    return chiml_write_flux_incd(path, freqs, nfreq, nfaces, face_reim, front_back, nullptr);
+}
+
+int chiml_write_flux_incd(const char* path, const double* freqs, int nfreq, int nfaces, const double* face_reim,
+                          int front_back, const double* incd_reim) {
     if (nfreq < 0 || nfaces < 1 || (nfreq > 0 && (!freqs || !face_reim))) return fail("null argument");
     using cplx = std::complex<double>;
     std::ofstream f;
@@ -102,8 +107,8 @@ int chiml_write_flux(const char* path, const double* freqs, int nfreq, int nface
         if (front_back) f << std::setw(16) << "\tabs(front)\treal(front)\timag(front)\tabs(back)\treal(back)\timag(back)";
     }
     f << std::setw(16) << "freq\tabs(total)\treal(total)\timag(total)\n";
-    const cplx incd(0.0, 0.0);
     for (int k = 0; k < nfreq; ++k) {
+        const cplx incd = incd_reim ? cplx(incd_reim[2 * k], incd_reim[2 * k + 1]) : cplx(0.0, 0.0);
         f << std::setw(18) << std::setprecision(15) << freqs[k] << "\t" << std::setw(16) << std::setprecision(15)
           << std::abs(incd) << "\t" << std::setw(16) << std::setprecision(15) << std::abs(std::real(incd)) << "\t"
           << std::setw(16) << std::setprecision(15) << std::imag(incd) << "\t";

@@ -349,3 +349,38 @@ def sweep_requests(n_req=512, seed=5, rate_hz=20.0, steps_range=(1000, 10000), e
             deck["ObjectList"].append(em)
         out.append((t, json.dumps(deck)))
     return out
+
+
+def tfsf_deck(n3, steps, box, m=None, angles=None, psi=90.0, pml=12, d=0.05, precision="fp64", fcen_nm=500.0,
+              circ=None, ndet=0, seed=7, tfsf_pml=20, objects=None, flux=None):
+    """A TFSF surface (the reference's "TFSF" list, parallelInputs.cpp:219-408)
+    centred in a vacuum cell of n3 points (n3[2] = 1 for 2D TMz): box = points
+    per axis of the total-field region, m = integer propagation vector or
+    angles = (theta, phi) in degrees, psi = polarisation angle in degrees, a
+    Gaussian pulse centred at fcen_nm.  ndet > 0 adds seeded point detectors of
+    every existing component inside and outside the box."""
+    deck, size, dt = _cell(n3, d, 0.5, steps, pml, precision)
+    fcen = A_UNIT / (fcen_nm * 1e-9)
+    t = {"size": [(b - 1) * d if n > 1 else 0.0 for b, n in zip(box, n3)], "loc": [0.0, 0.0, 0.0],
+         "psi": psi, "pmlThick": tfsf_pml, "PulseList": [gaussian_pulse(fcen, 0.5 * fcen, 4.0, 1.0)]}
+    if m is not None:
+        t["m"] = list(m)
+    else:
+        t["theta"], t["phi"] = angles
+    if circ is not None:
+        t["circPol"] = circ
+    deck["TFSF"].append(t)
+    if objects:
+        deck["ObjectList"].extend(objects)
+    if flux:
+        deck["FluxList"].append(flux)
+    if ndet:
+        comps = ["Ez", "Hx", "Hy"] if n3[2] <= 1 else ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]
+        rng = splitmix64(seed)
+        for k in range(ndet):
+            c = [pml + 2 + int(next(rng) * (nn - 2 * pml - 4)) if nn > 1 else 0 for nn in n3]
+            deck["DetectorList"].append({
+                "type": comps[k % len(comps)], "dtc_class": "cout", "fname": "dtc", "size": [0.0, 0.0, 0.0],
+                "loc": [_pos(c[a], n3[a], d) if n3[a] > 1 else 0.0 for a in range(3)],
+            })
+    return json.dumps(deck)

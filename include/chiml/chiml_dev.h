@@ -92,6 +92,54 @@ typedef struct {
     const double* phasor;     /* [nsteps/interval + 1][nfreq][2] exp(-i w t) */
 } chiml_dft_group_t;
 
+/* One rectangle of a TFSF surface: for i1 < n1 along axis a1 and i2 < n2
+ * along axis a2 from cell (x0, y0, z0), field component comp gains
+ * pf * Re(line[src][incd0 + i1 * incd_s1 + i2 * incd_s2]) (addIncdFields,
+ * src/SOURCE/parallelTFSF.cpp:78-84, with the cells and line indices of
+ * genSurface, parallelTFSF.hpp:814-1000). */
+typedef struct {
+    int comp;             /* 0..5 = Ex..Hz */
+    int src;              /* incident line read, 0..5 */
+    double pf;
+    int x0, y0, z0;
+    int a1, n1, a2, n2;
+    int incd0, incd_s1, incd_s2;
+} chiml_tfsf_surface_t;
+
+/* A total-field / scattered-field surface (parallelTFSFBase,
+ * src/SOURCE/parallelTFSF.hpp:100-1174): two sets of six complex FP64
+ * one-dimensional incident lines (set 0 of len0 points feeds the surfaces,
+ * set 1 of len1 points is the pulse line whose values are recorded for the
+ * flux normalisation), stepped once per time step after the H update.  Per
+ * line c and step, over [margin, len - margin):
+ *   u[c][n] += coef * u[src][n + pos] + (-coef) * u[src][n + neg]
+ * for each of its nterm terms, after amp[step][c][r] is added at
+ * pulse_at + r, then each CPML segment k (psi_n points from gl, gl = margin
+ * at the low end, len - 1 - psi_n - margin at the high end):
+ *   psi = b psi + c u[src][gl + off1 + i] + cOff u[src][gl + off2 + i],
+ *   u[c][gl + i] += Db psi
+ * (updateIncdField*, parallelTFSF.cpp:11-45; IncdCPML, src/PML/PML.hpp:210-269,
+ * PML.cpp:12-29).  The incident medium is vacuum, so U = D and H = B. */
+typedef struct {
+    int len0, len1;
+    int present[6];
+    int margin;
+    int nterm[6];
+    int term_src[6][2], term_pos[6][2], term_neg[6][2];
+    double term_coef[6][2];
+    int pulse_at, pulse_n;
+    const double* amp;        /* [nsteps][6][pulse_n][2] prefactor * sum of pulses */
+    int psi_n;
+    int nseg[6];              /* <= 4 */
+    int seg_src[6][4], seg_high[6][4], seg_off1[6][4], seg_off2[6][4];
+    double seg_db[6][4];
+    const double* seg_coef;   /* [6][4][psi_n][3] = b, c, cOff */
+    int rec_idx;              /* recorded point of the pulse lines */
+    int rec_off[6];           /* the offset point is rec_idx - rec_off[c] */
+    int nsurf_h, nsurf_e;     /* H surfaces first, then E surfaces, each in reference order */
+    const chiml_tfsf_surface_t* surf;
+} chiml_tfsf_t;
+
 typedef struct {
     int nx, ny, nz;       /* interior points per axis; nz == 1 selects 2D TMz */
     int precision;        /* 32 or 64: arithmetic type of fields, ADE and CPML */
@@ -167,6 +215,12 @@ typedef struct {
     /* DFT groups 1..ndftg (group 0 is ndft / dft_* above), at most 7. */
     int ndftg;
     const chiml_dft_group_t* dftg;
+
+    /* TFSF surfaces, stepped in order; each records the incident E and H of
+     * its pulse lines at every step (the last surface's values are kept,
+     * parallelFDTDField.hpp:1238-1255). */
+    int ntfsf;
+    const chiml_tfsf_t* tfsf;
 } chiml_problem_t;
 
 int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out);
@@ -190,6 +244,11 @@ int chiml_dev_read_dft(chiml_dev_t* ctx, double* reim, size_t cap);
  * reim[(cell * nfreq + f) * 2 + {0,1}].  Replaces the accumulators read by
  * parallelDetectorFREQ_Base::collectFreqFields (src/DTC/parallelDTC_FREQ.hpp:270-296). */
 int chiml_dev_read_dft_group(chiml_dev_t* ctx, int g, double* reim, size_t cap);
+/* Incident records of component comp (0..5 = Ex..Hz): two complex values per
+ * step (the recorded point and its offset), 2 (nsteps + 1) + 2 in all, zero
+ * where no TFSF surface recorded (E_incd_ / H_incd_,
+ * parallelFDTDField.hpp:250-255, 1238-1255; parallelFDTDField.cpp:838-851). */
+int chiml_dev_read_incd(chiml_dev_t* ctx, int comp, double* reim, size_t cap, size_t* n);
 /* Population series of detector j of emitter group q: [ns] complex. */
 int chiml_dev_read_pops(chiml_dev_t* ctx, int q, int j, double* reim, size_t cap, size_t* nsamples);
 /* Density matrix of (group q, member m, emitter e): N*N complex column-major. */

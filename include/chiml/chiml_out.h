@@ -39,6 +39,11 @@ int chiml_write_detector_bin(const char* path, const double* times, const double
  * columns). */
 int chiml_write_flux(const char* path, const double* freqs, int nfreq, int nfaces, const double* face_reim,
                      int front_back);
+/* The same table with the incident flux of the TFSF surfaces per frequency
+ * (incd_reim[k * 2 + {0,1}], NULL for zero), written as abs, |real| and imag
+ * (parallelFlux.hpp:455-483, 521). */
+int chiml_write_flux_incd(const char* path, const double* freqs, int nfreq, int nfaces, const double* face_reim,
+                          int front_back, const double* incd_reim);
 
 /* Frequency detector table (parallelDetectorFREQ_Base::toFile with incident
  * fields, src/DTC/parallelDTC_FREQ.hpp:398-444): a header, then per frequency

@@ -59,6 +59,13 @@ int chiml_sim_freq_fields(chiml_sim_t* s, int d, double* reim, size_t cap);
 /* The table's value of each grid at each frequency, reim[(grid * nfreq + f) * 2 + {0,1}]. */
 int chiml_sim_freq_values(chiml_sim_t* s, int d, double* reim, size_t cap);
 int chiml_sim_pops(chiml_sim_t* s, int q, int j, double* reim, size_t cap, size_t* nsamples);
+/* Incident field records of component comp (0..5 = Ex..Hz) written by the TFSF
+ * surfaces: two complex values per step (parallelFDTDField.hpp:1238-1255,
+ * FF.ExIncd() .. FF.HzIncd() in src/main.cpp:70); zero without a TFSF surface. */
+int chiml_sim_incd(chiml_sim_t* s, int comp, double* reim, size_t cap, size_t* n);
+/* Incident flux of flux box f per frequency, the incident columns of its table
+ * (parallelFlux.hpp:455-483, 521): nfreq complex values; zero without a TFSF surface. */
+int chiml_sim_flux_incd(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq);
 int chiml_sim_density(chiml_sim_t* s, int q, int member, int emitter, double* reim, int* nlevel);
 /* Cell-updates per step and the algorithmic HBM bytes per step of the field
  * kernels (update_h + update_e), used for the roofline figure. */

@@ -488,6 +488,68 @@ struct Source {
     std::vector<double> pulVec;
 };
 
+// ------------------------------------------------------------------ TFSF ----
+// parallelTFSFBase (src/SOURCE/parallelTFSF.hpp:100-1174, parallelTFSF.cpp:11-92)
+// and IncdCPML (src/PML/PML.hpp:37-350, PML.cpp:12-40) for real fields on one
+// rank.  Restated for an incident line whose medium is the background: every
+// far-side surface plane that genEpMu reads (parallelTFSF.hpp:572-686) holds
+// no object, so eps = mu = 1 and the incident phys id is 0 along the whole
+// line, the update lists are one run and every surface add goes to U
+// (addIncdFields, parallelTFSF.cpp:77-83).  The layered (EPChange) and
+// dispersive incident lines are not restated and throw.
+struct IncdPsiList {  // updatePsiParamsIncd + updateGridParamsIncd (PML.hpp:18-35)
+    int loc = 0, locOff = 0;
+    std::vector<double> b, c, cOff;
+    int gridLoc = 0, nAx = 0;
+    double Db = 0.0;
+};
+struct IncdCPMLo {  // IncdCPML<cplx>
+    int ci = 0, cj = 0, ck = 0;  // cor_ii_, cor_jj_, cor_kk_
+    bool E = false;
+    std::vector<cplx>* gi = nullptr;  // grid_i_
+    std::vector<cplx>* gj = nullptr;  // grid_j_
+    std::vector<cplx>* gk = nullptr;  // grid_k_
+    std::vector<std::vector<cplx>> psiJ, psiK;  // psi_j_, psi_k_ (empty vector: not made)
+    std::vector<IncdPsiList> listJ, listK;
+};
+struct IncdParam {  // paramUpIncdField (parallelTFSF.hpp:16-38) without pole terms
+    int indI, indPosJ, indNegJ, indPosK, indNegK, nSz;
+    double pj, pk;
+};
+struct IncdComp {  // one component's incident, D/B and pulse lines
+    bool present = false;
+    int mode = 0;  // updateIncdField: 1 J, 2 K, 3 JK
+    int distBase = 0;
+    std::vector<cplx> U, Dv, pulse;  // E_incd/H_incd, D_incd/B_incd, E_pulse/H_pulse
+    std::vector<double> epmu, epmuPul;
+    std::vector<int> phys, physPul;
+    std::vector<IncdParam> up, upPul;
+    std::vector<double> dist;
+    IncdCPMLo pml, pmlPul;
+};
+struct TfsfSurf {  // paramStoreTFSF (parallelTFSF.hpp:88-99), indsU_ only
+    int incdComp = 0;  // 0..2 E_incd, 3..5 H_incd
+    int strideIncd = 0, addIncdProp = 0, strideMain = 0;
+    double pf = 0.0;
+    std::array<int, 2> szTrans{{-1, -1}};
+    std::vector<int> indsU;
+};
+struct Tfsf {
+    int mTot = 0, mMax = 0, gridLen = 0, quadrant = 0, pmlThick = 0, t_step = 0;
+    double phi = 0, theta = 0, psi = 0, alpha = 0, phiP = 0, psiP = 0, dt = 0, dr = -1;
+    std::array<cplx, 3> prefH{{1.0, 1.0, 1.0}}, prefE{{1.0, 1.0, 1.0}};
+    std::array<int, 3> sz{}, loc{}, originLoc{}, m{};
+    std::array<double, 3> d{};
+    std::vector<PulseSpec> pul;
+    std::vector<cplx> pulseVec;
+    IncdComp H[3], E[3];
+    std::vector<TfsfSurf> hSur[3], eSur[3];
+    int recIdx() const { return pmlThick + mMax + 5 * mTot; }
+    int getIr(const std::array<int, 3>& l) const {
+        return std::abs(m[0] * l[0]) + std::abs(m[1] * l[1]) + std::abs(m[2] * l[2]);
+    }
+};
+
 // -------------------------------------------------------------- detectors ---
 struct PointDetector {
     DtcType type;
@@ -595,6 +657,8 @@ struct Sim {
     std::vector<Flux> fluxes;
     std::vector<QE> qes;
     std::vector<double> scratch;
+    std::vector<std::unique_ptr<Tfsf>> tfsfs;
+    std::array<std::vector<cplx>, 3> Eincd, Hincd;  // E_incd_ / H_incd_ (parallelFDTDField.hpp:143-144)
 
     explicit Sim(const std::string& json);
     void setupPhysFields(IGrid& phys, DGrid& epMu, bool Efield, std::array<double, 3> offPt, std::array<int, 3> endOff, int comp);
@@ -609,6 +673,8 @@ struct Sim {
     void initializeList(IGrid& phys, DGrid& epMu, CPML& pml, bool Efield, std::array<int, 3> derivOff, std::array<int, 3> fieldEnd,
                         double dj, double dk, std::vector<Run>& upU, std::vector<Run>& upDl, std::vector<Run>& upLorDl);
     void buildSources();
+    void buildTfsf();
+    void tfsfStepAll();
     void buildDetectors();
     void buildFlux();
     void buildQE();
@@ -623,6 +689,7 @@ struct Sim {
     void denDeriv(QE& q, const std::vector<cplx>& H, const cplx* den, cplx* out);
     void getHam(QE::Member& mem, const cplx Ex, const cplx Ey, const cplx Ez, std::vector<cplx>& H);
     std::vector<cplx> getFlux(int fi);
+    std::vector<cplx> getFluxIncd(int fi);
     std::vector<std::array<std::vector<cplx>, 4>> faceFields(int fi);
     void saveFields(int fi, const std::string& path);
     void loadFields(Flux& fx, const std::string& path);
@@ -879,8 +946,13 @@ Sim::Sim(const std::string& json) : IP(parseInputs(json)) {
         initializeList(physH[0], muRel[0], *HPML[0], false, {{0, 1, 0}}, {{0, 1, 1}}, d[1], d[2], upH[0], dummy1, dummy2);
         initializeList(physH[1], muRel[1], *HPML[1], false, {{0, 0, 1}}, {{1, 0, 1}}, d[2], d[0], upH[1], dummy3, dummy4);
     }
+    for (int c = 0; c < 3; ++c) {
+        Eincd[c].assign(2 * (IP.nSteps() + 1), 0.0);
+        Hincd[c].assign(2 * (IP.nSteps() + 1), 0.0);
+    }
     buildQE();
     buildSources();
+    buildTfsf();
     buildFlux();
     buildDetectors();
     // Detectors and flux take their t = 0 sample in the constructor
@@ -888,6 +960,565 @@ Sim::Sim(const std::string& json) : IP(parseInputs(json)) {
     outputDetectors();
     freqOutput();
     fluxFieldIn();
+    // incident records done twice for the offset (parallelFDTDField.cpp:838-851)
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 3; ++c) {
+            Hincd[c].push_back(0.0);
+            Eincd[c].push_back(0.0);
+        }
+}
+
+// ---- TFSF construction (parallelTFSF.hpp:268-440) ---------------------------
+// IncdCPML::getUpList, PML.hpp:210-257.
+static IncdPsiList incdUpList(const IncdCPMLo& P, int dir, double distOff, bool pl, int startPt, int dirMax, const Tfsf& T,
+                              double pmlM, double pmlMa, double aMax) {
+    IncdPsiList L;
+    const double dt = T.dt, kappaMax = 1.0;
+    const double sigmaMax = 0.8 * (pmlM + 1) / std::abs(T.dr);
+    const int cor = dir;
+    const bool E = P.E;
+    L.Db = dt / (T.d[cor] * 1.0);  // eps/mu of both ends are 1
+    L.gridLoc = startPt;
+    L.nAx = dirMax;
+    const int mc = T.m[cor];
+    const bool isJ = (dir == P.cj);
+    if (isJ ^ !E) {
+        L.loc = startPt + (E ? (mc) / 2 : -1 * (mc) / 2);
+        L.locOff = startPt + (E ? -1 * (mc + mc % 2) / 2 : (mc + mc % 2) / 2);
+    } else {
+        L.loc = startPt + (E ? -1 * (mc + mc % 2) / 2 : (mc + mc % 2) / 2);
+        L.locOff = startPt + (E ? (mc) / 2 : -1 * (mc) / 2);
+    }
+    const double iiMax = static_cast<double>(dirMax - 1);
+    auto sigma = [&](double ii, double sMax) { return (0.0 <= ii && ii <= iiMax) ? sMax * std::pow((iiMax - ii) / iiMax, pmlM) : 0.0; };
+    auto kappa = [&](double ii) { return (0.0 <= ii && ii <= iiMax) ? 1.0 + (kappaMax - 1.0) * std::pow((iiMax - ii) / iiMax, pmlM) : 1.0; };
+    auto aVal = [&](double ii) { return (0.0 <= ii && ii <= iiMax) ? aMax * std::pow(ii / iiMax, pmlMa) : 0.0; };
+    L.b.assign(dirMax, 0.0);
+    L.c.assign(dirMax, 0.0);
+    L.cOff.assign(dirMax, 0.0);
+    for (int dd = 0; dd < dirMax; ++dd) {
+        const double dist = pl ? dirMax - 1 - (static_cast<double>(dd) + distOff) : static_cast<double>(dd) + distOff;
+        const double sig = sigma(dist, sigmaMax / std::sqrt(1.0 * 1.0));
+        const double kap = kappa(dist);
+        const double a = aVal(dist);
+        L.b[dd] = std::exp(-1.0 * (sig / kap + a) * dt);
+        const double cv = (sig == 0 && a == 0) ? 0 : sig / (sig * kap + std::pow(kap, 2.0) * a) * (std::exp(-1.0 * (sig / kap + a) * dt) - 1.0);
+        L.c[dd] = cv;
+        L.cOff[dd] = cv;
+        if (!E) L.c[dd] *= -1.0;
+        else L.cOff[dd] *= -1.0;
+    }
+    return L;
+}
+
+// IncdCPML constructor, PML.hpp:104-197 (useMn = true, every eps/mu = 1).
+static void makeIncdCPML(IncdCPMLo& P, int comp, bool E, std::vector<cplx>* gi, std::vector<cplx>* gj, std::vector<cplx>* gk,
+                         const Tfsf& T, bool is3D, double pmlM, double pmlMa, double aMax) {
+    P.E = E;
+    P.gi = gi;
+    P.gj = gj;
+    P.gk = gk;
+    P.ci = comp;
+    P.cj = (comp + 1) % 3;
+    P.ck = (comp + 2) % 3;
+    double mOffHalf;
+    const auto& m = T.m;
+    if (comp == 0) mOffHalf = (!E ? m[1] % 2 + m[2] % 2 : m[0] % 2) / 2.0;
+    else if (comp == 1) mOffHalf = (!E ? m[2] % 2 + m[0] % 2 : m[1] % 2) / 2.0;
+    else mOffHalf = (!E ? m[0] % 2 + m[1] % 2 : m[2] % 2) / 2.0;
+    const int pmlThick = T.pmlThick;
+    const int len = static_cast<int>(gi->size());
+    if (gk && (is3D || P.cj != 2)) P.psiJ.assign(2, std::vector<cplx>(pmlThick, 0.0));
+    if (gj && (is3D || P.ck != 2)) P.psiK.assign(2, std::vector<cplx>(pmlThick, 0.0));
+    if (gk && m[P.cj] != 0) {
+        P.listJ.push_back(incdUpList(P, P.cj, mOffHalf, true, len - 1 - pmlThick - T.mTot, pmlThick, T, pmlM, pmlMa, aMax));
+        P.listJ.push_back(incdUpList(P, P.cj, mOffHalf, false, std::abs(T.mTot), pmlThick, T, pmlM, pmlMa, aMax));
+    }
+    if (gj && m[P.ck] != 0) {
+        P.listK.push_back(incdUpList(P, P.ck, mOffHalf, true, len - 1 - pmlThick - T.mTot, pmlThick, T, pmlM, pmlMa, aMax));
+        P.listK.push_back(incdUpList(P, P.ck, mOffHalf, false, std::abs(T.mTot), pmlThick, T, pmlM, pmlMa, aMax));
+    }
+    if ((!P.listJ.empty() && P.psiJ.empty()) || (!P.listK.empty() && P.psiK.empty()))
+        throw std::logic_error("incident CPML list without its psi field");
+}
+
+// IncdCPML::updateGrid with pmlUpdateFxnIncdCplx::addPsi / updatePsiField
+// (PML.hpp:263-269, PML.cpp:12-29).
+static void incdCPMLUpdate(IncdCPMLo& P) {
+    auto run = [&](std::vector<IncdPsiList>& lists, std::vector<std::vector<cplx>>& psis, std::vector<cplx>* grid) {
+        for (size_t ii = 0; ii < lists.size(); ++ii) {
+            IncdPsiList& L = lists[ii];
+            std::vector<cplx>& psi = psis[ii];
+            const int n = static_cast<int>(psi.size());
+            for (int k = 0; k < n; ++k) psi[k] = L.b[k] * psi[k];
+            for (int k = 0; k < n; ++k) psi[k] = psi[k] + L.c[k] * (*grid)[L.loc + k];
+            for (int k = 0; k < n; ++k) psi[k] = psi[k] + L.cOff[k] * (*grid)[L.locOff + k];
+            for (int k = 0; k < n; ++k) (*P.gi)[L.gridLoc + k] += L.Db * psi[k];
+        }
+    };
+    run(P.listJ, P.psiJ, P.gk);
+    run(P.listK, P.psiK, P.gj);
+}
+
+// genFieldUpParam, parallelTFSF.hpp:700-728 (background incident line: no poles).
+static std::vector<IncdParam> genFieldUpParam(bool E, const std::vector<int>& phys, int i, int j, int k, const Tfsf& T,
+                                              const std::vector<std::shared_ptr<Object>>& objs) {
+    (void)i;
+    std::vector<IncdParam> up;
+    const auto& m = T.m;
+    const int len = static_cast<int>(phys.size());
+    int ii = T.mTot;
+    while (ii < len - T.mTot) {
+        IncdParam p;
+        p.pj = T.dt / T.d[k];
+        p.pk = T.dt / T.d[j];
+        p.indI = ii;
+        p.indPosJ = ii + (E ? m[j] / 2 : -1 * m[j] / 2);
+        p.indNegJ = ii + (E ? -1 * (m[j] + m[j] % 2) / 2 : (m[j] + m[j] % 2) / 2);
+        p.indPosK = ii + (E ? -1 * (m[k] + m[k] % 2) / 2 : (m[k] + m[k] % 2) / 2);
+        p.indNegK = ii + (E ? m[k] / 2 : -1 * m[k] / 2);
+        const Object& o = *objs.at(phys[ii]);
+        if (!o.alpha.empty()) throw std::logic_error("a dispersive incident medium is not restated");
+        ++ii;
+        while (ii < len - T.mTot && phys[ii] == phys[ii - 1]) ++ii;
+        p.nSz = ii - p.indI;
+        up.push_back(p);
+    }
+    return up;
+}
+
+void Sim::buildTfsf() {
+    for (auto& spec : IP.tfsfs) {
+        auto tp = std::make_unique<Tfsf>();
+        Tfsf& T = *tp;
+        T.sz = spec.sz;
+        T.loc = spec.loc;
+        T.originLoc = spec.loc;
+        T.m = spec.m;
+        T.d = IP.d;
+        T.dt = dt;
+        T.phi = T.phiP = spec.phi;
+        T.theta = spec.theta;
+        T.psi = T.psiP = spec.psi;
+        T.pul = spec.pulses;
+        const int nom = spec.pmlThick;
+        T.mTot = std::abs(T.m[0]) + std::abs(T.m[1]) + std::abs(T.m[2]);
+        T.mMax = std::max(std::max(T.m[0], std::max(T.m[1], T.m[2])), std::abs(std::min(T.m[0], std::min(T.m[1], T.m[2]))));
+        T.gridLen = 2 * T.mMax + T.mTot * (T.sz[0] + T.sz[1] + T.sz[2] + 10 + 2 * nom);
+        T.pmlThick = nom * T.mTot;
+        T.pulseVec.assign(T.mTot, 0.0);
+        if (T.m[0] < 0 || T.m[1] < 0 || T.m[2] < 0)
+            throw std::logic_error("a TFSF surface with a negative m component is not restated");
+        int mInd = -1;
+        const auto& m = T.m;
+        if ((m[0] == 0 && m[1] == 0) || (m[2] == 0 && m[1] == 0) || (m[0] == 0 && m[2] == 0))
+            for (int c = 0; c < 3; ++c)
+                if (m[c] != 0) mInd = c;
+        const bool twoDim = !is3D;
+        if ((!twoDim && ((T.sz[0] <= 1 && T.sz[1] <= 1) || (T.sz[0] <= 1 && T.sz[2] <= 1) || (T.sz[1] <= 1 && T.sz[2] <= 1))) ||
+            (twoDim && T.sz[0] <= 1 && T.sz[1] <= 1 && lx != 3 && ly != 3))
+            throw std::logic_error("TFSF sources are not point/line sources sources");
+        if (T.mTot == 0) throw std::logic_error("The m array must have at least one non-zero number");
+        std::array<double, 3> drT = {{T.d[0] * std::cos(T.phi) * std::sin(T.theta) / static_cast<double>(m[0]),
+                                      T.d[1] * std::sin(T.phi) * std::sin(T.theta) / static_cast<double>(m[1]),
+                                      T.d[2] * std::cos(T.theta) / static_cast<double>(m[2])}};
+        for (int c = 0; c < 3; ++c)
+            if (m[c] != 0) T.dr = drT[c];
+        for (int c = 0; c < 3; ++c)
+            if (m[c] != 0 && std::abs(drT[c] - T.dr) > 1e-13) throw std::logic_error("p[ii]*d[ii]/m[ii] must equal dr for all ii in x, y, z");
+        const double PI = kPi;
+        if (T.phi <= PI / 2.0 && T.phi > 0) T.quadrant = 1;
+        else if (T.phi <= PI && T.phi > PI / 2.0) {
+            T.quadrant = 2;
+            T.originLoc[0] += T.sz[0] - 1;
+        } else if (T.phi <= 3 * PI / 2.0 && T.phi > PI) {
+            T.quadrant = 3;
+            T.originLoc[1] += T.sz[1] - 1;
+            T.originLoc[0] += T.sz[0] - 1;
+        } else if ((T.phi <= 2 * PI && T.phi > 3.0 * PI / 2.0) || T.phi == 0) {
+            T.quadrant = 4;
+            T.originLoc[1] += T.sz[1] - 1;
+        }
+        if (T.theta > PI / 2.0 || T.theta < 0.0) {
+            T.quadrant *= -1;
+            T.originLoc[2] += T.sz[2] - 1;
+        }
+        if (T.sz[1] == 1 && m[0] != 0 && m[2] != 0)
+            throw std::logic_error("Angle of propgation has a component along the TFSF surface without any other surfaces to cancel it.");
+        else if (T.sz[0] == 1 && m[2] != 0 && m[1] != 0)
+            throw std::logic_error("Angle of propgation has a component along the TFSF surface without any other surfaces to cancel it.");
+        else if (T.sz[2] == 1 && !twoDim && m[0] != 0 && m[1] != 0)
+            throw std::logic_error("Angle of propgation has a component along the TFSF surface without any other surfaces to cancel it.");
+        if (T.theta == 0 || T.theta == PI) T.phiP = PI / 2.0;
+        if (spec.circ != 0) {
+            const double c = std::pow(spec.kRatio, 2.0);
+            T.psiP = 0.5 * std::asin(std::sqrt((std::pow(std::cos(2 * T.psi), 2) * 4 * c + std::pow((1 + c) * std::sin(2 * T.psi), 2.0)) /
+                                               std::pow(1.0 + c, 2.0)));
+            T.alpha = std::acos(((c - 1.0) * std::sin(2 * T.psi)) /
+                                std::sqrt(std::pow(std::cos(2.0 * T.psi), 2.0) * 4.0 * c + std::pow((1.0 + c) * std::sin(2.0 * T.psi), 2.0)));
+            if (std::abs(std::tan(T.psi)) > 1) T.psiP = PI / 2.0 - T.psiP;
+            if (spec.circ > 0) T.alpha *= -1.0;
+        }
+        // initializeIncdFields (parallelTFSF.hpp:470-548) with genEpMu's
+        // homogeneous branches (parallelTFSF.hpp:572-686).
+        const int fieldOff = T.pmlThick + T.mMax + 5 * T.mTot;
+        auto initComp = [&](IncdComp& C, bool Efield, int comp, int distBase, bool hasJ, bool hasK) {
+            C.present = true;
+            C.distBase = distBase;
+            C.dist.assign(T.mTot, 0.0);
+            for (int rr = 0; rr < T.mTot; ++rr) C.dist[rr] = static_cast<double>(distBase + 2 * rr) / 2.0 * T.dr;
+            C.U.assign(T.gridLen, 0.0);
+            C.Dv.assign(T.gridLen, 0.0);
+            C.epmu.assign(T.gridLen, 0.0);
+            C.phys.assign(T.gridLen, 0);
+            // genEpMu: the planes on the far side of each surface
+            IGrid& phys = Efield ? physE[comp] : physH[comp];
+            const int off = Efield ? 0 : -1;
+            const int surInds[6] = {2, 2, 1, 1, 0, 0};
+            const int locOff[6] = {off, T.sz[2], off, T.sz[1], off, T.sz[0]};
+            const int indStart = is3D ? 0 : 2;
+            for (int ii = indStart; ii < 6; ++ii) {
+                const int sI = surInds[ii];
+                std::array<int, 2> pLoc = {{(sI != 2) ? T.loc[(sI + 2) % 3] : T.loc[0], (sI != 2) ? T.loc[(sI + 1) % 3] : T.loc[1]}};
+                std::array<int, 2> pSz = {{(sI != 2) ? T.sz[(sI + 2) % 3] : T.sz[0], (sI != 2) ? T.sz[(sI + 1) % 3] : T.sz[1]}};
+                std::array<int, 2> fSz = {{(sI != 2) ? phys.ln((sI + 2) % 3) : phys.ln(0), (sI != 2) ? phys.ln((sI + 1) % 3) : phys.ln(1)}};
+                if ((pSz[0] <= 1 && (is3D || (pSz[1] <= 1 && (fSz[0] > 3 || fSz[1] > 3)))) ||
+                    (pSz[1] <= 1 && (is3D || (pSz[0] <= 1 && (fSz[0] > 3 || fSz[1] > 3)))))
+                    continue;
+                else if (((m[0] < 0 || m[1] < 0 || m[2] < 0) && (ii % 2 == 1)) || (ii % 2 == 0))
+                    continue;
+                // getPlaneYZ / getPlaneXZ / getPlaneXY with a corner and size
+                // (parallelGrid.hpp:964-978, 1030-1073, 1128-1175, one rank):
+                // the YZ and XY planes clip y to the domain, unread entries stay 0
+                const int at = T.loc[sI] + locOff[ii];
+                const int z1 = is3D ? 1 : 0;
+                std::vector<int> vals(static_cast<size_t>(pSz[0]) * pSz[1], 0);
+                auto rd = [&](int x, int y, int z) {
+                    if (x < 0 || x >= phys.lx || y < 0 || y >= phys.ly || z < 0 || z >= phys.lz)
+                        throw std::logic_error("a TFSF surface plane outside the grid");
+                    return phys.at(x, y, z);
+                };
+                if (sI == 1) {
+                    for (int zz = 0; zz < pSz[1]; ++zz)
+                        for (int xx = 0; xx < pSz[0]; ++xx) vals[zz * pSz[0] + xx] = rd(pLoc[0] + 1 + xx, at + 1, pLoc[1] + z1 + zz);
+                } else {
+                    int ySz = 0, locY = 0;
+                    const int lnm2 = phys.ly - 2, l1 = pLoc[1], s1 = pSz[1];
+                    if (l1 > 0 && l1 <= lnm2) {
+                        ySz = (l1 + s1 > lnm2) ? lnm2 - l1 : s1;
+                        locY = l1 + 1;
+                    } else if (l1 <= 0 && l1 + s1 > 0) {
+                        ySz = (lnm2 > l1 + s1) ? s1 + l1 : lnm2;
+                        locY = 1;
+                    }
+                    for (int yy = 0; yy < ySz; ++yy)
+                        for (int q0 = 0; q0 < pSz[0]; ++q0)
+                            vals[q0 + yy * pSz[0]] = (sI == 0) ? rd(at + 1, yy + locY, q0 + pLoc[0] + z1) : rd(pLoc[0] + 1 + q0, yy + locY, at + z1);
+                }
+                for (int v : vals)
+                    if (v != 0) throw std::logic_error("an object or the grid edge on a TFSF surface plane (layered incident medium) is not restated");
+                const double ev = 1.0;  // epMuVals of the background id 0 (parallelTFSF.hpp:603-606)
+                if (sI != mInd || T.sz[0] == 1 || T.sz[1] == 1 || T.sz[2] == 1) {
+                    for (double a : C.epmu)
+                        if (a != ev && a != 0.0) throw std::logic_error("3) The material is not consistent across all TFSF surfaces");
+                    std::fill(C.epmu.begin(), C.epmu.end(), ev);
+                    std::fill(C.phys.begin(), C.phys.end(), 0);
+                }
+            }
+            if (C.epmu[fieldOff] == 0.0) throw std::logic_error("no TFSF surface plane defines the incident medium");
+            C.up = genFieldUpParam(Efield, C.phys, comp, (comp + 1) % 3, (comp + 2) % 3, T, IP.objs);
+            const int lenP = 2 * T.pmlThick + 20 * T.mTot;
+            C.pulse.assign(lenP, 0.0);
+            C.epmuPul.assign(lenP, C.epmu[fieldOff]);
+            C.physPul.assign(lenP, C.phys[fieldOff]);
+            C.upPul = genFieldUpParam(Efield, C.physPul, comp, (comp + 1) % 3, (comp + 2) % 3, T, IP.objs);
+            C.mode = (hasJ && hasK) ? 3 : (hasJ ? 1 : (hasK ? 2 : 0));
+        };
+        auto okH = [&](int c) { return H[c].ok(); };
+        auto okE = [&](int c) { return E[c].ok(); };
+        if (okH(0)) initComp(T.H[0], false, 0, m[1] + m[2], okE(1), okE(2));
+        if (okH(1)) initComp(T.H[1], false, 1, m[0] + m[2], okE(2), okE(0));
+        if (okH(2)) initComp(T.H[2], false, 2, m[0] + m[1], okE(0), okE(1));
+        if (okE(0)) initComp(T.E[0], true, 0, m[0], okH(1), okH(2));
+        if (okE(1)) initComp(T.E[1], true, 1, m[1], okH(2), okH(0));
+        if (okE(2)) initComp(T.E[2], true, 2, m[2], okH(0), okH(1));
+        // scaleDistVecSpeed (parallelTFSF.hpp:732-737)
+        auto scale = [&](IncdComp& ep, IncdComp& mu, IncdComp& dst) {
+            for (int r = 0; r < T.mTot; ++r) dst.dist[r] *= std::sqrt(ep.epmu[r] * mu.epmu[r]);
+        };
+        if (T.E[0].present && T.H[0].present) {
+            for (int c = 0; c < 3; ++c) {
+                scale(T.E[c], T.H[c], T.E[c]);
+                scale(T.E[c], T.H[c], T.H[c]);
+            }
+        } else if (T.E[0].present) {
+            scale(T.E[0], T.H[2], T.E[0]);
+            scale(T.E[1], T.H[2], T.E[1]);
+            scale(T.E[0], T.H[2], T.H[2]);
+        } else {
+            scale(T.E[2], T.H[0], T.H[0]);
+            scale(T.E[2], T.H[1], T.H[1]);
+            scale(T.E[2], T.H[0], T.E[2]);
+        }
+        // prefactors (parallelTFSF.hpp:415-425)
+        const cplx psiJ = std::cos(T.psiP);
+        const cplx psiK = std::sin(T.psiP) * std::exp(cplx(0.0, T.alpha));
+        const int fo = fieldOff;
+        auto emAt = [&](IncdComp& a, IncdComp& b) { return a.present ? a.epmu[fo] : b.epmu[fo]; };
+        const double base = T.dt / (T.mTot * T.dr);
+        T.prefH[0] = (psiK * std::sin(T.phiP) + psiJ * std::cos(T.theta) * std::cos(T.phiP)) * T.dt / (T.mTot * T.dr) / emAt(T.H[0], T.H[2]);
+        T.prefH[1] = (-1.0 * psiK * std::cos(T.phiP) + psiJ * std::cos(T.theta) * std::sin(T.phiP)) * T.dt / (T.mTot * T.dr) / emAt(T.H[1], T.H[2]);
+        T.prefH[2] = (-1.0 * psiJ * std::sin(T.theta)) * T.dt / (T.mTot * T.dr) / emAt(T.H[2], T.H[0]);
+        T.prefE[0] = (psiJ * std::sin(T.phiP) - psiK * std::cos(T.theta) * std::cos(T.phiP)) * T.dt / (T.mTot * T.dr) / emAt(T.E[0], T.E[2]);
+        T.prefE[1] = (-1.0 * psiJ * std::cos(T.phiP) - psiK * std::cos(T.theta) * std::sin(T.phiP)) * T.dt / (T.mTot * T.dr) / emAt(T.E[1], T.E[2]);
+        T.prefE[2] = (psiK * std::sin(T.theta)) * T.dt / (T.mTot * T.dr) / emAt(T.E[2], T.E[0]);
+        (void)base;
+        // surfaces (initializeIncdAddStructs / genSurface, parallelTFSF.hpp:739-1000)
+        const bool pl = (T.theta == 0 || ((T.theta == PI / 2.0 && std::abs(T.quadrant) == 4) || std::abs(T.quadrant) == 1)) ? false : true;
+        auto nvec = [&](int a) { return (a == 2 && !is3D) ? 1 : n[a] + 2; };
+        auto genSurface = [&](DGrid& mainGrid, int polMain, int polIncd, int dir, bool plS, bool Efield) -> std::unique_ptr<TfsfSurf> {
+            // polarisations as 0..5 = EX, EY, EZ, HX, HY, HZ
+            std::array<int, 3> fieldEnd{{0, 0, 0}};
+            double mOffSet = 0, mOffHalf = 0;
+            if (polMain == 0 || polMain == 5 || polMain == 4) fieldEnd[0] = 1;
+            if (polMain == 1 || polMain == 3 || polMain == 5) fieldEnd[1] = 1;
+            if (is3D && (polMain == 2 || polMain == 3 || polMain == 4)) fieldEnd[2] = 1;
+            if (polIncd == 0 || polIncd == 5 || polIncd == 4) {
+                mOffSet += static_cast<double>(m[0]) / 2.0;
+                mOffHalf += static_cast<double>(m[0] % 2) / 2.0;
+            }
+            if (polIncd == 1 || polIncd == 3 || polIncd == 5) {
+                mOffSet += static_cast<double>(m[1]) / 2.0;
+                mOffHalf += static_cast<double>(m[1] % 2) / 2.0;
+            }
+            if (polIncd == 2 || polIncd == 3 || polIncd == 4) {
+                mOffSet += static_cast<double>(m[2]) / 2.0;
+                mOffHalf += static_cast<double>(m[2] % 2) / 2.0;
+            }
+            bool isJ = true;
+            int cor = -1, t1 = -1, t2 = -1;
+            std::array<int, 3> add1{}, add2{};
+            auto S = std::make_unique<TfsfSurf>();
+            S->incdComp = polIncd;
+            if (dir == 0) {
+                if (polMain == 4 || polMain == 1) isJ = false;
+                if (is3D && mInd != 2) {
+                    cor = 0; t1 = 2; t2 = 1;
+                    add1 = {{0, 0, 1}}; add2 = {{0, 1, 0}};
+                    S->strideMain = mainGrid.lx;
+                } else {
+                    cor = 0; t1 = 1; t2 = 2;
+                    add1 = {{0, 1, 0}}; add2 = {{0, 0, 1}};
+                    S->strideMain = mainGrid.lx * mainGrid.lz;
+                }
+            } else if (dir == 1) {
+                if (polMain == 5 || polMain == 2) isJ = false;
+                if (mInd != 0) {
+                    cor = 1; t1 = 0; t2 = 2;
+                    add1 = {{1, 0, 0}}; add2 = {{0, 0, 1}};
+                    S->strideMain = 1;
+                } else {
+                    cor = 1; t1 = 2; t2 = 0;
+                    add1 = {{0, 0, 1}}; add2 = {{1, 0, 0}};
+                    S->strideMain = mainGrid.lx;
+                }
+            } else {
+                if (polMain == 3 || polMain == 0) isJ = false;
+                if (mInd != 0) {
+                    cor = 2; t1 = 0; t2 = 1;
+                    add1 = {{1, 0, 0}}; add2 = {{0, 1, 0}};
+                    S->strideMain = 1;
+                } else {
+                    cor = 2; t1 = 1; t2 = 0;
+                    add1 = {{0, 1, 0}}; add2 = {{1, 0, 0}};
+                    S->strideMain = mainGrid.lx * mainGrid.lz;
+                }
+            }
+            S->pf = (Efield ^ isJ ^ plS) ? T.dt / T.d[cor] : -1.0 * T.dt / T.d[cor];
+            S->strideIncd = m[t1];
+            S->addIncdProp = m[t2];
+            std::array<int, 3> lt = T.loc, szT = T.sz;
+            lt[cor] = plS ? (T.loc[cor] + T.sz[cor] - 1) : (Efield ? T.loc[cor] : T.loc[cor] - 1);
+            szT[cor] = 1;
+            // getSurLoc (one rank: procLoc 0, ln_vec - 2 = n, or -1 for a 2D z)
+            std::array<int, 3> sl{{-1, -1, -1}};
+            for (int a = 0; a < 3; ++a) {
+                const int lnm2 = mainGrid.ln(a) - 2;
+                if (lt[a] >= 0 && lt[a] < lnm2) sl[a] = lt[a] + 1;
+                else if (lt[a] < 0 && lt[a] + szT[a] > 0) sl[a] = 1;
+            }
+            if (!is3D) sl[2] = 0;
+            for (int a = 0; a < 3; ++a)
+                if (sl[a] == -1) return nullptr;
+            auto getSzVal = [&](int c, int fe, int sLoc) {
+                if (szT[c] + T.loc[c] > mainGrid.ln(c) - 2) return mainGrid.ln(c) - 2 - sLoc + 1;
+                return T.loc[c] + szT[c] - (0 + sLoc - 1) - fe;
+            };
+            const int tMax = getSzVal(t1, fieldEnd[t1], sl[t1]);
+            S->szTrans[1] = getSzVal(t2, fieldEnd[t2], sl[t2]);
+            if (!is3D) S->szTrans[1] = 1;
+            std::array<int, 3> rel = sl;
+            if (!plS && !Efield) rel[cor] += 1;
+            for (int a = 0; a < 3; ++a) rel[a] = (rel[a] - 1) + 0 - T.originLoc[a];
+            if (Efield && m[cor] > 0 && rel[cor] == 0) mOffSet -= m[cor];
+            const int mOff = static_cast<int>(std::floor(mOffSet - mOffHalf));
+            for (int ll = 0; ll < S->szTrans[1]; ++ll) {
+                int sz0 = 0;
+                while (sz0 < tMax) {
+                    const int start = T.getIr(rel) + T.pmlThick + mOff + 6 * T.mTot + ll * S->addIncdProp + sz0 * S->strideIncd;
+                    const int mainInd = mainGrid.ind(sl[0] + ll * add2[0] + sz0 * add1[0], sl[1] + ll * add2[1] + sz0 * add1[1],
+                                                     sl[2] + ll * add2[2] + sz0 * add1[2]);
+                    const int init = sz0;
+                    sz0 = tMax;  // a background incident line is one run
+                    S->szTrans[0] = sz0 - init;
+                    S->indsU.push_back(start);
+                    S->indsU.push_back(mainInd);
+                }
+            }
+            return S;
+        };
+        struct AddSpec { int comp; bool Efield; int incdJ, incdK; int dirJ, dirK; int ci, cj, ck; };
+        // (grid_i, incd_j, incd_k, j, k, cor_ii/jj/kk, pol_i, pol_j, pol_k)
+        const AddSpec specs[6] = {{0, false, 1, 2, 1, 2, 0, 1, 2}, {1, false, 2, 0, 2, 0, 1, 2, 0}, {2, false, 0, 1, 0, 1, 2, 0, 1},
+                                  {0, true, 1, 2, 1, 2, 0, 1, 2},  {1, true, 2, 0, 2, 0, 1, 2, 0},  {2, true, 0, 1, 0, 1, 2, 0, 1}};
+        for (const AddSpec& a : specs) {
+            DGrid& G = a.Efield ? E[a.comp] : H[a.comp];
+            if (!G.ok()) continue;
+            const int polMain = a.Efield ? a.comp : a.comp + 3;
+            // the incident lines a surface of this component reads: H_incd for E, E_incd for H
+            const int polJ = a.Efield ? a.incdJ + 3 : a.incdJ, polK = a.Efield ? a.incdK + 3 : a.incdK;
+            const bool haveJ = a.Efield ? T.H[a.incdJ].present : T.E[a.incdJ].present;
+            const bool haveK = a.Efield ? T.H[a.incdK].present : T.E[a.incdK].present;
+            auto& out = a.Efield ? T.eSur[a.comp] : T.hSur[a.comp];
+            auto push = [&](std::unique_ptr<TfsfSurf> s) {
+                if (s) out.push_back(*s);
+            };
+            if (T.sz[a.ci] > 1 || nvec(a.ci) <= 3) {
+                if (T.sz[a.cj] > 1 || nvec(2) == 1) {
+                    if (haveJ) {
+                        push(genSurface(G, polMain, polJ, a.dirK, pl, a.Efield));
+                        if (T.sz[a.ck] > 1) push(genSurface(G, polMain, polJ, a.dirK, !pl, a.Efield));
+                    }
+                }
+                if (T.sz[a.ck] > 1 || nvec(2) == 1) {
+                    if (haveK) {
+                        push(genSurface(G, polMain, polK, a.dirJ, pl, a.Efield));
+                        if (T.sz[a.cj] > 1) push(genSurface(G, polMain, polK, a.dirJ, !pl, a.Efield));
+                    }
+                }
+            }
+        }
+        // incident CPMLs (parallelTFSF.hpp:438-467)
+        for (int c = 0; c < 3; ++c) {
+            IncdComp& Hc = T.H[c];
+            if (Hc.present) {
+                auto* gj = T.E[(c + 1) % 3].present ? &T.E[(c + 1) % 3].U : nullptr;
+                auto* gk = T.E[(c + 2) % 3].present ? &T.E[(c + 2) % 3].U : nullptr;
+                makeIncdCPML(Hc.pml, c, false, &Hc.Dv, gj, gk, T, is3D, spec.pmlM, spec.pmlMa, spec.pmlAMax);
+                auto* pj = T.E[(c + 1) % 3].present ? &T.E[(c + 1) % 3].pulse : nullptr;
+                auto* pk = T.E[(c + 2) % 3].present ? &T.E[(c + 2) % 3].pulse : nullptr;
+                makeIncdCPML(Hc.pmlPul, c, false, &Hc.pulse, pj, pk, T, is3D, spec.pmlM, spec.pmlMa, spec.pmlAMax);
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            IncdComp& Ec = T.E[c];
+            if (Ec.present) {
+                auto* gj = T.H[(c + 1) % 3].present ? &T.H[(c + 1) % 3].U : nullptr;
+                auto* gk = T.H[(c + 2) % 3].present ? &T.H[(c + 2) % 3].U : nullptr;
+                makeIncdCPML(Ec.pml, c, true, &Ec.Dv, gj, gk, T, is3D, spec.pmlM, spec.pmlMa, spec.pmlAMax);
+                auto* pj = T.H[(c + 1) % 3].present ? &T.H[(c + 1) % 3].pulse : nullptr;
+                auto* pk = T.H[(c + 2) % 3].present ? &T.H[(c + 2) % 3].pulse : nullptr;
+                makeIncdCPML(Ec.pmlPul, c, true, &Ec.pulse, pj, pk, T, is3D, spec.pmlM, spec.pmlMa, spec.pmlAMax);
+            }
+        }
+        tfsfs.push_back(std::move(tp));
+    }
+}
+
+// updateIncdFieldJ / K / JK (parallelTFSF.cpp:11-45).
+static void updateIncdField(Tfsf& T, IncdComp& C, const cplx prefact, std::vector<cplx>& incdI, const std::vector<IncdParam>& up,
+                            std::vector<cplx>* incdJ, std::vector<cplx>* incdK, IncdCPMLo& pml) {
+    const int start = T.pmlThick + 3 * T.mTot;
+    for (int r = 0; r < T.mTot; ++r) incdI[start + r] = incdI[start + r] + prefact * T.pulseVec[r];
+    for (const IncdParam& p : up) {
+        if (C.mode & 1) {
+            for (int k = 0; k < p.nSz; ++k) incdI[p.indI + k] += p.pj * (*incdJ)[p.indPosK + k];
+            for (int k = 0; k < p.nSz; ++k) incdI[p.indI + k] += -1.0 * p.pj * (*incdJ)[p.indNegK + k];
+        }
+        if (C.mode & 2) {
+            for (int k = 0; k < p.nSz; ++k) incdI[p.indI + k] += p.pk * (*incdK)[p.indPosJ + k];
+            for (int k = 0; k < p.nSz; ++k) incdI[p.indI + k] += -1.0 * p.pk * (*incdK)[p.indNegJ + k];
+        }
+    }
+    incdCPMLUpdate(pml);
+}
+
+// fillPulseVec, parallelTFSF.cpp:72-76.
+static void fillPulseVec(Tfsf& T, const std::vector<double>& dist, double tt) {
+    std::fill(T.pulseVec.begin(), T.pulseVec.end(), 0.0);
+    for (auto& p : T.pul)
+        for (size_t r = 0; r < dist.size(); ++r) T.pulseVec[r] = T.pulseVec[r] + pulseValue(p, tt - dist[r]);
+}
+
+// parallelTFSFBase::step, parallelTFSF.hpp:1148-1174 (no poles: incdD2U is U = D / eps_mu).
+static void tfsfLineStep(Tfsf& T) {
+    for (int ii = 0; ii < 3; ++ii) {
+        IncdComp& C = T.H[ii];
+        if (!C.present) continue;
+        fillPulseVec(T, C.dist, T.t_step * T.dt);
+        auto* ej = T.E[(ii + 1) % 3].present ? &T.E[(ii + 1) % 3].U : nullptr;
+        auto* ek = T.E[(ii + 2) % 3].present ? &T.E[(ii + 2) % 3].U : nullptr;
+        updateIncdField(T, C, T.prefH[ii], C.Dv, C.up, ej, ek, C.pml);
+        auto* pj = T.E[(ii + 1) % 3].present ? &T.E[(ii + 1) % 3].pulse : nullptr;
+        auto* pk = T.E[(ii + 2) % 3].present ? &T.E[(ii + 2) % 3].pulse : nullptr;
+        updateIncdField(T, C, T.prefH[ii], C.pulse, C.upPul, pj, pk, C.pmlPul);
+        for (size_t k = 0; k < C.U.size(); ++k) C.U[k] = C.Dv[k] / C.epmu[k];
+    }
+    for (int ii = 0; ii < 3; ++ii) {
+        IncdComp& C = T.E[ii];
+        if (!C.present) continue;
+        fillPulseVec(T, C.dist, T.t_step * T.dt);
+        auto* hj = T.H[(ii + 1) % 3].present ? &T.H[(ii + 1) % 3].U : nullptr;
+        auto* hk = T.H[(ii + 2) % 3].present ? &T.H[(ii + 2) % 3].U : nullptr;
+        updateIncdField(T, C, T.prefE[ii], C.Dv, C.up, hj, hk, C.pml);
+        auto* pj = T.H[(ii + 1) % 3].present ? &T.H[(ii + 1) % 3].pulse : nullptr;
+        auto* pk = T.H[(ii + 2) % 3].present ? &T.H[(ii + 2) % 3].pulse : nullptr;
+        updateIncdField(T, C, T.prefE[ii], C.pulse, C.upPul, pj, pk, C.pmlPul);
+        for (size_t k = 0; k < C.U.size(); ++k) C.U[k] = C.Dv[k] / C.epmu[k];
+    }
+    ++T.t_step;
+}
+
+// tfsfUpdateFxnReal::addIncdFields (parallelTFSF.cpp:78-84): U += pf Re(incd).
+static void tfsfAdd(Tfsf& T, DGrid& U, const TfsfSurf& S) {
+    const std::vector<cplx>& incd = (S.incdComp < 3) ? T.E[S.incdComp].U : T.H[S.incdComp - 3].U;
+    for (size_t ll = 0; ll < S.indsU.size(); ll += 2)
+        for (int k = 0; k < S.szTrans[0]; ++k)
+            U.v[S.indsU[ll + 1] + k * S.strideMain] += S.pf * std::real(incd[S.indsU[ll] + k * S.strideIncd]);
+}
+
+// The TFSF part of step() (parallelFDTDField.hpp:1236-1256): record the
+// incident H, updateFields (H surfaces, line step, E surfaces), record E.
+void Sim::tfsfStepAll() {
+    for (auto& tp : tfsfs) {
+        Tfsf& T = *tp;
+        const int ri = T.recIdx();
+        for (int c = 0; c < 3; ++c) {
+            const IncdComp& C = T.H[c];
+            Hincd[c][2 * t_step + 0] = C.present ? C.pulse[ri] : cplx(0.0);
+            Hincd[c][2 * t_step + 1] = C.present ? C.pulse[ri - C.distBase % 2] : cplx(0.0);
+        }
+        for (int c = 0; c < 3; ++c)
+            for (auto& S : T.hSur[c]) tfsfAdd(T, H[c], S);
+        tfsfLineStep(T);
+        for (int c = 0; c < 3; ++c)
+            for (auto& S : T.eSur[c]) tfsfAdd(T, E[c], S);
+        for (int c = 0; c < 3; ++c) {
+            const IncdComp& C = T.E[c];
+            Eincd[c][2 * t_step + 0] = C.present ? C.pulse[ri] : cplx(0.0);
+            Eincd[c][2 * t_step + 1] = C.present ? C.pulse[ri - C.distBase % 2] : cplx(0.0);
+        }
+    }
 }
 
 // Source placement, parallelSourceNormal.hpp:69-150 (one rank).
@@ -1386,6 +2017,72 @@ std::vector<std::array<std::vector<cplx>, 4>> Sim::faceFields(int fi) {
     return res;
 }
 
+// The incident column of getFlux (parallelFlux.hpp:455-483) with
+// fluxIncdConv_ from the constructor (parallelFlux.hpp:147-173).
+std::vector<cplx> Sim::getFluxIncd(int fi) {
+    Flux& fx = fluxes[fi];
+    const FluxSpec& F = IP.fluxes[fi];
+    double cx = 1.0, cy = 1.0, cz = 1.0;
+    if (F.SI) {
+        cx = IP.I0 / IP.a * IP.I0 / (EPS0 * IP.a * SPEED_OF_LIGHT);
+        cy = IP.I0 / IP.a * IP.I0 / (EPS0 * IP.a * SPEED_OF_LIGHT);
+        cz = IP.I0 / IP.a * IP.I0 / (EPS0 * IP.a * SPEED_OF_LIGHT);
+    }
+    auto simps2DNorm = [&](int nx, int ny, double dx, double dy) {
+        std::vector<cplx> result(ny, 0.0);
+        for (int jj = 0; jj < ny; ++jj) {
+            std::vector<cplx> ones(nx, cplx(1.0, 0.0));
+            result[jj] = simps(ones.data(), nx, dx);
+        }
+        return simps(result.data(), ny, dy);
+    };
+    const auto& sz = fx.sz;
+    const auto& d = IP.d;
+    if (!F.crossSec) {
+        if (is3D) {
+            cx *= (sz[2] > 1 && sz[1] > 1) ? std::real(simps2DNorm(sz[1], sz[2], d[1], d[2])) : 0.0;
+            cy *= (sz[0] > 1 && sz[2] > 1) ? std::real(simps2DNorm(sz[0], sz[2], d[0], d[2])) : 0.0;
+            cz *= (sz[0] > 1 && sz[1] > 1) ? std::real(simps2DNorm(sz[0], sz[1], d[0], d[1])) : 0.0;
+        } else {
+            std::vector<cplx> szNorm(sz[0], cplx(1.0, 0.0));
+            cy *= std::real(simps(szNorm.data(), static_cast<int>(szNorm.size()), d[0]));
+            szNorm = std::vector<cplx>(sz[1], cplx(1.0, 0.0));
+            cx *= std::real(simps(szNorm.data(), static_cast<int>(szNorm.size()), d[1]));
+            cz = 0.0;
+        }
+    }
+    const int nt = fx.t_step;
+    std::vector<cplx> out;
+    auto gi = [](cplx a) { return cplx(std::real(a), 0.0); };  // getIncdField_ (parallelFlux.cpp:16)
+    for (double fr : fx.freqs) {
+        cplx Ex(0.0), Ey(0.0), Ez(0.0), Hx(0.0), Hy(0.0), Hz(0.0);
+        auto ph = [&](size_t tt) {
+            return std::exp(cplx(0.0, fr * static_cast<double>(tt / 2) * (dt / static_cast<double>(fx.timeInt))));
+        };
+        for (size_t tt = 0; tt < Eincd[0].size(); tt += 2) {
+            Ex += gi(0.5 * Eincd[0][tt]) * ph(tt);
+            Ey += gi(0.5 * Eincd[1][tt]) * ph(tt);
+            Ez += gi(0.5 * Eincd[2][tt]) * ph(tt);
+            Hx += gi(0.5 * Hincd[0][tt]) * ph(tt);
+            Hy += gi(0.5 * Hincd[1][tt]) * ph(tt);
+            Hz += gi(0.5 * Hincd[2][tt]) * ph(tt);
+        }
+        for (size_t tt = 1; tt < Eincd[0].size(); tt += 2) {
+            Ex += gi(0.5 * Eincd[0][tt]) * ph(tt);
+            Ey += gi(0.5 * Eincd[1][tt]) * ph(tt);
+            Ez += gi(0.5 * Eincd[2][tt]) * ph(tt);
+            Hx += gi(0.5 * Hincd[0][tt]) * ph(tt);
+            Hy += gi(0.5 * Hincd[1][tt]) * ph(tt);
+            Hz += gi(0.5 * Hincd[2][tt]) * ph(tt);
+        }
+        cplx fi2 = std::pow(cx * (Ey * std::conj(Hz) - Ez * std::conj(Hy)) / (std::pow(static_cast<double>(nt * fx.timeInt), 2.0)), 2.0);
+        fi2 += std::pow(cy * (Ez * std::conj(Hx) - Ex * std::conj(Hz)) / (std::pow(static_cast<double>(nt * fx.timeInt), 2.0)), 2.0);
+        fi2 += std::pow(cz * (Ex * std::conj(Hy) - Ey * std::conj(Hx)) / (std::pow(static_cast<double>(nt * fx.timeInt), 2.0)), 2.0);
+        out.push_back(std::sqrt(fi2));
+    }
+    return out;
+}
+
 std::vector<cplx> Sim::getFlux(int fi) {
     Flux& fx = fluxes[fi];
     int nf = static_cast<int>(fx.freqs.size());
@@ -1723,6 +2420,8 @@ void Sim::step() {
             }
         });
     }
+    // TFSF: H surfaces, incident line step, E surfaces (parallelFDTDField.hpp:1236-1256)
+    tfsfStepAll();
     for (int ii = 0; ii < 3; ++ii)
         if (HPML[ii] && H[ii].ok()) HPML[ii]->updateGrid();
     // Soft sources: U += dt * Re(sum of pulses at tcur).
@@ -1921,6 +2620,37 @@ int oracle_detector(void* hp, int d, double* times, double* values, int cap_samp
     return ns;
 }
 
+// TFSF surface t's incident line of component comp (0..5 = Ex..Hz): which 0
+// is E_incd/H_incd, 1 the D/B line, 2 the pulse line; returns its length (0
+// when the component has no line).
+int oracle_tfsf_line(void* hp, int t, int comp, int which, double* re, double* im, int cap) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    if (t < 0 || t >= static_cast<int>(S.tfsfs.size()) || comp < 0 || comp > 5) return -1;
+    orc::IncdComp& C = comp < 3 ? S.tfsfs[t]->E[comp] : S.tfsfs[t]->H[comp - 3];
+    if (!C.present) return 0;
+    const std::vector<chiml::cplx>& v = which == 0 ? C.U : (which == 1 ? C.Dv : C.pulse);
+    const int n = static_cast<int>(v.size());
+    for (int i = 0; i < n && i < cap; ++i) {
+        re[i] = v[i].real();
+        im[i] = v[i].imag();
+    }
+    return n;
+}
+
+// The incident records E_incd_ / H_incd_ (2 per step: the point and its
+// offset) of component comp (0..5); returns their count.
+int oracle_incd_record(void* hp, int comp, double* re, double* im, int cap) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    if (comp < 0 || comp > 5) return -1;
+    const std::vector<chiml::cplx>& v = comp < 3 ? S.Eincd[comp] : S.Hincd[comp - 3];
+    const int n = static_cast<int>(v.size());
+    for (int i = 0; i < n && i < cap; ++i) {
+        re[i] = v[i].real();
+        im[i] = v[i].imag();
+    }
+    return n;
+}
+
 // Writes flux f's face fields as saveFields does (getFlux with save, parallelFlux.hpp:535-536).
 int oracle_flux_save(void* hp, int f, const char* path) {
     auto& h = *static_cast<OracleHandle*>(hp);
@@ -1931,6 +2661,17 @@ int oracle_flux_save(void* hp, int f, const char* path) {
         return -1;
     }
     return 0;
+}
+
+int oracle_flux_incd(void* hp, int f, double* re, double* im, int cap) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    auto v = S.getFluxIncd(f);
+    const int n = static_cast<int>(v.size());
+    for (int i = 0; i < n && i < cap; ++i) {
+        re[i] = v[i].real();
+        im[i] = v[i].imag();
+    }
+    return n;
 }
 
 int oracle_flux(void* hp, int f, double* re, double* im, int cap) {

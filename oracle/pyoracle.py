@@ -53,6 +53,12 @@ def lib():
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_flux_save.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]
+        L.oracle_tfsf_line.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.oracle_flux_incd.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.oracle_incd_record.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.oracle_num_freq_detectors.argtypes = [ctypes.c_void_p]
         L.oracle_freq_info.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4
         L.oracle_freq_fields.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
@@ -149,10 +155,34 @@ class Oracle:
         lib().oracle_freq_values(self._h, d, _dp(re), _dp(im), _dp(fr))
         return fr, (re + 1j * im).reshape(ng, nf)
 
+    def tfsf_line(self, t, comp, which=0, nmax=1 << 16):
+        """TFSF surface t's incident line of component comp (0..5): which 0 =
+        E_incd/H_incd, 1 = D/B, 2 = the pulse line (None without one)."""
+        re = np.zeros(nmax)
+        im = np.zeros(nmax)
+        n = lib().oracle_tfsf_line(self._h, t, comp, which, _dp(re), _dp(im), nmax)
+        if n < 0:
+            raise IndexError("no such TFSF surface or component")
+        return (re[:n] + 1j * im[:n]) if n else None
+
+    def incd_record(self, comp, nmax=1 << 22):
+        """E_incd_ / H_incd_ records of component comp (0..5): 2 per step."""
+        re = np.zeros(nmax)
+        im = np.zeros(nmax)
+        n = lib().oracle_incd_record(self._h, comp, _dp(re), _dp(im), nmax)
+        return re[:n] + 1j * im[:n]
+
     def flux_save(self, f, path):
         """Writes flux f's face fields as the reference's saveFields does."""
         if lib().oracle_flux_save(self._h, f, str(path).encode()) != 0:
             raise RuntimeError(lib().oracle_last_error().decode())
+
+    def flux_incd(self, f, nmax=4096):
+        """Incident flux per frequency (getFlux's incident column)."""
+        re = np.zeros(nmax)
+        im = np.zeros(nmax)
+        n = lib().oracle_flux_incd(self._h, f, _dp(re), _dp(im), nmax)
+        return re[:n] + 1j * im[:n]
 
     def flux(self, f, nmax=4096):
         re = np.zeros(nmax)

@@ -63,7 +63,9 @@ def test_estimate_monotone():
 @pytest.mark.parametrize("mutate,message", [
     (lambda d: d["CompCell"].__setitem__("PBC", True), "periodic"),
     (lambda d: d["CompCell"].__setitem__("cplxFields", True), "complex fields"),
-    (lambda d: d["TFSF"].append({"size": [1, 1, 0], "loc": [0, 0, 0]}), "TFSF"),
+    (lambda d: d["TFSF"].append({"size": [1, 1, 0], "loc": [0, 0, 0], "theta": 200.0}), "theta value for the TFSF"),
+    (lambda d: d["TFSF"].append({"size": [0.5, 0.5, 0], "loc": [0, 0, 0], "m": [0, 0, 0]}), "direction of propgation"),
+    (lambda d: d["TFSF"].append({"size": [0.5, 0.5, 0], "loc": [9.0, 0, 0], "m": [0, 1, 0]}), "outside the FDTD cell"),
     (lambda d: d["GPU"].__setitem__("precision", "bf16"), "precision"),
     (lambda d: d["CompCell"].__setitem__("dt", 1.0), "stable time step"),
     (lambda d: d["PML"].__setitem__("thickness", [5.0, 5.0, 0.0]), "PML size is larger"),

@@ -1811,3 +1811,123 @@ def test_flux_load_checks_the_file(tmp_path):
     bad["FluxList"][0]["load"] = True
     with pytest.raises(Exception, match="without a valid path"):
         Oracle(json.dumps(bad))
+
+
+# ------------------------------------------------------------------ TFSF ----
+def _tfsf_run(n3, box, steps, **kw):
+    deck = decks.tfsf_deck(n3, steps, box, pml=kw.pop("pml", 6), **kw)
+    o = Oracle(deck)
+    o.step(steps)
+    return o
+
+
+def _tf_region(n3, box):
+    """Boolean masks (z, y, x) of the total-field box (one point of margin,
+    the staggered components' first points outside it) and of its complement."""
+    ctr = [(n - 1) // 2 for n in n3]
+    inside = np.zeros((n3[2], n3[1], n3[0]), dtype=bool)
+    core = np.zeros_like(inside)
+    sl, sc = [], []
+    for a in (2, 1, 0):
+        if n3[a] == 1:
+            sl.append(slice(None))
+            sc.append(slice(None))
+        else:
+            h = (box[a] - 1) // 2
+            sl.append(slice(ctr[a] - h - 1, ctr[a] + h + 2))
+            sc.append(slice(ctr[a] - h + 2, ctr[a] + h - 1))
+    inside[tuple(sl)] = True
+    core[tuple(sc)] = True
+    return inside, core
+
+
+@pytest.mark.parametrize("n3,box,steps,kw", [
+    ((80, 80, 1), (31, 31, 1), 120, dict(m=(0, 1, 0))),
+    ((80, 80, 1), (31, 31, 1), 120, dict(m=(1, 1, 0))),
+    ((80, 80, 1), (31, 31, 1), 120, dict(m=(2, 1, 0))),
+    ((34, 34, 34), (15, 15, 15), 45, dict(m=(0, 0, 1))),
+    ((34, 34, 34), (15, 15, 15), 45, dict(m=(1, 1, 1), psi=30.0)),
+    ((34, 34, 34), (15, 15, 15), 45, dict(m=(1, 2, 1), circ="L")),
+])
+def test_tfsf_scattered_field_region_stays_empty(n3, box, steps, kw):
+    """In vacuum the surfaces' corrections cancel the incident wave outside
+    the box to rounding: the incident line is the main grid's own update
+    along m (the Tan-Potter multi-point line matches the grid's dispersion),
+    so a wrong surface sign, a line index off by one or a missing edge term
+    leaves a scattered field of order the wave itself."""
+    o = _tfsf_run(n3, box, steps, **kw)
+    inside, _ = _tf_region(n3, box)
+    comps = (2, 3, 4) if n3[2] == 1 else range(6)
+    f = {c: o.field(c) for c in comps}
+    peak = max(np.abs(f[c]).max() for c in comps)
+    assert peak > 0.0
+    leak = max(np.abs(f[c][~inside]).max() for c in comps)
+    assert leak <= 1e-13 * peak, leak / peak
+
+
+def test_tfsf_interior_is_the_incident_line():
+    """Normal incidence along z: inside the box E_y depends on z only and, at
+    every interior point, equals the incident line's real part at one fixed
+    offset (the total field is the incident plane wave itself)."""
+    n3, box = (34, 34, 34), (15, 15, 15)
+    o = _tfsf_run(n3, box, 40, m=(0, 0, 1))
+    ey = o.field(1)
+    line = o.tfsf_line(0, 1).real
+    _, core = _tf_region(n3, box)
+    zs = np.where(core.any(axis=(1, 2)))[0]
+    ys = np.where(core.any(axis=(0, 2)))[0]
+    xs = np.where(core.any(axis=(0, 1)))[0]
+    col = ey[zs, ys[len(ys) // 2], xs[len(xs) // 2]]
+    assert np.abs(col).max() > 0.0
+    block = ey[np.ix_(zs, ys, xs)]
+    assert np.abs(block - col[:, None, None]).max() <= 1e-13 * np.abs(col).max()
+    # the one shift s with line[z + s] == E_y(z)
+    errs = [np.abs(line[zs + s] - col).max() for s in range(-zs[0], len(line) - zs[-1] - 1)]
+    best = int(np.argmin(errs))
+    assert errs[best] <= 1e-13 * np.abs(col).max()
+    assert sorted(errs)[1] > 1e-3 * np.abs(col).max()  # and no other
+
+
+@pytest.mark.parametrize("m,psi", [((1, 2, 1), 60.0), ((1, 1, 0), 30.0), ((0, 1, 1), 0.0)])
+def test_tfsf_polarisation_and_impedance(m, psi):
+    """Inside the box the plane wave is transverse (E.k = 0), H = k x E (unit
+    impedance in these units) and E lies at the angle psi from k x z towards
+    -k x (k x z): E ~ cos(psi) (sin phi, -cos phi, 0) + sin(psi) (-cos theta
+    cos phi, -cos theta sin phi, sin theta).  The Yee components are averaged
+    to the cell corner and E to H's half step (second order in a 100-cell
+    wavelength)."""
+    n3, box, steps = (40, 40, 40), (19, 19, 19), 80
+    deck = decks.tfsf_deck(n3, steps, box, pml=6, m=m, psi=psi)
+    o = Oracle(deck)
+    o.step(steps - 1)
+    e_prev = np.stack([o.field(c) for c in range(3)])
+    o.step(1)
+    e_now = np.stack([o.field(c) for c in range(3)])
+    H = np.stack([o.field(c) for c in range(3, 6)])
+    Eh = 0.5 * (e_prev + e_now)  # E at H's time level
+
+    def corner(F, comp, z, y, x):
+        # (z, y, x) arrays; E_c sits half a cell up along c, H_c half a cell up
+        # along the other two axes
+        ax = {0: 2, 1: 1, 2: 0}  # field axis -> array axis
+        pts = [(z, y, x)]
+        offs = [comp] if comp < 3 else [a for a in range(3) if a != comp - 3]
+        for a in offs:
+            pts = pts + [tuple(q - (1 if i == ax[a] else 0) for i, q in enumerate(p)) for p in pts]
+        return np.mean([F[p] for p in pts])
+
+    k = np.array(m, float) / np.linalg.norm(m)
+    th = math.acos(k[2])
+    ph = math.atan2(k[1], k[0]) if (k[0] or k[1]) else math.pi / 2.0
+    v = math.cos(math.radians(psi)) * np.array([math.sin(ph), -math.cos(ph), 0.0]) + \
+        math.sin(math.radians(psi)) * np.array([-math.cos(th) * math.cos(ph), -math.cos(th) * math.sin(ph), math.sin(th)])
+    _, core = _tf_region(n3, box)
+    mag = np.linalg.norm(Eh, axis=0) * core
+    z, y, x = np.unravel_index(np.argmax(mag), mag.shape)
+    e = np.array([corner(Eh[c], c, z, y, x) for c in range(3)])
+    h = np.array([corner(H[c], c + 3, z, y, x) for c in range(3)])
+    ne = np.linalg.norm(e)
+    assert ne > 0.0
+    assert abs(np.dot(e, k)) <= 1e-2 * ne
+    assert np.linalg.norm(h - np.cross(k, e)) <= 1e-2 * ne
+    assert abs(abs(np.dot(e / ne, v)) - 1.0) <= 1e-3
