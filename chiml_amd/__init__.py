@@ -48,6 +48,7 @@ def load_library():
     L.chiml_sim_freq_values.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t]
     L.chiml_sim_pops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_size_t, c_size_p]
     L.chiml_sim_incd.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t, c_size_p]
+    L.chiml_sim_freq_incd.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
     L.chiml_sim_flux_incd.argtypes = [ctypes.c_void_p, ctypes.c_int, c_dbl_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
     L.chiml_sim_density.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_dbl_p, c_int_p]
     L.chiml_sim_cost.argtypes = [ctypes.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]
@@ -75,6 +76,9 @@ def load_library():
     L.chiml_write_detector_txt.argtypes = [ctypes.c_char_p, c_dbl_p, c_dbl_p, ctypes.c_size_t, ctypes.c_int, c_dbl_p]
     L.chiml_write_detector_bin.argtypes = [ctypes.c_char_p, c_dbl_p, c_dbl_p, ctypes.c_size_t, c_int_p, c_int_p]
     L.chiml_write_flux.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int]
+    L.chiml_write_freq_table_incd.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int, c_dbl_p]
+    L.chiml_write_freq_maps_incd.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), c_dbl_p,
+                                             ctypes.c_int, c_dbl_p]
     L.chiml_write_flux_incd.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int, c_dbl_p]
     L.chiml_write_freq_table.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, c_dbl_p, ctypes.c_int]
     L.chiml_write_freq_maps.argtypes = [ctypes.c_char_p, c_dbl_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -208,6 +212,17 @@ class FreqDetector:
             _check(load_library().chiml_sim_freq_fields(self._sim._h, self.index, _dp(buf), buf.size))
         n = ng * npts * fr.size
         return (buf[0:2 * n:2] + 1j * buf[1:2 * n:2]).reshape(ng, npts, fr.size)
+
+    def incident(self):
+        """Incident values of the table, (nincd, nfreq) complex (zero without
+        a TFSF surface)."""
+        L = load_library()
+        ni = ctypes.c_int()
+        _check(L.chiml_sim_freq_incd(self._sim._h, self.index, None, 0, ctypes.byref(ni)))
+        nf = self.info()[0].size
+        buf = np.zeros(2 * ni.value * nf)
+        _check(L.chiml_sim_freq_incd(self._sim._h, self.index, _dp(buf), buf.size, ctypes.byref(ni)))
+        return (buf[0::2] + 1j * buf[1::2]).reshape(ni.value, nf)
 
     def values(self):
         """The table's values, shape (grids, nfreq)."""

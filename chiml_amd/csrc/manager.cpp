@@ -556,6 +556,8 @@ class FDTDField {
     int freqNt(int d);
     // toFile values: [grid][freq] (conv/2 sum F / nt, or conv/2 sum |F|^2 / nt^2)
     std::vector<cplx> freqValues(int d);
+    // toFile / toMap incident values [incident field][freq] from the TFSF records
+    std::vector<cplx> freqIncd(int d, int* nincd);
     // toMap values: [grid][freq][point jj] over the detector box
     std::vector<cplx> freqMapValues(int d);
 
@@ -1580,6 +1582,42 @@ static cplx simpson(const cplx* f, int n, double d) {
     return r;
 }
 
+// The incident fields of a frequency detector's table (main.cpp:77-94 picks
+// them by type; toFile / toMap, parallelDTC_FREQ.hpp:414-427, 529-544): per
+// incident component the halved real parts of its records phased at
+// exp(-i w step dt / timeInt), over t_step timeInt samples, squared for
+// power, times convFactor / 2.
+std::vector<cplx> FDTDField::freqIncd(int d, int* nincd) {
+    const FreqDet& fd = freqDets.at(d);
+    const chiml::DtcType ty = IP.dtcs.at(fd.deck).type;
+    std::vector<int> comps;
+    if (ty == chiml::DtcType::HPOW) comps = {3, 4, 5};
+    else if (ty == chiml::DtcType::EPOW) comps = {0, 1, 2};
+    else comps = {static_cast<int>(ty)};
+    *nincd = static_cast<int>(comps.size());
+    const int nf = static_cast<int>(fd.freqs.size());
+    std::vector<cplx> out(comps.size() * nf, cplx(0.0, 0.0));
+    const int nt = freqNt(d);
+    for (size_t q = 0; q < comps.size(); ++q) {
+        size_t n = 0;
+        if (chiml_dev_read_incd(dev_, comps[q], nullptr, 0, &n) != CHIML_OK) throw std::runtime_error(chiml_dev_last_error(dev_));
+        std::vector<double> b(2 * n);
+        if (chiml_dev_read_incd(dev_, comps[q], b.data(), b.size(), &n) != CHIML_OK) throw std::runtime_error(chiml_dev_last_error(dev_));
+        for (int k = 0; k < nf; ++k) {
+            cplx pt(0.0, 0.0);
+            for (int par = 0; par < 2; ++par)
+                for (size_t tt = par; tt < n; tt += 2)
+                    pt += std::real(0.5 * cplx(b[2 * tt], b[2 * tt + 1])) *
+                          std::exp(cplx(0.0, -1.0 * fd.freqs[k] * static_cast<double>(tt / 2) * (IP.dt / static_cast<double>(fd.timeInt))));
+            pt /= static_cast<double>(nt * fd.timeInt);
+            if (fd.pow) pt *= std::conj(pt);
+            pt *= fd.conv / 2.0;
+            out[q * nf + k] = pt;
+        }
+    }
+    return out;
+}
+
 int FDTDField::freqNt(int d) {
     const FreqDet& fd = freqDets.at(d);
     int steps = 0;
@@ -2079,6 +2117,23 @@ int chiml_sim_flux(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq) 
     }
 }
 
+int chiml_sim_freq_incd(chiml_sim_t* s, int d, double* reim, size_t cap, int* nincd) {
+    if (!s || !nincd) return sim_fail("null argument");
+    try {
+        if (d < 0 || d >= static_cast<int>(s->f->freqDets.size())) return sim_fail("frequency detector index out of range");
+        auto v = s->f->freqIncd(d, nincd);
+        if (!reim) return CHIML_OK;
+        if (cap < 2 * v.size()) return sim_fail("buffer too small");
+        for (size_t k = 0; k < v.size(); ++k) {
+            reim[2 * k] = v[k].real();
+            reim[2 * k + 1] = v[k].imag();
+        }
+    } catch (std::exception& e) {
+        return sim_fail(e.what(), CHIML_E_HIP);
+    }
+    return CHIML_OK;
+}
+
 int chiml_sim_flux_incd(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq) {
     if (!s || !nfreq) return sim_fail("null argument");
     try {
@@ -2173,10 +2228,19 @@ int chiml_sim_write_outputs(chiml_sim_t* s, const char* prefix) {
                 reim.push_back(c.imag());
             }
             const std::string path = place(fd.name);
-            int rc = fd.maps ? chiml_write_freq_maps(path.c_str(), freqs.data(), static_cast<int>(freqs.size()), static_cast<int>(fd.comps.size()),
-                                                     fd.sz.data(), reim.data(), nincd)
-                             : chiml_write_freq_table(path.c_str(), freqs.data(), static_cast<int>(freqs.size()),
-                                                      static_cast<int>(fd.comps.size()), reim.data(), nincd);
+            std::vector<double> ireim;
+            if (F.ntfsf > 0) {
+                int ni = 0;
+                for (const chiml::cplx& v : F.freqIncd(static_cast<int>(d), &ni)) {
+                    ireim.push_back(v.real());
+                    ireim.push_back(v.imag());
+                }
+            }
+            const double* ip = ireim.empty() ? nullptr : ireim.data();
+            int rc = fd.maps ? chiml_write_freq_maps_incd(path.c_str(), freqs.data(), static_cast<int>(freqs.size()),
+                                                          static_cast<int>(fd.comps.size()), fd.sz.data(), reim.data(), nincd, ip)
+                             : chiml_write_freq_table_incd(path.c_str(), freqs.data(), static_cast<int>(freqs.size()),
+                                                           static_cast<int>(fd.comps.size()), reim.data(), nincd, ip);
             if (rc) return rc;
         }
         for (size_t f = 0; f < F.fluxes.size(); ++f) {

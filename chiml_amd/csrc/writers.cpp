@@ -132,6 +132,11 @@ int chiml_write_flux_incd(const char* path, const double* freqs, int nfreq, int 
 // parallelDetectorFREQ_Base::toFile(incd, dt) (parallelDTC_FREQ.hpp:398-444)
 // on one process: nincd incident fields, each zero without a TFSF surface.
 int chiml_write_freq_table(const char* path, const double* freqs, int nfreq, int ngrid, const double* vals_reim, int nincd) {
+    return chiml_write_freq_table_incd(path, freqs, nfreq, ngrid, vals_reim, nincd, nullptr);
+}
+
+int chiml_write_freq_table_incd(const char* path, const double* freqs, int nfreq, int ngrid, const double* vals_reim, int nincd,
+                                const double* incd_reim) {
     if (nfreq < 0 || ngrid < 1 || nincd < 0 || (nfreq > 0 && (!freqs || !vals_reim))) return fail("null argument");
     using cplx = std::complex<double>;
     std::ofstream f;
@@ -146,9 +151,11 @@ int chiml_write_freq_table(const char* path, const double* freqs, int nfreq, int
         f << std::setw(16) << std::setprecision(12) << freqs[ii];
         cplx incd(0.0, 0.0);
         for (int k = 0; k < nincd; ++k) {
-            f << "\t" << std::setw(16) << std::setprecision(12) << std::real(zero) << "\t" << std::setw(16) << std::setprecision(12)
-              << std::imag(zero) << "\t" << std::setw(16) << std::setprecision(12) << std::abs(zero);
-            incd += zero;
+            const size_t q = (static_cast<size_t>(k) * nfreq + ii) * 2;
+            const cplx pt = incd_reim ? cplx(incd_reim[q], incd_reim[q + 1]) : zero;
+            f << "\t" << std::setw(16) << std::setprecision(12) << std::real(pt) << "\t" << std::setw(16) << std::setprecision(12)
+              << std::imag(pt) << "\t" << std::setw(16) << std::setprecision(12) << std::abs(pt);
+            incd += pt;
         }
         if (nincd > 2)
             f << "\t" << std::setw(16) << std::setprecision(12) << std::real(incd) << "\t" << std::setw(16) << std::setprecision(12)
@@ -174,6 +181,11 @@ int chiml_write_freq_table(const char* path, const double* freqs, int nfreq, int
 // one process: one file per frequency, <path>.<frequency as %f>.
 int chiml_write_freq_maps(const char* path, const double* freqs, int nfreq, int ngrid, const int sz[3], const double* vals_reim,
                           int nincd) {
+    return chiml_write_freq_maps_incd(path, freqs, nfreq, ngrid, sz, vals_reim, nincd, nullptr);
+}
+
+int chiml_write_freq_maps_incd(const char* path, const double* freqs, int nfreq, int ngrid, const int sz[3], const double* vals_reim,
+                               int nincd, const double* incd_reim) {
     if (!path || !sz || nfreq < 0 || ngrid < 1 || nincd < 0 || (nfreq > 0 && (!freqs || !vals_reim))) return fail("null argument");
     using cplx = std::complex<double>;
     const size_t npt = static_cast<size_t>(sz[0]) * sz[1] * sz[2];
@@ -190,10 +202,12 @@ int chiml_write_freq_maps(const char* path, const double* freqs, int nfreq, int 
         cplx incd(0.0, 0.0);
         const cplx zero(0.0, 0.0);
         for (int k = 0; k < nincd; ++k) {
-            incdOut.push_back(std::real(zero));
-            incdOut.push_back(std::imag(zero));
-            incdOut.push_back(std::abs(zero));
-            incd += zero;
+            const size_t q = (static_cast<size_t>(k) * nfreq + ii) * 2;
+            const cplx pt = incd_reim ? cplx(incd_reim[q], incd_reim[q + 1]) : zero;
+            incdOut.push_back(std::real(pt));
+            incdOut.push_back(std::imag(pt));
+            incdOut.push_back(std::abs(pt));
+            incd += pt;
         }
         if (nincd > 2) {
             incdOut.push_back(std::real(incd));

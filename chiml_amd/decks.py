@@ -352,7 +352,7 @@ def sweep_requests(n_req=512, seed=5, rate_hz=20.0, steps_range=(1000, 10000), e
 
 
 def tfsf_deck(n3, steps, box, m=None, angles=None, psi=90.0, pml=12, d=0.05, precision="fp64", fcen_nm=500.0,
-              circ=None, ndet=0, seed=7, tfsf_pml=20, objects=None, flux=None):
+              circ=None, ndet=0, seed=7, tfsf_pml=20, objects=None, flux=None, fwidth_ratio=0.5, cutoff=4.0):
     """A TFSF surface (the reference's "TFSF" list, parallelInputs.cpp:219-408)
     centred in a vacuum cell of n3 points (n3[2] = 1 for 2D TMz): box = points
     per axis of the total-field region, m = integer propagation vector or
@@ -362,7 +362,7 @@ def tfsf_deck(n3, steps, box, m=None, angles=None, psi=90.0, pml=12, d=0.05, pre
     deck, size, dt = _cell(n3, d, 0.5, steps, pml, precision)
     fcen = A_UNIT / (fcen_nm * 1e-9)
     t = {"size": [(b - 1) * d if n > 1 else 0.0 for b, n in zip(box, n3)], "loc": [0.0, 0.0, 0.0],
-         "psi": psi, "pmlThick": tfsf_pml, "PulseList": [gaussian_pulse(fcen, 0.5 * fcen, 4.0, 1.0)]}
+         "psi": psi, "pmlThick": tfsf_pml, "PulseList": [gaussian_pulse(fcen, fwidth_ratio * fcen, cutoff, 1.0)]}
     if m is not None:
         t["m"] = list(m)
     else:

@@ -52,6 +52,12 @@ int chiml_write_flux_incd(const char* path, const double* freqs, int nfreq, int 
  * of each grid's value and, with more than one grid, of their sum.
  * vals_reim[(grid * nfreq + f) * 2 + {0,1}]. */
 int chiml_write_freq_table(const char* path, const double* freqs, int nfreq, int ngrid, const double* vals_reim, int nincd);
+/* The same table with the incident values (incd_reim[(k * nfreq + f) * 2 +
+ * {0,1}] for incident field k: the DFT of its TFSF records over the
+ * detector's samples, squared for power, times conv / 2,
+ * parallelDTC_FREQ.hpp:414-427; NULL for zero). */
+int chiml_write_freq_table_incd(const char* path, const double* freqs, int nfreq, int ngrid, const double* vals_reim, int nincd,
+                                const double* incd_reim);
 
 /* Frequency detector maps (parallelDetectorFREQ_Base::toMap with incident
  * fields, src/DTC/parallelDTC_FREQ.hpp:506-580): one file per frequency,
@@ -61,6 +67,10 @@ int chiml_write_freq_table(const char* path, const double* freqs, int nfreq, int
  * jj = x + z sz[0] + y sz[0] sz[2]. */
 int chiml_write_freq_maps(const char* path, const double* freqs, int nfreq, int ngrid, const int sz[3], const double* vals_reim,
                           int nincd);
+/* The same maps with the incident values laid out as for
+ * chiml_write_freq_table_incd (parallelDTC_FREQ.hpp:529-552). */
+int chiml_write_freq_maps_incd(const char* path, const double* freqs, int nfreq, int ngrid, const int sz[3], const double* vals_reim,
+                               int nincd, const double* incd_reim);
 
 /* Population detector (QEPopDtc::toFile, src/ML/QEPopDtc.cpp:36-61): per
  * sample tt the time tt * interval * dt and real / imag / abs of the mean

@@ -66,6 +66,9 @@ int chiml_sim_incd(chiml_sim_t* s, int comp, double* reim, size_t cap, size_t* n
 /* Incident flux of flux box f per frequency, the incident columns of its table
  * (parallelFlux.hpp:455-483, 521): nfreq complex values; zero without a TFSF surface. */
 int chiml_sim_flux_incd(chiml_sim_t* s, int f, double* reim, size_t cap, int* nfreq);
+/* Incident values of frequency detector d's table, [nincd][nfreq] complex
+ * (3 fields for Epow / Hpow, else 1; parallelDTC_FREQ.hpp:414-427). */
+int chiml_sim_freq_incd(chiml_sim_t* s, int d, double* reim, size_t cap, int* nincd);
 int chiml_sim_density(chiml_sim_t* s, int q, int member, int emitter, double* reim, int* nlevel);
 /* Cell-updates per step and the algorithmic HBM bytes per step of the field
  * kernels (update_h + update_e), used for the roofline figure. */

@@ -13,6 +13,10 @@
 //   * Lorentz/Drude ADE in D form (UpdateLorPol + DtoU,
 //     src/UTIL/FDTD_up_eq.cpp:425-448, 838-848),
 //   * soft sources (src/SOURCE/parallelSourceNormal.{hpp:69-150,cpp:15-37}),
+//   * TFSF surfaces with their incident lines and incident CPML
+//     (src/SOURCE/parallelTFSF.{hpp:100-1309,cpp:11-92}, src/PML/PML.{hpp,cpp})
+//     for a vacuum incident medium, and the incident columns of the flux and
+//     frequency tables (parallelFlux.hpp:455-483, parallelDTC_FREQ.hpp:414-427),
 //   * point detectors (src/DTC/parallelDTC.hpp:44-95, parallelDTC_TXT.cpp:25-54),
 //   * DFT flux boxes (src/DTC/parallelFlux.{cpp:13-71,338-543},
 //     parallelFlux.hpp:296-615, parallelStorageFreqDTC.cpp:21-30),
@@ -57,6 +61,14 @@
 //   frequency detectors               per-point DFT of a seeded oblique discrete plane
 //                                     wave in closed form (1e-11), point order, table
 //                                     values and SI factors; 2D samples nothing
+//   TFSF surfaces                     scattered-field region below 1e-13 of the peak
+//                                     (2D y and oblique, 3D normal, (1,1,1), (1,2,1)
+//                                     elliptical); the interior equals the incident
+//                                     line at one offset; interior wave transverse,
+//                                     H = k x E, E at psi from k x z; incident flux =
+//                                     flux of the total field through a plane in the
+//                                     box (2e-6); frequency-table incident values =
+//                                     detected field (1e-5)
 //   grid / time step                  n = floor(size/d + 0.5) + 1, 2D dt with the
 //                                     z term, ceil(tLim/dt) steps
 //   sources                           cell placement, added after the H update
@@ -2596,6 +2608,41 @@ int oracle_freq_fields(void* hp, int d, double* re, double* im, int cap) {
 }
 
 // The table's values [grid][freq] and the printed frequencies.
+// The incident values of frequency detector d's table (main.cpp:77-94 picks
+// the incident fields by type; toFile, parallelDTC_FREQ.hpp:414-427):
+// [field][freq]; returns the number of fields.
+int oracle_freq_incd(void* hp, int d, double* re, double* im) {
+    auto& S = *static_cast<OracleHandle*>(hp)->sim;
+    auto& Q = S.freqDtcs.at(d);
+    std::vector<std::vector<chiml::cplx>*> incd;
+    if (Q.type == chiml::DtcType::HPOW) incd = {&S.Hincd[0], &S.Hincd[1], &S.Hincd[2]};
+    else if (Q.type == chiml::DtcType::EPOW) incd = {&S.Eincd[0], &S.Eincd[1], &S.Eincd[2]};
+    else {
+        const int c = static_cast<int>(Q.type);
+        incd = {c < 3 ? &S.Eincd[c] : &S.Hincd[c - 3]};
+    }
+    const double dt = S.dt;
+    auto gi = [](chiml::cplx a) { return std::real(a); };  // getIncdField_ (parallelDTC_FREQ.cpp:16)
+    size_t o = 0;
+    for (auto* fp : incd) {
+        const std::vector<chiml::cplx>& field = *fp;
+        for (double fr : Q.freqs) {
+            chiml::cplx pt(0.0, 0.0);
+            for (size_t tt = 0; tt < field.size(); tt += 2)
+                pt += gi(0.5 * field[tt]) * std::exp(chiml::cplx(0.0, -1.0 * fr * static_cast<double>(tt / 2) * (dt / static_cast<double>(Q.timeInt))));
+            for (size_t tt = 1; tt < field.size(); tt += 2)
+                pt += gi(0.5 * field[tt]) * std::exp(chiml::cplx(0.0, -1.0 * fr * static_cast<double>(tt / 2) * (dt / static_cast<double>(Q.timeInt))));
+            pt /= static_cast<double>(Q.t_step * Q.timeInt);
+            if (Q.pow) pt *= std::conj(pt);
+            pt *= Q.conv / 2.0;
+            re[o] = pt.real();
+            im[o] = pt.imag();
+            ++o;
+        }
+    }
+    return static_cast<int>(incd.size());
+}
+
 int oracle_freq_values(void* hp, int d, double* re, double* im, double* freqs_out) {
     auto& S = *static_cast<OracleHandle*>(hp)->sim;
     auto v = S.freqValues(d);

@@ -55,6 +55,7 @@ def lib():
         L.oracle_flux_save.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]
         L.oracle_tfsf_line.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.oracle_freq_incd.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.oracle_flux_incd.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.oracle_incd_record.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
@@ -176,6 +177,14 @@ class Oracle:
         """Writes flux f's face fields as the reference's saveFields does."""
         if lib().oracle_flux_save(self._h, f, str(path).encode()) != 0:
             raise RuntimeError(lib().oracle_last_error().decode())
+
+    def freq_incd(self, d, nmax=1 << 16):
+        """Incident values of frequency detector d's table, (fields, nfreq)."""
+        re = np.zeros(nmax)
+        im = np.zeros(nmax)
+        k = lib().oracle_freq_incd(self._h, d, _dp(re), _dp(im))
+        nf = self.freq_info(d)[0]
+        return (re[:k * nf] + 1j * im[:k * nf]).reshape(k, nf)
 
     def flux_incd(self, f, nmax=4096):
         """Incident flux per frequency (getFlux's incident column)."""

@@ -126,3 +126,67 @@ def test_tfsf_fp32_against_oracle(case, oracles):
         print(f"{case} fp32: comp {c} err/family peak {err:.2e}", flush=True)
         assert err <= 1e-5, (case, c, err)
     sim.close()
+
+
+def test_tfsf_flux_incident_column(tmp_path):
+    """A flux line inside a 2D TFSF box: the total flux and the incident
+    flux (getFlux's incident column from the incident records) against the
+    oracle at 1e-9, and the written table's incident columns."""
+    from chiml_amd import FDTDField
+    d, n, steps = 0.05, 120, 2600
+    flux = {"name": "fl", "size": [40 * d, 0.0, 0.0], "loc": [0.0, 5 * d, 0.0], "lamL": 4.0, "lamR": 8.0,
+            "nfreq": 9, "cross_sec": False}
+    deck = decks.tfsf_deck((n, n, 1), steps, (81, 81, 1), m=(0, 1, 0), pml=12, fwidth_ratio=1.5, cutoff=3.0, flux=flux)
+    o = Oracle(deck)
+    o.step(steps)
+    sim = FDTDField(deck)
+    sim.step(steps)
+    sim.sync()
+    fo, fio = o.flux(0), o.flux_incd(0)
+    fg, fig = sim.fluxArr()[0].getFlux(), sim.fluxArr()[0].getFluxIncd()
+    e1 = np.abs(fg - fo).max() / np.abs(fo).max()
+    e2 = np.abs(fig - fio).max() / np.abs(fio).max()
+    print(f"flux err {e1:.2e}, incident flux err {e2:.2e}", flush=True)
+    assert e1 <= 1e-9 and e2 <= 1e-9
+    sim.toFile(str(tmp_path))
+    tab = np.loadtxt(tmp_path / "fl.dat", comments="#")
+    np.testing.assert_allclose(tab[:, 1], np.abs(fio), rtol=1e-9)
+    np.testing.assert_allclose(tab[:, 2], np.abs(fio.real), rtol=1e-9)
+    np.testing.assert_allclose(tab[:, 3], fio.imag, rtol=1e-9, atol=1e-12 * np.abs(fio).max())
+    sim.close()
+
+
+def test_tfsf_freq_detector_incident_values(tmp_path):
+    """Frequency detectors (Ey, Epow, Hx) in a 3D TFSF box: table values and
+    the incident values (the DFT of the TFSF records) against the oracle at
+    1e-9, and the incident columns of the written Ey table."""
+    import json
+    from chiml_amd import FDTDField
+    d, n, steps = 0.05, 36, 800
+    deck = json.loads(decks.tfsf_deck((n, n, n), steps, (19, 19, 19), m=(0, 0, 1), pml=6, fwidth_ratio=3.0, cutoff=3.0))
+    for ty in ("Ey", "Epow", "Hx"):
+        deck["DetectorList"].append({"type": ty, "dtc_class": "freq", "fname": "fd_" + ty, "size": [0.0, 0.0, 0.0],
+                                     "loc": [2 * d, -3 * d, 1 * d], "freq_list": [], "lamL": 4.0, "lamR": 8.0, "nfreq": 7,
+                                     "SI": False})
+    deck = json.dumps(deck)
+    o = Oracle(deck)
+    o.step(steps)
+    sim = FDTDField(deck)
+    sim.step(steps)
+    sim.sync()
+    fds = sim.dtcFreqArr()
+    for k in range(3):
+        _, vo = o.freq_values(k)
+        io = o.freq_incd(k)
+        vg = fds[k].values()
+        ig = fds[k].incident()
+        e1 = np.abs(vg - vo).max() / np.abs(vo).max()
+        e2 = np.abs(ig - io).max() / np.abs(io).max()
+        print(f"freq detector {k}: values err {e1:.2e}, incident err {e2:.2e}", flush=True)
+        assert e1 <= 1e-9 and e2 <= 1e-9
+    sim.toFile(str(tmp_path))
+    tab = np.loadtxt(tmp_path / "fd_Ey_field_0.dat", comments="#")  # fname + "_field_" + deck index
+    io = o.freq_incd(0)[0]
+    np.testing.assert_allclose(tab[:, 1], io.real, rtol=1e-9, atol=1e-12 * np.abs(io).max())
+    np.testing.assert_allclose(tab[:, 2], io.imag, rtol=1e-9, atol=1e-12 * np.abs(io).max())
+    sim.close()

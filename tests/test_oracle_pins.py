@@ -1931,3 +1931,53 @@ def test_tfsf_polarisation_and_impedance(m, psi):
     assert abs(np.dot(e, k)) <= 1e-2 * ne
     assert np.linalg.norm(h - np.cross(k, e)) <= 1e-2 * ne
     assert abs(abs(np.dot(e / ne, v)) - 1.0) <= 1e-3
+
+
+def test_tfsf_incident_flux_equals_the_flux_through_the_box():
+    """The incident column of the flux table (getFlux with the TFSF incident
+    records, parallelFlux.hpp:455-483) normalises transmission: in vacuum,
+    once the pulse has passed, the flux of the total field through a line
+    inside the box, normal to the propagation, has the incident flux's
+    magnitude at every frequency (the records' point/offset halves, the
+    phase per step, nt^2 and the line-length factor of fluxIncdConv_)."""
+    d, n, steps = 0.05, 120, 2600
+    flux = {"name": "fl", "size": [40 * d, 0.0, 0.0], "loc": [0.0, 5 * d, 0.0], "lamL": 4.0, "lamR": 8.0,
+            "nfreq": 9, "cross_sec": False}
+    deck = decks.tfsf_deck((n, n, 1), steps, (81, 81, 1), m=(0, 1, 0), pml=12, fwidth_ratio=1.5, cutoff=3.0, flux=flux)
+    o = Oracle(deck)
+    o.step(steps)
+    rec = o.incd_record(2)
+    assert np.abs(rec[2 * steps - 10:2 * steps]).max() < 1e-3 * np.abs(rec).max()  # the pulse has passed
+    f, fi = o.flux(0), o.flux_incd(0)
+    assert np.abs(fi).min() > 0.0
+    np.testing.assert_allclose(np.abs(f), np.abs(fi), rtol=2e-6)
+
+
+def _tfsf_freq_deck(steps):
+    d, n = 0.05, 36
+    deck = json.loads(decks.tfsf_deck((n, n, n), steps, (19, 19, 19), m=(0, 0, 1), pml=6, fwidth_ratio=3.0, cutoff=3.0))
+    for ty in ("Ey", "Epow", "Hx"):
+        deck["DetectorList"].append({"type": ty, "dtc_class": "freq", "fname": "fd_" + ty, "size": [0.0, 0.0, 0.0],
+                                     "loc": [2 * d, -3 * d, 1 * d], "freq_list": [], "lamL": 4.0, "lamR": 8.0, "nfreq": 7,
+                                     "SI": False})
+    return json.dumps(deck)
+
+
+def test_tfsf_freq_detector_incident_values():
+    """Frequency detectors inside the box, after the pulse has passed: the
+    table's incident values (the DFT of the TFSF records, squared for power,
+    parallelDTC_FREQ.hpp:414-427) have the magnitude of the detected total
+    field: Ey and Epow to 1e-5; Hx to 2e-3 (its incident value averages the
+    line at two points half a cell apart)."""
+    steps = 800
+    o = Oracle(_tfsf_freq_deck(steps))
+    o.step(steps)
+    rec = o.incd_record(1)
+    assert np.abs(rec.real[2 * steps - 10:2 * steps]).max() < 1e-4 * np.abs(rec.real).max()  # the pulse has passed
+    for k, tol in ((0, 1e-5), (1, 1e-5), (2, 2e-3)):
+        _, vals = o.freq_values(k)
+        inc = o.freq_incd(k)
+        assert inc.shape[0] == (3 if k == 1 else 1)
+        a, b = np.abs(vals.sum(axis=0)), np.abs(inc.sum(axis=0))
+        assert b.min() > 0.0
+        np.testing.assert_allclose(a, b, rtol=tol)
