@@ -59,7 +59,7 @@ def parse():
     p.add_argument("--only-sweep", action="store_true", help="run the serving sweep alone and print its line")
     p.add_argument("--no-configs", action="store_true",
                    help="skip the other workloads (2D TMz, 2D and 3D Maxwell-Liouville)")
-    p.add_argument("--only", choices=["tmz", "ml2d", "ml3d"],
+    p.add_argument("--only", choices=["tmz", "ml2d", "ml3d", "tfsf3d"],
                    help="run one of the other workloads alone and print its line (for profiling)")
     p.add_argument("--only-steps", type=int, default=0, help="step count for --only (0: the workload's own)")
     p.add_argument("--copy-calibration", action="store_true",
@@ -247,6 +247,8 @@ def other_workload(name, steps=0):
         return decks.config1_2d(n=256, steps=steps or 2000, profile="gaussian"), 64, None
     if name == "ml2d":
         return decks.config3_ml2d(n=1024, steps=steps or 20000), 64, None
+    if name == "tfsf3d":
+        return decks.tfsf_deck((256, 256, 256), steps or 1000, (201, 201, 201), m=(0, 0, 1), pml=16, precision="fp32"), 0, None
     return decks.config4_ml3d(n=192, steps=steps or 5000), 0, "ml_density"
 
 
@@ -255,11 +257,13 @@ def run_configs():
     (2000 steps, FP64), 2D Maxwell-Liouville 1024^2 with 32,768 2-level
     emitters (20,000 steps, FP32 fields / FP64 rho) and 3D Maxwell-Liouville
     192^3 with 368,640 (192 x 192 x 10) 6-level emitters over a 20-plane
-    Drude film (5000 steps)."""
+    Drude film (5000 steps), and 3D 256^3 FP32 vacuum with a 201^3 TFSF box
+    (normal incidence, 1000 steps)."""
     res = {}
     res["tmz_256_cylinder_fp64"] = run_workload(*other_workload("tmz"))
     res["ml2d_1024_2level_fp32"] = run_workload(*other_workload("ml2d"))
     res["ml3d_192_6level_fp32"] = run_workload(*other_workload("ml3d"))
+    res["tfsf_256_box201_fp32"] = run_workload(*other_workload("tfsf3d"))
     return res
 
 

@@ -2255,13 +2255,14 @@ struct TfsfLineArgs {
 
 // One line point: pulse, the derivative terms in the reference's order, then
 // the CPML segments that cover it (updateIncdField*, parallelTFSF.cpp:11-45;
-// addPsi / updatePsiField, src/PML/PML.cpp:12-29).
-__device__ void tfsf_point(const TfsfLineArgs& t, int s, int c, int n, int step) {
+// addPsi / updatePsiField, src/PML/PML.cpp:12-29).  L: the set's lines, psi:
+// the set's CPML fields [6][4][psi_n], co: coefficients [6][4][psi_n][3],
+// amp: this step's pulse values [6][pulse_n].
+__device__ void tfsf_point(const TfsfLineArgs& t, double2* L, double2* psi, const double* co, const double2* amp, int s, int c, int n) {
     const int len = t.len[s];
-    double2* L = t.line[s];
     double2 v = L[c * len + n];
     const int pr = n - t.pulse_at;
-    if (pr >= 0 && pr < t.pulse_n) v = cadd(v, t.amp[(static_cast<long long>(step) * 6 + c) * t.pulse_n + pr]);
+    if (pr >= 0 && pr < t.pulse_n) v = cadd(v, amp[c * t.pulse_n + pr]);
     if (n >= t.margin && n < len - t.margin)
         for (int k = 0; k < t.nterm[c]; ++k) {
             const double2* u = L + t.tsrc[c][k] * len;
@@ -2273,27 +2274,69 @@ __device__ void tfsf_point(const TfsfLineArgs& t, int s, int c, int n, int step)
         const int gl = t.shigh[c][g] ? len - 1 - t.psi_n - t.margin : t.margin;
         const int i = n - gl;
         if (i < 0 || i >= t.psi_n) continue;
-        const double* co = t.scoef + ((c * 4 + g) * static_cast<long long>(t.psi_n) + i) * 3;
+        const double* q = co + ((c * 4 + g) * t.psi_n + i) * 3;
         const double2* u = L + t.ssrc[c][g] * len;
-        double2& ps = t.psi[((s * 6 + c) * 4 + g) * static_cast<long long>(t.psi_n) + i];
-        double2 p = cscale(ps, co[0]);
-        p = cadd(p, cscale(u[gl + t.soff1[c][g] + i], co[1]));
-        p = cadd(p, cscale(u[gl + t.soff2[c][g] + i], co[2]));
+        double2& ps = psi[(c * 4 + g) * t.psi_n + i];
+        double2 p = cscale(ps, q[0]);
+        p = cadd(p, cscale(u[gl + t.soff1[c][g] + i], q[1]));
+        p = cadd(p, cscale(u[gl + t.soff2[c][g] + i], q[2]));
         ps = p;
         v = cadd(v, cscale(p, t.sdb[c][g]));
     }
     L[c * len + n] = v;
 }
 
+// LDS bytes of the staged form of k_tfsf_line: both sets of lines, their
+// CPML fields, the coefficients and one step of pulse values.
+__host__ __device__ inline size_t tfsf_lds_bytes(const TfsfLineArgs& t) {
+    return 16 * (6 * static_cast<size_t>(t.len[0] + t.len[1]) + 2 * 6 * 4 * static_cast<size_t>(t.psi_n) + 6 * static_cast<size_t>(t.pulse_n)) +
+           8 * 6 * 4 * 3 * static_cast<size_t>(t.psi_n);
+}
+
 // parallelTFSFBase::step (parallelTFSF.hpp:1148-1174) for every line, with
 // the incident H recorded before it and the incident E after it
-// (parallelFDTDField.hpp:1240-1254).  One block.
-__global__ void __launch_bounds__(1024) k_tfsf_line(TfsfLineArgs t, const int* stepp, int stepv) {
+// (parallelFDTDField.hpp:1240-1254).  One block.  The step is a chain of
+// short dependent passes over small arrays that the field sweeps evict from
+// the caches between steps, so with LDS every array the passes touch is
+// staged in shared memory first (one batch of loads) and written back after;
+// lines too long for it are stepped in global memory.
+template <bool LDS>
+__global__ void __launch_bounds__(1024) k_tfsf_line(TfsfLineArgs ta, const int* stepp, int stepv) {
+    extern __shared__ __align__(16) unsigned char tfsf_lds[];
+    // the parameter block is read by every pass: one copy into LDS, read in
+    // parallel, instead of a chain of dependent argument loads per pass
+    __shared__ TfsfLineArgs t;
+    static_assert(sizeof(TfsfLineArgs) % 4 == 0, "parameter block is copied in 4-byte words");
+    for (unsigned w = threadIdx.x; w < sizeof(TfsfLineArgs) / 4; w += blockDim.x)
+        reinterpret_cast<int*>(&t)[w] = reinterpret_cast<const int*>(&ta)[w];
+    __syncthreads();
     const int step = step_now(stepp, stepv);
+    double2* L[2] = {t.line[0], t.line[1]};
+    const int np = 6 * 4 * t.psi_n;
+    double2* psi = t.psi;
+    const double* co = t.scoef;
+    const double2* amp = t.amp + static_cast<long long>(step) * 6 * t.pulse_n;
+    if (LDS) {
+        double2* sh = reinterpret_cast<double2*>(tfsf_lds);
+        L[0] = sh;
+        L[1] = L[0] + 6 * t.len[0];
+        double2* sp = L[1] + 6 * t.len[1];
+        double2* sa = sp + 2 * np;
+        double* sc = reinterpret_cast<double*>(sa + 6 * t.pulse_n);
+        for (int s = 0; s < 2; ++s)
+            for (int q = threadIdx.x; q < 6 * t.len[s]; q += blockDim.x) L[s][q] = t.line[s][q];
+        for (int q = threadIdx.x; q < 2 * np; q += blockDim.x) sp[q] = t.psi[q];
+        for (int q = threadIdx.x; q < 3 * np; q += blockDim.x) sc[q] = t.scoef[q];
+        for (int q = threadIdx.x; q < 6 * t.pulse_n; q += blockDim.x) sa[q] = amp[q];
+        psi = sp;
+        co = sc;
+        amp = sa;
+        __syncthreads();
+    }
     auto record = [&](int c0) {
         const int c = c0 + static_cast<int>(threadIdx.x);
         if (threadIdx.x < 3 && t.present[c]) {
-            const double2* P = t.line[1] + c * t.len[1];
+            const double2* P = L[1] + c * t.len[1];
             double2* r = t.rec + c * t.nrec + 2LL * step;
             r[0] = P[t.rec_idx];
             r[1] = P[t.rec_idx - t.rec_off[c]];
@@ -2301,20 +2344,21 @@ __global__ void __launch_bounds__(1024) k_tfsf_line(TfsfLineArgs t, const int* s
     };
     record(3);
     for (int ph = 0; ph < 2; ++ph) {  // H lines (3..5), then E lines (0..2)
-        const int c0 = ph == 0 ? 3 : 0;
-        const int tot = 3 * (t.len[0] + t.len[1]);
-        for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-            int r = q, s = 0;
-            if (r >= 3 * t.len[0]) {
-                r -= 3 * t.len[0];
-                s = 1;
-            }
-            const int c = c0 + r / t.len[s], n = r % t.len[s];
-            if (t.present[c]) tfsf_point(t, s, c, n, step);
+        // the line is uniform across the block, so its parameters are scalar
+        // loads; the lines of one phase are independent of each other
+        for (int l = 0; l < 6; ++l) {
+            const int s = l / 3, c = (ph == 0 ? 3 : 0) + l % 3;
+            if (!t.present[c]) continue;
+            for (int n = threadIdx.x; n < t.len[s]; n += blockDim.x) tfsf_point(t, L[s], psi + s * np, co, amp, s, c, n);
         }
         __syncthreads();
     }
     record(0);
+    if (LDS) {
+        for (int s = 0; s < 2; ++s)
+            for (int q = threadIdx.x; q < 6 * t.len[s]; q += blockDim.x) t.line[s][q] = L[s][q];
+        for (int q = threadIdx.x; q < 2 * np; q += blockDim.x) t.psi[q] = psi[q];
+    }
 }
 
 struct TfsfSurfD {
@@ -2332,21 +2376,31 @@ __device__ __forceinline__ bool tfsf_in(const TfsfSurfD& q, const int* x, int& i
 }
 
 // U += pf * Re(line[src][incd0 + i1 s1 + i2 s2]) over the surfaces of one
-// family (addIncdFields, parallelTFSF.cpp:78-84).  A cell on several surfaces
-// of its component (an edge of the box) takes all their adds in order from
-// the thread of its first surface, so the sums keep the reference's order.
+// family (addIncdFields, parallelTFSF.cpp:78-84).  Each surface starts on a
+// block boundary, so a block's surface and the table reads are uniform; the
+// threads of a block run along x where the surface contains x (coalesced).
+// A cell on several surfaces of its component (an edge of the box) takes all
+// their adds in order from the thread of its first surface, so the sums keep
+// the reference's order.
 template <typename T>
 __global__ void __launch_bounds__(256) k_tfsf_add(const TfsfSurfD* __restrict__ sf, int ns, long long total, T* F0, T* F1, T* F2,
                                                   T* F3, T* F4, T* F5, const double2* __restrict__ line, int len, int nx, int ny) {
-    const long long gid = blockIdx.x * 256LL + threadIdx.x;
-    if (gid >= total) return;
+    const long long b0 = blockIdx.x * 256LL;
+    if (b0 >= total) return;
     int s = 0;
-    while (s + 1 < ns && sf[s + 1].start <= gid) ++s;
+    while (s + 1 < ns && sf[s + 1].start <= b0) ++s;
     const TfsfSurfD& q = sf[s];
-    const long long pnt = gid - q.start;
+    const long long pnt = b0 + threadIdx.x - q.start;
+    const long long np = static_cast<long long>(q.n1) * q.n2;
+    if (pnt >= np) return;
     int x[3] = {q.o[0], q.o[1], q.o[2]};
-    x[q.a1] += static_cast<int>(pnt % q.n1);
-    x[q.a2] += static_cast<int>(pnt / q.n1);
+    if (q.a2 == 0) {  // rows along x: x fastest
+        x[q.a2] += static_cast<int>(pnt % q.n2);
+        x[q.a1] += static_cast<int>(pnt / q.n2);
+    } else {
+        x[q.a1] += static_cast<int>(pnt % q.n1);
+        x[q.a2] += static_cast<int>(pnt / q.n1);
+    }
     int i1, i2;
     for (int k = 0; k < s; ++k)
         if (sf[k].comp == q.comp && tfsf_in(sf[k], x, i1, i2)) return;
@@ -2623,6 +2677,8 @@ T* upload_cast(chiml_dev* c, const double* src, size_t n) {
     return c->upload<T>(tmp.data(), n);
 }
 
+constexpr size_t kTfsfMaxLds = 160 * 1024 - 1024;  // dynamic LDS of k_tfsf_line: a gfx950 workgroup's 160 KB less its static parameter copy
+
 // TFSF tables: every index a kernel will form is checked here, on the host.
 template <typename T>
 void build_tfsf(chiml_dev* c, const chiml_problem_t* p, const StepArgs<T>& a) {
@@ -2630,6 +2686,8 @@ void build_tfsf(chiml_dev* c, const chiml_problem_t* p, const StepArgs<T>& a) {
     if (p->ntfsf == 0) return;
     c->nrec = 2LL * (p->nsteps + 1) + 2;
     c->incd_rec = reinterpret_cast<double2*>(c->zeros<double>(2 * 6 * static_cast<size_t>(c->nrec)));
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tfsf_line<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(kTfsfMaxLds)));
     auto need = [](bool ok, const char* what) {
         if (!ok) throw std::runtime_error(std::string("TFSF table: ") + what);
     };
@@ -2724,7 +2782,7 @@ void build_tfsf(chiml_dev* c, const chiml_problem_t* p, const StepArgs<T>& a) {
             d.incd0 = u.incd0;
             d.s1 = u.incd_s1;
             d.s2 = u.incd_s2;
-            d.start = acc;
+            d.start = acc;  // a multiple of the add kernel's block size
             if (u.n1 > 0 && u.n2 > 0) {
                 for (int ax = 0; ax < 3; ++ax) {
                     const int ext = ax == u.a1 ? u.n1 - 1 : (ax == u.a2 ? u.n2 - 1 : 0);
@@ -2736,7 +2794,7 @@ void build_tfsf(chiml_dev* c, const chiml_problem_t* p, const StepArgs<T>& a) {
                         need(li >= 0 && li < tf.len0, "surface line index");
                     }
             }
-            acc += static_cast<long long>(u.n1) * u.n2;
+            acc += (static_cast<long long>(u.n1) * u.n2 + 255) / 256 * 256;
             sv.push_back(d);
         }
         if (tf.nsurf_e == 0) h.toth = acc;
@@ -3129,7 +3187,11 @@ void launch_tfsf(chiml_dev* c, StepArgs<T>& a) {
         if (h.toth > 0)
             hipLaunchKernelGGL(k_tfsf_add<T>, dim3(static_cast<unsigned>((h.toth + 255) / 256)), dim3(256), 0, c->stream, h.surf, h.nsh,
                                h.toth, a.F[0], a.F[1], a.F[2], a.F[3], a.F[4], a.F[5], t.line[0], t.len[0], c->nx, c->ny);
-        hipLaunchKernelGGL(k_tfsf_line, dim3(1), dim3(1024), 0, c->stream, t, a.step, a.stepv);
+        const size_t lds = tfsf_lds_bytes(t);
+        if (lds <= kTfsfMaxLds)
+            hipLaunchKernelGGL(k_tfsf_line<true>, dim3(1), dim3(1024), lds, c->stream, t, a.step, a.stepv);
+        else
+            hipLaunchKernelGGL(k_tfsf_line<false>, dim3(1), dim3(1024), 0, c->stream, t, a.step, a.stepv);
         if (h.tote > 0)
             hipLaunchKernelGGL(k_tfsf_add<T>, dim3(static_cast<unsigned>((h.tote + 255) / 256)), dim3(256), 0, c->stream, h.surf + h.nsh,
                                h.nse, h.tote, a.F[0], a.F[1], a.F[2], a.F[3], a.F[4], a.F[5], t.line[0], t.len[0], c->nx, c->ny);
