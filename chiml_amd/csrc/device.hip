@@ -1907,6 +1907,7 @@ struct QeDev {
     int pop_cap;
     int dim;
     const double* eps_node;
+    int fold_addp;          // 2D: the density kernel does addP for its own emitter cell (no k_ml_addp launch)
 };
 
 // addP (src/UTIL/FDTD_up_eq.cpp:1367-1380) in increment form: a cell's
@@ -2020,7 +2021,7 @@ __host__ __device__ constexpr int ml_rows(int n) { return n; }
 __host__ __device__ constexpr int ml_lanes(int n) { return n * ((n + ml_rows(n) - 1) / ml_rows(n)); }  // lanes per emitter
 
 template <typename T, int N, bool PK>
-__global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, const T* E1, const T* E2, int nx, int ny, int nz,
+__global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, T* E0, T* E1, T* E2, int nx, int ny, int nz,
                                                         const int* stepp, int stepv) {
     constexpr int S = N * N, R = ml_rows(N), NL = (N + R - 1) / R, LN = N * NL, EB = kMlThreads / LN, XS = S + 1,
                   SP = ml_store_words(N, PK);
@@ -2051,8 +2052,12 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
     double2 pop[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
     double P[3] = {0.0, 0.0, 0.0};
     double ef[3] = {0.0, 0.0, 0.0};
-    const T* Es[3] = {E0, E1, E2};
+    T* const Es[3] = {E0, E1, E2};
     int px = 0, py = 0, pz = 0;
+    // 2D with the fold: this emitter's addP result, stored after the first barrier
+    double fold_a = 0.0;
+    T fold_e = T(0);
+    long long fold_ai = -1, fold_ip = -1;
     if (act) {
         px = q.xyz[3 * e];
         py = q.xyz[3 * e + 1];
@@ -2061,8 +2066,21 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
         for (int c = 0; c < 3; ++c) {
             if (!Es[c]) continue;
             long long ip = px + static_cast<long long>(nx) * (py + static_cast<long long>(ny) * pz);
-            if (q.dim == 2) ef[c] = static_cast<double>(Es[c][ip]);
-            else {
+            if (q.dim == 2) {
+                double ev = static_cast<double>(Es[c][ip]);
+                if (q.fold_addp && c == 2) {
+                    // k_ml_addp for this emitter's own E cell (in 2D no other
+                    // emitter's polarisation reaches it): E += A_prev - A
+                    const int bx = px - q.bmin[0] + 1, by = py - q.bmin[1] + 1;
+                    const double p0 = q.Pbox[c][bx + (q.bn[0] + 2) * by];
+                    fold_a = 0.5 * p0 / q.eps_node[ip] + 0.5 * p0 / q.eps_node[ip];
+                    fold_ai = bx + static_cast<long long>(q.bn[0] + 1) * by;
+                    fold_ip = ip;
+                    fold_e = static_cast<T>(ev + (q.Aprev[c][fold_ai] - fold_a));
+                    ev = static_cast<double>(fold_e);
+                }
+                ef[c] = ev;
+            } else {
                 // mean of the two Yee edges E_c(p) and E_c(p - e_c) (getE_TE)
                 const int qx = px - (c == 0), qy = py - (c == 1), qz = pz - (c == 2);
                 double v1 = (qx >= 0 && qy >= 0 && qz >= 0) ? static_cast<double>(Es[c][qx + static_cast<long long>(nx) * (qy + static_cast<long long>(ny) * qz)]) : 0.0;
@@ -2079,7 +2097,11 @@ __global__ void __launch_bounds__(kMlThreads) k_ml_team(QeDev q, const T* E0, co
     // row i of this lane is kept in memory (upper triangle only when packed)
     auto kept = [&](int i) { return i < N && !(PK && i > j); };
     auto stored = [&](int i) -> long long { return PK ? j * (j + 1) / 2 + i : i + j * N; };
-    __syncthreads();  // mu and the Lindblad entries are staged
+    __syncthreads();  // mu and the Lindblad entries are staged; every lane has read E and A_prev
+    if (fold_ai >= 0 && cl == 0) {
+        q.Aprev[2][fold_ai] = fold_a;
+        Es[2][fold_ip] = fold_e;
+    }
     for (int m = 0; m < q.nmem; ++m) {
         const long long col = static_cast<long long>(m) * q.nemit + e;
         double2 rc[R], fp[R];
@@ -3064,6 +3086,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         q.hbar = qi.hbar;
         q.dt = p->dt;
         q.dim = is3 ? 3 : 2;
+        q.fold_addp = (!is3 && std::getenv("CHIML_ML_ADDP") == nullptr) ? 1 : 0;
         for (int k = 0; k < 3; ++k) {
             q.bmin[k] = qi.box_min[k];
             q.bn[k] = qi.box_n[k];
@@ -3328,8 +3351,9 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
         QeDev& q = qh.dev;
         int ex = q.bn[0] + 1, ey = q.bn[1] + 1, ez = (q.dim == 3) ? q.bn[2] + 1 : 1;
         int n = ex * ey * ez;
-        hipLaunchKernelGGL(k_ml_addp<T>, dim3((n + 255) / 256, 3), dim3(256), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny,
-                           c->nz);
+        if (!q.fold_addp)
+            hipLaunchKernelGGL(k_ml_addp<T>, dim3((n + 255) / 256, 3), dim3(256), 0, c->stream, q, a.F[0], a.F[1], a.F[2], c->nx, c->ny,
+                               c->nz);
         dim3 g(qh.nblocks);
         {
             Timed t(c, "ml_density");
