@@ -2438,6 +2438,27 @@ __global__ void __launch_bounds__(256) k_tfsf_add(const TfsfSurfD* __restrict__ 
     F[idx] = v;
 }
 
+// Per-cell soft-source adds (oblique sources, parallelSourceOblique.cpp:17-23),
+// between the H and the E update. The host sorts the cells by (component,
+// point), keeping the sources' order within a point: the first thread of a
+// run of equal keys adds the whole run, in order, so no two threads write
+// one point.
+template <typename T>
+__global__ void __launch_bounds__(256) k_cell_src(const int* __restrict__ comp, const long long* __restrict__ idx,
+                                                  const int* __restrict__ col, const T* __restrict__ amp, int n, int ncol,
+                                                  const int* stepp, int stepv, T* F0, T* F1, T* F2, T* F3, T* F4, T* F5) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int c = comp[k];
+    const long long i = idx[k];
+    if (k > 0 && comp[k - 1] == c && idx[k - 1] == i) return;
+    const long long row = static_cast<long long>(step_now(stepp, stepv)) * ncol;
+    T* const Fs[6] = {F0, F1, F2, F3, F4, F5};
+    T v = Fs[c][i];
+    for (int j = k; j < n && comp[j] == c && idx[j] == i; ++j) v += amp[row + col[j]];
+    Fs[c][i] = v;
+}
+
 // ------------------------------------------------------- batched launches ----
 // One launch per kernel class covers many simulations (the serving loop's
 // batches): slot k of a device-side descriptor table holds simulation k's
@@ -2636,6 +2657,13 @@ struct chiml_dev {
         long long toth = 0, tote = 0;
     };
     std::vector<TfsfHost> tfsf;
+    struct CellSrc {  // per-cell source adds, sorted by (component, point)
+        int n = 0, ncol = 0;
+        int* comp = nullptr;
+        long long* idx = nullptr;
+        int* col = nullptr;
+        void* amp = nullptr;  // T[nsteps][ncol]
+    } cells;
     double2* incd_rec = nullptr;  // [6][nrec] incident records (E_incd_, H_incd_)
     long long nrec = 0;
     size_t footprint = 0;
@@ -2700,6 +2728,40 @@ T* upload_cast(chiml_dev* c, const double* src, size_t n) {
 }
 
 constexpr size_t kTfsfMaxLds = 160 * 1024 - 1024;  // dynamic LDS of k_tfsf_line: a gfx950 workgroup's 160 KB less its static parameter copy
+
+// Per-cell source table: components, points and columns checked on the host.
+template <typename T>
+void build_cells(chiml_dev* c, const chiml_problem_t* p) {
+    const int n = p->ncell_src;
+    if (n < 0 || p->ncell_col < 0 || (n > 0 && (!p->cell_comp || !p->cell_xyz || !p->cell_col || !p->cell_amp || p->ncell_col == 0)))
+        throw std::runtime_error("invalid per-cell source table");
+    if (n == 0) return;
+    const bool is3 = p->nz > 1;
+    std::vector<std::pair<long long, int>> key(n);  // (component * points + point, position in the list)
+    const long long npt = static_cast<long long>(p->nx) * p->ny * p->nz;
+    for (int k = 0; k < n; ++k) {
+        const int cc = p->cell_comp[k], x = p->cell_xyz[3 * k], y = p->cell_xyz[3 * k + 1], z = p->cell_xyz[3 * k + 2];
+        if (cc < 0 || cc > 5 || (!is3 && cc != 2 && cc != 3 && cc != 4)) throw std::runtime_error("per-cell source on a missing component");
+        if (x < 0 || x >= p->nx || y < 0 || y >= p->ny || z < 0 || z >= p->nz) throw std::runtime_error("per-cell source outside the grid");
+        if (p->cell_col[k] < 0 || p->cell_col[k] >= p->ncell_col) throw std::runtime_error("per-cell source column out of range");
+        key[k] = {cc * npt + x + static_cast<long long>(p->nx) * (y + static_cast<long long>(p->ny) * z), k};
+    }
+    std::stable_sort(key.begin(), key.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<int> comp(n), col(n);
+    std::vector<long long> idx(n);
+    for (int k = 0; k < n; ++k) {
+        const int o = key[k].second;
+        comp[k] = static_cast<int>(key[k].first / npt);
+        idx[k] = key[k].first % npt;
+        col[k] = p->cell_col[o];
+    }
+    c->cells.n = n;
+    c->cells.ncol = p->ncell_col;
+    c->cells.comp = c->upload<int>(comp.data(), n);
+    c->cells.idx = c->upload<long long>(idx.data(), n);
+    c->cells.col = c->upload<int>(col.data(), n);
+    c->cells.amp = upload_cast<T>(c, p->cell_amp, static_cast<size_t>(p->nsteps) * p->ncell_col);
+}
 
 // TFSF tables: every index a kernel will form is checked here, on the host.
 template <typename T>
@@ -3120,6 +3182,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         c->qes.push_back(std::move(qh));
     }
     build_tfsf(c, p, a);
+    build_cells<T>(c, p);
 }
 
 template <typename T>
@@ -3221,6 +3284,14 @@ void launch_tfsf(chiml_dev* c, StepArgs<T>& a) {
     }
 }
 
+template <typename T>
+void launch_cells(chiml_dev* c, StepArgs<T>& a) {
+    const auto& q = c->cells;
+    if (q.n == 0) return;
+    hipLaunchKernelGGL(k_cell_src<T>, dim3(static_cast<unsigned>((q.n + 255) / 256)), dim3(256), 0, c->stream, q.comp, q.idx, q.col,
+                       static_cast<const T*>(q.amp), q.n, q.ncol, a.step, a.stepv, a.F[0], a.F[1], a.F[2], a.F[3], a.F[4], a.F[5]);
+}
+
 template <typename T, int DIM>
 void launch_fields(chiml_dev* c, StepArgs<T>& a) {
     const bool adv = a.step_next != nullptr;
@@ -3269,6 +3340,7 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
                 });
             }
             launch_tfsf<T>(c, a);
+            launch_cells<T>(c, a);
             if (ge.x > 0) {
                 Timed t(c, "update_e");
                 if (c->has_src)
@@ -3291,6 +3363,7 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
                 else hipLaunchKernelGGL((k_h3_slab<T, W, false>), gs, blk, 0, c->stream, a, c->slab);
             }
             launch_tfsf<T>(c, a);
+            launch_cells<T>(c, a);
             {
                 Timed t(c, "update_e");
                 if (c->has_src) hipLaunchKernelGGL((k_e3_int<T, W, true>), gi, blk, 0, c->stream, a, b, c->nmat[0], c->nmat[1], c->nmat[2]);
@@ -3315,6 +3388,7 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
             });
         }
         launch_tfsf<T>(c, a);
+        launch_cells<T>(c, a);
         {
             Timed t(c, "update_e");
             const SampleArgs<T>& sa = sample_args<T>(c);
@@ -3338,6 +3412,7 @@ void launch_fields(chiml_dev* c, StepArgs<T>& a) {
         });
     }
     launch_tfsf<T>(c, a);
+    launch_cells<T>(c, a);
     {
         Timed t(c, "update_e");
         if (c->has_src) hipLaunchKernelGGL((k_update_e<T, DIM, true>), grd, blk, 0, c->stream, a);
@@ -3496,7 +3571,7 @@ void* scratch_on(int dev) {
 // Kernel class of a context for batched launches: 1 = 2D W-cell kernels,
 // 2 = 3D split sweeps; 0 = not batched (stepped on its own stream).
 int batch_class(const chiml_dev* c) {
-    if (c->fused || c->capturing || !c->tfsf.empty()) return 0;  // TFSF steps stay on their own stream
+    if (c->fused || c->capturing || !c->tfsf.empty() || c->cells.n > 0) return 0;  // TFSF and per-cell source steps stay on their own stream
     if (c->dim == 2 && c->vec2d) return 1;
     if (c->dim == 3 && c->split) return 2;
     return 0;
@@ -3734,7 +3809,7 @@ void enable_fused(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, Sample
     constexpr int W = 16 / sizeof(T);
     const char* on = std::getenv("CHIML_FUSED");
     if (!on || std::atoi(on) != 1) return;
-    if (!c->tfsf.empty()) return;  // the TFSF adds sit between the H and E updates
+    if (!c->tfsf.empty() || c->cells.n > 0) return;  // TFSF and per-cell source adds sit between the H and E updates
     const int nx = p->nx, ny = p->ny, nz = p->nz;
     if (nx % W != 0 || p->pml_T[0] % W != 0 || c->hmap) return;
     FuseMap m{};

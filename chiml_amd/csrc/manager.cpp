@@ -23,6 +23,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -938,8 +939,29 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     std::vector<int> srcComp, srcBox;
     std::vector<double> srcAmp;
     std::vector<const SourceSpec*> activeSrc;
+    // An R or L source drives the two E components across its thinnest axis
+    // (first minimum of its size): x -> Ey, Ez; y -> Ex, Ez; z -> Ex, Ey
+    // (parallelFDTDField.cpp:548-622). Its polarisation prefactors are
+    // appended to each pulse's parameter list after the slot the pulse
+    // functions read their amplitude from (PulseFxn.hpp:21-88), so both
+    // components carry the unscaled pulse.
+    std::vector<std::pair<int, const SourceSpec*>> srcList;
     for (auto& s : IP.srcs) {
-        int comp = static_cast<int>(s.pol);
+        if (s.pol == Pol::L || s.pol == Pol::R) {
+            int amin = 0;
+            for (int a = 1; a < 3; ++a)
+                if (s.sz[a] < s.sz[amin]) amin = a;
+            srcList.emplace_back(amin == 0 ? 1 : 0, &s);
+            srcList.emplace_back(amin == 2 ? 1 : 2, &s);
+        } else if (static_cast<int>(std::round(s.phi)) % 90 != 0) {
+            continue;  // oblique: per-cell adds, built after the TFSF tables
+        } else {
+            srcList.emplace_back(static_cast<int>(s.pol), &s);
+        }
+    }
+    for (auto& cs : srcList) {
+        const SourceSpec& s = *cs.second;
+        const int comp = cs.first;
         bool present = is3D || comp == 2 || comp == 3 || comp == 4;
         if (!present) throw std::logic_error("source on a field component that does not exist in 2D TMz");
         std::array<int, 6> bx;
@@ -998,6 +1020,110 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     P.ntfsf = static_cast<int>(tfsfTab.t.size());
     P.tfsf = tfsfTab.t.empty() ? nullptr : tfsfTab.t.data();
     ntfsf = P.ntfsf;
+
+    // ---- oblique sources -------------------------------------------------------
+    // parallelSourceObliqueBase::genDatStruct (parallelSourceOblique.hpp:86-170):
+    // the cells of the box built around the source's centre point that lie
+    // within dx/sqrt(2) of the plane normal to the propagation direction; each
+    // adds scalefact * Re(pulse(t - t_off)) per step, with no dt factor
+    // (parallelSourceOblique.cpp:17-23). The polar angle is never read from the
+    // input (srcTheta_ stays 90, parallelInputs.cpp:44), so theta_ is 0 in 3D
+    // and pi/2 in 2D. An Ex/Ey source drives Ex and Ey, an Hx source Hy then
+    // Hx, an Hy source Hx then Hy (parallelFDTDField.cpp:479-544). Cells with
+    // the same source, scale and time offset share one amplitude column.
+    std::vector<int> cellComp, cellXyz, cellCol;
+    std::vector<double> cellAmp;
+    {
+        struct Col { const SourceSpec* s; double sf, toff; };
+        std::vector<Col> cols;
+        std::map<std::tuple<const SourceSpec*, double, double>, int> colOf;
+        for (auto& s : IP.srcs) {
+            if (s.pol == Pol::L || s.pol == Pol::R || static_cast<int>(std::round(s.phi)) % 90 == 0) continue;
+            std::vector<Pol> pols;
+            if (s.pol == Pol::EX || s.pol == Pol::EY) pols = {Pol::EX, Pol::EY};
+            else if (s.pol == Pol::HX) pols = {Pol::HY, Pol::HX};
+            else if (s.pol == Pol::HY) pols = {Pol::HX, Pol::HY};
+            else pols = {s.pol};
+            const double phi_ = std::fmod(s.phi, 360.0) * M_PI / 180.0;
+            const double theta_ = is3D ? std::fmod(90.0, 360.0) * M_PI / 180.0 - M_PI / 2.0 : M_PI / 2.0;
+            const double phi = std::fmod(phi_ + M_PI / 2.0, 2.0 * M_PI);
+            const double theta = (theta_ >= M_PI / 2) ? theta_ - M_PI / 2.0 : theta_ + M_PI / 2.0;
+            double nv[3] = {std::sin(theta_) * std::cos(phi_), std::sin(theta_) * std::sin(phi_), std::cos(theta_)};
+            const double inv = 1.0 / std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+            for (double& v : nv) v *= inv;
+            const double dxv = IP.d[0], dyv = IP.d[1], dzv = is3D ? IP.d[2] : 1.0;
+            const int L0 = s.loc[0], L1 = s.loc[1], L2 = s.loc[2];
+            auto half = [](double v) { return std::fabs(static_cast<int>(std::round(v / 2.0))); };
+            int xMin = static_cast<int>(L0 - half(s.sz[0] * std::cos(phi)));
+            xMin = std::min(xMin, static_cast<int>(std::fabs(std::round(xMin + s.sz[1] * std::cos(phi) * std::sin(theta)))));
+            int yMin = static_cast<int>(L1 - half(s.sz[0] * std::sin(phi)));
+            yMin = std::min(yMin, static_cast<int>(std::fabs(std::round(yMin + s.sz[1] * std::sin(phi) * std::sin(theta)))));
+            const int zMin = is3D ? static_cast<int>(L2 - half(s.sz[1] * std::cos(theta))) : 0;
+            int xMax = static_cast<int>(L0 + half(s.sz[0] * std::cos(phi)));
+            xMax = std::min(xMax, static_cast<int>(std::fabs(std::round(xMax + s.sz[1] * std::cos(phi) * std::sin(theta)))));
+            int yMax = static_cast<int>(L1 + half(s.sz[0] * std::sin(phi)));
+            yMax = std::min(yMax, static_cast<int>(std::fabs(std::round(yMax + s.sz[1] * std::sin(phi) * std::sin(theta)))));
+            const int zMax = is3D ? static_cast<int>(L2 + half(s.sz[1] * std::cos(theta))) : 0;
+            for (Pol pol : pols) {
+                const int comp = static_cast<int>(pol);
+                const double sf = (pol == Pol::EX || pol == Pol::HX)   ? std::cos(phi) * std::sin(theta_)
+                                  : (pol == Pol::EY || pol == Pol::HY) ? std::sin(phi) * std::sin(theta_)
+                                                                       : std::cos(theta_);
+                if (sf == 0.0) continue;  // adds exact zeros (Ex/Ey/Hx/Hy parts in 3D)
+                if (!is3D && comp != 2 && comp != 3 && comp != 4)
+                    throw std::logic_error("source on a field component that does not exist in 2D TMz");
+                double off[3] = {0.0, 0.0, 0.0};
+                if (pol == Pol::EX || pol == Pol::HY || pol == Pol::HZ) off[0] = 0.5;
+                if (pol == Pol::HX || pol == Pol::EY || pol == Pol::HZ) off[1] = 0.5;
+                if (is3D && (pol == Pol::HX || pol == Pol::HY || pol == Pol::EZ)) off[2] = 0.5;
+                for (int xx = xMin; xx <= xMax; ++xx)
+                    for (int yy = yMin; yy <= yMax; ++yy)
+                        for (int zz = zMin; zz <= zMax; ++zz) {
+                            const double pt[3] = {(xx - L0 + off[0]) * dxv, (yy - L1 + off[1]) * dyv, (zz - L2 + off[2]) * dzv};
+                            double dd = 0.0;  // ddot_, summed in order
+                            for (int i = 0; i < 3; ++i) dd += pt[i] * nv[i];
+                            if (!(std::fabs(dd) <= dxv / std::sqrt(2.0))) continue;
+                            if (physAt(comp, xx, yy, zz) < 0)
+                                throw std::logic_error("an oblique source cell falls outside the grid's physical points");
+                            if (comp < 3 && dcellAt(comp, xx, yy, zz))
+                                throw std::logic_error("an oblique source on an E cell derived from D (dispersive, emitter or CPML with material) is not supported");
+                            auto key = std::make_tuple(&s, sf, dd);
+                            auto it = colOf.find(key);
+                            int col;
+                            if (it == colOf.end()) {
+                                col = static_cast<int>(cols.size());
+                                colOf.emplace(key, col);
+                                cols.push_back({&s, sf, dd});
+                            } else {
+                                col = it->second;
+                            }
+                            cellComp.push_back(comp);
+                            cellXyz.insert(cellXyz.end(), {xx, yy, zz});
+                            cellCol.push_back(col);
+                        }
+            }
+        }
+        if (!cols.empty()) {
+            if (static_cast<double>(nsteps) * cols.size() * sizeof(double) > 4.0 * (1ull << 30))
+                throw std::logic_error("the oblique sources' amplitude table exceeds 4 GiB");
+            cellAmp.assign(static_cast<size_t>(nsteps) * cols.size(), 0.0);
+            double tcur = 0.0;
+            for (int st = 0; st < nsteps; ++st) {
+                for (size_t k = 0; k < cols.size(); ++k) {
+                    double v = 0.0;
+                    for (auto& pul : cols[k].s->pulses) v += cols[k].sf * std::real(pulseValue(pul, tcur - cols[k].toff));
+                    cellAmp[st * cols.size() + k] = v;
+                }
+                tcur += dt;
+            }
+        }
+        P.ncell_src = static_cast<int>(cellComp.size());
+        P.ncell_col = static_cast<int>(cols.size());
+        P.cell_comp = cellComp.data();
+        P.cell_xyz = cellXyz.data();
+        P.cell_col = cellCol.data();
+        P.cell_amp = cellAmp.data();
+    }
 
     // ---- point detectors -----------------------------------------------------
     std::vector<int> dInt, dNp, dComp, dPow, dNf;

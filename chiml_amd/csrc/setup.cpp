@@ -19,6 +19,8 @@ Pol string2pol(const std::string& p) {
     if (p == "Hx") return Pol::HX;
     if (p == "Hy") return Pol::HY;
     if (p == "Hz") return Pol::HZ;
+    if (p == "L") return Pol::L;
+    if (p == "R") return Pol::R;
     throw std::logic_error("POLARIZATION undefined or unsupported: " + p);
 }
 PulseShape string2prof(const std::string& p) {
@@ -549,13 +551,16 @@ ProgramInputs parseInputs(const std::string& text) {
         json::get_arr3(it, "size", tsz);
         for (int c = 0; c < 3; ++c) s.sz[c] = find_pt(tsz[c], P.d[c]) + 1;
         s.phi = json::get_num(it, "phi", 90);
-        if (!(s.phi == 90 || s.phi == 180 || s.phi == 270 || s.phi == 0))
-            throw std::logic_error("oblique sources are not supported");
+        // A linear source is oblique unless round(phi) is a multiple of 90
+        // (parallelFDTDField.cpp:479-544); R/L sources are always normal
+        // (parallelFDTDField.cpp:548-622). The corner of the box for phi exactly 0/90/180/270, else the centre
+        // point (parallelInputs.cpp:199-214).
+        const bool corner = (s.phi == 90 || s.phi == 180 || s.phi == 270 || s.phi == 0);
         json::get_arr3(it, "loc", tloc);
         for (int c = 0; c < 3; ++c) {
             if (tloc[c] + tsz[c] / 2.0 > P.size[c] / 2.0 || tloc[c] - tsz[c] / 2.0 < -1.0 * P.size[c] / 2.0)
                 throw std::logic_error("The source is at least partially outside the FDTD Cell");
-            s.loc[c] = find_pt(tloc[c] + P.size[c] / 2.0 - tsz[c] / 2.0, P.d[c]);
+            s.loc[c] = corner ? find_pt(tloc[c] + P.size[c] / 2.0 - tsz[c] / 2.0, P.d[c]) : find_pt(tloc[c] + P.size[c] / 2.0, P.d[c]);
         }
         P.srcs.push_back(s);
     }

@@ -26,7 +26,7 @@ const double EPS0 = 1.0 / (4.0 * kPi * 1.0e-7 * SPEED_OF_LIGHT * SPEED_OF_LIGHT)
 constexpr double ELEMENTARY_CHARGE = 1.6021766208e-19;
 constexpr double HBAR = 1.0545718001391127e-34;
 
-enum class Pol { EX = 0, EY, EZ, HX, HY, HZ };
+enum class Pol { EX = 0, EY, EZ, HX, HY, HZ, L, R };
 enum class PulseShape { GAUSSIAN, BH, RECT, CONTINUOUS, RICKER, RAMP_CONT };
 enum class DtcType { EX = 0, EY, EZ, HX, HY, HZ, EPOW, HPOW };
 enum class DtcClass { BIN, TXT, COUT, FREQ };

@@ -221,6 +221,17 @@ typedef struct {
      * parallelFDTDField.hpp:1238-1255). */
     int ntfsf;
     const chiml_tfsf_t* tfsf;
+    /* Per-cell soft-source adds (oblique sources, parallelSourceObliqueReal::
+     * addPul, parallelSourceOblique.cpp:17-23): at each step, after the H
+     * update and before the E update, cell k (component cell_comp[k] in 0..5,
+     * point cell_xyz[3k..3k+2]) gains cell_amp[step * ncell_col + cell_col[k]].
+     * E cells must not be D-derived (CHIML_MAT_DCELL). */
+    int ncell_src;
+    int ncell_col;
+    const int* cell_comp;
+    const int* cell_xyz;
+    const int* cell_col;
+    const double* cell_amp; /* nsteps x ncell_col */
 } chiml_problem_t;
 
 int chiml_dev_create(const chiml_problem_t* p, chiml_dev_t** out);

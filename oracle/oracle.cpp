@@ -500,6 +500,18 @@ struct Source {
     std::vector<double> pulVec;
 };
 
+// parallelSourceObliqueBase<double> (parallelSourceOblique.hpp:22-170): the
+// grid points it adds to, each with its time offset and scale factor.
+struct ObliqueSource {
+    std::vector<PulseSpec> pulses;
+    DGrid* grid = nullptr;
+    struct El {
+        size_t at;
+        double t_off, scalefact;
+    };
+    std::vector<El> els;
+};
+
 // ------------------------------------------------------------------ TFSF ----
 // parallelTFSFBase (src/SOURCE/parallelTFSF.hpp:100-1174, parallelTFSF.cpp:11-92)
 // and IncdCPML (src/PML/PML.hpp:37-350, PML.cpp:12-40) for real fields on one
@@ -664,6 +676,8 @@ struct Sim {
     std::vector<DGrid> lorP[3], prevLorP[3];
     std::unique_ptr<CPML> EPML[3], HPML[3];
     std::vector<Source> srcs;
+    std::vector<ObliqueSource> obl;
+    void buildOblique(const chiml::SourceSpec& s, DGrid* g, Pol pol);
     std::vector<PointDetector> dtcs;
     std::vector<FreqDetector> freqDtcs;
     std::vector<Flux> fluxes;
@@ -1534,11 +1548,110 @@ void Sim::tfsfStepAll() {
 }
 
 // Source placement, parallelSourceNormal.hpp:69-150 (one rank).
+// An R/L source (parallelFDTDField.cpp:548-622) is two normal sources on the
+// E components across the axis isamin_ picks from its size (the first
+// minimum: the int sizes are read as float bit patterns, which order the same
+// way). The branch computes psiPrefactCalc, alphaOff and prefactor_j/k and
+// appends Emax*prefactor_j (then *prefactor_k/prefactor_j*i*alphaOff) to each
+// pulse's parameters, but every pulse function reads Emax from its fixed slot
+// (PulseFxn.hpp:21 param[4], :34 param[1], :47 param[2], :60 param[7],
+// :74 param[4], :88 param[3]), filled before the branch
+// (parallelFDTDField.cpp:475): both components carry the unscaled pulse.
+// genDatStruct (parallelSourceOblique.hpp:86-170) with the constructor's
+// angles (:53-67). srcTheta_ is never read from the input and stays 90
+// (parallelInputs.cpp:44). ddot_ of three terms is summed in order.
+void Sim::buildOblique(const chiml::SourceSpec& s, DGrid* g, Pol pol) {
+    if (!g->ok()) throw std::logic_error("source on a field component that does not exist");
+    ObliqueSource ob;
+    ob.pulses = s.pulses;
+    ob.grid = g;
+    const double phi_ = std::fmod(s.phi, 360.0) * M_PI / 180.0;
+    double theta_ = std::fmod(90.0, 360.0) * M_PI / 180.0 - M_PI / 2.0;
+    if (std::abs(theta_) > M_PI) throw std::logic_error("The polar angle has to be between 0 and 180 degrees");
+    if (g->lz == 1) theta_ = M_PI / 2.0;
+    double phi = std::fmod(phi_ + M_PI / 2.0, 2.0 * M_PI);
+    double theta;
+    if (theta_ >= M_PI / 2) theta = theta_ - M_PI / 2.0;
+    else theta = theta_ + M_PI / 2.0;
+    std::vector<double> normVec = {std::sin(theta_) * std::cos(phi_), std::sin(theta_) * std::sin(phi_), std::cos(theta_)};
+    const double nrm = 1.0 / std::sqrt(std::pow(normVec[0], 2.0) + std::pow(normVec[1], 2.0) + std::pow(normVec[2], 2.0));
+    for (double& v : normVec) v = nrm * v;
+    const double dx = IP.d[0], dy = IP.d[1];
+    double dz = 1.0;
+    if (g->lz != 1) dz = IP.d[2];
+    const std::array<int, 3>& loc = s.loc;
+    const std::array<int, 3>& sz = s.sz;
+    int xMin = loc[0] - std::fabs(int(std::round(sz[0] * std::cos(phi) / 2.0)));
+    xMin = std::min(xMin, int(std::fabs(std::round(xMin + sz[1] * std::cos(phi) * std::sin(theta)))));
+    int yMin = loc[1] - std::fabs(int(std::round(sz[0] * std::sin(phi) / 2.0)));
+    yMin = std::min(yMin, int(std::fabs(std::round(yMin + sz[1] * std::sin(phi) * std::sin(theta)))));
+    int zMin = 0;
+    if (g->lz != 1) zMin = loc[2] - std::fabs(int(std::round(sz[1] * std::cos(theta) / 2.0)));
+    int xMax = loc[0] + std::fabs(int(std::round(sz[0] * std::cos(phi) / 2.0)));
+    xMax = std::min(xMax, int(std::fabs(std::round(xMax + sz[1] * std::cos(phi) * std::sin(theta)))));
+    int yMax = loc[1] + std::fabs(int(std::round(sz[0] * std::sin(phi) / 2.0)));
+    yMax = std::min(yMax, int(std::fabs(std::round(yMax + sz[1] * std::sin(phi) * std::sin(theta)))));
+    int zMax = 0;
+    if (g->lz != 1) zMax = loc[2] + std::fabs(int(std::round(sz[1] * std::cos(theta) / 2.0)));
+    std::vector<double> off(3, 0.0);
+    if (pol == Pol::EX || pol == Pol::HY || pol == Pol::HZ) off[0] = 0.5;
+    if (pol == Pol::HX || pol == Pol::EY || pol == Pol::HZ) off[1] = 0.5;
+    if (g->lz != 1 && (pol == Pol::HX || pol == Pol::HY || pol == Pol::EZ)) off[2] = 0.5;
+    auto ddot3 = [](const std::vector<double>& a, const std::vector<double>& b) {
+        double t = 0.0;
+        for (int i = 0; i < 3; ++i) t += a[i] * b[i];
+        return t;
+    };
+    for (int xx = xMin; xx <= xMax; ++xx)
+        for (int yy = yMin; yy <= yMax; ++yy)
+            for (int zz = zMin; zz <= zMax; ++zz) {
+                std::vector<double> pt = {(xx - loc[0] + off[0]) * dx, (yy - loc[1] + off[1]) * dy, (zz - loc[2] + off[2]) * dz};
+                if (std::fabs(ddot3(pt, normVec)) <= dx / std::sqrt(2.0)) {
+                    const int px = xx + 1, py = yy + 1, pz = (g->lz == 1) ? 0 : zz + 1;
+                    if (px < 0 || px >= g->lx || py < 0 || py >= g->ly || pz < 0 || pz >= g->lz)
+                        throw std::logic_error("an oblique source point lies outside the grid's storage");
+                    ObliqueSource::El el;
+                    el.at = g->ind(px, py, pz);
+                    if (pol == Pol::EX || pol == Pol::HX) el.scalefact = std::cos(phi) * std::sin(theta_);
+                    else if (pol == Pol::EY || pol == Pol::HY) el.scalefact = std::sin(phi) * std::sin(theta_);
+                    else el.scalefact = std::cos(theta_);
+                    el.t_off = ddot3(normVec, pt);
+                    ob.els.push_back(el);
+                }
+            }
+    obl.push_back(std::move(ob));
+}
+
 void Sim::buildSources() {
+    std::vector<std::pair<int, const chiml::SourceSpec*>> list;
     for (auto& s : IP.srcs) {
+        if (s.pol != Pol::L && s.pol != Pol::R && static_cast<int>(std::round(s.phi)) % 90 != 0) {
+            // oblique: parallelFDTDField.cpp:485-542
+            switch (s.pol) {
+                case Pol::EX: case Pol::EY: buildOblique(s, &E[0], Pol::EX); buildOblique(s, &E[1], Pol::EY); break;
+                case Pol::EZ: buildOblique(s, &E[2], Pol::EZ); break;
+                case Pol::HX: buildOblique(s, &H[1], Pol::HY); buildOblique(s, &H[0], Pol::HX); break;
+                case Pol::HY: buildOblique(s, &H[0], Pol::HX); buildOblique(s, &H[1], Pol::HY); break;
+                default: buildOblique(s, &H[2], Pol::HZ); break;
+            }
+            continue;
+        }
+        if (s.pol == Pol::L || s.pol == Pol::R) {
+            int amin = 0;
+            for (int i = 1; i < 3; ++i)
+                if (s.sz[i] < s.sz[amin]) amin = i;
+            static const int kJ[3] = {1, 0, 0}, kK[3] = {2, 2, 1};
+            list.emplace_back(kJ[amin], &s);
+            list.emplace_back(kK[amin], &s);
+        } else {
+            list.emplace_back(static_cast<int>(s.pol), &s);
+        }
+    }
+    for (auto& cs : list) {
+        const chiml::SourceSpec& s = *cs.second;
         Source src;
         src.pulses = s.pulses;
-        int c = static_cast<int>(s.pol);
+        int c = cs.first;
         src.grid = (c < 3) ? &E[c] : &H[c - 3];
         if (!src.grid->ok()) throw std::logic_error("source on a field component that does not exist");
         std::array<int, 3> ll, lsz = {{0, 0, 0}};
@@ -2449,6 +2562,10 @@ void Sim::step() {
                                   s.loc[2] + jj * s.add1[2] + kk * s.add2[2]),
                       s.stride);
     }
+    // Oblique sources: parallelSourceObliqueReal::addPul (parallelSourceOblique.cpp:17-23).
+    for (auto& ob : obl)
+        for (auto& p : ob.pulses)
+            for (auto& el : ob.els) ob.grid->v[el.at] += el.scalefact * std::real(pulseValue(p, tcur - el.t_off));
     // updatePolE: Lorentz ADE on the upLorD runs (E before the D update).
     for (int ii = 0; ii < 3; ++ii) {
         if (!E[ii].ok()) continue;

@@ -94,6 +94,24 @@ def test_deck_errors(mutate, message):
         _estimate(deck)
 
 
+def _oblique(pol, size, loc, phi=30.0):
+    return {"pol": pol, "size": size, "loc": loc, "phi": phi, "PulseList": [decks.gaussian_pulse(1.0, 1.0, 3.0, 1.0)]}
+
+
+@pytest.mark.parametrize("n3,src,message", [
+    ((32, 32, 1), _oblique("Hx", [1.4, 0.1, 0], [0, 0.7, 0]), "outside the grid's physical points"),
+    ((32, 32, 1), _oblique("Ex", [0.2, 0.2, 0], [0, 0, 0]), "does not exist in 2D TMz"),
+])
+def test_source_table_errors(n3, src, message):
+    """Errors raised while the host builds the source tables, before any
+    device call: oblique points past the grid's physical points and
+    components a 2D TMz grid lacks."""
+    deck = _base(n3)
+    deck["SourceList"].append(src)
+    with pytest.raises(chiml_amd.ChimlError, match=message):
+        chiml_amd.FDTDField(json.dumps(deck))
+
+
 def test_error_message_is_per_thread_and_cleared_by_success():
     deck = _base()
     deck["CompCell"]["PBC"] = True

@@ -58,6 +58,23 @@ def test_2d_tmz_cylinder_fp64(profile):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
+def test_2d_oblique_h_sources(precision, tol):
+    """Oblique Hx and Hy sheet sources (per-point time offsets, no dt factor)
+    beside the dielectric cylinder; the same adds in the oracle."""
+    d = json.loads(decks.config1_2d(n=64, steps=300, pml=10, cyl_radius_cells=12, src_cell=(20, 32), ndet=2))
+    d["GPU"]["precision"] = precision
+    pul = d["SourceList"][0]["PulseList"]
+    d["SourceList"] = [{"pol": "Hx", "size": [0.8, 0.4, 0.0], "loc": [-0.6, 0.3, 0.0], "phi": 30.0, "PulseList": pul},
+                       {"pol": "Hy", "size": [0.6, 0.2, 0.0], "loc": [0.7, -0.5, 0.0], "phi": 200.0, "PulseList": pul}]
+    deck = json.dumps(d)
+    print(f"oblique 2d precision={precision}", flush=True)
+    sim, orc = _pair(deck, 300)
+    _fields(sim, orc, (2, 3, 4), tol)
+    for k in range(orc.num_detectors()):
+        assert rel_linf(sim.dtcArr()[k].series()[1], orc.detector(k)[1]) <= tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
 def test_2d_detectors_on_every_component(precision, tol):
     """A small 2D deck whose detectors read Ez, Hx, Hy and E power (2D decks
     this small sample their point detectors inside the E kernel)."""
@@ -130,10 +147,23 @@ def _split_deck(variant, precision):
         for pol, loc in (("Hy", [-14 * dx, 0.0, 0.0]), ("Hz", [-13 * dx, 13 * dx, -14 * dx])):
             d["SourceList"].append({"pol": pol, "size": [0.0, 0.0, 0.0], "loc": loc,
                                     "PulseList": [decks.gaussian_pulse(5.0, 5.0, 3.0, 1.0)]})
+    elif variant == "circ":  # R and L sources (two E components each) and a centred phi = 90.3 source
+        pul = [decks.gaussian_pulse(5.0, 5.0, 3.0, 1.0)]
+        d["SourceList"] += [
+            {"pol": "R", "size": [0.0, 4 * dx, 6 * dx], "loc": [-5 * dx, 2 * dx, 0.0], "psi": 30.0, "PulseList": pul},
+            {"pol": "L", "size": [3 * dx, 3 * dx, 0.0], "loc": [4 * dx, -3 * dx, 5 * dx], "PulseList": pul},
+            {"pol": "Ez", "size": [2 * dx, 0.0, 0.0], "loc": [0.0, 6 * dx, -6 * dx], "phi": 90.3, "PulseList": pul}]
+    elif variant == "oblique":  # oblique Ez and Hz patches (half-cell delayed) and an Ex that adds nothing;
+        # the Ez patch stays off the material-filled CPML (its E points there are D-derived)
+        pul = [decks.gaussian_pulse(5.0, 5.0, 3.0, 1.0)]
+        d["SourceList"] = [
+            {"pol": "Ez", "size": [8 * dx, 4 * dx, 0.0], "loc": [2 * dx, 2 * dx, 6 * dx], "phi": 45.0, "PulseList": pul},
+            {"pol": "Hz", "size": [6 * dx, 6 * dx, 0.0], "loc": [4 * dx, -2 * dx, 8 * dx], "phi": 60.0, "PulseList": pul},
+            {"pol": "Ex", "size": [6 * dx, 6 * dx, 0.0], "loc": [0.0, 0.0, 9 * dx], "phi": 30.0, "PulseList": pul}]
     return json.dumps(d)
 
 
-@pytest.mark.parametrize("variant", ["cube", "wide", "thinz", "pmlsrc"])
+@pytest.mark.parametrize("variant", ["cube", "wide", "thinz", "pmlsrc", "circ", "oblique"])
 @pytest.mark.parametrize("precision,tol_f,tol_flux", [("fp64", FP64_TOL, 1e-8), ("fp32", FP32_TOL, FP32_TOL)])
 def test_3d_split_kernels(variant, precision, tol_f, tol_flux):
     deck = _split_deck(variant, precision)

@@ -811,6 +811,116 @@ def test_source_lands_on_its_cell_after_the_h_update():
     assert np.all(o.field(3) == 0) and np.all(o.field(4) == 0)
 
 
+def _circ_deck(pols, size_cells, psi=0.0, krat=1.0):
+    n, d = 16, 0.05
+    deck = decks.empty_deck((n, n, n), d, 6, 3)
+    pul = decks.gaussian_pulse(2.0, 2.0, 3.0, 1.0)
+    pul["t_0"] = 0.3
+    for pol in pols:
+        src = {"pol": pol, "size": [c * d for c in size_cells], "PulseList": [pul], "loc": [0.05, -0.1, 0.0]}
+        if pol in ("R", "L"):
+            src["psi"] = psi
+            src["ellpiticalKRat"] = krat
+        deck["SourceList"].append(src)
+    return json.dumps(deck)
+
+
+@pytest.mark.parametrize("size_cells, comps", [((0, 2, 3), (1, 2)), ((2, 0, 3), (0, 2)), ((2, 3, 0), (0, 1)),
+                                               ((2, 2, 2), (1, 2)), ((3, 2, 2), (0, 2))])
+def test_circular_source_is_two_unscaled_linear_sources(size_cells, comps):
+    """An R/L soft source drives the two E components across its thinnest
+    axis (first minimum of the size).  Its psi / axis-ratio prefactors land
+    after the parameter slot the pulse functions read, so after one step
+    each driven cell holds dt * pulse(0), unscaled, whatever psi, the axis
+    ratio or the handedness; after six steps the fields are the sum of the
+    two linear sources' runs (linearity of the update)."""
+    names = ("Ex", "Ey", "Ez")
+    ref = Oracle(_circ_deck([names[c] for c in comps], size_cells))
+    js = _circ_deck(["R"], size_cells, psi=30.0, krat=2.0)
+    o = Oracle(js)
+    o.step(1)
+    box = tuple(c + 1 for c in size_cells)
+    for c in range(3):
+        f = o.field(c)
+        if c in comps:
+            assert np.count_nonzero(f) == box[0] * box[1] * box[2]
+            assert np.all(f[f != 0] == pytest.approx(o.dt * pulse(js, 0, 0.0), rel=1e-14))
+        else:
+            assert np.all(f == 0)
+    o.step(5)
+    ref.step(6)
+    left = Oracle(_circ_deck(["L"], size_cells, psi=75.0))
+    left.step(6)
+    for c in range(6):
+        np.testing.assert_allclose(o.field(c), ref.field(c), rtol=0, atol=1e-15 * np.abs(ref.field(c)).max())
+        assert np.array_equal(o.field(c), left.field(c))
+
+
+def _oblique_deck(n3, pol, phi, size, steps=4):
+    d = 0.05
+    deck = decks.empty_deck(n3, d, steps, 3)
+    pul = decks.gaussian_pulse(2.0, 2.0, 3.0, 1.0)
+    pul["t_0"] = 0.3
+    deck["SourceList"].append({"pol": pol, "size": size, "phi": phi, "PulseList": [pul], "loc": [0.0, 0.0, 0.0]})
+    return json.dumps(deck)
+
+
+def test_oblique_source_3d_is_a_half_cell_delayed_z_patch():
+    """3D oblique sources: the polar angle is never read (90 deg), so the
+    propagation normal is z and only Ez/Hz carry a scale (1).  For phi = 45,
+    a 9 x 5 source centred at point 10 of a 21^3 grid, the reference's
+    box arithmetic gives x in [3, 9] (the lower bound pulled down by
+    |round(7 - 5 cos 135)| = 3, the upper by |round(13 - 3.54)| = 9),
+    y in [7, 13], z = 10.  After one step from rest each Ez point holds
+    Re(pulse(-dz/2)) -- no dt factor -- and nothing else is set."""
+    js = _oblique_deck((21, 21, 21), "Ez", 45.0, [0.4, 0.2, 0.0])
+    o = Oracle(js)
+    o.step(1)
+    ez = o.field(2)
+    want = np.zeros_like(ez)
+    want[10, 7:14, 3:10] = pulse(js, 0, -0.025)
+    assert np.count_nonzero(want) == 49 and want[10, 7, 3] != 0
+    np.testing.assert_array_equal(ez, want)
+    for c in (0, 1, 3, 4, 5):
+        assert np.all(o.field(c) == 0)
+    # Ex / Ey oblique sources in 3D scale by sin(0) = 0: nothing is ever added
+    ex = Oracle(_oblique_deck((21, 21, 21), "Ex", 30.0, [0.4, 0.2, 0.0]))
+    ex.step(4)
+    assert all(np.all(ex.field(c) == 0) for c in range(6))
+
+
+def test_oblique_source_2d_h_sheet_cells_scales_and_offsets():
+    """2D TMz oblique Hx source, phi = 30: theta is forced to 90 deg, so the
+    normal is n = (cos 30, sin 30) and H is driven along (cos 120, sin 120):
+    Hy by sin 120 and Hx by cos 120.  The box is x in [8, 12], y in [7, 13]
+    around point (10, 10) for a 7 x 5 source; the points kept are those within
+    dx/sqrt(2) of the line through the centre normal to n, at each component's
+    own Yee offset (Hy half a cell along x, Hx along y).  Each adds
+    scale * Re(pulse(-n.p)) at the first step, with no dt factor."""
+    d = 0.05
+    js = _oblique_deck((21, 21, 1), "Hx", 30.0, [0.3, 0.2, 0.0])
+    o = Oracle(js)
+    o.step(1)
+    nv = np.array([math.cos(math.pi / 6), math.sin(math.pi / 6)])
+    xs, ys = np.meshgrid(np.arange(8, 13), np.arange(7, 14), indexing="xy")
+    for comp, off, scale in ((4, (0.5, 0.0), math.sin(2 * math.pi / 3)), (3, (0.0, 0.5), math.cos(2 * math.pi / 3))):
+        px, py = (xs - 10 + off[0]) * d, (ys - 10 + off[1]) * d
+        toff = px * nv[0] + py * nv[1]
+        keep = np.abs(toff) <= d / math.sqrt(2.0)
+        assert 5 <= keep.sum() < keep.size
+        want = np.zeros((21, 21))
+        for x, y, t in zip(xs[keep], ys[keep], toff[keep]):
+            want[y, x] = scale * pulse(js, 0, -t)
+        got = o.field(comp)[0]
+        assert np.array_equal(got != 0, want != 0)
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+    # an oblique Ez source in 2D scales by cos(90 deg) = 6.1e-17
+    ez = Oracle(_oblique_deck((21, 21, 1), "Ez", 30.0, [0.3, 0.2, 0.0]))
+    ez.step(1)
+    f = ez.field(2)[0]
+    assert f[10, 10] == pytest.approx(math.cos(math.pi / 2) * pulse(js, 0, 0.0), rel=1e-13)
+
+
 # ------------------------------------------------ emitter back-coupling -----
 def test_emitter_slab_linear_response_equals_lorentz_slab():
     """In linear response a 2-level emitter with transition frequency w0,
