@@ -632,7 +632,6 @@ void FDTDField::scanMatInPML() {
                             if (!dielectricMatInPML && o.isObj(pt, d[0], false)) dielectricMatInPML = diel;
                         }
                     }
-            if (o.mu != 1.0) throw std::logic_error("magnetic materials (mu != 1) are outside the supported time-stepping path");
         }
     }
 }
@@ -647,8 +646,6 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
     const int nsteps = IP.nSteps();
     check_pml_axes(IP);
     scanMatInPML();
-    for (auto& o : IP.objs)
-        if (o->mu != 1.0) throw std::logic_error("magnetic materials (mu != 1) are outside the supported time-stepping path");
 
     chiml_problem_t P;
     std::memset(&P, 0, sizeof(P));
@@ -817,8 +814,14 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         P.mat_e_poleset[c] = psT[c].data();
         for (int j = 0; j < 6; ++j) P.disp_box[c][j] = dispBox[c][j];
     }
-    static const double kOne = 1.0;
+    // H points: object id (with emitters) and mu of the last object holding
+    // the point without the emitter test (setupPhysFields for physH_/muRel_,
+    // parallelFDTDField.hpp:270-272). With mu != 1 anywhere, an H map of
+    // material ids with coefficients dt/mu (the upH prefactor dt/(mu d),
+    // parallelFDTDField.hpp:628-666) is built per component.
     double chv = dt;
+    std::vector<uint8_t> matHv[3];
+    std::vector<double> chT[3];
     for (int hc : hcomps) {
         int c = hc - 3;
         P.mat_h[c] = nullptr;
@@ -826,18 +829,49 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         P.mat_h_ch[c] = &chv;
         const int* st = kStag[hc];
         physOf[hc].assign(N, 0);
+        std::vector<int> muObj(N, 0);
         parallel_chunks(nz * ny, [&](int lo, int hi, int) {
             for (int r = lo; r < hi; ++r) {
                 const int z = r / ny, y = r % ny;
                 for (int x = 0; x < nx; ++x) {
                     std::array<double, 3> pt = {{coord(0, x, 0.5 * st[0]), coord(1, y, 0.5 * st[1]), is3D ? coord(2, z, 0.5 * st[2]) : 0.0}};
                     if (!is3D) pt[2] = (-1 + 0.5 * st[2] - (nAll[2] - nAll[2] % 2) / 2.0) * IP.d[2];  // kk - 1 with kk = 0
-                    physOf[hc][x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)] = objAt(pt, true);
+                    const size_t i = x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z);
+                    int phys, mo;
+                    objAt2(pt, phys, mo);
+                    physOf[hc][i] = phys;
+                    muObj[i] = mo;
                 }
             }
         });
+        bool anyMu = false;
+        for (auto& op : IP.objs) anyMu = anyMu || op->mu != 1.0;
+        if (!anyMu) continue;
+        std::map<double, int> idOf;
+        chT[c].clear();
+        matHv[c].assign(N, 0);
+        for (size_t i = 0; i < N; ++i) {
+            const double mu = IP.objs[muObj[i]]->mu;
+            if (mu != 1.0) {
+                const int x = static_cast<int>(i % nx), y = static_cast<int>((i / nx) % ny), z = static_cast<int>(i / (static_cast<size_t>(nx) * ny));
+                if (inPML(x, y, z)) throw std::logic_error("magnetic material (mu != 1) inside the PML is outside the supported time-stepping path");
+            }
+            auto it = idOf.find(mu);
+            int id;
+            if (it == idOf.end()) {
+                id = static_cast<int>(chT[c].size());
+                if (id > 255) throw std::logic_error("more than 256 distinct mu values on one H component");
+                idOf.emplace(mu, id);
+                chT[c].push_back(dt / mu);
+            } else {
+                id = it->second;
+            }
+            matHv[c][i] = static_cast<uint8_t>(id);
+        }
+        P.mat_h[c] = matHv[c].data();
+        P.nmat_h[c] = static_cast<int>(chT[c].size());
+        P.mat_h_ch[c] = chT[c].data();
     }
-    (void)kOne;
     P.npoleset = static_cast<int>(pcount.size());
     P.pole_count = pcount.data();
     P.pole_coef = pcoef.data();

@@ -94,6 +94,16 @@ def test_deck_errors(mutate, message):
         _estimate(deck)
 
 
+def test_magnetic_material_in_the_cpml_is_rejected():
+    """mu != 1 on an H point inside the CPML takes the reference's B-field
+    path, which is not built: the host rejects it before any device call."""
+    deck = _base()
+    deck["ObjectList"].append({"shape": "block", "material": "custom", "eps": 1.0, "mu": 2.0, "pols": [],
+                               "size": [1.5, 0.4, 0.0], "loc": [0, 0, 0], "Basis_Set": []})
+    with pytest.raises(chiml_amd.ChimlError, match="inside the PML"):
+        chiml_amd.FDTDField(json.dumps(deck))
+
+
 def _oblique(pol, size, loc, phi=30.0):
     return {"pol": pol, "size": size, "loc": loc, "phi": phi, "PulseList": [decks.gaussian_pulse(1.0, 1.0, 3.0, 1.0)]}
 

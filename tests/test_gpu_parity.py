@@ -58,6 +58,19 @@ def test_2d_tmz_cylinder_fp64(profile):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
+def test_2d_magnetic_cylinder(precision, tol):
+    """A cylinder with mu = 2.5 and eps = 4: the Hx/Hy updates take dt/mu from
+    the H material map."""
+    d = json.loads(decks.config1_2d(n=64, steps=300, pml=10, cyl_radius_cells=12, src_cell=(20, 32), ndet=2))
+    d["GPU"]["precision"] = precision
+    for o in d["ObjectList"]:
+        o["mu"] = 2.5
+    print(f"mu 2d precision={precision}", flush=True)
+    sim, orc = _pair(json.dumps(d), 300)
+    _fields(sim, orc, (2, 3, 4), tol)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
 def test_2d_oblique_h_sources(precision, tol):
     """Oblique Hx and Hy sheet sources (per-point time offsets, no dt factor)
     beside the dielectric cylinder; the same adds in the oracle."""
@@ -160,10 +173,14 @@ def _split_deck(variant, precision):
             {"pol": "Ez", "size": [8 * dx, 4 * dx, 0.0], "loc": [2 * dx, 2 * dx, 6 * dx], "phi": 45.0, "PulseList": pul},
             {"pol": "Hz", "size": [6 * dx, 6 * dx, 0.0], "loc": [4 * dx, -2 * dx, 8 * dx], "phi": 60.0, "PulseList": pul},
             {"pol": "Ex", "size": [6 * dx, 6 * dx, 0.0], "loc": [0.0, 0.0, 9 * dx], "phi": 30.0, "PulseList": pul}]
+    elif variant == "mu":  # a magnetic dielectric block (mu = 1.8) around the sphere, clear of the CPML:
+        # the H map takes the generic update kernels
+        d["ObjectList"].append({"shape": "block", "material": "custom", "eps": 1.5, "mu": 1.8, "pols": [],
+                                "size": [10 * dx, 12 * dx, 8 * dx], "loc": [2 * dx, -dx, 3 * dx], "Basis_Set": []})
     return json.dumps(d)
 
 
-@pytest.mark.parametrize("variant", ["cube", "wide", "thinz", "pmlsrc", "circ", "oblique"])
+@pytest.mark.parametrize("variant", ["cube", "wide", "thinz", "pmlsrc", "circ", "oblique", "mu"])
 @pytest.mark.parametrize("precision,tol_f,tol_flux", [("fp64", FP64_TOL, 1e-8), ("fp32", FP32_TOL, FP32_TOL)])
 def test_3d_split_kernels(variant, precision, tol_f, tol_flux):
     deck = _split_deck(variant, precision)
