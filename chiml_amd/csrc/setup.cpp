@@ -178,6 +178,24 @@ bool Object::isObj(const std::array<double, 3>& v, double dx, bool useML) const 
             else if (t[1] < 0) ph += 2.0 * kPi;
             return !(ph < g[2] || ph > g[3]);
         }
+        case Shape::CONE: {  // src/OBJECTS/Obj.cpp:512-521: geo = radLarge, length, height
+            // The length bounds object y, but the radius test uses object z,
+            // and compares a distance with a squared radius.
+            auto t = toObjectSpace(*this, v);
+            if ((t[1] < -1.0 * g[1] / 2.0 - dx * 1e-6) || (t[1] > g[1] / 2.0 + dx * 1e-6)) return false;
+            if (std::sqrt(t[0] * t[0] + t[2] * t[2]) > std::pow(g[0] / g[2] * (t[2] - g[1] / 2.0), 2.0)) return false;
+            return true;
+        }
+        case Shape::TERS_TIP: {  // src/OBJECTS/Obj.cpp:535-548: geo = 2 rad_curve, base, length
+            auto t = toObjectSpace(*this, v);
+            double d2 = 0.0;
+            for (int i = 0; i < 3; ++i) d2 += (t[i] - radCen[i]) * (t[i] - radCen[i]);
+            if (std::sqrt(d2) < g[0] / 2.0) return true;
+            if ((t[2] < g[0] - g[2] / 2.0 - dx * 1e-6) || (t[2] > g[1] / 2.0 + dx * 1e-6)) return false;
+            // the cone's half-angle is taken as geo[0] (2 rad_curve), in radians
+            if ((t[0] * t[0] + t[1] * t[1]) * std::pow(std::cos(g[0]), 2.0) - std::pow(t[2] * std::sin(g[0]), 2.0) > dx * 1e-6) return false;
+            return true;
+        }
         case Shape::HEMIELLIPSOID: {  // src/OBJECTS/Obj.cpp:459-471
             auto t = toObjectSpace(*this, v);
             if (t[0] > dx / 1e6) return false;
@@ -329,6 +347,17 @@ static std::shared_ptr<Object> ptreeToObject(const Value& item, const ProgramInp
             o->cutPos[i] = cp[i];
             o->cutGlobal[i] = cg[i];
         }
+    } else if (s == "cone") {  // parallelInputs.cpp:1516-1527
+        o->shape = Shape::CONE;
+        const double rad1 = json::get_num(item, "radSmall", 0.0), rad2 = json::get_num(item, "radLarge", 0.0);
+        if (rad2 <= rad1) throw std::logic_error("radLarge for a cone must be greater than radSmall.");
+        const double len = json::get_num(item, "height", 0.0);
+        const double raiseAng = std::atan((rad2 - rad1) / len);
+        o->geo = {rad2, len, rad2 * std::tan(raiseAng)};
+    } else if (s == "ters_tip") {  // parallelInputs.cpp:1570-1574, ters_tip::ters_tip Obj.cpp:202-206
+        o->shape = Shape::TERS_TIP;
+        o->geo = {2.0 * json::get_num(item, "rad_curve", 0.0), json::get_num(item, "base", 0.0), json::get_num(item, "length", 0.0)};
+        o->radCen = {{0.0, 0.0, -1.0 * o->geo[2] / 2.0 + o->geo[0] / 2.0}};
     } else if (s == "hemisphere") {  // parallelInputs.cpp:1506-1510
         o->shape = Shape::HEMISPHERE;
         o->geo = {json::get_num(item, "radius", 0.0)};
@@ -360,6 +389,8 @@ static void addMLBuff(Object& o, double d) {
         case Shape::PARABOLOID:
         case Shape::TORUS: o.geoML[0] += 3.0 * d; o.geoML[1] += 3.0 * d; break;
         case Shape::CYLINDER: o.geoML[0] += 3.0 * d; o.geoML[1] += 3.0 * d; break;
+        case Shape::CONE: o.geoML[1] += 3.0 * d; break;
+        case Shape::TERS_TIP: for (int i = 0; i < 3; ++i) o.geoML[i] += 3.0 * d; break;
         case Shape::BLOCK:
         case Shape::ELLIPSOID:
         case Shape::HEMIELLIPSOID:

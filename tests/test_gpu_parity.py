@@ -326,6 +326,31 @@ def test_3d_hemisphere_hemiellipsoid_and_cut_ellipsoid_fp64():
     _fields(sim, orc, range(6), FP64_TOL)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
+def test_3d_cone_and_ters_tip(precision, tol):
+    """A rotated cone (dispersive) and a TERS tip through the product path against the oracle."""
+    d = decks.empty_deck((24, 24, 24), 0.02, 80, 4, precision=precision)
+    base = {"material": "custom", "Basis_Set": [],
+            "unit_vectors": [{"uvec": [0.8, 0.6, 0.0]}, {"uvec": [-0.6, 0.8, 0.0]}, {"uvec": [0.0, 0.0, 1.0]}]}
+    d["ObjectList"].append(dict(base, shape="cone", eps=1.0, pols=decks.gold_like_pols()[:2], radSmall=0.03,
+                                radLarge=0.09, height=0.12, loc=[-0.06, 0.05, 0.02]))
+    d["ObjectList"].append(dict(base, shape="ters_tip", eps=2.4, pols=[], rad_curve=0.04, base=0.16, length=0.2,
+                                loc=[0.07, -0.05, -0.01]))
+    fc = 1.0 / (8 * 0.02)
+    d["SourceList"].append({"pol": "Ez", "size": [0.0, 0.0, 0.0], "loc": [0.0, 0.0, 0.05],
+                            "PulseList": [decks.gaussian_pulse(fc, fc, 3.0, 1.0)]})
+    print(f"cone and TERS tip precision={precision}", flush=True)
+    sim, orc = _pair(json.dumps(d), 80)
+    # each component against its family's peak: Hz is 1% of Hx/Hy here, so its own peak
+    # would hold FP32 rounding of the larger components to a 100x tighter bound
+    for fam in ((0, 1, 2), (3, 4, 5)):
+        peak = max(np.abs(orc.field(c)).max() for c in fam)
+        for c in fam:
+            err = np.abs(sim.field(c) - orc.field(c)).max() / peak
+            print(f"field {c}: family peak {peak:.3e} err {err:.3e}", flush=True)
+            assert err <= tol, (c, err)
+
+
 def test_3d_basis_set_emitters_x_y_dipoles_fp64():
     """A 3x3x3 block of 4-level emitters whose dipoles come from the
     spherical-harmonic basis {(0,0), (1,-1), (1,0), (1,1)} (x and y matrix

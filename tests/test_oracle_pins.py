@@ -1277,6 +1277,53 @@ def test_paraboloid_rasterisation():
     _check_raster([_obj("paraboloid", eps, loc, a=a, b=b, length=length)], inside, eps, 23, d)
 
 
+def test_cone_rasterisation():
+    """Cone (Obj.cpp:512-521, parse parallelInputs.cpp:1516-1527): geo =
+    (R, L, h) with h = R tan(atan((R - r) / L)).  The reference bounds object
+    y by L / 2 but tests the radius in the x-z plane against a squared
+    height term in z: sqrt(t_x^2 + t_z^2) <= (R / h (t_z - L / 2))^2."""
+    d = 0.05
+    loc = np.array([0.31, -0.27, 0.43]) * d
+    R, r, L = 6.13 * d, 2.07 * d, 8.31 * d
+    h = R * math.tan(math.atan((R - r) / L))
+    eps = (2.0, 2.5, 3.0)
+
+    def inside(p):
+        t = [p[k] - loc[k] for k in range(3)]
+        rad = np.sqrt(t[0] ** 2 + t[2] ** 2)
+        lim = (R / h * (t[2] - L / 2)) ** 2
+        ins = (np.abs(t[1]) <= L / 2) & (rad <= lim)
+        return ins, min(float(np.min(np.abs(np.abs(t[1]) - L / 2))), float(np.min(np.abs(rad - lim))))
+
+    _check_raster([_obj("cone", eps, loc, radSmall=r, radLarge=R, height=L)], inside, eps, 23, d)
+
+
+def test_ters_tip_rasterisation():
+    """TERS tip (Obj.cpp:535-548, parse parallelInputs.cpp:1570-1574,
+    centre Obj.cpp:202-206): geo = (2 rc, base, length).  A point is inside
+    when it lies strictly within rc of (0, 0, rc - length / 2), or when
+    2 rc - length / 2 <= t_z <= base / 2 and
+    (t_x^2 + t_y^2) cos^2(2 rc) <= (t_z sin(2 rc))^2 -- the reference takes
+    2 rc, a length, as the cone's half-angle in radians."""
+    d = 0.05
+    loc = np.array([0.23, 0.37, -0.41]) * d
+    rc, base, length = 7.03 * d, 10.11 * d, 19.87 * d
+    eps = (2.0, 2.5, 3.0)
+    cen = np.array([0.0, 0.0, rc - length / 2])
+
+    def inside(p):
+        t = [p[k] - loc[k] for k in range(3)]
+        ball = np.sqrt(sum((t[k] - cen[k]) ** 2 for k in range(3)))
+        cone = (t[0] ** 2 + t[1] ** 2) * math.cos(2 * rc) ** 2 - (t[2] * math.sin(2 * rc)) ** 2
+        zin = (t[2] >= 2 * rc - length / 2) & (t[2] <= base / 2)
+        ins = (ball < rc) | (zin & (cone <= 0))
+        margin = min(float(np.min(np.abs(ball - rc))), float(np.min(np.abs(t[2] - (2 * rc - length / 2)))),
+                     float(np.min(np.abs(t[2] - base / 2))), float(np.min(np.abs(cone))) / d)
+        return ins, margin
+
+    _check_raster([_obj("ters_tip", eps, loc, rad_curve=rc, base=base, length=length)], inside, eps, 23, d)
+
+
 def test_partial_torus_rasterisation():
     """Torus section (Obj.cpp:562-575, parse parallelInputs.cpp:1580-1584):
     (rho - R)^2 <= r^2 - t_z^2 with rho the distance from the object z axis,
