@@ -124,6 +124,50 @@ static std::array<double, 3> toObjectSpace(const Object& o, const std::array<dou
     return r;
 }
 
+// Inverse of an n x n column-major matrix by Gauss-Jordan elimination with
+// partial pivoting (the reference uses dgetrf + dgetri); returns false when
+// the matrix is singular.
+static bool invertMatrix(std::vector<double> a, int n, std::vector<double>& inv) {
+    inv.assign(static_cast<size_t>(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) inv[i + n * i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::abs(a[r + n * c]) > std::abs(a[piv + n * c])) piv = r;
+        if (a[piv + n * c] == 0.0) return false;
+        if (piv != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(a[c + n * k], a[piv + n * k]);
+                std::swap(inv[c + n * k], inv[piv + n * k]);
+            }
+        const double s = 1.0 / a[c + n * c];
+        for (int k = 0; k < n; ++k) {
+            a[c + n * k] *= s;
+            inv[c + n * k] *= s;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = a[r + n * c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                a[r + n * k] -= f * a[c + n * k];
+                inv[r + n * k] -= f * inv[c + n * k];
+            }
+        }
+    }
+    return true;
+}
+
+// cart2bary (utilityFxns.hpp:130-141): dgemv('T') of the inverse vertex
+// matrix with [p, 1].
+static void cart2bary(const std::vector<double>& inv, int n, const double* p, double* bary) {
+    for (int j = 0; j < n; ++j) {
+        double acc = 0.0;
+        for (int r = 0; r < n; ++r) acc += inv[r + n * j] * (r < n - 1 ? p[r] : 1.0);
+        bary[j] = acc;
+    }
+}
+
 bool Object::isObj(const std::array<double, 3>& v, double dx, bool useML) const {
     const std::vector<double>& g = useML ? geoML : geo;
     switch (shape) {
@@ -194,6 +238,26 @@ bool Object::isObj(const std::array<double, 3>& v, double dx, bool useML) const 
             if ((t[2] < g[0] - g[2] / 2.0 - dx * 1e-6) || (t[2] > g[1] / 2.0 + dx * 1e-6)) return false;
             // the cone's half-angle is taken as geo[0] (2 rad_curve), in radians
             if ((t[0] * t[0] + t[1] * t[1]) * std::pow(std::cos(g[0]), 2.0) - std::pow(t[2] * std::sin(g[0]), 2.0) > dx * 1e-6) return false;
+            return true;
+        }
+        case Shape::TRI_PRISM: {  // src/OBJECTS/Obj.cpp:473-494; geo = barycentric cutoffs, length
+            auto t = toObjectSpace(*this, v);
+            if ((t[2] < -1.0 * g[3] / 2.0 - dx * 1e-6) || (t[2] > g[3] / 2.0 + dx * 1e-6)) return false;
+            double b[3];
+            cart2bary(invVert, 3, t.data(), b);
+            for (int i = 0; i < 3; ++i)
+                if (b[i] < -1e-13 || b[i] > g[i] + 1e-13) return false;
+            if (std::abs(1 - (b[0] + b[1] + b[2])) > 1e-14)
+                throw std::logic_error("The sum of the boundary checks for a triangular base does not equal the determinant of the verticies, despite passing the checks. This is an error.");
+            return true;
+        }
+        case Shape::TETRAHEDRON: {  // src/OBJECTS/Obj.cpp:496-510; real-space vertices, geo = cutoffs
+            double b[4];
+            cart2bary(invVert, 4, v.data(), b);
+            for (int i = 0; i < 4; ++i)
+                if (b[i] < -1e-6 * dx || (b[i] + dx * 1e-6) > g[i]) return false;
+            if (std::abs(1 - (((b[0] + b[1]) + b[2]) + b[3])) > 1e-14)
+                throw std::logic_error("The sum of the boundary checks for a tetrahedron does not equal the determinant of the verticies, despite passing the checks. This is an error.");
             return true;
         }
         case Shape::HEMIELLIPSOID: {  // src/OBJECTS/Obj.cpp:459-471
@@ -358,6 +422,90 @@ static std::shared_ptr<Object> ptreeToObject(const Value& item, const ProgramInp
         o->shape = Shape::TERS_TIP;
         o->geo = {2.0 * json::get_num(item, "rad_curve", 0.0), json::get_num(item, "base", 0.0), json::get_num(item, "length", 0.0)};
         o->radCen = {{0.0, 0.0, -1.0 * o->geo[2] / 2.0 + o->geo[0] / 2.0}};
+    } else if (s == "triangle_prism" || s == "trapezoid_prism") {
+        // parallelInputs.cpp:1528-1569, tri_prism::tri_prism Obj.cpp:147-200:
+        // vertices in object x-y, moved to their centroid (the object-space
+        // centroid is added to the real-space location as it is).
+        o->shape = Shape::TRI_PRISM;
+        std::array<std::array<double, 2>, 3> vl;
+        const Value& vls = json::get_child_or_empty(item, "vertLocs");
+        if (s == "trapezoid_prism") {
+            const double b1 = json::get_num(item, "base1", 0.0), b2 = json::get_num(item, "base2", 0.0);
+            const double base = std::max(b1, b2), height = json::get_num(item, "height", 0.0);
+            const double hTri = base * 2.0 * height / (b1 - b2);
+            vl = {{{{0, 0.5 * hTri}}, {{0.5 * base, -0.5 * hTri}}, {{-0.5 * base, -0.5 * hTri}}}};
+            o->geo = {std::abs(height / hTri), 1.0, 1.0, json::get_num(item, "length", 0.0)};
+        } else if (vls.size() > 0) {
+            if (vls.size() != 3) throw std::logic_error("A triangle has exactly 3 verticies.");
+            for (int k = 0; k < 3; ++k) {
+                std::vector<double> vv = json::get_vec(vls.child(k), "vert");
+                vl[k] = {{vv.size() > 0 ? vv[0] : 0.0, vv.size() > 1 ? vv[1] : 0.0}};
+            }
+            std::vector<double> cut = json::get_vec(item, "barry_center_cutoff");
+            if (cut.empty()) cut.assign(3, 1.0);
+            if (cut.size() != 3) throw std::logic_error("barry_center_cutoff needs 3 values for a triangle");
+            o->geo = cut;
+            o->geo.push_back(json::get_num(item, "length", 0.0));
+        } else {
+            const double base = json::get_num(item, "base", 0.0), height = json::get_num(item, "height", 0.0);
+            vl = {{{{0.5 * base, -0.5 * height}}, {{-0.5 * base, -0.5 * height}}, {{0.0, 0.5 * height}}}};
+            o->geo = {1.0, 1.0, 1.0, json::get_num(item, "length", 0.0)};
+        }
+        std::array<double, 3> cen{{0.0, 0.0, 0.0}};
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 2; ++a) cen[a] = cen[a] + vl[k][a] / 3.0;
+        for (int a = 0; a < 3; ++a) o->loc[a] = o->loc[a] + cen[a];
+        std::vector<double> m(9, 1.0);
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 2; ++a) m[k + 3 * a] = vl[k][a] - cen[a];
+        if (!invertMatrix(m, 3, o->invVert)) throw std::logic_error("The triangular base is a line.");
+    } else if (s == "tetrahedron") {
+        // parallelInputs.cpp:1585-1616, tetrahedron::tetrahedron Obj.cpp:215-276
+        o->shape = Shape::TETRAHEDRON;
+        std::array<std::array<double, 3>, 4> vl;
+        const Value& vls = json::get_child_or_empty(item, "vertLocs");
+        std::vector<double> m(16, 1.0);
+        if (vls.size() > 0) {
+            if (vls.size() != 4) throw std::logic_error("A tetrahedron has exactly 4 verticies.");
+            for (int k = 0; k < 4; ++k) {
+                double vv[3];
+                json::get_arr3(vls.child(k), "vert", vv);
+                vl[k] = {{vv[0], vv[1], vv[2]}};
+            }
+            std::vector<double> cut = json::get_vec(item, "barry_center_cutoff");
+            if (cut.empty()) cut.assign(4, 1.0);
+            if (cut.size() != 4) throw std::logic_error("barry_center_cutoff needs 4 values for a tetrahedron");
+            o->geo = cut;
+            o->loc = {{0.0, 0.0, 0.0}};
+            for (int k = 0; k < 4; ++k)
+                for (int a = 0; a < 3; ++a) m[k + 4 * a] = vl[k][a];
+            if (!invertMatrix(m, 4, o->invVert)) throw std::logic_error("The tetrahedron is planer.");
+        } else {
+            double side = json::get_num(item, "sideLen", 2.0 / std::sqrt(3.0) * json::get_num(item, "perpBiSect", -1.0));
+            if (side == -1) throw std::logic_error("please give a side length or prepindicular bisector length for the tetrahedron.");
+            const double height = json::get_num(item, "height", std::sqrt(2.0 / 3.0) * side);
+            const double canon[4][3] = {{std::sqrt(8.0 / 9.0), 0, -1.0 / 3.0},
+                                        {-1.0 * std::sqrt(2.0 / 9.0), std::sqrt(2.0 / 3.0), -1.0 / 3.0},
+                                        {-1.0 * std::sqrt(2.0 / 9.0), -1.0 * std::sqrt(2.0 / 3.0), -1.0 / 3.0},
+                                        {0, 0, 1}};
+            // vertex = T^T-free dgemv('T', coordTransform_) of the canonical
+            // vertex (component i = u_i . v), scaled by side sqrt(3/8), + loc
+            for (int k = 0; k < 4; ++k)
+                for (int a = 0; a < 3; ++a) {
+                    const double r = o->T[3 * a] * canon[k][0] + o->T[3 * a + 1] * canon[k][1] + o->T[3 * a + 2] * canon[k][2];
+                    vl[k][a] = r * (side * std::sqrt(3.0 / 8.0)) + o->loc[a];
+                }
+            for (int k = 0; k < 4; ++k)
+                for (int a = 0; a < 3; ++a) m[k + 4 * a] = vl[k][a];
+            if (!invertMatrix(m, 4, o->invVert)) throw std::logic_error("The tetrahedron is planer.");
+            // height cut: the fourth barycentric coordinate of loc + heightRat * vertex 3
+            // (vertex 3 already carries loc, Obj.cpp:246-250)
+            const double heightRat = (height / (side * std::sqrt(2.0 / 3.0)) * (5.0 / 4.0)) - 1.0 / 4.0;
+            double v4[3], b[4];
+            for (int a = 0; a < 3; ++a) v4[a] = o->loc[a] + heightRat * vl[3][a];
+            cart2bary(o->invVert, 4, v4, b);
+            o->geo = {1.0, 1.0, 1.0, b[3]};
+        }
     } else if (s == "hemisphere") {  // parallelInputs.cpp:1506-1510
         o->shape = Shape::HEMISPHERE;
         o->geo = {json::get_num(item, "radius", 0.0)};
@@ -391,6 +539,9 @@ static void addMLBuff(Object& o, double d) {
         case Shape::CYLINDER: o.geoML[0] += 3.0 * d; o.geoML[1] += 3.0 * d; break;
         case Shape::CONE: o.geoML[1] += 3.0 * d; break;
         case Shape::TERS_TIP: for (int i = 0; i < 3; ++i) o.geoML[i] += 3.0 * d; break;
+        case Shape::TRI_PRISM:
+        case Shape::TETRAHEDRON:
+            throw std::logic_error("emitters in a triangular prism or tetrahedron are not supported");
         case Shape::BLOCK:
         case Shape::ELLIPSOID:
         case Shape::HEMIELLIPSOID:

@@ -30,7 +30,7 @@ enum class Pol { EX = 0, EY, EZ, HX, HY, HZ, L, R };
 enum class PulseShape { GAUSSIAN, BH, RECT, CONTINUOUS, RICKER, RAMP_CONT };
 enum class DtcType { EX = 0, EY, EZ, HX, HY, HZ, EPOW, HPOW };
 enum class DtcClass { BIN, TXT, COUT, FREQ };
-enum class Shape { BLOCK, SPHERE, CYLINDER, ELLIPSOID, HEMISPHERE, HEMIELLIPSOID, PARABOLOID, TORUS, CONE, TERS_TIP };
+enum class Shape { BLOCK, SPHERE, CYLINDER, ELLIPSOID, HEMISPHERE, HEMIELLIPSOID, PARABOLOID, TORUS, CONE, TERS_TIP, TRI_PRISM, TETRAHEDRON };
 
 Pol string2pol(const std::string& p);
 PulseShape string2prof(const std::string& p);
@@ -57,6 +57,10 @@ struct Object {
     // ellipsoid cuts (cut_neg, cut_pos, cut_global), fractions of the half axes
     std::array<double, 3> cutNeg{{-1, -1, -1}}, cutPos{{1, 1, 1}}, cutGlobal{{1, 1, 1}};
     std::array<double, 3> radCen{{0, 0, 0}};  // TERS tip: centre of the apex sphere, from geo (not geoML)
+    // Triangular prism / tetrahedron: the inverse of the vertex matrix (rows
+    // [vertex, 1]), column-major, 3 x 3 or 4 x 4; barycentric coordinates of
+    // p are inv^T [p, 1].
+    std::vector<double> invVert;
     std::array<double, 3> eps{{1, 1, 1}};  // eps_infty per Yee component (x,y,z)
     double epsScalar = 1.0;  // the deck's scalar "eps" (reference epsInfty())
     double mu = 1.0;

@@ -327,8 +327,9 @@ def test_3d_hemisphere_hemiellipsoid_and_cut_ellipsoid_fp64():
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
-def test_3d_cone_and_ters_tip(precision, tol):
-    """A rotated cone (dispersive) and a TERS tip through the product path against the oracle."""
+def test_3d_cone_ters_tip_prisms_tetrahedron(precision, tol):
+    """Rotated cone (dispersive), TERS tip, triangular and trapezoidal prisms and a dispersive
+    tetrahedron through the product path against the oracle."""
     d = decks.empty_deck((24, 24, 24), 0.02, 80, 4, precision=precision)
     base = {"material": "custom", "Basis_Set": [],
             "unit_vectors": [{"uvec": [0.8, 0.6, 0.0]}, {"uvec": [-0.6, 0.8, 0.0]}, {"uvec": [0.0, 0.0, 1.0]}]}
@@ -336,6 +337,12 @@ def test_3d_cone_and_ters_tip(precision, tol):
                                 radLarge=0.09, height=0.12, loc=[-0.06, 0.05, 0.02]))
     d["ObjectList"].append(dict(base, shape="ters_tip", eps=2.4, pols=[], rad_curve=0.04, base=0.16, length=0.2,
                                 loc=[0.07, -0.05, -0.01]))
+    d["ObjectList"].append(dict(base, shape="triangle_prism", eps=1.9, pols=[], base=0.12, height=0.1, length=0.08,
+                                loc=[0.1, 0.1, 0.1]))
+    d["ObjectList"].append(dict(base, shape="trapezoid_prism", eps=2.1, pols=[], base1=0.14, base2=0.06, height=0.06,
+                                length=0.08, loc=[-0.12, -0.1, -0.12]))
+    d["ObjectList"].append(dict(base, shape="tetrahedron", eps=1.0, pols=decks.gold_like_pols()[:2], sideLen=0.14,
+                                loc=[0.11, -0.12, 0.11]))
     fc = 1.0 / (8 * 0.02)
     d["SourceList"].append({"pol": "Ez", "size": [0.0, 0.0, 0.0], "loc": [0.0, 0.0, 0.05],
                             "PulseList": [decks.gaussian_pulse(fc, fc, 3.0, 1.0)]})

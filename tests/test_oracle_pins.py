@@ -1324,6 +1324,104 @@ def test_ters_tip_rasterisation():
     _check_raster([_obj("ters_tip", eps, loc, rad_curve=rc, base=base, length=length)], inside, eps, 23, d)
 
 
+def _in_hull(pts, verts):
+    """Inside test of a convex polygon (2D) or tetrahedron (3D) by half-planes:
+    each face's outward normal from the opposite vertex; returns (inside,
+    smallest distance to a face plane)."""
+    verts = np.asarray(verts, float)
+    nv = len(verts)
+    ins = np.ones(pts[0].shape, bool)
+    margin = np.inf
+    for f in range(nv):
+        face = [verts[k] for k in range(nv) if k != f]
+        if nv == 3:
+            e = face[1] - face[0]
+            nrm = np.array([e[1], -e[0]])
+        else:
+            nrm = np.cross(face[1] - face[0], face[2] - face[0])
+        nrm = nrm / np.linalg.norm(nrm)
+        if np.dot(verts[f] - face[0], nrm) > 0:
+            nrm = -nrm
+        dist = sum((pts[k] - face[0][k]) * nrm[k] for k in range(len(nrm)))
+        ins &= dist <= 0
+        margin = min(margin, float(np.min(np.abs(dist))))
+    return ins, margin
+
+
+@pytest.mark.parametrize("kind", ["triangle", "trapezoid"])
+def test_triangle_and_trapezoid_prism_rasterisation(kind):
+    """Triangular prism (Obj.cpp:473-494; tri_prism::tri_prism Obj.cpp:147-200;
+    parse parallelInputs.cpp:1528-1569): the base triangle (+-b/2, -h/2),
+    (0, h/2) in object x-y around loc, |t_z| <= length / 2.  The trapezoid is
+    the triangle with apex (0, H/2), base corners (+-B/2, -H/2), H = 2 B
+    height / (base1 - base2), B = max(base1, base2), cut where the apex's
+    barycentric coordinate exceeds height / H, i.e. at y = height - H/2 (so its
+    top is (base1 + base2) / 2 wide, not base2)."""
+    d = 0.05
+    loc = np.array([0.27, -0.33, 0.41]) * d
+    eps = (2.0, 2.5, 3.0)
+    length = 12.37 * d
+    if kind == "triangle":
+        b, h = 13.13 * d, 11.71 * d
+        verts = [(b / 2, -h / 2), (-b / 2, -h / 2), (0.0, h / 2)]
+        obj = _obj("triangle_prism", eps, loc, base=b, height=h, length=length)
+        ycut = None
+    else:
+        b1, b2, height = 14.21 * d, 6.17 * d, 6.93 * d
+        B = max(b1, b2)
+        H = B * 2.0 * height / (b1 - b2)
+        verts = [(0.0, H / 2), (B / 2, -H / 2), (-B / 2, -H / 2)]
+        obj = _obj("trapezoid_prism", eps, loc, base1=b1, base2=b2, height=height, length=length)
+        ycut = height - H / 2
+
+    def inside(p):
+        t = [p[k] - loc[k] for k in range(3)]
+        ins, m = _in_hull((t[0], t[1]), verts)
+        ins &= np.abs(t[2]) <= length / 2
+        m = min(m, float(np.min(np.abs(np.abs(t[2]) - length / 2))))
+        if ycut is not None:
+            ins &= t[1] <= ycut
+            m = min(m, float(np.min(np.abs(t[1] - ycut))))
+        return ins, m
+
+    _check_raster([obj], inside, eps, 23, d)
+
+
+@pytest.mark.parametrize("kind", ["regular", "vertices"])
+def test_tetrahedron_rasterisation(kind):
+    """Tetrahedron (Obj.cpp:496-510; constructors Obj.cpp:215-276; parse
+    parallelInputs.cpp:1585-1616).  From sideLen s at loc = 0: the vertices
+    s sqrt(3/8) (sqrt(8/9), 0, -1/3), (-sqrt(2/9), +-sqrt(2/3), -1/3),
+    (0, 0, 1), cut parallel to the base where the top vertex's barycentric
+    coordinate exceeds that of heightRat * top, heightRat =
+    height / (s sqrt(2/3)) 5/4 - 1/4 (here 1/4 + 3/4 heightRat, the
+    vertices' centroid being 0).  From vertLocs: the real-space hull."""
+    d = 0.05
+    eps = (2.0, 2.5, 3.0)
+    if kind == "regular":
+        s_, height = 17.3 * d, 0.7 * math.sqrt(2.0 / 3.0) * 17.3 * d
+        sc = s_ * math.sqrt(3.0 / 8.0)
+        verts = [np.array(v) * sc for v in ((math.sqrt(8 / 9), 0, -1 / 3), (-math.sqrt(2 / 9), math.sqrt(2 / 3), -1 / 3),
+                                            (-math.sqrt(2 / 9), -math.sqrt(2 / 3), -1 / 3), (0, 0, 1))]
+        hr = height / (s_ * math.sqrt(2.0 / 3.0)) * 5.0 / 4.0 - 0.25
+        lam = 0.25 + 0.75 * hr
+        zcut = verts[0][2] + lam * (verts[3][2] - verts[0][2])
+        obj = _obj("tetrahedron", eps, (0.0, 0.0, 0.0), sideLen=s_, height=height)
+    else:
+        verts = [np.array(v) * d for v in ((-7.35, -6.15, -5.11), (8.5, -4.94, -4.4), (0.11, 8.61, -5.91), (0.5, 0.35, 8.26))]
+        zcut = None
+        obj = _obj("tetrahedron", eps, (0.0, 0.0, 0.0), vertLocs=[{"vert": list(v)} for v in verts])
+
+    def inside(p):
+        ins, m = _in_hull(p, verts)
+        if zcut is not None:
+            ins &= p[2] <= zcut
+            m = min(m, float(np.min(np.abs(p[2] - zcut))))
+        return ins, m
+
+    _check_raster([obj], inside, eps, 23, d)
+
+
 def test_partial_torus_rasterisation():
     """Torus section (Obj.cpp:562-575, parse parallelInputs.cpp:1580-1584):
     (rho - R)^2 <= r^2 - t_z^2 with rho the distance from the object z axis,
