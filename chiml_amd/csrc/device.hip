@@ -2236,13 +2236,14 @@ constexpr int kPopThreads = 1024;
 // Populations of this step: block j sums population j's per-block partials
 // (each thread a fixed strided subset, then a tree in LDS, so the order is
 // the same every run) and stores the mean over the emitters.
-__global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, int nblocks, const int* stepp, int stepv) {
+__global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, const double2* __restrict__ partial, int nblocks, int j0,
+                                                                const int* stepp, int stepv) {
     __shared__ double2 red[kPopThreads];
-    const int jj = blockIdx.x;
+    const int jj = j0 + blockIdx.x;
     const int step = step_now(stepp, stepv);
     if (step % q.pop_interval != 0 || step / q.pop_interval >= q.pop_cap) return;
     double2 sum = make_double2(0.0, 0.0);
-    for (int b = threadIdx.x; b < nblocks; b += kPopThreads) sum = cadd(sum, q.pop_partial[jj * nblocks + b]);
+    for (int b = threadIdx.x; b < nblocks; b += kPopThreads) sum = cadd(sum, partial[blockIdx.x * nblocks + b]);
     red[threadIdx.x] = sum;
     __syncthreads();
     for (int w = kPopThreads / 2; w > 0; w >>= 1) {
@@ -2251,6 +2252,44 @@ __global__ void __launch_bounds__(kPopThreads) k_ml_pop_finish(QeDev q, int nblo
     }
     if (threadIdx.x == 0)
         q.pop_out[static_cast<long long>(jj) * q.pop_cap + step / q.pop_interval] = make_double2(red[0].x / q.nemit, red[0].y / q.nemit);
+}
+
+// Population detectors past the four the density kernel sums in registers:
+// on recording steps, block (b, jj - j0) sums entry (ri, cj) of the stored
+// density matrices (the conjugate of (cj, ri) below the diagonal when
+// packed) over a strided subset of the member/emitter columns, then a tree in
+// LDS; k_ml_pop_finish adds the blocks' sums in a fixed order.
+__global__ void __launch_bounds__(256) k_ml_pop_gather(QeDev q, int j0, double2* __restrict__ partial, const int* stepp, int stepv) {
+    __shared__ double2 red[256];
+    const int step = step_now(stepp, stepv);
+    if (step % q.pop_interval != 0 || step / q.pop_interval >= q.pop_cap) return;
+    const int k = q.pop_flat[j0 + blockIdx.y];
+    const int N = q.N, ri = k % N, cj = k / N;
+    long long si;
+    bool conj = false;
+    if (q.packed) {
+        if (ri <= cj) si = cj * (cj + 1) / 2 + ri;
+        else {
+            si = ri * (ri + 1) / 2 + cj;
+            conj = true;
+        }
+    } else {
+        si = ri + static_cast<long long>(cj) * N;
+    }
+    const long long ld = static_cast<long long>(q.nmem) * q.nemit;
+    double2 sum = make_double2(0.0, 0.0);
+    for (long long col = blockIdx.x * 256LL + threadIdx.x; col < ld; col += gridDim.x * 256LL) {
+        double2 v = q.rho[si * ld + col];
+        if (conj) v.y = -v.y;
+        sum = cadd(sum, v);
+    }
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
 // ----------------------------------------------------------------- TFSF ----
@@ -2600,6 +2639,9 @@ struct QeHost {
     std::vector<std::unique_ptr<DevBuf>> bufs;
     int nblocks = 0;
     int N = 0;
+    int npop_all = 0;             // population detectors; the density kernel sums the first 4
+    int nb2 = 0;                  // blocks per extra population in k_ml_pop_gather
+    double2* partial2 = nullptr;  // [npop_all - 4][nb2]
 };
 
 struct chiml_dev {
@@ -3169,7 +3211,9 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         q.rho = reinterpret_cast<double2*>(c->upload<double>(rho0.data(), rho0.size()));
         q.hist = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * SW * ld));
         q.npop = std::min(qi.npop, 4);
-        if (qi.npop > 4) throw std::runtime_error("at most 4 population detectors per emitter group");
+        qh.npop_all = qi.npop;
+        for (int k = 0; k < qi.npop; ++k)
+            if (qi.pop_flat[k] < 0 || qi.pop_flat[k] >= qi.nlevel * qi.nlevel) throw std::runtime_error("population detector index out of range");
         q.pop_flat = c->upload<int>(qi.pop_flat, std::max(qi.npop, 1));
         q.pop_interval = qi.pop_interval > 0 ? qi.pop_interval : 1;
         // emitters per block of kMlThreads lanes
@@ -3177,7 +3221,12 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         qh.nblocks = (q.nemit + per_block - 1) / per_block;
         q.pop_partial = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(qh.nblocks)));
         q.pop_cap = p->nsteps / q.pop_interval + 1;
-        q.pop_out = reinterpret_cast<double2*>(c->zeros<double>(2 * 4 * static_cast<size_t>(q.pop_cap)));
+        q.pop_out = reinterpret_cast<double2*>(c->zeros<double>(2 * std::max(qi.npop, 4) * static_cast<size_t>(q.pop_cap)));
+        if (qi.npop > 4) {
+            const long long ldc = static_cast<long long>(q.nmem) * q.nemit;
+            qh.nb2 = static_cast<int>(std::min<long long>(1024, std::max<long long>(1, (ldc + 1023) / 1024)));
+            qh.partial2 = reinterpret_cast<double2*>(c->zeros<double>(2 * static_cast<size_t>(qi.npop - 4) * qh.nb2));
+        }
         q.eps_node = epsn;
         c->qes.push_back(std::move(qh));
     }
@@ -3452,7 +3501,14 @@ void launch_ml(chiml_dev* c, StepArgs<T>& a) {
 #undef CHIML_ML_TEAM
             }
         }
-        if (q.npop > 0) hipLaunchKernelGGL(k_ml_pop_finish, dim3(q.npop), dim3(kPopThreads), 0, c->stream, q, qh.nblocks, a.step, a.stepv);
+        if (q.npop > 0)
+            hipLaunchKernelGGL(k_ml_pop_finish, dim3(q.npop), dim3(kPopThreads), 0, c->stream, q, q.pop_partial, qh.nblocks, 0, a.step, a.stepv);
+        if (qh.npop_all > 4) {
+            hipLaunchKernelGGL(k_ml_pop_gather, dim3(static_cast<unsigned>(qh.nb2), static_cast<unsigned>(qh.npop_all - 4)), dim3(256), 0,
+                               c->stream, q, 4, qh.partial2, a.step, a.stepv);
+            hipLaunchKernelGGL(k_ml_pop_finish, dim3(static_cast<unsigned>(qh.npop_all - 4)), dim3(kPopThreads), 0, c->stream, q,
+                               qh.partial2, qh.nb2, 4, a.step, a.stepv);
+        }
     }
 }
 
@@ -4172,7 +4228,7 @@ int chiml_dev_read_incd(chiml_dev_t* c, int comp, double* reim, size_t cap, size
 int chiml_dev_read_pops(chiml_dev_t* c, int q, int j, double* reim, size_t cap, size_t* nsamples) {
     if (!c || q < 0 || q >= static_cast<int>(c->qes.size()) || !nsamples) return fail(c, CHIML_E_INVALID, "invalid argument");
     QeDev& d = c->qes[q].dev;
-    if (j < 0 || j >= d.npop) return fail(c, CHIML_E_INVALID, "population detector index out of range");
+    if (j < 0 || j >= c->qes[q].npop_all) return fail(c, CHIML_E_INVALID, "population detector index out of range");
     int ns = (c->steps_done - 1) / d.pop_interval + 1;
     if (c->steps_done == 0) ns = 0;
     if (ns > d.pop_cap) ns = d.pop_cap;

@@ -242,6 +242,30 @@ def test_3d_n_level_emitter_block_fp64(nlev):
     assert rel_linf(rg, ro) <= 1e-10
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_3d_six_level_block_eight_population_detectors(precision):
+    """Eight population detectors on one 6-level group: the six diagonals and
+    the (1, 0) / (0, 1) coherences.  The first four are summed in the density
+    kernel, the rest by the gather kernel (the (1, 0) entry read as the
+    conjugate of the stored (0, 1) with Hermitian packing)."""
+    d = json.loads(_n_level_block(6, precision))
+    levs = [k * 7 for k in range(6)] + [1, 6]
+    for o in d["ObjectList"]:
+        if "dtc_levs" in o:
+            o["dtc_levs"] = levs
+    print(f"8 populations precision={precision}", flush=True)
+    sim, orc = _pair(json.dumps(d), 90)
+    q = sim.qeArr()[0]
+    tol = 1e-10 if precision == "fp64" else FP32_TOL
+    for j in range(len(levs)):
+        pg, po = q.population(j), orc.pops(0, j)
+        assert pg.shape == po.shape
+        scale = max(np.abs(po).max(), 1e-300)
+        err = np.abs(pg - po).max() / scale
+        print(f"population {j} (flat {levs[j]}): peak {scale:.3e} err {err:.3e}", flush=True)
+        assert err <= tol, (j, err)
+
+
 @pytest.mark.parametrize("nlev", [2, 4, 6])
 def test_3d_n_level_emitter_block_fp32_fields(nlev):
     """The same block with FP32 fields and FP64 density matrices (the 3D
