@@ -1606,7 +1606,7 @@ __global__ void __launch_bounds__(256) k_ade3(StepArgs<T> a, int ade_bpc) {
 
 // One set of DFT accumulators: its sampled cells, sampling interval and
 // staging buffer [kDftBlock][n].
-constexpr int kMaxDftGroups = 8;
+constexpr int kMaxDftGroups = 16;  // flux boxes + up to 15 frequency detectors; the table rides in the kernel arguments
 template <typename T>
 struct DftSample {
     int n, interval;
@@ -3103,7 +3103,7 @@ void build(chiml_dev* c, const chiml_problem_t* p, StepArgs<T>& a, SampleArgs<T>
         s.det_cap = c->upload<int>(c->det_cap.data(), p->ndet);
     }
     // DFT groups: 0 = the flux boxes (ndft / dft_*), then p->dftg[0..ndftg)
-    if (p->ndftg < 0 || 1 + p->ndftg > kMaxDftGroups) throw std::runtime_error("too many DFT groups (at most 7 frequency detectors)");
+    if (p->ndftg < 0 || 1 + p->ndftg > kMaxDftGroups) throw std::runtime_error("too many DFT groups (at most 15 frequency detectors)");
     c->dgs.clear();
     c->ndft = 0;
     s.ndg = 1 + p->ndftg;

@@ -1469,7 +1469,7 @@ FDTDField::FDTDField(const std::string& deck) : IP(parseInputs(deck)) {
         g.phasor = fdPhasor[k].data();
         fdGroups.push_back(g);
     }
-    if (fdGroups.size() > 7) throw std::logic_error("at most 7 frequency detectors per simulation");
+    if (fdGroups.size() > 15) throw std::logic_error("at most 15 frequency detectors per simulation");
     P.ndftg = static_cast<int>(fdGroups.size());
     P.dftg = fdGroups.empty() ? nullptr : fdGroups.data();
 

@@ -487,6 +487,31 @@ def test_freq_detectors_3d(precision):
     assert rel_linf(v_g, v_o) <= tol
 
 
+def test_eleven_freq_detectors_3d():
+    """Eleven frequency detectors (twelve DFT groups with the flux boxes):
+    the three detectors of the deck above, repeated at shifted points."""
+    d = json.loads(_freq_deck("fp64"))
+    fq = [x for x in d["DetectorList"] if x["dtc_class"] == "freq"]
+    extra = []
+    for r in range(1, 4):
+        for x in fq:
+            y = json.loads(json.dumps(x))
+            y["fname"] = f"{x['fname']}{r}"
+            y["loc"] = [y["loc"][0] + 0.02 * r, y["loc"][1] - 0.02 * (r % 2), y["loc"][2]]
+            extra.append(y)
+    d["DetectorList"] = fq + extra[:8] + [x for x in d["DetectorList"] if x["dtc_class"] != "freq"]
+    print("eleven freq detectors", flush=True)
+    sim, orc = _pair(json.dumps(d), 160)
+    fds = sim.dtcFreqArr()
+    assert len(fds) == orc.num_freq_detectors() == 11
+    for k, fd in enumerate(fds):
+        F, F_o = fd.fields(), orc.freq_fields(k)
+        err_f = rel_linf(F, F_o)
+        err_v = rel_linf(fd.values(), orc.freq_values(k)[1])
+        print(f"freq detector {k}: fields {err_f:.3e}, values {err_v:.3e}", flush=True)
+        assert err_f <= FP64_TOL and err_v <= FP64_TOL, (k, err_f, err_v)
+
+
 def test_freq_detector_2d_is_zero():
     """In 2D the reference's frequency storage samples no point (its z extent
     is 0), so the table is zero; the product keeps that."""
